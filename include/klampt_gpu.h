@@ -1,0 +1,196 @@
+/* klampt_gpu.h — C ABI of the MI355X collision / distance / feasibility engine.
+ *
+ * Plain pointers and sizes only.  Every function returns KG_OK (0) or a
+ * negative KG_E* code; the message of the last error on the calling thread
+ * is available from kg_last_error().  Handles are not thread-safe; one HIP
+ * stream per context.  Host arrays passed in are copied; batch outputs are
+ * written to caller memory.  Functions suffixed _dev take device pointers
+ * and a hipStream_t (as void*) and run asynchronously on it.
+ *
+ * Transforms: R is a column-major 3x3 rotation (9 doubles) and t a 3-vector,
+ * as in Klampt (Python/klampt/math/so3.py:1-16).  A link transform array is
+ * 12 values per link: R (9, column-major) then t (3).
+ *
+ * Reference interfaces replaced are cited per entry point (paths relative to
+ * the Klampt source tree).
+ */
+#ifndef KLAMPT_GPU_H
+#define KLAMPT_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KG_OK 0
+#define KG_E_INVALID -1     /* bad argument (mirrors PyException on bad input) */
+#define KG_E_HIP -2         /* HIP runtime error */
+#define KG_E_UNSUPPORTED -3 /* unsupported geometry pair (Klampt: Inf distance -> raise) */
+#define KG_E_EMPTY -4       /* empty geometry where a value is required */
+#define KG_E_LIMIT -5       /* a documented size limit was exceeded */
+
+typedef struct kg_ctx_s* kg_ctx;
+typedef struct kg_geom_s* kg_geom;
+typedef struct kg_robot_s* kg_robot;
+typedef struct kg_world_s* kg_world;
+
+/* DistanceQuerySettings (Python/klampt/src/geometry.h:631-638) */
+typedef struct {
+  double relErr, absErr, upperBound;
+} kg_dist_settings;
+
+/* DistanceQueryResult (Python/klampt/src/geometry.h:663-673) */
+typedef struct {
+  double d;
+  int hasClosestPoints, hasGradients;
+  double cp1[3], cp2[3];
+  double grad1[3], grad2[3];
+  int elem1, elem2;
+} kg_dist_result;
+
+/* Last error message of the calling thread (the text SWIG's %exception
+ * raises, Python/klampt/src/robotsim.i:41-53). */
+const char* kg_last_error(void);
+int kg_version(void);
+
+/* ---- context ----------------------------------------------------------- */
+int kg_ctx_create(int device, kg_ctx* out);
+int kg_ctx_destroy(kg_ctx ctx);
+int kg_ctx_synchronize(kg_ctx ctx);
+
+/* ---- geometry ----------------------------------------------------------
+ * Triangle mesh: Geometry3D(TriangleMesh) with its collision data
+ * (Python/klampt/src/robotsim.cpp:2886-2949 vertices/indices; the BVH built
+ * once, as AnyCollisionGeometry3D::InitCollisionData does).  verts: nv x 3
+ * doubles, tris: nt x 3 int32. */
+int kg_geom_create_trimesh(kg_ctx ctx, const double* verts, int nverts, const int32_t* tris, int ntris,
+                           kg_geom* out);
+/* ImplicitSurface (Python/klampt/src/geometry.h:322-391): cell-centred values,
+ * C-order (i,j,k), dims[0] x dims[1] x dims[2], > 0 outside. */
+int kg_geom_create_sdf(kg_ctx ctx, const double bmin[3], const double bmax[3], const int32_t dims[3],
+                       const float* values, kg_geom* out);
+int kg_geom_destroy(kg_geom g);
+/* Geometry3D.setCurrentTransform / getCurrentTransform
+ * (Python/klampt/src/robotsim.cpp:1465-1473) */
+int kg_geom_set_transform(kg_geom g, const double R[9], const double t[3]);
+int kg_geom_get_transform(kg_geom g, double R[9], double t[3]);
+/* Geometry3D.getBB (Python/klampt/src/robotsim.cpp:1575-1586): world AABB of
+ * the transformed local bounding box. */
+int kg_geom_get_bb(kg_geom g, double bmin[3], double bmax[3]);
+/* Geometry3D.collides (Python/klampt/src/robotsim.cpp:1656-1662); empty
+ * geometry -> 0 (no error). */
+int kg_geom_collide(kg_geom a, kg_geom b, int* out);
+/* Geometry3D.withinDistance (Python/klampt/src/robotsim.cpp:1685-1692) */
+int kg_geom_within_distance(kg_geom a, kg_geom b, double tol, int* out);
+/* Geometry3D.distance_ext (Python/klampt/src/robotsim.cpp:1777-1819):
+ * unsigned TriangleMesh/TriangleMesh distance, 0 when touching. */
+int kg_geom_distance(kg_geom a, kg_geom b, const kg_dist_settings* settings, kg_dist_result* out);
+/* Geometry3D.distance_point_ext on an ImplicitSurface
+ * (Python/klampt/src/robotsim.cpp:1740-1775, geometry.h:1052-1063): signed
+ * trilinear value and gradient. */
+int kg_geom_distance_point(kg_geom g, const double pt[3], const kg_dist_settings* settings, kg_dist_result* out);
+/* Batched ImplicitSurface point queries (world points, FP32): d[n], grad[n][3]
+ * (grad nullable). */
+int kg_sdf_eval(kg_geom sdf, const float* pts, int n, float* d, float* grad);
+
+/* ---- robot -------------------------------------------------------------
+ * RobotModel kinematic tree (Cpp/Modeling/Robot.h:83-163; links set up as in
+ * Cpp/Modeling/Robot.cpp:957-981).  parents[n] (topologically sorted, -1 =
+ * world), T0[n][12] (T0_Parent), axes[n][3], jtype[n] (0 revolute, 1
+ * prismatic), qmin/qmax[n].  Self-collision pairs default to all pairs except
+ * link vs parent (Cpp/docs/Manual-FileTypes.md:212). */
+int kg_robot_create(kg_ctx ctx, int nlinks, const int32_t* parents, const double* T0, const double* axes,
+                    const int32_t* jtype, const double* qmin, const double* qmax, kg_robot* out);
+int kg_robot_destroy(kg_robot r);
+/* RobotModelLink.geometry() assignment; g may be NULL (empty geometry). */
+int kg_robot_set_link_geom(kg_robot r, int link, kg_geom g);
+/* RobotModel.enableSelfCollision (Python/klampt/src/robotsim.cpp:5504-5521) */
+int kg_robot_enable_self_collision(kg_robot r, int link1, int link2, int enabled);
+int kg_robot_self_collision_enabled(kg_robot r, int link1, int link2, int* out);
+/* RobotModel.setConfig (Python/klampt/src/robotsim.cpp:5311-5321):
+ * UpdateConfig (FK) + UpdateGeometry. */
+int kg_robot_set_config(kg_robot r, const double* q);
+int kg_robot_get_config(kg_robot r, double* q);
+/* RobotModelLink.getTransform after setConfig. */
+int kg_robot_get_link_transform(kg_robot r, int link, double R[9], double t[3]);
+/* RobotModel.selfCollides (Python/klampt/src/robotsim.cpp:5529-5539). */
+int kg_robot_self_collides(kg_robot r, int* out);
+/* Batched RobotModel::UpdateConfig in FP32: Q[N][n] -> T[N][n][12]. */
+int kg_fk_batch(kg_robot r, const float* Q, int N, float* T_out);
+int kg_fk_batch_dev(kg_robot r, const float* dQ, int N, float* dT_out, void* stream);
+
+/* ---- world -------------------------------------------------------------
+ * WorldModel ID layout (Cpp/Modeling/World.cpp:47-53,156-180): terrains, then
+ * rigid objects, then per robot [robot, link0..]; the collision mask is
+ * WorldPlannerSettings::InitializeDefault (Cpp/Planning/PlannerSettings.cpp:16-41),
+ * rebuilt (and user overrides cleared) whenever an item is added. */
+int kg_world_create(kg_ctx ctx, kg_world* out);
+int kg_world_destroy(kg_world w);
+int kg_world_add_terrain(kg_world w, kg_geom g, int* id_out);
+int kg_world_add_rigid_object(kg_world w, kg_geom g, const double R[9], const double t[3], int* id_out);
+int kg_world_set_rigid_object_transform(kg_world w, int object_index, const double R[9], const double t[3]);
+int kg_world_add_robot(kg_world w, kg_robot r, int* id_out);
+int kg_world_num_ids(kg_world w);
+/* WorldPlannerSettings::collisionEnabled(id1,id2) */
+int kg_world_set_collision_enabled(kg_world w, int id1, int id2, int enabled);
+int kg_world_get_collision_enabled(kg_world w, int id1, int id2, int* out);
+/* Pair list used by kg_distance_batch for robot `robot_index`: mask-enabled
+ * (link, env item) pairs in link-major order, then enabled self pairs (i<j);
+ * world IDs, 2 per pair. */
+int kg_world_pair_list(kg_world w, int robot_index, int32_t* pairs, int max_pairs, int* npairs);
+
+/* SingleRobotCSpace::IsFeasible over a batch
+ * (Cpp/Planning/RobotCSpace.cpp:786-823; broad phase and order of
+ * WorldPlannerSettings::CheckCollision, Cpp/Planning/PlannerSettings.cpp:214-331).
+ * Q[N][n] float.  feasible[N] required; env_hit, self_hit, first_pair (N x 2
+ * world IDs of the first colliding pair in Klampt's loop order, env before
+ * self, -1 if none) are optional (NULL).  Asking only for `feasible` enables
+ * whole-configuration early exit. */
+int kg_feasible_batch(kg_world w, int robot_index, const float* Q, int N, uint8_t* feasible, uint8_t* env_hit,
+                      uint8_t* self_hit, int32_t* first_pair);
+int kg_feasible_batch_dev(kg_world w, int robot_index, const float* dQ, int N, uint8_t* d_feasible,
+                          uint8_t* d_env_hit, uint8_t* d_self_hit, int32_t* d_first_pair, void* stream);
+
+/* Per-pair minimum distance for the kg_world_pair_list pairs (AnyCollisionQuery
+ * ::Distance, Cpp/Planning/PlannerSettings.cpp:109-115): out[N][npairs], 0 when
+ * touching. */
+int kg_distance_batch(kg_world w, int robot_index, const float* Q, int N, float* pair_dist);
+int kg_distance_batch_dev(kg_world w, int robot_index, const float* dQ, int N, float* d_pair_dist, void* stream);
+
+/* Straight-line edge checks (SingleRobotCSpace::PathChecker ->
+ * EpsilonEdgeChecker, Cpp/Planning/RobotCSpace.cpp:835-837; sample rule
+ * n = ceil(d/eps), u = i/n of Python/klampt/plan/robotcspace.py:216-228;
+ * interpolation Cpp/Modeling/Interpolate.cpp:13-14; d the L2 norm
+ * (norm_inf = 0, Klampt's default) or the L-infinity norm).  Interior samples
+ * i = 1..n-1 are checked in level-order bisection; first_u[s] is u of the
+ * first infeasible sample in that order, -1 if all are feasible. */
+int kg_edge_batch(kg_world w, int robot_index, const float* A, const float* B, int nseg, double eps, int norm_inf,
+                  float* first_u);
+
+/* SDF hinge penalty L(q) = sum_{links k} sum_{p in pts_k} max(0, dsafe - sdf(T_k(q) p))^2
+ * and its gradient by an FK-Jacobian adjoint (the distance-gradient rule of
+ * Python/klampt/math/autodiff/geometry_ad.py:250-293 chained with
+ * kinematics_ad.py:53-59).  local_pts: concatenated per-link points
+ * (pts_per_link[n]), link frames.  bwd recomputes FK and the SDF gathers. */
+int kg_sdf_penalty_fwd(kg_robot r, kg_geom sdf, const float* Q, int N, const float* local_pts,
+                       const int32_t* pts_per_link, float dsafe, float* loss);
+int kg_sdf_penalty_bwd(kg_robot r, kg_geom sdf, const float* Q, int N, const float* local_pts,
+                       const int32_t* pts_per_link, float dsafe, const float* dloss, float* dQ);
+int kg_sdf_penalty_fwd_dev(kg_robot r, kg_geom sdf, const float* dQ, int N, const float* d_local_pts,
+                           const int32_t* pts_per_link, float dsafe, float* d_loss, void* stream);
+int kg_sdf_penalty_bwd_dev(kg_robot r, kg_geom sdf, const float* dQ, int N, const float* d_local_pts,
+                           const int32_t* pts_per_link, float dsafe, const float* d_dloss, float* d_dQ,
+                           void* stream);
+
+/* Counters of the last batched call on this world: [0] configurations
+ * re-checked in FP64, [1] narrow-phase pair items, [2] spill events,
+ * [3] broad-phase time (ns), [4] FP32 traversal kernel time (ns), [5] FP64
+ * re-check kernel time (ns) -- HIP events on the launch stream -- [6] box
+ * tests, [7] triangle tests of the FP32 traversal. */
+int kg_world_last_stats(kg_world w, int64_t* stats, int nstats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KLAMPT_GPU_H */

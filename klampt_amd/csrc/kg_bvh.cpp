@@ -1,0 +1,229 @@
+// OBB tree construction (host, FP64) for triangle meshes.
+//
+// Top-down build, one triangle per leaf: each node's axes are the principal
+// axes of its triangles' vertices (Jacobi eigen-decomposition of the 3x3
+// covariance), and the node is split at the median of the triangle centroids
+// projected on the major axis, which bounds the depth by ceil(log2(T)).  The
+// leaf order is the depth-first order, so triangles of nearby leaves sit next
+// to each other in memory.
+//
+// Stored boxes are exact bounding volumes in FP32: axes and centre are
+// rounded to FP32 first, then the extents are measured in the dual basis of
+// the rounded axes (so slightly non-orthonormal FP32 axes still bound every
+// vertex) and rounded up.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "kg_types.h"
+#include "kg_internal.h"
+
+namespace {
+
+void jacobi3(double A[3][3], double V[3][3], double w[3]) {
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) V[i][j] = (i == j) ? 1.0 : 0.0;
+  for (int sweep = 0; sweep < 50; sweep++) {
+    double off = std::fabs(A[0][1]) + std::fabs(A[0][2]) + std::fabs(A[1][2]);
+    double scale = std::fabs(A[0][0]) + std::fabs(A[1][1]) + std::fabs(A[2][2]);
+    if (off <= 1e-30 * (scale + 1e-300)) break;
+    for (int p = 0; p < 2; p++)
+      for (int q = p + 1; q < 3; q++) {
+        if (std::fabs(A[p][q]) < 1e-300) continue;
+        double theta = (A[q][q] - A[p][p]) / (2 * A[p][q]);
+        double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1));
+        double c = 1 / std::sqrt(t * t + 1), s = t * c;
+        for (int k = 0; k < 3; k++) {
+          double akp = A[k][p], akq = A[k][q];
+          A[k][p] = c * akp - s * akq;
+          A[k][q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < 3; k++) {
+          double apk = A[p][k], aqk = A[q][k];
+          A[p][k] = c * apk - s * aqk;
+          A[q][k] = s * apk + c * aqk;
+        }
+        for (int k = 0; k < 3; k++) {
+          double vkp = V[k][p], vkq = V[k][q];
+          V[k][p] = c * vkp - s * vkq;
+          V[k][q] = s * vkp + c * vkq;
+        }
+      }
+  }
+  for (int i = 0; i < 3; i++) w[i] = A[i][i];
+}
+
+struct Builder {
+  const double* V;
+  const int32_t* T;
+  std::vector<double> cen;  // centroids (3 per triangle)
+  std::vector<int> idx;
+  std::vector<ObbNode> nodes;
+  std::vector<int> order;   // leaf order -> original triangle
+  int depth = 0;
+
+  void fit(ObbNode& nd, int b, int e) {
+    double mean[3] = {0, 0, 0};
+    int nv = 0;
+    for (int i = b; i < e; i++)
+      for (int v = 0; v < 3; v++) {
+        const double* p = V + 3 * T[3 * idx[i] + v];
+        for (int k = 0; k < 3; k++) mean[k] += p[k];
+        nv++;
+      }
+    for (int k = 0; k < 3; k++) mean[k] /= nv;
+    double C[3][3] = {{0}};
+    for (int i = b; i < e; i++)
+      for (int v = 0; v < 3; v++) {
+        const double* p = V + 3 * T[3 * idx[i] + v];
+        double d[3] = {p[0] - mean[0], p[1] - mean[1], p[2] - mean[2]};
+        for (int r = 0; r < 3; r++)
+          for (int c = 0; c < 3; c++) C[r][c] += d[r] * d[c];
+      }
+    double E[3][3], w[3];
+    jacobi3(C, E, w);
+    // sort axes by decreasing eigenvalue
+    int ord[3] = {0, 1, 2};
+    std::sort(ord, ord + 3, [&](int a, int c) { return w[a] > w[c]; });
+    double ax[3][3];
+    for (int k = 0; k < 3; k++)
+      for (int r = 0; r < 3; r++) ax[k][r] = E[r][ord[k]];
+    // right-handed: ax2 = ax0 x ax1
+    double cx[3] = {ax[0][1] * ax[1][2] - ax[0][2] * ax[1][1], ax[0][2] * ax[1][0] - ax[0][0] * ax[1][2],
+                    ax[0][0] * ax[1][1] - ax[0][1] * ax[1][0]};
+    double n = std::sqrt(cx[0] * cx[0] + cx[1] * cx[1] + cx[2] * cx[2]);
+    for (int r = 0; r < 3; r++) ax[2][r] = cx[r] / n;
+    // round axes to FP32
+    double Rf[3][3];
+    for (int k = 0; k < 3; k++)
+      for (int r = 0; r < 3; r++) {
+        nd.R[k * 3 + r] = (float)ax[k][r];
+        Rf[k][r] = (double)nd.R[k * 3 + r];
+      }
+    // centre from the extent midpoints, rounded to FP32
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int i = b; i < e; i++)
+      for (int v = 0; v < 3; v++) {
+        const double* p = V + 3 * T[3 * idx[i] + v];
+        for (int k = 0; k < 3; k++) {
+          double s = Rf[k][0] * p[0] + Rf[k][1] * p[1] + Rf[k][2] * p[2];
+          lo[k] = std::min(lo[k], s);
+          hi[k] = std::max(hi[k], s);
+        }
+      }
+    double c[3] = {0, 0, 0};
+    for (int k = 0; k < 3; k++) {
+      double mid = 0.5 * (lo[k] + hi[k]);
+      for (int r = 0; r < 3; r++) c[r] += mid * Rf[k][r];
+    }
+    for (int r = 0; r < 3; r++) nd.c[r] = (float)c[r];
+    // extents in the dual basis of the rounded axes: s = M^{-1}(p - c)
+    double M[3][3];  // columns = axes
+    for (int r = 0; r < 3; r++)
+      for (int k = 0; k < 3; k++) M[r][k] = Rf[k][r];
+    double det = M[0][0] * (M[1][1] * M[2][2] - M[1][2] * M[2][1]) -
+                 M[0][1] * (M[1][0] * M[2][2] - M[1][2] * M[2][0]) +
+                 M[0][2] * (M[1][0] * M[2][1] - M[1][1] * M[2][0]);
+    double Mi[3][3];
+    Mi[0][0] = (M[1][1] * M[2][2] - M[1][2] * M[2][1]) / det;
+    Mi[0][1] = (M[0][2] * M[2][1] - M[0][1] * M[2][2]) / det;
+    Mi[0][2] = (M[0][1] * M[1][2] - M[0][2] * M[1][1]) / det;
+    Mi[1][0] = (M[1][2] * M[2][0] - M[1][0] * M[2][2]) / det;
+    Mi[1][1] = (M[0][0] * M[2][2] - M[0][2] * M[2][0]) / det;
+    Mi[1][2] = (M[0][2] * M[1][0] - M[0][0] * M[1][2]) / det;
+    Mi[2][0] = (M[1][0] * M[2][1] - M[1][1] * M[2][0]) / det;
+    Mi[2][1] = (M[0][1] * M[2][0] - M[0][0] * M[2][1]) / det;
+    Mi[2][2] = (M[0][0] * M[1][1] - M[0][1] * M[1][0]) / det;
+    double cf[3] = {nd.c[0], nd.c[1], nd.c[2]};
+    double ext[3] = {0, 0, 0};
+    for (int i = b; i < e; i++)
+      for (int v = 0; v < 3; v++) {
+        const double* p = V + 3 * T[3 * idx[i] + v];
+        double d[3] = {p[0] - cf[0], p[1] - cf[1], p[2] - cf[2]};
+        for (int k = 0; k < 3; k++) {
+          double s = Mi[k][0] * d[0] + Mi[k][1] * d[1] + Mi[k][2] * d[2];
+          ext[k] = std::max(ext[k], std::fabs(s));
+        }
+      }
+    for (int k = 0; k < 3; k++) {
+      double padded = ext[k] * (1.0 + 1e-12) + 1e-300;
+      float f = (float)padded;
+      if ((double)f < padded) f = std::nextafter(f, 1e30f);
+      nd.e[k] = f;
+    }
+  }
+
+  int build(int b, int e, int d) {
+    int self = (int)nodes.size();
+    nodes.push_back(ObbNode{});
+    depth = std::max(depth, d);
+    fit(nodes[self], b, e);
+    if (e - b == 1) {
+      nodes[self].child = ~(int)order.size();
+      order.push_back(idx[b]);
+      return self;
+    }
+    // split at the median centroid along the major axis
+    double ax[3] = {nodes[self].R[0], nodes[self].R[1], nodes[self].R[2]};
+    int mid = (b + e) / 2;
+    std::nth_element(idx.begin() + b, idx.begin() + mid, idx.begin() + e, [&](int a, int c) {
+      double pa = ax[0] * cen[3 * a] + ax[1] * cen[3 * a + 1] + ax[2] * cen[3 * a + 2];
+      double pc = ax[0] * cen[3 * c] + ax[1] * cen[3 * c + 1] + ax[2] * cen[3 * c + 2];
+      return pa < pc || (pa == pc && a < c);
+    });
+    int l = (int)nodes.size();
+    nodes.push_back(ObbNode{});
+    nodes.push_back(ObbNode{});
+    nodes[self].child = l;
+    build_into(l, b, mid, d + 1);
+    build_into(l + 1, mid, e, d + 1);
+    return self;
+  }
+
+  // Build the subtree for [b,e) into the preallocated slot `at`.
+  void build_into(int at, int b, int e, int d) {
+    depth = std::max(depth, d);
+    fit(nodes[at], b, e);
+    if (e - b == 1) {
+      nodes[at].child = ~(int)order.size();
+      order.push_back(idx[b]);
+      return;
+    }
+    double ax[3] = {nodes[at].R[0], nodes[at].R[1], nodes[at].R[2]};
+    int mid = (b + e) / 2;
+    std::nth_element(idx.begin() + b, idx.begin() + mid, idx.begin() + e, [&](int a, int c) {
+      double pa = ax[0] * cen[3 * a] + ax[1] * cen[3 * a + 1] + ax[2] * cen[3 * a + 2];
+      double pc = ax[0] * cen[3 * c] + ax[1] * cen[3 * c + 1] + ax[2] * cen[3 * c + 2];
+      return pa < pc || (pa == pc && a < c);
+    });
+    int l = (int)nodes.size();
+    nodes.push_back(ObbNode{});
+    nodes.push_back(ObbNode{});
+    nodes[at].child = l;
+    build_into(l, b, mid, d + 1);
+    build_into(l + 1, mid, e, d + 1);
+  }
+};
+
+}  // namespace
+
+void kg_build_obb_tree(const double* V, int nv, const int32_t* T, int nt, BuiltTree& out) {
+  (void)nv;
+  Builder B;
+  B.V = V;
+  B.T = T;
+  B.cen.resize(3 * (size_t)nt);
+  B.idx.resize(nt);
+  for (int i = 0; i < nt; i++) {
+    B.idx[i] = i;
+    for (int k = 0; k < 3; k++)
+      B.cen[3 * i + k] = (V[3 * T[3 * i] + k] + V[3 * T[3 * i + 1] + k] + V[3 * T[3 * i + 2] + k]) / 3.0;
+  }
+  B.nodes.reserve(2 * (size_t)nt);
+  B.order.reserve(nt);
+  if (nt > 0) B.build(0, nt, 0);
+  out.nodes.swap(B.nodes);
+  out.order.swap(B.order);
+  out.depth = B.depth;
+}

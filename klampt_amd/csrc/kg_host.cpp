@@ -1,0 +1,1182 @@
+// C-ABI implementation: handles, geometry pool, world plans and the batched
+// feasibility pipeline.  Every public function maps a C++ exception or HIP
+// error to a KG_E* code with a thread-local message.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/klampt_gpu.h"
+#include "kg_internal.h"
+
+hipError_t kgl_finalize_list(const FinArgs& a, const int* list, const int* count, int maxn, hipStream_t s);
+hipError_t kgl_fill_int(int* p, int n, int v, hipStream_t s);
+int kgl_collide_grid(int exact, int num_cus);
+
+
+
+namespace {
+
+thread_local std::string g_err;
+
+struct KgError : std::runtime_error {
+  int code;
+  KgError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] void fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  throw KgError(code, buf);
+}
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) fail(KG_E_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+template <class F>
+int guard(F f) {
+  try {
+    f();
+    return KG_OK;
+  } catch (const KgError& e) {
+    g_err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_err = "out of host memory";
+    return KG_E_LIMIT;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return KG_E_INVALID;
+  }
+}
+
+void need(bool ok, const char* msg) {
+  if (!ok) fail(KG_E_INVALID, "%s", msg);
+}
+
+// ---------------------------------------------------------------- FP64 math
+
+Xf64 xf_identity() {
+  Xf64 X;
+  for (int i = 0; i < 9; i++) X.R[i] = (i % 4 == 0) ? 1.0 : 0.0;
+  X.t[0] = X.t[1] = X.t[2] = 0;
+  return X;
+}
+
+Xf64 xf_from(const double* R, const double* t) {
+  Xf64 X;
+  for (int i = 0; i < 9; i++) X.R[i] = R[i];
+  for (int i = 0; i < 3; i++) X.t[i] = t[i];
+  return X;
+}
+
+Xf64 xf_mul(const Xf64& A, const Xf64& B) {
+  Xf64 C;
+  for (int c = 0; c < 3; c++)
+    for (int r = 0; r < 3; r++)
+      C.R[c * 3 + r] = A.R[r] * B.R[c * 3] + A.R[3 + r] * B.R[c * 3 + 1] + A.R[6 + r] * B.R[c * 3 + 2];
+  for (int r = 0; r < 3; r++) C.t[r] = A.R[r] * B.t[0] + A.R[3 + r] * B.t[1] + A.R[6 + r] * B.t[2] + A.t[r];
+  return C;
+}
+
+Xf64 joint_xf(int type, const double* w, double q) {
+  Xf64 J = xf_identity();
+  if (type == 1) {
+    J.t[0] = w[0] * q; J.t[1] = w[1] * q; J.t[2] = w[2] * q;
+    return J;
+  }
+  double c = std::cos(q), s = std::sin(q), v = 1.0 - c;
+  for (int col = 0; col < 3; col++)
+    for (int row = 0; row < 3; row++) J.R[col * 3 + row] = w[row] * w[col] * v;
+  J.R[0] += c; J.R[4] += c; J.R[8] += c;
+  J.R[1] += s * w[2]; J.R[2] -= s * w[1];
+  J.R[3] -= s * w[2]; J.R[5] += s * w[0];
+  J.R[6] += s * w[1]; J.R[7] -= s * w[0];
+  return J;
+}
+
+void robot_fk(const kg_robot_s* r, const double* q, std::vector<Xf64>& T) {
+  T.resize(r->n);
+  for (int l = 0; l < r->n; l++) {
+    const LinkDesc& L = r->links[l];
+    Xf64 T0;
+    for (int i = 0; i < 9; i++) T0.R[i] = L.T0[i];
+    for (int i = 0; i < 3; i++) T0.t[i] = L.T0[9 + i];
+    Xf64 Lx = xf_mul(T0, joint_xf(L.jtype, L.axis, q[l]));
+    T[l] = L.parent < 0 ? Lx : xf_mul(T[L.parent], Lx);
+  }
+}
+
+void world_aabb(const GeomDesc& g, const Xf64& X, double* lo, double* hi) {
+  for (int r = 0; r < 3; r++) {
+    double wc = X.R[r] * g.lc[0] + X.R[3 + r] * g.lc[1] + X.R[6 + r] * g.lc[2] + X.t[r];
+    double ext = std::fabs(X.R[r]) * g.le[0] + std::fabs(X.R[3 + r]) * g.le[1] + std::fabs(X.R[6 + r]) * g.le[2];
+    lo[r] = wc - ext;
+    hi[r] = wc + ext;
+  }
+}
+
+double norm3(const double* t) { return std::sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]); }
+
+// ---------------------------------------------------------------- pool
+
+template <class T>
+void upload(DevBuf& b, const std::vector<T>& v) {
+  hip_check(b.ensure(std::max<size_t>(sizeof(T), v.size() * sizeof(T))), "hipMalloc(pool)");
+  if (!v.empty()) hip_check(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy(pool)");
+}
+
+void sync_pool(kg_ctx_s* c) {
+  if (!c->pool_dirty) return;
+  hip_check(hipSetDevice(c->device), "hipSetDevice");
+  upload(c->dnodes, c->nodes);
+  upload(c->dtv0, c->tv0);
+  upload(c->dtv1, c->tv1);
+  upload(c->dtv2, c->tv2);
+  upload(c->dtv64, c->tv64);
+  upload(c->dgeoms, c->geoms);
+  upload(c->dtrisrc, c->trisrc);
+  c->pool.nodes = c->dnodes.as<ObbNode>();
+  c->pool.tv0 = c->dtv0.as<float4>();
+  c->pool.tv1 = c->dtv1.as<float4>();
+  c->pool.tv2 = c->dtv2.as<float4>();
+  c->pool.tv64 = c->dtv64.as<double>();
+  c->pool.geoms = c->dgeoms.as<GeomDesc>();
+  c->pool.trisrc = c->dtrisrc.as<int>();
+  c->pool_dirty = false;
+  c->pool_version++;
+}
+
+bool geom_empty(const kg_geom_s* g) {
+  return !g || g->kind != 0 || g->ctx->geoms[g->index].ntris == 0;
+}
+
+// ---------------------------------------------------------------- world mask
+
+int robot_id(const kg_world_s* w, int r) {
+  int id = (int)w->terrains.size() + (int)w->objects.size();
+  for (int i = 0; i < r; i++) id += w->robots[i]->n + 1;
+  return id;
+}
+int link_id(const kg_world_s* w, int r, int l) { return robot_id(w, r) + 1 + l; }
+int num_ids(const kg_world_s* w) {
+  int n = (int)w->terrains.size() + (int)w->objects.size() + (int)w->robots.size();
+  for (auto* r : w->robots) n += r->n;
+  return n;
+}
+
+struct Override {
+  int a, b, en;
+};
+std::vector<std::pair<const kg_world_s*, std::vector<Override>>> g_overrides;  // per world
+
+std::vector<Override>& overrides(const kg_world_s* w) {
+  for (auto& p : g_overrides)
+    if (p.first == w) return p.second;
+  g_overrides.push_back({w, {}});
+  return g_overrides.back().second;
+}
+
+// WorldPlannerSettings::InitializeDefault (PlannerSettings.cpp:16-41)
+void rebuild_mask(kg_world_s* w) {
+  int n = num_ids(w);
+  w->nids = n;
+  w->mask.assign((size_t)n * n, 1);
+  for (int i = 0; i < n; i++) w->mask[(size_t)i * n + i] = 0;
+  for (size_t r = 0; r < w->robots.size(); r++) {
+    int k = robot_id(w, (int)r);
+    w->mask[(size_t)k * n + k] = 1;
+    int base = link_id(w, (int)r, 0);
+    const kg_robot_s* R = w->robots[r];
+    for (int j = 0; j < R->n; j++) {
+      w->mask[(size_t)(base + j) * n + k] = 0;
+      w->mask[(size_t)k * n + base + j] = 0;
+    }
+    for (int j = 0; j < R->n; j++)
+      for (int l = 0; l < R->n; l++) w->mask[(size_t)(base + j) * n + base + l] = R->selfcol[j * R->n + l];
+    for (int j = 0; j < R->n; j++)
+      if (R->links[j].parent == -1)
+        for (size_t t = 0; t < w->terrains.size(); t++) {
+          w->mask[(size_t)(base + j) * n + t] = 0;
+          w->mask[(size_t)t * n + base + j] = 0;
+        }
+  }
+  for (auto& o : overrides(w)) w->mask[(size_t)o.a * n + o.b] = (uint8_t)o.en;
+}
+
+inline bool en(const kg_world_s* w, int a, int b) { return w->mask[(size_t)a * w->nids + b] != 0; }
+
+int world_version(const kg_world_s* w) {
+  int v = w->version * 7919;
+  for (auto* r : w->robots) v = v * 31 + r->version;
+  return v;
+}
+
+// Build (or reuse) the device tables for checking robot `ri` against the rest
+// of the world.
+RobotPlan& get_plan(kg_world_s* w, int ri) {
+  need(ri >= 0 && ri < (int)w->robots.size(), "robot index out of range");
+  kg_ctx_s* c = w->ctx;
+  sync_pool(c);
+  if (w->plans.size() < w->robots.size()) w->plans.resize(w->robots.size());
+  RobotPlan& P = w->plans[ri];
+  int ver = world_version(w) * 131 + c->pool_version;
+  if (P.built_version == ver) return P;
+  rebuild_mask(w);
+  kg_robot_s* R = w->robots[ri];
+  // links
+  std::vector<LinkDesc> links = R->links;
+  for (int l = 0; l < R->n; l++) {
+    links[l].geom = geom_empty(R->link_geoms[l]) ? -1 : R->link_geoms[l]->index;
+    links[l].wid = link_id(w, ri, l);
+  }
+  // env items: GetGeometries over terrains, rigid objects, other robots
+  P.env.clear();
+  auto add_env = [&](kg_geom_s* g, const Xf64& X, int wid) {
+    if (geom_empty(g)) return;
+    EnvItem E;
+    for (int i = 0; i < 9; i++) E.X[i] = X.R[i];
+    for (int i = 0; i < 3; i++) E.X[9 + i] = X.t[i];
+    world_aabb(c->geoms[g->index], X, E.lo, E.hi);
+    E.geom = g->index;
+    E.wid = wid;
+    P.env.push_back(E);
+  };
+  for (size_t t = 0; t < w->terrains.size(); t++) add_env(w->terrains[t], w->terrains[t]->X, (int)t);
+  for (size_t o = 0; o < w->objects.size(); o++)
+    add_env(w->objects[o].first, w->objects[o].second, (int)(w->terrains.size() + o));
+  for (size_t r2 = 0; r2 < w->robots.size(); r2++) {
+    if ((int)r2 == ri) continue;
+    kg_robot_s* R2 = w->robots[r2];
+    for (int l = 0; l < R2->n; l++) add_env(R2->link_geoms[l], R2->Tw[l], link_id(w, (int)r2, l));
+  }
+  P.nenv = (int)P.env.size();
+  need(P.nenv < 32768, "too many environment geometries (limit 32767)");
+  P.lemask.assign((size_t)R->n * std::max(1, P.nenv), 0);
+  for (int l = 0; l < R->n; l++) {
+    if (links[l].geom < 0) continue;
+    for (int e = 0; e < P.nenv; e++) {
+      int a = links[l].wid, b = P.env[e].wid;
+      P.lemask[(size_t)l * P.nenv + e] = (en(w, a, b) || en(w, b, a)) ? 1 : 0;
+    }
+  }
+  // self pairs: (i,j) i<j over links with geometry, mask (i,j)||(i,i)
+  // (PlannerSettings.cpp:258)
+  P.selfpairs.clear();
+  for (int i = 0; i < R->n; i++) {
+    if (links[i].geom < 0) continue;
+    for (int j = i + 1; j < R->n; j++) {
+      if (links[j].geom < 0) continue;
+      int a = links[i].wid, b = links[j].wid;
+      if (en(w, a, b) || en(w, a, a)) {
+        P.selfpairs.push_back(i);
+        P.selfpairs.push_back(j);
+      }
+    }
+  }
+  P.nself = (int)P.selfpairs.size() / 2;
+  // fixed pair list for distance queries
+  P.pairs.clear();
+  P.pair_a.clear();
+  P.pair_b.clear();
+  P.pair_kind.clear();
+  for (int l = 0; l < R->n; l++)
+    for (int e = 0; e < P.nenv; e++)
+      if (links[l].geom >= 0 && P.lemask[(size_t)l * P.nenv + e]) {
+        P.pairs.push_back({links[l].wid, P.env[e].wid});
+        P.pair_a.push_back(l);
+        P.pair_b.push_back(e);
+        P.pair_kind.push_back(0);
+      }
+  for (int k = 0; k < P.nself; k++) {
+    int i = P.selfpairs[2 * k], j = P.selfpairs[2 * k + 1];
+    P.pairs.push_back({links[i].wid, links[j].wid});
+    P.pair_a.push_back(i);
+    P.pair_b.push_back(j);
+    P.pair_kind.push_back(1);
+  }
+  // error bounds of the FP32 narrow phase
+  double reach = 0, Pmax = 0;
+  for (int l = 0; l < R->n; l++) {
+    reach += norm3(links[l].T0 + 9);
+    if (links[l].jtype == 1) reach += std::max(std::fabs(R->qmin[l]), std::fabs(R->qmax[l]));
+    if (links[l].geom >= 0) Pmax = std::max(Pmax, c->geoms[links[l].geom].rmax);
+  }
+  reach += Pmax;
+  double L = 2 * reach + Pmax;
+  int dA = 0, dB = 0;
+  for (int l = 0; l < R->n; l++)
+    if (links[l].geom >= 0) dA = std::max(dA, c->geoms[links[l].geom].depth);
+  for (auto& E : P.env) {
+    L = std::max(L, norm3(E.X + 9) + reach + Pmax);
+    L = std::max(L, c->geoms[E.geom].rmax);
+    dB = std::max(dB, c->geoms[E.geom].depth);
+  }
+  const double u = std::ldexp(1.0, -24);
+  double ev = 2 * u * (10 * Pmax + 5 * L);
+  P.tol32.ev = (float)ev;
+  P.tol32.tau = (float)(2 * ev);
+  P.tol32.pad_bv = (float)(2 * ev + 32 * u * L);
+  P.tol32.pad_rel = 2e-6f;
+  P.tol64.ev = 0;
+  P.tol64.tau = 0;
+  P.tol64.pad_bv = (float)(1e-9 + 1e-12 * L);
+  P.tol64.pad_rel = 2e-6f;
+  P.maxdepth_a = dA;
+  P.maxdepth_b = std::max(dA, dB);
+  // device tables
+  std::vector<double> qmin = R->qmin, qmax = R->qmax;
+  upload(P.dlinks, links);
+  upload(P.dqmin, qmin);
+  upload(P.dqmax, qmax);
+  upload(P.denv, P.env);
+  upload(P.dlemask, P.lemask);
+  upload(P.dself, P.selfpairs);
+  P.dev.links = P.dlinks.as<LinkDesc>();
+  P.dev.qmin = P.dqmin.as<double>();
+  P.dev.qmax = P.dqmax.as<double>();
+  P.dev.env = P.denv.as<EnvItem>();
+  P.dev.lemask = P.dlemask.as<uint8_t>();
+  P.dev.selfpairs = P.dself.as<int>();
+  P.dev.nlinks = R->n;
+  P.dev.nenv = P.nenv;
+  P.dev.nself = P.nself;
+  P.built_version = ver;
+  return P;
+}
+
+int env_int(const char* name, int dflt) {
+  const char* v = std::getenv(name);
+  return v ? std::atoi(v) : dflt;
+}
+
+struct BatchStats {
+  int64_t redo = 0, items = 0, spills = 0;
+  int64_t t_prep_ns = 0, t_trav32_ns = 0, t_trav64_ns = 0, box_tests = 0, tri_tests = 0;
+};
+std::vector<std::pair<const kg_world_s*, BatchStats>> g_stats;
+BatchStats& stats_of(const kg_world_s* w) {
+  for (auto& p : g_stats)
+    if (p.first == w) return p.second;
+  g_stats.push_back({w, {}});
+  return g_stats.back().second;
+}
+
+// Batched feasibility on device buffers.  Q is float (Qf) or double (Qd).
+void feasible_dev(kg_world_s* w, int ri, const float* Qf, const double* Qd, int N, uint8_t* feas, uint8_t* env,
+                  uint8_t* self, int32_t* fp, hipStream_t s) {
+  kg_ctx_s* c = w->ctx;
+  hip_check(hipSetDevice(c->device), "hipSetDevice");
+  RobotPlan& P = get_plan(w, ri);
+  BatchStats& st = stats_of(w);
+  st = BatchStats{};
+  if (N <= 0) return;
+  const int n = P.dev.nlinks;
+  const int mode = fp ? KG_MODE_ORDERED : ((env || self) ? KG_MODE_GROUPS : KG_MODE_ANY);
+  hip_check(c->wsT64.ensure((size_t)N * n * 12 * sizeof(double)), "hipMalloc(T64)");
+  hip_check(c->wsLB.ensure((size_t)N * n * 6 * sizeof(double)), "hipMalloc(LB)");
+  hip_check(c->wsJL.ensure((size_t)N), "hipMalloc(jl)");
+  hip_check(c->wsCounts.ensure((size_t)(N + 1) * sizeof(int)), "hipMalloc(counts)");
+  hip_check(c->wsOffsets.ensure((size_t)(N + 1) * sizeof(int)), "hipMalloc(offsets)");
+  PrepArgs pa{};
+  pa.Qf = Qf;
+  pa.Qd = Qd;
+  pa.N = N;
+  pa.mode = mode;
+  pa.plan = P.dev;
+  pa.pool = c->pool;
+  pa.T64 = c->wsT64.as<double>();
+  pa.LB = c->wsLB.as<double>();
+  pa.jl = c->wsJL.as<uint8_t>();
+  pa.counts = c->wsCounts.as<int>();
+  pa.offsets = c->wsOffsets.as<int>();
+  hipEvent_t ev[6];
+  for (auto& e : ev) hip_check(hipEventCreate(&e), "hipEventCreate");
+  struct EvGuard {
+    hipEvent_t* e;
+    ~EvGuard() {
+      for (int i = 0; i < 6; i++) (void)hipEventDestroy(e[i]);
+    }
+  } evg{ev};
+  hip_check(hipEventRecord(ev[0], s), "hipEventRecord");
+  hip_check(kgl_prep_count(pa, s), "prep_count");
+  hip_check(kgl_scan(pa.counts, pa.offsets, N + 1, c->wsScanTmp, s), "scan");
+  int nact = 0;
+  for (int l = 0; l < n; l++)
+    if (w->robots[ri]->link_geoms[l] && !geom_empty(w->robots[ri]->link_geoms[l])) nact++;
+  size_t bound = (size_t)nact * P.nenv + P.nself;
+  size_t total;
+  if (bound * (size_t)N * sizeof(PairItem) <= (512u << 20)) {
+    total = bound * (size_t)N;
+  } else {
+    int t = 0;
+    hip_check(hipMemcpyAsync(&t, pa.offsets + N, sizeof(int), hipMemcpyDeviceToHost, s), "copy total");
+    hip_check(hipStreamSynchronize(s), "sync");
+    total = (size_t)t;
+  }
+  hip_check(c->wsItems.ensure(std::max<size_t>(1, total) * sizeof(PairItem)), "hipMalloc(items)");
+  pa.items = c->wsItems.as<PairItem>();
+  hip_check(kgl_prep_fill(pa, s), "prep_fill");
+  hip_check(hipEventRecord(ev[1], s), "hipEventRecord");
+
+  // FP32 traversal
+  const int grid32 = kgl_collide_grid(0, c->num_cus);
+  const int grid64 = kgl_collide_grid(1, c->num_cus);
+  const int grid = std::max(grid32, grid64);
+  const int spill_cap = env_int("KG_SPILL_CAP", 16384);
+  hip_check(c->wsBest.ensure((size_t)N * 2 * sizeof(unsigned long long)), "hipMalloc(best)");
+  hip_check(c->wsUnc.ensure((size_t)N * 2 * sizeof(int)), "hipMalloc(unc)");
+  hip_check(c->wsChunk.ensure(64), "hipMalloc(chunk)");
+  hip_check(c->wsSpill.ensure((size_t)grid * spill_cap * sizeof(uint64_t)), "hipMalloc(spill)");
+  hip_check(c->wsStatus.ensure(64), "hipMalloc(status)");
+  hip_check(c->wsRedo.ensure((size_t)(N + 16) * sizeof(int)), "hipMalloc(redo)");
+  int* status = c->wsStatus.as<int>();
+  int* redo_count = c->wsRedo.as<int>();
+  int* redo_list = redo_count + 16;
+  hip_check(kgl_fill_int(status, 16, 0, s), "clear status");
+  hip_check(kgl_fill_int(redo_count, 1, 0, s), "clear redo");
+  unsigned long long* counters = (unsigned long long*)(status + 4);
+  hip_check(kgl_fill_int(status + 4, 4, 0, s), "clear counters");
+
+  TravArgs ta{};
+  ta.pool = c->pool;
+  ta.plan = P.dev;
+  ta.T64 = pa.T64;
+  ta.items = pa.items;
+  ta.offsets = pa.offsets;
+  ta.cfg_list = nullptr;
+  ta.ncfg = N;
+  ta.mode = mode;
+  ta.tol = P.tol32;
+  ta.best = c->wsBest.as<unsigned long long>();
+  ta.unc = c->wsUnc.as<int>();
+  ta.chunk_counter = c->wsChunk.as<int>();
+  ta.spill = c->wsSpill.as<uint64_t>();
+  ta.spill_cap = spill_cap;
+  ta.status = status;
+  ta.cpw = std::max(1, std::min(64, env_int("KG_CPW", 16)));
+  ta.grid = grid32;
+  ta.counters = counters;
+  hip_check(hipEventRecord(ev[2], s), "hipEventRecord");
+  hip_check(kgl_collide_f32(ta, s), "collide_f32");
+  hip_check(hipEventRecord(ev[3], s), "hipEventRecord");
+
+  FinArgs fa{};
+  fa.N = N;
+  fa.mode = mode;
+  fa.jl = pa.jl;
+  fa.best = ta.best;
+  fa.unc = ta.unc;
+  fa.items = pa.items;
+  fa.plan = P.dev;
+  fa.feasible = feas;
+  fa.env_hit = env;
+  fa.self_hit = self;
+  fa.first_pair = fp;
+  fa.redo_list = redo_list;
+  fa.redo_count = redo_count;
+  fa.final_pass = 0;
+  hip_check(kgl_finalize(fa, s), "finalize");
+
+  int nredo = 0;
+  hip_check(hipMemcpyAsync(&nredo, redo_count, sizeof(int), hipMemcpyDeviceToHost, s), "copy redo");
+  hip_check(hipStreamSynchronize(s), "sync");
+  st.redo = nredo;
+  if (nredo > 0) {
+    TravArgs tb = ta;
+    tb.cfg_list = redo_list;
+    tb.ncfg = nredo;
+    tb.tol = P.tol64;
+    tb.cpw = 4;
+    tb.grid = grid64;
+    tb.counters = nullptr;
+    hip_check(hipEventRecord(ev[4], s), "hipEventRecord");
+    hip_check(kgl_collide_f64(tb, s), "collide_f64");
+    hip_check(hipEventRecord(ev[5], s), "hipEventRecord");
+    FinArgs fb = fa;
+    fb.final_pass = 1;
+    hip_check(kgl_finalize_list(fb, redo_list, redo_count, nredo, s), "finalize_f64");
+  }
+  int hstat[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  hip_check(hipMemcpyAsync(hstat, status, 8 * sizeof(int), hipMemcpyDeviceToHost, s), "copy status");
+  hip_check(hipStreamSynchronize(s), "sync");
+  float ms = 0;
+  if (hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) st.t_prep_ns = (int64_t)(ms * 1e6);
+  if (hipEventElapsedTime(&ms, ev[2], ev[3]) == hipSuccess) st.t_trav32_ns = (int64_t)(ms * 1e6);
+  if (nredo > 0 && hipEventElapsedTime(&ms, ev[4], ev[5]) == hipSuccess) st.t_trav64_ns = (int64_t)(ms * 1e6);
+  unsigned long long cnt[2];
+  std::memcpy(cnt, hstat + 4, sizeof(cnt));
+  st.box_tests = (int64_t)cnt[0];
+  st.tri_tests = (int64_t)cnt[1];
+  {
+    int tot = 0;
+    hip_check(hipMemcpy(&tot, pa.offsets + N, sizeof(int), hipMemcpyDeviceToHost), "copy total");
+    st.items = tot;
+  }
+  if (hstat[0]) {
+    st.spills = 1;
+    if (nredo > 0) fail(KG_E_LIMIT, "BVH traversal stack exceeded its HBM spill capacity (KG_SPILL_CAP)");
+  }
+}
+
+kg_world_s* solo_world(kg_robot_s* r);
+
+}  // namespace
+
+// ================================================================== C ABI
+
+extern "C" {
+
+const char* kg_last_error(void) { return g_err.c_str(); }
+int kg_version(void) { return 1; }
+
+int kg_ctx_create(int device, kg_ctx* out) {
+  return guard([&] {
+    need(out != nullptr, "out is NULL");
+    int ndev = 0;
+    hip_check(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
+    need(device >= 0 && device < ndev, "device index out of range");
+    hip_check(hipSetDevice(device), "hipSetDevice");
+    kg_ctx_s* c = new kg_ctx_s();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      delete c;
+      hip_check(e, "hipStreamCreate");
+    }
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+      c->num_cus = cus;
+    *out = c;
+  });
+}
+
+int kg_ctx_destroy(kg_ctx c) {
+  return guard([&] {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    DevBuf* bufs[] = {&c->dnodes, &c->dtv0, &c->dtv1, &c->dtv2, &c->dtv64, &c->dgeoms, &c->dtrisrc,
+                      &c->wsQ, &c->wsT64, &c->wsLB, &c->wsJL, &c->wsCounts, &c->wsOffsets, &c->wsItems,
+                      &c->wsScanTmp, &c->wsBest, &c->wsUnc, &c->wsChunk, &c->wsSpill, &c->wsStatus, &c->wsRedo,
+                      &c->wsOut8a, &c->wsOut8b, &c->wsOut8c, &c->wsOut32, &c->wsF, &c->wsG};
+    for (auto* b : bufs) b->release();
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+  });
+}
+
+int kg_ctx_synchronize(kg_ctx c) {
+  return guard([&] {
+    need(c != nullptr, "ctx is NULL");
+    hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  });
+}
+
+int kg_geom_create_trimesh(kg_ctx c, const double* V, int nv, const int32_t* T, int nt, kg_geom* out) {
+  return guard([&] {
+    need(c && out, "NULL argument");
+    need(nv >= 0 && nt >= 0, "negative size");
+    need(nt == 0 || (V && T), "NULL vertex or triangle array");
+    for (int i = 0; i < 3 * nt; i++) need(T[i] >= 0 && T[i] < nv, "triangle index out of range");
+    for (int i = 0; i < 3 * nv; i++) need(std::isfinite(V[i]), "non-finite vertex coordinate");
+    kg_geom_s* g = new kg_geom_s();
+    g->ctx = c;
+    g->kind = 0;
+    g->X = xf_identity();
+    g->V.assign(V, V + 3 * (size_t)nv);
+    g->T.assign(T, T + 3 * (size_t)nt);
+    GeomDesc d{};
+    d.ntris = nt;
+    d.tri0 = (int)c->tv0.size();
+    double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0}, rmax = 0;
+    if (nv > 0)
+      for (int k = 0; k < 3; k++) lo[k] = hi[k] = V[k];
+    for (int i = 0; i < nv; i++) {
+      for (int k = 0; k < 3; k++) {
+        lo[k] = std::min(lo[k], V[3 * i + k]);
+        hi[k] = std::max(hi[k], V[3 * i + k]);
+      }
+      rmax = std::max(rmax, norm3(V + 3 * i));
+    }
+    for (int k = 0; k < 3; k++) {
+      d.lc[k] = 0.5 * (lo[k] + hi[k]);
+      d.le[k] = 0.5 * (hi[k] - lo[k]);
+    }
+    d.rmax = rmax;
+    if (nt > 0) {
+      BuiltTree bt;
+      kg_build_obb_tree(V, nv, T, nt, bt);
+      need(c->nodes.size() + bt.nodes.size() < (1u << 26), "geometry pool exceeds 2^26 BVH nodes");
+      d.root = (int)c->nodes.size();
+      d.nnodes = (int)bt.nodes.size();
+      d.depth = bt.depth;
+      for (auto nd : bt.nodes) {
+        nd.child = nd.child >= 0 ? nd.child + d.root : ~(~nd.child + d.tri0);
+        c->nodes.push_back(nd);
+      }
+      for (int li = 0; li < nt; li++) {
+        int t = bt.order[li];
+        const double* p0 = V + 3 * T[3 * t];
+        const double* p1 = V + 3 * T[3 * t + 1];
+        const double* p2 = V + 3 * T[3 * t + 2];
+        c->tv0.push_back(make_float4((float)p0[0], (float)p0[1], (float)p0[2], 0.f));
+        c->tv1.push_back(make_float4((float)p1[0], (float)p1[1], (float)p1[2], 0.f));
+        c->tv2.push_back(make_float4((float)p2[0], (float)p2[1], (float)p2[2], 0.f));
+        for (int k = 0; k < 3; k++) c->tv64.push_back(p0[k]);
+        for (int k = 0; k < 3; k++) c->tv64.push_back(p1[k]);
+        for (int k = 0; k < 3; k++) c->tv64.push_back(p2[k]);
+        c->trisrc.push_back(t);
+      }
+    } else {
+      d.root = -1;
+    }
+    g->index = (int)c->geoms.size();
+    c->geoms.push_back(d);
+    c->pool_dirty = true;
+    *out = g;
+  });
+}
+
+int kg_geom_create_sdf(kg_ctx c, const double bmin[3], const double bmax[3], const int32_t dims[3],
+                       const float* values, kg_geom* out) {
+  return guard([&] {
+    need(c && bmin && bmax && dims && values && out, "NULL argument");
+    for (int k = 0; k < 3; k++) {
+      need(dims[k] >= 1, "SDF dims must be >= 1");
+      need(bmax[k] > bmin[k], "SDF bmax must exceed bmin");
+    }
+    size_t nvals = (size_t)dims[0] * dims[1] * dims[2];
+    kg_geom_s* g = new kg_geom_s();
+    g->ctx = c;
+    g->kind = 1;
+    g->index = -1;
+    g->X = xf_identity();
+    for (int k = 0; k < 3; k++) {
+      g->dims[k] = dims[k];
+      g->bmin[k] = bmin[k];
+      g->bmax[k] = bmax[k];
+    }
+    hip_check(hipSetDevice(c->device), "hipSetDevice");
+    hipError_t e = g->dvals.ensure(nvals * sizeof(float));
+    if (e != hipSuccess) {
+      delete g;
+      hip_check(e, "hipMalloc(sdf)");
+    }
+    hip_check(hipMemcpy(g->dvals.p, values, nvals * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy(sdf)");
+    *out = g;
+  });
+}
+
+int kg_geom_destroy(kg_geom g) {
+  return guard([&] {
+    if (!g) return;
+    g->dvals.release();
+    g->alive = false;
+    // mesh data stays in the context pool until the context is destroyed
+    delete g;
+  });
+}
+
+int kg_geom_set_transform(kg_geom g, const double R[9], const double t[3]) {
+  return guard([&] {
+    need(g && R && t, "NULL argument");
+    g->X = xf_from(R, t);
+  });
+}
+
+int kg_geom_get_transform(kg_geom g, double R[9], double t[3]) {
+  return guard([&] {
+    need(g && R && t, "NULL argument");
+    std::memcpy(R, g->X.R, sizeof(g->X.R));
+    std::memcpy(t, g->X.t, sizeof(g->X.t));
+  });
+}
+
+int kg_geom_get_bb(kg_geom g, double bmin[3], double bmax[3]) {
+  return guard([&] {
+    need(g && bmin && bmax, "NULL argument");
+    if (g->kind == 1) {
+      // transformed grid box
+      GeomDesc d{};
+      for (int k = 0; k < 3; k++) {
+        d.lc[k] = 0.5 * (g->bmin[k] + g->bmax[k]);
+        d.le[k] = 0.5 * (g->bmax[k] - g->bmin[k]);
+      }
+      world_aabb(d, g->X, bmin, bmax);
+      return;
+    }
+    const GeomDesc& d = g->ctx->geoms[g->index];
+    if (d.ntris == 0) {
+      for (int k = 0; k < 3; k++) {
+        bmin[k] = INFINITY;
+        bmax[k] = -INFINITY;
+      }
+      return;
+    }
+    world_aabb(d, g->X, bmin, bmax);
+  });
+}
+
+int kg_robot_create(kg_ctx c, int n, const int32_t* parents, const double* T0, const double* axes,
+                    const int32_t* jtype, const double* qmin, const double* qmax, kg_robot* out) {
+  return guard([&] {
+    need(c && parents && T0 && axes && jtype && qmin && qmax && out, "NULL argument");
+    need(n >= 1 && n <= KG_MAX_LINKS, "robot must have 1..64 links");
+    kg_robot_s* r = new kg_robot_s();
+    r->ctx = c;
+    r->n = n;
+    r->links.resize(n);
+    for (int l = 0; l < n; l++) {
+      if (!(parents[l] >= -1 && parents[l] < l)) {
+        delete r;
+        fail(KG_E_INVALID, "parents must be topologically sorted (parent < index)");
+      }
+      if (jtype[l] != 0 && jtype[l] != 1) {
+        delete r;
+        fail(KG_E_INVALID, "joint type must be 0 (revolute) or 1 (prismatic)");
+      }
+      LinkDesc& L = r->links[l];
+      std::memcpy(L.T0, T0 + 12 * l, 12 * sizeof(double));
+      std::memcpy(L.axis, axes + 3 * l, 3 * sizeof(double));
+      L.parent = parents[l];
+      L.jtype = jtype[l];
+      L.geom = -1;
+      L.wid = l + 1;
+    }
+    r->link_geoms.assign(n, nullptr);
+    r->qmin.assign(qmin, qmin + n);
+    r->qmax.assign(qmax, qmax + n);
+    r->q.assign(n, 0.0);
+    for (int l = 0; l < n; l++) r->q[l] = std::min(std::max(0.0, r->qmin[l]), r->qmax[l]);
+    r->selfcol.assign((size_t)n * n, 0);
+    for (int i = 0; i < n; i++)
+      for (int j = i + 1; j < n; j++)
+        if (parents[j] != i && parents[i] != j) r->selfcol[i * n + j] = 1;
+    robot_fk(r, r->q.data(), r->Tw);
+    *out = r;
+  });
+}
+
+int kg_robot_destroy(kg_robot r) {
+  return guard([&] {
+    if (!r) return;
+    r->dlinks.release();
+    delete r;
+  });
+}
+
+int kg_robot_set_link_geom(kg_robot r, int link, kg_geom g) {
+  return guard([&] {
+    need(r != nullptr, "robot is NULL");
+    need(link >= 0 && link < r->n, "link index out of range");
+    need(!g || g->kind == 0, "link geometry must be a triangle mesh");
+    need(!g || g->ctx == r->ctx, "geometry belongs to another context");
+    r->link_geoms[link] = g;
+    if (g) g->X = r->Tw[link];
+    r->version++;
+  });
+}
+
+int kg_robot_enable_self_collision(kg_robot r, int a, int b, int enabled) {
+  return guard([&] {
+    need(r != nullptr, "robot is NULL");
+    need(a >= 0 && a < r->n && b >= 0 && b < r->n && a != b, "invalid link pair");
+    int i = std::min(a, b), j = std::max(a, b);
+    r->selfcol[i * r->n + j] = enabled ? 1 : 0;
+    r->version++;
+  });
+}
+
+int kg_robot_self_collision_enabled(kg_robot r, int a, int b, int* out) {
+  return guard([&] {
+    need(r && out, "NULL argument");
+    need(a >= 0 && a < r->n && b >= 0 && b < r->n, "invalid link pair");
+    int i = std::min(a, b), j = std::max(a, b);
+    *out = (i != j && r->selfcol[i * r->n + j]) ? 1 : 0;
+  });
+}
+
+int kg_robot_set_config(kg_robot r, const double* q) {
+  return guard([&] {
+    need(r && q, "NULL argument");
+    for (int l = 0; l < r->n; l++) need(std::isfinite(q[l]), "non-finite configuration");
+    r->q.assign(q, q + r->n);
+    robot_fk(r, q, r->Tw);
+    for (int l = 0; l < r->n; l++)
+      if (r->link_geoms[l]) r->link_geoms[l]->X = r->Tw[l];
+    r->version++;
+  });
+}
+
+int kg_robot_get_config(kg_robot r, double* q) {
+  return guard([&] {
+    need(r && q, "NULL argument");
+    std::memcpy(q, r->q.data(), r->n * sizeof(double));
+  });
+}
+
+int kg_robot_get_link_transform(kg_robot r, int link, double R[9], double t[3]) {
+  return guard([&] {
+    need(r && R && t, "NULL argument");
+    need(link >= 0 && link < r->n, "link index out of range");
+    std::memcpy(R, r->Tw[link].R, 9 * sizeof(double));
+    std::memcpy(t, r->Tw[link].t, 3 * sizeof(double));
+  });
+}
+
+int kg_fk_batch_dev(kg_robot r, const float* dQ, int N, float* dT, void* stream) {
+  return guard([&] {
+    need(r != nullptr, "robot is NULL");
+    need(N >= 0, "negative batch size");
+    need(N == 0 || (dQ && dT), "NULL buffer");
+    hip_check(hipSetDevice(r->ctx->device), "hipSetDevice");
+    if (r->dlinks.bytes == 0 || r->dirty) {
+      upload(r->dlinks, r->links);
+      r->dirty = false;
+    }
+    hipStream_t s = stream ? (hipStream_t)stream : r->ctx->stream;
+    hip_check(kgl_fk_f32(r->dlinks.as<LinkDesc>(), r->n, dQ, N, dT, s), "fk_f32");
+  });
+}
+
+int kg_fk_batch(kg_robot r, const float* Q, int N, float* T) {
+  return guard([&] {
+    need(r != nullptr, "robot is NULL");
+    need(N >= 0, "negative batch size");
+    if (N == 0) return;
+    need(Q && T, "NULL buffer");
+    kg_ctx_s* c = r->ctx;
+    hip_check(hipSetDevice(c->device), "hipSetDevice");
+    size_t qb = (size_t)N * r->n * sizeof(float), tb = qb * 12;
+    hip_check(c->wsQ.ensure(qb), "hipMalloc");
+    hip_check(c->wsF.ensure(tb), "hipMalloc");
+    hip_check(hipMemcpyAsync(c->wsQ.p, Q, qb, hipMemcpyHostToDevice, c->stream), "copy Q");
+    int rc = kg_fk_batch_dev(r, c->wsQ.as<float>(), N, c->wsF.as<float>(), c->stream);
+    if (rc != KG_OK) fail(rc, "%s", g_err.c_str());
+    hip_check(hipMemcpyAsync(T, c->wsF.p, tb, hipMemcpyDeviceToHost, c->stream), "copy T");
+    hip_check(hipStreamSynchronize(c->stream), "sync");
+  });
+}
+
+int kg_world_create(kg_ctx c, kg_world* out) {
+  return guard([&] {
+    need(c && out, "NULL argument");
+    kg_world_s* w = new kg_world_s();
+    w->ctx = c;
+    rebuild_mask(w);
+    *out = w;
+  });
+}
+
+int kg_world_destroy(kg_world w) {
+  return guard([&] {
+    if (!w) return;
+    for (auto& p : w->plans) {
+      DevBuf* bufs[] = {&p.dlinks, &p.dqmin, &p.dqmax, &p.denv, &p.dlemask, &p.dself};
+      for (auto* b : bufs) b->release();
+    }
+    for (auto it = g_overrides.begin(); it != g_overrides.end(); ++it)
+      if (it->first == w) {
+        g_overrides.erase(it);
+        break;
+      }
+    for (auto it = g_stats.begin(); it != g_stats.end(); ++it)
+      if (it->first == w) {
+        g_stats.erase(it);
+        break;
+      }
+    delete w;
+  });
+}
+
+static void world_changed(kg_world_s* w) {
+  overrides(w).clear();
+  w->version++;
+  rebuild_mask(w);
+}
+
+int kg_world_add_terrain(kg_world w, kg_geom g, int* id_out) {
+  return guard([&] {
+    need(w && g, "NULL argument");
+    need(g->kind == 0, "only triangle-mesh terrains are supported");
+    need(g->ctx == w->ctx, "geometry belongs to another context");
+    w->terrains.push_back(g);
+    world_changed(w);
+    if (id_out) *id_out = (int)w->terrains.size() - 1;
+  });
+}
+
+int kg_world_add_rigid_object(kg_world w, kg_geom g, const double R[9], const double t[3], int* id_out) {
+  return guard([&] {
+    need(w && g && R && t, "NULL argument");
+    need(g->kind == 0, "only triangle-mesh rigid objects are supported");
+    need(g->ctx == w->ctx, "geometry belongs to another context");
+    w->objects.push_back({g, xf_from(R, t)});
+    world_changed(w);
+    if (id_out) *id_out = (int)(w->terrains.size() + w->objects.size()) - 1;
+  });
+}
+
+int kg_world_set_rigid_object_transform(kg_world w, int oi, const double R[9], const double t[3]) {
+  return guard([&] {
+    need(w && R && t, "NULL argument");
+    need(oi >= 0 && oi < (int)w->objects.size(), "object index out of range");
+    w->objects[oi].second = xf_from(R, t);
+    w->version++;
+  });
+}
+
+int kg_world_add_robot(kg_world w, kg_robot r, int* id_out) {
+  return guard([&] {
+    need(w && r, "NULL argument");
+    need(r->ctx == w->ctx, "robot belongs to another context");
+    for (auto* x : w->robots) need(x != r, "robot already in this world");
+    w->robots.push_back(r);
+    world_changed(w);
+    if (id_out) *id_out = robot_id(w, (int)w->robots.size() - 1);
+  });
+}
+
+int kg_world_num_ids(kg_world w) { return w ? num_ids(w) : 0; }
+
+int kg_world_set_collision_enabled(kg_world w, int a, int b, int enabled) {
+  return guard([&] {
+    need(w != nullptr, "world is NULL");
+    int n = num_ids(w);
+    need(a >= 0 && a < n && b >= 0 && b < n, "world ID out of range");
+    overrides(w).push_back({a, b, enabled ? 1 : 0});
+    w->version++;
+    rebuild_mask(w);
+  });
+}
+
+int kg_world_get_collision_enabled(kg_world w, int a, int b, int* out) {
+  return guard([&] {
+    need(w && out, "NULL argument");
+    rebuild_mask(w);
+    need(a >= 0 && a < w->nids && b >= 0 && b < w->nids, "world ID out of range");
+    *out = en(w, a, b) ? 1 : 0;
+  });
+}
+
+int kg_world_pair_list(kg_world w, int ri, int32_t* pairs, int maxp, int* np) {
+  return guard([&] {
+    need(w && np, "NULL argument");
+    RobotPlan& P = get_plan(w, ri);
+    *np = (int)P.pairs.size();
+    for (int i = 0; i < (int)P.pairs.size() && i < maxp && pairs; i++) {
+      pairs[2 * i] = P.pairs[i].first;
+      pairs[2 * i + 1] = P.pairs[i].second;
+    }
+  });
+}
+
+int kg_feasible_batch_dev(kg_world w, int ri, const float* dQ, int N, uint8_t* feas, uint8_t* env, uint8_t* self,
+                          int32_t* fp, void* stream) {
+  return guard([&] {
+    need(w != nullptr, "world is NULL");
+    need(N >= 0, "negative batch size");
+    need(N == 0 || (dQ && feas), "NULL buffer");
+    hipStream_t s = stream ? (hipStream_t)stream : w->ctx->stream;
+    feasible_dev(w, ri, dQ, nullptr, N, feas, env, self, fp, s);
+  });
+}
+
+int kg_feasible_batch(kg_world w, int ri, const float* Q, int N, uint8_t* feas, uint8_t* env, uint8_t* self,
+                      int32_t* fp) {
+  return guard([&] {
+    need(w != nullptr, "world is NULL");
+    need(N >= 0, "negative batch size");
+    need(ri >= 0 && ri < (int)w->robots.size(), "robot index out of range");
+    if (N == 0) return;
+    need(Q && feas, "NULL buffer");
+    kg_ctx_s* c = w->ctx;
+    hip_check(hipSetDevice(c->device), "hipSetDevice");
+    int n = w->robots[ri]->n;
+    size_t qb = (size_t)N * n * sizeof(float);
+    hip_check(c->wsQ.ensure(qb), "hipMalloc");
+    hip_check(c->wsOut8a.ensure(N), "hipMalloc");
+    hip_check(c->wsOut8b.ensure(N), "hipMalloc");
+    hip_check(c->wsOut8c.ensure(N), "hipMalloc");
+    hip_check(c->wsOut32.ensure((size_t)N * 2 * sizeof(int)), "hipMalloc");
+    hip_check(hipMemcpyAsync(c->wsQ.p, Q, qb, hipMemcpyHostToDevice, c->stream), "copy Q");
+    uint8_t* dfe = c->wsOut8a.as<uint8_t>();
+    uint8_t* den = env ? c->wsOut8b.as<uint8_t>() : nullptr;
+    uint8_t* dse = self ? c->wsOut8c.as<uint8_t>() : nullptr;
+    int32_t* dfp = fp ? c->wsOut32.as<int32_t>() : nullptr;
+    feasible_dev(w, ri, c->wsQ.as<float>(), nullptr, N, dfe, den, dse, dfp, c->stream);
+    hip_check(hipMemcpyAsync(feas, dfe, N, hipMemcpyDeviceToHost, c->stream), "copy out");
+    if (env) hip_check(hipMemcpyAsync(env, den, N, hipMemcpyDeviceToHost, c->stream), "copy out");
+    if (self) hip_check(hipMemcpyAsync(self, dse, N, hipMemcpyDeviceToHost, c->stream), "copy out");
+    if (fp) hip_check(hipMemcpyAsync(fp, dfp, (size_t)N * 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream), "copy");
+    hip_check(hipStreamSynchronize(c->stream), "sync");
+  });
+}
+
+int kg_world_last_stats(kg_world w, int64_t* out, int nstats) {
+  return guard([&] {
+    need(w && out, "NULL argument");
+    BatchStats& st = stats_of(w);
+    int64_t v[8] = {st.redo, st.items, st.spills, st.t_prep_ns, st.t_trav32_ns, st.t_trav64_ns, st.box_tests,
+                    st.tri_tests};
+    for (int i = 0; i < nstats && i < 8; i++) out[i] = v[i];
+  });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- robot-only world
+
+namespace {
+std::vector<std::pair<const kg_robot_s*, kg_world_s*>> g_solo;
+
+kg_world_s* solo_world(kg_robot_s* r) {
+  for (auto& p : g_solo)
+    if (p.first == r) return p.second;
+  kg_world_s* w = new kg_world_s();
+  w->ctx = r->ctx;
+  w->robots.push_back(r);
+  rebuild_mask(w);
+  g_solo.push_back({r, w});
+  return w;
+}
+}  // namespace
+
+extern "C" int kg_robot_self_collides(kg_robot r, int* out) {
+  return guard([&] {
+    need(r && out, "NULL argument");
+    kg_world_s* w = solo_world(r);
+    std::vector<float> q(r->n);
+    // the current configuration in FP64
+    kg_ctx_s* c = r->ctx;
+    hip_check(hipSetDevice(c->device), "hipSetDevice");
+    hip_check(c->wsG.ensure(r->n * sizeof(double) + 64), "hipMalloc");
+    hip_check(hipMemcpyAsync(c->wsG.p, r->q.data(), r->n * sizeof(double), hipMemcpyHostToDevice, c->stream),
+              "copy q");
+    hip_check(c->wsOut8a.ensure(64), "hipMalloc");
+    hip_check(c->wsOut8b.ensure(64), "hipMalloc");
+    hip_check(c->wsOut8c.ensure(64), "hipMalloc");
+    feasible_dev(w, 0, nullptr, c->wsG.as<double>(), 1, c->wsOut8a.as<uint8_t>(), c->wsOut8b.as<uint8_t>(),
+                 c->wsOut8c.as<uint8_t>(), nullptr, c->stream);
+    uint8_t h = 0;
+    hip_check(hipMemcpyAsync(&h, c->wsOut8c.p, 1, hipMemcpyDeviceToHost, c->stream), "copy");
+    hip_check(hipStreamSynchronize(c->stream), "sync");
+    *out = h;
+  });
+}
+
+// ---------------------------------------------------------------- single pairs
+//
+// A pair query (a at X_a, b at X_b) runs through the batch pipeline on an
+// internal world: a one-link robot whose T0_Parent is X_a (q = 0 gives the
+// joint transform I exactly, so the link frame is X_a bit for bit) and a rigid
+// object b at X_b.
+namespace {
+struct PairWorld {
+  kg_robot_s* r = nullptr;
+  kg_world_s* w = nullptr;
+};
+std::vector<std::pair<const kg_ctx_s*, PairWorld>> g_pairworlds;
+
+PairWorld& pair_world(kg_ctx_s* c, kg_geom_s* a, kg_geom_s* b) {
+  PairWorld* pw = nullptr;
+  for (auto& p : g_pairworlds)
+    if (p.first == c) pw = &p.second;
+  if (!pw) {
+    g_pairworlds.push_back({c, PairWorld{}});
+    pw = &g_pairworlds.back().second;
+    kg_robot_s* r = new kg_robot_s();
+    r->ctx = c;
+    r->n = 1;
+    r->links.resize(1);
+    LinkDesc& L = r->links[0];
+    for (int i = 0; i < 12; i++) L.T0[i] = (i % 4 == 0 && i < 9) ? 1.0 : 0.0;
+    L.axis[0] = 0; L.axis[1] = 0; L.axis[2] = 1;
+    L.parent = -1;
+    L.jtype = 0;
+    L.geom = -1;
+    L.wid = 1;
+    r->link_geoms.assign(1, nullptr);
+    r->qmin.assign(1, 0.0);
+    r->qmax.assign(1, 0.0);
+    r->q.assign(1, 0.0);
+    r->selfcol.assign(1, 0);
+    r->Tw.assign(1, xf_identity());
+    kg_world_s* w = new kg_world_s();
+    w->ctx = c;
+    w->robots.push_back(r);
+    w->objects.push_back({nullptr, xf_identity()});
+    pw->r = r;
+    pw->w = w;
+  }
+  for (int i = 0; i < 9; i++) pw->r->links[0].T0[i] = a->X.R[i];
+  for (int i = 0; i < 3; i++) pw->r->links[0].T0[9 + i] = a->X.t[i];
+  pw->r->link_geoms[0] = a;
+  pw->r->Tw[0] = a->X;
+  pw->r->version++;
+  pw->w->objects[0] = {b, b->X};
+  pw->w->version++;
+  rebuild_mask(pw->w);
+  return *pw;
+}
+}  // namespace
+
+extern "C" int kg_geom_collide(kg_geom a, kg_geom b, int* out) {
+  return guard([&] {
+    need(a && b && out, "NULL argument");
+    need(a->ctx == b->ctx, "geometries belong to different contexts");
+    *out = 0;
+    if (a->kind != 0 || b->kind != 0) fail(KG_E_UNSUPPORTED, "collide is implemented for TriangleMesh pairs");
+    if (geom_empty(a) || geom_empty(b)) return;  // Geometry3D.collides: empty -> False
+    kg_ctx_s* c = a->ctx;
+    PairWorld& pw = pair_world(c, a, b);
+    hip_check(hipSetDevice(c->device), "hipSetDevice");
+    hip_check(c->wsG.ensure(64), "hipMalloc");
+    double q0 = 0.0;
+    hip_check(hipMemcpyAsync(c->wsG.p, &q0, sizeof(double), hipMemcpyHostToDevice, c->stream), "copy q");
+    hip_check(c->wsOut8a.ensure(64), "hipMalloc");
+    hip_check(c->wsOut8b.ensure(64), "hipMalloc");
+    feasible_dev(pw.w, 0, nullptr, c->wsG.as<double>(), 1, c->wsOut8a.as<uint8_t>(), c->wsOut8b.as<uint8_t>(),
+                 nullptr, nullptr, c->stream);
+    uint8_t h = 0;
+    hip_check(hipMemcpyAsync(&h, c->wsOut8b.p, 1, hipMemcpyDeviceToHost, c->stream), "copy");
+    hip_check(hipStreamSynchronize(c->stream), "sync");
+    *out = h;
+  });
+}
+
+#define KG_TODO(sig, what)                                                      \
+  extern "C" int sig {                                                          \
+    return guard([&] { fail(KG_E_UNSUPPORTED, "%s: not available yet", what); }); \
+  }
+KG_TODO(kg_geom_within_distance(kg_geom, kg_geom, double, int*), "kg_geom_within_distance")
+KG_TODO(kg_geom_distance(kg_geom, kg_geom, const kg_dist_settings*, kg_dist_result*), "kg_geom_distance")
+KG_TODO(kg_geom_distance_point(kg_geom, const double*, const kg_dist_settings*, kg_dist_result*),
+        "kg_geom_distance_point")
+KG_TODO(kg_sdf_eval(kg_geom, const float*, int, float*, float*), "kg_sdf_eval")
+KG_TODO(kg_distance_batch(kg_world, int, const float*, int, float*), "kg_distance_batch")
+KG_TODO(kg_distance_batch_dev(kg_world, int, const float*, int, float*, void*), "kg_distance_batch_dev")
+KG_TODO(kg_edge_batch(kg_world, int, const float*, const float*, int, double, int, float*), "kg_edge_batch")
+KG_TODO(kg_sdf_penalty_fwd(kg_robot, kg_geom, const float*, int, const float*, const int32_t*, float, float*),
+        "kg_sdf_penalty_fwd")
+KG_TODO(kg_sdf_penalty_bwd(kg_robot, kg_geom, const float*, int, const float*, const int32_t*, float, const float*,
+                           float*),
+        "kg_sdf_penalty_bwd")
+KG_TODO(kg_sdf_penalty_fwd_dev(kg_robot, kg_geom, const float*, int, const float*, const int32_t*, float, float*,
+                               void*),
+        "kg_sdf_penalty_fwd_dev")
+KG_TODO(kg_sdf_penalty_bwd_dev(kg_robot, kg_geom, const float*, int, const float*, const int32_t*, float,
+                               const float*, float*, void*),
+        "kg_sdf_penalty_bwd_dev")

@@ -1,0 +1,233 @@
+// Host-side objects behind the C-ABI handles and the kernel launch interface.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "kg_types.h"
+
+struct BuiltTree {
+  std::vector<ObbNode> nodes;  // local indices (child relative to node 0)
+  std::vector<int> order;      // leaf order -> original triangle
+  int depth = 0;
+};
+void kg_build_obb_tree(const double* V, int nv, const int32_t* T, int nt, BuiltTree& out);
+
+// Grow-only device buffer.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  hipError_t ensure(size_t n) {
+    if (n <= bytes) return hipSuccess;
+    if (p) {
+      hipError_t e = hipFree(p);
+      if (e != hipSuccess) return e;
+      p = nullptr;
+      bytes = 0;
+    }
+    size_t want = n < 256 ? 256 : n;
+    want = (want + 4095) & ~(size_t)4095;
+    hipError_t e = hipMalloc(&p, want);
+    if (e == hipSuccess) bytes = want;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T>
+  T* as() const {
+    return (T*)p;
+  }
+};
+
+struct PoolDev {
+  const ObbNode* nodes;
+  const float4* tv0;
+  const float4* tv1;
+  const float4* tv2;
+  const double* tv64;
+  const GeomDesc* geoms;
+  const int* trisrc;
+};
+
+struct SdfDev {
+  const float* v;
+  int dims[3];
+  float bmin[3];
+  float inv_h[3];  // 1/h
+  float h[3];
+};
+
+// Robot/world tables for one batched check, resident in HBM.
+struct PlanDev {
+  const LinkDesc* links;
+  const double* qmin;
+  const double* qmax;
+  const EnvItem* env;
+  const uint8_t* lemask;  // nlinks x nenv: link/env pair enabled
+  const int* selfpairs;   // 2 x nself, link indices, Klampt loop order
+  int nlinks;
+  int nenv;
+  int nself;
+  int pad_;
+};
+
+// Error-bound parameters of the FP32 narrow phase.
+struct Tol {
+  float ev;       // bound on a transformed vertex's position error (m)
+  float tau;      // separation certainty threshold (normalised gap)
+  float pad_bv;   // absolute bounding-volume padding
+  float pad_rel;  // relative bounding-volume padding (x extents)
+};
+
+enum { KG_MODE_ANY = 0, KG_MODE_GROUPS = 1, KG_MODE_ORDERED = 2 };
+
+// ---- kernel launchers (kg_kernels.hip)
+hipError_t kgl_fk_f32(const LinkDesc* links, int nlinks, const float* Q, int N, float* Tout, hipStream_t s);
+
+struct PrepArgs {
+  const float* Qf;     // either Qf or Qd
+  const double* Qd;
+  int N;
+  int mode;
+  PlanDev plan;
+  PoolDev pool;
+  double* T64;         // N x nlinks x 12
+  double* LB;          // N x nlinks x 6 (world AABBs)
+  uint8_t* jl;         // N: 1 if within joint limits
+  int* counts;         // N+1
+  int* offsets;        // N+1
+  PairItem* items;
+};
+hipError_t kgl_prep_count(const PrepArgs& a, hipStream_t s);
+hipError_t kgl_scan(int* counts, int* offsets, int n, DevBuf& tmp, hipStream_t s);
+hipError_t kgl_prep_fill(const PrepArgs& a, hipStream_t s);
+
+struct TravArgs {
+  PoolDev pool;
+  PlanDev plan;
+  const double* T64;
+  const PairItem* items;
+  const int* offsets;     // per config item range
+  const int* cfg_list;    // nullable: configs to process (else 0..N-1)
+  int ncfg;
+  int mode;
+  Tol tol;
+  unsigned long long* best;   // N x 2: (rank<<32 | item) per group, ~0 = none
+  int* unc;                   // N x 2: min uncertain rank per group
+  int* chunk_counter;
+  uint64_t* spill;            // per-wave global overflow stacks
+  int spill_cap;              // entries per wave
+  int* status;                // [0]: spill overflow flag
+  int cpw;                    // configs per chunk
+  int grid;                   // number of waves launched
+  unsigned long long* counters;  // [0] box tests, [1] triangle tests (nullable)
+};
+hipError_t kgl_collide_f32(const TravArgs& a, hipStream_t s);
+hipError_t kgl_collide_f64(const TravArgs& a, hipStream_t s);
+
+struct FinArgs {
+  int N;
+  int mode;
+  const uint8_t* jl;
+  const unsigned long long* best;
+  const int* unc;
+  const PairItem* items;
+  PlanDev plan;
+  uint8_t* feasible;
+  uint8_t* env_hit;
+  uint8_t* self_hit;
+  int* first_pair;
+  int* redo_list;   // configs needing the FP64 pass
+  int* redo_count;
+  int final_pass;   // 1: uncertainty cannot remain
+};
+hipError_t kgl_finalize(const FinArgs& a, hipStream_t s);
+
+// ---- C-ABI object layouts
+
+struct kg_ctx_s;
+struct kg_geom_s;
+struct kg_robot_s;
+struct kg_world_s;
+
+struct Xf64 {
+  double R[9];
+  double t[3];
+};
+
+struct kg_geom_s {
+  kg_ctx_s* ctx;
+  int kind;        // 0 triangle mesh, 1 SDF
+  int index;       // GeomDesc index (mesh)
+  Xf64 X;          // current transform
+  // mesh data (kept for host-side queries)
+  std::vector<double> V;
+  std::vector<int32_t> T;
+  // SDF
+  int dims[3] = {0, 0, 0};
+  double bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
+  DevBuf dvals;
+  bool alive = true;
+};
+
+struct kg_robot_s {
+  kg_ctx_s* ctx;
+  int n;
+  std::vector<LinkDesc> links;
+  std::vector<kg_geom_s*> link_geoms;
+  std::vector<double> qmin, qmax, q;
+  std::vector<uint8_t> selfcol;  // n x n, upper triangular (i<j)
+  std::vector<Xf64> Tw;          // current link transforms
+  DevBuf dlinks;
+  bool dirty = true;
+  int version = 0;               // bumped on any change affecting plans
+};
+
+struct RobotPlan {
+  int built_version = -1;
+  int nenv = 0, nself = 0;
+  std::vector<EnvItem> env;
+  std::vector<uint8_t> lemask;
+  std::vector<int> selfpairs;
+  std::vector<std::pair<int, int>> pairs;  // world ID pairs for distance output
+  std::vector<int> pair_a, pair_b, pair_kind;  // link, env-or-link, 0 env / 1 self
+  DevBuf dlinks, dqmin, dqmax, denv, dlemask, dself;
+  PlanDev dev{};
+  Tol tol32{}, tol64{};
+  int maxdepth_a = 0, maxdepth_b = 0;
+};
+
+struct kg_world_s {
+  kg_ctx_s* ctx;
+  std::vector<kg_geom_s*> terrains;
+  std::vector<std::pair<kg_geom_s*, Xf64>> objects;
+  std::vector<kg_robot_s*> robots;
+  std::vector<uint8_t> mask;
+  int nids = 0;
+  int version = 0;
+  std::vector<RobotPlan> plans;
+};
+
+struct kg_ctx_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  // geometry pool, host mirror
+  std::vector<ObbNode> nodes;
+  std::vector<float4> tv0, tv1, tv2;
+  std::vector<double> tv64;
+  std::vector<int> trisrc;
+  std::vector<GeomDesc> geoms;
+  bool pool_dirty = true;
+  DevBuf dnodes, dtv0, dtv1, dtv2, dtv64, dgeoms, dtrisrc;
+  PoolDev pool{};
+  int pool_version = 0;
+  // workspace
+  DevBuf wsQ, wsT64, wsLB, wsJL, wsCounts, wsOffsets, wsItems, wsScanTmp, wsBest, wsUnc, wsChunk, wsSpill,
+      wsStatus, wsRedo, wsOut8a, wsOut8b, wsOut8c, wsOut32, wsF, wsG;
+  int num_cus = 256;
+};

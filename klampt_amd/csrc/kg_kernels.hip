@@ -1,0 +1,862 @@
+// HIP kernels for gfx950: forward kinematics, broad phase, BVH-vs-BVH
+// collision traversal, and result assembly.
+//
+// Collision traversal (k_collide) runs one wavefront per workgroup.  A wave
+// owns a chunk of configurations and streams their narrow-phase pair items
+// through 64 slots (one per lane).  All node-pair tasks of the wave's slots
+// share one LIFO task stack in LDS (spilling its bottom half to a per-wave
+// HBM region when full): each step every lane pops one task, tests the two
+// oriented boxes (15-axis separating-axis test) or, for two leaves, the two
+// triangles (17-axis test), and pushes up to two child tasks.  Lanes stay busy
+// as long as any slot has work, whatever the shape of the individual
+// traversals.  A certain hit kills the remaining tasks of its configuration
+// (any-hit), of its group (env / self), or of the items later in Klampt's loop
+// order (ordered first-hit), checked when a task is popped.
+//
+// Precision: the FP32 pass carries certified error bounds (Tol); a leaf test
+// that cannot be decided within them marks its configuration for the FP64
+// pass, which re-runs that configuration's items with FP64 transforms and
+// FP64 triangles.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include "kg_internal.h"
+
+#define KG_SCAP 1024
+#define KG_CPW_MAX 64
+#define KG_NONE64 0xffffffffffffffffull
+
+namespace {
+
+// ------------------------------------------------------------ FP64 transforms
+
+struct X64 {
+  double R[9];
+  double t[3];
+};
+
+__device__ inline void x64_load(const double* p, X64& X) {
+#pragma unroll
+  for (int i = 0; i < 9; i++) X.R[i] = p[i];
+#pragma unroll
+  for (int i = 0; i < 3; i++) X.t[i] = p[9 + i];
+}
+
+__device__ inline void x64_store(double* p, const X64& X) {
+#pragma unroll
+  for (int i = 0; i < 9; i++) p[i] = X.R[i];
+#pragma unroll
+  for (int i = 0; i < 3; i++) p[9 + i] = X.t[i];
+}
+
+__device__ inline void x64_mul(const X64& A, const X64& B, X64& C) {
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+      C.R[c * 3 + r] = A.R[r] * B.R[c * 3] + A.R[3 + r] * B.R[c * 3 + 1] + A.R[6 + r] * B.R[c * 3 + 2];
+#pragma unroll
+  for (int r = 0; r < 3; r++) C.t[r] = A.R[r] * B.t[0] + A.R[3 + r] * B.t[1] + A.R[6 + r] * B.t[2] + A.t[r];
+}
+
+// Joint transform: Rodrigues rotation about w (revolute) or translation w*q.
+__device__ inline void joint64(int type, const double* w, double q, X64& J) {
+  if (type == 1) {
+#pragma unroll
+    for (int i = 0; i < 9; i++) J.R[i] = (i % 4 == 0) ? 1.0 : 0.0;
+    J.t[0] = w[0] * q; J.t[1] = w[1] * q; J.t[2] = w[2] * q;
+    return;
+  }
+  double s, c;
+  sincos(q, &s, &c);
+  double v = 1.0 - c;
+#pragma unroll
+  for (int col = 0; col < 3; col++)
+#pragma unroll
+    for (int row = 0; row < 3; row++) J.R[col * 3 + row] = w[row] * w[col] * v;
+  J.R[0] += c; J.R[4] += c; J.R[8] += c;
+  J.R[1] += s * w[2]; J.R[2] -= s * w[1];
+  J.R[3] -= s * w[2]; J.R[5] += s * w[0];
+  J.R[6] += s * w[1]; J.R[7] -= s * w[0];
+  J.t[0] = J.t[1] = J.t[2] = 0.0;
+}
+
+// ------------------------------------------------------------ FK (FP32)
+
+// One lane per configuration; the link table sits in LDS.
+__global__ __launch_bounds__(256) void k_fk_f32(const LinkDesc* __restrict__ links, int n,
+                                                 const float* __restrict__ Q, int N, float* __restrict__ Tout) {
+  __shared__ float sT0[KG_MAX_LINKS][12];
+  __shared__ float sAx[KG_MAX_LINKS][3];
+  __shared__ int sPar[KG_MAX_LINKS], sTy[KG_MAX_LINKS];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    for (int k = 0; k < 12; k++) sT0[i][k] = (float)links[i].T0[k];
+    for (int k = 0; k < 3; k++) sAx[i][k] = (float)links[i].axis[k];
+    sPar[i] = links[i].parent;
+    sTy[i] = links[i].jtype;
+  }
+  __syncthreads();
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  float* out = Tout + (size_t)c * n * 12;
+  for (int l = 0; l < n; l++) {
+    float q = Q[(size_t)c * n + l];
+    float J[12];
+    const float* w = sAx[l];
+    if (sTy[l] == 1) {
+      for (int i = 0; i < 9; i++) J[i] = (i % 4 == 0) ? 1.f : 0.f;
+      J[9] = w[0] * q; J[10] = w[1] * q; J[11] = w[2] * q;
+    } else {
+      float s, co;
+      sincosf(q, &s, &co);
+      float v = 1.f - co;
+      for (int col = 0; col < 3; col++)
+        for (int row = 0; row < 3; row++) J[col * 3 + row] = w[row] * w[col] * v;
+      J[0] += co; J[4] += co; J[8] += co;
+      J[1] += s * w[2]; J[2] -= s * w[1];
+      J[3] -= s * w[2]; J[5] += s * w[0];
+      J[6] += s * w[1]; J[7] -= s * w[0];
+      J[9] = J[10] = J[11] = 0.f;
+    }
+    // L = T0 * J
+    float L[12];
+    const float* A = sT0[l];
+    for (int col = 0; col < 3; col++)
+      for (int r = 0; r < 3; r++)
+        L[col * 3 + r] = A[r] * J[col * 3] + A[3 + r] * J[col * 3 + 1] + A[6 + r] * J[col * 3 + 2];
+    for (int r = 0; r < 3; r++) L[9 + r] = A[r] * J[9] + A[3 + r] * J[10] + A[6 + r] * J[11] + A[9 + r];
+    float* o = out + l * 12;
+    int p = sPar[l];
+    if (p < 0) {
+      for (int i = 0; i < 12; i++) o[i] = L[i];
+    } else {
+      const float* P = out + p * 12;
+      float Pv[12];
+      for (int i = 0; i < 12; i++) Pv[i] = P[i];
+      for (int col = 0; col < 3; col++)
+        for (int r = 0; r < 3; r++)
+          o[col * 3 + r] = Pv[r] * L[col * 3] + Pv[3 + r] * L[col * 3 + 1] + Pv[6 + r] * L[col * 3 + 2];
+      for (int r = 0; r < 3; r++) o[9 + r] = Pv[r] * L[9] + Pv[3 + r] * L[10] + Pv[6 + r] * L[11] + Pv[9 + r];
+    }
+  }
+}
+
+// ------------------------------------------------------------ broad phase
+
+__device__ inline bool box_hit(const double* alo, const double* ahi, const double* blo, const double* bhi) {
+  return !(alo[0] > bhi[0] || ahi[0] < blo[0] || alo[1] > bhi[1] || ahi[1] < blo[1] || alo[2] > bhi[2] ||
+           ahi[2] < blo[2]);
+}
+
+// Position of survivor j after Klampt's set-level quick reject, which erases
+// a rejected entry by moving the current last entry into its place
+// (PlannerSettings.cpp:290-317).  Survivors below m (the survivor count) keep
+// their index; the k-th survivor counted from the end fills the k-th hole
+// below m.  `rej(i)` says whether entry i was rejected.
+template <class F>
+__device__ int swap_erase_pos(int j, int E, int m, F rej) {
+  if (j < m) return j;
+  int k = 0;
+  for (int i = j + 1; i < E; i++)
+    if (!rej(i)) k++;
+  int h = 0;
+  for (int i = 0; i < m; i++)
+    if (rej(i)) {
+      if (h == k) return i;
+      h++;
+    }
+  return -1;
+}
+
+template <bool FILL>
+__global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.N) return;
+  const PlanDev& P = a.plan;
+  const int n = P.nlinks;
+  double* T = a.T64 + (size_t)c * n * 12;
+  double* LB = a.LB + (size_t)c * n * 6;
+  if (!FILL) {
+    bool jl = true;
+    for (int l = 0; l < n; l++) {
+      double q = a.Qf ? (double)a.Qf[(size_t)c * n + l] : a.Qd[(size_t)c * n + l];
+      if (q < P.qmin[l] || q > P.qmax[l]) jl = false;
+      const LinkDesc& L = P.links[l];
+      X64 J, T0, Lx, X;
+      joint64(L.jtype, L.axis, q, J);
+      x64_load(L.T0, T0);
+      x64_mul(T0, J, Lx);
+      if (L.parent >= 0) {
+        X64 Pp;
+        x64_load(T + 12 * L.parent, Pp);
+        x64_mul(Pp, Lx, X);
+      } else {
+        X = Lx;
+      }
+      x64_store(T + 12 * l, X);
+      if (L.geom >= 0) {
+        const GeomDesc& g = a.pool.geoms[L.geom];
+        for (int r = 0; r < 3; r++) {
+          double wc = X.R[r] * g.lc[0] + X.R[3 + r] * g.lc[1] + X.R[6 + r] * g.lc[2] + X.t[r];
+          double ext = fabs(X.R[r]) * g.le[0] + fabs(X.R[3 + r]) * g.le[1] + fabs(X.R[6 + r]) * g.le[2];
+          LB[6 * l + r] = wc - ext;
+          LB[6 * l + 3 + r] = wc + ext;
+        }
+      }
+    }
+    a.jl[c] = jl ? 1 : 0;
+    if (a.mode == KG_MODE_ANY && !jl) {
+      a.counts[c] = 0;
+      return;
+    }
+  } else {
+    if (a.mode == KG_MODE_ANY && !a.jl[c]) return;
+  }
+  const bool ordered = a.mode == KG_MODE_ORDERED;
+  // union of the active links' boxes
+  double ulo[3] = {1e300, 1e300, 1e300}, uhi[3] = {-1e300, -1e300, -1e300};
+  uint64_t active = 0;
+  for (int l = 0; l < n; l++)
+    if (P.links[l].geom >= 0) {
+      active |= 1ull << l;
+      for (int r = 0; r < 3; r++) {
+        ulo[r] = fmin(ulo[r], LB[6 * l + r]);
+        uhi[r] = fmax(uhi[r], LB[6 * l + 3 + r]);
+      }
+    }
+  // env entries surviving against the link union; union of survivors
+  double elo[3] = {1e300, 1e300, 1e300}, ehi[3] = {-1e300, -1e300, -1e300};
+  int m2 = 0;
+  for (int e = 0; e < P.nenv; e++) {
+    const EnvItem& E = P.env[e];
+    if (box_hit(E.lo, E.hi, ulo, uhi)) {
+      m2++;
+      for (int r = 0; r < 3; r++) {
+        elo[r] = fmin(elo[r], E.lo[r]);
+        ehi[r] = fmax(ehi[r], E.hi[r]);
+      }
+    }
+  }
+  // links surviving against the env survivor union (positions in the active list)
+  int nact = __popcll(active);
+  uint64_t lrej = 0;  // bit = active-list position rejected
+  {
+    int pos = 0;
+    for (int l = 0; l < n; l++)
+      if (active >> l & 1) {
+        if (!box_hit(LB + 6 * l, LB + 6 * l + 3, elo, ehi)) lrej |= 1ull << pos;
+        pos++;
+      }
+  }
+  int m1 = nact - __popcll(lrej);
+  int cnt = 0;
+  int base = FILL ? a.offsets[c] : 0;
+  auto env_rej = [&](int i) { return !box_hit(P.env[i].lo, P.env[i].hi, ulo, uhi); };
+  auto link_rej = [&](int i) { return (lrej >> i & 1) != 0; };
+  // env pairs
+  if (m2 > 0) {
+    int pos = -1;
+    for (int l = 0; l < n; l++) {
+      if (!(active >> l & 1)) continue;
+      pos++;
+      if (lrej >> pos & 1) continue;
+      int p1 = ordered ? swap_erase_pos(pos, nact, m1, link_rej) : 0;
+      for (int e = 0; e < P.nenv; e++) {
+        const EnvItem& E = P.env[e];
+        if (!P.lemask[l * P.nenv + e]) continue;
+        if (!box_hit(E.lo, E.hi, ulo, uhi)) continue;  // rejected env entry
+        if (!box_hit(LB + 6 * l, LB + 6 * l + 3, E.lo, E.hi)) continue;
+        if (FILL) {
+          int p2 = ordered ? swap_erase_pos(e, P.nenv, m2, env_rej) : 0;
+          PairItem it;
+          it.cfg = c;
+          it.a = (int16_t)l;
+          it.b = (int16_t)e;
+          it.rank = ordered ? (p1 * 65536 + p2) : 0;
+          a.items[base + cnt] = it;
+        }
+        cnt++;
+      }
+    }
+  }
+  // self pairs
+  for (int k = 0; k < P.nself; k++) {
+    int i = P.selfpairs[2 * k], j = P.selfpairs[2 * k + 1];
+    if (!box_hit(LB + 6 * i, LB + 6 * i + 3, LB + 6 * j, LB + 6 * j + 3)) continue;
+    if (FILL) {
+      PairItem it;
+      it.cfg = c;
+      it.a = (int16_t)i;
+      it.b = (int16_t)j;
+      it.rank = (1 << KG_RANK_GROUP_BIT) | (ordered ? k : 0);
+      a.items[base + cnt] = it;
+    }
+    cnt++;
+  }
+  if (!FILL) a.counts[c] = cnt;
+}
+
+// ------------------------------------------------------------ narrow phase
+
+template <typename Real>
+struct V3T {
+  Real x, y, z;
+};
+template <typename Real>
+__device__ inline V3T<Real> v3(Real x, Real y, Real z) {
+  return V3T<Real>{x, y, z};
+}
+template <typename Real>
+__device__ inline V3T<Real> vsub(V3T<Real> a, V3T<Real> b) {
+  return v3<Real>(a.x - b.x, a.y - b.y, a.z - b.z);
+}
+template <typename Real>
+__device__ inline Real vdot(V3T<Real> a, V3T<Real> b) {
+  return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+template <typename Real>
+__device__ inline V3T<Real> vcross(V3T<Real> a, V3T<Real> b) {
+  return v3<Real>(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+template <typename Real>
+__device__ inline Real vl1(V3T<Real> a) {
+  return fabs(a.x) + fabs(a.y) + fabs(a.z);
+}
+
+enum { TT_SEP = 0, TT_HIT = 1, TT_UNSURE = 2 };
+
+// One separating-axis candidate.  Returns true if the axis certainly
+// separates; clears `sure` when the axis does not certainly overlap.
+template <typename Real, bool EXACT>
+__device__ inline bool sat_axis(V3T<Real> a, Real aerr, const V3T<Real>* P, const V3T<Real>* Q, Real tau, Real RD,
+                                bool& sure) {
+  Real p0 = vdot(a, P[0]), p1 = vdot(a, P[1]), p2 = vdot(a, P[2]);
+  Real q0 = vdot(a, Q[0]), q1 = vdot(a, Q[1]), q2 = vdot(a, Q[2]);
+  Real pmin = fmin(p0, fmin(p1, p2)), pmax = fmax(p0, fmax(p1, p2));
+  Real qmin = fmin(q0, fmin(q1, q2)), qmax = fmax(q0, fmax(q1, q2));
+  Real gap = fmax(qmin - pmax, pmin - qmax);
+  if (EXACT) return gap > (Real)0;
+  Real aa = vdot(a, a);
+  if (gap > (Real)0 && gap * gap > tau * tau * aa) return true;
+  Real al = sqrt(aa);
+  if (!(-gap > tau * al + aerr * RD)) sure = false;
+  return false;
+}
+
+// Closed triangle/triangle test by 17 separating axes (2 normals, 9 edge
+// crosses, 6 in-plane edge normals), in coordinates shifted to q[0].
+// EXACT (FP64): TT_SEP / TT_HIT.  Otherwise TT_UNSURE when the decision is
+// within the error bound: tau bounds the vertex position error (x2, both
+// triangles) plus projection rounding; aerr bounds each axis' own rounding.
+template <typename Real, bool EXACT>
+__device__ int tri_tri_sat(const V3T<Real>* p, const V3T<Real>* q, Real tau) {
+  V3T<Real> P[3], Q[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    P[i] = vsub(p[i], q[0]);
+    Q[i] = vsub(q[i], q[0]);
+  }
+  V3T<Real> eP[3] = {vsub(P[1], P[0]), vsub(P[2], P[1]), vsub(P[0], P[2])};
+  V3T<Real> eQ[3] = {vsub(Q[1], Q[0]), vsub(Q[2], Q[1]), vsub(Q[0], Q[2])};
+  const Real u = EXACT ? (Real)0 : (Real)5.9604645e-08;
+  Real RD = 0;
+  if (!EXACT) {
+#pragma unroll
+    for (int i = 0; i < 3; i++) RD = fmax(RD, fmax(vl1(P[i]), vl1(Q[i])));
+    RD *= (Real)2;
+    tau += (Real)8 * u * RD;
+  }
+  bool sure = true;
+  V3T<Real> nP = vcross(eP[0], eP[1]);
+  V3T<Real> nQ = vcross(eQ[0], eQ[1]);
+  Real lP0 = EXACT ? 0 : vl1(eP[0]), lP1 = EXACT ? 0 : vl1(eP[1]), lP2 = EXACT ? 0 : vl1(eP[2]);
+  Real lQ0 = EXACT ? 0 : vl1(eQ[0]), lQ1 = EXACT ? 0 : vl1(eQ[1]), lQ2 = EXACT ? 0 : vl1(eQ[2]);
+  Real lP[3] = {lP0, lP1, lP2}, lQ[3] = {lQ0, lQ1, lQ2};
+  Real enP = (Real)8 * u * lP0 * lP1, enQ = (Real)8 * u * lQ0 * lQ1;
+  if (sat_axis<Real, EXACT>(nP, enP, P, Q, tau, RD, sure)) return TT_SEP;
+  if (sat_axis<Real, EXACT>(nQ, enQ, P, Q, tau, RD, sure)) return TT_SEP;
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+      if (sat_axis<Real, EXACT>(vcross(eP[i], eQ[j]), (Real)8 * u * lP[i] * lQ[j], P, Q, tau, RD, sure))
+        return TT_SEP;
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    if (sat_axis<Real, EXACT>(vcross(nP, eP[i]), (Real)16 * u * lP0 * lP1 * lP[i], P, Q, tau, RD, sure))
+      return TT_SEP;
+    if (sat_axis<Real, EXACT>(vcross(nQ, eQ[i]), (Real)16 * u * lQ0 * lQ1 * lQ[i], P, Q, tau, RD, sure))
+      return TT_SEP;
+  }
+  return (EXACT || sure) ? TT_HIT : TT_UNSURE;
+}
+
+template <typename Real>
+struct NodeT {
+  Real c[3], e[3], R[9];
+  int child;
+};
+
+__device__ inline void load_node(const ObbNode* __restrict__ nodes, int i, float* c, float* e, float* R, int& child) {
+  const float4* p = reinterpret_cast<const float4*>(nodes + i);
+  float4 a = p[0], b = p[1], d = p[2], f = p[3];
+  c[0] = a.x; c[1] = a.y; c[2] = a.z; e[0] = a.w;
+  e[1] = b.x; e[2] = b.y; R[0] = b.z; R[1] = b.w;
+  R[2] = d.x; R[3] = d.y; R[4] = d.z; R[5] = d.w;
+  R[6] = f.x; R[7] = f.y; R[8] = f.z;
+  child = __float_as_int(f.w);
+}
+
+// OBB overlap: box A (frame A) mapped by rel (A-local -> B-local) against box
+// B (frame B).  15 axes, every right-hand side padded by `pad`.
+template <typename Real>
+__device__ inline bool obb_overlap(const float* cAf, const float* eAf, const float* RAf, const float* cBf,
+                                   const float* eBf, const float* RBf, const Real* rel, Real pad_abs, Real pad_rel) {
+  Real RA[9], RB[9], cA[3], cB[3], eA[3], eB[3];
+#pragma unroll
+  for (int i = 0; i < 9; i++) { RA[i] = (Real)RAf[i]; RB[i] = (Real)RBf[i]; }
+#pragma unroll
+  for (int i = 0; i < 3; i++) { cA[i] = cAf[i]; cB[i] = cBf[i]; eA[i] = eAf[i]; eB[i] = eBf[i]; }
+  // A's axes and centre in frame B
+  Real Ap[9], cp[3];
+#pragma unroll
+  for (int col = 0; col < 3; col++)
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+      Ap[col * 3 + r] = rel[r] * RA[col * 3] + rel[3 + r] * RA[col * 3 + 1] + rel[6 + r] * RA[col * 3 + 2];
+#pragma unroll
+  for (int r = 0; r < 3; r++) cp[r] = rel[r] * cA[0] + rel[3 + r] * cA[1] + rel[6 + r] * cA[2] + rel[9 + r] - cB[r];
+  // in box B's axes: C[i][j] = B_i . A'_j, T_i = B_i . (cA' - cB)
+  Real C[3][3], Qa[3][3], T[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    T[i] = RB[i * 3] * cp[0] + RB[i * 3 + 1] * cp[1] + RB[i * 3 + 2] * cp[2];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      C[i][j] = RB[i * 3] * Ap[j * 3] + RB[i * 3 + 1] * Ap[j * 3 + 1] + RB[i * 3 + 2] * Ap[j * 3 + 2];
+      Qa[i][j] = fabs(C[i][j]);
+    }
+  }
+  Real pad = pad_abs + pad_rel * (eA[0] + eA[1] + eA[2] + eB[0] + eB[1] + eB[2]);
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+    if (fabs(T[i]) > eB[i] + eA[0] * Qa[i][0] + eA[1] * Qa[i][1] + eA[2] * Qa[i][2] + pad) return false;
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    Real t = T[0] * C[0][j] + T[1] * C[1][j] + T[2] * C[2][j];
+    if (fabs(t) > eB[0] * Qa[0][j] + eB[1] * Qa[1][j] + eB[2] * Qa[2][j] + eA[j] + pad) return false;
+  }
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    const int i1 = (i + 1) % 3, i2 = (i + 2) % 3;
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      const int j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+      Real t = T[i2] * C[i1][j] - T[i1] * C[i2][j];
+      Real ra = eB[i1] * Qa[i2][j] + eB[i2] * Qa[i1][j];
+      Real rb = eA[j1] * Qa[i][j2] + eA[j2] * Qa[i][j1];
+      if (fabs(t) > ra + rb + pad) return false;
+    }
+  }
+  return true;
+}
+
+template <typename Real>
+__device__ inline void load_tri(const PoolDev& pool, int t, V3T<Real>* v);
+
+template <>
+__device__ inline void load_tri<float>(const PoolDev& pool, int t, V3T<float>* v) {
+  float4 a = pool.tv0[t], b = pool.tv1[t], c = pool.tv2[t];
+  v[0] = v3<float>(a.x, a.y, a.z);
+  v[1] = v3<float>(b.x, b.y, b.z);
+  v[2] = v3<float>(c.x, c.y, c.z);
+}
+
+template <>
+__device__ inline void load_tri<double>(const PoolDev& pool, int t, V3T<double>* v) {
+  const double* p = pool.tv64 + 9 * (size_t)t;
+#pragma unroll
+  for (int k = 0; k < 3; k++) v[k] = v3<double>(p[3 * k], p[3 * k + 1], p[3 * k + 2]);
+}
+
+template <typename Real>
+__device__ inline V3T<Real> xf_apply(const Real* X, V3T<Real> p) {
+  return v3<Real>(X[0] * p.x + X[3] * p.y + X[6] * p.z + X[9], X[1] * p.x + X[4] * p.y + X[7] * p.z + X[10],
+                  X[2] * p.x + X[5] * p.y + X[8] * p.z + X[11]);
+}
+
+__device__ inline uint64_t ballot64(bool p) { return __ballot(p); }
+__device__ inline int lane_prefix(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+}
+
+// Rank/kill helpers.  rank word: group in bit 30, order below.
+__device__ inline bool item_dead(int mode, const unsigned long long* best, int g, int r) {
+  if (mode == KG_MODE_ANY) return best[0] != KG_NONE64 || best[1] != KG_NONE64;
+  unsigned long long b = best[g];
+  if (b == KG_NONE64) return false;
+  if (mode == KG_MODE_GROUPS) return true;
+  return (int)(b >> 32) < r;
+}
+
+template <typename Real, bool EXACT>
+__global__ __launch_bounds__(64) void k_collide(TravArgs a) {
+  __shared__ uint64_t stk[KG_SCAP];
+  __shared__ Real s_rel[12][64];
+  __shared__ int s_rootA[64], s_rootB[64], s_cfgk[64], s_rank[64], s_item[64], s_pend[64], s_unc[64];
+  __shared__ unsigned long long s_best[KG_CPW_MAX][2];
+  __shared__ int s_uncmin[KG_CPW_MAX][2];
+  __shared__ int s_chunk;
+
+  const int lane = threadIdx.x;
+  const int mode = a.mode;
+  const Real tau = EXACT ? (Real)0 : (Real)a.tol.tau;
+  const Real pad_abs = (Real)a.tol.pad_bv, pad_rel = (Real)a.tol.pad_rel;
+  uint64_t* gsp_base = a.spill + (size_t)blockIdx.x * a.spill_cap;
+  const int ncfg = a.ncfg;
+  unsigned n_box = 0, n_tri = 0;
+
+  for (;;) {
+    if (lane == 0) s_chunk = atomicAdd(a.chunk_counter, 1);
+    __syncthreads();
+    const int chunk = s_chunk;
+    const int k0 = chunk * a.cpw;
+    if (k0 >= ncfg) break;
+    const int nk = min(a.cpw, ncfg - k0);
+    for (int i = lane; i < KG_CPW_MAX; i += 64) {
+      s_best[i][0] = s_best[i][1] = KG_NONE64;
+      s_uncmin[i][0] = s_uncmin[i][1] = KG_RANK_NONE;
+    }
+    s_pend[lane] = 0;
+    s_unc[lane] = 0;
+    __syncthreads();
+
+    auto cfg_of = [&](int k) { return a.cfg_list ? a.cfg_list[k0 + k] : k0 + k; };
+    int kc = 0;
+    int it = a.offsets[cfg_of(0)], itend = a.offsets[cfg_of(0) + 1];
+    int sp = 0, gsp = 0;
+
+    // Make room for `room` more stack entries: move the bottom half of the
+    // LDS stack to this wave's HBM spill region (LIFO order is preserved:
+    // the HBM region holds the oldest entries).  If that region is full, the
+    // bottom half is dropped and its items marked uncertain.  Wave-uniform.
+    auto ensure_room = [&](int room) {
+      if (sp + room <= KG_SCAP) return;
+      const int h = KG_SCAP / 2;
+      if (gsp + h <= a.spill_cap) {
+        for (int i = lane; i < h; i += 64) gsp_base[gsp + i] = stk[i];
+        gsp += h;
+      } else {
+        for (int i = lane; i < h; i += 64) {
+          uint64_t t = stk[i];
+          int sl = (int)(t >> 58);
+          s_unc[sl] = 1;
+          if (atomicAdd(&s_pend[sl], -1) == 1) {
+            const int rk2 = s_rank[sl];
+            atomicMin(&s_uncmin[s_cfgk[sl]][(rk2 >> KG_RANK_GROUP_BIT) & 1], rk2 & ((1 << KG_RANK_GROUP_BIT) - 1));
+          }
+        }
+        if (lane == 0) atomicExch(a.status, 1);
+      }
+      __syncthreads();
+      for (int i = lane; i < sp - h; i += 64) stk[i] = stk[i + h];
+      sp -= h;
+      __syncthreads();
+    };
+
+    for (;;) {
+      // ---- refill free slots with the next items of the chunk
+      const bool isfree = s_pend[lane] == 0;
+      const uint64_t fm = ballot64(isfree);
+      const int need = __popcll(fm);
+      const int myp = lane_prefix(fm);
+      int myitem = -1, mycfgk = -1;
+      int assigned = 0;
+      while (assigned < need && kc < nk) {
+        if (it >= itend) {
+          kc++;
+          if (kc < nk) {
+            int cc = cfg_of(kc);
+            it = a.offsets[cc];
+            itend = a.offsets[cc + 1];
+          }
+          continue;
+        }
+        int take = min(need - assigned, itend - it);
+        if (isfree && myp >= assigned && myp < assigned + take) {
+          myitem = it + (myp - assigned);
+          mycfgk = kc;
+        }
+        assigned += take;
+        it += take;
+      }
+      const bool got = myitem >= 0;
+      ensure_room(64);
+      if (got) {
+        const PairItem P = a.items[myitem];
+        const LinkDesc& LA = a.plan.links[P.a];
+        const int g = (P.rank >> KG_RANK_GROUP_BIT) & 1;
+        const double* XA = a.T64 + ((size_t)P.cfg * a.plan.nlinks + P.a) * 12;
+        const double* XB;
+        int geomB;
+        if (g == 0) {
+          XB = a.plan.env[P.b].X;
+          geomB = a.plan.env[P.b].geom;
+        } else {
+          XB = a.T64 + ((size_t)P.cfg * a.plan.nlinks + P.b) * 12;
+          geomB = a.plan.links[P.b].geom;
+        }
+        // rel = XB^-1 XA : A-local -> B-local
+        double d0 = XA[9] - XB[9], d1 = XA[10] - XB[10], d2 = XA[11] - XB[11];
+#pragma unroll
+        for (int col = 0; col < 3; col++)
+#pragma unroll
+          for (int r = 0; r < 3; r++) {
+            double v = XB[r * 3] * XA[col * 3] + XB[r * 3 + 1] * XA[col * 3 + 1] + XB[r * 3 + 2] * XA[col * 3 + 2];
+            s_rel[col * 3 + r][lane] = (Real)v;
+          }
+#pragma unroll
+        for (int r = 0; r < 3; r++) s_rel[9 + r][lane] = (Real)(XB[r * 3] * d0 + XB[r * 3 + 1] * d1 + XB[r * 3 + 2] * d2);
+        s_rootA[lane] = a.pool.geoms[LA.geom].root;
+        s_rootB[lane] = a.pool.geoms[geomB].root;
+        s_cfgk[lane] = mycfgk;
+        s_rank[lane] = P.rank;
+        s_item[lane] = myitem;
+        s_pend[lane] = 1;
+        s_unc[lane] = 0;
+      }
+      {
+        const uint64_t gm = ballot64(got);
+        const int tot = __popcll(gm);
+        if (got) {
+          uint64_t task = ((uint64_t)lane << 58) | ((uint64_t)(uint32_t)s_rootA[lane] << 32) | (uint32_t)s_rootB[lane];
+          stk[sp + lane_prefix(gm)] = task;
+        }
+        sp += tot;
+      }
+      __syncthreads();
+
+      // ---- bring spilled tasks back when the LDS stack runs low
+      if (sp < 64 && gsp > 0) {
+        const int k = min(gsp, KG_SCAP / 2);
+        for (int i = sp - 1 - lane; i >= 0; i -= 64) stk[i + k] = stk[i];
+        __syncthreads();
+        for (int i = lane; i < k; i += 64) stk[i] = gsp_base[gsp - k + i];
+        gsp -= k;
+        sp += k;
+        __syncthreads();
+      }
+
+      // ---- termination: every outstanding task of a busy slot is on the
+      // stack (LDS or spilled), and the refill above gave every free slot an
+      // item while items remained, so an empty stack means the chunk is done
+      if (sp == 0) break;
+
+      // ---- pop
+      const int npop = min(sp, 64);
+      const bool has = lane < npop;
+      uint64_t task = has ? stk[sp - 1 - lane] : 0;
+      sp -= npop;
+
+      // ---- process
+      int nchild = 0;
+      uint64_t ch0 = 0, ch1 = 0;
+      if (has) {
+        const int slot = (int)(task >> 58);
+        const int na = (int)((task >> 32) & 0x3ffffff);
+        const int nb = (int)(task & 0xffffffffu);
+        const int rk = s_rank[slot];
+        const int g = (rk >> KG_RANK_GROUP_BIT) & 1;
+        const int r = rk & ((1 << KG_RANK_GROUP_BIT) - 1);
+        const int ck = s_cfgk[slot];
+        if (!item_dead(mode, s_best[ck], g, r)) {
+          float cA[3], eA[3], RA[9], cB[3], eB[3], RB[9];
+          int chA, chB;
+          load_node(a.pool.nodes, na, cA, eA, RA, chA);
+          load_node(a.pool.nodes, nb, cB, eB, RB, chB);
+          Real rel[12];
+#pragma unroll
+          for (int k = 0; k < 12; k++) rel[k] = s_rel[k][slot];
+          n_box++;
+          if (obb_overlap<Real>(cA, eA, RA, cB, eB, RB, rel, pad_abs, pad_rel)) {
+            const bool la = chA < 0, lb = chB < 0;
+            if (la && lb) {
+              n_tri++;
+              V3T<Real> tp[3], tq[3];
+              load_tri<Real>(a.pool, ~chA, tp);
+              load_tri<Real>(a.pool, ~chB, tq);
+#pragma unroll
+              for (int v = 0; v < 3; v++) tp[v] = xf_apply<Real>(rel, tp[v]);
+              int res = tri_tri_sat<Real, EXACT>(tp, tq, tau);
+              if (res == TT_HIT) {
+                unsigned long long key = ((unsigned long long)(uint32_t)r << 32) | (uint32_t)s_item[slot];
+                atomicMin(&s_best[ck][g], key);
+              } else if (res == TT_UNSURE) {
+                s_unc[slot] = 1;
+              }
+            } else {
+              bool splitA;
+              if (la) splitA = false;
+              else if (lb) splitA = true;
+              else splitA = (eA[0] + eA[1] + eA[2]) >= (eB[0] + eB[1] + eB[2]);
+              const uint64_t hi = (uint64_t)slot << 58;
+              if (splitA) {
+                ch0 = hi | ((uint64_t)(uint32_t)chA << 32) | (uint32_t)nb;
+                ch1 = hi | ((uint64_t)(uint32_t)(chA + 1) << 32) | (uint32_t)nb;
+              } else {
+                ch0 = hi | ((uint64_t)(uint32_t)na << 32) | (uint32_t)chB;
+                ch1 = hi | ((uint64_t)(uint32_t)na << 32) | (uint32_t)(chB + 1);
+              }
+              nchild = 2;
+            }
+          }
+        }
+        const int old = atomicAdd(&s_pend[slot], nchild - 1);
+        if (old + nchild - 1 == 0) {
+          // item complete: an undecided leaf test makes it uncertain unless a
+          // certain hit already settles its place
+          if (s_unc[slot] && !item_dead(mode, s_best[ck], g, r)) atomicMin(&s_uncmin[ck][g], r);
+        }
+      }
+
+      // ---- push (spill the bottom half to HBM first if needed)
+      const uint64_t b1 = ballot64(nchild >= 1), b2 = ballot64(nchild >= 2);
+      const int total = __popcll(b1) + __popcll(b2);
+      ensure_room(total);
+      {
+        const int pos = sp + lane_prefix(b1) + lane_prefix(b2);
+        if (nchild >= 1) stk[pos] = ch0;
+        if (nchild >= 2) stk[pos + 1] = ch1;
+        sp += total;
+      }
+      __syncthreads();
+    }
+    // chunk results
+    for (int k = lane; k < nk; k += 64) {
+      int cc = cfg_of(k);
+      a.best[2 * cc] = s_best[k][0];
+      a.best[2 * cc + 1] = s_best[k][1];
+      a.unc[2 * cc] = s_uncmin[k][0];
+      a.unc[2 * cc + 1] = s_uncmin[k][1];
+    }
+    __syncthreads();
+  }
+  if (a.counters) {
+    atomicAdd(&a.counters[0], (unsigned long long)n_box);
+    atomicAdd(&a.counters[1], (unsigned long long)n_tri);
+  }
+}
+
+__global__ void k_fill_int(int* p, int n, int v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ __launch_bounds__(256) void k_finalize(FinArgs a, const int* list, const int* list_count) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  int n = list ? *list_count : a.N;
+  if (k >= n) return;
+  int c = list ? list[k] : k;
+  const unsigned long long b0 = a.best[2 * c], b1 = a.best[2 * c + 1];
+  const int u0 = a.unc[2 * c], u1 = a.unc[2 * c + 1];
+  const bool jl = a.jl[c] != 0;
+  const bool h0 = b0 != KG_NONE64, h1 = b1 != KG_NONE64;
+  bool sure;
+  if (a.mode == KG_MODE_ANY) {
+    sure = !jl || h0 || h1 || (u0 == KG_RANK_NONE && u1 == KG_RANK_NONE);
+  } else if (a.mode == KG_MODE_GROUPS) {
+    sure = (h0 || u0 == KG_RANK_NONE) && (h1 || u1 == KG_RANK_NONE);
+  } else {
+    const int r0 = h0 ? (int)(b0 >> 32) : KG_RANK_NONE;
+    const int r1 = h1 ? (int)(b1 >> 32) : KG_RANK_NONE;
+    sure = (u0 > r0 || u0 == KG_RANK_NONE) && (u1 > r1 || u1 == KG_RANK_NONE);
+  }
+  if (!sure && !a.final_pass) {
+    int slot = atomicAdd(a.redo_count, 1);
+    a.redo_list[slot] = c;
+    return;
+  }
+  if (a.feasible) a.feasible[c] = (jl && !h0 && !h1) ? 1 : 0;
+  if (a.env_hit) a.env_hit[c] = h0 ? 1 : 0;
+  if (a.self_hit) a.self_hit[c] = h1 ? 1 : 0;
+  if (a.first_pair) {
+    int id1 = -1, id2 = -1;
+    if (h0 || h1) {
+      const PairItem it = a.items[(uint32_t)((h0 ? b0 : b1) & 0xffffffffu)];
+      id1 = a.plan.links[it.a].wid;
+      id2 = h0 ? a.plan.env[it.b].wid : a.plan.links[it.b].wid;
+    }
+    a.first_pair[2 * c] = id1;
+    a.first_pair[2 * c + 1] = id2;
+  }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ launchers
+
+hipError_t kgl_fk_f32(const LinkDesc* links, int nlinks, const float* Q, int N, float* Tout, hipStream_t s) {
+  if (N <= 0) return hipSuccess;
+  k_fk_f32<<<(N + 255) / 256, 256, 0, s>>>(links, nlinks, Q, N, Tout);
+  return hipGetLastError();
+}
+
+hipError_t kgl_prep_count(const PrepArgs& a, hipStream_t s) {
+  if (a.N <= 0) return hipSuccess;
+  k_prep<false><<<(a.N + 255) / 256, 256, 0, s>>>(a);
+  k_fill_int<<<1, 1, 0, s>>>(a.counts + a.N, 1, 0);
+  return hipGetLastError();
+}
+
+hipError_t kgl_scan(int* counts, int* offsets, int n, DevBuf& tmp, hipStream_t s) {
+  size_t bytes = 0;
+  hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, counts, offsets, n, s);
+  if (e != hipSuccess) return e;
+  e = tmp.ensure(bytes);
+  if (e != hipSuccess) return e;
+  return hipcub::DeviceScan::ExclusiveSum(tmp.p, bytes, counts, offsets, n, s);
+}
+
+hipError_t kgl_prep_fill(const PrepArgs& a, hipStream_t s) {
+  if (a.N <= 0) return hipSuccess;
+  k_prep<true><<<(a.N + 255) / 256, 256, 0, s>>>(a);
+  return hipGetLastError();
+}
+
+template <typename Real, bool EXACT>
+static hipError_t launch_collide(const TravArgs& a, hipStream_t s) {
+  k_fill_int<<<1, 1, 0, s>>>(a.chunk_counter, 1, 0);
+  k_collide<Real, EXACT><<<a.grid, 64, 0, s>>>(a);
+  return hipGetLastError();
+}
+
+hipError_t kgl_collide_f32(const TravArgs& a, hipStream_t s) { return launch_collide<float, false>(a, s); }
+hipError_t kgl_collide_f64(const TravArgs& a, hipStream_t s) { return launch_collide<double, true>(a, s); }
+
+hipError_t kgl_finalize(const FinArgs& a, hipStream_t s) {
+  if (a.N <= 0) return hipSuccess;
+  k_finalize<<<(a.N + 255) / 256, 256, 0, s>>>(a, nullptr, nullptr);
+  return hipGetLastError();
+}
+
+hipError_t kgl_finalize_list(const FinArgs& a, const int* list, const int* count, int maxn, hipStream_t s) {
+  if (maxn <= 0) return hipSuccess;
+  k_finalize<<<(maxn + 255) / 256, 256, 0, s>>>(a, list, count);
+  return hipGetLastError();
+}
+
+hipError_t kgl_fill_int(int* p, int n, int v, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  k_fill_int<<<(n + 255) / 256, 256, 0, s>>>(p, n, v);
+  return hipGetLastError();
+}
+
+int kgl_collide_grid(int exact, int num_cus) {
+  int nb = 0;
+  hipError_t e = exact ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_collide<double, true>, 64, 0)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_collide<float, false>, 64, 0);
+  if (e != hipSuccess || nb < 1) nb = 8;
+  const char* v = getenv("KG_WAVES_PER_CU");
+  if (v && atoi(v) > 0) nb = atoi(v);
+  return nb * num_cus;
+}

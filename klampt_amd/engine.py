@@ -1,0 +1,323 @@
+"""Object wrappers over the C ABI: contexts, geometries, robots, worlds.
+
+Host (numpy) batch calls stage through the library's own device buffers;
+``*_dev`` methods take torch tensors already resident on the GPU and run on
+the current torch stream.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import call, lib, DistSettings, DistResult
+
+_pd = ctypes.POINTER(ctypes.c_double)
+_pf = ctypes.POINTER(ctypes.c_float)
+_pi = ctypes.POINTER(ctypes.c_int32)
+_pu8 = ctypes.POINTER(ctypes.c_uint8)
+
+
+def _d(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _f(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _i(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _p(a, t):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def _torch_stream():
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _tp(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+class Context:
+    """One GPU device and one HIP stream (``kg_ctx``)."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        call("kg_ctx_create", device, ctypes.byref(h))
+        self.h = h
+        self.device = device
+
+    def synchronize(self):
+        call("kg_ctx_synchronize", self.h)
+
+    def close(self):
+        if self.h:
+            lib().kg_ctx_destroy(self.h)
+            self.h = None
+
+
+class Geom:
+    """A collision geometry: triangle mesh (with its OBB tree) or SDF grid."""
+
+    def __init__(self, ctx, h, kind):
+        self.ctx, self.h, self.kind = ctx, h, kind
+
+    @classmethod
+    def trimesh(cls, ctx, verts, tris):
+        V, T = _d(verts).reshape(-1, 3), _i(tris).reshape(-1, 3)
+        h = ctypes.c_void_p()
+        call("kg_geom_create_trimesh", ctx.h, _p(V, _pd), V.shape[0], _p(T, _pi), T.shape[0], ctypes.byref(h))
+        return cls(ctx, h, "TriangleMesh")
+
+    @classmethod
+    def sdf(cls, ctx, bmin, bmax, values):
+        vals = _f(values)
+        dims = _i(vals.shape)
+        h = ctypes.c_void_p()
+        call("kg_geom_create_sdf", ctx.h, _p(_d(bmin), _pd), _p(_d(bmax), _pd), _p(dims, _pi), _p(vals, _pf),
+             ctypes.byref(h))
+        g = cls(ctx, h, "ImplicitSurface")
+        g.dims = tuple(int(x) for x in vals.shape)
+        return g
+
+    def set_transform(self, R, t):
+        call("kg_geom_set_transform", self.h, _p(_d(R), _pd), _p(_d(t), _pd))
+
+    def get_transform(self):
+        R, t = np.zeros(9), np.zeros(3)
+        call("kg_geom_get_transform", self.h, _p(R, _pd), _p(t, _pd))
+        return R, t
+
+    def get_bb(self):
+        lo, hi = np.zeros(3), np.zeros(3)
+        call("kg_geom_get_bb", self.h, _p(lo, _pd), _p(hi, _pd))
+        return lo, hi
+
+    def collides(self, other):
+        out = ctypes.c_int32()
+        call("kg_geom_collide", self.h, other.h, ctypes.byref(out))
+        return bool(out.value)
+
+    def within_distance(self, other, tol):
+        out = ctypes.c_int32()
+        call("kg_geom_within_distance", self.h, other.h, float(tol), ctypes.byref(out))
+        return bool(out.value)
+
+    def distance(self, other, relErr=0.0, absErr=0.0, upperBound=float("inf")):
+        s = DistSettings(relErr, absErr, upperBound)
+        r = DistResult()
+        call("kg_geom_distance", self.h, other.h, ctypes.byref(s), ctypes.byref(r))
+        return r
+
+    def distance_point(self, pt, relErr=0.0, absErr=0.0, upperBound=float("inf")):
+        s = DistSettings(relErr, absErr, upperBound)
+        r = DistResult()
+        call("kg_geom_distance_point", self.h, _p(_d(pt), _pd), ctypes.byref(s), ctypes.byref(r))
+        return r
+
+    def sdf_eval(self, pts):
+        P = _f(pts).reshape(-1, 3)
+        d = np.zeros(P.shape[0], np.float32)
+        g = np.zeros((P.shape[0], 3), np.float32)
+        call("kg_sdf_eval", self.h, _p(P, _pf), P.shape[0], _p(d, _pf), _p(g, _pf))
+        return d, g
+
+    def close(self):
+        if self.h:
+            lib().kg_geom_destroy(self.h)
+            self.h = None
+
+
+class Robot:
+    """Kinematic tree with per-link meshes (``kg_robot``)."""
+
+    def __init__(self, ctx, parents, T0, axes, jtype, qmin, qmax):
+        self.ctx = ctx
+        self.n = len(parents)
+        h = ctypes.c_void_p()
+        call("kg_robot_create", ctx.h, self.n, _p(_i(parents), _pi), _p(_d(T0), _pd), _p(_d(axes), _pd),
+             _p(_i(jtype), _pi), _p(_d(qmin), _pd), _p(_d(qmax), _pd), ctypes.byref(h))
+        self.h = h
+        self.link_geoms = [None] * self.n
+
+    @classmethod
+    def from_spec(cls, ctx, spec):
+        r = cls(ctx, spec.parents, spec.T0, spec.axes, spec.jtype, spec.qmin, spec.qmax)
+        for l, m in enumerate(spec.link_meshes):
+            if m is not None:
+                r.set_link_geom(l, Geom.trimesh(ctx, m.verts, m.tris))
+        if spec.self_pairs is not None:
+            for i in range(r.n):
+                for j in range(i + 1, r.n):
+                    r.enable_self_collision(i, j, False)
+            for i, j in spec.self_pairs:
+                r.enable_self_collision(int(i), int(j), True)
+        return r
+
+    def set_link_geom(self, link, geom):
+        call("kg_robot_set_link_geom", self.h, link, geom.h if geom is not None else None)
+        self.link_geoms[link] = geom
+
+    def enable_self_collision(self, i, j, enabled=True):
+        call("kg_robot_enable_self_collision", self.h, i, j, 1 if enabled else 0)
+
+    def self_collision_enabled(self, i, j):
+        out = ctypes.c_int32()
+        call("kg_robot_self_collision_enabled", self.h, i, j, ctypes.byref(out))
+        return bool(out.value)
+
+    def set_config(self, q):
+        call("kg_robot_set_config", self.h, _p(_d(q), _pd))
+
+    def get_config(self):
+        q = np.zeros(self.n)
+        call("kg_robot_get_config", self.h, _p(q, _pd))
+        return q
+
+    def link_transform(self, link):
+        R, t = np.zeros(9), np.zeros(3)
+        call("kg_robot_get_link_transform", self.h, link, _p(R, _pd), _p(t, _pd))
+        return R, t
+
+    def self_collides(self):
+        out = ctypes.c_int32()
+        call("kg_robot_self_collides", self.h, ctypes.byref(out))
+        return bool(out.value)
+
+    def fk_batch(self, Q):
+        Q = _f(Q).reshape(-1, self.n)
+        T = np.zeros((Q.shape[0], self.n, 12), np.float32)
+        call("kg_fk_batch", self.h, _p(Q, _pf), Q.shape[0], _p(T, _pf))
+        return T
+
+    def fk_batch_dev(self, Q, out=None):
+        import torch
+        assert Q.is_cuda and Q.dtype == torch.float32 and Q.is_contiguous()
+        N = Q.shape[0]
+        if out is None:
+            out = torch.empty((N, self.n, 12), dtype=torch.float32, device=Q.device)
+        call("kg_fk_batch_dev", self.h, _tp(Q), N, _tp(out), _torch_stream())
+        return out
+
+
+class World:
+    """World model with Klampt's ID layout and default collision mask."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        call("kg_world_create", ctx.h, ctypes.byref(h))
+        self.h = h
+        self.robots = []
+        self._keep = []
+
+    def add_terrain(self, geom):
+        out = ctypes.c_int32()
+        call("kg_world_add_terrain", self.h, geom.h, ctypes.byref(out))
+        self._keep.append(geom)
+        return out.value
+
+    def add_rigid_object(self, geom, xf12):
+        x = _d(xf12)
+        out = ctypes.c_int32()
+        call("kg_world_add_rigid_object", self.h, geom.h, _p(x[:9].copy(), _pd), _p(x[9:].copy(), _pd),
+             ctypes.byref(out))
+        self._keep.append(geom)
+        return out.value
+
+    def add_robot(self, robot):
+        out = ctypes.c_int32()
+        call("kg_world_add_robot", self.h, robot.h, ctypes.byref(out))
+        self.robots.append(robot)
+        return len(self.robots) - 1
+
+    def num_ids(self):
+        return lib().kg_world_num_ids(self.h)
+
+    def set_collision_enabled(self, a, b, enabled):
+        call("kg_world_set_collision_enabled", self.h, a, b, 1 if enabled else 0)
+
+    def collision_enabled(self, a, b):
+        out = ctypes.c_int32()
+        call("kg_world_get_collision_enabled", self.h, a, b, ctypes.byref(out))
+        return bool(out.value)
+
+    def pair_list(self, robot=0):
+        n = ctypes.c_int32()
+        call("kg_world_pair_list", self.h, robot, None, 0, ctypes.byref(n))
+        out = np.zeros((max(1, n.value), 2), np.int32)
+        call("kg_world_pair_list", self.h, robot, _p(out, _pi), n.value, ctypes.byref(n))
+        return out[:n.value].copy()
+
+    def feasible_batch(self, Q, robot=0, flags=False, first_pair=False):
+        n = self.robots[robot].n
+        Q = _f(Q).reshape(-1, n)
+        N = Q.shape[0]
+        feas = np.zeros(N, np.uint8)
+        env = np.zeros(N, np.uint8) if flags else None
+        slf = np.zeros(N, np.uint8) if flags else None
+        fp = np.zeros((N, 2), np.int32) if first_pair else None
+        call("kg_feasible_batch", self.h, robot, _p(Q, _pf), N, _p(feas, _pu8), _p(env, _pu8), _p(slf, _pu8),
+             _p(fp, _pi))
+        return feas, env, slf, fp
+
+    def feasible_batch_dev(self, Q, robot=0, feasible=None, env=None, self_hit=None, first_pair=None):
+        import torch
+        assert Q.is_cuda and Q.dtype == torch.float32 and Q.is_contiguous()
+        N = Q.shape[0]
+        if feasible is None:
+            feasible = torch.empty(N, dtype=torch.uint8, device=Q.device)
+        call("kg_feasible_batch_dev", self.h, robot, _tp(Q), N, _tp(feasible), _tp(env), _tp(self_hit),
+             _tp(first_pair), _torch_stream())
+        return feasible
+
+    def distance_batch(self, Q, robot=0):
+        n = self.robots[robot].n
+        Q = _f(Q).reshape(-1, n)
+        npairs = len(self.pair_list(robot))
+        out = np.zeros((Q.shape[0], npairs), np.float32)
+        call("kg_distance_batch", self.h, robot, _p(Q, _pf), Q.shape[0], _p(out, _pf))
+        return out
+
+    def distance_batch_dev(self, Q, robot=0, out=None):
+        import torch
+        N = Q.shape[0]
+        npairs = len(self.pair_list(robot))
+        if out is None:
+            out = torch.empty((N, npairs), dtype=torch.float32, device=Q.device)
+        call("kg_distance_batch_dev", self.h, robot, _tp(Q), N, _tp(out), _torch_stream())
+        return out
+
+    def edge_batch(self, A, B, eps=0.01, norm_inf=True, robot=0):
+        n = self.robots[robot].n
+        A, B = _f(A).reshape(-1, n), _f(B).reshape(-1, n)
+        out = np.zeros(A.shape[0], np.float32)
+        call("kg_edge_batch", self.h, robot, _p(A, _pf), _p(B, _pf), A.shape[0], float(eps), 1 if norm_inf else 0,
+             _p(out, _pf))
+        return out
+
+    def last_stats(self):
+        s = np.zeros(8, np.int64)
+        call("kg_world_last_stats", self.h, s.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), 8)
+        keys = ["fp64_rechecks", "pair_items", "spills", "t_prep_ns", "t_trav32_ns", "t_trav64_ns",
+                "box_tests", "tri_tests"]
+        return {k: int(v) for k, v in zip(keys, s)}
+
+
+def world_from_scene(ctx, scene):
+    """World for a ``klampt_amd.synthetic.Scene``: terrains, rigid objects,
+    then the robot (Klampt ID order)."""
+    w = World(ctx)
+    for t in scene.terrains:
+        w.add_terrain(Geom.trimesh(ctx, t.verts, t.tris))
+    for mesh, xf in scene.objects:
+        w.add_rigid_object(Geom.trimesh(ctx, mesh.verts, mesh.tris), xf)
+    r = Robot.from_spec(ctx, scene.robot)
+    w.add_robot(r)
+    return w, r

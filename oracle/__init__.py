@@ -1,0 +1,277 @@
+"""ctypes wrapper of the FP64 CPU oracle (``oracle/kg_oracle.cpp``).
+
+TEST INFRASTRUCTURE ONLY: imported by ``tests/``, ``__graft_entry__.smoke()``
+and the ``cpu_baseline`` leg of ``bench.py`` as the checker.  The product
+library (``klampt_amd``) never imports it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_SO = os.path.join(_HERE, "build", "liboracle.so")
+_lib = None
+
+_d = ctypes.POINTER(ctypes.c_double)
+_i = ctypes.POINTER(ctypes.c_int32)
+_u8 = ctypes.POINTER(ctypes.c_uint8)
+_vp = ctypes.c_void_p
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_SO):
+            build()
+        L = ctypes.CDLL(_SO)
+        sig = {
+            "or_world_new": (_vp, []),
+            "or_world_free": (None, [_vp]),
+            "or_add_mesh": (ctypes.c_int, [_vp, _d, ctypes.c_int, _i, ctypes.c_int]),
+            "or_add_terrain": (ctypes.c_int, [_vp, ctypes.c_int]),
+            "or_add_object": (ctypes.c_int, [_vp, ctypes.c_int, _d]),
+            "or_add_robot": (ctypes.c_int, [_vp, ctypes.c_int, _i, _d, _d, _i, _d, _d, _i]),
+            "or_set_self_collision": (None, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+            "or_init_default": (None, [_vp]),
+            "or_num_ids": (ctypes.c_int, [_vp]),
+            "or_set_collision_enabled": (None, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+            "or_get_collision_enabled": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+            "or_set_robot_config": (None, [_vp, ctypes.c_int, _d]),
+            "or_fk": (None, [_vp, ctypes.c_int, _d, _d]),
+            "or_feasible_batch": (None, [_vp, ctypes.c_int, _d, ctypes.c_int, _u8, _u8, _u8, _i,
+                                         ctypes.c_int, ctypes.c_int]),
+            "or_pair_list": (ctypes.c_int, [_vp, ctypes.c_int, _i, ctypes.c_int]),
+            "or_isfeasible_batch": (None, [_vp, ctypes.c_int, _d, ctypes.c_int, _u8, ctypes.c_int]),
+            "or_distance_batch": (None, [_vp, ctypes.c_int, _d, ctypes.c_int, _d, ctypes.c_int,
+                                         ctypes.c_int, ctypes.c_int]),
+            "or_edge_batch": (None, [_vp, ctypes.c_int, _d, _d, ctypes.c_int, ctypes.c_double,
+                                     ctypes.c_int, _d, _i, _i, ctypes.c_int]),
+            "or_bisection_order": (None, [ctypes.c_int, _i]),
+            "or_mesh_collide": (ctypes.c_int, [_vp, ctypes.c_int, _d, ctypes.c_int, _d, ctypes.c_int]),
+            "or_mesh_distance": (ctypes.c_double, [_vp, ctypes.c_int, _d, ctypes.c_int, _d, ctypes.c_int,
+                                                   _d, _d, _i]),
+            "or_mesh_separation": (ctypes.c_double, [_vp, ctypes.c_int, _d, ctypes.c_int, _d]),
+            "or_tritri": (ctypes.c_int, [_d, _d]),
+            "or_tritri_dist": (ctypes.c_double, [_d, _d]),
+            "or_tritri_depth": (ctypes.c_double, [_d, _d]),
+            "or_world_aabb": (None, [_vp, ctypes.c_int, _d, _d, _d]),
+            "or_sdf_eval": (None, [_d, _i, _d, _d, _d, ctypes.c_int, _d, _d]),
+            "or_sdf_penalty": (None, [_vp, ctypes.c_int, _d, _i, _d, _d, _d, ctypes.c_int, _d, _i,
+                                      ctypes.c_double, _d, _d, ctypes.c_double, ctypes.c_int]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+def _dd(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _ii(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def nthreads_default():
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+class OracleWorld:
+    """FP64 restatement of a Klampt ``WorldModel`` + ``WorldPlannerSettings``."""
+
+    def __init__(self):
+        self.L = lib()
+        self.h = self.L.or_world_new()
+        self._robots = []
+
+    def __del__(self):
+        try:
+            self.L.or_world_free(self.h)
+        except Exception:
+            pass
+
+    def add_mesh(self, mesh):
+        V = _dd(mesh.verts)
+        T = _ii(mesh.tris)
+        return self.L.or_add_mesh(self.h, _p(V, _d), V.shape[0], _p(T, _i), T.shape[0])
+
+    def add_terrain(self, mesh):
+        return self.L.or_add_terrain(self.h, self.add_mesh(mesh))
+
+    def add_object(self, mesh, xf12):
+        x = _dd(xf12)
+        return self.L.or_add_object(self.h, self.add_mesh(mesh), _p(x, _d))
+
+    def add_robot(self, spec):
+        geoms = [self.add_mesh(m) if m is not None else -1 for m in spec.link_meshes]
+        g = _ii(geoms)
+        par, T0, ax = _ii(spec.parents), _dd(spec.T0), _dd(spec.axes)
+        jt, lo, hi = _ii(spec.jtype), _dd(spec.qmin), _dd(spec.qmax)
+        r = self.L.or_add_robot(self.h, spec.n, _p(par, _i), _p(T0, _d), _p(ax, _d), _p(jt, _i),
+                                _p(lo, _d), _p(hi, _d), _p(g, _i))
+        if spec.self_pairs is not None:
+            for i in range(spec.n):
+                for j in range(i + 1, spec.n):
+                    self.L.or_set_self_collision(self.h, r, i, j, 0)
+            for i, j in spec.self_pairs:
+                self.L.or_set_self_collision(self.h, r, int(min(i, j)), int(max(i, j)), 1)
+        self._robots.append(spec)
+        return r
+
+    def init_default(self):
+        self.L.or_init_default(self.h)
+
+    def num_ids(self):
+        return self.L.or_num_ids(self.h)
+
+    def set_collision_enabled(self, a, b, en):
+        self.L.or_set_collision_enabled(self.h, a, b, 1 if en else 0)
+
+    def set_robot_config(self, r, q):
+        q = _dd(q)
+        self.L.or_set_robot_config(self.h, r, _p(q, _d))
+
+    def fk(self, r, q):
+        q = _dd(q)
+        n = self._robots[r].n
+        out = np.zeros((n, 12))
+        self.L.or_fk(self.h, r, _p(q, _d), _p(out, _d))
+        return out
+
+    def feasible_batch(self, r, Q, brute=False, nthreads=None):
+        Q = _dd(Q)
+        N = Q.shape[0]
+        feas = np.zeros(N, np.uint8)
+        env = np.zeros(N, np.uint8)
+        slf = np.zeros(N, np.uint8)
+        fp = np.zeros((N, 2), np.int32)
+        self.L.or_feasible_batch(self.h, r, _p(Q, _d), N, _p(feas, _u8), _p(env, _u8), _p(slf, _u8),
+                                 _p(fp, _i), 1 if brute else 0, nthreads or nthreads_default())
+        return feas, env, slf, fp
+
+    def isfeasible_batch(self, r, Q, nthreads=None):
+        """IsFeasible with Klampt's early returns (the timed CPU path)."""
+        Q = _dd(Q)
+        feas = np.zeros(Q.shape[0], np.uint8)
+        self.L.or_isfeasible_batch(self.h, r, _p(Q, _d), Q.shape[0], _p(feas, _u8), nthreads or nthreads_default())
+        return feas
+
+    def pair_list(self, r):
+        out = np.zeros((4096, 2), np.int32)
+        n = self.L.or_pair_list(self.h, r, _p(out, _i), 4096)
+        return out[:n].copy()
+
+    def distance_batch(self, r, Q, brute=False, signed=False, nthreads=None):
+        Q = _dd(Q)
+        np_ = len(self.pair_list(r))
+        out = np.zeros((Q.shape[0], np_))
+        self.L.or_distance_batch(self.h, r, _p(Q, _d), Q.shape[0], _p(out, _d), 1 if brute else 0,
+                                 1 if signed else 0, nthreads or nthreads_default())
+        return out
+
+    def edge_batch(self, r, A, B, eps=0.01, norm_inf=True, nthreads=None):
+        A, B = _dd(A), _dd(B)
+        ns = A.shape[0]
+        fu = np.zeros(ns)
+        nsamp = np.zeros(ns, np.int32)
+        nchk = np.zeros(ns, np.int32)
+        self.L.or_edge_batch(self.h, r, _p(A, _d), _p(B, _d), ns, eps, 1 if norm_inf else 0, _p(fu, _d),
+                             _p(nsamp, _i), _p(nchk, _i), nthreads or nthreads_default())
+        return fu, nsamp, nchk
+
+    def mesh_collide(self, ga, xa, gb, xb, brute=False):
+        xa, xb = _dd(xa), _dd(xb)
+        return bool(self.L.or_mesh_collide(self.h, ga, _p(xa, _d), gb, _p(xb, _d), 1 if brute else 0))
+
+    def mesh_distance(self, ga, xa, gb, xb, brute=False):
+        xa, xb = _dd(xa), _dd(xb)
+        cp1, cp2 = np.zeros(3), np.zeros(3)
+        el = np.zeros(2, np.int32)
+        d = self.L.or_mesh_distance(self.h, ga, _p(xa, _d), gb, _p(xb, _d), 1 if brute else 0,
+                                    _p(cp1, _d), _p(cp2, _d), _p(el, _i))
+        return d, cp1, cp2, el
+
+    def mesh_separation(self, ga, xa, gb, xb):
+        xa, xb = _dd(xa), _dd(xb)
+        return self.L.or_mesh_separation(self.h, ga, _p(xa, _d), gb, _p(xb, _d))
+
+    def world_aabb(self, g, x):
+        x = _dd(x)
+        lo, hi = np.zeros(3), np.zeros(3)
+        self.L.or_world_aabb(self.h, g, _p(x, _d), _p(lo, _d), _p(hi, _d))
+        return lo, hi
+
+    def sdf_penalty(self, r, grid, Q, pts, pts_per_link, dsafe=0.05, grad=True, fd_h=0.0, nthreads=None):
+        vals = _dd(grid.values)
+        dims = _ii(grid.values.shape)
+        bmin, bmax = _dd(grid.bmin), _dd(grid.bmax)
+        Q, P, ppl = _dd(Q), _dd(pts), _ii(pts_per_link)
+        N, n = Q.shape
+        loss = np.zeros(N)
+        g = np.zeros((N, n)) if grad else None
+        self.L.or_sdf_penalty(self.h, r, _p(vals, _d), _p(dims, _i), _p(bmin, _d), _p(bmax, _d), _p(Q, _d), N,
+                              _p(P, _d), _p(ppl, _i), dsafe, _p(loss, _d), _p(g, _d), fd_h,
+                              nthreads or nthreads_default())
+        return loss, g
+
+
+def tritri(a, b):
+    a, b = _dd(a), _dd(b)
+    return bool(lib().or_tritri(_p(a, _d), _p(b, _d)))
+
+
+def tritri_dist(a, b):
+    a, b = _dd(a), _dd(b)
+    return lib().or_tritri_dist(_p(a, _d), _p(b, _d))
+
+
+def tritri_depth(a, b):
+    a, b = _dd(a), _dd(b)
+    return lib().or_tritri_depth(_p(a, _d), _p(b, _d))
+
+
+def bisection_order(n):
+    out = np.zeros(max(0, n - 1), np.int32)
+    if n >= 2:
+        lib().or_bisection_order(n, _p(out, _i))
+    return out
+
+
+def sdf_eval(grid, pts):
+    vals = _dd(grid.values)
+    dims = _ii(grid.values.shape)
+    bmin, bmax = _dd(grid.bmin), _dd(grid.bmax)
+    P = _dd(pts)
+    d = np.zeros(P.shape[0])
+    g = np.zeros((P.shape[0], 3))
+    lib().or_sdf_eval(_p(vals, _d), _p(dims, _i), _p(bmin, _d), _p(bmax, _d), _p(P, _d), P.shape[0],
+                      _p(d, _d), _p(g, _d))
+    return d, g
+
+
+def world_from_scene(scene):
+    """OracleWorld for a ``klampt_amd.synthetic.Scene``: terrains, then rigid
+    objects, then the robot (Klampt ID order), with the default mask."""
+    w = OracleWorld()
+    for t in scene.terrains:
+        w.add_terrain(t)
+    for mesh, xf in scene.objects:
+        w.add_object(mesh, xf)
+    r = w.add_robot(scene.robot)
+    w.init_default()
+    return w, r
