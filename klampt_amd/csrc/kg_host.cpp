@@ -365,23 +365,30 @@ struct BatchStats {
   int64_t redo = 0, items = 0, spills = 0;
   int64_t t_prep_ns = 0, t_trav32_ns = 0, t_trav64_ns = 0, box_tests = 0, tri_tests = 0;
 };
-std::vector<std::pair<const kg_world_s*, BatchStats>> g_stats;
-BatchStats& stats_of(const kg_world_s* w) {
-  for (auto& p : g_stats)
-    if (p.first == w) return p.second;
-  g_stats.push_back({w, {}});
-  return g_stats.back().second;
-}
 
-// Batched feasibility on device buffers.  Q is float (Qf) or double (Qd).
+// Status words of one batch (device, zeroed by the prep kernel):
+//   [0] spill overflow  [1] FP64 re-check count  [2] FP32 chunk counter
+//   [3] FP64 chunk counter  [4..9] u64 counters: box tests, triangle tests,
+//   pair items.
+enum { ST_SPILL = 0, ST_REDO = 1, ST_CHUNK32 = 2, ST_CHUNK64 = 3, ST_CNT = 4, ST_WORDS = 16 };
+
+// Batched feasibility on device buffers, asynchronous on stream s.  Q is
+// float (Qf) or double (Qd).
 void feasible_dev(kg_world_s* w, int ri, const float* Qf, const double* Qd, int N, uint8_t* feas, uint8_t* env,
                   uint8_t* self, int32_t* fp, hipStream_t s) {
   kg_ctx_s* c = w->ctx;
   hip_check(hipSetDevice(c->device), "hipSetDevice");
   RobotPlan& P = get_plan(w, ri);
-  BatchStats& st = stats_of(w);
-  st = BatchStats{};
-  if (N <= 0) return;
+  if (!c->hstatus) {
+    hip_check(hipHostMalloc((void**)&c->hstatus, ST_WORDS * sizeof(int), hipHostMallocDefault), "hipHostMalloc");
+    for (auto& e : c->ev) hip_check(hipEventCreate(&e), "hipEventCreate");
+  }
+  c->last_N = N;
+  c->last_stream = s;
+  if (N <= 0) {
+    std::memset(c->hstatus, 0, ST_WORDS * sizeof(int));
+    return;
+  }
   const int n = P.dev.nlinks;
   const int mode = fp ? KG_MODE_ORDERED : ((env || self) ? KG_MODE_GROUPS : KG_MODE_ANY);
   hip_check(c->wsT64.ensure((size_t)N * n * 12 * sizeof(double)), "hipMalloc(T64)");
@@ -389,6 +396,10 @@ void feasible_dev(kg_world_s* w, int ri, const float* Qf, const double* Qd, int 
   hip_check(c->wsJL.ensure((size_t)N), "hipMalloc(jl)");
   hip_check(c->wsCounts.ensure((size_t)(N + 1) * sizeof(int)), "hipMalloc(counts)");
   hip_check(c->wsOffsets.ensure((size_t)(N + 1) * sizeof(int)), "hipMalloc(offsets)");
+  hip_check(c->wsStatus.ensure(ST_WORDS * sizeof(int)), "hipMalloc(status)");
+  hip_check(c->wsRedo.ensure((size_t)(N + 16) * sizeof(int)), "hipMalloc(redo)");
+  int* status = c->wsStatus.as<int>();
+  int* redo_list = c->wsRedo.as<int>();
   PrepArgs pa{};
   pa.Qf = Qf;
   pa.Qd = Qd;
@@ -400,54 +411,42 @@ void feasible_dev(kg_world_s* w, int ri, const float* Qf, const double* Qd, int 
   pa.LB = c->wsLB.as<double>();
   pa.jl = c->wsJL.as<uint8_t>();
   pa.counts = c->wsCounts.as<int>();
-  pa.offsets = c->wsOffsets.as<int>();
-  hipEvent_t ev[6];
-  for (auto& e : ev) hip_check(hipEventCreate(&e), "hipEventCreate");
-  struct EvGuard {
-    hipEvent_t* e;
-    ~EvGuard() {
-      for (int i = 0; i < 6; i++) (void)hipEventDestroy(e[i]);
-    }
-  } evg{ev};
-  hip_check(hipEventRecord(ev[0], s), "hipEventRecord");
-  hip_check(kgl_prep_count(pa, s), "prep_count");
-  hip_check(kgl_scan(pa.counts, pa.offsets, N + 1, c->wsScanTmp, s), "scan");
+  pa.reset = status;
+  pa.nreset = ST_WORDS;
   int nact = 0;
   for (int l = 0; l < n; l++)
     if (w->robots[ri]->link_geoms[l] && !geom_empty(w->robots[ri]->link_geoms[l])) nact++;
-  size_t bound = (size_t)nact * P.nenv + P.nself;
-  size_t total;
-  if (bound * (size_t)N * sizeof(PairItem) <= (512u << 20)) {
-    total = bound * (size_t)N;
+  const size_t cap = (size_t)nact * P.nenv + P.nself;
+  const bool fixed = cap * (size_t)N * sizeof(PairItem) <= (512u << 20);
+  hip_check(hipEventRecord(c->ev[0], s), "hipEventRecord");
+  if (fixed) {
+    hip_check(c->wsItems.ensure(std::max<size_t>(1, cap * (size_t)N) * sizeof(PairItem)), "hipMalloc(items)");
+    pa.items = c->wsItems.as<PairItem>();
+    pa.cap = (int)cap;
+    pa.offsets = nullptr;
+    hip_check(kgl_prep_single(pa, s), "prep");
   } else {
+    // large environments: count, scan, then fill exactly
+    pa.offsets = c->wsOffsets.as<int>();
+    hip_check(kgl_fill_int(status, ST_WORDS, 0, s), "clear status");
+    pa.nreset = 0;
+    hip_check(kgl_prep_count(pa, s), "prep_count");
+    hip_check(kgl_scan(pa.counts, pa.offsets, N + 1, c->wsScanTmp, s), "scan");
     int t = 0;
     hip_check(hipMemcpyAsync(&t, pa.offsets + N, sizeof(int), hipMemcpyDeviceToHost, s), "copy total");
     hip_check(hipStreamSynchronize(s), "sync");
-    total = (size_t)t;
+    hip_check(c->wsItems.ensure(std::max<size_t>(1, (size_t)t) * sizeof(PairItem)), "hipMalloc(items)");
+    pa.items = c->wsItems.as<PairItem>();
+    hip_check(kgl_prep_fill(pa, s), "prep_fill");
   }
-  hip_check(c->wsItems.ensure(std::max<size_t>(1, total) * sizeof(PairItem)), "hipMalloc(items)");
-  pa.items = c->wsItems.as<PairItem>();
-  hip_check(kgl_prep_fill(pa, s), "prep_fill");
-  hip_check(hipEventRecord(ev[1], s), "hipEventRecord");
+  hip_check(hipEventRecord(c->ev[1], s), "hipEventRecord");
 
-  // FP32 traversal
   const int grid32 = kgl_collide_grid(0, c->num_cus);
-  const int grid64 = kgl_collide_grid(1, c->num_cus);
-  const int grid = std::max(grid32, grid64);
+  const int grid64 = std::min(kgl_collide_grid(1, c->num_cus), std::max(1, c->num_cus));
   const int spill_cap = env_int("KG_SPILL_CAP", 16384);
   hip_check(c->wsBest.ensure((size_t)N * 2 * sizeof(unsigned long long)), "hipMalloc(best)");
   hip_check(c->wsUnc.ensure((size_t)N * 2 * sizeof(int)), "hipMalloc(unc)");
-  hip_check(c->wsChunk.ensure(64), "hipMalloc(chunk)");
-  hip_check(c->wsSpill.ensure((size_t)grid * spill_cap * sizeof(uint64_t)), "hipMalloc(spill)");
-  hip_check(c->wsStatus.ensure(64), "hipMalloc(status)");
-  hip_check(c->wsRedo.ensure((size_t)(N + 16) * sizeof(int)), "hipMalloc(redo)");
-  int* status = c->wsStatus.as<int>();
-  int* redo_count = c->wsRedo.as<int>();
-  int* redo_list = redo_count + 16;
-  hip_check(kgl_fill_int(status, 16, 0, s), "clear status");
-  hip_check(kgl_fill_int(redo_count, 1, 0, s), "clear redo");
-  unsigned long long* counters = (unsigned long long*)(status + 4);
-  hip_check(kgl_fill_int(status + 4, 4, 0, s), "clear counters");
+  hip_check(c->wsSpill.ensure((size_t)std::max(grid32, grid64) * spill_cap * sizeof(uint64_t)), "hipMalloc(spill)");
 
   TravArgs ta{};
   ta.pool = c->pool;
@@ -455,22 +454,27 @@ void feasible_dev(kg_world_s* w, int ri, const float* Qf, const double* Qd, int 
   ta.T64 = pa.T64;
   ta.items = pa.items;
   ta.offsets = pa.offsets;
+  ta.counts = pa.counts;
+  ta.cap = pa.cap;
   ta.cfg_list = nullptr;
   ta.ncfg = N;
+  ta.ncfg_dev = nullptr;
   ta.mode = mode;
   ta.tol = P.tol32;
   ta.best = c->wsBest.as<unsigned long long>();
   ta.unc = c->wsUnc.as<int>();
-  ta.chunk_counter = c->wsChunk.as<int>();
+  ta.chunk_counter = status + ST_CHUNK32;
   ta.spill = c->wsSpill.as<uint64_t>();
   ta.spill_cap = spill_cap;
-  ta.status = status;
+  ta.stack_cap = 0;
+  ta.status = status + ST_SPILL;
   ta.cpw = std::max(1, std::min(64, env_int("KG_CPW", 16)));
+  ta.leaf_min = std::max(1, std::min(64, env_int("KG_LEAF_MIN", 1)));
   ta.grid = grid32;
-  ta.counters = counters;
-  hip_check(hipEventRecord(ev[2], s), "hipEventRecord");
+  ta.counters = (unsigned long long*)(status + ST_CNT);
+  hip_check(hipEventRecord(c->ev[2], s), "hipEventRecord");
   hip_check(kgl_collide_f32(ta, s), "collide_f32");
-  hip_check(hipEventRecord(ev[3], s), "hipEventRecord");
+  hip_check(hipEventRecord(c->ev[3], s), "hipEventRecord");
 
   FinArgs fa{};
   fa.N = N;
@@ -485,49 +489,57 @@ void feasible_dev(kg_world_s* w, int ri, const float* Qf, const double* Qd, int 
   fa.self_hit = self;
   fa.first_pair = fp;
   fa.redo_list = redo_list;
-  fa.redo_count = redo_count;
+  fa.redo_count = status + ST_REDO;
   fa.final_pass = 0;
   hip_check(kgl_finalize(fa, s), "finalize");
 
-  int nredo = 0;
-  hip_check(hipMemcpyAsync(&nredo, redo_count, sizeof(int), hipMemcpyDeviceToHost, s), "copy redo");
-  hip_check(hipStreamSynchronize(s), "sync");
-  st.redo = nredo;
-  if (nredo > 0) {
-    TravArgs tb = ta;
-    tb.cfg_list = redo_list;
-    tb.ncfg = nredo;
-    tb.tol = P.tol64;
-    tb.cpw = 4;
-    tb.grid = grid64;
-    tb.counters = nullptr;
-    hip_check(hipEventRecord(ev[4], s), "hipEventRecord");
-    hip_check(kgl_collide_f64(tb, s), "collide_f64");
-    hip_check(hipEventRecord(ev[5], s), "hipEventRecord");
-    FinArgs fb = fa;
-    fb.final_pass = 1;
-    hip_check(kgl_finalize_list(fb, redo_list, redo_count, nredo, s), "finalize_f64");
-  }
-  int hstat[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  hip_check(hipMemcpyAsync(hstat, status, 8 * sizeof(int), hipMemcpyDeviceToHost, s), "copy status");
-  hip_check(hipStreamSynchronize(s), "sync");
-  float ms = 0;
-  if (hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) st.t_prep_ns = (int64_t)(ms * 1e6);
-  if (hipEventElapsedTime(&ms, ev[2], ev[3]) == hipSuccess) st.t_trav32_ns = (int64_t)(ms * 1e6);
-  if (nredo > 0 && hipEventElapsedTime(&ms, ev[4], ev[5]) == hipSuccess) st.t_trav64_ns = (int64_t)(ms * 1e6);
-  unsigned long long cnt[2];
-  std::memcpy(cnt, hstat + 4, sizeof(cnt));
+  // FP64 re-check of the configurations the FP32 pass could not certify;
+  // the count stays on the device (waves exit at once when it is zero)
+  TravArgs tb = ta;
+  tb.cfg_list = redo_list;
+  tb.ncfg_dev = status + ST_REDO;
+  tb.tol = P.tol64;
+  tb.cpw = 1;
+  tb.grid = grid64;
+  tb.counters = nullptr;
+  tb.chunk_counter = status + ST_CHUNK64;
+  hip_check(hipEventRecord(c->ev[4], s), "hipEventRecord");
+  hip_check(kgl_collide_f64(tb, s), "collide_f64");
+  hip_check(hipEventRecord(c->ev[5], s), "hipEventRecord");
+  FinArgs fb = fa;
+  fb.final_pass = 1;
+  hip_check(kgl_finalize_list(fb, redo_list, status + ST_REDO, N, s), "finalize_f64");
+  hip_check(hipMemcpyAsync(c->hstatus, status, ST_WORDS * sizeof(int), hipMemcpyDeviceToHost, s), "copy status");
+}
+
+// Read the status of the last batch (waits for it).
+void read_stats(kg_world_s* w, BatchStats& st) {
+  kg_ctx_s* c = w->ctx;
+  st = BatchStats{};
+  if (!c->hstatus || !c->last_stream) return;
+  hip_check(hipStreamSynchronize(c->last_stream), "sync");
+  if (c->last_N <= 0) return;
+  const int* h = c->hstatus;
+  st.spills = h[ST_SPILL];
+  st.redo = h[ST_REDO];
+  unsigned long long cnt[3];
+  std::memcpy(cnt, h + ST_CNT, sizeof(cnt));
   st.box_tests = (int64_t)cnt[0];
   st.tri_tests = (int64_t)cnt[1];
-  {
-    int tot = 0;
-    hip_check(hipMemcpy(&tot, pa.offsets + N, sizeof(int), hipMemcpyDeviceToHost), "copy total");
-    st.items = tot;
-  }
-  if (hstat[0]) {
-    st.spills = 1;
-    if (nredo > 0) fail(KG_E_LIMIT, "BVH traversal stack exceeded its HBM spill capacity (KG_SPILL_CAP)");
-  }
+  st.items = (int64_t)cnt[2];
+  float ms = 0;
+  if (hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) st.t_prep_ns = (int64_t)(ms * 1e6);
+  if (hipEventElapsedTime(&ms, c->ev[2], c->ev[3]) == hipSuccess) st.t_trav32_ns = (int64_t)(ms * 1e6);
+  if (hipEventElapsedTime(&ms, c->ev[4], c->ev[5]) == hipSuccess) st.t_trav64_ns = (int64_t)(ms * 1e6);
+}
+
+// Synchronous wrapper for internal single queries: errors of the FP64 pass
+// (spill overflow) surface here.
+void feasible_sync_check(kg_world_s* w) {
+  BatchStats st;
+  read_stats(w, st);
+  if (st.spills && st.redo > 0)
+    fail(KG_E_LIMIT, "BVH traversal stack exceeded its HBM spill capacity (KG_SPILL_CAP)");
 }
 
 kg_world_s* solo_world(kg_robot_s* r);
@@ -572,6 +584,9 @@ int kg_ctx_destroy(kg_ctx c) {
                       &c->wsScanTmp, &c->wsBest, &c->wsUnc, &c->wsChunk, &c->wsSpill, &c->wsStatus, &c->wsRedo,
                       &c->wsOut8a, &c->wsOut8b, &c->wsOut8c, &c->wsOut32, &c->wsF, &c->wsG};
     for (auto* b : bufs) b->release();
+    if (c->hstatus) (void)hipHostFree(c->hstatus);
+    for (auto& e : c->ev)
+      if (e) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream);
     delete c;
   });
@@ -892,11 +907,6 @@ int kg_world_destroy(kg_world w) {
         g_overrides.erase(it);
         break;
       }
-    for (auto it = g_stats.begin(); it != g_stats.end(); ++it)
-      if (it->first == w) {
-        g_stats.erase(it);
-        break;
-      }
     delete w;
   });
 }
@@ -1022,13 +1032,15 @@ int kg_feasible_batch(kg_world w, int ri, const float* Q, int N, uint8_t* feas, 
     if (self) hip_check(hipMemcpyAsync(self, dse, N, hipMemcpyDeviceToHost, c->stream), "copy out");
     if (fp) hip_check(hipMemcpyAsync(fp, dfp, (size_t)N * 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream), "copy");
     hip_check(hipStreamSynchronize(c->stream), "sync");
+    feasible_sync_check(w);
   });
 }
 
 int kg_world_last_stats(kg_world w, int64_t* out, int nstats) {
   return guard([&] {
     need(w && out, "NULL argument");
-    BatchStats& st = stats_of(w);
+    BatchStats st;
+    read_stats(w, st);
     int64_t v[8] = {st.redo, st.items, st.spills, st.t_prep_ns, st.t_trav32_ns, st.t_trav64_ns, st.box_tests,
                     st.tri_tests};
     for (int i = 0; i < nstats && i < 8; i++) out[i] = v[i];
@@ -1073,6 +1085,7 @@ extern "C" int kg_robot_self_collides(kg_robot r, int* out) {
     uint8_t h = 0;
     hip_check(hipMemcpyAsync(&h, c->wsOut8c.p, 1, hipMemcpyDeviceToHost, c->stream), "copy");
     hip_check(hipStreamSynchronize(c->stream), "sync");
+    feasible_sync_check(w);
     *out = h;
   });
 }
@@ -1153,6 +1166,7 @@ extern "C" int kg_geom_collide(kg_geom a, kg_geom b, int* out) {
     uint8_t h = 0;
     hip_check(hipMemcpyAsync(&h, c->wsOut8b.p, 1, hipMemcpyDeviceToHost, c->stream), "copy");
     hip_check(hipStreamSynchronize(c->stream), "sync");
+    feasible_sync_check(pw.w);
     *out = h;
   });
 }

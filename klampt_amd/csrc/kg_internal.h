@@ -100,21 +100,28 @@ struct PrepArgs {
   double* LB;          // N x nlinks x 6 (world AABBs)
   uint8_t* jl;         // N: 1 if within joint limits
   int* counts;         // N+1
-  int* offsets;        // N+1
+  int* offsets;        // N+1 (two-pass layout) or nullptr (fixed capacity)
   PairItem* items;
+  int cap;             // items per configuration in the fixed-capacity layout
+  int* reset;          // nreset ints zeroed by the prep kernel
+  int nreset;
 };
 hipError_t kgl_prep_count(const PrepArgs& a, hipStream_t s);
 hipError_t kgl_scan(int* counts, int* offsets, int n, DevBuf& tmp, hipStream_t s);
 hipError_t kgl_prep_fill(const PrepArgs& a, hipStream_t s);
+hipError_t kgl_prep_single(const PrepArgs& a, hipStream_t s);
 
 struct TravArgs {
   PoolDev pool;
   PlanDev plan;
   const double* T64;
   const PairItem* items;
-  const int* offsets;     // per config item range
+  const int* offsets;     // per config item range [offsets[c], offsets[c+1]), or nullptr:
+  const int* counts;      //   [c*cap, c*cap + counts[c])
+  int cap;
   const int* cfg_list;    // nullable: configs to process (else 0..N-1)
   int ncfg;
+  const int* ncfg_dev;    // nullable: device-side configuration count
   int mode;
   Tol tol;
   unsigned long long* best;   // N x 2: (rank<<32 | item) per group, ~0 = none
@@ -124,8 +131,10 @@ struct TravArgs {
   int spill_cap;              // entries per wave
   int* status;                // [0]: spill overflow flag
   int cpw;                    // configs per chunk
+  int leaf_min;               // lanes holding leaf pairs that trigger a triangle-test step
+  int stack_cap;              // per-lane traversal stack entries (LDS)
   int grid;                   // number of waves launched
-  unsigned long long* counters;  // [0] box tests, [1] triangle tests (nullable)
+  unsigned long long* counters;  // [0] box tests, [1] triangle tests, [2] items (nullable)
 };
 hipError_t kgl_collide_f32(const TravArgs& a, hipStream_t s);
 hipError_t kgl_collide_f64(const TravArgs& a, hipStream_t s);
@@ -147,6 +156,9 @@ struct FinArgs {
   int final_pass;   // 1: uncertainty cannot remain
 };
 hipError_t kgl_finalize(const FinArgs& a, hipStream_t s);
+hipError_t kgl_finalize_list(const FinArgs& a, const int* list, const int* count, int maxn, hipStream_t s);
+hipError_t kgl_fill_int(int* p, int n, int v, hipStream_t s);
+int kgl_collide_grid(int exact, int num_cus);
 
 // ---- C-ABI object layouts
 
@@ -227,6 +239,10 @@ struct kg_ctx_s {
   PoolDev pool{};
   int pool_version = 0;
   // workspace
+  int* hstatus = nullptr;  // pinned copy of the last batch's status words
+  hipStream_t last_stream = nullptr;
+  hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  int last_N = 0;
   DevBuf wsQ, wsT64, wsLB, wsJL, wsCounts, wsOffsets, wsItems, wsScanTmp, wsBest, wsUnc, wsChunk, wsSpill,
       wsStatus, wsRedo, wsOut8a, wsOut8b, wsOut8c, wsOut32, wsF, wsG;
   int num_cus = 256;

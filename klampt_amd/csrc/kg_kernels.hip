@@ -1,17 +1,15 @@
 // HIP kernels for gfx950: forward kinematics, broad phase, BVH-vs-BVH
 // collision traversal, and result assembly.
 //
-// Collision traversal (k_collide) runs one wavefront per workgroup.  A wave
-// owns a chunk of configurations and streams their narrow-phase pair items
-// through 64 slots (one per lane).  All node-pair tasks of the wave's slots
-// share one LIFO task stack in LDS (spilling its bottom half to a per-wave
-// HBM region when full): each step every lane pops one task, tests the two
-// oriented boxes (15-axis separating-axis test) or, for two leaves, the two
-// triangles (17-axis test), and pushes up to two child tasks.  Lanes stay busy
-// as long as any slot has work, whatever the shape of the individual
-// traversals.  A certain hit kills the remaining tasks of its configuration
-// (any-hit), of its group (env / self), or of the items later in Klampt's loop
-// order (ordered first-hit), checked when a task is popped.
+// Collision traversal (k_collide) runs one wavefront per workgroup over
+// narrow-phase pair items (a link mesh against an environment mesh or another
+// link), walking the two OBB trees: each step takes a node pair whose boxes
+// overlap, tests both children of the larger node against the other (15-axis
+// separating-axis test), continues depth first with one overlapping child
+// pair, pools the other, and queues leaf pairs for the 17-axis triangle test.
+// A certain hit kills the remaining work of its configuration (any-hit), of
+// its group (env / self), or of the items later in Klampt's loop order
+// (ordered first-hit).
 //
 // Precision: the FP32 pass carries certified error bounds (Tol); a leaf test
 // that cannot be decided within them marks its configuration for the FP64
@@ -22,7 +20,6 @@
 
 #include "kg_internal.h"
 
-#define KG_SCAP 1024
 #define KG_CPW_MAX 64
 #define KG_NONE64 0xffffffffffffffffull
 
@@ -168,15 +165,17 @@ __device__ int swap_erase_pos(int j, int E, int m, F rej) {
   return -1;
 }
 
-template <bool FILL>
+template <int MODE>
 __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
+  constexpr bool FILL = MODE != 0;
   int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (MODE != 1 && c < a.nreset) a.reset[c] = 0;
   if (c >= a.N) return;
   const PlanDev& P = a.plan;
   const int n = P.nlinks;
   double* T = a.T64 + (size_t)c * n * 12;
   double* LB = a.LB + (size_t)c * n * 6;
-  if (!FILL) {
+  if (MODE != 1) {
     bool jl = true;
     for (int l = 0; l < n; l++) {
       double q = a.Qf ? (double)a.Qf[(size_t)c * n + l] : a.Qd[(size_t)c * n + l];
@@ -250,7 +249,7 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
   }
   int m1 = nact - __popcll(lrej);
   int cnt = 0;
-  int base = FILL ? a.offsets[c] : 0;
+  int base = MODE == 1 ? a.offsets[c] : (MODE == 2 ? c * a.cap : 0);
   auto env_rej = [&](int i) { return !box_hit(P.env[i].lo, P.env[i].hi, ulo, uhi); };
   auto link_rej = [&](int i) { return (lrej >> i & 1) != 0; };
   // env pairs
@@ -293,7 +292,7 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
     }
     cnt++;
   }
-  if (!FILL) a.counts[c] = cnt;
+  if (MODE != 1) a.counts[c] = cnt;
 }
 
 // ------------------------------------------------------------ narrow phase
@@ -407,6 +406,30 @@ __device__ inline void load_node(const ObbNode* __restrict__ nodes, int i, float
   child = __float_as_int(f.w);
 }
 
+struct NodeF {
+  float c[3], e[3], R[9];
+  int ch;
+};
+
+__device__ inline NodeF load_nodef(const ObbNode* __restrict__ nodes, int i) {
+  NodeF n;
+  load_node(nodes, i, n.c, n.e, n.R, n.ch);
+  return n;
+}
+
+__device__ inline NodeF sel(bool p, const NodeF& a, const NodeF& b) {
+  NodeF r;
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    r.c[i] = p ? a.c[i] : b.c[i];
+    r.e[i] = p ? a.e[i] : b.e[i];
+  }
+#pragma unroll
+  for (int i = 0; i < 9; i++) r.R[i] = p ? a.R[i] : b.R[i];
+  r.ch = p ? a.ch : b.ch;
+  return r;
+}
+
 // OBB overlap: box A (frame A) mapped by rel (A-local -> B-local) against box
 // B (frame B).  15 axes, every right-hand side padded by `pad`.
 template <typename Real>
@@ -499,22 +522,65 @@ __device__ inline bool item_dead(int mode, const unsigned long long* best, int g
   return (int)(b >> 32) < r;
 }
 
+// Per-slot error bound for the FP32 narrow phase.  A vertex p of mesh A
+// (|p| <= rA) mapped by the FP32-rounded rel (R,t) to frame B lands within
+// u*(8 rA + 4|t|) of its exact image; a stored vertex of B (|v| <= rB) is
+// within u*rB of its FP64 value.  tau doubles the sum for safety margin.
+template <typename Real, bool EXACT>
+__device__ inline void slot_bounds(const TravArgs& a, double tnorm, double rA, double rB, float& tau, float& pad) {
+  if (EXACT) {
+    tau = 0.f;
+    pad = a.tol.pad_bv;
+    return;
+  }
+  const double u = 5.9604644775390625e-08;
+  double ev = u * (8.0 * rA + 4.0 * tnorm) + u * rB;
+  tau = (float)(2.0 * ev);
+  pad = (float)(2.0 * ev + 16.0 * u * (tnorm + rA + rB));
+}
+
+// Task word: slot(6) | side(1) | single(1) | X(28) | Y(28).
+//   single = 1: test the pair (X as A-node, Y as B-node).
+//   single = 0: X is the first of two sibling children; test (X+k, Y) when
+//   side = 0 (A split) or (Y, X+k) when side = 1 (B split), k = 0,1.
+// Leaf word: slot(6) | A-triangle(29) | B-triangle(29).
+__device__ inline uint64_t mk_task(int slot, int side, int single, int x, int y) {
+  return ((uint64_t)slot << 58) | ((uint64_t)side << 57) | ((uint64_t)single << 56) |
+         ((uint64_t)(uint32_t)x << 28) | (uint64_t)(uint32_t)y;
+}
+__device__ inline uint64_t mk_leaf(int slot, int ta, int tb) {
+  return ((uint64_t)slot << 58) | ((uint64_t)(uint32_t)ta << 29) | (uint64_t)(uint32_t)tb;
+}
+
+#define KG_SCAP 1024
+#define KG_LQ 256
+#define KG_NOTASK 0xffffffffffffffffull
+
+// One wavefront per workgroup.  Each lane follows one depth-first path (its
+// continuation, kept in registers); when a task yields two overlapping child
+// pairs, the lane continues with one and pushes the other onto the wave's
+// shared LIFO pool in LDS.  Lanes whose path ended pop the pool, and only when
+// the pool is empty do they start new pair items, so every item is explored
+// depth first while heavy items spread over idle lanes.  Leaf pairs are
+// queued and tested wave-coherently.  Items stream from a chunk of
+// configurations whose kill state sits in LDS.
 template <typename Real, bool EXACT>
 __global__ __launch_bounds__(64) void k_collide(TravArgs a) {
   __shared__ uint64_t stk[KG_SCAP];
+  __shared__ uint64_t lq[KG_LQ];
   __shared__ Real s_rel[12][64];
-  __shared__ int s_rootA[64], s_rootB[64], s_cfgk[64], s_rank[64], s_item[64], s_pend[64], s_unc[64];
+  __shared__ float s_tau[64], s_pad[64];
+  __shared__ int s_cfgk[64], s_rank[64], s_item[64], s_pend[64], s_unc[64];
   __shared__ unsigned long long s_best[KG_CPW_MAX][2];
   __shared__ int s_uncmin[KG_CPW_MAX][2];
   __shared__ int s_chunk;
 
   const int lane = threadIdx.x;
   const int mode = a.mode;
-  const Real tau = EXACT ? (Real)0 : (Real)a.tol.tau;
-  const Real pad_abs = (Real)a.tol.pad_bv, pad_rel = (Real)a.tol.pad_rel;
+  const Real pad_rel = (Real)a.tol.pad_rel;
   uint64_t* gsp_base = a.spill + (size_t)blockIdx.x * a.spill_cap;
-  const int ncfg = a.ncfg;
-  unsigned n_box = 0, n_tri = 0;
+  const int ncfg = a.ncfg_dev ? *a.ncfg_dev : a.ncfg;
+  unsigned n_box = 0, n_tri = 0, n_items = 0;
 
   for (;;) {
     if (lane == 0) s_chunk = atomicAdd(a.chunk_counter, 1);
@@ -532,14 +598,20 @@ __global__ __launch_bounds__(64) void k_collide(TravArgs a) {
     __syncthreads();
 
     auto cfg_of = [&](int k) { return a.cfg_list ? a.cfg_list[k0 + k] : k0 + k; };
-    int kc = 0;
-    int it = a.offsets[cfg_of(0)], itend = a.offsets[cfg_of(0) + 1];
-    int sp = 0, gsp = 0;
+    auto range = [&](int c, int& b, int& e) {
+      if (a.offsets) {
+        b = a.offsets[c];
+        e = a.offsets[c + 1];
+      } else {
+        b = c * a.cap;
+        e = b + a.counts[c];
+      }
+    };
+    int kc = 0, it, itend;
+    range(cfg_of(0), it, itend);
+    int sp = 0, gsp = 0, nl = 0;
+    uint64_t cont = KG_NOTASK;  // this lane's depth-first continuation
 
-    // Make room for `room` more stack entries: move the bottom half of the
-    // LDS stack to this wave's HBM spill region (LIFO order is preserved:
-    // the HBM region holds the oldest entries).  If that region is full, the
-    // bottom half is dropped and its items marked uncertain.  Wave-uniform.
     auto ensure_room = [&](int room) {
       if (sp + room <= KG_SCAP) return;
       const int h = KG_SCAP / 2;
@@ -564,35 +636,104 @@ __global__ __launch_bounds__(64) void k_collide(TravArgs a) {
       __syncthreads();
     };
 
-    for (;;) {
-      // ---- refill free slots with the next items of the chunk
-      const bool isfree = s_pend[lane] == 0;
-      const uint64_t fm = ballot64(isfree);
-      const int need = __popcll(fm);
-      const int myp = lane_prefix(fm);
-      int myitem = -1, mycfgk = -1;
-      int assigned = 0;
-      while (assigned < need && kc < nk) {
-        if (it >= itend) {
-          kc++;
-          if (kc < nk) {
-            int cc = cfg_of(kc);
-            it = a.offsets[cc];
-            itend = a.offsets[cc + 1];
-          }
-          continue;
-        }
-        int take = min(need - assigned, itend - it);
-        if (isfree && myp >= assigned && myp < assigned + take) {
-          myitem = it + (myp - assigned);
-          mycfgk = kc;
-        }
-        assigned += take;
-        it += take;
+    auto settle = [&](int slot, int delta) {
+      const int old = atomicAdd(&s_pend[slot], delta);
+      if (old + delta == 0) {
+        const int rk = s_rank[slot];
+        const int g = (rk >> KG_RANK_GROUP_BIT) & 1, r = rk & ((1 << KG_RANK_GROUP_BIT) - 1);
+        if (s_unc[slot] && !item_dead(mode, s_best[s_cfgk[slot]], g, r)) atomicMin(&s_uncmin[s_cfgk[slot]][g], r);
       }
-      const bool got = myitem >= 0;
-      ensure_room(64);
-      if (got) {
+    };
+
+    auto run_leaves = [&]() {
+      const int m = min(nl, 64);
+      if (lane < m) {
+        const uint64_t e = lq[nl - 1 - lane];
+        const int slot = (int)(e >> 58);
+        const int ta = (int)((e >> 29) & 0x1fffffff), tb = (int)(e & 0x1fffffff);
+        const int rk = s_rank[slot];
+        const int g = (rk >> KG_RANK_GROUP_BIT) & 1, r = rk & ((1 << KG_RANK_GROUP_BIT) - 1);
+        const int ck = s_cfgk[slot];
+        if (!item_dead(mode, s_best[ck], g, r)) {
+          n_tri++;
+          Real rel[12];
+#pragma unroll
+          for (int k = 0; k < 12; k++) rel[k] = s_rel[k][slot];
+          V3T<Real> tp[3], tq[3];
+          load_tri<Real>(a.pool, ta, tp);
+          load_tri<Real>(a.pool, tb, tq);
+#pragma unroll
+          for (int v = 0; v < 3; v++) tp[v] = xf_apply<Real>(rel, tp[v]);
+          const int res = tri_tri_sat<Real, EXACT>(tp, tq, (Real)s_tau[slot]);
+          if (res == TT_HIT) {
+            const unsigned long long key = ((unsigned long long)(uint32_t)r << 32) | (uint32_t)s_item[slot];
+            atomicMin(&s_best[ck][g], key);
+          } else if (res == TT_UNSURE) {
+            s_unc[slot] = 1;
+          }
+        }
+        settle(slot, -1);
+      }
+      nl -= m;
+      __syncthreads();
+    };
+
+    for (;;) {
+      // ---- lanes without a continuation pop the pool (bring spilled tasks
+      // back first when it runs low)
+      const bool hungry = cont == KG_NOTASK;
+      const uint64_t hm = ballot64(hungry);
+      const int nh = __popcll(hm);
+      if (nh > sp && gsp > 0) {
+        const int k = min(gsp, KG_SCAP / 2);
+        for (int i = sp - 1 - lane; i >= 0; i -= 64) stk[i + k] = stk[i];
+        __syncthreads();
+        for (int i = lane; i < k; i += 64) stk[i] = gsp_base[gsp - k + i];
+        gsp -= k;
+        sp += k;
+        __syncthreads();
+      }
+      const int hp = lane_prefix(hm);
+      const int npop = min(nh, sp);
+      if (hungry && hp < npop) cont = stk[sp - 1 - hp];
+      sp -= npop;
+      // ---- lanes still without work start new items on free slots
+      const bool want = cont == KG_NOTASK;
+      const uint64_t wm = ballot64(want);
+      const uint64_t fm = ballot64(s_pend[lane] == 0);
+      int navail = min(__popcll(wm), __popcll(fm));
+      int myitem = -1, mycfgk = -1;
+      {
+        // the j-th wanting lane takes the j-th free slot and the j-th item
+        const int wp = lane_prefix(wm);
+        int assigned = 0;
+        while (assigned < navail && kc < nk) {
+          if (it >= itend) {
+            kc++;
+            if (kc < nk) range(cfg_of(kc), it, itend);
+            continue;
+          }
+          const int take = min(navail - assigned, itend - it);
+          if (want && wp >= assigned && wp < assigned + take) {
+            myitem = it + (wp - assigned);
+            mycfgk = kc;
+          }
+          assigned += take;
+          it += take;
+        }
+        navail = assigned;
+      }
+      // slot for each new item: the j-th free slot (by lane index)
+      int myslot = -1;
+      if (myitem >= 0) {
+        const int j = lane_prefix(wm);  // rank among wanting lanes (< navail)
+        uint64_t f = fm;
+        for (int q = 0; q < j; q++) f &= f - 1;
+        myslot = __ffsll((unsigned long long)f) - 1;
+      }
+      __syncthreads();
+      if (myitem >= 0) {
+        n_items++;
         const PairItem P = a.items[myitem];
         const LinkDesc& LA = a.plan.links[P.a];
         const int g = (P.rank >> KG_RANK_GROUP_BIT) & 1;
@@ -606,134 +747,121 @@ __global__ __launch_bounds__(64) void k_collide(TravArgs a) {
           XB = a.T64 + ((size_t)P.cfg * a.plan.nlinks + P.b) * 12;
           geomB = a.plan.links[P.b].geom;
         }
-        // rel = XB^-1 XA : A-local -> B-local
-        double d0 = XA[9] - XB[9], d1 = XA[10] - XB[10], d2 = XA[11] - XB[11];
+        const double d0 = XA[9] - XB[9], d1 = XA[10] - XB[10], d2 = XA[11] - XB[11];
 #pragma unroll
         for (int col = 0; col < 3; col++)
 #pragma unroll
-          for (int r = 0; r < 3; r++) {
-            double v = XB[r * 3] * XA[col * 3] + XB[r * 3 + 1] * XA[col * 3 + 1] + XB[r * 3 + 2] * XA[col * 3 + 2];
-            s_rel[col * 3 + r][lane] = (Real)v;
-          }
+          for (int r = 0; r < 3; r++)
+            s_rel[col * 3 + r][myslot] =
+                (Real)(XB[r * 3] * XA[col * 3] + XB[r * 3 + 1] * XA[col * 3 + 1] + XB[r * 3 + 2] * XA[col * 3 + 2]);
+        double tn = 0;
 #pragma unroll
-        for (int r = 0; r < 3; r++) s_rel[9 + r][lane] = (Real)(XB[r * 3] * d0 + XB[r * 3 + 1] * d1 + XB[r * 3 + 2] * d2);
-        s_rootA[lane] = a.pool.geoms[LA.geom].root;
-        s_rootB[lane] = a.pool.geoms[geomB].root;
-        s_cfgk[lane] = mycfgk;
-        s_rank[lane] = P.rank;
-        s_item[lane] = myitem;
-        s_pend[lane] = 1;
-        s_unc[lane] = 0;
-      }
-      {
-        const uint64_t gm = ballot64(got);
-        const int tot = __popcll(gm);
-        if (got) {
-          uint64_t task = ((uint64_t)lane << 58) | ((uint64_t)(uint32_t)s_rootA[lane] << 32) | (uint32_t)s_rootB[lane];
-          stk[sp + lane_prefix(gm)] = task;
+        for (int r = 0; r < 3; r++) {
+          const double v = XB[r * 3] * d0 + XB[r * 3 + 1] * d1 + XB[r * 3 + 2] * d2;
+          s_rel[9 + r][myslot] = (Real)v;
+          tn += v * v;
         }
-        sp += tot;
+        const GeomDesc& GA = a.pool.geoms[LA.geom];
+        const GeomDesc& GB = a.pool.geoms[geomB];
+        float tau, pad;
+        slot_bounds<Real, EXACT>(a, sqrt(tn), GA.rmax, GB.rmax, tau, pad);
+        s_tau[myslot] = tau;
+        s_pad[myslot] = pad;
+        s_cfgk[myslot] = mycfgk;
+        s_rank[myslot] = P.rank;
+        s_item[myslot] = myitem;
+        s_pend[myslot] = 1;
+        s_unc[myslot] = 0;
+        cont = mk_task(myslot, 0, 1, GA.root, GB.root);
       }
       __syncthreads();
 
-      // ---- bring spilled tasks back when the LDS stack runs low
-      if (sp < 64 && gsp > 0) {
-        const int k = min(gsp, KG_SCAP / 2);
-        for (int i = sp - 1 - lane; i >= 0; i -= 64) stk[i + k] = stk[i];
-        __syncthreads();
-        for (int i = lane; i < k; i += 64) stk[i] = gsp_base[gsp - k + i];
-        gsp -= k;
-        sp += k;
-        __syncthreads();
+      // ---- termination: no path, nothing pooled, no items left
+      const bool working = cont != KG_NOTASK;
+      if (!ballot64(working)) {
+        if (sp == 0 && gsp == 0) {
+          if (nl == 0) break;
+          run_leaves();
+        }
+        continue;
       }
 
-      // ---- termination: every outstanding task of a busy slot is on the
-      // stack (LDS or spilled), and the refill above gave every free slot an
-      // item while items remained, so an empty stack means the chunk is done
-      if (sp == 0) break;
-
-      // ---- pop
-      const int npop = min(sp, 64);
-      const bool has = lane < npop;
-      uint64_t task = has ? stk[sp - 1 - lane] : 0;
-      sp -= npop;
-
-      // ---- process
-      int nchild = 0;
-      uint64_t ch0 = 0, ch1 = 0;
-      if (has) {
-        const int slot = (int)(task >> 58);
-        const int na = (int)((task >> 32) & 0x3ffffff);
-        const int nb = (int)(task & 0xffffffffu);
+      // ---- process the continuation: up to two box tests
+      uint64_t out0 = 0, out1 = 0, lf0 = 0, lf1 = 0;
+      int nout = 0, nlf = 0, slot = 0;
+      if (working) {
+        const uint64_t task = cont;
+        cont = KG_NOTASK;
+        slot = (int)(task >> 58);
+        const int side = (int)((task >> 57) & 1), single = (int)((task >> 56) & 1);
+        const int x = (int)((task >> 28) & 0xfffffff), y = (int)(task & 0xfffffff);
         const int rk = s_rank[slot];
-        const int g = (rk >> KG_RANK_GROUP_BIT) & 1;
-        const int r = rk & ((1 << KG_RANK_GROUP_BIT) - 1);
-        const int ck = s_cfgk[slot];
-        if (!item_dead(mode, s_best[ck], g, r)) {
-          float cA[3], eA[3], RA[9], cB[3], eB[3], RB[9];
-          int chA, chB;
-          load_node(a.pool.nodes, na, cA, eA, RA, chA);
-          load_node(a.pool.nodes, nb, cB, eB, RB, chB);
+        const int g = (rk >> KG_RANK_GROUP_BIT) & 1, r = rk & ((1 << KG_RANK_GROUP_BIT) - 1);
+        if (!item_dead(mode, s_best[s_cfgk[slot]], g, r)) {
+          const NodeF O = load_nodef(a.pool.nodes, y);
+          const NodeF S0 = load_nodef(a.pool.nodes, x);
+          const NodeF S1 = load_nodef(a.pool.nodes, single ? x : x + 1);
           Real rel[12];
 #pragma unroll
           for (int k = 0; k < 12; k++) rel[k] = s_rel[k][slot];
-          n_box++;
-          if (obb_overlap<Real>(cA, eA, RA, cB, eB, RB, rel, pad_abs, pad_rel)) {
-            const bool la = chA < 0, lb = chB < 0;
+          const Real pad = (Real)s_pad[slot];
+          const bool aIsS = single || side == 0;
+#pragma unroll 1
+          for (int k = 0; k < (single ? 1 : 2); k++) {
+            const NodeF S = sel(k == 0, S0, S1);
+            const NodeF A = sel(aIsS, S, O);
+            const NodeF B = sel(aIsS, O, S);
+            n_box++;
+            if (!obb_overlap<Real>(A.c, A.e, A.R, B.c, B.e, B.R, rel, pad, pad_rel)) continue;
+            const bool la = A.ch < 0, lb = B.ch < 0;
             if (la && lb) {
-              n_tri++;
-              V3T<Real> tp[3], tq[3];
-              load_tri<Real>(a.pool, ~chA, tp);
-              load_tri<Real>(a.pool, ~chB, tq);
-#pragma unroll
-              for (int v = 0; v < 3; v++) tp[v] = xf_apply<Real>(rel, tp[v]);
-              int res = tri_tri_sat<Real, EXACT>(tp, tq, tau);
-              if (res == TT_HIT) {
-                unsigned long long key = ((unsigned long long)(uint32_t)r << 32) | (uint32_t)s_item[slot];
-                atomicMin(&s_best[ck][g], key);
-              } else if (res == TT_UNSURE) {
-                s_unc[slot] = 1;
-              }
+              const uint64_t w = mk_leaf(slot, ~A.ch, ~B.ch);
+              if (nlf == 0) lf0 = w; else lf1 = w;
+              nlf++;
             } else {
               bool splitA;
               if (la) splitA = false;
               else if (lb) splitA = true;
-              else splitA = (eA[0] + eA[1] + eA[2]) >= (eB[0] + eB[1] + eB[2]);
-              const uint64_t hi = (uint64_t)slot << 58;
-              if (splitA) {
-                ch0 = hi | ((uint64_t)(uint32_t)chA << 32) | (uint32_t)nb;
-                ch1 = hi | ((uint64_t)(uint32_t)(chA + 1) << 32) | (uint32_t)nb;
-              } else {
-                ch0 = hi | ((uint64_t)(uint32_t)na << 32) | (uint32_t)chB;
-                ch1 = hi | ((uint64_t)(uint32_t)na << 32) | (uint32_t)(chB + 1);
-              }
-              nchild = 2;
+              else splitA = (A.e[0] + A.e[1] + A.e[2]) >= (B.e[0] + B.e[1] + B.e[2]);
+              const int nodeS = x + k;
+              const int nodeA = aIsS ? nodeS : y;
+              const int nodeB = aIsS ? y : nodeS;
+              const uint64_t w = splitA ? mk_task(slot, 0, 0, A.ch, nodeB) : mk_task(slot, 1, 0, B.ch, nodeA);
+              if (nout == 0) out0 = w; else out1 = w;
+              nout++;
             }
           }
         }
-        const int old = atomicAdd(&s_pend[slot], nchild - 1);
-        if (old + nchild - 1 == 0) {
-          // item complete: an undecided leaf test makes it uncertain unless a
-          // certain hit already settles its place
-          if (s_unc[slot] && !item_dead(mode, s_best[ck], g, r)) atomicMin(&s_uncmin[ck][g], r);
-        }
       }
+      // keep the first child pair as this lane's continuation
+      if (nout >= 1) cont = out0;
 
-      // ---- push (spill the bottom half to HBM first if needed)
-      const uint64_t b1 = ballot64(nchild >= 1), b2 = ballot64(nchild >= 2);
-      const int total = __popcll(b1) + __popcll(b2);
-      ensure_room(total);
+      // ---- queue leaves (flush batches first if the queue could overflow)
       {
-        const int pos = sp + lane_prefix(b1) + lane_prefix(b2);
-        if (nchild >= 1) stk[pos] = ch0;
-        if (nchild >= 2) stk[pos + 1] = ch1;
-        sp += total;
+        const uint64_t l1 = ballot64(nlf >= 1), l2 = ballot64(nlf >= 2);
+        const int ltot = __popcll(l1) + __popcll(l2);
+        if (nl + ltot > KG_LQ) {
+          __syncthreads();
+          while (nl + ltot > KG_LQ) run_leaves();
+        }
+        const int pos = nl + lane_prefix(l1) + lane_prefix(l2);
+        if (nlf >= 1) lq[pos] = lf0;
+        if (nlf >= 2) lq[pos + 1] = lf1;
+        nl += ltot;
       }
+      if (working) settle(slot, nout + nlf - 1);
+
+      // ---- push the second child pair to the pool
+      const uint64_t pm = ballot64(nout >= 2);
+      const int ptot = __popcll(pm);
+      ensure_room(ptot);
+      if (nout >= 2) stk[sp + lane_prefix(pm)] = out1;
+      sp += ptot;
       __syncthreads();
+      if (nl >= a.leaf_min) run_leaves();
     }
-    // chunk results
     for (int k = lane; k < nk; k += 64) {
-      int cc = cfg_of(k);
+      const int cc = cfg_of(k);
       a.best[2 * cc] = s_best[k][0];
       a.best[2 * cc + 1] = s_best[k][1];
       a.unc[2 * cc] = s_uncmin[k][0];
@@ -744,6 +872,7 @@ __global__ __launch_bounds__(64) void k_collide(TravArgs a) {
   if (a.counters) {
     atomicAdd(&a.counters[0], (unsigned long long)n_box);
     atomicAdd(&a.counters[1], (unsigned long long)n_tri);
+    atomicAdd(&a.counters[2], (unsigned long long)n_items);
   }
 }
 
@@ -803,7 +932,7 @@ hipError_t kgl_fk_f32(const LinkDesc* links, int nlinks, const float* Q, int N, 
 
 hipError_t kgl_prep_count(const PrepArgs& a, hipStream_t s) {
   if (a.N <= 0) return hipSuccess;
-  k_prep<false><<<(a.N + 255) / 256, 256, 0, s>>>(a);
+  k_prep<0><<<(a.N + 255) / 256, 256, 0, s>>>(a);
   k_fill_int<<<1, 1, 0, s>>>(a.counts + a.N, 1, 0);
   return hipGetLastError();
 }
@@ -819,13 +948,19 @@ hipError_t kgl_scan(int* counts, int* offsets, int n, DevBuf& tmp, hipStream_t s
 
 hipError_t kgl_prep_fill(const PrepArgs& a, hipStream_t s) {
   if (a.N <= 0) return hipSuccess;
-  k_prep<true><<<(a.N + 255) / 256, 256, 0, s>>>(a);
+  k_prep<1><<<(a.N + 255) / 256, 256, 0, s>>>(a);
+  return hipGetLastError();
+}
+
+hipError_t kgl_prep_single(const PrepArgs& a, hipStream_t s) {
+  const int n = a.N > a.nreset ? a.N : a.nreset;
+  if (n <= 0) return hipSuccess;
+  k_prep<2><<<(n + 255) / 256, 256, 0, s>>>(a);
   return hipGetLastError();
 }
 
 template <typename Real, bool EXACT>
 static hipError_t launch_collide(const TravArgs& a, hipStream_t s) {
-  k_fill_int<<<1, 1, 0, s>>>(a.chunk_counter, 1, 0);
   k_collide<Real, EXACT><<<a.grid, 64, 0, s>>>(a);
   return hipGetLastError();
 }
@@ -853,8 +988,9 @@ hipError_t kgl_fill_int(int* p, int n, int v, hipStream_t s) {
 
 int kgl_collide_grid(int exact, int num_cus) {
   int nb = 0;
-  hipError_t e = exact ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_collide<double, true>, 64, 0)
-                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_collide<float, false>, 64, 0);
+  const size_t lds = 0;
+  hipError_t e = exact ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_collide<double, true>, 64, lds)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_collide<float, false>, 64, lds);
   if (e != hipSuccess || nb < 1) nb = 8;
   const char* v = getenv("KG_WAVES_PER_CU");
   if (v && atoi(v) > 0) nb = atoi(v);
