@@ -92,9 +92,22 @@ class KlamptGPUError(RuntimeError):
         self.code = code
 
 
+def _share_torch_hip_runtime():
+    """PyTorch-ROCm wheels bundle their own libamdhip64 / libhsa-runtime64
+    with the same SONAMEs as /opt/rocm.  Whichever is loaded first serves the
+    whole process; two copies cannot share a device.  Loading torch first
+    makes this library bind to torch's runtime, so tensors and kernels live
+    in one HIP runtime.  Without torch, /opt/rocm's runtime is used."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def lib():
     global _lib
     if _lib is None:
+        _share_torch_hip_runtime()
         if not os.path.exists(LIB_PATH):
             raise ImportError(
                 f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "

@@ -15,6 +15,7 @@
 
 #include "../../include/klampt_gpu.h"
 #include "kg_internal.h"
+#include "kg_geom_math.h"
 
 hipError_t kgl_finalize_list(const FinArgs& a, const int* list, const int* count, int maxn, hipStream_t s);
 hipError_t kgl_fill_int(int* p, int n, int v, hipStream_t s);
@@ -343,6 +344,9 @@ RobotPlan& get_plan(kg_world_s* w, int ri) {
   upload(P.denv, P.env);
   upload(P.dlemask, P.lemask);
   upload(P.dself, P.selfpairs);
+  upload(P.dpa, P.pair_a);
+  upload(P.dpb, P.pair_b);
+  upload(P.dpk, P.pair_kind);
   P.dev.links = P.dlinks.as<LinkDesc>();
   P.dev.qmin = P.dqmin.as<double>();
   P.dev.qmax = P.dqmax.as<double>();
@@ -899,7 +903,7 @@ int kg_world_destroy(kg_world w) {
   return guard([&] {
     if (!w) return;
     for (auto& p : w->plans) {
-      DevBuf* bufs[] = {&p.dlinks, &p.dqmin, &p.dqmax, &p.denv, &p.dlemask, &p.dself};
+      DevBuf* bufs[] = {&p.dlinks, &p.dqmin, &p.dqmax, &p.denv, &p.dlemask, &p.dself, &p.dpa, &p.dpb, &p.dpk};
       for (auto* b : bufs) b->release();
     }
     for (auto it = g_overrides.begin(); it != g_overrides.end(); ++it)
@@ -1171,26 +1175,485 @@ extern "C" int kg_geom_collide(kg_geom a, kg_geom b, int* out) {
   });
 }
 
-#define KG_TODO(sig, what)                                                      \
-  extern "C" int sig {                                                          \
-    return guard([&] { fail(KG_E_UNSUPPORTED, "%s: not available yet", what); }); \
+// ---------------------------------------------------------------- SDF
+namespace {
+
+SdfDev sdf_dev(kg_geom_s* g) {
+  need(g && g->kind == 1, "geometry is not an ImplicitSurface");
+  SdfDev d{};
+  d.v = g->dvals.as<float>();
+  for (int k = 0; k < 3; k++) {
+    d.dims[k] = g->dims[k];
+    const double h = (g->bmax[k] - g->bmin[k]) / g->dims[k];
+    d.bmin[k] = (float)g->bmin[k];
+    d.h[k] = (float)h;
+    d.inv_h[k] = (float)(1.0 / h);
+    d.bmin64[k] = g->bmin[k];
+    d.inv_h64[k] = 1.0 / h;
   }
-KG_TODO(kg_geom_within_distance(kg_geom, kg_geom, double, int*), "kg_geom_within_distance")
-KG_TODO(kg_geom_distance(kg_geom, kg_geom, const kg_dist_settings*, kg_dist_result*), "kg_geom_distance")
-KG_TODO(kg_geom_distance_point(kg_geom, const double*, const kg_dist_settings*, kg_dist_result*),
-        "kg_geom_distance_point")
-KG_TODO(kg_sdf_eval(kg_geom, const float*, int, float*, float*), "kg_sdf_eval")
-KG_TODO(kg_distance_batch(kg_world, int, const float*, int, float*), "kg_distance_batch")
-KG_TODO(kg_distance_batch_dev(kg_world, int, const float*, int, float*, void*), "kg_distance_batch_dev")
-KG_TODO(kg_edge_batch(kg_world, int, const float*, const float*, int, double, int, float*), "kg_edge_batch")
-KG_TODO(kg_sdf_penalty_fwd(kg_robot, kg_geom, const float*, int, const float*, const int32_t*, float, float*),
-        "kg_sdf_penalty_fwd")
-KG_TODO(kg_sdf_penalty_bwd(kg_robot, kg_geom, const float*, int, const float*, const int32_t*, float, const float*,
-                           float*),
-        "kg_sdf_penalty_bwd")
-KG_TODO(kg_sdf_penalty_fwd_dev(kg_robot, kg_geom, const float*, int, const float*, const int32_t*, float, float*,
-                               void*),
-        "kg_sdf_penalty_fwd_dev")
-KG_TODO(kg_sdf_penalty_bwd_dev(kg_robot, kg_geom, const float*, int, const float*, const int32_t*, float,
-                               const float*, float*, void*),
-        "kg_sdf_penalty_bwd_dev")
+  // Xinv = X^-1
+  const Xf64& X = g->X;
+  bool ident = true;
+  for (int i = 0; i < 9; i++) ident = ident && X.R[i] == ((i % 4 == 0) ? 1.0 : 0.0);
+  for (int i = 0; i < 3; i++) ident = ident && X.t[i] == 0.0;
+  d.ident = ident ? 1 : 0;
+  for (int c = 0; c < 3; c++)
+    for (int r = 0; r < 3; r++) d.Xinv[c * 3 + r] = (float)X.R[r * 3 + c];
+  for (int r = 0; r < 3; r++)
+    d.Xinv[9 + r] = (float)(-(X.R[r * 3] * X.t[0] + X.R[r * 3 + 1] * X.t[1] + X.R[r * 3 + 2] * X.t[2]));
+  return d;
+}
+
+void robot_links_dev(kg_robot_s* r) {
+  if (r->dlinks.bytes == 0 || r->dirty) {
+    upload(r->dlinks, r->links);
+    r->dirty = false;
+  }
+}
+
+// Penalty on device buffers; pts_per_link is a host array.
+void sdf_penalty_dev(kg_robot_s* r, kg_geom_s* sdf, const float* dQ, int N, const float* dpts, const int32_t* ppl,
+                     float dsafe, float* dloss, const float* ddl, float* ddQ, bool bwd, hipStream_t s) {
+  need(r != nullptr, "robot is NULL");
+  need(sdf != nullptr && sdf->kind == 1, "sdf must be an ImplicitSurface geometry");
+  need(ppl != nullptr, "pts_per_link is NULL");
+  need(N >= 0, "negative batch size");
+  if (N == 0) return;
+  need(dQ && dpts, "NULL buffer");
+  need(bwd ? ddQ != nullptr : dloss != nullptr, "NULL output");
+  kg_ctx_s* c = r->ctx;
+  hip_check(hipSetDevice(c->device), "hipSetDevice");
+  const int n = r->n;
+  std::vector<int> off(n + 1, 0);
+  for (int k = 0; k < n; k++) {
+    need(ppl[k] >= 0, "negative point count");
+    off[k + 1] = off[k] + ppl[k];
+  }
+  robot_links_dev(r);
+  hip_check(c->wsPoff.ensure((n + 1) * sizeof(int)), "hipMalloc");
+  hip_check(hipMemcpyAsync(c->wsPoff.p, off.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice, s), "copy offsets");
+  hip_check(c->wsSdfScratch.ensure((size_t)N * 2 * n * 12 * sizeof(float)), "hipMalloc(scratch)");
+  SdfDev g = sdf_dev(sdf);
+  hip_check(kgl_sdf_penalty(r->dlinks.as<LinkDesc>(), n, g, dQ, N, dpts, c->wsPoff.as<int>(), dsafe, dloss, ddl, ddQ,
+                            c->wsSdfScratch.as<float>(), bwd, s),
+            "sdf_penalty");
+}
+
+int total_points(const kg_robot_s* r, const int32_t* ppl) {
+  int t = 0;
+  for (int k = 0; k < r->n; k++) t += ppl[k];
+  return t;
+}
+
+}  // namespace
+
+extern "C" int kg_sdf_eval(kg_geom sdf, const float* pts, int n, float* d, float* grad) {
+  return guard([&] {
+    need(sdf && sdf->kind == 1, "geometry is not an ImplicitSurface");
+    need(n >= 0, "negative size");
+    if (n == 0) return;
+    need(pts && d, "NULL buffer");
+    kg_ctx_s* c = sdf->ctx;
+    hip_check(hipSetDevice(c->device), "hipSetDevice");
+    hip_check(c->wsPts.ensure((size_t)n * 3 * sizeof(float)), "hipMalloc");
+    hip_check(c->wsLoss.ensure((size_t)n * sizeof(float)), "hipMalloc");
+    hip_check(c->wsDQ.ensure((size_t)n * 3 * sizeof(float)), "hipMalloc");
+    hip_check(hipMemcpyAsync(c->wsPts.p, pts, (size_t)n * 3 * sizeof(float), hipMemcpyHostToDevice, c->stream), "copy");
+    hip_check(kgl_sdf_eval(sdf_dev(sdf), c->wsPts.as<float>(), n, c->wsLoss.as<float>(), grad ? c->wsDQ.as<float>() : nullptr,
+                           c->stream),
+              "sdf_eval");
+    hip_check(hipMemcpyAsync(d, c->wsLoss.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, c->stream), "copy");
+    if (grad)
+      hip_check(hipMemcpyAsync(grad, c->wsDQ.p, (size_t)n * 3 * sizeof(float), hipMemcpyDeviceToHost, c->stream), "copy");
+    hip_check(hipStreamSynchronize(c->stream), "sync");
+  });
+}
+
+extern "C" int kg_geom_distance_point(kg_geom g, const double pt[3], const kg_dist_settings* settings,
+                                      kg_dist_result* out) {
+  return guard([&] {
+    need(g && pt && out, "NULL argument");
+    (void)settings;
+    if (g->kind != 1) fail(KG_E_UNSUPPORTED, "distance_point is implemented for ImplicitSurface geometries");
+    float p[3] = {(float)pt[0], (float)pt[1], (float)pt[2]}, d = 0, gr[3] = {0, 0, 0};
+    int rc = kg_sdf_eval(g, p, 1, &d, gr);
+    if (rc != KG_OK) fail(rc, "%s", g_err.c_str());
+    std::memset(out, 0, sizeof(*out));
+    out->d = d;
+    // closest point on the zero level set along the gradient; gradients as
+    // in Python/klampt/src/robotsim.cpp:1768-1773 (grad2 = -grad1)
+    out->hasClosestPoints = 1;
+    out->hasGradients = 1;
+    for (int k = 0; k < 3; k++) {
+      out->grad1[k] = gr[k];
+      out->grad2[k] = -gr[k];
+      out->cp1[k] = pt[k] - d * gr[k];
+      out->cp2[k] = pt[k];
+    }
+    out->elem1 = out->elem2 = -1;
+  });
+}
+
+extern "C" int kg_sdf_penalty_fwd_dev(kg_robot r, kg_geom sdf, const float* dQ, int N, const float* dpts,
+                                      const int32_t* ppl, float dsafe, float* dloss, void* stream) {
+  return guard([&] {
+    need(r != nullptr, "robot is NULL");
+    hipStream_t s = stream ? (hipStream_t)stream : r->ctx->stream;
+    sdf_penalty_dev(r, sdf, dQ, N, dpts, ppl, dsafe, dloss, nullptr, nullptr, false, s);
+  });
+}
+
+extern "C" int kg_sdf_penalty_bwd_dev(kg_robot r, kg_geom sdf, const float* dQ, int N, const float* dpts,
+                                      const int32_t* ppl, float dsafe, const float* ddl, float* ddQ, void* stream) {
+  return guard([&] {
+    need(r != nullptr, "robot is NULL");
+    hipStream_t s = stream ? (hipStream_t)stream : r->ctx->stream;
+    sdf_penalty_dev(r, sdf, dQ, N, dpts, ppl, dsafe, nullptr, ddl, ddQ, true, s);
+  });
+}
+
+static void sdf_penalty_host(kg_robot_s* r, kg_geom_s* sdf, const float* Q, int N, const float* pts, const int32_t* ppl,
+                             float dsafe, float* loss, const float* dl, float* dQ, bool bwd) {
+  need(r && sdf && ppl, "NULL argument");
+  need(N >= 0, "negative batch size");
+  if (N == 0) return;
+  need(Q && pts, "NULL buffer");
+  kg_ctx_s* c = r->ctx;
+  hip_check(hipSetDevice(c->device), "hipSetDevice");
+  const int n = r->n, P = total_points(r, ppl);
+  hip_check(c->wsQ.ensure((size_t)N * n * sizeof(float)), "hipMalloc");
+  hip_check(c->wsPts.ensure((size_t)std::max(1, P) * 3 * sizeof(float)), "hipMalloc");
+  hip_check(c->wsLoss.ensure((size_t)N * sizeof(float)), "hipMalloc");
+  hip_check(c->wsDQ.ensure((size_t)N * n * sizeof(float)), "hipMalloc");
+  hip_check(c->wsDl.ensure((size_t)N * sizeof(float)), "hipMalloc");
+  hip_check(hipMemcpyAsync(c->wsQ.p, Q, (size_t)N * n * sizeof(float), hipMemcpyHostToDevice, c->stream), "copy");
+  if (P > 0) hip_check(hipMemcpyAsync(c->wsPts.p, pts, (size_t)P * 3 * sizeof(float), hipMemcpyHostToDevice, c->stream), "copy");
+  if (bwd && dl) hip_check(hipMemcpyAsync(c->wsDl.p, dl, (size_t)N * sizeof(float), hipMemcpyHostToDevice, c->stream), "copy");
+  sdf_penalty_dev(r, sdf, c->wsQ.as<float>(), N, c->wsPts.as<float>(), ppl, dsafe, bwd ? nullptr : c->wsLoss.as<float>(),
+                  (bwd && dl) ? c->wsDl.as<float>() : nullptr, bwd ? c->wsDQ.as<float>() : nullptr, bwd, c->stream);
+  if (bwd)
+    hip_check(hipMemcpyAsync(dQ, c->wsDQ.p, (size_t)N * n * sizeof(float), hipMemcpyDeviceToHost, c->stream), "copy");
+  else
+    hip_check(hipMemcpyAsync(loss, c->wsLoss.p, (size_t)N * sizeof(float), hipMemcpyDeviceToHost, c->stream), "copy");
+  hip_check(hipStreamSynchronize(c->stream), "sync");
+}
+
+extern "C" int kg_sdf_penalty_fwd(kg_robot r, kg_geom sdf, const float* Q, int N, const float* pts,
+                                  const int32_t* ppl, float dsafe, float* loss) {
+  return guard([&] {
+    need(loss != nullptr || N == 0, "loss is NULL");
+    sdf_penalty_host(r, sdf, Q, N, pts, ppl, dsafe, loss, nullptr, nullptr, false);
+  });
+}
+
+extern "C" int kg_sdf_penalty_bwd(kg_robot r, kg_geom sdf, const float* Q, int N, const float* pts,
+                                  const int32_t* ppl, float dsafe, const float* dl, float* dQ) {
+  return guard([&] {
+    need(dQ != nullptr || N == 0, "dQ is NULL");
+    sdf_penalty_host(r, sdf, Q, N, pts, ppl, dsafe, nullptr, dl, dQ, true);
+  });
+}
+
+// ---------------------------------------------------------------- distance
+namespace {
+
+// Per-pair minimum distances for robot `ri` on device buffers (async).
+void distance_dev(kg_world_s* w, int ri, const float* Qf, const double* Qd, int N, float* out, int* out_tri,
+                  float stop_below, float upper, hipStream_t s) {
+  kg_ctx_s* c = w->ctx;
+  hip_check(hipSetDevice(c->device), "hipSetDevice");
+  RobotPlan& P = get_plan(w, ri);
+  const int npairs = (int)P.pairs.size();
+  if (N <= 0 || npairs == 0) return;
+  const int n = P.dev.nlinks;
+  hip_check(c->wsT64.ensure((size_t)N * n * 12 * sizeof(double)), "hipMalloc(T64)");
+  hip_check(c->wsStatus.ensure(ST_WORDS * sizeof(int)), "hipMalloc(status)");
+  int* status = c->wsStatus.as<int>();
+  hip_check(kgl_fk64(P.dev.links, n, Qf, Qd, N, c->wsT64.as<double>(), status, ST_WORDS, s), "fk64");
+  const int grid = kgl_distance_grid(c->num_cus);
+  const int spill_cap = env_int("KG_SPILL_CAP", 16384);
+  hip_check(c->wsSpill.ensure((size_t)grid * spill_cap * sizeof(uint64_t)), "hipMalloc(spill)");
+  DistArgs da{};
+  da.pool = c->pool;
+  da.plan = P.dev;
+  da.T64 = c->wsT64.as<double>();
+  da.N = N;
+  da.npairs = npairs;
+  da.pair_a = P.dpa.as<int>();
+  da.pair_b = P.dpb.as<int>();
+  da.pair_kind = P.dpk.as<int>();
+  da.stop_below = stop_below;
+  da.upper = upper;
+  da.out = out;
+  da.out_tri = out_tri;
+  da.chunk_counter = status + ST_CHUNK32;
+  da.spill = c->wsSpill.as<uint64_t>();
+  da.spill_cap = spill_cap;
+  da.status = status + ST_SPILL;
+  da.grid = grid;
+  da.ipw = std::max(1, env_int("KG_DIST_IPW", 64));
+  da.counters = (unsigned long long*)(status + ST_CNT);
+  hip_check(hipEventRecord(c->ev[2], s), "hipEventRecord");
+  hip_check(kgl_distance(da, s), "distance");
+  hip_check(hipEventRecord(c->ev[3], s), "hipEventRecord");
+  c->last_N = N;
+  c->last_stream = s;
+  hip_check(hipMemcpyAsync(c->hstatus, status, ST_WORDS * sizeof(int), hipMemcpyDeviceToHost, s), "copy status");
+}
+
+void ensure_status_host(kg_ctx_s* c) {
+  if (!c->hstatus) {
+    hip_check(hipHostMalloc((void**)&c->hstatus, ST_WORDS * sizeof(int), hipHostMallocDefault), "hipHostMalloc");
+    for (auto& e : c->ev) hip_check(hipEventCreate(&e), "hipEventCreate");
+  }
+}
+
+// FP64 distance and closest points of one triangle pair (original indices)
+double pair_tri_dist(const kg_geom_s* a, int ta, const kg_geom_s* b, int tb, double* cp1, double* cp2) {
+  kgm::V<double> P[3], Q[3];
+  for (int v = 0; v < 3; v++) {
+    const double* pa = &a->V[3 * a->T[3 * ta + v]];
+    const double* pb = &b->V[3 * b->T[3 * tb + v]];
+    double wa[3], wb[3];
+    for (int r = 0; r < 3; r++) {
+      wa[r] = a->X.R[r] * pa[0] + a->X.R[3 + r] * pa[1] + a->X.R[6 + r] * pa[2] + a->X.t[r];
+      wb[r] = b->X.R[r] * pb[0] + b->X.R[3 + r] * pb[1] + b->X.R[6 + r] * pb[2] + b->X.t[r];
+    }
+    P[v] = kgm::mk(wa[0], wa[1], wa[2]);
+    Q[v] = kgm::mk(wb[0], wb[1], wb[2]);
+  }
+  kgm::V<double> c1, c2;
+  const double d = kgm::tri_dist<double>(P, Q, c1, c2);
+  cp1[0] = c1.x; cp1[1] = c1.y; cp1[2] = c1.z;
+  cp2[0] = c2.x; cp2[1] = c2.y; cp2[2] = c2.z;
+  return d;
+}
+
+// Single pair: distance with closest points via the pair world.
+void pair_distance(kg_geom_s* a, kg_geom_s* b, float stop_below, double upper, kg_dist_result* out, float* dgpu) {
+  kg_ctx_s* c = a->ctx;
+  ensure_status_host(c);
+  PairWorld& pw = pair_world(c, a, b);
+  hip_check(hipSetDevice(c->device), "hipSetDevice");
+  hip_check(c->wsG.ensure(64), "hipMalloc");
+  hip_check(c->wsOut32.ensure(64), "hipMalloc");
+  double q0 = 0.0;
+  hip_check(hipMemcpyAsync(c->wsG.p, &q0, sizeof(double), hipMemcpyHostToDevice, c->stream), "copy q");
+  float* dout = (float*)c->wsOut32.p;
+  int* dtri = (int*)((char*)c->wsOut32.p + 16);
+  const float up = std::isfinite(upper) ? (float)upper : 3.0e38f;
+  distance_dev(pw.w, 0, nullptr, c->wsG.as<double>(), 1, dout, dtri, stop_below, up, c->stream);
+  float d = 0;
+  int tri[2] = {-1, -1};
+  hip_check(hipMemcpyAsync(&d, dout, sizeof(float), hipMemcpyDeviceToHost, c->stream), "copy");
+  hip_check(hipMemcpyAsync(tri, dtri, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream), "copy");
+  hip_check(hipStreamSynchronize(c->stream), "sync");
+  if (c->hstatus[ST_SPILL]) fail(KG_E_LIMIT, "distance traversal exceeded its stack capacity (KG_SPILL_CAP)");
+  if (dgpu) *dgpu = d;
+  std::memset(out, 0, sizeof(*out));
+  out->d = d;
+  out->elem1 = out->elem2 = -1;
+  if (tri[0] >= 0 && tri[1] >= 0) {
+    const int ta = c->trisrc[tri[0]], tb = c->trisrc[tri[1]];
+    double cp1[3], cp2[3];
+    const double d64 = pair_tri_dist(a, ta, b, tb, cp1, cp2);
+    out->d = d64;
+    out->hasClosestPoints = 1;
+    out->elem1 = ta;
+    out->elem2 = tb;
+    for (int k = 0; k < 3; k++) {
+      out->cp1[k] = cp1[k];
+      out->cp2[k] = cp2[k];
+    }
+    if (d64 > 0) {
+      // grad1 points from the other object toward this one
+      // (Python/klampt/src/geometry.h:650-662: move object1 along grad1 by -d to touch)
+      out->hasGradients = 1;
+      for (int k = 0; k < 3; k++) {
+        out->grad1[k] = (cp1[k] - cp2[k]) / d64;
+        out->grad2[k] = -out->grad1[k];
+      }
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" int kg_distance_batch_dev(kg_world w, int ri, const float* dQ, int N, float* dout, void* stream) {
+  return guard([&] {
+    need(w != nullptr, "world is NULL");
+    need(N >= 0, "negative batch size");
+    need(N == 0 || (dQ && dout), "NULL buffer");
+    ensure_status_host(w->ctx);
+    hipStream_t s = stream ? (hipStream_t)stream : w->ctx->stream;
+    distance_dev(w, ri, dQ, nullptr, N, dout, nullptr, 0.0f, 3.0e38f, s);
+  });
+}
+
+extern "C" int kg_distance_batch(kg_world w, int ri, const float* Q, int N, float* out) {
+  return guard([&] {
+    need(w != nullptr, "world is NULL");
+    need(N >= 0, "negative batch size");
+    need(ri >= 0 && ri < (int)w->robots.size(), "robot index out of range");
+    if (N == 0) return;
+    need(Q && out, "NULL buffer");
+    kg_ctx_s* c = w->ctx;
+    ensure_status_host(c);
+    hip_check(hipSetDevice(c->device), "hipSetDevice");
+    RobotPlan& P = get_plan(w, ri);
+    const int n = w->robots[ri]->n, np = (int)P.pairs.size();
+    if (np == 0) return;
+    hip_check(c->wsQ.ensure((size_t)N * n * sizeof(float)), "hipMalloc");
+    hip_check(c->wsF.ensure((size_t)N * np * sizeof(float)), "hipMalloc");
+    hip_check(hipMemcpyAsync(c->wsQ.p, Q, (size_t)N * n * sizeof(float), hipMemcpyHostToDevice, c->stream), "copy");
+    distance_dev(w, ri, c->wsQ.as<float>(), nullptr, N, c->wsF.as<float>(), nullptr, 0.0f, 3.0e38f, c->stream);
+    hip_check(hipMemcpyAsync(out, c->wsF.p, (size_t)N * np * sizeof(float), hipMemcpyDeviceToHost, c->stream), "copy");
+    hip_check(hipStreamSynchronize(c->stream), "sync");
+    if (c->hstatus[ST_SPILL]) fail(KG_E_LIMIT, "distance traversal exceeded its stack capacity (KG_SPILL_CAP)");
+  });
+}
+
+extern "C" int kg_geom_distance(kg_geom a, kg_geom b, const kg_dist_settings* settings, kg_dist_result* out) {
+  return guard([&] {
+    need(a && b && out, "NULL argument");
+    need(a->ctx == b->ctx, "geometries belong to different contexts");
+    if (a->kind != 0 || b->kind != 0) fail(KG_E_UNSUPPORTED, "distance is implemented for TriangleMesh pairs");
+    if (geom_empty(a)) fail(KG_E_EMPTY, "Geometry3D.distance: Geometry is empty");
+    if (geom_empty(b)) fail(KG_E_EMPTY, "Geometry3D.distance: Other geometry is empty");
+    const double upper = settings ? settings->upperBound : INFINITY;
+    pair_distance(a, b, 0.0f, upper, out, nullptr);
+  });
+}
+
+extern "C" int kg_geom_within_distance(kg_geom a, kg_geom b, double tol, int* out) {
+  return guard([&] {
+    need(a && b && out, "NULL argument");
+    need(tol >= 0, "tolerance must be >= 0");
+    *out = 0;
+    if (a->kind != 0 || b->kind != 0) fail(KG_E_UNSUPPORTED, "withinDistance is implemented for TriangleMesh pairs");
+    if (geom_empty(a) || geom_empty(b)) return;
+    kg_dist_result r;
+    float dg = 0;
+    pair_distance(a, b, (float)tol, INFINITY, &r, &dg);
+    *out = r.d <= tol ? 1 : 0;
+  });
+}
+
+// ---------------------------------------------------------------- edge checks
+namespace {
+
+// Level-order bisection over the interior sample indices 1..n-1 of [0,n].
+void bisection_order(int n, std::vector<int>& order) {
+  order.clear();
+  if (n < 2) return;
+  std::vector<std::pair<int, int>> q;
+  q.reserve(n);
+  q.push_back({0, n});
+  for (size_t h = 0; h < q.size(); h++) {
+    const int lo = q[h].first, hi = q[h].second;
+    if (hi - lo < 2) continue;
+    const int mid = lo + (hi - lo) / 2;
+    order.push_back(mid);
+    q.push_back({lo, mid});
+    q.push_back({mid, hi});
+  }
+}
+
+}  // namespace
+
+extern "C" int kg_edge_batch(kg_world w, int ri, const float* A, const float* B, int nseg, double eps, int norm_inf,
+                             float* first_u) {
+  return guard([&] {
+    need(w != nullptr, "world is NULL");
+    need(ri >= 0 && ri < (int)w->robots.size(), "robot index out of range");
+    need(nseg >= 0, "negative segment count");
+    need(eps > 0 && std::isfinite(eps), "eps must be positive");  // motionplanning.cpp:727-728
+    if (nseg == 0) return;
+    need(A && B && first_u, "NULL buffer");
+    kg_ctx_s* c = w->ctx;
+    ensure_status_host(c);
+    hip_check(hipSetDevice(c->device), "hipSetDevice");
+    const int n = w->robots[ri]->n;
+    // sample counts n_s = ceil(d/eps) (robotcspace.py:216-228), d in L2 or Linf
+    std::vector<int> ns(nseg), ooff(nseg);
+    std::vector<int> order_all;
+    std::vector<int> tmp;
+    std::vector<std::pair<int, int>> cache;  // n -> offset
+    for (int s = 0; s < nseg; s++) {
+      // no FMA contraction: n_s must equal ceil(d/eps) computed with separate
+      // roundings, or a sample count (and every u) would change
+#pragma clang fp contract(off)
+      double acc = 0;
+      for (int k = 0; k < n; k++) {
+        const double d = (double)A[(size_t)s * n + k] - (double)B[(size_t)s * n + k];
+        if (norm_inf) acc = std::max(acc, std::fabs(d));
+        else acc += d * d;
+      }
+      const double dist = norm_inf ? acc : std::sqrt(acc);
+      const double m = std::ceil(dist / eps);
+      need(m < 1e7, "segment needs more than 1e7 samples");
+      ns[s] = (int)m;
+      int off = -1;
+      for (auto& p : cache)
+        if (p.first == ns[s]) off = p.second;
+      if (off < 0) {
+        bisection_order(ns[s], tmp);
+        off = (int)order_all.size();
+        order_all.insert(order_all.end(), tmp.begin(), tmp.end());
+        cache.push_back({ns[s], off});
+      }
+      ooff[s] = off;
+      first_u[s] = -1.0f;
+    }
+    hipStream_t st = c->stream;
+    auto up = [&](DevBuf& b, const void* src, size_t bytes) {
+      hip_check(b.ensure(std::max<size_t>(bytes, 16)), "hipMalloc");
+      if (bytes) hip_check(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, st), "copy");
+    };
+    up(c->wsEA, A, (size_t)nseg * n * sizeof(float));
+    up(c->wsEB, B, (size_t)nseg * n * sizeof(float));
+    up(c->wsENs, ns.data(), nseg * sizeof(int));
+    up(c->wsEOoff, ooff.data(), nseg * sizeof(int));
+    up(c->wsEOrder, order_all.data(), order_all.size() * sizeof(int));
+    std::vector<float> fu(nseg, -1.0f);
+    std::vector<uint8_t> done(nseg, 0);
+    up(c->wsEU, fu.data(), nseg * sizeof(float));
+    up(c->wsEDone, done.data(), nseg);
+    // rounds over bisection positions [pos0, pos1): 1, 1, 2, 4, ... wide
+    int pos0 = 0, width = 1;
+    std::vector<int> live, soff;
+    for (;;) {
+      live.clear();
+      soff.clear();
+      int total = 0;
+      for (int s = 0; s < nseg; s++) {
+        const int m = ns[s] - 1;  // interior samples
+        if (done[s] || pos0 >= m) continue;
+        live.push_back(s);
+        soff.push_back(total);
+        total += std::min(width, m - pos0);
+      }
+      if (live.empty()) break;
+      up(c->wsESeg, live.data(), live.size() * sizeof(int));
+      up(c->wsESoff, soff.data(), soff.size() * sizeof(int));
+      hip_check(c->wsEQ.ensure((size_t)total * n * sizeof(double)), "hipMalloc");
+      hip_check(c->wsEFeas.ensure((size_t)total), "hipMalloc");
+      hip_check(kgl_edge_samples(c->wsEA.as<float>(), c->wsEB.as<float>(), n, c->wsESeg.as<int>(), c->wsESoff.as<int>(),
+                                 (int)live.size(), c->wsENs.as<int>(), c->wsEOoff.as<int>(), c->wsEOrder.as<int>(), pos0,
+                                 total, c->wsEQ.as<double>(), st),
+                "edge_samples");
+      feasible_dev(w, ri, nullptr, c->wsEQ.as<double>(), total, c->wsEFeas.as<uint8_t>(), nullptr, nullptr, nullptr, st);
+      hip_check(kgl_edge_reduce(c->wsESeg.as<int>(), c->wsESoff.as<int>(), (int)live.size(), total,
+                                c->wsEFeas.as<uint8_t>(), c->wsENs.as<int>(), c->wsEOoff.as<int>(), c->wsEOrder.as<int>(),
+                                pos0, c->wsEU.as<float>(), c->wsEDone.as<uint8_t>(), st),
+                "edge_reduce");
+      hip_check(hipMemcpyAsync(done.data(), c->wsEDone.p, nseg, hipMemcpyDeviceToHost, st), "copy");
+      hip_check(hipStreamSynchronize(st), "sync");
+      feasible_sync_check(w);
+      pos0 += width;
+      width = std::min(width * 2, 1 << 20);
+    }
+    hip_check(hipMemcpyAsync(first_u, c->wsEU.p, nseg * sizeof(float), hipMemcpyDeviceToHost, st), "copy");
+    hip_check(hipStreamSynchronize(st), "sync");
+  });
+}

@@ -60,6 +60,10 @@ struct SdfDev {
   float bmin[3];
   float inv_h[3];  // 1/h
   float h[3];
+  float Xinv[12];  // world -> grid frame (column-major R, then t)
+  int ident;       // grid frame == world frame
+  double bmin64[3];
+  double inv_h64[3];
 };
 
 // Robot/world tables for one batched check, resident in HBM.
@@ -139,6 +143,38 @@ struct TravArgs {
 hipError_t kgl_collide_f32(const TravArgs& a, hipStream_t s);
 hipError_t kgl_collide_f64(const TravArgs& a, hipStream_t s);
 
+struct DistArgs {
+  PoolDev pool;
+  PlanDev plan;
+  const double* T64;
+  int N;
+  int npairs;
+  const int* pair_a;     // link index
+  const int* pair_b;     // env item (kind 0) or link index (kind 1)
+  const int* pair_kind;
+  float stop_below;      // an item stops once its distance is <= this
+  float upper;           // initial bound (DistanceQuerySettings::upperBound)
+  float* out;            // N x npairs
+  int* out_tri;          // nullable: N x npairs x 2 closest triangle pair (pool numbering)
+  int* chunk_counter;
+  uint64_t* spill;
+  int spill_cap;
+  int* status;
+  int grid;
+  int ipw;               // items per chunk
+  unsigned long long* counters;
+};
+hipError_t kgl_distance(const DistArgs& a, hipStream_t s);
+int kgl_distance_grid(int num_cus);
+hipError_t kgl_fk64(const LinkDesc* links, int n, const float* Qf, const double* Qd, int N, double* T64, int* reset,
+                    int nreset, hipStream_t s);
+
+hipError_t kgl_edge_samples(const float* A, const float* B, int n, const int* seg, const int* soff, int nlive,
+                            const int* nsamp, const int* ooff, const int* order, int pos0, int total, double* Qd,
+                            hipStream_t s);
+hipError_t kgl_edge_reduce(const int* seg, const int* soff, int nlive, int total, const uint8_t* feas, const int* nsamp,
+                           const int* ooff, const int* order, int pos0, float* first_u, uint8_t* done, hipStream_t s);
+
 struct FinArgs {
   int N;
   int mode;
@@ -159,6 +195,10 @@ hipError_t kgl_finalize(const FinArgs& a, hipStream_t s);
 hipError_t kgl_finalize_list(const FinArgs& a, const int* list, const int* count, int maxn, hipStream_t s);
 hipError_t kgl_fill_int(int* p, int n, int v, hipStream_t s);
 int kgl_collide_grid(int exact, int num_cus);
+hipError_t kgl_sdf_eval(const SdfDev& g, const float* pts, int n, float* d, float* grad, hipStream_t s);
+hipError_t kgl_sdf_penalty(const LinkDesc* links, int n, const SdfDev& g, const float* Q, int N, const float* pts,
+                           const int* poff, float dsafe, float* loss, const float* dloss, float* dQ, float* scratch,
+                           bool bwd, hipStream_t s);
 
 // ---- C-ABI object layouts
 
@@ -208,7 +248,7 @@ struct RobotPlan {
   std::vector<int> selfpairs;
   std::vector<std::pair<int, int>> pairs;  // world ID pairs for distance output
   std::vector<int> pair_a, pair_b, pair_kind;  // link, env-or-link, 0 env / 1 self
-  DevBuf dlinks, dqmin, dqmax, denv, dlemask, dself;
+  DevBuf dlinks, dqmin, dqmax, denv, dlemask, dself, dpa, dpb, dpk;
   PlanDev dev{};
   Tol tol32{}, tol64{};
   int maxdepth_a = 0, maxdepth_b = 0;
@@ -244,6 +284,7 @@ struct kg_ctx_s {
   hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   int last_N = 0;
   DevBuf wsQ, wsT64, wsLB, wsJL, wsCounts, wsOffsets, wsItems, wsScanTmp, wsBest, wsUnc, wsChunk, wsSpill,
-      wsStatus, wsRedo, wsOut8a, wsOut8b, wsOut8c, wsOut32, wsF, wsG;
+      wsStatus, wsRedo, wsOut8a, wsOut8b, wsOut8c, wsOut32, wsF, wsG, wsSdfScratch, wsPoff, wsPts, wsLoss, wsDQ,
+      wsDl, wsEA, wsEB, wsESeg, wsESoff, wsENs, wsEOoff, wsEOrder, wsEQ, wsEFeas, wsEU, wsEDone;
   int num_cus = 256;
 };

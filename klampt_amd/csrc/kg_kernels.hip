@@ -996,3 +996,706 @@ int kgl_collide_grid(int exact, int num_cus) {
   if (v && atoi(v) > 0) nb = atoi(v);
   return nb * num_cus;
 }
+
+// ------------------------------------------------------------ SDF queries
+
+namespace {
+
+// Cell-centred trilinear value and gradient (ImplicitSurface convention:
+// samples at bmin + (i+1/2) h).  Points outside the box of cell centres are
+// clamped to it; the gradient along a clamped axis is 0.
+// The continuous cell coordinate and its fraction are formed in FP64 (a few
+// operations per point): in FP32 their rounding, amplified by steep fields,
+// would dominate the error; the lerps run in FP32.
+__device__ inline float sdf_trilinear(const SdfDev& g, float x, float y, float z, float* gr) {
+  const double u[3] = {((double)x - g.bmin64[0]) * g.inv_h64[0] - 0.5, ((double)y - g.bmin64[1]) * g.inv_h64[1] - 0.5,
+                       ((double)z - g.bmin64[2]) * g.inv_h64[2] - 0.5};
+  int i0[3], i1[3];
+  float f[3], gs[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    gs[a] = g.inv_h[a];
+    const double um = (double)(g.dims[a] - 1);
+    double v = u[a];
+    if (v <= 0.0) { v = 0.0; gs[a] = 0.f; }
+    else if (v >= um) { v = um; gs[a] = 0.f; }
+    int i = (int)floor(v);
+    i = min(i, g.dims[a] - 2);
+    i = max(i, 0);
+    i0[a] = i;
+    i1[a] = min(i + 1, g.dims[a] - 1);
+    f[a] = g.dims[a] > 1 ? (float)(v - (double)i) : 0.f;
+    if (g.dims[a] == 1) gs[a] = 0.f;
+  }
+  const int s1 = g.dims[2], s0 = g.dims[1] * g.dims[2];
+  const float* v = g.v;
+  const float c000 = v[i0[0] * s0 + i0[1] * s1 + i0[2]], c001 = v[i0[0] * s0 + i0[1] * s1 + i1[2]];
+  const float c010 = v[i0[0] * s0 + i1[1] * s1 + i0[2]], c011 = v[i0[0] * s0 + i1[1] * s1 + i1[2]];
+  const float c100 = v[i1[0] * s0 + i0[1] * s1 + i0[2]], c101 = v[i1[0] * s0 + i0[1] * s1 + i1[2]];
+  const float c110 = v[i1[0] * s0 + i1[1] * s1 + i0[2]], c111 = v[i1[0] * s0 + i1[1] * s1 + i1[2]];
+  const float fx = f[0], fy = f[1], fz = f[2];
+  const float c00 = c000 * (1.f - fz) + c001 * fz, c01 = c010 * (1.f - fz) + c011 * fz;
+  const float c10 = c100 * (1.f - fz) + c101 * fz, c11 = c110 * (1.f - fz) + c111 * fz;
+  const float c0 = c00 * (1.f - fy) + c01 * fy, c1 = c10 * (1.f - fy) + c11 * fy;
+  if (gr) {
+    const float dz0 = (c001 - c000) * (1.f - fy) + (c011 - c010) * fy;
+    const float dz1 = (c101 - c100) * (1.f - fy) + (c111 - c110) * fy;
+    gr[0] = (c1 - c0) * gs[0];
+    gr[1] = ((c01 - c00) * (1.f - fx) + (c11 - c10) * fx) * gs[1];
+    gr[2] = (dz0 * (1.f - fx) + dz1 * fx) * gs[2];
+  }
+  return c0 * (1.f - fx) + c1 * fx;
+}
+
+// World-frame query: map into the grid frame, rotate the gradient back.
+__device__ inline float sdf_world(const SdfDev& g, float x, float y, float z, float* gr) {
+  if (g.ident) return sdf_trilinear(g, x, y, z, gr);
+  const float* X = g.Xinv;
+  const float lx = X[0] * x + X[3] * y + X[6] * z + X[9];
+  const float ly = X[1] * x + X[4] * y + X[7] * z + X[10];
+  const float lz = X[2] * x + X[5] * y + X[8] * z + X[11];
+  float gl[3];
+  const float d = sdf_trilinear(g, lx, ly, lz, gr ? gl : nullptr);
+  if (gr) {  // R = Xinv^T
+    gr[0] = X[0] * gl[0] + X[1] * gl[1] + X[2] * gl[2];
+    gr[1] = X[3] * gl[0] + X[4] * gl[1] + X[5] * gl[2];
+    gr[2] = X[6] * gl[0] + X[7] * gl[1] + X[8] * gl[2];
+  }
+  return d;
+}
+
+__global__ __launch_bounds__(256) void k_sdf_eval(SdfDev g, const float* __restrict__ P, int n, float* __restrict__ d,
+                                                  float* __restrict__ grad) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float gr[3];
+  const float v = sdf_world(g, P[3 * i], P[3 * i + 1], P[3 * i + 2], grad ? gr : nullptr);
+  d[i] = v;
+  if (grad) {
+    grad[3 * i] = gr[0];
+    grad[3 * i + 1] = gr[1];
+    grad[3 * i + 2] = gr[2];
+  }
+}
+
+// SDF hinge penalty, one lane per configuration.
+//   L(q) = sum_k sum_{p in link k} max(0, dsafe - sdf(T_k(q) p))^2
+// BWD: dQ[c][j] = dloss[c] * dL/dq_j by the FK-Jacobian adjoint: with
+// g_p = dL/dx_p = -2 max(0, dsafe - d) grad_d(x_p), each link k accumulates
+// f_k = sum g_p and m_k = sum x_p x g_p; for joint j with world axis w_j and
+// origin o_j, dL/dq_j = w_j . (M_j - o_j x F_j) (revolute) or w_j . F_j
+// (prismatic), F_j, M_j summed over the subtree of j.  FK is recomputed here
+// (nothing is kept from the forward pass).
+//
+// Scratch (structure of arrays over configurations, [row][comp][c]):
+//   rows [0,n)   full link transform, written only for links that are the
+//                parent of a non-consecutive child (branching trees);
+//   rows [n,2n)  w(3) o(3) f(3) m(3) of every link (BWD only).
+template <bool BWD>
+__global__ __launch_bounds__(256) void k_sdf_penalty(const LinkDesc* __restrict__ links, int n, SdfDev g,
+                                                      const float* __restrict__ Q, int N, const float* __restrict__ pts,
+                                                      const int* __restrict__ poff, float dsafe,
+                                                      float* __restrict__ loss_out, const float* __restrict__ dloss,
+                                                      float* __restrict__ dQ, float* __restrict__ scratch) {
+  __shared__ float sT0[KG_MAX_LINKS][12];
+  __shared__ float sAx[KG_MAX_LINKS][3];
+  __shared__ int sPar[KG_MAX_LINKS], sTy[KG_MAX_LINKS], sOff[KG_MAX_LINKS + 1], sKeep[KG_MAX_LINKS];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    for (int k = 0; k < 12; k++) sT0[i][k] = (float)links[i].T0[k];
+    for (int k = 0; k < 3; k++) sAx[i][k] = (float)links[i].axis[k];
+    sPar[i] = links[i].parent;
+    sTy[i] = links[i].jtype;
+    int keep = 0;
+    for (int j = i + 2; j < n; j++)
+      if (links[j].parent == i) keep = 1;
+    sKeep[i] = keep;
+  }
+  for (int i = threadIdx.x; i <= n; i += blockDim.x) sOff[i] = poff[i];
+  __syncthreads();
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  auto S = [&](int row, int comp) -> float& { return scratch[((size_t)row * 12 + comp) * N + c]; };
+  float T[12];
+  float loss = 0.f;
+  for (int k = 0; k < n; k++) {
+    const float q = Q[(size_t)c * n + k];
+    const float* w = sAx[k];
+    float J[12];
+    if (sTy[k] == 1) {
+#pragma unroll
+      for (int i = 0; i < 9; i++) J[i] = (i % 4 == 0) ? 1.f : 0.f;
+      J[9] = w[0] * q; J[10] = w[1] * q; J[11] = w[2] * q;
+    } else {
+      float s, co;
+      sincosf(q, &s, &co);
+      const float v = 1.f - co;
+#pragma unroll
+      for (int col = 0; col < 3; col++)
+#pragma unroll
+        for (int row = 0; row < 3; row++) J[col * 3 + row] = w[row] * w[col] * v;
+      J[0] += co; J[4] += co; J[8] += co;
+      J[1] += s * w[2]; J[2] -= s * w[1];
+      J[3] -= s * w[2]; J[5] += s * w[0];
+      J[6] += s * w[1]; J[7] -= s * w[0];
+      J[9] = J[10] = J[11] = 0.f;
+    }
+    float L[12];
+    const float* A = sT0[k];
+#pragma unroll
+    for (int col = 0; col < 3; col++)
+#pragma unroll
+      for (int r = 0; r < 3; r++) L[col * 3 + r] = A[r] * J[col * 3] + A[3 + r] * J[col * 3 + 1] + A[6 + r] * J[col * 3 + 2];
+#pragma unroll
+    for (int r = 0; r < 3; r++) L[9 + r] = A[r] * J[9] + A[3 + r] * J[10] + A[6 + r] * J[11] + A[9 + r];
+    const int p = sPar[k];
+    if (p < 0) {
+#pragma unroll
+      for (int i = 0; i < 12; i++) T[i] = L[i];
+    } else {
+      float Pv[12];
+      if (p == k - 1) {
+#pragma unroll
+        for (int i = 0; i < 12; i++) Pv[i] = T[i];
+      } else {
+#pragma unroll
+        for (int i = 0; i < 12; i++) Pv[i] = S(p, i);
+      }
+#pragma unroll
+      for (int col = 0; col < 3; col++)
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+          T[col * 3 + r] = Pv[r] * L[col * 3] + Pv[3 + r] * L[col * 3 + 1] + Pv[6 + r] * L[col * 3 + 2];
+#pragma unroll
+      for (int r = 0; r < 3; r++) T[9 + r] = Pv[r] * L[9] + Pv[3 + r] * L[10] + Pv[6 + r] * L[11] + Pv[9 + r];
+    }
+    if (sKeep[k]) {
+#pragma unroll
+      for (int i = 0; i < 12; i++) S(k, i) = T[i];
+    }
+    float f0 = 0.f, f1 = 0.f, f2 = 0.f, m0 = 0.f, m1 = 0.f, m2 = 0.f;
+    for (int pi = sOff[k]; pi < sOff[k + 1]; pi++) {
+      const float px = pts[3 * pi], py = pts[3 * pi + 1], pz = pts[3 * pi + 2];
+      const float x = T[0] * px + T[3] * py + T[6] * pz + T[9];
+      const float y = T[1] * px + T[4] * py + T[7] * pz + T[10];
+      const float z = T[2] * px + T[5] * py + T[8] * pz + T[11];
+      float gr[3];
+      const float d = sdf_world(g, x, y, z, BWD ? gr : nullptr);
+      const float h = dsafe - d;
+      if (h > 0.f) {
+        loss += h * h;
+        if (BWD) {
+          const float s2 = -2.f * h;
+          const float g0 = s2 * gr[0], g1 = s2 * gr[1], g2 = s2 * gr[2];
+          f0 += g0; f1 += g1; f2 += g2;
+          m0 += y * g2 - z * g1;
+          m1 += z * g0 - x * g2;
+          m2 += x * g1 - y * g0;
+        }
+      }
+    }
+    if (BWD) {
+      S(n + k, 0) = T[0] * w[0] + T[3] * w[1] + T[6] * w[2];
+      S(n + k, 1) = T[1] * w[0] + T[4] * w[1] + T[7] * w[2];
+      S(n + k, 2) = T[2] * w[0] + T[5] * w[1] + T[8] * w[2];
+      S(n + k, 3) = T[9]; S(n + k, 4) = T[10]; S(n + k, 5) = T[11];
+      S(n + k, 6) = f0; S(n + k, 7) = f1; S(n + k, 8) = f2;
+      S(n + k, 9) = m0; S(n + k, 10) = m1; S(n + k, 11) = m2;
+    }
+  }
+  if (!BWD) {
+    loss_out[c] = loss;
+    return;
+  }
+  const float dl = dloss ? dloss[c] : 1.f;
+  // reverse sweep: subtree sums carried in registers along chains, added to
+  // the parent's scratch row for branching children
+  float cF0 = 0.f, cF1 = 0.f, cF2 = 0.f, cM0 = 0.f, cM1 = 0.f, cM2 = 0.f;
+  for (int j = n - 1; j >= 0; j--) {
+    float F0 = S(n + j, 6), F1 = S(n + j, 7), F2 = S(n + j, 8);
+    float M0 = S(n + j, 9), M1 = S(n + j, 10), M2 = S(n + j, 11);
+    if (j + 1 < n && sPar[j + 1] == j) {
+      F0 += cF0; F1 += cF1; F2 += cF2;
+      M0 += cM0; M1 += cM1; M2 += cM2;
+    }
+    const float w0 = S(n + j, 0), w1 = S(n + j, 1), w2 = S(n + j, 2);
+    const float o0 = S(n + j, 3), o1 = S(n + j, 4), o2 = S(n + j, 5);
+    float gq;
+    if (sTy[j] == 1) {
+      gq = w0 * F0 + w1 * F1 + w2 * F2;
+    } else {
+      gq = w0 * (M0 - (o1 * F2 - o2 * F1)) + w1 * (M1 - (o2 * F0 - o0 * F2)) + w2 * (M2 - (o0 * F1 - o1 * F0));
+    }
+    dQ[(size_t)c * n + j] = dl * gq;
+    const int p = sPar[j];
+    if (p >= 0 && p != j - 1) {
+      S(n + p, 6) += F0; S(n + p, 7) += F1; S(n + p, 8) += F2;
+      S(n + p, 9) += M0; S(n + p, 10) += M1; S(n + p, 11) += M2;
+    }
+    cF0 = F0; cF1 = F1; cF2 = F2; cM0 = M0; cM1 = M1; cM2 = M2;
+  }
+  if (loss_out) loss_out[c] = loss;
+}
+
+}  // namespace
+
+hipError_t kgl_sdf_eval(const SdfDev& g, const float* pts, int n, float* d, float* grad, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  k_sdf_eval<<<(n + 255) / 256, 256, 0, s>>>(g, pts, n, d, grad);
+  return hipGetLastError();
+}
+
+hipError_t kgl_sdf_penalty(const LinkDesc* links, int n, const SdfDev& g, const float* Q, int N, const float* pts,
+                           const int* poff, float dsafe, float* loss, const float* dloss, float* dQ, float* scratch,
+                           bool bwd, hipStream_t s) {
+  if (N <= 0) return hipSuccess;
+  if (bwd)
+    k_sdf_penalty<true><<<(N + 255) / 256, 256, 0, s>>>(links, n, g, Q, N, pts, poff, dsafe, loss, dloss, dQ, scratch);
+  else
+    k_sdf_penalty<false><<<(N + 255) / 256, 256, 0, s>>>(links, n, g, Q, N, pts, poff, dsafe, loss, dloss, dQ,
+                                                          scratch);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------ distance
+
+#include "kg_geom_math.h"
+
+namespace {
+
+// Lower bound on the distance between box A (frame A, mapped by rel into
+// frame B) and box B: the larger of the bounding-sphere gap and the largest
+// normalised projection gap over the 15 separating axes (projections onto a
+// unit axis are 1-Lipschitz).
+template <typename Real>
+__device__ inline Real obb_lower_bound(const NodeF& A, const NodeF& B, const Real* rel) {
+  Real Ap[9], cp[3];
+#pragma unroll
+  for (int col = 0; col < 3; col++)
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+      Ap[col * 3 + r] = rel[r] * A.R[col * 3] + rel[3 + r] * A.R[col * 3 + 1] + rel[6 + r] * A.R[col * 3 + 2];
+#pragma unroll
+  for (int r = 0; r < 3; r++) cp[r] = rel[r] * A.c[0] + rel[3 + r] * A.c[1] + rel[6 + r] * A.c[2] + rel[9 + r] - B.c[r];
+  Real C[3][3], Qa[3][3], T[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    T[i] = B.R[i * 3] * cp[0] + B.R[i * 3 + 1] * cp[1] + B.R[i * 3 + 2] * cp[2];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      C[i][j] = B.R[i * 3] * Ap[j * 3] + B.R[i * 3 + 1] * Ap[j * 3 + 1] + B.R[i * 3 + 2] * Ap[j * 3 + 2];
+      Qa[i][j] = fabs(C[i][j]);
+    }
+  }
+  const Real eA0 = A.e[0], eA1 = A.e[1], eA2 = A.e[2], eB0 = B.e[0], eB1 = B.e[1], eB2 = B.e[2];
+  const Real eA[3] = {eA0, eA1, eA2}, eB[3] = {eB0, eB1, eB2};
+  Real lb = sqrt(T[0] * T[0] + T[1] * T[1] + T[2] * T[2]) - sqrt(eA0 * eA0 + eA1 * eA1 + eA2 * eA2) -
+            sqrt(eB0 * eB0 + eB1 * eB1 + eB2 * eB2);
+#pragma unroll
+  for (int i = 0; i < 3; i++) lb = fmax(lb, fabs(T[i]) - (eB[i] + eA[0] * Qa[i][0] + eA[1] * Qa[i][1] + eA[2] * Qa[i][2]));
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    const Real t = T[0] * C[0][j] + T[1] * C[1][j] + T[2] * C[2][j];
+    lb = fmax(lb, fabs(t) - (eB[0] * Qa[0][j] + eB[1] * Qa[1][j] + eB[2] * Qa[2][j] + eA[j]));
+  }
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    const int i1 = (i + 1) % 3, i2 = (i + 2) % 3;
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      const int j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+      const Real l2 = (Real)1 - C[i][j] * C[i][j];
+      if (l2 < (Real)1e-6) continue;
+      const Real t = T[i2] * C[i1][j] - T[i1] * C[i2][j];
+      const Real ra = eB[i1] * Qa[i2][j] + eB[i2] * Qa[i1][j];
+      const Real rb = eA[j1] * Qa[i][j2] + eA[j2] * Qa[i][j1];
+      lb = fmax(lb, (fabs(t) - ra - rb) * rsqrt(l2));
+    }
+  }
+  return fmax(lb, (Real)0);
+}
+
+// Minimum distance per (configuration, pair) item; same pool scheme as
+// k_collide (depth-first continuation per lane, shared LIFO pool, batched
+// leaves).  A child pair is dropped when its lower bound exceeds the item's
+// best distance by more than the FP32 error pad; the continuation is the
+// child pair with the smaller bound.
+__global__ __launch_bounds__(64) void k_distance(DistArgs a) {
+  __shared__ uint64_t stk[KG_SCAP];
+  __shared__ uint64_t lq[KG_LQ];
+  __shared__ float s_rel[12][64];
+  __shared__ float s_pad[64];
+  __shared__ unsigned s_best[64];
+  __shared__ int s_item[64], s_pend[64], s_ta[64], s_tb[64];
+  __shared__ int s_chunk;
+
+  const int lane = threadIdx.x;
+  uint64_t* gsp_base = a.spill + (size_t)blockIdx.x * a.spill_cap;
+  const int nitems = a.N * a.npairs;
+  const unsigned stop = __float_as_uint(a.stop_below);
+  unsigned n_box = 0, n_tri = 0;
+
+  for (;;) {
+    if (lane == 0) s_chunk = atomicAdd(a.chunk_counter, 1);
+    __syncthreads();
+    const int i0 = s_chunk * a.ipw;
+    if (i0 >= nitems) break;
+    const int i1 = min(i0 + a.ipw, nitems);
+    s_pend[lane] = 0;
+    __syncthreads();
+    int it = i0, sp = 0, gsp = 0, nl = 0;
+    uint64_t cont = KG_NOTASK;
+
+    auto ensure_room = [&](int room) {
+      if (sp + room <= KG_SCAP) return;
+      const int h = KG_SCAP / 2;
+      if (gsp + h <= a.spill_cap) {
+        for (int i = lane; i < h; i += 64) gsp_base[gsp + i] = stk[i];
+        gsp += h;
+      } else {
+        // overflow: drop the bottom half (the results become upper bounds)
+        for (int i = lane; i < h; i += 64) atomicAdd(&s_pend[(int)(stk[i] >> 58)], -1);
+        if (lane == 0) atomicExch(a.status, 1);
+      }
+      __syncthreads();
+      for (int i = lane; i < sp - h; i += 64) stk[i] = stk[i + h];
+      sp -= h;
+      __syncthreads();
+    };
+    auto finish = [&](int slot, int delta) {
+      if (atomicAdd(&s_pend[slot], delta) + delta == 0) {
+        const int item = s_item[slot];
+        a.out[item] = __uint_as_float(s_best[slot]);
+        if (a.out_tri) {
+          a.out_tri[2 * item] = s_ta[slot];
+          a.out_tri[2 * item + 1] = s_tb[slot];
+        }
+      }
+    };
+    auto run_leaves = [&]() {
+      const int m = min(nl, 64);
+      float dl = 3e38f;
+      int slot = 0, ta = 0, tb = 0;
+      if (lane < m) {
+        const uint64_t e = lq[nl - 1 - lane];
+        slot = (int)(e >> 58);
+        ta = (int)((e >> 29) & 0x1fffffff);
+        tb = (int)(e & 0x1fffffff);
+        if (s_best[slot] > stop) {
+          n_tri++;
+          float rel[12];
+#pragma unroll
+          for (int k = 0; k < 12; k++) rel[k] = s_rel[k][slot];
+          V3T<float> tp[3], tq[3];
+          load_tri<float>(a.pool, ta, tp);
+          load_tri<float>(a.pool, tb, tq);
+          kgm::V<float> P[3], Q[3], cp, cq;
+#pragma unroll
+          for (int v = 0; v < 3; v++) {
+            const V3T<float> w = xf_apply<float>(rel, tp[v]);
+            P[v] = kgm::mk(w.x, w.y, w.z);
+            Q[v] = kgm::mk(tq[v].x, tq[v].y, tq[v].z);
+          }
+          dl = kgm::tri_dist<float>(P, Q, cp, cq);
+          atomicMin(&s_best[slot], __float_as_uint(dl));
+        }
+      }
+      __syncthreads();
+      if (lane < m) {
+        if (__float_as_uint(dl) == s_best[slot]) {
+          s_ta[slot] = ta;
+          s_tb[slot] = tb;
+        }
+      }
+      __syncthreads();
+      if (lane < m) finish(slot, -1);
+      nl -= m;
+      __syncthreads();
+    };
+
+    for (;;) {
+      const bool hungry = cont == KG_NOTASK;
+      const uint64_t hm = ballot64(hungry);
+      const int nh = __popcll(hm);
+      if (nh > sp && gsp > 0) {
+        const int k = min(gsp, KG_SCAP / 2);
+        for (int i = sp - 1 - lane; i >= 0; i -= 64) stk[i + k] = stk[i];
+        __syncthreads();
+        for (int i = lane; i < k; i += 64) stk[i] = gsp_base[gsp - k + i];
+        gsp -= k;
+        sp += k;
+        __syncthreads();
+      }
+      const int hp = lane_prefix(hm);
+      const int npop = min(nh, sp);
+      if (hungry && hp < npop) cont = stk[sp - 1 - hp];
+      sp -= npop;
+      // new items on free slots
+      const bool want = cont == KG_NOTASK;
+      const uint64_t wm = ballot64(want);
+      const uint64_t fm = ballot64(s_pend[lane] == 0);
+      const int navail = min(min(__popcll(wm), __popcll(fm)), i1 - it);
+      int myitem = -1, myslot = -1;
+      const int wp = lane_prefix(wm);
+      if (want && wp < navail) {
+        myitem = it + wp;
+        uint64_t f = fm;
+        for (int q = 0; q < wp; q++) f &= f - 1;
+        myslot = __ffsll((unsigned long long)f) - 1;
+      }
+      it += navail;
+      __syncthreads();
+      if (myitem >= 0) {
+        const int c = myitem / a.npairs, pidx = myitem % a.npairs;
+        const int la = a.pair_a[pidx], lb = a.pair_b[pidx], kind = a.pair_kind[pidx];
+        const double* XA = a.T64 + ((size_t)c * a.plan.nlinks + la) * 12;
+        const double* XB;
+        int geomB;
+        if (kind == 0) {
+          XB = a.plan.env[lb].X;
+          geomB = a.plan.env[lb].geom;
+        } else {
+          XB = a.T64 + ((size_t)c * a.plan.nlinks + lb) * 12;
+          geomB = a.plan.links[lb].geom;
+        }
+        const double d0 = XA[9] - XB[9], d1 = XA[10] - XB[10], d2 = XA[11] - XB[11];
+#pragma unroll
+        for (int col = 0; col < 3; col++)
+#pragma unroll
+          for (int r = 0; r < 3; r++)
+            s_rel[col * 3 + r][myslot] =
+                (float)(XB[r * 3] * XA[col * 3] + XB[r * 3 + 1] * XA[col * 3 + 1] + XB[r * 3 + 2] * XA[col * 3 + 2]);
+        double tn = 0;
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+          const double v = XB[r * 3] * d0 + XB[r * 3 + 1] * d1 + XB[r * 3 + 2] * d2;
+          s_rel[9 + r][myslot] = (float)v;
+          tn += v * v;
+        }
+        const GeomDesc& GA = a.pool.geoms[a.plan.links[la].geom];
+        const GeomDesc& GB = a.pool.geoms[geomB];
+        const double u = 5.9604644775390625e-08;
+        const double ev = u * (8.0 * GA.rmax + 4.0 * sqrt(tn)) + u * GB.rmax;
+        s_pad[myslot] = (float)(2.0 * ev + 16.0 * u * (sqrt(tn) + GA.rmax + GB.rmax));
+        s_best[myslot] = __float_as_uint(a.upper);
+        s_item[myslot] = myitem;
+        s_ta[myslot] = -1;
+        s_tb[myslot] = -1;
+        s_pend[myslot] = 1;
+        cont = mk_task(myslot, 0, 1, GA.root, GB.root);
+      }
+      __syncthreads();
+
+      const bool working = cont != KG_NOTASK;
+      if (!ballot64(working)) {
+        if (sp == 0 && gsp == 0) {
+          if (nl == 0) break;
+          run_leaves();
+        }
+        continue;
+      }
+
+      uint64_t keep = KG_NOTASK, push = KG_NOTASK, lf0 = 0, lf1 = 0;
+      int nlf = 0, nout = 0, slot = 0;
+      if (working) {
+        const uint64_t task = cont;
+        cont = KG_NOTASK;
+        slot = (int)(task >> 58);
+        const int side = (int)((task >> 57) & 1), single = (int)((task >> 56) & 1);
+        const int x = (int)((task >> 28) & 0xfffffff), y = (int)(task & 0xfffffff);
+        const float best = __uint_as_float(s_best[slot]);
+        if (s_best[slot] > stop) {
+          const NodeF O = load_nodef(a.pool.nodes, y);
+          const NodeF S0 = load_nodef(a.pool.nodes, x);
+          const NodeF S1 = load_nodef(a.pool.nodes, single ? x : x + 1);
+          float rel[12];
+#pragma unroll
+          for (int k = 0; k < 12; k++) rel[k] = s_rel[k][slot];
+          const float pad = s_pad[slot];
+          const bool aIsS = single || side == 0;
+          float lbk[2] = {3e38f, 3e38f};
+          uint64_t tk[2] = {KG_NOTASK, KG_NOTASK};
+#pragma unroll 1
+          for (int k = 0; k < (single ? 1 : 2); k++) {
+            const NodeF S = sel(k == 0, S0, S1);
+            const NodeF A = sel(aIsS, S, O);
+            const NodeF B = sel(aIsS, O, S);
+            n_box++;
+            const float lb = obb_lower_bound<float>(A, B, rel);
+            if (lb > best + pad) continue;
+            const bool la = A.ch < 0, lbf = B.ch < 0;
+            if (la && lbf) {
+              const uint64_t w = mk_leaf(slot, ~A.ch, ~B.ch);
+              if (nlf == 0) lf0 = w; else lf1 = w;
+              nlf++;
+            } else {
+              bool splitA;
+              if (la) splitA = false;
+              else if (lbf) splitA = true;
+              else splitA = (A.e[0] + A.e[1] + A.e[2]) >= (B.e[0] + B.e[1] + B.e[2]);
+              const int nodeS = x + k;
+              const int nodeA = aIsS ? nodeS : y;
+              const int nodeB = aIsS ? y : nodeS;
+              const uint64_t w = splitA ? mk_task(slot, 0, 0, A.ch, nodeB) : mk_task(slot, 1, 0, B.ch, nodeA);
+              if (k == 0) { tk[0] = w; lbk[0] = lb; } else { tk[1] = w; lbk[1] = lb; }
+              nout++;
+            }
+          }
+          // continue with the nearer child pair, pool the other
+          const bool swap = lbk[1] < lbk[0];
+          keep = swap ? tk[1] : tk[0];
+          push = swap ? tk[0] : tk[1];
+          if (keep == KG_NOTASK) {
+            keep = push;
+            push = KG_NOTASK;
+          }
+        }
+      }
+      cont = keep;
+      {
+        const uint64_t l1 = ballot64(nlf >= 1), l2 = ballot64(nlf >= 2);
+        const int ltot = __popcll(l1) + __popcll(l2);
+        if (nl + ltot > KG_LQ) {
+          __syncthreads();
+          while (nl + ltot > KG_LQ) run_leaves();
+        }
+        const int pos = nl + lane_prefix(l1) + lane_prefix(l2);
+        if (nlf >= 1) lq[pos] = lf0;
+        if (nlf >= 2) lq[pos + 1] = lf1;
+        nl += ltot;
+      }
+      if (working) finish(slot, nout + nlf - 1);
+      const bool haspush = push != KG_NOTASK;
+      const uint64_t pm = ballot64(haspush);
+      const int ptot = __popcll(pm);
+      ensure_room(ptot);
+      if (haspush) stk[sp + lane_prefix(pm)] = push;
+      sp += ptot;
+      __syncthreads();
+      if (nl >= 1) run_leaves();
+    }
+    __syncthreads();
+  }
+  if (a.counters) {
+    atomicAdd(&a.counters[0], (unsigned long long)n_box);
+    atomicAdd(&a.counters[1], (unsigned long long)n_tri);
+  }
+}
+
+}  // namespace
+
+hipError_t kgl_distance(const DistArgs& a, hipStream_t s) {
+  if (a.N <= 0 || a.npairs <= 0) return hipSuccess;
+  k_distance<<<a.grid, 64, 0, s>>>(a);
+  return hipGetLastError();
+}
+
+int kgl_distance_grid(int num_cus) {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_distance, 64, 0) != hipSuccess || nb < 1) nb = 8;
+  return nb * num_cus;
+}
+
+namespace {
+__global__ __launch_bounds__(256) void k_fk64(const LinkDesc* __restrict__ links, int n, const float* __restrict__ Qf,
+                                             const double* __restrict__ Qd, int N, double* __restrict__ T64,
+                                             int* reset, int nreset) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < nreset) reset[c] = 0;
+  if (c >= N) return;
+  double* T = T64 + (size_t)c * n * 12;
+  for (int l = 0; l < n; l++) {
+    const double q = Qf ? (double)Qf[(size_t)c * n + l] : Qd[(size_t)c * n + l];
+    const LinkDesc& L = links[l];
+    X64 J, T0, Lx, X;
+    joint64(L.jtype, L.axis, q, J);
+    x64_load(L.T0, T0);
+    x64_mul(T0, J, Lx);
+    if (L.parent >= 0) {
+      X64 Pp;
+      x64_load(T + 12 * L.parent, Pp);
+      x64_mul(Pp, Lx, X);
+    } else {
+      X = Lx;
+    }
+    x64_store(T + 12 * l, X);
+  }
+}
+}  // namespace
+
+hipError_t kgl_fk64(const LinkDesc* links, int n, const float* Qf, const double* Qd, int N, double* T64, int* reset,
+                    int nreset, hipStream_t s) {
+  const int m = N > nreset ? N : nreset;
+  if (m <= 0) return hipSuccess;
+  k_fk64<<<(m + 255) / 256, 256, 0, s>>>(links, n, Qf, Qd, N, T64, reset, nreset);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------ edge checks
+
+namespace {
+
+// Sample configurations for one round: slot j belongs to live segment
+// seg[s] (s found by binary search in the per-segment slot offsets) at
+// bisection position pos0 + (j - soff[s]); q = a*(1-u) + b*u with u = i/n
+// (Cpp/Modeling/Interpolate.cpp:13-14), i the sample index at that position.
+__global__ __launch_bounds__(256) void k_edge_samples(const float* __restrict__ A, const float* __restrict__ B, int n,
+                                                     const int* __restrict__ seg, const int* __restrict__ soff,
+                                                     int nlive, const int* __restrict__ nsamp,
+                                                     const int* __restrict__ ooff, const int* __restrict__ order,
+                                                     int pos0, int total, double* __restrict__ Qd) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= total) return;
+  int lo = 0, hi = nlive - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (soff[mid] <= j) lo = mid; else hi = mid - 1;
+  }
+  const int s = seg[lo];
+  const int k = pos0 + (j - soff[lo]);
+  const int ns = nsamp[s];
+  const int i = order[ooff[s] + k];
+  const double u = (double)i / (double)ns;
+  for (int d = 0; d < n; d++) {
+#pragma clang fp contract(off)
+    double q = (double)A[(size_t)s * n + d] * (1.0 - u);
+    q += (double)B[(size_t)s * n + d] * u;
+    Qd[(size_t)j * n + d] = q;
+  }
+}
+
+// First infeasible position of each live segment in this round.
+__global__ __launch_bounds__(256) void k_edge_reduce(const int* __restrict__ seg, const int* __restrict__ soff, int nlive,
+                                                    int total, const uint8_t* __restrict__ feas,
+                                                    const int* __restrict__ nsamp, const int* __restrict__ ooff,
+                                                    const int* __restrict__ order, int pos0, float* __restrict__ first_u,
+                                                    uint8_t* __restrict__ done) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nlive) return;
+  const int b = soff[t], e = t + 1 < nlive ? soff[t + 1] : total;
+  const int s = seg[t];
+  for (int j = b; j < e; j++)
+    if (!feas[j]) {
+      const int i = order[ooff[s] + pos0 + (j - b)];
+      first_u[s] = (float)((double)i / (double)nsamp[s]);
+      done[s] = 1;
+      return;
+    }
+}
+
+}  // namespace
+
+hipError_t kgl_edge_samples(const float* A, const float* B, int n, const int* seg, const int* soff, int nlive,
+                            const int* nsamp, const int* ooff, const int* order, int pos0, int total, double* Qd,
+                            hipStream_t s) {
+  if (total <= 0) return hipSuccess;
+  k_edge_samples<<<(total + 255) / 256, 256, 0, s>>>(A, B, n, seg, soff, nlive, nsamp, ooff, order, pos0, total, Qd);
+  return hipGetLastError();
+}
+
+hipError_t kgl_edge_reduce(const int* seg, const int* soff, int nlive, int total, const uint8_t* feas, const int* nsamp,
+                           const int* ooff, const int* order, int pos0, float* first_u, uint8_t* done, hipStream_t s) {
+  if (nlive <= 0) return hipSuccess;
+  k_edge_reduce<<<(nlive + 255) / 256, 256, 0, s>>>(seg, soff, nlive, total, feas, nsamp, ooff, order, pos0, first_u,
+                                                     done);
+  return hipGetLastError();
+}

@@ -206,6 +206,43 @@ class Robot:
         return out
 
 
+    # ---- SDF hinge penalty (sum over links/points of max(0, dsafe - d)^2)
+    def sdf_penalty(self, sdf, Q, pts, pts_per_link, dsafe=0.05):
+        Q = _f(Q).reshape(-1, self.n)
+        P, ppl = _f(pts).reshape(-1, 3), _i(pts_per_link)
+        loss = np.zeros(Q.shape[0], np.float32)
+        call("kg_sdf_penalty_fwd", self.h, sdf.h, _p(Q, _pf), Q.shape[0], _p(P, _pf), _p(ppl, _pi), float(dsafe),
+             _p(loss, _pf))
+        return loss
+
+    def sdf_penalty_grad(self, sdf, Q, pts, pts_per_link, dsafe=0.05, dloss=None):
+        Q = _f(Q).reshape(-1, self.n)
+        P, ppl = _f(pts).reshape(-1, 3), _i(pts_per_link)
+        dl = None if dloss is None else _f(dloss)
+        dQ = np.zeros_like(Q)
+        call("kg_sdf_penalty_bwd", self.h, sdf.h, _p(Q, _pf), Q.shape[0], _p(P, _pf), _p(ppl, _pi), float(dsafe),
+             _p(dl, _pf), _p(dQ, _pf))
+        return dQ
+
+    def sdf_penalty_dev(self, sdf, Q, pts, pts_per_link, dsafe=0.05, out=None):
+        import torch
+        ppl = _i(pts_per_link)
+        if out is None:
+            out = torch.empty(Q.shape[0], dtype=torch.float32, device=Q.device)
+        call("kg_sdf_penalty_fwd_dev", self.h, sdf.h, _tp(Q), Q.shape[0], _tp(pts), _p(ppl, _pi), float(dsafe),
+             _tp(out), _torch_stream())
+        return out
+
+    def sdf_penalty_grad_dev(self, sdf, Q, pts, pts_per_link, dsafe=0.05, dloss=None, out=None):
+        import torch
+        ppl = _i(pts_per_link)
+        if out is None:
+            out = torch.empty_like(Q)
+        call("kg_sdf_penalty_bwd_dev", self.h, sdf.h, _tp(Q), Q.shape[0], _tp(pts), _p(ppl, _pi), float(dsafe),
+             _tp(dloss), _tp(out), _torch_stream())
+        return out
+
+
 class World:
     """World model with Klampt's ID layout and default collision mask."""
 
