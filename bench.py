@@ -36,6 +36,10 @@ def main():
     ap.add_argument("--configs", type=int, default=65536)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true")
+    ap.add_argument("--dist-configs", type=int, default=8192)
+    ap.add_argument("--edges", type=int, default=8192)
+    ap.add_argument("--traj", type=int, default=16384)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: multi-GPU runs are not part of this build "
@@ -110,6 +114,8 @@ def main():
                "gpu_over_cpu": round(value / cpu_rate, 2), "results_equal": agree}
         print(f"[bench] cpu {cpu_rate:.4g} checks/s on {nthr} threads; equal={agree}", file=sys.stderr, flush=True)
 
+    secondary = None if args.no_secondary else run_secondary(args, ctx, sc, world, robot)
+
     line = {
         "metric": "feasibility checks/sec (7-DOF, 200k-tri env)",
         "value": round(value, 1),
@@ -128,8 +134,117 @@ def main():
                    "feasible_fraction": round(feas_frac, 4)},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "secondary": secondary,
     }
     print(json.dumps(line), flush=True)
+
+
+def _time_dev(fn, reps):
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t) / reps
+
+
+def run_secondary(args, ctx, sc, world, robot):
+    """Distances (C2), edge checks (C4), SDF penalty gradients (C5),
+    heterogeneous batches (C3)."""
+    import numpy as np
+    import torch
+    from klampt_amd import engine, synthetic as S
+    out = {}
+    dev = torch.device("cuda:0")
+
+    # C2 distances: FP32 min distance per link-env and self pair
+    Qd = torch.from_numpy(S.uniform_configs(sc.robot, args.dist_configs, seed=3)).to(dev)
+    npairs = len(world.pair_list(0))
+    dd = torch.empty((args.dist_configs, npairs), dtype=torch.float32, device=dev)
+    t = _time_dev(lambda: world.distance_batch_dev(Qd, out=dd), 3)
+    out["c2_distance"] = {"configs": args.dist_configs, "pairs_per_config": npairs,
+                          "pair_distances_per_s": round(args.dist_configs * npairs / t, 1),
+                          "configs_per_s": round(args.dist_configs / t, 1), "ms": round(t * 1e3, 3)}
+    print(f"[bench] C2 distances: {out['c2_distance']}", file=sys.stderr, flush=True)
+
+    # C4 edges: 8192 segments, L-infinity step 0.01 rad, 10..500 samples
+    A, B = S.edge_segments(sc.robot, args.edges, eps=0.01, smin=10, smax=500, seed=8)
+    t0 = time.perf_counter()
+    fu = world.edge_batch(A, B, eps=0.01, norm_inf=True)
+    t1 = time.perf_counter()
+    fu = world.edge_batch(A, B, eps=0.01, norm_inf=True)
+    te = time.perf_counter() - t1
+    out["c4_edges"] = {"segments": args.edges, "edges_per_s": round(args.edges / te, 1), "ms": round(te * 1e3, 2),
+                       "free_fraction": round(float((fu < 0).mean()), 4),
+                       "note": "host-pointer call including H2D/D2H copies and per-round host sync"}
+    print(f"[bench] C4 edges: {out['c4_edges']}", file=sys.stderr, flush=True)
+
+    # C5 SDF hinge penalty, forward + backward
+    t0 = time.perf_counter()
+    P, inside = S.noisy_cloud(0, npoints=1_000_000)
+    grid = S.sdf_from_cloud(P, inside, dims=256, voxel=0.01)
+    sdf = engine.Geom.sdf(ctx, grid.bmin, grid.bmax, grid.values)
+    pts = np.concatenate([S.surface_samples(m, 64, seed=i) for i, m in enumerate(sc.robot.link_meshes)])
+    ppl = np.full(sc.robot.n, 64, np.int32)
+    Qw = torch.from_numpy(S.trajectories(sc.robot, args.traj, 32, seed=4)).to(dev)
+    dpts = torch.from_numpy(pts.astype(np.float32)).to(dev)
+    Nw = Qw.shape[0]
+    loss = torch.empty(Nw, dtype=torch.float32, device=dev)
+    dQ = torch.empty_like(Qw)
+    ones = torch.ones(Nw, dtype=torch.float32, device=dev)
+    print(f"[bench] SDF grid built in {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+
+    def fwd_bwd():
+        robot.sdf_penalty_dev(sdf, Qw, dpts, ppl, 0.05, out=loss)
+        robot.sdf_penalty_grad_dev(sdf, Qw, dpts, ppl, 0.05, dloss=ones, out=dQ)
+    tb = _time_dev(fwd_bwd, 5)
+    # plain per-configuration path: one drop-in call per waypoint
+    nplain = 512
+    q1 = [Qw[i:i + 1] for i in range(nplain)]
+    l1 = torch.empty(1, dtype=torch.float32, device=dev)
+    g1 = torch.empty((1, sc.robot.n), dtype=torch.float32, device=dev)
+    o1 = torch.ones(1, dtype=torch.float32, device=dev)
+
+    def plain():
+        for q in q1:
+            robot.sdf_penalty_dev(sdf, q, dpts, ppl, 0.05, out=l1)
+            robot.sdf_penalty_grad_dev(sdf, q, dpts, ppl, 0.05, dloss=o1, out=g1)
+    tp = _time_dev(plain, 1)
+    bytes_per_wp = 448 * 8 * 4 * 2 + 28 + 28
+    out["c5_sdf_grad"] = {"waypoints": Nw, "evals_per_s": round(Nw / tb, 1), "ms": round(tb * 1e3, 3),
+                          "plain_evals_per_s": round(nplain / tp, 1),
+                          "batched_over_plain": round((Nw / tb) / (nplain / tp), 1),
+                          "active_fraction": round(float((loss > 0).float().mean().item()), 4),
+                          "roofline": {"bound": "hbm", "achieved": round(Nw * bytes_per_wp / tb / 1e9, 1),
+                                       "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                       "frac": round(Nw * bytes_per_wp / tb / 1e9 / HBM_PEAK_GBS, 4),
+                                       "bytes_per_waypoint": bytes_per_wp}}
+    print(f"[bench] C5 SDF: {out['c5_sdf_grad']}", file=sys.stderr, flush=True)
+
+    # C3 heterogeneous robots in one launch
+    soup_geom = world._keep[0]
+    robots, counts = S.hetero_robots(seed=0, nrobots=64)
+    worlds = []
+    for spec in robots:
+        w = engine.World(ctx)
+        w.add_terrain(soup_geom)
+        w.add_robot(engine.Robot.from_spec(ctx, spec))
+        worlds.append(w)
+    Qh = torch.from_numpy(np.concatenate([S.uniform_configs(sp, c, seed=20 + i).ravel()
+                                          for i, (sp, c) in enumerate(zip(robots, counts))])).to(dev)
+    choice = engine.autotune(worlds, [0] * 64, counts, Qh)
+    fh = torch.empty(int(sum(counts)), dtype=torch.uint8, device=dev)
+    th = _time_dev(lambda: engine.feasible_multi_dev(worlds, [0] * 64, counts, Qh, out=fh), 3)
+    out["c3_hetero"] = {"robots": 64, "configs": int(sum(counts)), "checks_per_s": round(sum(counts) / th, 1),
+                        "ms": round(th * 1e3, 3), "dof_range": [int(min(r.n for r in robots)), int(max(r.n for r in robots))],
+                        "autotune": {"cpw": choice[0], "leaf_min": choice[1], "scap": choice[2],
+                                     "waves_per_cu": choice[3]},
+                        "feasible_fraction": round(float(fh.float().mean().item()), 4)}
+    engine.set_tuning(ctx)
+    print(f"[bench] C3 hetero: {out['c3_hetero']}", file=sys.stderr, flush=True)
+    return out
 
 
 if __name__ == "__main__":

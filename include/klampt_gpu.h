@@ -2,10 +2,15 @@
  *
  * Plain pointers and sizes only.  Every function returns KG_OK (0) or a
  * negative KG_E* code; the message of the last error on the calling thread
- * is available from kg_last_error().  Handles are not thread-safe; one HIP
- * stream per context.  Host arrays passed in are copied; batch outputs are
- * written to caller memory.  Functions suffixed _dev take device pointers
- * and a hipStream_t (as void*) and run asynchronously on it.
+ * is available from kg_last_error().  Handles are not thread-safe.  Host
+ * arrays passed in are copied; batch outputs are written to caller memory
+ * and host-pointer calls return when they are complete (they run on a
+ * private stream of the context).  Functions suffixed _dev take device
+ * pointers and a hipStream_t (as void*; NULL is the default stream) and run
+ * asynchronously on it: their inputs must be ready in stream order, and
+ * their outputs are ready in that stream's order.  All calls of one context
+ * share its workspaces and execute in call order, whatever streams they
+ * name: a call on a different stream waits for the previous call's work.
  *
  * Transforms: R is a column-major 3x3 rotation (9 doubles) and t a 3-vector,
  * as in Klampt (Python/klampt/math/so3.py:1-16).  A link transform array is
@@ -182,6 +187,25 @@ int kg_sdf_penalty_fwd_dev(kg_robot r, kg_geom sdf, const float* dQ, int N, cons
 int kg_sdf_penalty_bwd_dev(kg_robot r, kg_geom sdf, const float* dQ, int N, const float* d_local_pts,
                            const int32_t* pts_per_link, float dsafe, const float* d_dloss, float* d_dQ,
                            void* stream);
+
+/* Heterogeneous batch: robot robot_indices[b] of worlds[b] at counts[b]
+ * configurations each (b < nbatch), checked as IsFeasible in one broad-phase
+ * launch and one persistent traversal launch.  Q is the concatenation of the
+ * per-batch configuration arrays (counts[b] x n_b floats), feasible the
+ * concatenation of the outputs.  No Klampt counterpart: it batches many
+ * SingleRobotCSpace::IsFeasible calls (Cpp/Planning/RobotCSpace.cpp:786). */
+int kg_feasible_multi(int nbatch, const kg_world* worlds, const int32_t* robot_indices, const int32_t* counts,
+                      const float* Q, uint8_t* feasible);
+int kg_feasible_multi_dev(int nbatch, const kg_world* worlds, const int32_t* robot_indices, const int32_t* counts,
+                          const float* dQ, uint8_t* d_feasible, void* stream);
+/* Autotuner: times the traversal settings (configurations per wave chunk,
+ * leaf batch threshold, LDS pool capacity, waves per CU) on the given batch
+ * (device Q; the device is synchronized first, so dQ may come from any
+ * stream) and keeps the fastest on the context.  chosen[4] (nullable)
+ * receives {cpw, leaf_min, scap, waves_per_cu (0 = occupancy limit)}. */
+int kg_autotune(int nbatch, const kg_world* worlds, const int32_t* robot_indices, const int32_t* counts,
+                const float* dQ, int32_t* chosen, float* best_ms);
+int kg_ctx_set_tuning(kg_ctx ctx, int cpw, int leaf_min, int scap, int waves_per_cu);
 
 /* Counters of the last batched call on this world: [0] configurations
  * re-checked in FP64, [1] narrow-phase pair items, [2] spill events,

@@ -80,6 +80,10 @@ SIGNATURES = {
     "kg_sdf_penalty_fwd_dev": [_vp, _vp, _vp, _c_int, _vp, _pi, _c_float, _vp, _vp],
     "kg_sdf_penalty_bwd_dev": [_vp, _vp, _vp, _c_int, _vp, _pi, _c_float, _vp, _vp, _vp],
     "kg_world_last_stats": [_vp, _pi64, _c_int],
+    "kg_feasible_multi": [_c_int, _pvp, _pi, _pi, _pf, _pu8],
+    "kg_feasible_multi_dev": [_c_int, _pvp, _pi, _pi, _vp, _vp, _vp],
+    "kg_autotune": [_c_int, _pvp, _pi, _pi, _vp, _pi, _pf],
+    "kg_ctx_set_tuning": [_vp, _c_int, _c_int, _c_int, _c_int],
 }
 _RESTYPE = {"kg_last_error": ctypes.c_char_p}
 

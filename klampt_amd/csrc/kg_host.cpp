@@ -132,11 +132,39 @@ double norm3(const double* t) { return std::sqrt(t[0] * t[0] + t[1] * t[1] + t[2
 
 // ---------------------------------------------------------------- pool
 
+// Tables (geometry pool, plans, batch layouts) change only when the scene or
+// the batch shape changes.  Kernels queued earlier on any stream may still
+// read the old contents, so the device is drained before the copy.
 template <class T>
 void upload(DevBuf& b, const std::vector<T>& v) {
+  hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
   hip_check(b.ensure(std::max<size_t>(sizeof(T), v.size() * sizeof(T))), "hipMalloc(pool)");
   if (!v.empty()) hip_check(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy(pool)");
 }
+
+// Every call of a context shares its workspaces, so all of its GPU work forms
+// one sequence whatever stream each call names.  A call on a stream other
+// than the previous call's first makes its stream wait for that call's end
+// event; when the call returns (or throws) its own end is recorded.  The
+// event, not the stream, is kept, so a caller may destroy a stream after use.
+struct StreamScope {
+  kg_ctx_s* c;
+  hipStream_t s;
+  StreamScope(kg_ctx_s* c_, hipStream_t s_) : c(c_), s(s_) {
+    hip_check(hipSetDevice(c->device), "hipSetDevice");
+    if (!c->evDone) hip_check(hipEventCreateWithFlags(&c->evDone, hipEventDisableTiming), "hipEventCreate");
+    if (c->done_valid && c->done_stream != s) hip_check(hipStreamWaitEvent(s, c->evDone, 0), "hipStreamWaitEvent");
+  }
+  ~StreamScope() {
+    if (hipEventRecord(c->evDone, s) == hipSuccess) {
+      c->done_stream = s;
+      c->done_valid = true;
+    }
+  }
+  StreamScope(const StreamScope&) = delete;
+  StreamScope& operator=(const StreamScope&) = delete;
+};
+
 
 void sync_pool(kg_ctx_s* c) {
   if (!c->pool_dirty) return;
@@ -375,28 +403,43 @@ struct BatchStats {
 //   [3] FP64 chunk counter  [4..9] u64 counters: box tests, triangle tests,
 //   pair items.
 enum { ST_SPILL = 0, ST_REDO = 1, ST_CHUNK32 = 2, ST_CHUNK64 = 3, ST_CNT = 4, ST_WORDS = 16 };
-
-// Batched feasibility on device buffers, asynchronous on stream s.  Q is
-// float (Qf) or double (Qd).
-void feasible_dev(kg_world_s* w, int ri, const float* Qf, const double* Qd, int N, uint8_t* feas, uint8_t* env,
-                  uint8_t* self, int32_t* fp, hipStream_t s) {
-  kg_ctx_s* c = w->ctx;
-  hip_check(hipSetDevice(c->device), "hipSetDevice");
-  RobotPlan& P = get_plan(w, ri);
+void ensure_status_host(kg_ctx_s* c) {
   if (!c->hstatus) {
     hip_check(hipHostMalloc((void**)&c->hstatus, ST_WORDS * sizeof(int), hipHostMallocDefault), "hipHostMalloc");
     for (auto& e : c->ev) hip_check(hipEventCreate(&e), "hipEventCreate");
+    hip_check(hipEventCreateWithFlags(&c->evStat, hipEventDisableTiming), "hipEventCreate");
   }
+}
+
+// Layout of one batched feasibility call: a single robot plan, or one plan
+// per configuration (heterogeneous batches, device tables in `multi`).
+struct Layout {
+  const RobotPlan* P = nullptr;     // single-plan batches
+  bool multi = false;
+  const PlanDev* plans = nullptr;   // device, heterogeneous batches
+  const int* cfg_plan = nullptr;
+  const int* qoff = nullptr;
+  const int* toff = nullptr;
+  const int* ibase = nullptr;
+  size_t nrows = 0;                 // frame rows (links) over the batch
+  size_t ncap = 0;                  // item capacity over the batch
+  int cap = 0;                      // per-configuration capacity (single)
+  Tol tol32{}, tol64{};
+};
+
+void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* Qd, int N, uint8_t* feas,
+                   uint8_t* env, uint8_t* self, int32_t* fp, hipStream_t s) {
+  hip_check(hipSetDevice(c->device), "hipSetDevice");
+  ensure_status_host(c);
   c->last_N = N;
-  c->last_stream = s;
+  c->has_batch = true;
   if (N <= 0) {
     std::memset(c->hstatus, 0, ST_WORDS * sizeof(int));
     return;
   }
-  const int n = P.dev.nlinks;
   const int mode = fp ? KG_MODE_ORDERED : ((env || self) ? KG_MODE_GROUPS : KG_MODE_ANY);
-  hip_check(c->wsT64.ensure((size_t)N * n * 12 * sizeof(double)), "hipMalloc(T64)");
-  hip_check(c->wsLB.ensure((size_t)N * n * 6 * sizeof(double)), "hipMalloc(LB)");
+  hip_check(c->wsT64.ensure(L.nrows * 12 * sizeof(double)), "hipMalloc(T64)");
+  hip_check(c->wsLB.ensure(L.nrows * 6 * sizeof(double)), "hipMalloc(LB)");
   hip_check(c->wsJL.ensure((size_t)N), "hipMalloc(jl)");
   hip_check(c->wsCounts.ensure((size_t)(N + 1) * sizeof(int)), "hipMalloc(counts)");
   hip_check(c->wsOffsets.ensure((size_t)(N + 1) * sizeof(int)), "hipMalloc(offsets)");
@@ -409,7 +452,7 @@ void feasible_dev(kg_world_s* w, int ri, const float* Qf, const double* Qd, int 
   pa.Qd = Qd;
   pa.N = N;
   pa.mode = mode;
-  pa.plan = P.dev;
+  if (L.P) pa.plan = L.P->dev;
   pa.pool = c->pool;
   pa.T64 = c->wsT64.as<double>();
   pa.LB = c->wsLB.as<double>();
@@ -417,16 +460,18 @@ void feasible_dev(kg_world_s* w, int ri, const float* Qf, const double* Qd, int 
   pa.counts = c->wsCounts.as<int>();
   pa.reset = status;
   pa.nreset = ST_WORDS;
-  int nact = 0;
-  for (int l = 0; l < n; l++)
-    if (w->robots[ri]->link_geoms[l] && !geom_empty(w->robots[ri]->link_geoms[l])) nact++;
-  const size_t cap = (size_t)nact * P.nenv + P.nself;
-  const bool fixed = cap * (size_t)N * sizeof(PairItem) <= (512u << 20);
+  pa.plans = L.plans;
+  pa.cfg_plan = L.cfg_plan;
+  pa.qoff = L.qoff;
+  pa.toff = L.toff;
+  pa.ibase = L.ibase;
+  const bool fixed = L.ncap * sizeof(PairItem) <= (2048ull << 20);
   hip_check(hipEventRecord(c->ev[0], s), "hipEventRecord");
-  if (fixed) {
-    hip_check(c->wsItems.ensure(std::max<size_t>(1, cap * (size_t)N) * sizeof(PairItem)), "hipMalloc(items)");
+  if (fixed || L.multi) {
+    need(fixed, "heterogeneous batch exceeds the item capacity limit (2 GiB of pair items)");
+    hip_check(c->wsItems.ensure(std::max<size_t>(1, L.ncap) * sizeof(PairItem)), "hipMalloc(items)");
     pa.items = c->wsItems.as<PairItem>();
-    pa.cap = (int)cap;
+    pa.cap = L.cap;
     pa.offsets = nullptr;
     hip_check(kgl_prep_single(pa, s), "prep");
   } else {
@@ -445,8 +490,10 @@ void feasible_dev(kg_world_s* w, int ri, const float* Qf, const double* Qd, int 
   }
   hip_check(hipEventRecord(c->ev[1], s), "hipEventRecord");
 
-  const int grid32 = kgl_collide_grid(0, c->num_cus);
-  const int grid64 = std::min(kgl_collide_grid(1, c->num_cus), std::max(1, c->num_cus));
+  const int scap = env_int("KG_SCAP", c->tune_scap ? c->tune_scap : 1024) == 512 ? 512 : 1024;
+  const int wpc = env_int("KG_WAVES_PER_CU", c->tune_wpc);
+  const int grid32 = wpc > 0 ? wpc * c->num_cus : kgl_collide_grid(0, c->num_cus, scap);
+  const int grid64 = std::min(kgl_collide_grid(1, c->num_cus, scap), std::max(1, c->num_cus));
   const int spill_cap = env_int("KG_SPILL_CAP", 16384);
   hip_check(c->wsBest.ensure((size_t)N * 2 * sizeof(unsigned long long)), "hipMalloc(best)");
   hip_check(c->wsUnc.ensure((size_t)N * 2 * sizeof(int)), "hipMalloc(unc)");
@@ -454,7 +501,7 @@ void feasible_dev(kg_world_s* w, int ri, const float* Qf, const double* Qd, int 
 
   TravArgs ta{};
   ta.pool = c->pool;
-  ta.plan = P.dev;
+  if (L.P) ta.plan = L.P->dev;
   ta.T64 = pa.T64;
   ta.items = pa.items;
   ta.offsets = pa.offsets;
@@ -464,7 +511,7 @@ void feasible_dev(kg_world_s* w, int ri, const float* Qf, const double* Qd, int 
   ta.ncfg = N;
   ta.ncfg_dev = nullptr;
   ta.mode = mode;
-  ta.tol = P.tol32;
+  ta.tol = L.tol32;
   ta.best = c->wsBest.as<unsigned long long>();
   ta.unc = c->wsUnc.as<int>();
   ta.chunk_counter = status + ST_CHUNK32;
@@ -472,10 +519,15 @@ void feasible_dev(kg_world_s* w, int ri, const float* Qf, const double* Qd, int 
   ta.spill_cap = spill_cap;
   ta.stack_cap = 0;
   ta.status = status + ST_SPILL;
-  ta.cpw = std::max(1, std::min(64, env_int("KG_CPW", 16)));
-  ta.leaf_min = std::max(1, std::min(64, env_int("KG_LEAF_MIN", 1)));
+  ta.cpw = std::max(1, std::min(64, env_int("KG_CPW", c->tune_cpw ? c->tune_cpw : 16)));
+  ta.leaf_min = std::max(1, std::min(64, env_int("KG_LEAF_MIN", c->tune_leaf ? c->tune_leaf : 1)));
   ta.grid = grid32;
   ta.counters = (unsigned long long*)(status + ST_CNT);
+  ta.plans = L.plans;
+  ta.cfg_plan = L.cfg_plan;
+  ta.toff = L.toff;
+  ta.ibase = L.ibase;
+  ta.scap = scap;
   hip_check(hipEventRecord(c->ev[2], s), "hipEventRecord");
   hip_check(kgl_collide_f32(ta, s), "collide_f32");
   hip_check(hipEventRecord(c->ev[3], s), "hipEventRecord");
@@ -487,7 +539,9 @@ void feasible_dev(kg_world_s* w, int ri, const float* Qf, const double* Qd, int 
   fa.best = ta.best;
   fa.unc = ta.unc;
   fa.items = pa.items;
-  fa.plan = P.dev;
+  if (L.P) fa.plan = L.P->dev;
+  fa.plans = L.plans;
+  fa.cfg_plan = L.cfg_plan;
   fa.feasible = feas;
   fa.env_hit = env;
   fa.self_hit = self;
@@ -502,7 +556,7 @@ void feasible_dev(kg_world_s* w, int ri, const float* Qf, const double* Qd, int 
   TravArgs tb = ta;
   tb.cfg_list = redo_list;
   tb.ncfg_dev = status + ST_REDO;
-  tb.tol = P.tol64;
+  tb.tol = L.tol64;
   tb.cpw = 1;
   tb.grid = grid64;
   tb.counters = nullptr;
@@ -514,15 +568,36 @@ void feasible_dev(kg_world_s* w, int ri, const float* Qf, const double* Qd, int 
   fb.final_pass = 1;
   hip_check(kgl_finalize_list(fb, redo_list, status + ST_REDO, N, s), "finalize_f64");
   hip_check(hipMemcpyAsync(c->hstatus, status, ST_WORDS * sizeof(int), hipMemcpyDeviceToHost, s), "copy status");
+  hip_check(hipEventRecord(c->evStat, s), "hipEventRecord");
+}
+
+int active_links(const kg_robot_s* r) {
+  int nact = 0;
+  for (int l = 0; l < r->n; l++)
+    if (r->link_geoms[l] && !geom_empty(r->link_geoms[l])) nact++;
+  return nact;
+}
+
+// Batched feasibility of one robot on device buffers (async on stream s).
+void feasible_dev(kg_world_s* w, int ri, const float* Qf, const double* Qd, int N, uint8_t* feas, uint8_t* env,
+                  uint8_t* self, int32_t* fp, hipStream_t s) {
+  RobotPlan& P = get_plan(w, ri);
+  Layout L;
+  L.P = &P;
+  L.cap = active_links(w->robots[ri]) * P.nenv + P.nself;
+  L.nrows = (size_t)std::max(N, 0) * P.dev.nlinks;
+  L.ncap = (size_t)L.cap * std::max(N, 0);
+  L.tol32 = P.tol32;
+  L.tol64 = P.tol64;
+  feasible_core(w->ctx, L, Qf, Qd, N, feas, env, self, fp, s);
 }
 
 // Read the status of the last batch (waits for it).
 void read_stats(kg_world_s* w, BatchStats& st) {
   kg_ctx_s* c = w->ctx;
   st = BatchStats{};
-  if (!c->hstatus || !c->last_stream) return;
-  hip_check(hipStreamSynchronize(c->last_stream), "sync");
-  if (c->last_N <= 0) return;
+  if (!c->hstatus || !c->has_batch || c->last_N <= 0) return;
+  hip_check(hipEventSynchronize(c->evStat), "sync");
   const int* h = c->hstatus;
   st.spills = h[ST_SPILL];
   st.redo = h[ST_REDO];
@@ -582,15 +657,20 @@ int kg_ctx_destroy(kg_ctx c) {
   return guard([&] {
     if (!c) return;
     (void)hipSetDevice(c->device);
-    (void)hipStreamSynchronize(c->stream);
+    (void)hipDeviceSynchronize();
     DevBuf* bufs[] = {&c->dnodes, &c->dtv0, &c->dtv1, &c->dtv2, &c->dtv64, &c->dgeoms, &c->dtrisrc,
                       &c->wsQ, &c->wsT64, &c->wsLB, &c->wsJL, &c->wsCounts, &c->wsOffsets, &c->wsItems,
                       &c->wsScanTmp, &c->wsBest, &c->wsUnc, &c->wsChunk, &c->wsSpill, &c->wsStatus, &c->wsRedo,
-                      &c->wsOut8a, &c->wsOut8b, &c->wsOut8c, &c->wsOut32, &c->wsF, &c->wsG};
+                      &c->wsOut8a, &c->wsOut8b, &c->wsOut8c, &c->wsOut32, &c->wsF, &c->wsG,
+                      &c->wsSdfScratch, &c->wsPoff, &c->wsPts, &c->wsLoss, &c->wsDQ, &c->wsDl, &c->wsEA, &c->wsEB,
+                      &c->wsESeg, &c->wsESoff, &c->wsENs, &c->wsEOoff, &c->wsEOrder, &c->wsEQ, &c->wsEFeas,
+                      &c->wsEU, &c->wsEDone, &c->mPlans, &c->mCfgPlan, &c->mQoff, &c->mToff, &c->mIbase};
     for (auto* b : bufs) b->release();
     if (c->hstatus) (void)hipHostFree(c->hstatus);
     for (auto& e : c->ev)
       if (e) (void)hipEventDestroy(e);
+    if (c->evStat) (void)hipEventDestroy(c->evStat);
+    if (c->evDone) (void)hipEventDestroy(c->evDone);
     (void)hipStreamDestroy(c->stream);
     delete c;
   });
@@ -599,6 +679,8 @@ int kg_ctx_destroy(kg_ctx c) {
 int kg_ctx_synchronize(kg_ctx c) {
   return guard([&] {
     need(c != nullptr, "ctx is NULL");
+    hip_check(hipSetDevice(c->device), "hipSetDevice");
+    if (c->done_valid) hip_check(hipEventSynchronize(c->evDone), "hipEventSynchronize");
     hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
   });
 }
@@ -865,7 +947,8 @@ int kg_fk_batch_dev(kg_robot r, const float* dQ, int N, float* dT, void* stream)
       upload(r->dlinks, r->links);
       r->dirty = false;
     }
-    hipStream_t s = stream ? (hipStream_t)stream : r->ctx->stream;
+    hipStream_t s = (hipStream_t)stream;
+    StreamScope scope(r->ctx, s);
     hip_check(kgl_fk_f32(r->dlinks.as<LinkDesc>(), r->n, dQ, N, dT, s), "fk_f32");
   });
 }
@@ -878,6 +961,7 @@ int kg_fk_batch(kg_robot r, const float* Q, int N, float* T) {
     need(Q && T, "NULL buffer");
     kg_ctx_s* c = r->ctx;
     hip_check(hipSetDevice(c->device), "hipSetDevice");
+    StreamScope scope(c, c->stream);
     size_t qb = (size_t)N * r->n * sizeof(float), tb = qb * 12;
     hip_check(c->wsQ.ensure(qb), "hipMalloc");
     hip_check(c->wsF.ensure(tb), "hipMalloc");
@@ -1003,7 +1087,8 @@ int kg_feasible_batch_dev(kg_world w, int ri, const float* dQ, int N, uint8_t* f
     need(w != nullptr, "world is NULL");
     need(N >= 0, "negative batch size");
     need(N == 0 || (dQ && feas), "NULL buffer");
-    hipStream_t s = stream ? (hipStream_t)stream : w->ctx->stream;
+    hipStream_t s = (hipStream_t)stream;
+    StreamScope scope(w->ctx, s);
     feasible_dev(w, ri, dQ, nullptr, N, feas, env, self, fp, s);
   });
 }
@@ -1018,6 +1103,7 @@ int kg_feasible_batch(kg_world w, int ri, const float* Q, int N, uint8_t* feas, 
     need(Q && feas, "NULL buffer");
     kg_ctx_s* c = w->ctx;
     hip_check(hipSetDevice(c->device), "hipSetDevice");
+    StreamScope scope(c, c->stream);
     int n = w->robots[ri]->n;
     size_t qb = (size_t)N * n * sizeof(float);
     hip_check(c->wsQ.ensure(qb), "hipMalloc");
@@ -1078,6 +1164,7 @@ extern "C" int kg_robot_self_collides(kg_robot r, int* out) {
     // the current configuration in FP64
     kg_ctx_s* c = r->ctx;
     hip_check(hipSetDevice(c->device), "hipSetDevice");
+    StreamScope scope(c, c->stream);
     hip_check(c->wsG.ensure(r->n * sizeof(double) + 64), "hipMalloc");
     hip_check(hipMemcpyAsync(c->wsG.p, r->q.data(), r->n * sizeof(double), hipMemcpyHostToDevice, c->stream),
               "copy q");
@@ -1160,6 +1247,7 @@ extern "C" int kg_geom_collide(kg_geom a, kg_geom b, int* out) {
     kg_ctx_s* c = a->ctx;
     PairWorld& pw = pair_world(c, a, b);
     hip_check(hipSetDevice(c->device), "hipSetDevice");
+    StreamScope scope(c, c->stream);
     hip_check(c->wsG.ensure(64), "hipMalloc");
     double q0 = 0.0;
     hip_check(hipMemcpyAsync(c->wsG.p, &q0, sizeof(double), hipMemcpyHostToDevice, c->stream), "copy q");
@@ -1255,6 +1343,7 @@ extern "C" int kg_sdf_eval(kg_geom sdf, const float* pts, int n, float* d, float
     need(pts && d, "NULL buffer");
     kg_ctx_s* c = sdf->ctx;
     hip_check(hipSetDevice(c->device), "hipSetDevice");
+    StreamScope scope(c, c->stream);
     hip_check(c->wsPts.ensure((size_t)n * 3 * sizeof(float)), "hipMalloc");
     hip_check(c->wsLoss.ensure((size_t)n * sizeof(float)), "hipMalloc");
     hip_check(c->wsDQ.ensure((size_t)n * 3 * sizeof(float)), "hipMalloc");
@@ -1298,7 +1387,8 @@ extern "C" int kg_sdf_penalty_fwd_dev(kg_robot r, kg_geom sdf, const float* dQ, 
                                       const int32_t* ppl, float dsafe, float* dloss, void* stream) {
   return guard([&] {
     need(r != nullptr, "robot is NULL");
-    hipStream_t s = stream ? (hipStream_t)stream : r->ctx->stream;
+    hipStream_t s = (hipStream_t)stream;
+    StreamScope scope(r->ctx, s);
     sdf_penalty_dev(r, sdf, dQ, N, dpts, ppl, dsafe, dloss, nullptr, nullptr, false, s);
   });
 }
@@ -1307,7 +1397,8 @@ extern "C" int kg_sdf_penalty_bwd_dev(kg_robot r, kg_geom sdf, const float* dQ, 
                                       const int32_t* ppl, float dsafe, const float* ddl, float* ddQ, void* stream) {
   return guard([&] {
     need(r != nullptr, "robot is NULL");
-    hipStream_t s = stream ? (hipStream_t)stream : r->ctx->stream;
+    hipStream_t s = (hipStream_t)stream;
+    StreamScope scope(r->ctx, s);
     sdf_penalty_dev(r, sdf, dQ, N, dpts, ppl, dsafe, nullptr, ddl, ddQ, true, s);
   });
 }
@@ -1320,6 +1411,7 @@ static void sdf_penalty_host(kg_robot_s* r, kg_geom_s* sdf, const float* Q, int 
   need(Q && pts, "NULL buffer");
   kg_ctx_s* c = r->ctx;
   hip_check(hipSetDevice(c->device), "hipSetDevice");
+  StreamScope scope(c, c->stream);
   const int n = r->n, P = total_points(r, ppl);
   hip_check(c->wsQ.ensure((size_t)N * n * sizeof(float)), "hipMalloc");
   hip_check(c->wsPts.ensure((size_t)std::max(1, P) * 3 * sizeof(float)), "hipMalloc");
@@ -1397,16 +1489,11 @@ void distance_dev(kg_world_s* w, int ri, const float* Qf, const double* Qd, int 
   hip_check(kgl_distance(da, s), "distance");
   hip_check(hipEventRecord(c->ev[3], s), "hipEventRecord");
   c->last_N = N;
-  c->last_stream = s;
+  c->has_batch = true;
   hip_check(hipMemcpyAsync(c->hstatus, status, ST_WORDS * sizeof(int), hipMemcpyDeviceToHost, s), "copy status");
+  hip_check(hipEventRecord(c->evStat, s), "hipEventRecord");
 }
 
-void ensure_status_host(kg_ctx_s* c) {
-  if (!c->hstatus) {
-    hip_check(hipHostMalloc((void**)&c->hstatus, ST_WORDS * sizeof(int), hipHostMallocDefault), "hipHostMalloc");
-    for (auto& e : c->ev) hip_check(hipEventCreate(&e), "hipEventCreate");
-  }
-}
 
 // FP64 distance and closest points of one triangle pair (original indices)
 double pair_tri_dist(const kg_geom_s* a, int ta, const kg_geom_s* b, int tb, double* cp1, double* cp2) {
@@ -1435,6 +1522,7 @@ void pair_distance(kg_geom_s* a, kg_geom_s* b, float stop_below, double upper, k
   ensure_status_host(c);
   PairWorld& pw = pair_world(c, a, b);
   hip_check(hipSetDevice(c->device), "hipSetDevice");
+  StreamScope scope(c, c->stream);
   hip_check(c->wsG.ensure(64), "hipMalloc");
   hip_check(c->wsOut32.ensure(64), "hipMalloc");
   double q0 = 0.0;
@@ -1485,7 +1573,8 @@ extern "C" int kg_distance_batch_dev(kg_world w, int ri, const float* dQ, int N,
     need(N >= 0, "negative batch size");
     need(N == 0 || (dQ && dout), "NULL buffer");
     ensure_status_host(w->ctx);
-    hipStream_t s = stream ? (hipStream_t)stream : w->ctx->stream;
+    hipStream_t s = (hipStream_t)stream;
+    StreamScope scope(w->ctx, s);
     distance_dev(w, ri, dQ, nullptr, N, dout, nullptr, 0.0f, 3.0e38f, s);
   });
 }
@@ -1500,6 +1589,7 @@ extern "C" int kg_distance_batch(kg_world w, int ri, const float* Q, int N, floa
     kg_ctx_s* c = w->ctx;
     ensure_status_host(c);
     hip_check(hipSetDevice(c->device), "hipSetDevice");
+    StreamScope scope(c, c->stream);
     RobotPlan& P = get_plan(w, ri);
     const int n = w->robots[ri]->n, np = (int)P.pairs.size();
     if (np == 0) return;
@@ -1573,6 +1663,7 @@ extern "C" int kg_edge_batch(kg_world w, int ri, const float* A, const float* B,
     kg_ctx_s* c = w->ctx;
     ensure_status_host(c);
     hip_check(hipSetDevice(c->device), "hipSetDevice");
+    StreamScope scope(c, c->stream);
     const int n = w->robots[ri]->n;
     // sample counts n_s = ceil(d/eps) (robotcspace.py:216-228), d in L2 or Linf
     std::vector<int> ns(nseg), ooff(nseg);
@@ -1655,5 +1746,196 @@ extern "C" int kg_edge_batch(kg_world w, int ri, const float* A, const float* B,
     }
     hip_check(hipMemcpyAsync(first_u, c->wsEU.p, nseg * sizeof(float), hipMemcpyDeviceToHost, st), "copy");
     hip_check(hipStreamSynchronize(st), "sync");
+  });
+}
+
+// ---------------------------------------------------------------- heterogeneous batches
+namespace {
+
+struct MultiBatch {
+  std::vector<RobotPlan*> plans;
+  std::vector<int> nlinks;
+  size_t N = 0;
+};
+
+// Build (or reuse) the per-configuration layout tables of a heterogeneous
+// batch: plan index, Q offset, frame offset, item base.
+Layout multi_layout(kg_ctx_s* c, int nb, kg_world_s* const* worlds, const int32_t* ris, const int32_t* counts,
+                    hipStream_t s, size_t* Nout) {
+  std::vector<PlanDev> devplans(nb);
+  std::vector<uint64_t> sig;
+  size_t N = 0, rows = 0, caps = 0;
+  float pad_rel = 0, pad_bv64 = 0;
+  Tol t32{}, t64{};
+  std::vector<int> caps_b(nb), n_b(nb);
+  for (int b = 0; b < nb; b++) {
+    need(worlds[b] != nullptr, "NULL world");
+    need(worlds[b]->ctx == c, "worlds belong to different contexts");
+    need(counts[b] >= 0, "negative configuration count");
+    RobotPlan& P = get_plan(worlds[b], ris[b]);
+    devplans[b] = P.dev;
+    n_b[b] = P.dev.nlinks;
+    caps_b[b] = active_links(worlds[b]->robots[ris[b]]) * P.nenv + P.nself;
+    N += counts[b];
+    rows += (size_t)counts[b] * n_b[b];
+    caps += (size_t)counts[b] * caps_b[b];
+    pad_rel = std::max(pad_rel, P.tol32.pad_rel);
+    pad_bv64 = std::max(pad_bv64, P.tol64.pad_bv);
+    t32 = P.tol32;
+    sig.push_back((uint64_t)(uintptr_t)worlds[b]);
+    sig.push_back(((uint64_t)ris[b] << 32) | (uint32_t)counts[b]);
+    sig.push_back((uint64_t)P.built_version);
+  }
+  need(N < (1ull << 31) && rows < (1ull << 31) && caps < (1ull << 31), "heterogeneous batch too large");
+  t32.pad_rel = pad_rel;
+  t64 = t32;
+  t64.ev = 0;
+  t64.tau = 0;
+  t64.pad_bv = pad_bv64;
+  if (sig != c->mSig) {
+    std::vector<int> cfg_plan(N), qoff(N), toff(N), ibase(N);
+    size_t k = 0, q = 0, t = 0, ib = 0;
+    for (int b = 0; b < nb; b++)
+      for (int i = 0; i < counts[b]; i++, k++) {
+        cfg_plan[k] = b;
+        qoff[k] = (int)q;
+        toff[k] = (int)t;
+        ibase[k] = (int)ib;
+        q += n_b[b];
+        t += n_b[b];
+        ib += caps_b[b];
+      }
+    upload(c->mCfgPlan, cfg_plan);
+    upload(c->mQoff, qoff);
+    upload(c->mToff, toff);
+    upload(c->mIbase, ibase);
+    c->mSig = sig;
+  }
+  if (devplans.size() != c->mPlansHost.size() ||
+      std::memcmp(devplans.data(), c->mPlansHost.data(), devplans.size() * sizeof(PlanDev)) != 0) {
+    upload(c->mPlans, devplans);
+    c->mPlansHost = devplans;
+  }
+  Layout L;
+  L.multi = true;
+  L.plans = c->mPlans.as<PlanDev>();
+  L.cfg_plan = c->mCfgPlan.as<int>();
+  L.qoff = c->mQoff.as<int>();
+  L.toff = c->mToff.as<int>();
+  L.ibase = c->mIbase.as<int>();
+  L.nrows = rows;
+  L.ncap = caps;
+  L.tol32 = t32;
+  L.tol64 = t64;
+  (void)s;
+  *Nout = N;
+  return L;
+}
+
+}  // namespace
+
+extern "C" int kg_feasible_multi_dev(int nb, const kg_world* worlds, const int32_t* ris, const int32_t* counts,
+                                     const float* dQ, uint8_t* dfeas, void* stream) {
+  return guard([&] {
+    need(nb >= 1 && worlds && ris && counts, "NULL argument");
+    kg_ctx_s* c = worlds[0]->ctx;
+    hipStream_t s = (hipStream_t)stream;
+    StreamScope scope(c, s);
+    size_t N = 0;
+    Layout L = multi_layout(c, nb, worlds, ris, counts, s, &N);
+    need(N == 0 || (dQ && dfeas), "NULL buffer");
+    feasible_core(c, L, dQ, nullptr, (int)N, dfeas, nullptr, nullptr, nullptr, s);
+  });
+}
+
+extern "C" int kg_feasible_multi(int nb, const kg_world* worlds, const int32_t* ris, const int32_t* counts,
+                                 const float* Q, uint8_t* feas) {
+  return guard([&] {
+    need(nb >= 1 && worlds && ris && counts, "NULL argument");
+    kg_ctx_s* c = worlds[0]->ctx;
+    hip_check(hipSetDevice(c->device), "hipSetDevice");
+    StreamScope scope(c, c->stream);
+    size_t N = 0, nq = 0;
+    Layout L = multi_layout(c, nb, worlds, ris, counts, c->stream, &N);
+    for (int b = 0; b < nb; b++) nq += (size_t)counts[b] * worlds[b]->robots[ris[b]]->n;
+    if (N == 0) return;
+    need(Q && feas, "NULL buffer");
+    hip_check(c->wsQ.ensure(nq * sizeof(float)), "hipMalloc");
+    hip_check(c->wsOut8a.ensure(N), "hipMalloc");
+    hip_check(hipMemcpyAsync(c->wsQ.p, Q, nq * sizeof(float), hipMemcpyHostToDevice, c->stream), "copy Q");
+    feasible_core(c, L, c->wsQ.as<float>(), nullptr, (int)N, c->wsOut8a.as<uint8_t>(), nullptr, nullptr, nullptr,
+                  c->stream);
+    hip_check(hipMemcpyAsync(feas, c->wsOut8a.p, N, hipMemcpyDeviceToHost, c->stream), "copy out");
+    hip_check(hipStreamSynchronize(c->stream), "sync");
+    if (c->hstatus[ST_SPILL] && c->hstatus[ST_REDO] > 0)
+      fail(KG_E_LIMIT, "BVH traversal stack exceeded its HBM spill capacity (KG_SPILL_CAP)");
+  });
+}
+
+// Autotuner: times the traversal on the given heterogeneous batch (device Q)
+// for each candidate setting and keeps the fastest on the context.
+extern "C" int kg_autotune(int nb, const kg_world* worlds, const int32_t* ris, const int32_t* counts, const float* dQ,
+                           int32_t* chosen, float* best_ms) {
+  return guard([&] {
+    need(nb >= 1 && worlds && ris && counts && dQ, "NULL argument");
+    kg_ctx_s* c = worlds[0]->ctx;
+    hip_check(hipSetDevice(c->device), "hipSetDevice");
+    // dQ may have been written on any stream of the caller
+    hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    StreamScope scope(c, c->stream);
+    hipStream_t s = c->stream;
+    size_t N = 0;
+    Layout L = multi_layout(c, nb, worlds, ris, counts, s, &N);
+    if (N == 0) return;
+    hip_check(c->wsOut8b.ensure(N), "hipMalloc");
+    const int cpws[] = {4, 8, 16, 32};
+    const int leafs[] = {1, 8};
+    const int scaps[] = {512, 1024};
+    const int wpcs[] = {0, 6};
+    float best = 1e30f;
+    int bc[4] = {16, 1, 1024, 0};
+    for (int scap : scaps)
+      for (int wpc : wpcs)
+        for (int cpw : cpws)
+          for (int leaf : leafs) {
+            c->tune_cpw = cpw;
+            c->tune_leaf = leaf;
+            c->tune_scap = scap;
+            c->tune_wpc = wpc;
+            float tot = 0;
+            for (int rep = 0; rep < 3; rep++) {
+              feasible_core(c, L, dQ, nullptr, (int)N, c->wsOut8b.as<uint8_t>(), nullptr, nullptr, nullptr, s);
+              hip_check(hipStreamSynchronize(s), "sync");
+              float ms = 0;
+              hip_check(hipEventElapsedTime(&ms, c->ev[0], c->ev[5]), "hipEventElapsedTime");
+              if (rep > 0) tot += ms;
+            }
+            if (tot < best) {
+              best = tot;
+              bc[0] = cpw;
+              bc[1] = leaf;
+              bc[2] = scap;
+              bc[3] = wpc;
+            }
+          }
+    c->tune_cpw = bc[0];
+    c->tune_leaf = bc[1];
+    c->tune_scap = bc[2];
+    c->tune_wpc = bc[3];
+    if (chosen)
+      for (int i = 0; i < 4; i++) chosen[i] = bc[i];
+    if (best_ms) *best_ms = best / 2;
+  });
+}
+
+extern "C" int kg_ctx_set_tuning(kg_ctx c, int cpw, int leaf_min, int scap, int waves_per_cu) {
+  return guard([&] {
+    need(c != nullptr, "ctx is NULL");
+    need(cpw >= 0 && cpw <= 64 && leaf_min >= 0 && leaf_min <= 64, "tuning value out of range");
+    need(scap == 0 || scap == 512 || scap == 1024, "scap must be 512 or 1024");
+    c->tune_cpw = cpw;
+    c->tune_leaf = leaf_min;
+    c->tune_scap = scap;
+    c->tune_wpc = waves_per_cu;
   });
 }

@@ -22,7 +22,10 @@ struct DevBuf {
   hipError_t ensure(size_t n) {
     if (n <= bytes) return hipSuccess;
     if (p) {
-      hipError_t e = hipFree(p);
+      // kernels still queued on any stream may read the old block
+      hipError_t e = hipDeviceSynchronize();
+      if (e != hipSuccess) return e;
+      e = hipFree(p);
       if (e != hipSuccess) return e;
       p = nullptr;
       bytes = 0;
@@ -34,7 +37,10 @@ struct DevBuf {
     return e;
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      (void)hipDeviceSynchronize();
+      (void)hipFree(p);
+    }
     p = nullptr;
     bytes = 0;
   }
@@ -109,6 +115,13 @@ struct PrepArgs {
   int cap;             // items per configuration in the fixed-capacity layout
   int* reset;          // nreset ints zeroed by the prep kernel
   int nreset;
+  // heterogeneous batches (all nullable): per-configuration plan index, Q
+  // offset (floats), frame offset (links) and item base
+  const PlanDev* plans;
+  const int* cfg_plan;
+  const int* qoff;
+  const int* toff;
+  const int* ibase;
 };
 hipError_t kgl_prep_count(const PrepArgs& a, hipStream_t s);
 hipError_t kgl_scan(int* counts, int* offsets, int n, DevBuf& tmp, hipStream_t s);
@@ -139,6 +152,11 @@ struct TravArgs {
   int stack_cap;              // per-lane traversal stack entries (LDS)
   int grid;                   // number of waves launched
   unsigned long long* counters;  // [0] box tests, [1] triangle tests, [2] items (nullable)
+  const PlanDev* plans;       // heterogeneous batches (nullable, see PrepArgs)
+  const int* cfg_plan;
+  const int* toff;
+  const int* ibase;
+  int scap;                   // LDS pool capacity variant (512 or 1024)
 };
 hipError_t kgl_collide_f32(const TravArgs& a, hipStream_t s);
 hipError_t kgl_collide_f64(const TravArgs& a, hipStream_t s);
@@ -190,11 +208,13 @@ struct FinArgs {
   int* redo_list;   // configs needing the FP64 pass
   int* redo_count;
   int final_pass;   // 1: uncertainty cannot remain
+  const PlanDev* plans;  // heterogeneous batches (nullable)
+  const int* cfg_plan;
 };
 hipError_t kgl_finalize(const FinArgs& a, hipStream_t s);
 hipError_t kgl_finalize_list(const FinArgs& a, const int* list, const int* count, int maxn, hipStream_t s);
 hipError_t kgl_fill_int(int* p, int n, int v, hipStream_t s);
-int kgl_collide_grid(int exact, int num_cus);
+int kgl_collide_grid(int exact, int num_cus, int scap);
 hipError_t kgl_sdf_eval(const SdfDev& g, const float* pts, int n, float* d, float* grad, hipStream_t s);
 hipError_t kgl_sdf_penalty(const LinkDesc* links, int n, const SdfDev& g, const float* Q, int N, const float* pts,
                            const int* poff, float dsafe, float* loss, const float* dloss, float* dQ, float* scratch,
@@ -280,11 +300,23 @@ struct kg_ctx_s {
   int pool_version = 0;
   // workspace
   int* hstatus = nullptr;  // pinned copy of the last batch's status words
-  hipStream_t last_stream = nullptr;
+  bool has_batch = false;     // a feasibility batch has been issued
+  hipEvent_t evStat = nullptr;  // recorded after the status words are copied to hstatus
   hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   int last_N = 0;
+  // ordering of work across the streams callers name: evDone is recorded on
+  // done_stream at the end of every call that issued GPU work
+  hipEvent_t evDone = nullptr;
+  hipStream_t done_stream = nullptr;
+  bool done_valid = false;
   DevBuf wsQ, wsT64, wsLB, wsJL, wsCounts, wsOffsets, wsItems, wsScanTmp, wsBest, wsUnc, wsChunk, wsSpill,
       wsStatus, wsRedo, wsOut8a, wsOut8b, wsOut8c, wsOut32, wsF, wsG, wsSdfScratch, wsPoff, wsPts, wsLoss, wsDQ,
       wsDl, wsEA, wsEB, wsESeg, wsESoff, wsENs, wsEOoff, wsEOrder, wsEQ, wsEFeas, wsEU, wsEDone;
   int num_cus = 256;
+  // traversal settings (autotuner; 0 = default)
+  int tune_cpw = 0, tune_leaf = 0, tune_wpc = 0, tune_scap = 0;
+  // heterogeneous-batch layout cache
+  DevBuf mPlans, mCfgPlan, mQoff, mToff, mIbase;
+  std::vector<uint64_t> mSig;
+  std::vector<PlanDev> mPlansHost;
 };

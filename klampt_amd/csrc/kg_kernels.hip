@@ -171,14 +171,16 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (MODE != 1 && c < a.nreset) a.reset[c] = 0;
   if (c >= a.N) return;
-  const PlanDev& P = a.plan;
+  const PlanDev& P = a.cfg_plan ? a.plans[a.cfg_plan[c]] : a.plan;
   const int n = P.nlinks;
-  double* T = a.T64 + (size_t)c * n * 12;
-  double* LB = a.LB + (size_t)c * n * 6;
+  const size_t tb = a.toff ? (size_t)a.toff[c] : (size_t)c * n;
+  const size_t qb = a.qoff ? (size_t)a.qoff[c] : (size_t)c * n;
+  double* T = a.T64 + tb * 12;
+  double* LB = a.LB + tb * 6;
   if (MODE != 1) {
     bool jl = true;
     for (int l = 0; l < n; l++) {
-      double q = a.Qf ? (double)a.Qf[(size_t)c * n + l] : a.Qd[(size_t)c * n + l];
+      double q = a.Qf ? (double)a.Qf[qb + l] : a.Qd[qb + l];
       if (q < P.qmin[l] || q > P.qmax[l]) jl = false;
       const LinkDesc& L = P.links[l];
       X64 J, T0, Lx, X;
@@ -249,7 +251,7 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
   }
   int m1 = nact - __popcll(lrej);
   int cnt = 0;
-  int base = MODE == 1 ? a.offsets[c] : (MODE == 2 ? c * a.cap : 0);
+  int base = MODE == 1 ? a.offsets[c] : (MODE == 2 ? (a.ibase ? a.ibase[c] : c * a.cap) : 0);
   auto env_rej = [&](int i) { return !box_hit(P.env[i].lo, P.env[i].hi, ulo, uhi); };
   auto link_rej = [&](int i) { return (lrej >> i & 1) != 0; };
   // env pairs
@@ -552,7 +554,7 @@ __device__ inline uint64_t mk_leaf(int slot, int ta, int tb) {
   return ((uint64_t)slot << 58) | ((uint64_t)(uint32_t)ta << 29) | (uint64_t)(uint32_t)tb;
 }
 
-#define KG_SCAP 1024
+#define KG_SCAP 1024  // LDS pool entries of k_distance
 #define KG_LQ 256
 #define KG_NOTASK 0xffffffffffffffffull
 
@@ -564,9 +566,9 @@ __device__ inline uint64_t mk_leaf(int slot, int ta, int tb) {
 // depth first while heavy items spread over idle lanes.  Leaf pairs are
 // queued and tested wave-coherently.  Items stream from a chunk of
 // configurations whose kill state sits in LDS.
-template <typename Real, bool EXACT>
+template <typename Real, bool EXACT, int SCAP>
 __global__ __launch_bounds__(64) void k_collide(TravArgs a) {
-  __shared__ uint64_t stk[KG_SCAP];
+  __shared__ uint64_t stk[SCAP];
   __shared__ uint64_t lq[KG_LQ];
   __shared__ Real s_rel[12][64];
   __shared__ float s_tau[64], s_pad[64];
@@ -603,7 +605,7 @@ __global__ __launch_bounds__(64) void k_collide(TravArgs a) {
         b = a.offsets[c];
         e = a.offsets[c + 1];
       } else {
-        b = c * a.cap;
+        b = a.ibase ? a.ibase[c] : c * a.cap;
         e = b + a.counts[c];
       }
     };
@@ -613,8 +615,8 @@ __global__ __launch_bounds__(64) void k_collide(TravArgs a) {
     uint64_t cont = KG_NOTASK;  // this lane's depth-first continuation
 
     auto ensure_room = [&](int room) {
-      if (sp + room <= KG_SCAP) return;
-      const int h = KG_SCAP / 2;
+      if (sp + room <= SCAP) return;
+      const int h = SCAP / 2;
       if (gsp + h <= a.spill_cap) {
         for (int i = lane; i < h; i += 64) gsp_base[gsp + i] = stk[i];
         gsp += h;
@@ -685,7 +687,7 @@ __global__ __launch_bounds__(64) void k_collide(TravArgs a) {
       const uint64_t hm = ballot64(hungry);
       const int nh = __popcll(hm);
       if (nh > sp && gsp > 0) {
-        const int k = min(gsp, KG_SCAP / 2);
+        const int k = min(gsp, SCAP / 2);
         for (int i = sp - 1 - lane; i >= 0; i -= 64) stk[i + k] = stk[i];
         __syncthreads();
         for (int i = lane; i < k; i += 64) stk[i] = gsp_base[gsp - k + i];
@@ -735,17 +737,19 @@ __global__ __launch_bounds__(64) void k_collide(TravArgs a) {
       if (myitem >= 0) {
         n_items++;
         const PairItem P = a.items[myitem];
-        const LinkDesc& LA = a.plan.links[P.a];
+        const PlanDev& PL = a.cfg_plan ? a.plans[a.cfg_plan[P.cfg]] : a.plan;
+        const size_t tb = a.toff ? (size_t)a.toff[P.cfg] : (size_t)P.cfg * PL.nlinks;
+        const LinkDesc& LA = PL.links[P.a];
         const int g = (P.rank >> KG_RANK_GROUP_BIT) & 1;
-        const double* XA = a.T64 + ((size_t)P.cfg * a.plan.nlinks + P.a) * 12;
+        const double* XA = a.T64 + (tb + P.a) * 12;
         const double* XB;
         int geomB;
         if (g == 0) {
-          XB = a.plan.env[P.b].X;
-          geomB = a.plan.env[P.b].geom;
+          XB = PL.env[P.b].X;
+          geomB = PL.env[P.b].geom;
         } else {
-          XB = a.T64 + ((size_t)P.cfg * a.plan.nlinks + P.b) * 12;
-          geomB = a.plan.links[P.b].geom;
+          XB = a.T64 + (tb + P.b) * 12;
+          geomB = PL.links[P.b].geom;
         }
         const double d0 = XA[9] - XB[9], d1 = XA[10] - XB[10], d2 = XA[11] - XB[11];
 #pragma unroll
@@ -912,8 +916,9 @@ __global__ __launch_bounds__(256) void k_finalize(FinArgs a, const int* list, co
     int id1 = -1, id2 = -1;
     if (h0 || h1) {
       const PairItem it = a.items[(uint32_t)((h0 ? b0 : b1) & 0xffffffffu)];
-      id1 = a.plan.links[it.a].wid;
-      id2 = h0 ? a.plan.env[it.b].wid : a.plan.links[it.b].wid;
+      const PlanDev& PL = a.cfg_plan ? a.plans[a.cfg_plan[c]] : a.plan;
+      id1 = PL.links[it.a].wid;
+      id2 = h0 ? PL.env[it.b].wid : PL.links[it.b].wid;
     }
     a.first_pair[2 * c] = id1;
     a.first_pair[2 * c + 1] = id2;
@@ -961,7 +966,10 @@ hipError_t kgl_prep_single(const PrepArgs& a, hipStream_t s) {
 
 template <typename Real, bool EXACT>
 static hipError_t launch_collide(const TravArgs& a, hipStream_t s) {
-  k_collide<Real, EXACT><<<a.grid, 64, 0, s>>>(a);
+  if (a.scap == 512)
+    k_collide<Real, EXACT, 512><<<a.grid, 64, 0, s>>>(a);
+  else
+    k_collide<Real, EXACT, 1024><<<a.grid, 64, 0, s>>>(a);
   return hipGetLastError();
 }
 
@@ -986,11 +994,15 @@ hipError_t kgl_fill_int(int* p, int n, int v, hipStream_t s) {
   return hipGetLastError();
 }
 
-int kgl_collide_grid(int exact, int num_cus) {
+int kgl_collide_grid(int exact, int num_cus, int scap) {
   int nb = 0;
-  const size_t lds = 0;
-  hipError_t e = exact ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_collide<double, true>, 64, lds)
-                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_collide<float, false>, 64, lds);
+  hipError_t e;
+  if (scap == 512)
+    e = exact ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_collide<double, true, 512>, 64, 0)
+              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_collide<float, false, 512>, 64, 0);
+  else
+    e = exact ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_collide<double, true, 1024>, 64, 0)
+              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_collide<float, false, 1024>, 64, 0);
   if (e != hipSuccess || nb < 1) nb = 8;
   const char* v = getenv("KG_WAVES_PER_CU");
   if (v && atoi(v) > 0) nb = atoi(v);

@@ -358,3 +358,44 @@ def world_from_scene(ctx, scene):
     r = Robot.from_spec(ctx, scene.robot)
     w.add_robot(r)
     return w, r
+
+
+def _multi_args(worlds, robots, counts):
+    nb = len(worlds)
+    W = (ctypes.c_void_p * nb)(*[w.h.value if hasattr(w.h, "value") else w.h for w in worlds])
+    R = _i(robots)
+    C = _i(counts)
+    return nb, W, R, C
+
+
+def feasible_multi(worlds, robots, counts, Qs):
+    """IsFeasible over a heterogeneous batch in one launch: Qs[b] is a
+    (counts[b], n_b) array for robot robots[b] of worlds[b]."""
+    nb, W, R, C = _multi_args(worlds, robots, counts)
+    Q = np.concatenate([_f(q).ravel() for q in Qs]) if nb else np.zeros(0, np.float32)
+    out = np.zeros(int(np.sum(counts)), np.uint8)
+    call("kg_feasible_multi", nb, W, _p(R, _pi), _p(C, _pi), _p(_f(Q), _pf), _p(out, _pu8))
+    return np.split(out, np.cumsum(counts)[:-1])
+
+
+def feasible_multi_dev(worlds, robots, counts, dQ, out=None):
+    import torch
+    nb, W, R, C = _multi_args(worlds, robots, counts)
+    if out is None:
+        out = torch.empty(int(np.sum(counts)), dtype=torch.uint8, device=dQ.device)
+    call("kg_feasible_multi_dev", nb, W, _p(R, _pi), _p(C, _pi), _tp(dQ), _tp(out), _torch_stream())
+    return out
+
+
+def autotune(worlds, robots, counts, dQ):
+    """Pick the fastest traversal settings for this batch; returns
+    (cpw, leaf_min, scap, waves_per_cu, ms)."""
+    nb, W, R, C = _multi_args(worlds, robots, counts)
+    ch = np.zeros(4, np.int32)
+    ms = ctypes.c_float()
+    call("kg_autotune", nb, W, _p(R, _pi), _p(C, _pi), _tp(dQ), _p(ch, _pi), ctypes.byref(ms))
+    return tuple(int(x) for x in ch) + (ms.value,)
+
+
+def set_tuning(ctx, cpw=0, leaf_min=0, scap=0, waves_per_cu=0):
+    call("kg_ctx_set_tuning", ctx.h, cpw, leaf_min, scap, waves_per_cu)
