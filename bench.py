@@ -7,8 +7,13 @@ environment, self collision) over the whole batch with inputs resident in
 HBM.  Prints one JSON line.
 
 Also reported: the roofline of the traversal kernel (algorithmic bytes from
-its own box/triangle-test counters over its HIP-event duration), and the FP64
-CPU path (IsFeasible with Klampt's early returns) timed on the host cores.
+its own box/triangle-test counters over its average HIP-event duration in
+the timed steps; HBM traffic from the committed rocprofv3 counter summary),
+and the FP64 CPU path (IsFeasible with Klampt's early returns) timed on the
+host cores, all threads and one thread.
+
+``python bench.py --copy-probe`` only runs a plain 256 MiB device copy five
+times: the calibration workload for the FETCH_SIZE / WRITE_SIZE counters.
 """
 import argparse
 import json
@@ -19,13 +24,36 @@ import time
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
-# Bytes each test reads (record sizes of the HBM layout): a box test reads two
-# 64 B node records; a triangle test reads two triangles of three 16 B
-# vertices; a pair item reads its 12 B record plus two 96 B FP64 transforms.
-BYTES_PER_BOX_TEST = 128
-BYTES_PER_TRI_TEST = 96
-BYTES_PER_ITEM = 12 + 192
+# Algorithmic bytes of one feasibility check: 4 B per joint value in, 2 B of
+# result out, and per candidate pair 64 B per environment OBB node visited
+# (one per box test) and 36 B per environment triangle tested (three FP32
+# vertices).  Robot-side nodes and triangles are a few hundred KB shared by
+# every configuration and are not counted.
+BYTES_PER_ENV_NODE = 64
+BYTES_PER_ENV_TRI = 36
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(HERE, "profiles", "k_collide_f32_soup200k_65536.traffic.json")
+
+
+def copy_probe():
+    import torch
+    x = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")   # 256 MiB
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print("copied 256 MiB x5", flush=True)
+
+
+def measured_traffic():
+    """HBM bytes per launch of the traversal kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (None if absent)."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+        return float(t["bytes_per_dispatch"]), os.path.relpath(TRAFFIC_FILE, HERE)
+    except (OSError, KeyError, ValueError):
+        return None, None
 
 
 def main():
@@ -40,7 +68,11 @@ def main():
     ap.add_argument("--dist-configs", type=int, default=8192)
     ap.add_argument("--edges", type=int, default=8192)
     ap.add_argument("--traj", type=int, default=16384)
+    ap.add_argument("--copy-probe", action="store_true")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: multi-GPU runs are not part of this build "
               "(configurations shard independently; see DESIGN.md)", file=sys.stderr)
@@ -65,34 +97,37 @@ def main():
         world.feasible_batch_dev(dQ, feasible=dfeas)
     torch.cuda.synchronize()
 
-    trav_ns, redo, boxes, tris, items = [], [], [], [], []
+    ctx.kernel_times(reset=True)
     torch.cuda.synchronize()
     t_start = time.perf_counter()
     for _ in range(args.steps):
         world.feasible_batch_dev(dQ, feasible=dfeas)
-        st = world.last_stats()
-        trav_ns.append(st["t_trav32_ns"] + st["t_trav64_ns"])
-        redo.append(st["fp64_rechecks"])
-        boxes.append(st["box_tests"])
-        tris.append(st["tri_tests"])
-        items.append(st["pair_items"])
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
     ms_per_step = elapsed / args.steps * 1e3
     value = args.configs * args.steps / elapsed
+    # traversal kernel duration: HIP events on the launch stream, every step
+    ksum, kcnt = ctx.kernel_times(reset=True)
+    kernel_s = ksum / max(1, kcnt)
+    # counters are deterministic per batch: read those of the last step
+    st = world.last_stats()
     feas_frac = float(dfeas.float().mean().item())
-    print(f"[bench] {value:.4g} checks/s, {ms_per_step:.3f} ms/step, feasible {feas_frac:.3f}, "
-          f"fp64 rechecks/step {np.mean(redo):.1f}", file=sys.stderr, flush=True)
+    print(f"[bench] {value:.4g} checks/s, {ms_per_step:.3f} ms/step, traversal kernel {kernel_s * 1e3:.4f} ms "
+          f"(avg of {kcnt}), feasible {feas_frac:.3f}, fp64 rechecks/step {st['fp64_rechecks']}",
+          file=sys.stderr, flush=True)
 
-    kernel_s = float(np.mean(trav_ns)) * 1e-9
-    alg_bytes = (float(np.mean(boxes)) * BYTES_PER_BOX_TEST + float(np.mean(tris)) * BYTES_PER_TRI_TEST
-                 + float(np.mean(items)) * BYTES_PER_ITEM)
+    nj = sc.robot.n
+    alg_bytes = (args.configs * (4 * nj + 2) + st["box_tests"] * BYTES_PER_ENV_NODE
+                 + st["tri_tests"] * BYTES_PER_ENV_TRI)
     achieved = alg_bytes / kernel_s / 1e9 if kernel_s > 0 else 0.0
+    traffic, traffic_src = measured_traffic()
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                "kernel": "k_collide<float>", "kernel_ms": round(kernel_s * 1e3, 4),
-                "box_tests": int(np.mean(boxes)), "tri_tests": int(np.mean(tris)),
-                "pair_items": int(np.mean(items))}
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                "traffic_source": traffic_src,
+                "kernel": "k_collide<float,false,1024>", "kernel_ms": round(kernel_s * 1e3, 4),
+                "kernel_launches_timed": kcnt, "algorithmic_bytes": int(alg_bytes),
+                "box_tests": int(st["box_tests"]), "tri_tests": int(st["tri_tests"]),
+                "pair_items": int(st["pair_items"]), "fp64_rechecks": int(st["fp64_rechecks"])}
 
     cpu = None
     if not args.no_cpu:
@@ -108,11 +143,24 @@ def main():
             passes += 1
         cpu_rate = passes * len(Q) / t_cpu
         agree = bool(np.array_equal(f_cpu, dfeas.cpu().numpy()))
+        # one thread, on the first 8192 configurations
+        Q1 = Q64[:8192]
+        p1, t1s = 0, 0.0
+        while t1s < args.cpu_seconds / 3 or p1 == 0:
+            t1 = time.perf_counter()
+            ow.isfeasible_batch(orb, Q1, nthreads=1)
+            t1s += time.perf_counter() - t1
+            p1 += 1
+        rate1 = p1 * len(Q1) / t1s
         cpu = {"value": round(cpu_rate, 1), "unit": "checks/s", "cores": nthr, "kind": "port",
                "sample": f"{passes} pass(es) over the same {len(Q)} configurations, FP64 IsFeasible with "
-                         f"Klampt's early returns, {nthr} threads, {t_cpu:.1f}s",
-               "gpu_over_cpu": round(value / cpu_rate, 2), "results_equal": agree}
-        print(f"[bench] cpu {cpu_rate:.4g} checks/s on {nthr} threads; equal={agree}", file=sys.stderr, flush=True)
+                         f"Klampt's early returns, {nthr} threads, {t_cpu:.1f}s; single thread: {p1} pass(es) "
+                         f"over the first {len(Q1)}, {t1s:.1f}s",
+               "single_thread_value": round(rate1, 1),
+               "gpu_over_cpu": round(value / cpu_rate, 2), "gpu_over_one_thread": round(value / rate1, 1),
+               "results_equal": agree}
+        print(f"[bench] cpu {cpu_rate:.4g} checks/s on {nthr} threads, {rate1:.4g} on one; equal={agree}",
+              file=sys.stderr, flush=True)
 
     secondary = None if args.no_secondary else run_secondary(args, ctx, sc, world, robot)
 

@@ -213,6 +213,11 @@ int kg_ctx_set_tuning(kg_ctx ctx, int cpw, int leaf_min, int scap, int waves_per
  * re-check kernel time (ns) -- HIP events on the launch stream -- [6] box
  * tests, [7] triangle tests of the FP32 traversal. */
 int kg_world_last_stats(kg_world w, int64_t* stats, int nstats);
+/* Sum of the FP32 traversal kernel durations (HIP events on the launch
+ * stream) of the feasibility batches issued on this context since the last
+ * reset, at most the 64 most recent; count receives how many were summed.
+ * Waits for those batches.  No Klampt counterpart (measurement). */
+int kg_ctx_kernel_times(kg_ctx ctx, int reset, int64_t* total_ns, int32_t* count);
 
 #ifdef __cplusplus
 }

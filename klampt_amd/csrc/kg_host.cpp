@@ -408,6 +408,10 @@ void ensure_status_host(kg_ctx_s* c) {
     hip_check(hipHostMalloc((void**)&c->hstatus, ST_WORDS * sizeof(int), hipHostMallocDefault), "hipHostMalloc");
     for (auto& e : c->ev) hip_check(hipEventCreate(&e), "hipEventCreate");
     hip_check(hipEventCreateWithFlags(&c->evStat, hipEventDisableTiming), "hipEventCreate");
+    for (int i = 0; i < kg_ctx_s::TRING; i++) {
+      hip_check(hipEventCreate(&c->tr_beg[i]), "hipEventCreate");
+      hip_check(hipEventCreate(&c->tr_end[i]), "hipEventCreate");
+    }
   }
 }
 
@@ -528,9 +532,13 @@ void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* 
   ta.toff = L.toff;
   ta.ibase = L.ibase;
   ta.scap = scap;
+  const int slot = (int)(c->tr_count % kg_ctx_s::TRING);
   hip_check(hipEventRecord(c->ev[2], s), "hipEventRecord");
+  hip_check(hipEventRecord(c->tr_beg[slot], s), "hipEventRecord");
   hip_check(kgl_collide_f32(ta, s), "collide_f32");
+  hip_check(hipEventRecord(c->tr_end[slot], s), "hipEventRecord");
   hip_check(hipEventRecord(c->ev[3], s), "hipEventRecord");
+  c->tr_count++;
 
   FinArgs fa{};
   fa.N = N;
@@ -669,6 +677,10 @@ int kg_ctx_destroy(kg_ctx c) {
     if (c->hstatus) (void)hipHostFree(c->hstatus);
     for (auto& e : c->ev)
       if (e) (void)hipEventDestroy(e);
+    for (int i = 0; i < kg_ctx_s::TRING; i++) {
+      if (c->tr_beg[i]) (void)hipEventDestroy(c->tr_beg[i]);
+      if (c->tr_end[i]) (void)hipEventDestroy(c->tr_end[i]);
+    }
     if (c->evStat) (void)hipEventDestroy(c->evStat);
     if (c->evDone) (void)hipEventDestroy(c->evDone);
     (void)hipStreamDestroy(c->stream);
@@ -1134,6 +1146,28 @@ int kg_world_last_stats(kg_world w, int64_t* out, int nstats) {
     int64_t v[8] = {st.redo, st.items, st.spills, st.t_prep_ns, st.t_trav32_ns, st.t_trav64_ns, st.box_tests,
                     st.tri_tests};
     for (int i = 0; i < nstats && i < 8; i++) out[i] = v[i];
+  });
+}
+
+// Sum of the FP32 traversal kernel durations of the feasibility batches
+// issued since the last reset (at most the last TRING of them; *count says
+// how many were summed).  Waits for them to finish.
+int kg_ctx_kernel_times(kg_ctx c, int reset, int64_t* total_ns, int32_t* count) {
+  return guard([&] {
+    need(c && total_ns && count, "NULL argument");
+    hip_check(hipSetDevice(c->device), "hipSetDevice");
+    const int n = (int)std::min<int64_t>(c->tr_count, kg_ctx_s::TRING);
+    double ms_sum = 0;
+    for (int k = 0; k < n; k++) {
+      int slot = (int)((c->tr_count - 1 - k) % kg_ctx_s::TRING);
+      hip_check(hipEventSynchronize(c->tr_end[slot]), "hipEventSynchronize");
+      float ms = 0;
+      hip_check(hipEventElapsedTime(&ms, c->tr_beg[slot], c->tr_end[slot]), "hipEventElapsedTime");
+      ms_sum += ms;
+    }
+    *total_ns = (int64_t)(ms_sum * 1e6);
+    *count = n;
+    if (reset) c->tr_count = 0;
   });
 }
 

@@ -307,6 +307,11 @@ struct kg_ctx_s {
   // ordering of work across the streams callers name: evDone is recorded on
   // done_stream at the end of every call that issued GPU work
   hipEvent_t evDone = nullptr;
+  // begin/end events of the FP32 traversal kernel of the most recent
+  // feasibility batches (ring), for averaging its duration over many calls
+  static constexpr int TRING = 64;
+  hipEvent_t tr_beg[TRING] = {}, tr_end[TRING] = {};
+  int64_t tr_count = 0;
   hipStream_t done_stream = nullptr;
   bool done_valid = false;
   DevBuf wsQ, wsT64, wsLB, wsJL, wsCounts, wsOffsets, wsItems, wsScanTmp, wsBest, wsUnc, wsChunk, wsSpill,

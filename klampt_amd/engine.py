@@ -56,6 +56,13 @@ class Context:
     def synchronize(self):
         call("kg_ctx_synchronize", self.h)
 
+    def kernel_times(self, reset=True):
+        """(total seconds, count) of the FP32 traversal kernel over the
+        feasibility batches since the last reset (HIP events)."""
+        ns, cnt = ctypes.c_int64(), ctypes.c_int32()
+        call("kg_ctx_kernel_times", self.h, 1 if reset else 0, ctypes.byref(ns), ctypes.byref(cnt))
+        return ns.value * 1e-9, cnt.value
+
     def close(self):
         if self.h:
             lib().kg_ctx_destroy(self.h)
