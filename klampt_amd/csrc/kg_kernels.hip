@@ -21,6 +21,7 @@
 #include "kg_internal.h"
 
 #define KG_CPW_MAX 64
+#define KG_NE 64  // configuration entries per wave (ring)
 #define KG_NONE64 0xffffffffffffffffull
 
 namespace {
@@ -432,44 +433,63 @@ __device__ inline NodeF sel(bool p, const NodeF& a, const NodeF& b) {
   return r;
 }
 
-// OBB overlap: box A (frame A) mapped by rel (A-local -> B-local) against box
-// B (frame B).  15 axes, every right-hand side padded by `pad`.
+// Frame of a task's two box tests.  S is the node whose children are tested
+// (the split node), O the other node; rel' maps S's mesh frame to O's.
+// With M = R_O^T R' and t0 = R_O^T (t' - c_O), a child box s of S has
+// C = M R_s (its axes in O's axes) and T = M c_s + t0 (its centre), so the
+// shared part is computed once per task instead of once per test.
 template <typename Real>
-__device__ inline bool obb_overlap(const float* cAf, const float* eAf, const float* RAf, const float* cBf,
-                                   const float* eBf, const float* RBf, const Real* rel, Real pad_abs, Real pad_rel) {
-  Real RA[9], RB[9], cA[3], cB[3], eA[3], eB[3];
+struct TaskFrame {
+  Real M[9];   // M[i*3+m] = O_i . (R' e_m)
+  Real t0[3];
+  Real eO[3];
+  Real sumO;
+};
+
+template <typename Real>
+__device__ inline void task_frame(const NodeF& O, const Real* rel, TaskFrame<Real>& F) {
+  Real RO[9];
 #pragma unroll
-  for (int i = 0; i < 9; i++) { RA[i] = (Real)RAf[i]; RB[i] = (Real)RBf[i]; }
+  for (int i = 0; i < 9; i++) RO[i] = (Real)O.R[i];
+  Real d[3];
 #pragma unroll
-  for (int i = 0; i < 3; i++) { cA[i] = cAf[i]; cB[i] = cBf[i]; eA[i] = eAf[i]; eB[i] = eBf[i]; }
-  // A's axes and centre in frame B
-  Real Ap[9], cp[3];
-#pragma unroll
-  for (int col = 0; col < 3; col++)
-#pragma unroll
-    for (int r = 0; r < 3; r++)
-      Ap[col * 3 + r] = rel[r] * RA[col * 3] + rel[3 + r] * RA[col * 3 + 1] + rel[6 + r] * RA[col * 3 + 2];
-#pragma unroll
-  for (int r = 0; r < 3; r++) cp[r] = rel[r] * cA[0] + rel[3 + r] * cA[1] + rel[6 + r] * cA[2] + rel[9 + r] - cB[r];
-  // in box B's axes: C[i][j] = B_i . A'_j, T_i = B_i . (cA' - cB)
-  Real C[3][3], Qa[3][3], T[3];
+  for (int r = 0; r < 3; r++) d[r] = rel[9 + r] - (Real)O.c[r];
 #pragma unroll
   for (int i = 0; i < 3; i++) {
-    T[i] = RB[i * 3] * cp[0] + RB[i * 3 + 1] * cp[1] + RB[i * 3 + 2] * cp[2];
+#pragma unroll
+    for (int m = 0; m < 3; m++)
+      F.M[i * 3 + m] = RO[i * 3] * rel[m * 3] + RO[i * 3 + 1] * rel[m * 3 + 1] + RO[i * 3 + 2] * rel[m * 3 + 2];
+    F.t0[i] = RO[i * 3] * d[0] + RO[i * 3 + 1] * d[1] + RO[i * 3 + 2] * d[2];
+    F.eO[i] = (Real)O.e[i];
+  }
+  F.sumO = F.eO[0] + F.eO[1] + F.eO[2];
+}
+
+// 15-axis separating-axis test of child box S against O in the task frame;
+// every right-hand side padded by pad_abs + pad_rel * (sum of extents).
+template <typename Real>
+__device__ inline bool frame_overlap(const TaskFrame<Real>& F, const NodeF& S, Real pad_abs, Real pad_rel) {
+  Real C[3][3], Qa[3][3], T[3], eA[3];
+#pragma unroll
+  for (int j = 0; j < 3; j++) eA[j] = (Real)S.e[j];
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    T[i] = F.M[i * 3] * (Real)S.c[0] + F.M[i * 3 + 1] * (Real)S.c[1] + F.M[i * 3 + 2] * (Real)S.c[2] + F.t0[i];
 #pragma unroll
     for (int j = 0; j < 3; j++) {
-      C[i][j] = RB[i * 3] * Ap[j * 3] + RB[i * 3 + 1] * Ap[j * 3 + 1] + RB[i * 3 + 2] * Ap[j * 3 + 2];
+      C[i][j] = F.M[i * 3] * (Real)S.R[j * 3] + F.M[i * 3 + 1] * (Real)S.R[j * 3 + 1] + F.M[i * 3 + 2] * (Real)S.R[j * 3 + 2];
       Qa[i][j] = fabs(C[i][j]);
     }
   }
-  Real pad = pad_abs + pad_rel * (eA[0] + eA[1] + eA[2] + eB[0] + eB[1] + eB[2]);
+  const Real* eB = F.eO;
+  const Real pad = pad_abs + pad_rel * (eA[0] + eA[1] + eA[2] + F.sumO);
+  bool sep = false;
 #pragma unroll
-  for (int i = 0; i < 3; i++)
-    if (fabs(T[i]) > eB[i] + eA[0] * Qa[i][0] + eA[1] * Qa[i][1] + eA[2] * Qa[i][2] + pad) return false;
+  for (int i = 0; i < 3; i++) sep |= fabs(T[i]) > eB[i] + eA[0] * Qa[i][0] + eA[1] * Qa[i][1] + eA[2] * Qa[i][2] + pad;
 #pragma unroll
   for (int j = 0; j < 3; j++) {
-    Real t = T[0] * C[0][j] + T[1] * C[1][j] + T[2] * C[2][j];
-    if (fabs(t) > eB[0] * Qa[0][j] + eB[1] * Qa[1][j] + eB[2] * Qa[2][j] + eA[j] + pad) return false;
+    const Real t = T[0] * C[0][j] + T[1] * C[1][j] + T[2] * C[2][j];
+    sep |= fabs(t) > eB[0] * Qa[0][j] + eB[1] * Qa[1][j] + eB[2] * Qa[2][j] + eA[j] + pad;
   }
 #pragma unroll
   for (int i = 0; i < 3; i++) {
@@ -477,13 +497,13 @@ __device__ inline bool obb_overlap(const float* cAf, const float* eAf, const flo
 #pragma unroll
     for (int j = 0; j < 3; j++) {
       const int j1 = (j + 1) % 3, j2 = (j + 2) % 3;
-      Real t = T[i2] * C[i1][j] - T[i1] * C[i2][j];
-      Real ra = eB[i1] * Qa[i2][j] + eB[i2] * Qa[i1][j];
-      Real rb = eA[j1] * Qa[i][j2] + eA[j2] * Qa[i][j1];
-      if (fabs(t) > ra + rb + pad) return false;
+      const Real t = T[i2] * C[i1][j] - T[i1] * C[i2][j];
+      const Real ra = eB[i1] * Qa[i2][j] + eB[i2] * Qa[i1][j];
+      const Real rb = eA[j1] * Qa[i][j2] + eA[j2] * Qa[i][j1];
+      sep |= fabs(t) > ra + rb + pad;
     }
   }
-  return true;
+  return !sep;
 }
 
 template <typename Real>
@@ -538,7 +558,9 @@ __device__ inline void slot_bounds(const TravArgs& a, double tnorm, double rA, d
   const double u = 5.9604644775390625e-08;
   double ev = u * (8.0 * rA + 4.0 * tnorm) + u * rB;
   tau = (float)(2.0 * ev);
-  pad = (float)(2.0 * ev + 16.0 * u * (tnorm + rA + rB));
+  // box tests: products of up to three rounded factors over coordinates
+  // bounded by |t| + rA + rB (including the inverted item transform)
+  pad = (float)(2.0 * ev + 32.0 * u * (tnorm + rA + rB));
 }
 
 // Task word: slot(6) | side(1) | single(1) | X(28) | Y(28).
@@ -564,8 +586,14 @@ __device__ inline uint64_t mk_leaf(int slot, int ta, int tb) {
 // shared LIFO pool in LDS.  Lanes whose path ended pop the pool, and only when
 // the pool is empty do they start new pair items, so every item is explored
 // depth first while heavy items spread over idle lanes.  Leaf pairs are
-// queued and tested wave-coherently.  Items stream from a chunk of
-// configurations whose kill state sits in LDS.
+// queued and tested wave-coherently.
+//
+// Configurations stream through a ring of KG_NE entries in LDS holding their
+// kill state.  A wave fetches configurations from the global counter in
+// chunks that shrink as the work runs out (at most cpw, about the remaining
+// count over the grid size), starts their items as lanes and slots free up,
+// and writes a configuration's result as soon as its last item finishes, so
+// a heavy item never holds the wave's other lanes idle at a chunk boundary.
 template <typename Real, bool EXACT, int SCAP>
 __global__ __launch_bounds__(64) void k_collide(TravArgs a) {
   __shared__ uint64_t stk[SCAP];
@@ -573,8 +601,11 @@ __global__ __launch_bounds__(64) void k_collide(TravArgs a) {
   __shared__ Real s_rel[12][64];
   __shared__ float s_tau[64], s_pad[64];
   __shared__ int s_cfgk[64], s_rank[64], s_item[64], s_pend[64], s_unc[64];
-  __shared__ unsigned long long s_best[KG_CPW_MAX][2];
-  __shared__ int s_uncmin[KG_CPW_MAX][2];
+  // configuration entries: kill state, open-item count (+1 while items of
+  // the entry remain to be started), configuration index (-1 = free)
+  __shared__ unsigned long long s_best[KG_NE][2];
+  __shared__ int s_uncmin[KG_NE][2];
+  __shared__ int e_cfg[KG_NE], e_open[KG_NE];
   __shared__ int s_chunk;
 
   const int lane = threadIdx.x;
@@ -584,294 +615,346 @@ __global__ __launch_bounds__(64) void k_collide(TravArgs a) {
   const int ncfg = a.ncfg_dev ? *a.ncfg_dev : a.ncfg;
   unsigned n_box = 0, n_tri = 0, n_items = 0;
 
-  for (;;) {
-    if (lane == 0) s_chunk = atomicAdd(a.chunk_counter, 1);
-    __syncthreads();
-    const int chunk = s_chunk;
-    const int k0 = chunk * a.cpw;
-    if (k0 >= ncfg) break;
-    const int nk = min(a.cpw, ncfg - k0);
-    for (int i = lane; i < KG_CPW_MAX; i += 64) {
-      s_best[i][0] = s_best[i][1] = KG_NONE64;
-      s_uncmin[i][0] = s_uncmin[i][1] = KG_RANK_NONE;
+  for (int i = lane; i < KG_NE; i += 64) {
+    e_cfg[i] = -1;
+    e_open[i] = 0;
+  }
+  s_pend[lane] = 0;
+  s_unc[lane] = 0;
+  __syncthreads();
+
+  auto cfg_of = [&](int k) { return a.cfg_list ? a.cfg_list[k] : k; };
+  auto range = [&](int c, int& b, int& e) {
+    if (a.offsets) {
+      b = a.offsets[c];
+      e = a.offsets[c + 1];
+    } else {
+      b = a.ibase ? a.ibase[c] : c * a.cap;
+      e = b + a.counts[c];
     }
-    s_pend[lane] = 0;
-    s_unc[lane] = 0;
+  };
+  // ring positions (monotonic, wave-uniform): [head, tail) fetched entries,
+  // fe = entry whose items are being started, [it, itend) its next items
+  int head = 0, tail = 0, fe = 0, it = 0, itend = 0, last_k0 = 0;
+  bool fe_open = false, exhausted = false;
+  int sp = 0, gsp = 0, nl = 0;
+  uint64_t cont = KG_NOTASK;  // this lane's depth-first continuation
+
+  // the last open item of an entry has finished: publish its result
+  auto settle_entry = [&](int e, int delta) {
+    if (atomicAdd(&e_open[e], delta) + delta == 0) {
+      const int cc = e_cfg[e];
+      a.best[2 * cc] = s_best[e][0];
+      a.best[2 * cc + 1] = s_best[e][1];
+      a.unc[2 * cc] = s_uncmin[e][0];
+      a.unc[2 * cc + 1] = s_uncmin[e][1];
+      e_cfg[e] = -1;
+    }
+  };
+  auto settle = [&](int slot, int delta) {
+    if (atomicAdd(&s_pend[slot], delta) + delta == 0) {
+      const int e = s_cfgk[slot];
+      const int rk = s_rank[slot];
+      const int g = (rk >> KG_RANK_GROUP_BIT) & 1, r = rk & ((1 << KG_RANK_GROUP_BIT) - 1);
+      if (s_unc[slot] && !item_dead(mode, s_best[e], g, r)) atomicMin(&s_uncmin[e][g], r);
+      settle_entry(e, -1);
+    }
+  };
+
+  auto ensure_room = [&](int room) {
+    if (sp + room <= SCAP) return;
+    const int h = SCAP / 2;
+    if (gsp + h <= a.spill_cap) {
+      for (int i = lane; i < h; i += 64) gsp_base[gsp + i] = stk[i];
+      gsp += h;
+    } else {
+      // spill region full: drop the oldest half, marking its items for the
+      // exact re-check
+      for (int i = lane; i < h; i += 64) {
+        const int sl = (int)(stk[i] >> 58);
+        s_unc[sl] = 1;
+        settle(sl, -1);
+      }
+      if (lane == 0) atomicExch(a.status, 1);
+    }
+    __syncthreads();
+    for (int i = lane; i < sp - h; i += 64) stk[i] = stk[i + h];
+    sp -= h;
+    __syncthreads();
+  };
+
+  auto run_leaves = [&]() {
+    const int m = min(nl, 64);
+    if (lane < m) {
+      const uint64_t e = lq[nl - 1 - lane];
+      const int slot = (int)(e >> 58);
+      const int ta = (int)((e >> 29) & 0x1fffffff), tb = (int)(e & 0x1fffffff);
+      const int rk = s_rank[slot];
+      const int g = (rk >> KG_RANK_GROUP_BIT) & 1, r = rk & ((1 << KG_RANK_GROUP_BIT) - 1);
+      const int ck = s_cfgk[slot];
+      if (!item_dead(mode, s_best[ck], g, r)) {
+        n_tri++;
+        Real rel[12];
+#pragma unroll
+        for (int k = 0; k < 12; k++) rel[k] = s_rel[k][slot];
+        V3T<Real> tp[3], tq[3];
+        load_tri<Real>(a.pool, ta, tp);
+        load_tri<Real>(a.pool, tb, tq);
+#pragma unroll
+        for (int v = 0; v < 3; v++) tp[v] = xf_apply<Real>(rel, tp[v]);
+        const int res = tri_tri_sat<Real, EXACT>(tp, tq, (Real)s_tau[slot]);
+        if (res == TT_HIT) {
+          const unsigned long long key = ((unsigned long long)(uint32_t)r << 32) | (uint32_t)s_item[slot];
+          atomicMin(&s_best[ck][g], key);
+        } else if (res == TT_UNSURE) {
+          s_unc[slot] = 1;
+        }
+      }
+      settle(slot, -1);
+    }
+    nl -= m;
+    __syncthreads();
+  };
+
+  for (;;) {
+    // ---- lanes without a continuation pop the pool (bring spilled tasks
+    // back first when it runs low)
+    const bool hungry = cont == KG_NOTASK;
+    const uint64_t hm = ballot64(hungry);
+    const int nh = __popcll(hm);
+    if (nh > sp && gsp > 0) {
+      const int k = min(gsp, SCAP / 2);
+      for (int i = sp - 1 - lane; i >= 0; i -= 64) stk[i + k] = stk[i];
+      __syncthreads();
+      for (int i = lane; i < k; i += 64) stk[i] = gsp_base[gsp - k + i];
+      gsp -= k;
+      sp += k;
+      __syncthreads();
+    }
+    const int hp = lane_prefix(hm);
+    const int npop = min(nh, sp);
+    if (hungry && hp < npop) cont = stk[sp - 1 - hp];
+    sp -= npop;
+    // ---- lanes still without work start new items on free slots
+    const bool want = cont == KG_NOTASK;
+    const uint64_t wm = ballot64(want);
+    const uint64_t fm = ballot64(s_pend[lane] == 0);
+    const int navail = min(__popcll(wm), __popcll(fm));
+    if (navail > 0 && fe == tail && !exhausted) {
+      // fetch configurations into the ring (wave-uniform)
+      while (head < tail && e_cfg[head % KG_NE] < 0) head++;
+      const int room = KG_NE - (tail - head);
+      if (room > 0) {
+        int ch = max(1, min(a.cpw, (ncfg - last_k0) / (int)gridDim.x));
+        ch = min(ch, room);
+        if (lane == 0) s_chunk = atomicAdd(a.chunk_counter, ch);
+        __syncthreads();
+        const int k0 = s_chunk;
+        __syncthreads();
+        if (k0 >= ncfg) {
+          exhausted = true;
+        } else {
+          last_k0 = k0;
+          const int nk = min(ch, ncfg - k0);
+          if (lane < nk) {
+            const int e = (tail + lane) % KG_NE;
+            e_cfg[e] = cfg_of(k0 + lane);
+            e_open[e] = 1;
+            s_best[e][0] = s_best[e][1] = KG_NONE64;
+            s_uncmin[e][0] = s_uncmin[e][1] = KG_RANK_NONE;
+          }
+          tail += nk;
+          __syncthreads();
+        }
+      }
+    }
+    int myitem = -1, myent = -1;
+    const int rel_from = fe;
+    {
+      // the j-th wanting lane takes the j-th item in ring order; entries
+      // whose items are all started are passed (their start hold released)
+      const int wp = lane_prefix(wm);
+      int assigned = 0;
+      while (fe < tail) {
+        if (!fe_open) {
+          range(e_cfg[fe % KG_NE], it, itend);
+          fe_open = true;
+        }
+        if (it >= itend) {
+          fe++;
+          fe_open = false;
+          continue;
+        }
+        if (assigned >= navail) break;
+        const int take = min(navail - assigned, itend - it);
+        if (want && wp >= assigned && wp < assigned + take) {
+          myitem = it + (wp - assigned);
+          myent = fe % KG_NE;
+        }
+        assigned += take;
+        it += take;
+      }
+    }
+    // slot for each new item: the j-th free slot (by lane index)
+    int myslot = -1;
+    if (myitem >= 0) {
+      const int j = lane_prefix(wm);  // rank among wanting lanes (< navail)
+      uint64_t f = fm;
+      for (int q = 0; q < j; q++) f &= f - 1;
+      myslot = __ffsll((unsigned long long)f) - 1;
+    }
+    __syncthreads();
+    if (myitem >= 0) {
+      n_items++;
+      atomicAdd(&e_open[myent], 1);
+      const PairItem P = a.items[myitem];
+      const PlanDev& PL = a.cfg_plan ? a.plans[a.cfg_plan[P.cfg]] : a.plan;
+      const size_t tb = a.toff ? (size_t)a.toff[P.cfg] : (size_t)P.cfg * PL.nlinks;
+      const LinkDesc& LA = PL.links[P.a];
+      const int g = (P.rank >> KG_RANK_GROUP_BIT) & 1;
+      const double* XA = a.T64 + (tb + P.a) * 12;
+      const double* XB;
+      int geomB;
+      if (g == 0) {
+        XB = PL.env[P.b].X;
+        geomB = PL.env[P.b].geom;
+      } else {
+        XB = a.T64 + (tb + P.b) * 12;
+        geomB = PL.links[P.b].geom;
+      }
+      const double d0 = XA[9] - XB[9], d1 = XA[10] - XB[10], d2 = XA[11] - XB[11];
+#pragma unroll
+      for (int col = 0; col < 3; col++)
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+          s_rel[col * 3 + r][myslot] =
+              (Real)(XB[r * 3] * XA[col * 3] + XB[r * 3 + 1] * XA[col * 3 + 1] + XB[r * 3 + 2] * XA[col * 3 + 2]);
+      double tn = 0;
+#pragma unroll
+      for (int r = 0; r < 3; r++) {
+        const double v = XB[r * 3] * d0 + XB[r * 3 + 1] * d1 + XB[r * 3 + 2] * d2;
+        s_rel[9 + r][myslot] = (Real)v;
+        tn += v * v;
+      }
+      const GeomDesc& GA = a.pool.geoms[LA.geom];
+      const GeomDesc& GB = a.pool.geoms[geomB];
+      float tau, pad;
+      slot_bounds<Real, EXACT>(a, sqrt(tn), GA.rmax, GB.rmax, tau, pad);
+      s_tau[myslot] = tau;
+      s_pad[myslot] = pad;
+      s_cfgk[myslot] = myent;
+      s_rank[myslot] = P.rank;
+      s_item[myslot] = myitem;
+      s_pend[myslot] = 1;
+      s_unc[myslot] = 0;
+      cont = mk_task(myslot, 0, 1, GA.root, GB.root);
+    }
+    __syncthreads();
+    // release the start hold of the entries passed above (after the items
+    // started from them are counted)
+    if (lane == 0)
+      for (int e = rel_from; e < fe; e++) settle_entry(e % KG_NE, -1);
     __syncthreads();
 
-    auto cfg_of = [&](int k) { return a.cfg_list ? a.cfg_list[k0 + k] : k0 + k; };
-    auto range = [&](int c, int& b, int& e) {
-      if (a.offsets) {
-        b = a.offsets[c];
-        e = a.offsets[c + 1];
-      } else {
-        b = a.ibase ? a.ibase[c] : c * a.cap;
-        e = b + a.counts[c];
-      }
-    };
-    int kc = 0, it, itend;
-    range(cfg_of(0), it, itend);
-    int sp = 0, gsp = 0, nl = 0;
-    uint64_t cont = KG_NOTASK;  // this lane's depth-first continuation
-
-    auto ensure_room = [&](int room) {
-      if (sp + room <= SCAP) return;
-      const int h = SCAP / 2;
-      if (gsp + h <= a.spill_cap) {
-        for (int i = lane; i < h; i += 64) gsp_base[gsp + i] = stk[i];
-        gsp += h;
-      } else {
-        for (int i = lane; i < h; i += 64) {
-          uint64_t t = stk[i];
-          int sl = (int)(t >> 58);
-          s_unc[sl] = 1;
-          if (atomicAdd(&s_pend[sl], -1) == 1) {
-            const int rk2 = s_rank[sl];
-            atomicMin(&s_uncmin[s_cfgk[sl]][(rk2 >> KG_RANK_GROUP_BIT) & 1], rk2 & ((1 << KG_RANK_GROUP_BIT) - 1));
-          }
+    // ---- termination: no path, nothing pooled, no leaves, nothing to start
+    const bool working = cont != KG_NOTASK;
+    if (!ballot64(working)) {
+      if (sp == 0 && gsp == 0) {
+        if (nl > 0) {
+          run_leaves();
+        } else if (fe == tail && exhausted) {
+          break;
         }
-        if (lane == 0) atomicExch(a.status, 1);
       }
-      __syncthreads();
-      for (int i = lane; i < sp - h; i += 64) stk[i] = stk[i + h];
-      sp -= h;
-      __syncthreads();
-    };
+      continue;
+    }
 
-    auto settle = [&](int slot, int delta) {
-      const int old = atomicAdd(&s_pend[slot], delta);
-      if (old + delta == 0) {
-        const int rk = s_rank[slot];
-        const int g = (rk >> KG_RANK_GROUP_BIT) & 1, r = rk & ((1 << KG_RANK_GROUP_BIT) - 1);
-        if (s_unc[slot] && !item_dead(mode, s_best[s_cfgk[slot]], g, r)) atomicMin(&s_uncmin[s_cfgk[slot]][g], r);
-      }
-    };
-
-    auto run_leaves = [&]() {
-      const int m = min(nl, 64);
-      if (lane < m) {
-        const uint64_t e = lq[nl - 1 - lane];
-        const int slot = (int)(e >> 58);
-        const int ta = (int)((e >> 29) & 0x1fffffff), tb = (int)(e & 0x1fffffff);
-        const int rk = s_rank[slot];
-        const int g = (rk >> KG_RANK_GROUP_BIT) & 1, r = rk & ((1 << KG_RANK_GROUP_BIT) - 1);
-        const int ck = s_cfgk[slot];
-        if (!item_dead(mode, s_best[ck], g, r)) {
-          n_tri++;
-          Real rel[12];
+    // ---- process the continuation: up to two box tests
+    uint64_t out0 = 0, out1 = 0, lf0 = 0, lf1 = 0;
+    int nout = 0, nlf = 0, slot = 0;
+    if (working) {
+      const uint64_t task = cont;
+      cont = KG_NOTASK;
+      slot = (int)(task >> 58);
+      const int side = (int)((task >> 57) & 1), single = (int)((task >> 56) & 1);
+      const int x = (int)((task >> 28) & 0xfffffff), y = (int)(task & 0xfffffff);
+      const int rk = s_rank[slot];
+      const int g = (rk >> KG_RANK_GROUP_BIT) & 1, r = rk & ((1 << KG_RANK_GROUP_BIT) - 1);
+      if (!item_dead(mode, s_best[s_cfgk[slot]], g, r)) {
+        const NodeF O = load_nodef(a.pool.nodes, y);
+        const NodeF S0 = load_nodef(a.pool.nodes, x);
+        const NodeF S1 = load_nodef(a.pool.nodes, single ? x : x + 1);
+        // S (the split node) is in mesh A when A was split (or for a
+        // single pair); otherwise the item transform is inverted
+        const bool aIsS = single || side == 0;
+        Real rel[12];
 #pragma unroll
-          for (int k = 0; k < 12; k++) rel[k] = s_rel[k][slot];
-          V3T<Real> tp[3], tq[3];
-          load_tri<Real>(a.pool, ta, tp);
-          load_tri<Real>(a.pool, tb, tq);
+        for (int k = 0; k < 12; k++) rel[k] = s_rel[k][slot];
+        if (!aIsS) {
+          Real inv[12];
 #pragma unroll
-          for (int v = 0; v < 3; v++) tp[v] = xf_apply<Real>(rel, tp[v]);
-          const int res = tri_tri_sat<Real, EXACT>(tp, tq, (Real)s_tau[slot]);
-          if (res == TT_HIT) {
-            const unsigned long long key = ((unsigned long long)(uint32_t)r << 32) | (uint32_t)s_item[slot];
-            atomicMin(&s_best[ck][g], key);
-          } else if (res == TT_UNSURE) {
-            s_unc[slot] = 1;
-          }
+          for (int c = 0; c < 3; c++)
+#pragma unroll
+            for (int r2 = 0; r2 < 3; r2++) inv[c * 3 + r2] = rel[r2 * 3 + c];
+#pragma unroll
+          for (int r2 = 0; r2 < 3; r2++)
+            inv[9 + r2] = -(rel[r2 * 3] * rel[9] + rel[r2 * 3 + 1] * rel[10] + rel[r2 * 3 + 2] * rel[11]);
+#pragma unroll
+          for (int k = 0; k < 12; k++) rel[k] = inv[k];
         }
-        settle(slot, -1);
-      }
-      nl -= m;
-      __syncthreads();
-    };
-
-    for (;;) {
-      // ---- lanes without a continuation pop the pool (bring spilled tasks
-      // back first when it runs low)
-      const bool hungry = cont == KG_NOTASK;
-      const uint64_t hm = ballot64(hungry);
-      const int nh = __popcll(hm);
-      if (nh > sp && gsp > 0) {
-        const int k = min(gsp, SCAP / 2);
-        for (int i = sp - 1 - lane; i >= 0; i -= 64) stk[i + k] = stk[i];
-        __syncthreads();
-        for (int i = lane; i < k; i += 64) stk[i] = gsp_base[gsp - k + i];
-        gsp -= k;
-        sp += k;
-        __syncthreads();
-      }
-      const int hp = lane_prefix(hm);
-      const int npop = min(nh, sp);
-      if (hungry && hp < npop) cont = stk[sp - 1 - hp];
-      sp -= npop;
-      // ---- lanes still without work start new items on free slots
-      const bool want = cont == KG_NOTASK;
-      const uint64_t wm = ballot64(want);
-      const uint64_t fm = ballot64(s_pend[lane] == 0);
-      int navail = min(__popcll(wm), __popcll(fm));
-      int myitem = -1, mycfgk = -1;
-      {
-        // the j-th wanting lane takes the j-th free slot and the j-th item
-        const int wp = lane_prefix(wm);
-        int assigned = 0;
-        while (assigned < navail && kc < nk) {
-          if (it >= itend) {
-            kc++;
-            if (kc < nk) range(cfg_of(kc), it, itend);
+        TaskFrame<Real> F;
+        task_frame<Real>(O, rel, F);
+        const Real pad = (Real)s_pad[slot];
+        const bool lO = O.ch < 0;
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+          if (k == 1 && single) break;
+          const NodeF& S = k == 0 ? S0 : S1;  // k is a constant after unrolling
+          const int nodeS = x + k;
+          n_box++;
+          if (!frame_overlap<Real>(F, S, pad, pad_rel)) continue;
+          const bool lS = S.ch < 0;
+          if (lS && lO) {
+            const uint64_t w = aIsS ? mk_leaf(slot, ~S.ch, ~O.ch) : mk_leaf(slot, ~O.ch, ~S.ch);
+            if (nlf == 0) lf0 = w; else lf1 = w;
+            nlf++;
             continue;
           }
-          const int take = min(navail - assigned, itend - it);
-          if (want && wp >= assigned && wp < assigned + take) {
-            myitem = it + (wp - assigned);
-            mycfgk = kc;
-          }
-          assigned += take;
-          it += take;
-        }
-        navail = assigned;
-      }
-      // slot for each new item: the j-th free slot (by lane index)
-      int myslot = -1;
-      if (myitem >= 0) {
-        const int j = lane_prefix(wm);  // rank among wanting lanes (< navail)
-        uint64_t f = fm;
-        for (int q = 0; q < j; q++) f &= f - 1;
-        myslot = __ffsll((unsigned long long)f) - 1;
-      }
-      __syncthreads();
-      if (myitem >= 0) {
-        n_items++;
-        const PairItem P = a.items[myitem];
-        const PlanDev& PL = a.cfg_plan ? a.plans[a.cfg_plan[P.cfg]] : a.plan;
-        const size_t tb = a.toff ? (size_t)a.toff[P.cfg] : (size_t)P.cfg * PL.nlinks;
-        const LinkDesc& LA = PL.links[P.a];
-        const int g = (P.rank >> KG_RANK_GROUP_BIT) & 1;
-        const double* XA = a.T64 + (tb + P.a) * 12;
-        const double* XB;
-        int geomB;
-        if (g == 0) {
-          XB = PL.env[P.b].X;
-          geomB = PL.env[P.b].geom;
-        } else {
-          XB = a.T64 + (tb + P.b) * 12;
-          geomB = PL.links[P.b].geom;
-        }
-        const double d0 = XA[9] - XB[9], d1 = XA[10] - XB[10], d2 = XA[11] - XB[11];
-#pragma unroll
-        for (int col = 0; col < 3; col++)
-#pragma unroll
-          for (int r = 0; r < 3; r++)
-            s_rel[col * 3 + r][myslot] =
-                (Real)(XB[r * 3] * XA[col * 3] + XB[r * 3 + 1] * XA[col * 3 + 1] + XB[r * 3 + 2] * XA[col * 3 + 2]);
-        double tn = 0;
-#pragma unroll
-        for (int r = 0; r < 3; r++) {
-          const double v = XB[r * 3] * d0 + XB[r * 3 + 1] * d1 + XB[r * 3 + 2] * d2;
-          s_rel[9 + r][myslot] = (Real)v;
-          tn += v * v;
-        }
-        const GeomDesc& GA = a.pool.geoms[LA.geom];
-        const GeomDesc& GB = a.pool.geoms[geomB];
-        float tau, pad;
-        slot_bounds<Real, EXACT>(a, sqrt(tn), GA.rmax, GB.rmax, tau, pad);
-        s_tau[myslot] = tau;
-        s_pad[myslot] = pad;
-        s_cfgk[myslot] = mycfgk;
-        s_rank[myslot] = P.rank;
-        s_item[myslot] = myitem;
-        s_pend[myslot] = 1;
-        s_unc[myslot] = 0;
-        cont = mk_task(myslot, 0, 1, GA.root, GB.root);
-      }
-      __syncthreads();
-
-      // ---- termination: no path, nothing pooled, no items left
-      const bool working = cont != KG_NOTASK;
-      if (!ballot64(working)) {
-        if (sp == 0 && gsp == 0) {
-          if (nl == 0) break;
-          run_leaves();
-        }
-        continue;
-      }
-
-      // ---- process the continuation: up to two box tests
-      uint64_t out0 = 0, out1 = 0, lf0 = 0, lf1 = 0;
-      int nout = 0, nlf = 0, slot = 0;
-      if (working) {
-        const uint64_t task = cont;
-        cont = KG_NOTASK;
-        slot = (int)(task >> 58);
-        const int side = (int)((task >> 57) & 1), single = (int)((task >> 56) & 1);
-        const int x = (int)((task >> 28) & 0xfffffff), y = (int)(task & 0xfffffff);
-        const int rk = s_rank[slot];
-        const int g = (rk >> KG_RANK_GROUP_BIT) & 1, r = rk & ((1 << KG_RANK_GROUP_BIT) - 1);
-        if (!item_dead(mode, s_best[s_cfgk[slot]], g, r)) {
-          const NodeF O = load_nodef(a.pool.nodes, y);
-          const NodeF S0 = load_nodef(a.pool.nodes, x);
-          const NodeF S1 = load_nodef(a.pool.nodes, single ? x : x + 1);
-          Real rel[12];
-#pragma unroll
-          for (int k = 0; k < 12; k++) rel[k] = s_rel[k][slot];
-          const Real pad = (Real)s_pad[slot];
-          const bool aIsS = single || side == 0;
-#pragma unroll 1
-          for (int k = 0; k < (single ? 1 : 2); k++) {
-            const NodeF S = sel(k == 0, S0, S1);
-            const NodeF A = sel(aIsS, S, O);
-            const NodeF B = sel(aIsS, O, S);
-            n_box++;
-            if (!obb_overlap<Real>(A.c, A.e, A.R, B.c, B.e, B.R, rel, pad, pad_rel)) continue;
-            const bool la = A.ch < 0, lb = B.ch < 0;
-            if (la && lb) {
-              const uint64_t w = mk_leaf(slot, ~A.ch, ~B.ch);
-              if (nlf == 0) lf0 = w; else lf1 = w;
-              nlf++;
-            } else {
-              bool splitA;
-              if (la) splitA = false;
-              else if (lb) splitA = true;
-              else splitA = (A.e[0] + A.e[1] + A.e[2]) >= (B.e[0] + B.e[1] + B.e[2]);
-              const int nodeS = x + k;
-              const int nodeA = aIsS ? nodeS : y;
-              const int nodeB = aIsS ? y : nodeS;
-              const uint64_t w = splitA ? mk_task(slot, 0, 0, A.ch, nodeB) : mk_task(slot, 1, 0, B.ch, nodeA);
-              if (nout == 0) out0 = w; else out1 = w;
-              nout++;
-            }
-          }
+          const bool splitS = lO || (!lS && (S.e[0] + S.e[1] + S.e[2]) >= (O.e[0] + O.e[1] + O.e[2]));
+          // side 0: children of an A-node against a B-node; side 1: the reverse
+          const int nside = (splitS == aIsS) ? 0 : 1;
+          const uint64_t w = splitS ? mk_task(slot, nside, 0, S.ch, y) : mk_task(slot, nside, 0, O.ch, nodeS);
+          if (nout == 0) out0 = w; else out1 = w;
+          nout++;
         }
       }
-      // keep the first child pair as this lane's continuation
-      if (nout >= 1) cont = out0;
-
-      // ---- queue leaves (flush batches first if the queue could overflow)
-      {
-        const uint64_t l1 = ballot64(nlf >= 1), l2 = ballot64(nlf >= 2);
-        const int ltot = __popcll(l1) + __popcll(l2);
-        if (nl + ltot > KG_LQ) {
-          __syncthreads();
-          while (nl + ltot > KG_LQ) run_leaves();
-        }
-        const int pos = nl + lane_prefix(l1) + lane_prefix(l2);
-        if (nlf >= 1) lq[pos] = lf0;
-        if (nlf >= 2) lq[pos + 1] = lf1;
-        nl += ltot;
-      }
-      if (working) settle(slot, nout + nlf - 1);
-
-      // ---- push the second child pair to the pool
-      const uint64_t pm = ballot64(nout >= 2);
-      const int ptot = __popcll(pm);
-      ensure_room(ptot);
-      if (nout >= 2) stk[sp + lane_prefix(pm)] = out1;
-      sp += ptot;
-      __syncthreads();
-      if (nl >= a.leaf_min) run_leaves();
     }
-    for (int k = lane; k < nk; k += 64) {
-      const int cc = cfg_of(k);
-      a.best[2 * cc] = s_best[k][0];
-      a.best[2 * cc + 1] = s_best[k][1];
-      a.unc[2 * cc] = s_uncmin[k][0];
-      a.unc[2 * cc + 1] = s_uncmin[k][1];
+    // keep the first child pair as this lane's continuation
+    if (nout >= 1) cont = out0;
+
+    // ---- queue leaves (flush batches first if the queue could overflow)
+    {
+      const uint64_t l1 = ballot64(nlf >= 1), l2 = ballot64(nlf >= 2);
+      const int ltot = __popcll(l1) + __popcll(l2);
+      if (nl + ltot > KG_LQ) {
+        __syncthreads();
+        while (nl + ltot > KG_LQ) run_leaves();
+      }
+      const int pos = nl + lane_prefix(l1) + lane_prefix(l2);
+      if (nlf >= 1) lq[pos] = lf0;
+      if (nlf >= 2) lq[pos + 1] = lf1;
+      nl += ltot;
     }
+    if (working) settle(slot, nout + nlf - 1);
+
+    // ---- push the second child pair to the pool
+    const uint64_t pm = ballot64(nout >= 2);
+    const int ptot = __popcll(pm);
+    ensure_room(ptot);
+    if (nout >= 2) stk[sp + lane_prefix(pm)] = out1;
+    sp += ptot;
     __syncthreads();
+    if (nl >= a.leaf_min) run_leaves();
   }
   if (a.counters) {
     atomicAdd(&a.counters[0], (unsigned long long)n_box);
