@@ -113,6 +113,9 @@ int kg_robot_set_link_geom(kg_robot r, int link, kg_geom g);
 /* RobotModel.enableSelfCollision (Python/klampt/src/robotsim.cpp:5504-5521) */
 int kg_robot_enable_self_collision(kg_robot r, int link1, int link2, int enabled);
 int kg_robot_self_collision_enabled(kg_robot r, int link1, int link2, int* out);
+/* Replaces the self-collision pair set by the npairs pairs[2k], pairs[2k+1]
+ * (the .rob "selfcollision" list, Cpp/docs/Manual-FileTypes.md:212). */
+int kg_robot_set_self_pairs(kg_robot r, const int32_t* pairs, int npairs);
 /* RobotModel.setConfig (Python/klampt/src/robotsim.cpp:5311-5321):
  * UpdateConfig (FK) + UpdateGeometry. */
 int kg_robot_set_config(kg_robot r, const double* q);
@@ -124,6 +127,9 @@ int kg_robot_self_collides(kg_robot r, int* out);
 /* Batched RobotModel::UpdateConfig in FP32: Q[N][n] -> T[N][n][12]. */
 int kg_fk_batch(kg_robot r, const float* Q, int N, float* T_out);
 int kg_fk_batch_dev(kg_robot r, const float* dQ, int N, float* dT_out, void* stream);
+/* The same in FP64 (the arithmetic of the feasibility broad phase). */
+int kg_fk_batch64(kg_robot r, const double* Q, int N, double* T_out);
+int kg_fk_batch64_dev(kg_robot r, const double* dQ, int N, double* dT_out, void* stream);
 
 /* ---- world -------------------------------------------------------------
  * WorldModel ID layout (Cpp/Modeling/World.cpp:47-53,156-180): terrains, then

@@ -60,6 +60,9 @@ SIGNATURES = {
     "kg_robot_self_collides": [_vp, _pi],
     "kg_fk_batch": [_vp, _pf, _c_int, _pf],
     "kg_fk_batch_dev": [_vp, _vp, _c_int, _vp, _vp],
+    "kg_fk_batch64": [_vp, _pd, _c_int, _pd],
+    "kg_fk_batch64_dev": [_vp, _vp, _c_int, _vp, _vp],
+    "kg_robot_set_self_pairs": [_vp, _pi, _c_int],
     "kg_world_create": [_vp, _pvp],
     "kg_world_destroy": [_vp],
     "kg_world_add_terrain": [_vp, _vp, _pi],

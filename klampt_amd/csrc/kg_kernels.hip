@@ -178,8 +178,12 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
   const size_t qb = a.qoff ? (size_t)a.qoff[c] : (size_t)c * n;
   double* T = a.T64 + tb * 12;
   double* LB = a.LB + tb * 6;
+  // union of the active links' boxes
+  double ulo[3] = {1e300, 1e300, 1e300}, uhi[3] = {-1e300, -1e300, -1e300};
+  uint64_t active = 0;
   if (MODE != 1) {
     bool jl = true;
+    X64 Xprev;  // frame of link l-1, kept in registers for serial chains
     for (int l = 0; l < n; l++) {
       double q = a.Qf ? (double)a.Qf[qb + l] : a.Qd[qb + l];
       if (q < P.qmin[l] || q > P.qmax[l]) jl = false;
@@ -190,19 +194,24 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
       x64_mul(T0, J, Lx);
       if (L.parent >= 0) {
         X64 Pp;
-        x64_load(T + 12 * L.parent, Pp);
+        if (L.parent == l - 1) Pp = Xprev;
+        else x64_load(T + 12 * L.parent, Pp);
         x64_mul(Pp, Lx, X);
       } else {
         X = Lx;
       }
+      Xprev = X;
       x64_store(T + 12 * l, X);
       if (L.geom >= 0) {
         const GeomDesc& g = a.pool.geoms[L.geom];
+        active |= 1ull << l;
         for (int r = 0; r < 3; r++) {
           double wc = X.R[r] * g.lc[0] + X.R[3 + r] * g.lc[1] + X.R[6 + r] * g.lc[2] + X.t[r];
           double ext = fabs(X.R[r]) * g.le[0] + fabs(X.R[3 + r]) * g.le[1] + fabs(X.R[6 + r]) * g.le[2];
           LB[6 * l + r] = wc - ext;
           LB[6 * l + 3 + r] = wc + ext;
+          ulo[r] = fmin(ulo[r], wc - ext);
+          uhi[r] = fmax(uhi[r], wc + ext);
         }
       }
     }
@@ -215,17 +224,15 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
     if (a.mode == KG_MODE_ANY && !a.jl[c]) return;
   }
   const bool ordered = a.mode == KG_MODE_ORDERED;
-  // union of the active links' boxes
-  double ulo[3] = {1e300, 1e300, 1e300}, uhi[3] = {-1e300, -1e300, -1e300};
-  uint64_t active = 0;
-  for (int l = 0; l < n; l++)
-    if (P.links[l].geom >= 0) {
-      active |= 1ull << l;
-      for (int r = 0; r < 3; r++) {
-        ulo[r] = fmin(ulo[r], LB[6 * l + r]);
-        uhi[r] = fmax(uhi[r], LB[6 * l + 3 + r]);
+  if (MODE == 1)
+    for (int l = 0; l < n; l++)
+      if (P.links[l].geom >= 0) {
+        active |= 1ull << l;
+        for (int r = 0; r < 3; r++) {
+          ulo[r] = fmin(ulo[r], LB[6 * l + r]);
+          uhi[r] = fmax(uhi[r], LB[6 * l + 3 + r]);
+        }
       }
-    }
   // env entries surviving against the link union; union of survivors
   double elo[3] = {1e300, 1e300, 1e300}, ehi[3] = {-1e300, -1e300, -1e300};
   int m2 = 0;
@@ -531,6 +538,22 @@ __device__ inline V3T<Real> xf_apply(const Real* X, V3T<Real> p) {
 }
 
 __device__ inline uint64_t ballot64(bool p) { return __ballot(p); }
+
+// Position of the j-th (0-based) set bit of m; requires popcount(m) > j.
+__device__ inline int nth_set_bit(uint64_t m, int j) {
+  int pos = 0, c = __popc((uint32_t)m);
+  if (j >= c) { j -= c; m >>= 32; pos += 32; }
+  c = __popc((uint32_t)m & 0xffffu);
+  if (j >= c) { j -= c; m >>= 16; pos += 16; }
+  c = __popc((uint32_t)m & 0xffu);
+  if (j >= c) { j -= c; m >>= 8; pos += 8; }
+  c = __popc((uint32_t)m & 0xfu);
+  if (j >= c) { j -= c; m >>= 4; pos += 4; }
+  c = __popc((uint32_t)m & 0x3u);
+  if (j >= c) { j -= c; m >>= 2; pos += 2; }
+  if (j >= (int)(m & 1)) pos += 1;
+  return pos;
+}
 __device__ inline int lane_prefix(uint64_t m) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
 }
@@ -797,12 +820,7 @@ __global__ __launch_bounds__(64) void k_collide(TravArgs a) {
     }
     // slot for each new item: the j-th free slot (by lane index)
     int myslot = -1;
-    if (myitem >= 0) {
-      const int j = lane_prefix(wm);  // rank among wanting lanes (< navail)
-      uint64_t f = fm;
-      for (int q = 0; q < j; q++) f &= f - 1;
-      myslot = __ffsll((unsigned long long)f) - 1;
-    }
+    if (myitem >= 0) myslot = nth_set_bit(fm, lane_prefix(wm));  // rank among wanting lanes < navail
     __syncthreads();
     if (myitem >= 0) {
       n_items++;
@@ -1698,6 +1716,7 @@ __global__ __launch_bounds__(256) void k_fk64(const LinkDesc* __restrict__ links
   if (c < nreset) reset[c] = 0;
   if (c >= N) return;
   double* T = T64 + (size_t)c * n * 12;
+  X64 Xprev;  // frame of link l-1, kept in registers for serial chains
   for (int l = 0; l < n; l++) {
     const double q = Qf ? (double)Qf[(size_t)c * n + l] : Qd[(size_t)c * n + l];
     const LinkDesc& L = links[l];
@@ -1707,11 +1726,13 @@ __global__ __launch_bounds__(256) void k_fk64(const LinkDesc* __restrict__ links
     x64_mul(T0, J, Lx);
     if (L.parent >= 0) {
       X64 Pp;
-      x64_load(T + 12 * L.parent, Pp);
+      if (L.parent == l - 1) Pp = Xprev;
+      else x64_load(T + 12 * L.parent, Pp);
       x64_mul(Pp, Lx, X);
     } else {
       X = Lx;
     }
+    Xprev = X;
     x64_store(T + 12 * l, X);
   }
 }

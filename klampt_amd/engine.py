@@ -174,6 +174,10 @@ class Robot:
     def enable_self_collision(self, i, j, enabled=True):
         call("kg_robot_enable_self_collision", self.h, i, j, 1 if enabled else 0)
 
+    def set_self_pairs(self, pairs):
+        P = _i(pairs).reshape(-1, 2)
+        call("kg_robot_set_self_pairs", self.h, _p(P, _pi), P.shape[0])
+
     def self_collision_enabled(self, i, j):
         out = ctypes.c_int32()
         call("kg_robot_self_collision_enabled", self.h, i, j, ctypes.byref(out))
@@ -196,6 +200,12 @@ class Robot:
         out = ctypes.c_int32()
         call("kg_robot_self_collides", self.h, ctypes.byref(out))
         return bool(out.value)
+
+    def fk_batch64(self, Q):
+        Q = _d(Q).reshape(-1, self.n)
+        T = np.zeros((Q.shape[0], self.n, 12))
+        call("kg_fk_batch64", self.h, _p(Q, _pd), Q.shape[0], _p(T, _pd))
+        return T
 
     def fk_batch(self, Q):
         Q = _f(Q).reshape(-1, self.n)

@@ -178,3 +178,33 @@ def test_empty_batch_and_no_geometry(ctx, c1):
     wb.add_robot(bare)
     Q = S.uniform_configs(spec, 100, seed=1)
     assert wb.feasible_batch(Q)[0].all()
+
+
+def test_fk64_matches_oracle(ctx):
+    """kg_fk_batch64: the FP64 FK of the broad phase equals the oracle's."""
+    spec = S.arm7()
+    r = engine.Robot.from_spec(ctx, spec)
+    Q = S.uniform_configs(spec, 256, seed=5).astype(np.float64)
+    T = r.fk_batch64(Q)
+    ow = oracle.OracleWorld()
+    orb = ow.add_robot(spec)
+    err = max(float(np.abs(T[c] - ow.fk(orb, Q[c])).max()) for c in range(0, 256, 4))
+    assert err < 1e-12
+
+
+def test_set_self_pairs_replaces_defaults(ctx):
+    """kg_robot_set_self_pairs: only the listed pairs are self-checked."""
+    import dataclasses
+    sc = S.boxes_scene(0)
+    pairs = np.array([[0, 3], [1, 5], [6, 2]], np.int32)
+    w, r = engine.world_from_scene(ctx, sc)
+    r.set_self_pairs(pairs)
+    assert r.self_collision_enabled(2, 6) and not r.self_collision_enabled(0, 2)
+    spec2 = dataclasses.replace(sc.robot, self_pairs=np.sort(pairs, axis=1))
+    ow, orb = oracle.world_from_scene(S.Scene(spec2, sc.terrains, sc.objects))
+    Q = S.uniform_configs(sc.robot, 2048, seed=9)
+    feas, env, slf, fp = w.feasible_batch(Q, flags=True, first_pair=True)
+    f64, e64, s64, p64 = ow.feasible_batch(orb, Q)
+    assert np.array_equal(w.pair_list(0), ow.pair_list(orb))
+    n = _check_exact(ow, orb, Q, np.c_[feas, env, slf, fp], np.c_[f64, e64, s64, p64], "custom self pairs")
+    assert n <= 2
