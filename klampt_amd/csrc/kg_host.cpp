@@ -428,6 +428,7 @@ struct Layout {
   size_t nrows = 0;                 // frame rows (links) over the batch
   size_t ncap = 0;                  // item capacity over the batch
   int cap = 0;                      // per-configuration capacity (single)
+  int maxcap = 0;                   // largest per-configuration capacity
   Tol tol32{}, tol64{};
 };
 
@@ -470,6 +471,7 @@ void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* 
   pa.toff = L.toff;
   pa.ibase = L.ibase;
   const bool fixed = L.ncap * sizeof(PairItem) <= (2048ull << 20);
+  int item_total = 0;  // items of the count/scan/fill layout
   hip_check(hipEventRecord(c->ev[0], s), "hipEventRecord");
   if (fixed || L.multi) {
     need(fixed, "heterogeneous batch exceeds the item capacity limit (2 GiB of pair items)");
@@ -488,6 +490,7 @@ void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* 
     int t = 0;
     hip_check(hipMemcpyAsync(&t, pa.offsets + N, sizeof(int), hipMemcpyDeviceToHost, s), "copy total");
     hip_check(hipStreamSynchronize(s), "sync");
+    item_total = t;
     hip_check(c->wsItems.ensure(std::max<size_t>(1, (size_t)t) * sizeof(PairItem)), "hipMalloc(items)");
     pa.items = c->wsItems.as<PairItem>();
     hip_check(kgl_prep_fill(pa, s), "prep_fill");
@@ -495,8 +498,9 @@ void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* 
   hip_check(hipEventRecord(c->ev[1], s), "hipEventRecord");
 
   // defaults measured on 65,536 configurations of the 7-DOF arm against the
-  // 200k-triangle clutter: the 512-entry pool leaves LDS for 3 waves/SIMD
-  const int scap = env_int("KG_SCAP", c->tune_scap ? c->tune_scap : 512) == 1024 ? 1024 : 512;
+  // 200k-triangle clutter: the 256-entry pool leaves LDS for 4 waves/SIMD
+  const int scap_req = env_int("KG_SCAP", c->tune_scap ? c->tune_scap : 256);
+  const int scap = scap_req == 1024 ? 1024 : (scap_req == 256 ? 256 : 512);
   const int wpc = env_int("KG_WAVES_PER_CU", c->tune_wpc);
   const int grid32 = wpc > 0 ? wpc * c->num_cus : kgl_collide_grid(0, c->num_cus, scap);
   const int grid64 = std::min(kgl_collide_grid(1, c->num_cus, scap), std::max(1, c->num_cus));
@@ -525,7 +529,7 @@ void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* 
   ta.spill_cap = spill_cap;
   ta.stack_cap = 0;
   ta.status = status + ST_SPILL;
-  ta.cpw = std::max(1, std::min(64, env_int("KG_CPW", c->tune_cpw ? c->tune_cpw : 8)));
+  ta.cpw = std::max(1, std::min(64, env_int("KG_CPW", c->tune_cpw ? c->tune_cpw : 16)));
   ta.leaf_min = std::max(1, std::min(64, env_int("KG_LEAF_MIN", c->tune_leaf ? c->tune_leaf : 1)));
   ta.grid = grid32;
   ta.counters = (unsigned long long*)(status + ST_CNT);
@@ -534,6 +538,11 @@ void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* 
   ta.toff = L.toff;
   ta.ibase = L.ibase;
   ta.scap = scap;
+  // item start records (after the items exist, before the traversal)
+  const size_t nrec = pa.offsets ? (size_t)std::max(1, item_total) : std::max<size_t>(1, L.ncap);
+  hip_check(c->wsRec.ensure(nrec * sizeof(ItemRec)), "hipMalloc(records)");
+  ta.recs = c->wsRec.as<ItemRec>();
+  hip_check(kgl_items(ta, N, std::max(1, L.maxcap), c->wsRec.as<ItemRec>(), s), "items");
   const int slot = (int)(c->tr_count % kg_ctx_s::TRING);
   hip_check(hipEventRecord(c->ev[2], s), "hipEventRecord");
   hip_check(hipEventRecord(c->tr_beg[slot], s), "hipEventRecord");
@@ -595,6 +604,7 @@ void feasible_dev(kg_world_s* w, int ri, const float* Qf, const double* Qd, int 
   Layout L;
   L.P = &P;
   L.cap = active_links(w->robots[ri]) * P.nenv + P.nself;
+  L.maxcap = L.cap;
   L.nrows = (size_t)std::max(N, 0) * P.dev.nlinks;
   L.ncap = (size_t)L.cap * std::max(N, 0);
   L.tol32 = P.tol32;
@@ -674,7 +684,7 @@ int kg_ctx_destroy(kg_ctx c) {
                       &c->wsOut8a, &c->wsOut8b, &c->wsOut8c, &c->wsOut32, &c->wsF, &c->wsG,
                       &c->wsSdfScratch, &c->wsPoff, &c->wsPts, &c->wsLoss, &c->wsDQ, &c->wsDl, &c->wsEA, &c->wsEB,
                       &c->wsESeg, &c->wsESoff, &c->wsENs, &c->wsEOoff, &c->wsEOrder, &c->wsEQ, &c->wsEFeas,
-                      &c->wsEU, &c->wsEDone, &c->mPlans, &c->mCfgPlan, &c->mQoff, &c->mToff, &c->mIbase};
+                      &c->wsEU, &c->wsEDone, &c->wsRec, &c->mPlans, &c->mCfgPlan, &c->mQoff, &c->mToff, &c->mIbase};
     for (auto* b : bufs) b->release();
     if (c->hstatus) (void)hipHostFree(c->hstatus);
     for (auto& e : c->ev)
@@ -1914,6 +1924,7 @@ Layout multi_layout(kg_ctx_s* c, int nb, kg_world_s* const* worlds, const int32_
   L.ibase = c->mIbase.as<int>();
   L.nrows = rows;
   L.ncap = caps;
+  L.maxcap = *std::max_element(caps_b.begin(), caps_b.end());
   L.tol32 = t32;
   L.tol64 = t64;
   (void)s;
@@ -1979,10 +1990,10 @@ extern "C" int kg_autotune(int nb, const kg_world* worlds, const int32_t* ris, c
     hip_check(c->wsOut8b.ensure(N), "hipMalloc");
     const int cpws[] = {4, 8, 16, 32};
     const int leafs[] = {1, 8};
-    const int scaps[] = {512, 1024};
-    const int wpcs[] = {0, 6};
+    const int scaps[] = {256, 512};
+    const int wpcs[] = {0};
     float best = 1e30f;
-    int bc[4] = {8, 1, 512, 0};
+    int bc[4] = {16, 1, 256, 0};
     for (int scap : scaps)
       for (int wpc : wpcs)
         for (int cpw : cpws)

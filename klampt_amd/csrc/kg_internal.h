@@ -157,8 +157,12 @@ struct TravArgs {
   const int* toff;
   const int* ibase;
   int scap;                   // LDS pool capacity variant (512 or 1024)
+  const ItemRec* recs;        // per item start records (FP32 pass)
 };
 hipError_t kgl_collide_f32(const TravArgs& a, hipStream_t s);
+// Item start records for every item of every configuration (lane per
+// configuration x item position below `maxcap`).
+hipError_t kgl_items(const TravArgs& a, int N, int maxcap, ItemRec* recs, hipStream_t s);
 hipError_t kgl_collide_f64(const TravArgs& a, hipStream_t s);
 
 struct DistArgs {
@@ -316,7 +320,7 @@ struct kg_ctx_s {
   bool done_valid = false;
   DevBuf wsQ, wsT64, wsLB, wsJL, wsCounts, wsOffsets, wsItems, wsScanTmp, wsBest, wsUnc, wsChunk, wsSpill,
       wsStatus, wsRedo, wsOut8a, wsOut8b, wsOut8c, wsOut32, wsF, wsG, wsSdfScratch, wsPoff, wsPts, wsLoss, wsDQ,
-      wsDl, wsEA, wsEB, wsESeg, wsESoff, wsENs, wsEOoff, wsEOrder, wsEQ, wsEFeas, wsEU, wsEDone;
+      wsDl, wsEA, wsEB, wsESeg, wsESoff, wsENs, wsEOoff, wsEOrder, wsEQ, wsEFeas, wsEU, wsEDone, wsRec;
   int num_cus = 256;
   // traversal settings (autotuner; 0 = default)
   int tune_cpw = 0, tune_leaf = 0, tune_wpc = 0, tune_scap = 0;

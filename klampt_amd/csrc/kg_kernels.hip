@@ -586,6 +586,45 @@ __device__ inline void slot_bounds(const TravArgs& a, double tnorm, double rA, d
   pad = (float)(2.0 * ev + 32.0 * u * (tnorm + rA + rB));
 }
 
+// Frame of pair item P: rel maps mesh A's frame to mesh B's (FP64 product,
+// rounded to Real), with its error bounds and the two BVH roots.
+template <typename Real, bool EXACT>
+__device__ inline void item_frame(const TravArgs& a, const PairItem& P, Real* rel, float& tau, float& pad, int& rootA,
+                                  int& rootB) {
+  const PlanDev& PL = a.cfg_plan ? a.plans[a.cfg_plan[P.cfg]] : a.plan;
+  const size_t tb = a.toff ? (size_t)a.toff[P.cfg] : (size_t)P.cfg * PL.nlinks;
+  const LinkDesc& LA = PL.links[P.a];
+  const int g = (P.rank >> KG_RANK_GROUP_BIT) & 1;
+  const double* XA = a.T64 + (tb + P.a) * 12;
+  const double* XB;
+  int geomB;
+  if (g == 0) {
+    XB = PL.env[P.b].X;
+    geomB = PL.env[P.b].geom;
+  } else {
+    XB = a.T64 + (tb + P.b) * 12;
+    geomB = PL.links[P.b].geom;
+  }
+  const double d0 = XA[9] - XB[9], d1 = XA[10] - XB[10], d2 = XA[11] - XB[11];
+#pragma unroll
+  for (int col = 0; col < 3; col++)
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+      rel[col * 3 + r] = (Real)(XB[r * 3] * XA[col * 3] + XB[r * 3 + 1] * XA[col * 3 + 1] + XB[r * 3 + 2] * XA[col * 3 + 2]);
+  double tn = 0;
+#pragma unroll
+  for (int r = 0; r < 3; r++) {
+    const double v = XB[r * 3] * d0 + XB[r * 3 + 1] * d1 + XB[r * 3 + 2] * d2;
+    rel[9 + r] = (Real)v;
+    tn += v * v;
+  }
+  const GeomDesc& GA = a.pool.geoms[LA.geom];
+  const GeomDesc& GB = a.pool.geoms[geomB];
+  rootA = GA.root;
+  rootB = GB.root;
+  slot_bounds<Real, EXACT>(a, sqrt(tn), GA.rmax, GB.rmax, tau, pad);
+}
+
 // Task word: slot(6) | side(1) | single(1) | X(28) | Y(28).
 //   single = 1: test the pair (X as A-node, Y as B-node).
 //   single = 0: X is the first of two sibling children; test (X+k, Y) when
@@ -619,16 +658,20 @@ __device__ inline uint64_t mk_leaf(int slot, int ta, int tb) {
 // a heavy item never holds the wave's other lanes idle at a chunk boundary.
 template <typename Real, bool EXACT, int SCAP>
 __global__ __launch_bounds__(64) void k_collide(TravArgs a) {
+  // the small-pool variant also halves the leaf queue and the ring so that
+  // four waves fit per SIMD
+  constexpr int LQ = SCAP >= 512 ? KG_LQ : KG_LQ / 2;
+  constexpr int NE = SCAP >= 512 ? KG_NE : KG_NE / 2;
   __shared__ uint64_t stk[SCAP];
-  __shared__ uint64_t lq[KG_LQ];
+  __shared__ uint64_t lq[LQ];
   __shared__ Real s_rel[12][64];
   __shared__ float s_tau[64], s_pad[64];
   __shared__ int s_cfgk[64], s_rank[64], s_item[64], s_pend[64], s_unc[64];
   // configuration entries: kill state, open-item count (+1 while items of
   // the entry remain to be started), configuration index (-1 = free)
-  __shared__ unsigned long long s_best[KG_NE][2];
-  __shared__ int s_uncmin[KG_NE][2];
-  __shared__ int e_cfg[KG_NE], e_open[KG_NE];
+  __shared__ unsigned long long s_best[NE][2];
+  __shared__ int s_uncmin[NE][2];
+  __shared__ int e_cfg[NE], e_open[NE], e_beg[NE], e_end[NE];
   __shared__ int s_chunk;
 
   const int lane = threadIdx.x;
@@ -638,7 +681,7 @@ __global__ __launch_bounds__(64) void k_collide(TravArgs a) {
   const int ncfg = a.ncfg_dev ? *a.ncfg_dev : a.ncfg;
   unsigned n_box = 0, n_tri = 0, n_items = 0;
 
-  for (int i = lane; i < KG_NE; i += 64) {
+  for (int i = lane; i < NE; i += 64) {
     e_cfg[i] = -1;
     e_open[i] = 0;
   }
@@ -765,8 +808,8 @@ __global__ __launch_bounds__(64) void k_collide(TravArgs a) {
     const int navail = min(__popcll(wm), __popcll(fm));
     if (navail > 0 && fe == tail && !exhausted) {
       // fetch configurations into the ring (wave-uniform)
-      while (head < tail && e_cfg[head % KG_NE] < 0) head++;
-      const int room = KG_NE - (tail - head);
+      while (head < tail && e_cfg[head % NE] < 0) head++;
+      const int room = NE - (tail - head);
       if (room > 0) {
         int ch = max(1, min(a.cpw, (ncfg - last_k0) / (int)gridDim.x));
         ch = min(ch, room);
@@ -780,8 +823,10 @@ __global__ __launch_bounds__(64) void k_collide(TravArgs a) {
           last_k0 = k0;
           const int nk = min(ch, ncfg - k0);
           if (lane < nk) {
-            const int e = (tail + lane) % KG_NE;
-            e_cfg[e] = cfg_of(k0 + lane);
+            const int e = (tail + lane) % NE;
+            const int cc = cfg_of(k0 + lane);
+            e_cfg[e] = cc;
+            range(cc, e_beg[e], e_end[e]);
             e_open[e] = 1;
             s_best[e][0] = s_best[e][1] = KG_NONE64;
             s_uncmin[e][0] = s_uncmin[e][1] = KG_RANK_NONE;
@@ -800,7 +845,8 @@ __global__ __launch_bounds__(64) void k_collide(TravArgs a) {
       int assigned = 0;
       while (fe < tail) {
         if (!fe_open) {
-          range(e_cfg[fe % KG_NE], it, itend);
+          it = e_beg[fe % NE];
+          itend = e_end[fe % NE];
           fe_open = true;
         }
         if (it >= itend) {
@@ -812,7 +858,7 @@ __global__ __launch_bounds__(64) void k_collide(TravArgs a) {
         const int take = min(navail - assigned, itend - it);
         if (want && wp >= assigned && wp < assigned + take) {
           myitem = it + (wp - assigned);
-          myent = fe % KG_NE;
+          myent = fe % NE;
         }
         assigned += take;
         it += take;
@@ -826,52 +872,38 @@ __global__ __launch_bounds__(64) void k_collide(TravArgs a) {
       n_items++;
       atomicAdd(&e_open[myent], 1);
       const PairItem P = a.items[myitem];
-      const PlanDev& PL = a.cfg_plan ? a.plans[a.cfg_plan[P.cfg]] : a.plan;
-      const size_t tb = a.toff ? (size_t)a.toff[P.cfg] : (size_t)P.cfg * PL.nlinks;
-      const LinkDesc& LA = PL.links[P.a];
-      const int g = (P.rank >> KG_RANK_GROUP_BIT) & 1;
-      const double* XA = a.T64 + (tb + P.a) * 12;
-      const double* XB;
-      int geomB;
-      if (g == 0) {
-        XB = PL.env[P.b].X;
-        geomB = PL.env[P.b].geom;
+      int rootA, rootB;
+      if (!EXACT) {
+        const float4* q = reinterpret_cast<const float4*>(a.recs + myitem);
+        const float4 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
+        s_rel[0][myslot] = q0.x; s_rel[1][myslot] = q0.y; s_rel[2][myslot] = q0.z; s_rel[3][myslot] = q0.w;
+        s_rel[4][myslot] = q1.x; s_rel[5][myslot] = q1.y; s_rel[6][myslot] = q1.z; s_rel[7][myslot] = q1.w;
+        s_rel[8][myslot] = q2.x; s_rel[9][myslot] = q2.y; s_rel[10][myslot] = q2.z; s_rel[11][myslot] = q2.w;
+        s_tau[myslot] = q3.x;
+        s_pad[myslot] = q3.y;
+        rootA = __float_as_int(q3.z);
+        rootB = __float_as_int(q3.w);
       } else {
-        XB = a.T64 + (tb + P.b) * 12;
-        geomB = PL.links[P.b].geom;
+        Real rel[12];
+        float tau, pad;
+        item_frame<Real, EXACT>(a, P, rel, tau, pad, rootA, rootB);
+#pragma unroll
+        for (int k = 0; k < 12; k++) s_rel[k][myslot] = rel[k];
+        s_tau[myslot] = tau;
+        s_pad[myslot] = pad;
       }
-      const double d0 = XA[9] - XB[9], d1 = XA[10] - XB[10], d2 = XA[11] - XB[11];
-#pragma unroll
-      for (int col = 0; col < 3; col++)
-#pragma unroll
-        for (int r = 0; r < 3; r++)
-          s_rel[col * 3 + r][myslot] =
-              (Real)(XB[r * 3] * XA[col * 3] + XB[r * 3 + 1] * XA[col * 3 + 1] + XB[r * 3 + 2] * XA[col * 3 + 2]);
-      double tn = 0;
-#pragma unroll
-      for (int r = 0; r < 3; r++) {
-        const double v = XB[r * 3] * d0 + XB[r * 3 + 1] * d1 + XB[r * 3 + 2] * d2;
-        s_rel[9 + r][myslot] = (Real)v;
-        tn += v * v;
-      }
-      const GeomDesc& GA = a.pool.geoms[LA.geom];
-      const GeomDesc& GB = a.pool.geoms[geomB];
-      float tau, pad;
-      slot_bounds<Real, EXACT>(a, sqrt(tn), GA.rmax, GB.rmax, tau, pad);
-      s_tau[myslot] = tau;
-      s_pad[myslot] = pad;
       s_cfgk[myslot] = myent;
       s_rank[myslot] = P.rank;
       s_item[myslot] = myitem;
       s_pend[myslot] = 1;
       s_unc[myslot] = 0;
-      cont = mk_task(myslot, 0, 1, GA.root, GB.root);
+      cont = mk_task(myslot, 0, 1, rootA, rootB);
     }
     __syncthreads();
     // release the start hold of the entries passed above (after the items
     // started from them are counted)
     if (lane == 0)
-      for (int e = rel_from; e < fe; e++) settle_entry(e % KG_NE, -1);
+      for (int e = rel_from; e < fe; e++) settle_entry(e % NE, -1);
     __syncthreads();
 
     // ---- termination: no path, nothing pooled, no leaves, nothing to start
@@ -954,9 +986,9 @@ __global__ __launch_bounds__(64) void k_collide(TravArgs a) {
     {
       const uint64_t l1 = ballot64(nlf >= 1), l2 = ballot64(nlf >= 2);
       const int ltot = __popcll(l1) + __popcll(l2);
-      if (nl + ltot > KG_LQ) {
+      if (nl + ltot > LQ) {
         __syncthreads();
-        while (nl + ltot > KG_LQ) run_leaves();
+        while (nl + ltot > LQ) run_leaves();
       }
       const int pos = nl + lane_prefix(l1) + lane_prefix(l2);
       if (nlf >= 1) lq[pos] = lf0;
@@ -979,6 +1011,23 @@ __global__ __launch_bounds__(64) void k_collide(TravArgs a) {
     atomicAdd(&a.counters[1], (unsigned long long)n_tri);
     atomicAdd(&a.counters[2], (unsigned long long)n_items);
   }
+}
+
+// Item start records: lane per (configuration, item position).
+__global__ __launch_bounds__(256) void k_items(TravArgs a, int N, int maxcap, ItemRec* __restrict__ recs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = i / maxcap, k = i - c * maxcap;
+  if (c >= N || k >= a.counts[c]) return;
+  const int b = a.offsets ? a.offsets[c] : (a.ibase ? a.ibase[c] : c * a.cap);
+  const PairItem P = a.items[b + k];
+  float rel[12], tau, pad;
+  int rootA, rootB;
+  item_frame<float, false>(a, P, rel, tau, pad, rootA, rootB);
+  float4* q = reinterpret_cast<float4*>(recs + b + k);
+  q[0] = make_float4(rel[0], rel[1], rel[2], rel[3]);
+  q[1] = make_float4(rel[4], rel[5], rel[6], rel[7]);
+  q[2] = make_float4(rel[8], rel[9], rel[10], rel[11]);
+  q[3] = make_float4(tau, pad, __int_as_float(rootA), __int_as_float(rootB));
 }
 
 __global__ void k_fill_int(int* p, int n, int v) {
@@ -1067,7 +1116,9 @@ hipError_t kgl_prep_single(const PrepArgs& a, hipStream_t s) {
 
 template <typename Real, bool EXACT>
 static hipError_t launch_collide(const TravArgs& a, hipStream_t s) {
-  if (a.scap == 512)
+  if (a.scap == 256)
+    k_collide<Real, EXACT, 256><<<a.grid, 64, 0, s>>>(a);
+  else if (a.scap == 512)
     k_collide<Real, EXACT, 512><<<a.grid, 64, 0, s>>>(a);
   else
     k_collide<Real, EXACT, 1024><<<a.grid, 64, 0, s>>>(a);
@@ -1076,6 +1127,14 @@ static hipError_t launch_collide(const TravArgs& a, hipStream_t s) {
 
 hipError_t kgl_collide_f32(const TravArgs& a, hipStream_t s) { return launch_collide<float, false>(a, s); }
 hipError_t kgl_collide_f64(const TravArgs& a, hipStream_t s) { return launch_collide<double, true>(a, s); }
+
+hipError_t kgl_items(const TravArgs& a, int N, int maxcap, ItemRec* recs, hipStream_t s) {
+  const long long n = (long long)N * maxcap;
+  if (n <= 0) return hipSuccess;
+  if (n >= (1ll << 31)) return hipErrorInvalidValue;
+  k_items<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(a, N, maxcap, recs);
+  return hipGetLastError();
+}
 
 hipError_t kgl_finalize(const FinArgs& a, hipStream_t s) {
   if (a.N <= 0) return hipSuccess;
@@ -1098,7 +1157,10 @@ hipError_t kgl_fill_int(int* p, int n, int v, hipStream_t s) {
 int kgl_collide_grid(int exact, int num_cus, int scap) {
   int nb = 0;
   hipError_t e;
-  if (scap == 512)
+  if (scap == 256)
+    e = exact ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_collide<double, true, 256>, 64, 0)
+              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_collide<float, false, 256>, 64, 0);
+  else if (scap == 512)
     e = exact ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_collide<double, true, 512>, 64, 0)
               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_collide<float, false, 512>, 64, 0);
   else

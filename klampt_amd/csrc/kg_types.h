@@ -63,5 +63,14 @@ struct PairItem {
   int rank;   // order inside the group (Klampt loop order); group in bit 30
 };
 
+// What the FP32 traversal needs to start a pair item, computed once per item
+// by k_items: the item transform (mesh A frame -> mesh B frame, FP64
+// product rounded to FP32), its error bounds and the two BVH roots.
+struct __attribute__((aligned(16))) ItemRec {
+  float rel[12];  // column-major R, then t
+  float tau, pad;
+  int rootA, rootB;
+};
+
 #define KG_RANK_GROUP_BIT 30
 #define KG_RANK_NONE 0x7fffffff
