@@ -479,6 +479,7 @@ void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* 
     pa.items = c->wsItems.as<PairItem>();
     pa.cap = L.cap;
     pa.offsets = nullptr;
+    pa.lds_links = L.multi ? 0 : 1;
     hip_check(kgl_prep_single(pa, s), "prep");
   } else {
     // large environments: count, scan, then fill exactly

@@ -122,6 +122,7 @@ struct PrepArgs {
   const int* qoff;
   const int* toff;
   const int* ibase;
+  int lds_links;       // single-plan batch: link boxes may live in LDS (no LB writes)
 };
 hipError_t kgl_prep_count(const PrepArgs& a, hipStream_t s);
 hipError_t kgl_scan(int* counts, int* offsets, int n, DevBuf& tmp, hipStream_t s);

@@ -22,6 +22,8 @@
 
 #define KG_CPW_MAX 64
 #define KG_NE 64  // configuration entries per wave (ring)
+#define KG_PREP_LDS_LINKS 8    // link boxes in LDS for robots of at most this many links
+#define KG_PREP_LDS_BLOCK 128  // block size of that variant (48 KB of LDS)
 #define KG_NONE64 0xffffffffffffffffull
 
 namespace {
@@ -166,9 +168,13 @@ __device__ int swap_erase_pos(int j, int E, int m, F rej) {
   return -1;
 }
 
-template <int MODE>
+// LDSN > 0: single-plan batches of at most LDSN links keep the link boxes in
+// LDS (structure of arrays over the block's lanes) instead of HBM.
+template <int MODE, int LDSN>
 __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
   constexpr bool FILL = MODE != 0;
+  constexpr int BS = LDSN > 0 ? KG_PREP_LDS_BLOCK : 256;
+  __shared__ double sLB[LDSN > 0 ? LDSN * 6 * BS : 1];
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (MODE != 1 && c < a.nreset) a.reset[c] = 0;
   if (c >= a.N) return;
@@ -178,6 +184,16 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
   const size_t qb = a.qoff ? (size_t)a.qoff[c] : (size_t)c * n;
   double* T = a.T64 + tb * 12;
   double* LB = a.LB + tb * 6;
+  // link box component r (0..2 lo, 3..5 hi) of link l
+  auto lb = [&](int l, int r) -> double& {
+    if (LDSN > 0) return sLB[(l * 6 + r) * BS + threadIdx.x];
+    return LB[6 * l + r];
+  };
+  // box of link l against [blo, bhi] (closed intervals)
+  auto box_hit_l = [&](int l, const double* blo, const double* bhi) {
+    return !(lb(l, 0) > bhi[0] || lb(l, 3) < blo[0] || lb(l, 1) > bhi[1] || lb(l, 4) < blo[1] ||
+             lb(l, 2) > bhi[2] || lb(l, 5) < blo[2]);
+  };
   // union of the active links' boxes
   double ulo[3] = {1e300, 1e300, 1e300}, uhi[3] = {-1e300, -1e300, -1e300};
   uint64_t active = 0;
@@ -208,8 +224,8 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
         for (int r = 0; r < 3; r++) {
           double wc = X.R[r] * g.lc[0] + X.R[3 + r] * g.lc[1] + X.R[6 + r] * g.lc[2] + X.t[r];
           double ext = fabs(X.R[r]) * g.le[0] + fabs(X.R[3 + r]) * g.le[1] + fabs(X.R[6 + r]) * g.le[2];
-          LB[6 * l + r] = wc - ext;
-          LB[6 * l + 3 + r] = wc + ext;
+          lb(l, r) = wc - ext;
+          lb(l, 3 + r) = wc + ext;
           ulo[r] = fmin(ulo[r], wc - ext);
           uhi[r] = fmax(uhi[r], wc + ext);
         }
@@ -229,8 +245,8 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
       if (P.links[l].geom >= 0) {
         active |= 1ull << l;
         for (int r = 0; r < 3; r++) {
-          ulo[r] = fmin(ulo[r], LB[6 * l + r]);
-          uhi[r] = fmax(uhi[r], LB[6 * l + 3 + r]);
+          ulo[r] = fmin(ulo[r], lb(l, r));
+          uhi[r] = fmax(uhi[r], lb(l, 3 + r));
         }
       }
   // env entries surviving against the link union; union of survivors
@@ -253,7 +269,7 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
     int pos = 0;
     for (int l = 0; l < n; l++)
       if (active >> l & 1) {
-        if (!box_hit(LB + 6 * l, LB + 6 * l + 3, elo, ehi)) lrej |= 1ull << pos;
+        if (!box_hit_l(l, elo, ehi)) lrej |= 1ull << pos;
         pos++;
       }
   }
@@ -274,7 +290,7 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
         const EnvItem& E = P.env[e];
         if (!P.lemask[l * P.nenv + e]) continue;
         if (!box_hit(E.lo, E.hi, ulo, uhi)) continue;  // rejected env entry
-        if (!box_hit(LB + 6 * l, LB + 6 * l + 3, E.lo, E.hi)) continue;
+        if (!box_hit_l(l, E.lo, E.hi)) continue;
         if (FILL) {
           int p2 = ordered ? swap_erase_pos(e, P.nenv, m2, env_rej) : 0;
           PairItem it;
@@ -291,7 +307,9 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
   // self pairs
   for (int k = 0; k < P.nself; k++) {
     int i = P.selfpairs[2 * k], j = P.selfpairs[2 * k + 1];
-    if (!box_hit(LB + 6 * i, LB + 6 * i + 3, LB + 6 * j, LB + 6 * j + 3)) continue;
+    if (lb(i, 0) > lb(j, 3) || lb(i, 3) < lb(j, 0) || lb(i, 1) > lb(j, 4) || lb(i, 4) < lb(j, 1) ||
+        lb(i, 2) > lb(j, 5) || lb(i, 5) < lb(j, 2))
+      continue;
     if (FILL) {
       PairItem it;
       it.cfg = c;
@@ -1087,7 +1105,7 @@ hipError_t kgl_fk_f32(const LinkDesc* links, int nlinks, const float* Q, int N, 
 
 hipError_t kgl_prep_count(const PrepArgs& a, hipStream_t s) {
   if (a.N <= 0) return hipSuccess;
-  k_prep<0><<<(a.N + 255) / 256, 256, 0, s>>>(a);
+  k_prep<0, 0><<<(a.N + 255) / 256, 256, 0, s>>>(a);
   k_fill_int<<<1, 1, 0, s>>>(a.counts + a.N, 1, 0);
   return hipGetLastError();
 }
@@ -1103,14 +1121,17 @@ hipError_t kgl_scan(int* counts, int* offsets, int n, DevBuf& tmp, hipStream_t s
 
 hipError_t kgl_prep_fill(const PrepArgs& a, hipStream_t s) {
   if (a.N <= 0) return hipSuccess;
-  k_prep<1><<<(a.N + 255) / 256, 256, 0, s>>>(a);
+  k_prep<1, 0><<<(a.N + 255) / 256, 256, 0, s>>>(a);
   return hipGetLastError();
 }
 
 hipError_t kgl_prep_single(const PrepArgs& a, hipStream_t s) {
   const int n = a.N > a.nreset ? a.N : a.nreset;
   if (n <= 0) return hipSuccess;
-  k_prep<2><<<(n + 255) / 256, 256, 0, s>>>(a);
+  if (a.lds_links && a.plan.nlinks <= KG_PREP_LDS_LINKS)
+    k_prep<2, KG_PREP_LDS_LINKS><<<(n + KG_PREP_LDS_BLOCK - 1) / KG_PREP_LDS_BLOCK, KG_PREP_LDS_BLOCK, 0, s>>>(a);
+  else
+    k_prep<2, 0><<<(n + 255) / 256, 256, 0, s>>>(a);
   return hipGetLastError();
 }
 
