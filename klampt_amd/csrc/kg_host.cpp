@@ -402,7 +402,11 @@ struct BatchStats {
 //   [0] spill overflow  [1] FP64 re-check count  [2] FP32 chunk counter
 //   [3] FP64 chunk counter  [4..9] u64 counters: box tests, triangle tests,
 //   pair items.
-enum { ST_SPILL = 0, ST_REDO = 1, ST_CHUNK32 = 2, ST_CHUNK64 = 3, ST_CNT = 4, ST_WORDS = 16 };
+// status words: flags and counters, then the traversal's chunk counters, one
+// per 128-byte line so that waves fetching from different ranges do not
+// queue on one address
+enum { ST_SPILL = 0, ST_REDO = 1, ST_CHUNK32 = 2, ST_CHUNK64 = 3, ST_CNT = 4, ST_DYN = 32, ST_NSUB = 32,
+       ST_WORDS = ST_DYN + 32 * ST_NSUB };
 void ensure_status_host(kg_ctx_s* c) {
   if (!c->hstatus) {
     hip_check(hipHostMalloc((void**)&c->hstatus, ST_WORDS * sizeof(int), hipHostMallocDefault), "hipHostMalloc");
@@ -525,7 +529,11 @@ void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* 
   ta.tol = L.tol32;
   ta.best = c->wsBest.as<unsigned long long>();
   ta.unc = c->wsUnc.as<int>();
-  ta.chunk_counter = status + ST_CHUNK32;
+  ta.chunk_counter = status + ST_DYN;
+  {
+    const int nsub = std::max(1, std::min((int)ST_NSUB, env_int("KG_NSUB", 8)));
+    ta.nsub = N >= 64 * nsub ? nsub : 1;
+  }
   ta.spill = c->wsSpill.as<uint64_t>();
   ta.spill_cap = spill_cap;
   ta.stack_cap = 0;
@@ -539,6 +547,7 @@ void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* 
   ta.toff = L.toff;
   ta.ibase = L.ibase;
   ta.scap = scap;
+  ta.static_pct = std::max(0, std::min(100, env_int("KG_STATIC_PCT", 0)));
   // item start records (after the items exist, before the traversal)
   const size_t nrec = pa.offsets ? (size_t)std::max(1, item_total) : std::max<size_t>(1, L.ncap);
   hip_check(c->wsRec.ensure(nrec * sizeof(ItemRec)), "hipMalloc(records)");
@@ -581,6 +590,8 @@ void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* 
   tb.grid = grid64;
   tb.counters = nullptr;
   tb.chunk_counter = status + ST_CHUNK64;
+  tb.static_pct = 0;
+  tb.nsub = 1;
   hip_check(hipEventRecord(c->ev[4], s), "hipEventRecord");
   hip_check(kgl_collide_f64(tb, s), "collide_f64");
   hip_check(hipEventRecord(c->ev[5], s), "hipEventRecord");

@@ -159,6 +159,8 @@ struct TravArgs {
   const int* ibase;
   int scap;                   // LDS pool capacity variant (512 or 1024)
   const ItemRec* recs;        // per item start records (FP32 pass)
+  int static_pct;             // percent of the configurations split statically over the waves
+  int nsub;                   // dynamic ranges, each with a counter at chunk_counter + 32 r
 };
 hipError_t kgl_collide_f32(const TravArgs& a, hipStream_t s);
 // Item start records for every item of every configuration (lane per

@@ -22,6 +22,7 @@
 
 #define KG_CPW_MAX 64
 #define KG_NE 64  // configuration entries per wave (ring)
+#define KG_COUNTER_STRIDE 32   // ints between the traversal's chunk counters (128 B)
 #define KG_PREP_LDS_LINKS 8    // link boxes in LDS for robots of at most this many links
 #define KG_PREP_LDS_BLOCK 128  // block size of that variant (48 KB of LDS)
 #define KG_NONE64 0xffffffffffffffffull
@@ -719,7 +720,14 @@ __global__ __launch_bounds__(64) void k_collide(TravArgs a) {
   };
   // ring positions (monotonic, wave-uniform): [head, tail) fetched entries,
   // fe = entry whose items are being started, [it, itend) its next items
-  int head = 0, tail = 0, fe = 0, it = 0, itend = 0, last_k0 = 0;
+  // static share: S0 configurations per wave, the rest dynamic from dyn0
+  const int S0 = (int)(((long long)ncfg * a.static_pct / 100) / (int)gridDim.x);
+  const int dyn0 = S0 * (int)gridDim.x;
+  const int nsub = max(1, a.nsub);
+  const int rlen = (ncfg - dyn0 + nsub - 1) / nsub;  // configurations per dynamic range
+  const int wpr = max(1, (int)gridDim.x / nsub);     // waves per range
+  int sub = (int)blockIdx.x % nsub, tried = 0;
+  int head = 0, tail = 0, fe = 0, it = 0, itend = 0, last_k0 = dyn0 + sub * rlen, soff = 0;
   bool fe_open = false, exhausted = false;
   int sp = 0, gsp = 0, nl = 0;
   uint64_t cont = KG_NOTASK;  // this lane's depth-first continuation
@@ -825,21 +833,42 @@ __global__ __launch_bounds__(64) void k_collide(TravArgs a) {
     const uint64_t fm = ballot64(s_pend[lane] == 0);
     const int navail = min(__popcll(wm), __popcll(fm));
     if (navail > 0 && fe == tail && !exhausted) {
-      // fetch configurations into the ring (wave-uniform)
+      // fetch configurations into the ring (wave-uniform): first this wave's
+      // static share, then chunks from the global counter
       while (head < tail && e_cfg[head % NE] < 0) head++;
       const int room = NE - (tail - head);
       if (room > 0) {
-        int ch = max(1, min(a.cpw, (ncfg - last_k0) / (int)gridDim.x));
-        ch = min(ch, room);
-        if (lane == 0) s_chunk = atomicAdd(a.chunk_counter, ch);
-        __syncthreads();
-        const int k0 = s_chunk;
-        __syncthreads();
-        if (k0 >= ncfg) {
-          exhausted = true;
+        int k0 = 0, nk = 0;
+        if (soff < S0) {
+          k0 = (int)blockIdx.x * S0 + soff;
+          nk = min(room, S0 - soff);
+          soff += nk;
         } else {
-          last_k0 = k0;
-          const int nk = min(ch, ncfg - k0);
+          // the dynamic part is split into nsub ranges with a counter each
+          // (own cache line); a wave starts on range blockIdx % nsub and
+          // moves on when it runs dry
+          for (;;) {
+            if (tried >= nsub) {
+              exhausted = true;
+              break;
+            }
+            const int r0 = dyn0 + sub * rlen, r1 = min(ncfg, r0 + rlen);
+            const int ch = min(room, max(1, min(a.cpw, (r1 - last_k0) / wpr)));
+            if (lane == 0) s_chunk = r0 + atomicAdd(a.chunk_counter + sub * KG_COUNTER_STRIDE, ch);
+            __syncthreads();
+            k0 = s_chunk;
+            __syncthreads();
+            if (k0 < r1) {
+              last_k0 = k0;
+              nk = min(ch, r1 - k0);
+              break;
+            }
+            sub = sub + 1 == nsub ? 0 : sub + 1;
+            tried++;
+            last_k0 = dyn0 + sub * rlen;
+          }
+        }
+        if (nk > 0) {
           if (lane < nk) {
             const int e = (tail + lane) % NE;
             const int cc = cfg_of(k0 + lane);
