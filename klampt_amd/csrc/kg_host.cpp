@@ -17,7 +17,6 @@
 #include "kg_internal.h"
 #include "kg_geom_math.h"
 
-hipError_t kgl_finalize_list(const FinArgs& a, const int* list, const int* count, int maxn, hipStream_t s);
 hipError_t kgl_fill_int(int* p, int n, int v, hipStream_t s);
 int kgl_collide_grid(int exact, int num_cus);
 
@@ -419,6 +418,20 @@ void ensure_status_host(kg_ctx_s* c) {
   }
 }
 
+// Status words of the last batch, copied to the host on request (waits for
+// that batch; a batch issued later re-records evStat and stat_src).
+const int* fetch_status(kg_ctx_s* c) {
+  ensure_status_host(c);
+  if (!c->stat_src) {
+    std::memset(c->hstatus, 0, ST_WORDS * sizeof(int));
+    return c->hstatus;
+  }
+  hip_check(hipSetDevice(c->device), "hipSetDevice");
+  hip_check(hipEventSynchronize(c->evStat), "sync");
+  hip_check(hipMemcpy(c->hstatus, c->stat_src, ST_WORDS * sizeof(int), hipMemcpyDeviceToHost), "copy status");
+  return c->hstatus;
+}
+
 // Layout of one batched feasibility call: a single robot plan, or one plan
 // per configuration (heterogeneous batches, device tables in `multi`).
 struct Layout {
@@ -464,6 +477,7 @@ void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* 
   if (L.P) pa.plan = L.P->dev;
   pa.pool = c->pool;
   pa.T64 = c->wsT64.as<double>();
+  pa.t_rows = L.nrows;
   pa.LB = c->wsLB.as<double>();
   pa.jl = c->wsJL.as<uint8_t>();
   pa.counts = c->wsCounts.as<int>();
@@ -475,12 +489,13 @@ void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* 
   pa.toff = L.toff;
   pa.ibase = L.ibase;
   const bool fixed = L.ncap * sizeof(PairItem) <= (2048ull << 20);
-  int item_total = 0;  // items of the count/scan/fill layout
   hip_check(hipEventRecord(c->ev[0], s), "hipEventRecord");
   if (fixed || L.multi) {
     need(fixed, "heterogeneous batch exceeds the item capacity limit (2 GiB of pair items)");
     hip_check(c->wsItems.ensure(std::max<size_t>(1, L.ncap) * sizeof(PairItem)), "hipMalloc(items)");
+    hip_check(c->wsRec.ensure(std::max<size_t>(1, L.ncap) * sizeof(ItemRec)), "hipMalloc(records)");
     pa.items = c->wsItems.as<PairItem>();
+    pa.recs = c->wsRec.as<ItemRec>();
     pa.cap = L.cap;
     pa.offsets = nullptr;
     pa.lds_links = L.multi ? 0 : 1;
@@ -495,9 +510,10 @@ void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* 
     int t = 0;
     hip_check(hipMemcpyAsync(&t, pa.offsets + N, sizeof(int), hipMemcpyDeviceToHost, s), "copy total");
     hip_check(hipStreamSynchronize(s), "sync");
-    item_total = t;
     hip_check(c->wsItems.ensure(std::max<size_t>(1, (size_t)t) * sizeof(PairItem)), "hipMalloc(items)");
+    hip_check(c->wsRec.ensure(std::max<size_t>(1, (size_t)t) * sizeof(ItemRec)), "hipMalloc(records)");
     pa.items = c->wsItems.as<PairItem>();
+    pa.recs = c->wsRec.as<ItemRec>();
     hip_check(kgl_prep_fill(pa, s), "prep_fill");
   }
   hip_check(hipEventRecord(c->ev[1], s), "hipEventRecord");
@@ -510,14 +526,14 @@ void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* 
   const int grid32 = wpc > 0 ? wpc * c->num_cus : kgl_collide_grid(0, c->num_cus, scap);
   const int grid64 = std::min(kgl_collide_grid(1, c->num_cus, scap), std::max(1, c->num_cus));
   const int spill_cap = env_int("KG_SPILL_CAP", 16384);
-  hip_check(c->wsBest.ensure((size_t)N * 2 * sizeof(unsigned long long)), "hipMalloc(best)");
-  hip_check(c->wsUnc.ensure((size_t)N * 2 * sizeof(int)), "hipMalloc(unc)");
   hip_check(c->wsSpill.ensure((size_t)std::max(grid32, grid64) * spill_cap * sizeof(uint64_t)), "hipMalloc(spill)");
 
   TravArgs ta{};
   ta.pool = c->pool;
   if (L.P) ta.plan = L.P->dev;
   ta.T64 = pa.T64;
+  ta.t_rows = L.nrows;
+  ta.t_n = N;
   ta.items = pa.items;
   ta.offsets = pa.offsets;
   ta.counts = pa.counts;
@@ -527,8 +543,6 @@ void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* 
   ta.ncfg_dev = nullptr;
   ta.mode = mode;
   ta.tol = L.tol32;
-  ta.best = c->wsBest.as<unsigned long long>();
-  ta.unc = c->wsUnc.as<int>();
   ta.chunk_counter = status + ST_DYN;
   {
     const int nsub = std::max(1, std::min((int)ST_NSUB, env_int("KG_NSUB", 8)));
@@ -548,25 +562,13 @@ void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* 
   ta.ibase = L.ibase;
   ta.scap = scap;
   ta.static_pct = std::max(0, std::min(100, env_int("KG_STATIC_PCT", 0)));
-  // item start records (after the items exist, before the traversal)
-  const size_t nrec = pa.offsets ? (size_t)std::max(1, item_total) : std::max<size_t>(1, L.ncap);
-  hip_check(c->wsRec.ensure(nrec * sizeof(ItemRec)), "hipMalloc(records)");
-  ta.recs = c->wsRec.as<ItemRec>();
-  hip_check(kgl_items(ta, N, std::max(1, L.maxcap), c->wsRec.as<ItemRec>(), s), "items");
-  const int slot = (int)(c->tr_count % kg_ctx_s::TRING);
-  hip_check(hipEventRecord(c->ev[2], s), "hipEventRecord");
-  hip_check(hipEventRecord(c->tr_beg[slot], s), "hipEventRecord");
-  hip_check(kgl_collide_f32(ta, s), "collide_f32");
-  hip_check(hipEventRecord(c->tr_end[slot], s), "hipEventRecord");
-  hip_check(hipEventRecord(c->ev[3], s), "hipEventRecord");
-  c->tr_count++;
-
-  FinArgs fa{};
+  ta.recs = pa.recs;
+  // results are assembled inside the traversal as each configuration's last
+  // item finishes; uncertain ones go to the re-check list
+  FinArgs& fa = ta.fin;
   fa.N = N;
   fa.mode = mode;
   fa.jl = pa.jl;
-  fa.best = ta.best;
-  fa.unc = ta.unc;
   fa.items = pa.items;
   if (L.P) fa.plan = L.P->dev;
   fa.plans = L.plans;
@@ -578,7 +580,13 @@ void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* 
   fa.redo_list = redo_list;
   fa.redo_count = status + ST_REDO;
   fa.final_pass = 0;
-  hip_check(kgl_finalize(fa, s), "finalize");
+  const int slot = (int)(c->tr_count % kg_ctx_s::TRING);
+  hip_check(hipEventRecord(c->ev[2], s), "hipEventRecord");
+  hip_check(hipEventRecord(c->tr_beg[slot], s), "hipEventRecord");
+  hip_check(kgl_collide_f32(ta, s), "collide_f32");
+  hip_check(hipEventRecord(c->tr_end[slot], s), "hipEventRecord");
+  hip_check(hipEventRecord(c->ev[3], s), "hipEventRecord");
+  c->tr_count++;
 
   // FP64 re-check of the configurations the FP32 pass could not certify;
   // the count stays on the device (waves exit at once when it is zero)
@@ -592,13 +600,12 @@ void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* 
   tb.chunk_counter = status + ST_CHUNK64;
   tb.static_pct = 0;
   tb.nsub = 1;
+  tb.fin.final_pass = 1;
   hip_check(hipEventRecord(c->ev[4], s), "hipEventRecord");
   hip_check(kgl_collide_f64(tb, s), "collide_f64");
   hip_check(hipEventRecord(c->ev[5], s), "hipEventRecord");
-  FinArgs fb = fa;
-  fb.final_pass = 1;
-  hip_check(kgl_finalize_list(fb, redo_list, status + ST_REDO, N, s), "finalize_f64");
-  hip_check(hipMemcpyAsync(c->hstatus, status, ST_WORDS * sizeof(int), hipMemcpyDeviceToHost, s), "copy status");
+  // the status words stay on the device until someone asks (fetch_status)
+  c->stat_src = status;
   hip_check(hipEventRecord(c->evStat, s), "hipEventRecord");
 }
 
@@ -629,8 +636,7 @@ void read_stats(kg_world_s* w, BatchStats& st) {
   kg_ctx_s* c = w->ctx;
   st = BatchStats{};
   if (!c->hstatus || !c->has_batch || c->last_N <= 0) return;
-  hip_check(hipEventSynchronize(c->evStat), "sync");
-  const int* h = c->hstatus;
+  const int* h = fetch_status(c);
   st.spills = h[ST_SPILL];
   st.redo = h[ST_REDO];
   unsigned long long cnt[3];
@@ -1601,7 +1607,7 @@ void distance_dev(kg_world_s* w, int ri, const float* Qf, const double* Qd, int 
   hip_check(hipEventRecord(c->ev[3], s), "hipEventRecord");
   c->last_N = N;
   c->has_batch = true;
-  hip_check(hipMemcpyAsync(c->hstatus, status, ST_WORDS * sizeof(int), hipMemcpyDeviceToHost, s), "copy status");
+  c->stat_src = status;
   hip_check(hipEventRecord(c->evStat, s), "hipEventRecord");
 }
 
@@ -1647,7 +1653,7 @@ void pair_distance(kg_geom_s* a, kg_geom_s* b, float stop_below, double upper, k
   hip_check(hipMemcpyAsync(&d, dout, sizeof(float), hipMemcpyDeviceToHost, c->stream), "copy");
   hip_check(hipMemcpyAsync(tri, dtri, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream), "copy");
   hip_check(hipStreamSynchronize(c->stream), "sync");
-  if (c->hstatus[ST_SPILL]) fail(KG_E_LIMIT, "distance traversal exceeded its stack capacity (KG_SPILL_CAP)");
+  if (fetch_status(c)[ST_SPILL]) fail(KG_E_LIMIT, "distance traversal exceeded its stack capacity (KG_SPILL_CAP)");
   if (dgpu) *dgpu = d;
   std::memset(out, 0, sizeof(*out));
   out->d = d;
@@ -1710,7 +1716,7 @@ extern "C" int kg_distance_batch(kg_world w, int ri, const float* Q, int N, floa
     distance_dev(w, ri, c->wsQ.as<float>(), nullptr, N, c->wsF.as<float>(), nullptr, 0.0f, 3.0e38f, c->stream);
     hip_check(hipMemcpyAsync(out, c->wsF.p, (size_t)N * np * sizeof(float), hipMemcpyDeviceToHost, c->stream), "copy");
     hip_check(hipStreamSynchronize(c->stream), "sync");
-    if (c->hstatus[ST_SPILL]) fail(KG_E_LIMIT, "distance traversal exceeded its stack capacity (KG_SPILL_CAP)");
+    if (fetch_status(c)[ST_SPILL]) fail(KG_E_LIMIT, "distance traversal exceeded its stack capacity (KG_SPILL_CAP)");
   });
 }
 
@@ -1979,7 +1985,8 @@ extern "C" int kg_feasible_multi(int nb, const kg_world* worlds, const int32_t* 
                   c->stream);
     hip_check(hipMemcpyAsync(feas, c->wsOut8a.p, N, hipMemcpyDeviceToHost, c->stream), "copy out");
     hip_check(hipStreamSynchronize(c->stream), "sync");
-    if (c->hstatus[ST_SPILL] && c->hstatus[ST_REDO] > 0)
+    const int* st = fetch_status(c);
+    if (st[ST_SPILL] && st[ST_REDO] > 0)
       fail(KG_E_LIMIT, "BVH traversal stack exceeded its HBM spill capacity (KG_SPILL_CAP)");
   });
 }

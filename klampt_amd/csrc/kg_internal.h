@@ -123,11 +123,30 @@ struct PrepArgs {
   const int* toff;
   const int* ibase;
   int lds_links;       // single-plan batch: link boxes may live in LDS (no LB writes)
+  ItemRec* recs;       // nullable: item start records, written beside the items
+  size_t t_rows;       // frame rows of T64 (structure of arrays, see t_row)
 };
 hipError_t kgl_prep_count(const PrepArgs& a, hipStream_t s);
 hipError_t kgl_scan(int* counts, int* offsets, int n, DevBuf& tmp, hipStream_t s);
 hipError_t kgl_prep_fill(const PrepArgs& a, hipStream_t s);
 hipError_t kgl_prep_single(const PrepArgs& a, hipStream_t s);
+
+struct FinArgs {
+  int N;
+  int mode;
+  const uint8_t* jl;
+  const PairItem* items;
+  PlanDev plan;
+  uint8_t* feasible;
+  uint8_t* env_hit;
+  uint8_t* self_hit;
+  int* first_pair;
+  int* redo_list;   // configs needing the FP64 pass
+  int* redo_count;
+  int final_pass;   // 1: uncertainty cannot remain
+  const PlanDev* plans;  // heterogeneous batches (nullable)
+  const int* cfg_plan;
+};
 
 struct TravArgs {
   PoolDev pool;
@@ -142,8 +161,6 @@ struct TravArgs {
   const int* ncfg_dev;    // nullable: device-side configuration count
   int mode;
   Tol tol;
-  unsigned long long* best;   // N x 2: (rank<<32 | item) per group, ~0 = none
-  int* unc;                   // N x 2: min uncertain rank per group
   int* chunk_counter;
   uint64_t* spill;            // per-wave global overflow stacks
   int spill_cap;              // entries per wave
@@ -161,11 +178,12 @@ struct TravArgs {
   const ItemRec* recs;        // per item start records (FP32 pass)
   int static_pct;             // percent of the configurations split statically over the waves
   int nsub;                   // dynamic ranges, each with a counter at chunk_counter + 32 r
+  FinArgs fin;                // result assembly when a configuration's last item finishes
+  size_t t_rows;              // frame rows of T64 and the batch size (t_row layout)
+  int t_n;
 };
 hipError_t kgl_collide_f32(const TravArgs& a, hipStream_t s);
-// Item start records for every item of every configuration (lane per
-// configuration x item position below `maxcap`).
-hipError_t kgl_items(const TravArgs& a, int N, int maxcap, ItemRec* recs, hipStream_t s);
+
 hipError_t kgl_collide_f64(const TravArgs& a, hipStream_t s);
 
 struct DistArgs {
@@ -200,26 +218,6 @@ hipError_t kgl_edge_samples(const float* A, const float* B, int n, const int* se
 hipError_t kgl_edge_reduce(const int* seg, const int* soff, int nlive, int total, const uint8_t* feas, const int* nsamp,
                            const int* ooff, const int* order, int pos0, float* first_u, uint8_t* done, hipStream_t s);
 
-struct FinArgs {
-  int N;
-  int mode;
-  const uint8_t* jl;
-  const unsigned long long* best;
-  const int* unc;
-  const PairItem* items;
-  PlanDev plan;
-  uint8_t* feasible;
-  uint8_t* env_hit;
-  uint8_t* self_hit;
-  int* first_pair;
-  int* redo_list;   // configs needing the FP64 pass
-  int* redo_count;
-  int final_pass;   // 1: uncertainty cannot remain
-  const PlanDev* plans;  // heterogeneous batches (nullable)
-  const int* cfg_plan;
-};
-hipError_t kgl_finalize(const FinArgs& a, hipStream_t s);
-hipError_t kgl_finalize_list(const FinArgs& a, const int* list, const int* count, int maxn, hipStream_t s);
 hipError_t kgl_fill_int(int* p, int n, int v, hipStream_t s);
 int kgl_collide_grid(int exact, int num_cus, int scap);
 hipError_t kgl_sdf_eval(const SdfDev& g, const float* pts, int n, float* d, float* grad, hipStream_t s);
@@ -308,7 +306,8 @@ struct kg_ctx_s {
   // workspace
   int* hstatus = nullptr;  // pinned copy of the last batch's status words
   bool has_batch = false;     // a feasibility batch has been issued
-  hipEvent_t evStat = nullptr;  // recorded after the status words are copied to hstatus
+  hipEvent_t evStat = nullptr;  // recorded at the end of the last batch
+  const int* stat_src = nullptr;  // device status words of the last batch
   hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   int last_N = 0;
   // ordering of work across the streams callers name: evDone is recorded on

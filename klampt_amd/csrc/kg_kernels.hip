@@ -50,6 +50,27 @@ __device__ inline void x64_store(double* p, const X64& X) {
   for (int i = 0; i < 3; i++) p[9 + i] = X.t[i];
 }
 
+// Per-batch link frames are stored structure-of-arrays: component k (0..8 R
+// column-major, 9..11 t) of frame row r at T[k * rows + r].  Single-plan
+// batches use row l * N + c (consecutive configurations adjacent, so a
+// wave's stores and loads are coalesced); heterogeneous batches use
+// toff[c] + l.
+__device__ inline size_t t_row(const int* toff, int N, int c, int l) {
+  return toff ? (size_t)toff[c] + l : (size_t)l * N + c;
+}
+__device__ inline void t_load(const double* T, size_t rows, size_t row, X64& X) {
+#pragma unroll
+  for (int i = 0; i < 9; i++) X.R[i] = T[i * rows + row];
+#pragma unroll
+  for (int i = 0; i < 3; i++) X.t[i] = T[(9 + i) * rows + row];
+}
+__device__ inline void t_store(double* T, size_t rows, size_t row, const X64& X) {
+#pragma unroll
+  for (int i = 0; i < 9; i++) T[i * rows + row] = X.R[i];
+#pragma unroll
+  for (int i = 0; i < 3; i++) T[(9 + i) * rows + row] = X.t[i];
+}
+
 __device__ inline void x64_mul(const X64& A, const X64& B, X64& C) {
 #pragma unroll
   for (int c = 0; c < 3; c++)
@@ -142,6 +163,49 @@ __global__ __launch_bounds__(256) void k_fk_f32(const LinkDesc* __restrict__ lin
   }
 }
 
+// ------------------------------------------------------------ item records
+
+// Error bounds of the FP32 narrow phase for an item whose transform has
+// translation norm tnorm, between meshes of radii rA and rB.  A vertex p of
+// mesh A (|p| <= rA) mapped by the FP32-rounded transform (R, t) lands within
+// u(8 rA + 4|t|) of its exact image; a stored vertex of B (|v| <= rB) is
+// within u rB of its FP64 value.  tau doubles the sum for safety margin; box
+// tests multiply up to three rounded factors over coordinates bounded by
+// |t| + rA + rB (including the inverted item transform).
+__device__ inline void fp32_item_bounds(double tnorm, double rA, double rB, float& tau, float& pad) {
+  const double u = 5.9604644775390625e-08;
+  const double ev = u * (8.0 * rA + 4.0 * tnorm) + u * rB;
+  tau = (float)(2.0 * ev);
+  pad = (float)(2.0 * ev + 32.0 * u * (tnorm + rA + rB));
+}
+
+// Start record of an item: XA, XB are the FP64 world frames of meshes A and
+// B (column-major R, then t); rel = XB^-1 XA rounded to FP32.
+__device__ inline void write_item_rec(ItemRec* __restrict__ out, const double* XA, const double* XB, const GeomDesc& GA,
+                                      const GeomDesc& GB) {
+  float rel[12];
+  const double d0 = XA[9] - XB[9], d1 = XA[10] - XB[10], d2 = XA[11] - XB[11];
+#pragma unroll
+  for (int col = 0; col < 3; col++)
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+      rel[col * 3 + r] = (float)(XB[r * 3] * XA[col * 3] + XB[r * 3 + 1] * XA[col * 3 + 1] + XB[r * 3 + 2] * XA[col * 3 + 2]);
+  double tn = 0;
+#pragma unroll
+  for (int r = 0; r < 3; r++) {
+    const double v = XB[r * 3] * d0 + XB[r * 3 + 1] * d1 + XB[r * 3 + 2] * d2;
+    rel[9 + r] = (float)v;
+    tn += v * v;
+  }
+  float tau, pad;
+  fp32_item_bounds(sqrt(tn), GA.rmax, GB.rmax, tau, pad);
+  float4* q = reinterpret_cast<float4*>(out);
+  q[0] = make_float4(rel[0], rel[1], rel[2], rel[3]);
+  q[1] = make_float4(rel[4], rel[5], rel[6], rel[7]);
+  q[2] = make_float4(rel[8], rel[9], rel[10], rel[11]);
+  q[3] = make_float4(tau, pad, __int_as_float(GA.root), __int_as_float(GB.root));
+}
+
 // ------------------------------------------------------------ broad phase
 
 __device__ inline bool box_hit(const double* alo, const double* ahi, const double* blo, const double* bhi) {
@@ -183,7 +247,7 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
   const int n = P.nlinks;
   const size_t tb = a.toff ? (size_t)a.toff[c] : (size_t)c * n;
   const size_t qb = a.qoff ? (size_t)a.qoff[c] : (size_t)c * n;
-  double* T = a.T64 + tb * 12;
+  auto trow = [&](int l) { return t_row(a.toff, a.N, c, l); };
   double* LB = a.LB + tb * 6;
   // link box component r (0..2 lo, 3..5 hi) of link l
   auto lb = [&](int l, int r) -> double& {
@@ -212,13 +276,13 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
       if (L.parent >= 0) {
         X64 Pp;
         if (L.parent == l - 1) Pp = Xprev;
-        else x64_load(T + 12 * L.parent, Pp);
+        else t_load(a.T64, a.t_rows, trow(L.parent), Pp);
         x64_mul(Pp, Lx, X);
       } else {
         X = Lx;
       }
       Xprev = X;
-      x64_store(T + 12 * l, X);
+      t_store(a.T64, a.t_rows, trow(l), X);
       if (L.geom >= 0) {
         const GeomDesc& g = a.pool.geoms[L.geom];
         active |= 1ull << l;
@@ -300,6 +364,11 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
           it.b = (int16_t)e;
           it.rank = ordered ? (p1 * 65536 + p2) : 0;
           a.items[base + cnt] = it;
+          if (a.recs) {
+            X64 XA;
+            t_load(a.T64, a.t_rows, trow(l), XA);
+            write_item_rec(a.recs + base + cnt, XA.R, E.X, a.pool.geoms[P.links[l].geom], a.pool.geoms[E.geom]);
+          }
         }
         cnt++;
       }
@@ -318,10 +387,173 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
       it.b = (int16_t)j;
       it.rank = (1 << KG_RANK_GROUP_BIT) | (ordered ? k : 0);
       a.items[base + cnt] = it;
+      if (a.recs) {
+        X64 XA, XB;
+        t_load(a.T64, a.t_rows, trow(i), XA);
+        t_load(a.T64, a.t_rows, trow(j), XB);
+        write_item_rec(a.recs + base + cnt, XA.R, XB.R, a.pool.geoms[P.links[i].geom], a.pool.geoms[P.links[j].geom]);
+      }
     }
     cnt++;
   }
   if (MODE != 1) a.counts[c] = cnt;
+}
+
+// Broad phase with one lane per (configuration, link), for single-plan
+// batches of at most 8 links and a small environment (fixed-capacity item
+// layout).  Each lane computes its joint frame T0 J(q) in FP64; lane 0 of
+// the configuration's group composes the chain into world frames in LDS;
+// each lane then forms its link box, and the group emits the items: a link's
+// environment pairs by its own lane, the self pairs round-robin over the
+// group.  Item positions inside a configuration come from a counter in LDS;
+// their order does not matter, since the ranks carry Klampt's loop order.
+#define KG_PL_GROUPS 32  // configurations per 256-thread block
+__global__ __launch_bounds__(256) void k_prep_links(PrepArgs a) {
+  __shared__ double sF[KG_PL_GROUPS][8][12];
+  __shared__ double sB[KG_PL_GROUPS][8][6];
+  __shared__ int scnt[KG_PL_GROUPS], sjl[KG_PL_GROUPS];
+  const int t = threadIdx.x, g = t >> 3, l = t & 7;
+  const int c = blockIdx.x * KG_PL_GROUPS + g;
+  {
+    const int r = blockIdx.x * blockDim.x + t;
+    if (r < a.nreset) a.reset[r] = 0;
+  }
+  const PlanDev& P = a.plan;
+  const int n = P.nlinks;
+  const bool valid = c < a.N;
+  const bool mine = valid && l < n;
+  if (l == 0) {
+    scnt[g] = 0;
+    sjl[g] = 1;
+  }
+  __syncthreads();
+  // joint frames
+  if (mine) {
+    const double q = a.Qf ? (double)a.Qf[(size_t)c * n + l] : a.Qd[(size_t)c * n + l];
+    if (q < P.qmin[l] || q > P.qmax[l]) sjl[g] = 0;
+    const LinkDesc& L = P.links[l];
+    X64 J, T0, Lx;
+    joint64(L.jtype, L.axis, q, J);
+    x64_load(L.T0, T0);
+    x64_mul(T0, J, Lx);
+    x64_store(sF[g][l], Lx);
+  }
+  __syncthreads();
+  // world frames: parents precede their children
+  if (valid && l == 0)
+    for (int k = 0; k < n; k++) {
+      const int p = P.links[k].parent;
+      if (p >= 0) {
+        X64 Pp, Lx, X;
+        x64_load(sF[g][p], Pp);
+        x64_load(sF[g][k], Lx);
+        x64_mul(Pp, Lx, X);
+        x64_store(sF[g][k], X);
+      }
+    }
+  __syncthreads();
+  X64 X;
+  if (mine) {
+    x64_load(sF[g][l], X);
+    t_store(a.T64, a.t_rows, t_row(nullptr, a.N, c, l), X);
+    const LinkDesc& L = P.links[l];
+    if (L.geom >= 0) {
+      const GeomDesc& gd = a.pool.geoms[L.geom];
+      for (int r = 0; r < 3; r++) {
+        const double wc = X.R[r] * gd.lc[0] + X.R[3 + r] * gd.lc[1] + X.R[6 + r] * gd.lc[2] + X.t[r];
+        const double ext = fabs(X.R[r]) * gd.le[0] + fabs(X.R[3 + r]) * gd.le[1] + fabs(X.R[6 + r]) * gd.le[2];
+        sB[g][l][r] = wc - ext;
+        sB[g][l][3 + r] = wc + ext;
+      }
+    }
+  }
+  __syncthreads();
+  const bool jl = sjl[g] != 0;
+  const bool go = valid && (a.mode != KG_MODE_ANY || jl);
+  const bool ordered = a.mode == KG_MODE_ORDERED;
+  if (go) {
+    // union of the active links' boxes
+    double ulo[3] = {1e300, 1e300, 1e300}, uhi[3] = {-1e300, -1e300, -1e300};
+    uint32_t active = 0;
+    for (int k = 0; k < n; k++)
+      if (P.links[k].geom >= 0) {
+        active |= 1u << k;
+        for (int r = 0; r < 3; r++) {
+          ulo[r] = fmin(ulo[r], sB[g][k][r]);
+          uhi[r] = fmax(uhi[r], sB[g][k][3 + r]);
+        }
+      }
+    // environment entries surviving against the link union, their union
+    double elo[3] = {1e300, 1e300, 1e300}, ehi[3] = {-1e300, -1e300, -1e300};
+    int m2 = 0;
+    for (int e = 0; e < P.nenv; e++) {
+      const EnvItem& E = P.env[e];
+      if (box_hit(E.lo, E.hi, ulo, uhi)) {
+        m2++;
+        for (int r = 0; r < 3; r++) {
+          elo[r] = fmin(elo[r], E.lo[r]);
+          ehi[r] = fmax(ehi[r], E.hi[r]);
+        }
+      }
+    }
+    // links surviving against the environment union (active-list positions)
+    uint32_t lrej = 0;
+    {
+      int pos = 0;
+      for (int k = 0; k < n; k++)
+        if (active >> k & 1) {
+          if (!box_hit(sB[g][k], sB[g][k] + 3, elo, ehi)) lrej |= 1u << pos;
+          pos++;
+        }
+    }
+    const int nact = __popc(active), m1 = nact - __popc(lrej);
+    const int base = c * a.cap;
+    auto env_rej = [&](int i) { return !box_hit(P.env[i].lo, P.env[i].hi, ulo, uhi); };
+    auto link_rej = [&](int i) { return (lrej >> i & 1) != 0; };
+    // this link's environment pairs
+    if (l < n && m2 > 0 && (active >> l & 1)) {
+      const int pos = __popc(active & ((1u << l) - 1));
+      if (!(lrej >> pos & 1)) {
+        const int p1 = ordered ? swap_erase_pos(pos, nact, m1, link_rej) : 0;
+        const LinkDesc& L = P.links[l];
+        for (int e = 0; e < P.nenv; e++) {
+          const EnvItem& E = P.env[e];
+          if (!P.lemask[l * P.nenv + e]) continue;
+          if (!box_hit(E.lo, E.hi, ulo, uhi)) continue;
+          if (!box_hit(sB[g][l], sB[g][l] + 3, E.lo, E.hi)) continue;
+          const int p2 = ordered ? swap_erase_pos(e, P.nenv, m2, env_rej) : 0;
+          const int idx = base + atomicAdd(&scnt[g], 1);
+          PairItem it;
+          it.cfg = c;
+          it.a = (int16_t)l;
+          it.b = (int16_t)e;
+          it.rank = ordered ? (p1 * 65536 + p2) : 0;
+          a.items[idx] = it;
+          if (a.recs) write_item_rec(a.recs + idx, X.R, E.X, a.pool.geoms[L.geom], a.pool.geoms[E.geom]);
+        }
+      }
+    }
+    // self pairs, round-robin over the group
+    for (int k = l; k < P.nself; k += 8) {
+      const int i = P.selfpairs[2 * k], j = P.selfpairs[2 * k + 1];
+      if (!box_hit(sB[g][i], sB[g][i] + 3, sB[g][j], sB[g][j] + 3)) continue;
+      const int idx = base + atomicAdd(&scnt[g], 1);
+      PairItem it;
+      it.cfg = c;
+      it.a = (int16_t)i;
+      it.b = (int16_t)j;
+      it.rank = (1 << KG_RANK_GROUP_BIT) | (ordered ? k : 0);
+      a.items[idx] = it;
+      if (a.recs)
+        write_item_rec(a.recs + idx, sF[g][i], sF[g][j], a.pool.geoms[P.links[i].geom],
+                       a.pool.geoms[P.links[j].geom]);
+    }
+  }
+  __syncthreads();
+  if (valid && l == 0) {
+    a.jl[c] = jl ? 1 : 0;
+    a.counts[c] = go ? scnt[g] : 0;
+  }
 }
 
 // ------------------------------------------------------------ narrow phase
@@ -597,12 +829,46 @@ __device__ inline void slot_bounds(const TravArgs& a, double tnorm, double rA, d
     pad = a.tol.pad_bv;
     return;
   }
-  const double u = 5.9604644775390625e-08;
-  double ev = u * (8.0 * rA + 4.0 * tnorm) + u * rB;
-  tau = (float)(2.0 * ev);
-  // box tests: products of up to three rounded factors over coordinates
-  // bounded by |t| + rA + rB (including the inverted item transform)
-  pad = (float)(2.0 * ev + 32.0 * u * (tnorm + rA + rB));
+  fp32_item_bounds(tnorm, rA, rB, tau, pad);
+}
+
+// Result of configuration c from its kill state: best hit key and lowest
+// uncertain rank per group (env, self).  A configuration whose answer an
+// uncertain triangle test could still change goes to the re-check list
+// unless this is the exact pass.
+__device__ inline void finalize_cfg(const FinArgs& a, int c, unsigned long long b0, unsigned long long b1, int u0,
+                                    int u1) {
+  const bool jl = a.jl[c] != 0;
+  const bool h0 = b0 != KG_NONE64, h1 = b1 != KG_NONE64;
+  bool sure;
+  if (a.mode == KG_MODE_ANY) {
+    sure = !jl || h0 || h1 || (u0 == KG_RANK_NONE && u1 == KG_RANK_NONE);
+  } else if (a.mode == KG_MODE_GROUPS) {
+    sure = (h0 || u0 == KG_RANK_NONE) && (h1 || u1 == KG_RANK_NONE);
+  } else {
+    const int r0 = h0 ? (int)(b0 >> 32) : KG_RANK_NONE;
+    const int r1 = h1 ? (int)(b1 >> 32) : KG_RANK_NONE;
+    sure = (u0 > r0 || u0 == KG_RANK_NONE) && (u1 > r1 || u1 == KG_RANK_NONE);
+  }
+  if (!sure && !a.final_pass) {
+    const int slot = atomicAdd(a.redo_count, 1);
+    a.redo_list[slot] = c;
+    return;
+  }
+  if (a.feasible) a.feasible[c] = (jl && !h0 && !h1) ? 1 : 0;
+  if (a.env_hit) a.env_hit[c] = h0 ? 1 : 0;
+  if (a.self_hit) a.self_hit[c] = h1 ? 1 : 0;
+  if (a.first_pair) {
+    int id1 = -1, id2 = -1;
+    if (h0 || h1) {
+      const PairItem it = a.items[(uint32_t)((h0 ? b0 : b1) & 0xffffffffu)];
+      const PlanDev& PL = a.cfg_plan ? a.plans[a.cfg_plan[c]] : a.plan;
+      id1 = PL.links[it.a].wid;
+      id2 = h0 ? PL.env[it.b].wid : PL.links[it.b].wid;
+    }
+    a.first_pair[2 * c] = id1;
+    a.first_pair[2 * c + 1] = id2;
+  }
 }
 
 // Frame of pair item P: rel maps mesh A's frame to mesh B's (FP64 product,
@@ -611,17 +877,19 @@ template <typename Real, bool EXACT>
 __device__ inline void item_frame(const TravArgs& a, const PairItem& P, Real* rel, float& tau, float& pad, int& rootA,
                                   int& rootB) {
   const PlanDev& PL = a.cfg_plan ? a.plans[a.cfg_plan[P.cfg]] : a.plan;
-  const size_t tb = a.toff ? (size_t)a.toff[P.cfg] : (size_t)P.cfg * PL.nlinks;
   const LinkDesc& LA = PL.links[P.a];
   const int g = (P.rank >> KG_RANK_GROUP_BIT) & 1;
-  const double* XA = a.T64 + (tb + P.a) * 12;
+  X64 FA, FB;
+  t_load(a.T64, a.t_rows, t_row(a.toff, a.t_n, P.cfg, P.a), FA);
+  const double* XA = FA.R;
   const double* XB;
   int geomB;
   if (g == 0) {
     XB = PL.env[P.b].X;
     geomB = PL.env[P.b].geom;
   } else {
-    XB = a.T64 + (tb + P.b) * 12;
+    t_load(a.T64, a.t_rows, t_row(a.toff, a.t_n, P.cfg, P.b), FB);
+    XB = FB.R;
     geomB = PL.links[P.b].geom;
   }
   const double d0 = XA[9] - XB[9], d1 = XA[10] - XB[10], d2 = XA[11] - XB[11];
@@ -676,7 +944,8 @@ __device__ inline uint64_t mk_leaf(int slot, int ta, int tb) {
 // and writes a configuration's result as soon as its last item finishes, so
 // a heavy item never holds the wave's other lanes idle at a chunk boundary.
 template <typename Real, bool EXACT, int SCAP>
-__global__ __launch_bounds__(64) void k_collide(TravArgs a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SCAP == 256 && !EXACT ? 4 : 1))) void k_collide(
+    TravArgs a) {
   // the small-pool variant also halves the leaf queue and the ring so that
   // four waves fit per SIMD
   constexpr int LQ = SCAP >= 512 ? KG_LQ : KG_LQ / 2;
@@ -732,16 +1001,18 @@ __global__ __launch_bounds__(64) void k_collide(TravArgs a) {
   int sp = 0, gsp = 0, nl = 0;
   uint64_t cont = KG_NOTASK;  // this lane's depth-first continuation
 
-  // the last open item of an entry has finished: publish its result
-  auto settle_entry = [&](int e, int delta) {
-    if (atomicAdd(&e_open[e], delta) + delta == 0) {
-      const int cc = e_cfg[e];
-      a.best[2 * cc] = s_best[e][0];
-      a.best[2 * cc + 1] = s_best[e][1];
-      a.unc[2 * cc] = s_uncmin[e][0];
-      a.unc[2 * cc + 1] = s_uncmin[e][1];
-      e_cfg[e] = -1;
+  // an entry whose open count reaches 0 is done; its result is published
+  // (and the entry freed) by finalize_done at the next fetch or at exit
+  auto settle_entry = [&](int e, int delta) { atomicAdd(&e_open[e], delta); };
+  auto finalize_done = [&]() {
+    if (lane < NE) {
+      const int cc = e_cfg[lane];
+      if (cc >= 0 && e_open[lane] == 0) {
+        finalize_cfg(a.fin, cc, s_best[lane][0], s_best[lane][1], s_uncmin[lane][0], s_uncmin[lane][1]);
+        e_cfg[lane] = -1;
+      }
     }
+    __syncthreads();
   };
   auto settle = [&](int slot, int delta) {
     if (atomicAdd(&s_pend[slot], delta) + delta == 0) {
@@ -835,6 +1106,7 @@ __global__ __launch_bounds__(64) void k_collide(TravArgs a) {
     if (navail > 0 && fe == tail && !exhausted) {
       // fetch configurations into the ring (wave-uniform): first this wave's
       // static share, then chunks from the global counter
+      finalize_done();
       while (head < tail && e_cfg[head % NE] < 0) head++;
       const int room = NE - (tail - head);
       if (room > 0) {
@@ -960,6 +1232,7 @@ __global__ __launch_bounds__(64) void k_collide(TravArgs a) {
         if (nl > 0) {
           run_leaves();
         } else if (fe == tail && exhausted) {
+          finalize_done();
           break;
         }
       }
@@ -1060,67 +1333,11 @@ __global__ __launch_bounds__(64) void k_collide(TravArgs a) {
   }
 }
 
-// Item start records: lane per (configuration, item position).
-__global__ __launch_bounds__(256) void k_items(TravArgs a, int N, int maxcap, ItemRec* __restrict__ recs) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int c = i / maxcap, k = i - c * maxcap;
-  if (c >= N || k >= a.counts[c]) return;
-  const int b = a.offsets ? a.offsets[c] : (a.ibase ? a.ibase[c] : c * a.cap);
-  const PairItem P = a.items[b + k];
-  float rel[12], tau, pad;
-  int rootA, rootB;
-  item_frame<float, false>(a, P, rel, tau, pad, rootA, rootB);
-  float4* q = reinterpret_cast<float4*>(recs + b + k);
-  q[0] = make_float4(rel[0], rel[1], rel[2], rel[3]);
-  q[1] = make_float4(rel[4], rel[5], rel[6], rel[7]);
-  q[2] = make_float4(rel[8], rel[9], rel[10], rel[11]);
-  q[3] = make_float4(tau, pad, __int_as_float(rootA), __int_as_float(rootB));
-}
-
 __global__ void k_fill_int(int* p, int n, int v) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
 
-__global__ __launch_bounds__(256) void k_finalize(FinArgs a, const int* list, const int* list_count) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  int n = list ? *list_count : a.N;
-  if (k >= n) return;
-  int c = list ? list[k] : k;
-  const unsigned long long b0 = a.best[2 * c], b1 = a.best[2 * c + 1];
-  const int u0 = a.unc[2 * c], u1 = a.unc[2 * c + 1];
-  const bool jl = a.jl[c] != 0;
-  const bool h0 = b0 != KG_NONE64, h1 = b1 != KG_NONE64;
-  bool sure;
-  if (a.mode == KG_MODE_ANY) {
-    sure = !jl || h0 || h1 || (u0 == KG_RANK_NONE && u1 == KG_RANK_NONE);
-  } else if (a.mode == KG_MODE_GROUPS) {
-    sure = (h0 || u0 == KG_RANK_NONE) && (h1 || u1 == KG_RANK_NONE);
-  } else {
-    const int r0 = h0 ? (int)(b0 >> 32) : KG_RANK_NONE;
-    const int r1 = h1 ? (int)(b1 >> 32) : KG_RANK_NONE;
-    sure = (u0 > r0 || u0 == KG_RANK_NONE) && (u1 > r1 || u1 == KG_RANK_NONE);
-  }
-  if (!sure && !a.final_pass) {
-    int slot = atomicAdd(a.redo_count, 1);
-    a.redo_list[slot] = c;
-    return;
-  }
-  if (a.feasible) a.feasible[c] = (jl && !h0 && !h1) ? 1 : 0;
-  if (a.env_hit) a.env_hit[c] = h0 ? 1 : 0;
-  if (a.self_hit) a.self_hit[c] = h1 ? 1 : 0;
-  if (a.first_pair) {
-    int id1 = -1, id2 = -1;
-    if (h0 || h1) {
-      const PairItem it = a.items[(uint32_t)((h0 ? b0 : b1) & 0xffffffffu)];
-      const PlanDev& PL = a.cfg_plan ? a.plans[a.cfg_plan[c]] : a.plan;
-      id1 = PL.links[it.a].wid;
-      id2 = h0 ? PL.env[it.b].wid : PL.links[it.b].wid;
-    }
-    a.first_pair[2 * c] = id1;
-    a.first_pair[2 * c + 1] = id2;
-  }
-}
 
 }  // namespace
 
@@ -1157,7 +1374,10 @@ hipError_t kgl_prep_fill(const PrepArgs& a, hipStream_t s) {
 hipError_t kgl_prep_single(const PrepArgs& a, hipStream_t s) {
   const int n = a.N > a.nreset ? a.N : a.nreset;
   if (n <= 0) return hipSuccess;
-  if (a.lds_links && a.plan.nlinks <= KG_PREP_LDS_LINKS)
+  if (a.lds_links && a.plan.nlinks <= 8 && a.plan.nenv <= 64) {
+    const int blocks = max((a.N + KG_PL_GROUPS - 1) / KG_PL_GROUPS, (a.nreset + 255) / 256);
+    k_prep_links<<<blocks, 256, 0, s>>>(a);
+  } else if (a.lds_links && a.plan.nlinks <= KG_PREP_LDS_LINKS)
     k_prep<2, KG_PREP_LDS_LINKS><<<(n + KG_PREP_LDS_BLOCK - 1) / KG_PREP_LDS_BLOCK, KG_PREP_LDS_BLOCK, 0, s>>>(a);
   else
     k_prep<2, 0><<<(n + 255) / 256, 256, 0, s>>>(a);
@@ -1178,25 +1398,6 @@ static hipError_t launch_collide(const TravArgs& a, hipStream_t s) {
 hipError_t kgl_collide_f32(const TravArgs& a, hipStream_t s) { return launch_collide<float, false>(a, s); }
 hipError_t kgl_collide_f64(const TravArgs& a, hipStream_t s) { return launch_collide<double, true>(a, s); }
 
-hipError_t kgl_items(const TravArgs& a, int N, int maxcap, ItemRec* recs, hipStream_t s) {
-  const long long n = (long long)N * maxcap;
-  if (n <= 0) return hipSuccess;
-  if (n >= (1ll << 31)) return hipErrorInvalidValue;
-  k_items<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(a, N, maxcap, recs);
-  return hipGetLastError();
-}
-
-hipError_t kgl_finalize(const FinArgs& a, hipStream_t s) {
-  if (a.N <= 0) return hipSuccess;
-  k_finalize<<<(a.N + 255) / 256, 256, 0, s>>>(a, nullptr, nullptr);
-  return hipGetLastError();
-}
-
-hipError_t kgl_finalize_list(const FinArgs& a, const int* list, const int* count, int maxn, hipStream_t s) {
-  if (maxn <= 0) return hipSuccess;
-  k_finalize<<<(maxn + 255) / 256, 256, 0, s>>>(a, list, count);
-  return hipGetLastError();
-}
 
 hipError_t kgl_fill_int(int* p, int n, int v, hipStream_t s) {
   if (n <= 0) return hipSuccess;
