@@ -616,8 +616,9 @@ __device__ int tri_tri_sat(const V3T<Real>* p, const V3T<Real>* q, Real tau) {
     P[i] = vsub(p[i], q[0]);
     Q[i] = vsub(q[i], q[0]);
   }
-  V3T<Real> eP[3] = {vsub(P[1], P[0]), vsub(P[2], P[1]), vsub(P[0], P[2])};
-  V3T<Real> eQ[3] = {vsub(Q[1], Q[0]), vsub(Q[2], Q[1]), vsub(Q[0], Q[2])};
+  // edges are re-formed from the vertices where needed (fewer live registers)
+  auto eP = [&](int i) { return vsub(P[i == 2 ? 0 : i + 1], P[i]); };
+  auto eQ = [&](int i) { return vsub(Q[i == 2 ? 0 : i + 1], Q[i]); };
   const Real u = EXACT ? (Real)0 : (Real)5.9604645e-08;
   Real RD = 0;
   if (!EXACT) {
@@ -627,25 +628,26 @@ __device__ int tri_tri_sat(const V3T<Real>* p, const V3T<Real>* q, Real tau) {
     tau += (Real)8 * u * RD;
   }
   bool sure = true;
-  V3T<Real> nP = vcross(eP[0], eP[1]);
-  V3T<Real> nQ = vcross(eQ[0], eQ[1]);
-  Real lP0 = EXACT ? 0 : vl1(eP[0]), lP1 = EXACT ? 0 : vl1(eP[1]), lP2 = EXACT ? 0 : vl1(eP[2]);
-  Real lQ0 = EXACT ? 0 : vl1(eQ[0]), lQ1 = EXACT ? 0 : vl1(eQ[1]), lQ2 = EXACT ? 0 : vl1(eQ[2]);
-  Real lP[3] = {lP0, lP1, lP2}, lQ[3] = {lQ0, lQ1, lQ2};
-  Real enP = (Real)8 * u * lP0 * lP1, enQ = (Real)8 * u * lQ0 * lQ1;
+  const V3T<Real> nP = vcross(eP(0), eP(1));
+  const V3T<Real> nQ = vcross(eQ(0), eQ(1));
+  const Real lP0 = EXACT ? 0 : vl1(eP(0)), lP1 = EXACT ? 0 : vl1(eP(1)), lP2 = EXACT ? 0 : vl1(eP(2));
+  const Real lQ0 = EXACT ? 0 : vl1(eQ(0)), lQ1 = EXACT ? 0 : vl1(eQ(1)), lQ2 = EXACT ? 0 : vl1(eQ(2));
+  auto lP = [&](int i) { return i == 0 ? lP0 : (i == 1 ? lP1 : lP2); };
+  auto lQ = [&](int i) { return i == 0 ? lQ0 : (i == 1 ? lQ1 : lQ2); };
+  const Real enP = (Real)8 * u * lP0 * lP1, enQ = (Real)8 * u * lQ0 * lQ1;
   if (sat_axis<Real, EXACT>(nP, enP, P, Q, tau, RD, sure)) return TT_SEP;
   if (sat_axis<Real, EXACT>(nQ, enQ, P, Q, tau, RD, sure)) return TT_SEP;
 #pragma unroll
   for (int i = 0; i < 3; i++)
 #pragma unroll
     for (int j = 0; j < 3; j++)
-      if (sat_axis<Real, EXACT>(vcross(eP[i], eQ[j]), (Real)8 * u * lP[i] * lQ[j], P, Q, tau, RD, sure))
+      if (sat_axis<Real, EXACT>(vcross(eP(i), eQ(j)), (Real)8 * u * lP(i) * lQ(j), P, Q, tau, RD, sure))
         return TT_SEP;
 #pragma unroll
   for (int i = 0; i < 3; i++) {
-    if (sat_axis<Real, EXACT>(vcross(nP, eP[i]), (Real)16 * u * lP0 * lP1 * lP[i], P, Q, tau, RD, sure))
+    if (sat_axis<Real, EXACT>(vcross(nP, eP(i)), (Real)16 * u * lP0 * lP1 * lP(i), P, Q, tau, RD, sure))
       return TT_SEP;
-    if (sat_axis<Real, EXACT>(vcross(nQ, eQ[i]), (Real)16 * u * lQ0 * lQ1 * lQ[i], P, Q, tau, RD, sure))
+    if (sat_axis<Real, EXACT>(vcross(nQ, eQ(i)), (Real)16 * u * lQ0 * lQ1 * lQ(i), P, Q, tau, RD, sure))
       return TT_SEP;
   }
   return (EXACT || sure) ? TT_HIT : TT_UNSURE;
@@ -836,6 +838,7 @@ __device__ inline void slot_bounds(const TravArgs& a, double tnorm, double rA, d
 // uncertain rank per group (env, self).  A configuration whose answer an
 // uncertain triangle test could still change goes to the re-check list
 // unless this is the exact pass.
+template <bool FPAIR>
 __device__ inline void finalize_cfg(const FinArgs& a, int c, unsigned long long b0, unsigned long long b1, int u0,
                                     int u1) {
   const bool jl = a.jl[c] != 0;
@@ -858,7 +861,7 @@ __device__ inline void finalize_cfg(const FinArgs& a, int c, unsigned long long 
   if (a.feasible) a.feasible[c] = (jl && !h0 && !h1) ? 1 : 0;
   if (a.env_hit) a.env_hit[c] = h0 ? 1 : 0;
   if (a.self_hit) a.self_hit[c] = h1 ? 1 : 0;
-  if (a.first_pair) {
+  if (FPAIR && a.first_pair) {
     int id1 = -1, id2 = -1;
     if (h0 || h1) {
       const PairItem it = a.items[(uint32_t)((h0 ? b0 : b1) & 0xffffffffu)];
@@ -943,7 +946,8 @@ __device__ inline uint64_t mk_leaf(int slot, int ta, int tb) {
 // count over the grid size), starts their items as lanes and slots free up,
 // and writes a configuration's result as soon as its last item finishes, so
 // a heavy item never holds the wave's other lanes idle at a chunk boundary.
-template <typename Real, bool EXACT, int SCAP>
+// FPAIR: the first-colliding-pair output is requested (ordered mode).
+template <typename Real, bool EXACT, int SCAP, bool FPAIR>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SCAP == 256 && !EXACT ? 4 : 1))) void k_collide(
     TravArgs a) {
   // the small-pool variant also halves the leaf queue and the ring so that
@@ -1008,7 +1012,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SCAP == 256 
     if (lane < NE) {
       const int cc = e_cfg[lane];
       if (cc >= 0 && e_open[lane] == 0) {
-        finalize_cfg(a.fin, cc, s_best[lane][0], s_best[lane][1], s_uncmin[lane][0], s_uncmin[lane][1]);
+        finalize_cfg<FPAIR>(a.fin, cc, s_best[lane][0], s_best[lane][1], s_uncmin[lane][0], s_uncmin[lane][1]);
         e_cfg[lane] = -1;
       }
     }
@@ -1079,7 +1083,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SCAP == 256 
     __syncthreads();
   };
 
+  bool quit = false;
   for (;;) {
+    // publish finished configurations and free their entries (the only
+    // place results are written)
+    finalize_done();
+    if (quit) break;
     // ---- lanes without a continuation pop the pool (bring spilled tasks
     // back first when it runs low)
     const bool hungry = cont == KG_NOTASK;
@@ -1106,8 +1115,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SCAP == 256 
     if (navail > 0 && fe == tail && !exhausted) {
       // fetch configurations into the ring (wave-uniform): first this wave's
       // static share, then chunks from the global counter
-      finalize_done();
-      while (head < tail && e_cfg[head % NE] < 0) head++;
+      while (head < tail && __builtin_amdgcn_readfirstlane(e_cfg[head % NE]) < 0) head++;
       const int room = NE - (tail - head);
       if (room > 0) {
         int k0 = 0, nk = 0;
@@ -1128,7 +1136,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SCAP == 256 
             const int ch = min(room, max(1, min(a.cpw, (r1 - last_k0) / wpr)));
             if (lane == 0) s_chunk = r0 + atomicAdd(a.chunk_counter + sub * KG_COUNTER_STRIDE, ch);
             __syncthreads();
-            k0 = s_chunk;
+            k0 = __builtin_amdgcn_readfirstlane(s_chunk);
             __syncthreads();
             if (k0 < r1) {
               last_k0 = k0;
@@ -1164,8 +1172,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SCAP == 256 
       int assigned = 0;
       while (fe < tail) {
         if (!fe_open) {
-          it = e_beg[fe % NE];
-          itend = e_end[fe % NE];
+          it = __builtin_amdgcn_readfirstlane(e_beg[fe % NE]);
+          itend = __builtin_amdgcn_readfirstlane(e_end[fe % NE]);
           fe_open = true;
         }
         if (it >= itend) {
@@ -1232,8 +1240,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SCAP == 256 
         if (nl > 0) {
           run_leaves();
         } else if (fe == tail && exhausted) {
-          finalize_done();
-          break;
+          quit = true;
         }
       }
       continue;
@@ -1384,14 +1391,22 @@ hipError_t kgl_prep_single(const PrepArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+template <typename Real, bool EXACT, int SCAP>
+static void launch_scap(const TravArgs& a, hipStream_t s) {
+  if (a.fin.first_pair)
+    k_collide<Real, EXACT, SCAP, true><<<a.grid, 64, 0, s>>>(a);
+  else
+    k_collide<Real, EXACT, SCAP, false><<<a.grid, 64, 0, s>>>(a);
+}
+
 template <typename Real, bool EXACT>
 static hipError_t launch_collide(const TravArgs& a, hipStream_t s) {
   if (a.scap == 256)
-    k_collide<Real, EXACT, 256><<<a.grid, 64, 0, s>>>(a);
+    launch_scap<Real, EXACT, 256>(a, s);
   else if (a.scap == 512)
-    k_collide<Real, EXACT, 512><<<a.grid, 64, 0, s>>>(a);
+    launch_scap<Real, EXACT, 512>(a, s);
   else
-    k_collide<Real, EXACT, 1024><<<a.grid, 64, 0, s>>>(a);
+    launch_scap<Real, EXACT, 1024>(a, s);
   return hipGetLastError();
 }
 
@@ -1409,14 +1424,14 @@ int kgl_collide_grid(int exact, int num_cus, int scap) {
   int nb = 0;
   hipError_t e;
   if (scap == 256)
-    e = exact ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_collide<double, true, 256>, 64, 0)
-              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_collide<float, false, 256>, 64, 0);
+    e = exact ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_collide<double, true, 256, false>, 64, 0)
+              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_collide<float, false, 256, false>, 64, 0);
   else if (scap == 512)
-    e = exact ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_collide<double, true, 512>, 64, 0)
-              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_collide<float, false, 512>, 64, 0);
+    e = exact ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_collide<double, true, 512, false>, 64, 0)
+              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_collide<float, false, 512, false>, 64, 0);
   else
-    e = exact ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_collide<double, true, 1024>, 64, 0)
-              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_collide<float, false, 1024>, 64, 0);
+    e = exact ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_collide<double, true, 1024, false>, 64, 0)
+              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_collide<float, false, 1024, false>, 64, 0);
   if (e != hipSuccess || nb < 1) nb = 8;
   const char* v = getenv("KG_WAVES_PER_CU");
   if (v && atoi(v) > 0) nb = atoi(v);
