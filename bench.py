@@ -215,6 +215,15 @@ def run_secondary(args, ctx, sc, world, robot):
     out["c2_distance"] = {"configs": args.dist_configs, "pairs_per_config": npairs,
                           "pair_distances_per_s": round(args.dist_configs * npairs / t, 1),
                           "configs_per_s": round(args.dist_configs / t, 1), "ms": round(t * 1e3, 3)}
+    if not args.no_cpu:
+        import oracle
+        ow, orb = oracle.world_from_scene(sc)
+        Qc = Qd[:512].cpu().numpy().astype(np.float64)
+        t1 = time.perf_counter()
+        ow.distance_batch(orb, Qc)
+        tc = time.perf_counter() - t1
+        out["c2_distance"]["cpu_pair_distances_per_s"] = round(len(Qc) * npairs / tc, 1)
+        out["c2_distance"]["cpu_sample"] = f"{len(Qc)} configurations, FP64 oracle, {oracle.nthreads_default()} threads"
     print(f"[bench] C2 distances: {out['c2_distance']}", file=sys.stderr, flush=True)
 
     # C4 edges: 8192 segments, L-infinity step 0.01 rad, 10..500 samples
@@ -227,6 +236,12 @@ def run_secondary(args, ctx, sc, world, robot):
     out["c4_edges"] = {"segments": args.edges, "edges_per_s": round(args.edges / te, 1), "ms": round(te * 1e3, 2),
                        "free_fraction": round(float((fu < 0).mean()), 4),
                        "note": "host-pointer call including H2D/D2H copies and per-round host sync"}
+    if not args.no_cpu:
+        t1 = time.perf_counter()
+        ow.edge_batch(orb, A[:1024], B[:1024], eps=0.01, norm_inf=True)
+        tc = time.perf_counter() - t1
+        out["c4_edges"]["cpu_edges_per_s"] = round(1024 / tc, 1)
+        out["c4_edges"]["cpu_sample"] = f"first 1024 segments, FP64 oracle, {oracle.nthreads_default()} threads"
     print(f"[bench] C4 edges: {out['c4_edges']}", file=sys.stderr, flush=True)
 
     # C5 SDF hinge penalty, forward + backward
@@ -269,6 +284,18 @@ def run_secondary(args, ctx, sc, world, robot):
                                        "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                        "frac": round(Nw * bytes_per_wp / tb / 1e9 / HBM_PEAK_GBS, 4),
                                        "bytes_per_waypoint": bytes_per_wp}}
+    if not args.no_cpu:
+        orw = oracle.OracleWorld()
+        orr = orw.add_robot(sc.robot)
+        g64 = S.SDFGrid(grid.bmin, grid.bmax, grid.values.astype(np.float64))
+        Qc = Qw[:4096].cpu().numpy().astype(np.float64)
+        t1 = time.perf_counter()
+        orw.sdf_penalty(orr, g64, Qc, pts, ppl, dsafe=0.05, grad=True)
+        tc = time.perf_counter() - t1
+        out["c5_sdf_grad"]["cpu_evals_per_s"] = round(len(Qc) / tc, 1)
+        out["c5_sdf_grad"]["cpu_sample"] = (f"first {len(Qc)} waypoints, FP64 oracle loss and analytic gradient, "
+                                            f"{oracle.nthreads_default()} threads")
+        del g64
     print(f"[bench] C5 SDF: {out['c5_sdf_grad']}", file=sys.stderr, flush=True)
 
     # C3 heterogeneous robots in one launch

@@ -80,6 +80,16 @@ int kg_geom_destroy(kg_geom g);
  * (Python/klampt/src/robotsim.cpp:1465-1473) */
 int kg_geom_set_transform(kg_geom g, const double R[9], const double t[3]);
 int kg_geom_get_transform(kg_geom g, double R[9], double t[3]);
+/* Geometry3D.setCollisionMargin / getCollisionMargin
+ * (Python/klampt/src/robotsim.cpp:1561-1573): a nonnegative margin that
+ * fattens the geometry for collision testing (Cpp/docs/Manual-Geometry.md,
+ * "Collision geometries").  kg_geom_collide then tests distance <= the summed
+ * margins, kg_geom_within_distance adds them to the tolerance and
+ * kg_geom_distance subtracts them (negative when the fattened geometries
+ * overlap; defined by this build).  Batched feasibility calls reject worlds
+ * with margins (KG_E_UNSUPPORTED). */
+int kg_geom_set_collision_margin(kg_geom g, double margin);
+int kg_geom_get_collision_margin(kg_geom g, double* margin);
 /* Geometry3D.getBB (Python/klampt/src/robotsim.cpp:1575-1586): world AABB of
  * the transformed local bounding box. */
 int kg_geom_get_bb(kg_geom g, double bmin[3], double bmax[3]);

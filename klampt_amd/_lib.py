@@ -44,6 +44,8 @@ SIGNATURES = {
     "kg_geom_set_transform": [_vp, _pd, _pd],
     "kg_geom_get_transform": [_vp, _pd, _pd],
     "kg_geom_get_bb": [_vp, _pd, _pd],
+    "kg_geom_set_collision_margin": [_vp, _c_double],
+    "kg_geom_get_collision_margin": [_vp, _pd],
     "kg_geom_collide": [_vp, _vp, _pi],
     "kg_geom_within_distance": [_vp, _vp, _c_double, _pi],
     "kg_geom_distance": [_vp, _vp, ctypes.POINTER(DistSettings), ctypes.POINTER(DistResult)],

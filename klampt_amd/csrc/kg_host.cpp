@@ -617,8 +617,24 @@ int active_links(const kg_robot_s* r) {
 }
 
 // Batched feasibility of one robot on device buffers (async on stream s).
+// Collision margins turn a collide into a distance-threshold test; the
+// batched traversal decides intersection only, so batches reject them.
+void reject_margins(const kg_world_s* w) {
+  auto has = [](const kg_geom_s* g) { return g && g->margin > 0; };
+  bool any = false;
+  for (auto* g : w->terrains) any |= has(g);
+  for (auto& o : w->objects) any |= has(o.first);
+  for (auto* r : w->robots)
+    for (auto* g : r->link_geoms) any |= has(g);
+  if (any)
+    fail(KG_E_UNSUPPORTED,
+         "collision margins are supported by single-pair queries (collide, withinDistance, distance), not by "
+         "batched feasibility");
+}
+
 void feasible_dev(kg_world_s* w, int ri, const float* Qf, const double* Qd, int N, uint8_t* feas, uint8_t* env,
                   uint8_t* self, int32_t* fp, hipStream_t s) {
+  reject_margins(w);
   RobotPlan& P = get_plan(w, ri);
   Layout L;
   L.P = &P;
@@ -844,6 +860,21 @@ int kg_geom_get_transform(kg_geom g, double R[9], double t[3]) {
     need(g && R && t, "NULL argument");
     std::memcpy(R, g->X.R, sizeof(g->X.R));
     std::memcpy(t, g->X.t, sizeof(g->X.t));
+  });
+}
+
+int kg_geom_set_collision_margin(kg_geom g, double margin) {
+  return guard([&] {
+    need(g != nullptr, "geometry is NULL");
+    need(margin >= 0 && std::isfinite(margin), "margin must be finite and >= 0");
+    g->margin = margin;
+  });
+}
+
+int kg_geom_get_collision_margin(kg_geom g, double* margin) {
+  return guard([&] {
+    need(g && margin, "NULL argument");
+    *margin = g->margin;
   });
 }
 
@@ -1354,6 +1385,10 @@ PairWorld& pair_world(kg_ctx_s* c, kg_geom_s* a, kg_geom_s* b) {
 }
 }  // namespace
 
+namespace {
+void pair_distance(kg_geom_s* a, kg_geom_s* b, float stop_below, double upper, kg_dist_result* out, float* dgpu);
+}  // namespace
+
 extern "C" int kg_geom_collide(kg_geom a, kg_geom b, int* out) {
   return guard([&] {
     need(a && b && out, "NULL argument");
@@ -1361,6 +1396,15 @@ extern "C" int kg_geom_collide(kg_geom a, kg_geom b, int* out) {
     *out = 0;
     if (a->kind != 0 || b->kind != 0) fail(KG_E_UNSUPPORTED, "collide is implemented for TriangleMesh pairs");
     if (geom_empty(a) || geom_empty(b)) return;  // Geometry3D.collides: empty -> False
+    const double m = a->margin + b->margin;
+    if (m > 0) {
+      // fattened geometries collide when within the summed margins
+      kg_dist_result r;
+      float dg = 0;
+      pair_distance(a, b, (float)m, INFINITY, &r, &dg);
+      *out = r.d <= m ? 1 : 0;
+      return;
+    }
     kg_ctx_s* c = a->ctx;
     PairWorld& pw = pair_world(c, a, b);
     hip_check(hipSetDevice(c->device), "hipSetDevice");
@@ -1727,8 +1771,11 @@ extern "C" int kg_geom_distance(kg_geom a, kg_geom b, const kg_dist_settings* se
     if (a->kind != 0 || b->kind != 0) fail(KG_E_UNSUPPORTED, "distance is implemented for TriangleMesh pairs");
     if (geom_empty(a)) fail(KG_E_EMPTY, "Geometry3D.distance: Geometry is empty");
     if (geom_empty(b)) fail(KG_E_EMPTY, "Geometry3D.distance: Other geometry is empty");
-    const double upper = settings ? settings->upperBound : INFINITY;
+    const double m = a->margin + b->margin;
+    const double upper = settings ? settings->upperBound + m : INFINITY;
     pair_distance(a, b, 0.0f, upper, out, nullptr);
+    // margins shrink the distance (negative when the fattened geometries overlap)
+    out->d -= m;
   });
 }
 
@@ -1739,10 +1786,11 @@ extern "C" int kg_geom_within_distance(kg_geom a, kg_geom b, double tol, int* ou
     *out = 0;
     if (a->kind != 0 || b->kind != 0) fail(KG_E_UNSUPPORTED, "withinDistance is implemented for TriangleMesh pairs");
     if (geom_empty(a) || geom_empty(b)) return;
+    const double thr = tol + a->margin + b->margin;
     kg_dist_result r;
     float dg = 0;
-    pair_distance(a, b, (float)tol, INFINITY, &r, &dg);
-    *out = r.d <= tol ? 1 : 0;
+    pair_distance(a, b, (float)thr, INFINITY, &r, &dg);
+    *out = r.d <= thr ? 1 : 0;
   });
 }
 
@@ -1888,6 +1936,7 @@ Layout multi_layout(kg_ctx_s* c, int nb, kg_world_s* const* worlds, const int32_
   for (int b = 0; b < nb; b++) {
     need(worlds[b] != nullptr, "NULL world");
     need(worlds[b]->ctx == c, "worlds belong to different contexts");
+    reject_margins(worlds[b]);
     need(counts[b] >= 0, "negative configuration count");
     RobotPlan& P = get_plan(worlds[b], ris[b]);
     devplans[b] = P.dev;

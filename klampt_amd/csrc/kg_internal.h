@@ -250,6 +250,7 @@ struct kg_geom_s {
   double bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
   DevBuf dvals;
   bool alive = true;
+  double margin = 0;  // collision margin (Geometry3D.setCollisionMargin)
 };
 
 struct kg_robot_s {

@@ -106,6 +106,14 @@ class Geom:
         call("kg_geom_get_bb", self.h, _p(lo, _pd), _p(hi, _pd))
         return lo, hi
 
+    def set_margin(self, margin):
+        call("kg_geom_set_collision_margin", self.h, float(margin))
+
+    def get_margin(self):
+        m = ctypes.c_double()
+        call("kg_geom_get_collision_margin", self.h, ctypes.byref(m))
+        return m.value
+
     def collides(self, other):
         out = ctypes.c_int32()
         call("kg_geom_collide", self.h, other.h, ctypes.byref(out))

@@ -109,6 +109,13 @@ class Geometry3D:
         R, t = self._g.get_transform()
         return list(R), list(t)
 
+    def setCollisionMargin(self, margin: float):
+        """Fattens the geometry for collision testing (robotsim.cpp:1561-1566)."""
+        self._g.set_margin(margin)
+
+    def getCollisionMargin(self) -> float:
+        return self._g.get_margin()
+
     def getBB(self):
         lo, hi = self._g.get_bb()
         return list(lo), list(hi)

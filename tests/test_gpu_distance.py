@@ -69,3 +69,44 @@ def test_pair_distance_closest_points_and_within(ctx):
         for tol in (0.01, 0.05):
             if abs(d64 - tol) > 1e-6:
                 assert ga.within_distance(gb, tol) == (d64 <= tol)
+
+
+def test_collision_margins_single_pair(ctx):
+    """A margin fattens the geometry: collides <=> distance <= summed margins;
+    withinDistance adds the margins; distance subtracts them; batched
+    feasibility rejects worlds with margins."""
+    sc = S.boxes_scene(0)
+    cube = S.box_mesh(np.array([0.2, 0.2, 0.2]))
+    ga = engine.Geom.trimesh(ctx, cube.verts, cube.tris)
+    gb = engine.Geom.trimesh(ctx, cube.verts, cube.tris)
+    ow = oracle.OracleWorld()
+    ia, ib = ow.add_mesh(cube), ow.add_mesh(cube)
+    rng = np.random.default_rng(4)
+    for k in range(24):
+        xa = S.f32r(S._xf(S.random_rotation(rng), rng.uniform(-0.1, 0.1, 3)))
+        xb = S.f32r(S._xf(S.random_rotation(rng), rng.uniform(-0.8, 0.8, 3)))
+        ga.set_transform(xa[:9], xa[9:])
+        gb.set_transform(xb[:9], xb[9:])
+        d64, _, _, _ = ow.mesh_distance(ia, xa, ib, xb)
+        ma, mb = 0.03, 0.05
+        ga.set_margin(ma)
+        gb.set_margin(mb)
+        assert ga.get_margin() == ma
+        if abs(d64 - (ma + mb)) > 1e-5:
+            assert ga.collides(gb) == (d64 <= ma + mb), k
+            assert ga.within_distance(gb, 0.1) == (d64 <= 0.1 + ma + mb), k
+        r = ga.distance(gb)
+        assert abs(r.d - (d64 - ma - mb)) < 1e-5, k
+        ga.set_margin(0.0)
+        gb.set_margin(0.0)
+        if abs(d64) > 1e-6:
+            assert ga.collides(gb) == (d64 == 0), k
+    # batched feasibility with a margin is refused, not silently wrong
+    from klampt_amd._lib import KlamptGPUError
+    terrain = engine.Geom.trimesh(ctx, cube.verts, cube.tris)
+    terrain.set_margin(0.01)
+    w2 = engine.World(ctx)
+    w2.add_terrain(terrain)
+    w2.add_robot(engine.Robot.from_spec(ctx, sc.robot))
+    with pytest.raises(KlamptGPUError):
+        w2.feasible_batch(S.uniform_configs(sc.robot, 4, seed=1))
