@@ -113,7 +113,8 @@ def main():
     st = world.last_stats()
     feas_frac = float(dfeas.float().mean().item())
     print(f"[bench] {value:.4g} checks/s, {ms_per_step:.3f} ms/step, traversal kernel {kernel_s * 1e3:.4f} ms "
-          f"(avg of {kcnt}), feasible {feas_frac:.3f}, fp64 rechecks/step {st['fp64_rechecks']}",
+          f"(avg of {kcnt}), broad phase {st['t_prep_ns'] / 1e6:.4f} ms (last step), feasible {feas_frac:.3f}, "
+          f"fp64 rechecks/step {st['fp64_rechecks']}",
           file=sys.stderr, flush=True)
 
     nj = sc.robot.n

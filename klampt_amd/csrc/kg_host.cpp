@@ -446,6 +446,7 @@ struct Layout {
   size_t ncap = 0;                  // item capacity over the batch
   int cap = 0;                      // per-configuration capacity (single)
   int maxcap = 0;                   // largest per-configuration capacity
+  int maxlinks = 0;                 // largest link count over the plans
   Tol tol32{}, tol64{};
 };
 
@@ -493,12 +494,20 @@ void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* 
   if (fixed || L.multi) {
     need(fixed, "heterogeneous batch exceeds the item capacity limit (2 GiB of pair items)");
     hip_check(c->wsItems.ensure(std::max<size_t>(1, L.ncap) * sizeof(PairItem)), "hipMalloc(items)");
-    hip_check(c->wsRec.ensure(std::max<size_t>(1, L.ncap) * sizeof(ItemRec)), "hipMalloc(records)");
     pa.items = c->wsItems.as<PairItem>();
-    pa.recs = c->wsRec.as<ItemRec>();
     pa.cap = L.cap;
     pa.offsets = nullptr;
-    pa.lds_links = L.multi ? 0 : 1;
+    // robots of at most 32 links get the lane-per-link broad phase, which
+    // also writes the item start records; otherwise the traversal forms each
+    // item's frame when it starts it
+    const int ml = L.multi ? L.maxlinks : L.P->dev.nlinks;
+    pa.group = ml <= 8 ? 8 : (ml <= 16 ? 16 : (ml <= 32 ? 32 : 0));
+    if (env_int("KG_PREP_GROUP", -1) == 0) pa.group = 0;
+    pa.lds_links = !pa.group && !L.multi && ml <= 8;
+    if (pa.group) {
+      hip_check(c->wsRec.ensure(std::max<size_t>(1, L.ncap) * sizeof(ItemRec)), "hipMalloc(records)");
+      pa.recs = c->wsRec.as<ItemRec>();
+    }
     hip_check(kgl_prep_single(pa, s), "prep");
   } else {
     // large environments: count, scan, then fill exactly
@@ -511,9 +520,7 @@ void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* 
     hip_check(hipMemcpyAsync(&t, pa.offsets + N, sizeof(int), hipMemcpyDeviceToHost, s), "copy total");
     hip_check(hipStreamSynchronize(s), "sync");
     hip_check(c->wsItems.ensure(std::max<size_t>(1, (size_t)t) * sizeof(PairItem)), "hipMalloc(items)");
-    hip_check(c->wsRec.ensure(std::max<size_t>(1, (size_t)t) * sizeof(ItemRec)), "hipMalloc(records)");
     pa.items = c->wsItems.as<PairItem>();
-    pa.recs = c->wsRec.as<ItemRec>();
     hip_check(kgl_prep_fill(pa, s), "prep_fill");
   }
   hip_check(hipEventRecord(c->ev[1], s), "hipEventRecord");
@@ -521,7 +528,7 @@ void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* 
   // defaults measured on 65,536 configurations of the 7-DOF arm against the
   // 200k-triangle clutter: the 256-entry pool leaves LDS for 4 waves/SIMD
   const int scap_req = env_int("KG_SCAP", c->tune_scap ? c->tune_scap : 256);
-  const int scap = scap_req == 1024 ? 1024 : (scap_req == 256 ? 256 : 512);
+  const int scap = scap_req == 256 ? 256 : 512;
   const int wpc = env_int("KG_WAVES_PER_CU", c->tune_wpc);
   const int grid32 = wpc > 0 ? wpc * c->num_cus : kgl_collide_grid(0, c->num_cus, scap);
   const int grid64 = std::min(kgl_collide_grid(1, c->num_cus, scap), std::max(1, c->num_cus));
@@ -1992,6 +1999,7 @@ Layout multi_layout(kg_ctx_s* c, int nb, kg_world_s* const* worlds, const int32_
   L.nrows = rows;
   L.ncap = caps;
   L.maxcap = *std::max_element(caps_b.begin(), caps_b.end());
+  L.maxlinks = *std::max_element(n_b.begin(), n_b.end());
   L.tol32 = t32;
   L.tol64 = t64;
   (void)s;

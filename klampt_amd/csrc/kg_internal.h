@@ -122,8 +122,9 @@ struct PrepArgs {
   const int* qoff;
   const int* toff;
   const int* ibase;
-  int lds_links;       // single-plan batch: link boxes may live in LDS (no LB writes)
-  ItemRec* recs;       // nullable: item start records, written beside the items
+  int lds_links;       // serial single-plan variant with the link boxes in LDS
+  int group;           // > 0: one lane per link in groups of this many lanes (8, 16, 32)
+  ItemRec* recs;       // nullable: item start records (written by the per-link variant only)
   size_t t_rows;       // frame rows of T64 (structure of arrays, see t_row)
 };
 hipError_t kgl_prep_count(const PrepArgs& a, hipStream_t s);

@@ -399,63 +399,97 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
   if (MODE != 1) a.counts[c] = cnt;
 }
 
-// Broad phase with one lane per (configuration, link), for single-plan
-// batches of at most 8 links and a small environment (fixed-capacity item
-// layout).  Each lane computes its joint frame T0 J(q) in FP64; lane 0 of
-// the configuration's group composes the chain into world frames in LDS;
+// Broad phase with one lane per (configuration, link): groups of G lanes
+// (G = 8, 16 or 32, at least the robot's link count) take one configuration
+// each, of a single plan or of the configuration's own plan in a
+// heterogeneous batch (fixed-capacity item layout).  Each lane computes its
+// joint frame T0 J(q) in FP64; the group composes the world frames in LDS,
+// for G = 8 by its first lane down the link list (parents precede their
+// children), for longer robots by pointer jumping over the parent links
+// (ceil(log2 depth) rounds of all lanes);
 // each lane then forms its link box, and the group emits the items: a link's
 // environment pairs by its own lane, the self pairs round-robin over the
-// group.  Item positions inside a configuration come from a counter in LDS;
-// their order does not matter, since the ranks carry Klampt's loop order.
-#define KG_PL_GROUPS 32  // configurations per 256-thread block
-__global__ __launch_bounds__(256) void k_prep_links(PrepArgs a) {
-  __shared__ double sF[KG_PL_GROUPS][8][12];
-  __shared__ double sB[KG_PL_GROUPS][8][6];
-  __shared__ int scnt[KG_PL_GROUPS], sjl[KG_PL_GROUPS];
-  const int t = threadIdx.x, g = t >> 3, l = t & 7;
-  const int c = blockIdx.x * KG_PL_GROUPS + g;
+// group, each with its start record.  Item positions inside a configuration
+// come from a counter in LDS; their order does not matter, since the ranks
+// carry Klampt's loop order.  MULTI: per-configuration plans (the single
+// plan is then read from the kernel arguments with scalar loads).
+template <int G, bool MULTI>
+__global__ __launch_bounds__(256) void k_prep_group(PrepArgs a) {
+  constexpr int NG = 256 / G;  // configurations per block
+  __shared__ double sF[NG][G][12];
+  __shared__ double sB[NG][G][6];
+  __shared__ int sAnc[NG][G];
+  __shared__ int scnt[NG], sjl[NG];
+  const int t = threadIdx.x, g = t / G, l = t % G;
+  const int c = blockIdx.x * NG + g;
   {
     const int r = blockIdx.x * blockDim.x + t;
     if (r < a.nreset) a.reset[r] = 0;
   }
-  const PlanDev& P = a.plan;
-  const int n = P.nlinks;
   const bool valid = c < a.N;
-  const bool mine = valid && l < n;
+  const PlanDev& P = MULTI ? a.plans[valid ? a.cfg_plan[c] : 0] : a.plan;
+  const int n = valid ? P.nlinks : 0;
+  const bool mine = l < n;
   if (l == 0) {
     scnt[g] = 0;
     sjl[g] = 1;
   }
   __syncthreads();
   // joint frames
+  X64 X;
+  int anc = -1;
   if (mine) {
-    const double q = a.Qf ? (double)a.Qf[(size_t)c * n + l] : a.Qd[(size_t)c * n + l];
+    const size_t qb = MULTI ? (size_t)a.qoff[c] : (size_t)c * n;
+    const double q = a.Qf ? (double)a.Qf[qb + l] : a.Qd[qb + l];
     if (q < P.qmin[l] || q > P.qmax[l]) sjl[g] = 0;
     const LinkDesc& L = P.links[l];
-    X64 J, T0, Lx;
+    X64 J, T0;
     joint64(L.jtype, L.axis, q, J);
     x64_load(L.T0, T0);
-    x64_mul(T0, J, Lx);
-    x64_store(sF[g][l], Lx);
+    x64_mul(T0, J, X);
+    anc = L.parent;
+    x64_store(sF[g][l], X);
+    sAnc[g][l] = anc;
   }
-  __syncthreads();
-  // world frames: parents precede their children
-  if (valid && l == 0)
-    for (int k = 0; k < n; k++) {
-      const int p = P.links[k].parent;
-      if (p >= 0) {
-        X64 Pp, Lx, X;
-        x64_load(sF[g][p], Pp);
-        x64_load(sF[g][k], Lx);
-        x64_mul(Pp, Lx, X);
-        x64_store(sF[g][k], X);
+  if constexpr (G == 8) {
+    __syncthreads();
+    if (valid && l == 0)
+      for (int k = 0; k < n; k++) {
+        const int p = P.links[k].parent;
+        if (p >= 0) {
+          X64 Pp, Lx, W;
+          x64_load(sF[g][p], Pp);
+          x64_load(sF[g][k], Lx);
+          x64_mul(Pp, Lx, W);
+          x64_store(sF[g][k], W);
+        }
+      }
+    __syncthreads();
+    if (mine) x64_load(sF[g][l], X);
+  } else {
+    // world frames: after each round X is the product of the joint frames
+    // from the link `anc` (exclusive) down to this one, and `anc` is twice as
+    // far up
+    while (__syncthreads_or(anc >= 0)) {
+      X64 Y;
+      int up = -1;
+      if (anc >= 0) {
+        x64_load(sF[g][anc], Y);
+        up = sAnc[g][anc];
+      }
+      __syncthreads();
+      if (anc >= 0) {
+        X64 Z;
+        x64_mul(Y, X, Z);
+        X = Z;
+        anc = up;
+        x64_store(sF[g][l], X);
+        sAnc[g][l] = anc;
       }
     }
-  __syncthreads();
-  X64 X;
+  }
   if (mine) {
-    x64_load(sF[g][l], X);
-    t_store(a.T64, a.t_rows, t_row(nullptr, a.N, c, l), X);
+    t_store(a.T64, a.t_rows, MULTI ? (size_t)a.toff[c] + l : (size_t)l * a.N + c, X);
     const LinkDesc& L = P.links[l];
     if (L.geom >= 0) {
       const GeomDesc& gd = a.pool.geoms[L.geom];
@@ -467,6 +501,9 @@ __global__ __launch_bounds__(256) void k_prep_links(PrepArgs a) {
       }
     }
   }
+  // links with geometry, as a mask over the group's lanes
+  const uint64_t bal = __ballot(mine && P.links[l].geom >= 0);
+  const uint32_t active = (uint32_t)(bal >> ((t & 63) & ~(G - 1))) & (G == 32 ? 0xffffffffu : ((1u << G) - 1));
   __syncthreads();
   const bool jl = sjl[g] != 0;
   const bool go = valid && (a.mode != KG_MODE_ANY || jl);
@@ -474,10 +511,8 @@ __global__ __launch_bounds__(256) void k_prep_links(PrepArgs a) {
   if (go) {
     // union of the active links' boxes
     double ulo[3] = {1e300, 1e300, 1e300}, uhi[3] = {-1e300, -1e300, -1e300};
-    uint32_t active = 0;
     for (int k = 0; k < n; k++)
-      if (P.links[k].geom >= 0) {
-        active |= 1u << k;
+      if (active >> k & 1) {
         for (int r = 0; r < 3; r++) {
           ulo[r] = fmin(ulo[r], sB[g][k][r]);
           uhi[r] = fmax(uhi[r], sB[g][k][3 + r]);
@@ -507,11 +542,11 @@ __global__ __launch_bounds__(256) void k_prep_links(PrepArgs a) {
         }
     }
     const int nact = __popc(active), m1 = nact - __popc(lrej);
-    const int base = c * a.cap;
+    const int base = MULTI ? a.ibase[c] : c * a.cap;
     auto env_rej = [&](int i) { return !box_hit(P.env[i].lo, P.env[i].hi, ulo, uhi); };
     auto link_rej = [&](int i) { return (lrej >> i & 1) != 0; };
     // this link's environment pairs
-    if (l < n && m2 > 0 && (active >> l & 1)) {
+    if (mine && m2 > 0 && (active >> l & 1)) {
       const int pos = __popc(active & ((1u << l) - 1));
       if (!(lrej >> pos & 1)) {
         const int p1 = ordered ? swap_erase_pos(pos, nact, m1, link_rej) : 0;
@@ -534,7 +569,7 @@ __global__ __launch_bounds__(256) void k_prep_links(PrepArgs a) {
       }
     }
     // self pairs, round-robin over the group
-    for (int k = l; k < P.nself; k += 8) {
+    for (int k = l; k < P.nself; k += G) {
       const int i = P.selfpairs[2 * k], j = P.selfpairs[2 * k + 1];
       if (!box_hit(sB[g][i], sB[g][i] + 3, sB[g][j], sB[g][j] + 3)) continue;
       const int idx = base + atomicAdd(&scnt[g], 1);
@@ -882,17 +917,20 @@ __device__ inline void item_frame(const TravArgs& a, const PairItem& P, Real* re
   const PlanDev& PL = a.cfg_plan ? a.plans[a.cfg_plan[P.cfg]] : a.plan;
   const LinkDesc& LA = PL.links[P.a];
   const int g = (P.rank >> KG_RANK_GROUP_BIT) & 1;
+  // both frames are copied into registers: a pointer that may select a
+  // private array would place that array in scratch memory
   X64 FA, FB;
   t_load(a.T64, a.t_rows, t_row(a.toff, a.t_n, P.cfg, P.a), FA);
   const double* XA = FA.R;
-  const double* XB;
+  const double* XB = FB.R;
   int geomB;
   if (g == 0) {
-    XB = PL.env[P.b].X;
+    const double* XE = PL.env[P.b].X;
+#pragma unroll
+    for (int k = 0; k < 12; k++) FB.R[k] = XE[k];
     geomB = PL.env[P.b].geom;
   } else {
     t_load(a.T64, a.t_rows, t_row(a.toff, a.t_n, P.cfg, P.b), FB);
-    XB = FB.R;
     geomB = PL.links[P.b].geom;
   }
   const double d0 = XA[9] - XB[9], d1 = XA[10] - XB[10], d2 = XA[11] - XB[11];
@@ -947,8 +985,9 @@ __device__ inline uint64_t mk_leaf(int slot, int ta, int tb) {
 // and writes a configuration's result as soon as its last item finishes, so
 // a heavy item never holds the wave's other lanes idle at a chunk boundary.
 // FPAIR: the first-colliding-pair output is requested (ordered mode).
-template <typename Real, bool EXACT, int SCAP, bool FPAIR>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SCAP == 256 && !EXACT ? 4 : 1))) void k_collide(
+// REC: item start records exist (else each item's frame is formed here).
+template <typename Real, bool EXACT, int SCAP, bool FPAIR, bool REC>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SCAP == 256 && REC && !FPAIR ? 4 : 1))) void k_collide(
     TravArgs a) {
   // the small-pool variant also halves the leaf queue and the ring so that
   // four waves fit per SIMD
@@ -1200,7 +1239,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SCAP == 256 
       atomicAdd(&e_open[myent], 1);
       const PairItem P = a.items[myitem];
       int rootA, rootB;
-      if (!EXACT) {
+      if (!EXACT && REC) {
         const float4* q = reinterpret_cast<const float4*>(a.recs + myitem);
         const float4 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
         s_rel[0][myslot] = q0.x; s_rel[1][myslot] = q0.y; s_rel[2][myslot] = q0.z; s_rel[3][myslot] = q0.w;
@@ -1381,32 +1420,45 @@ hipError_t kgl_prep_fill(const PrepArgs& a, hipStream_t s) {
 hipError_t kgl_prep_single(const PrepArgs& a, hipStream_t s) {
   const int n = a.N > a.nreset ? a.N : a.nreset;
   if (n <= 0) return hipSuccess;
-  if (a.lds_links && a.plan.nlinks <= 8 && a.plan.nenv <= 64) {
-    const int blocks = max((a.N + KG_PL_GROUPS - 1) / KG_PL_GROUPS, (a.nreset + 255) / 256);
-    k_prep_links<<<blocks, 256, 0, s>>>(a);
-  } else if (a.lds_links && a.plan.nlinks <= KG_PREP_LDS_LINKS)
+  if (a.group > 0) {
+    const int ng = 256 / a.group;
+    const int blocks = max((a.N + ng - 1) / ng, (a.nreset + 255) / 256);
+    const bool m = a.cfg_plan != nullptr;
+    if (a.group == 8)
+      m ? k_prep_group<8, true><<<blocks, 256, 0, s>>>(a) : k_prep_group<8, false><<<blocks, 256, 0, s>>>(a);
+    else if (a.group == 16)
+      m ? k_prep_group<16, true><<<blocks, 256, 0, s>>>(a) : k_prep_group<16, false><<<blocks, 256, 0, s>>>(a);
+    else
+      m ? k_prep_group<32, true><<<blocks, 256, 0, s>>>(a) : k_prep_group<32, false><<<blocks, 256, 0, s>>>(a);
+  } else if (a.lds_links)
     k_prep<2, KG_PREP_LDS_LINKS><<<(n + KG_PREP_LDS_BLOCK - 1) / KG_PREP_LDS_BLOCK, KG_PREP_LDS_BLOCK, 0, s>>>(a);
   else
     k_prep<2, 0><<<(n + 255) / 256, 256, 0, s>>>(a);
   return hipGetLastError();
 }
 
+template <typename Real, bool EXACT, int SCAP, bool FPAIR>
+static void launch_rec(const TravArgs& a, hipStream_t s) {
+  if (!EXACT && a.recs)
+    k_collide<Real, EXACT, SCAP, FPAIR, !EXACT><<<a.grid, 64, 0, s>>>(a);
+  else
+    k_collide<Real, EXACT, SCAP, FPAIR, false><<<a.grid, 64, 0, s>>>(a);
+}
+
 template <typename Real, bool EXACT, int SCAP>
 static void launch_scap(const TravArgs& a, hipStream_t s) {
   if (a.fin.first_pair)
-    k_collide<Real, EXACT, SCAP, true><<<a.grid, 64, 0, s>>>(a);
+    launch_rec<Real, EXACT, SCAP, true>(a, s);
   else
-    k_collide<Real, EXACT, SCAP, false><<<a.grid, 64, 0, s>>>(a);
+    launch_rec<Real, EXACT, SCAP, false>(a, s);
 }
 
 template <typename Real, bool EXACT>
 static hipError_t launch_collide(const TravArgs& a, hipStream_t s) {
   if (a.scap == 256)
     launch_scap<Real, EXACT, 256>(a, s);
-  else if (a.scap == 512)
-    launch_scap<Real, EXACT, 512>(a, s);
   else
-    launch_scap<Real, EXACT, 1024>(a, s);
+    launch_scap<Real, EXACT, 512>(a, s);
   return hipGetLastError();
 }
 
@@ -1424,14 +1476,11 @@ int kgl_collide_grid(int exact, int num_cus, int scap) {
   int nb = 0;
   hipError_t e;
   if (scap == 256)
-    e = exact ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_collide<double, true, 256, false>, 64, 0)
-              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_collide<float, false, 256, false>, 64, 0);
-  else if (scap == 512)
-    e = exact ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_collide<double, true, 512, false>, 64, 0)
-              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_collide<float, false, 512, false>, 64, 0);
+    e = exact ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_collide<double, true, 256, false, false>, 64, 0)
+              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_collide<float, false, 256, false, true>, 64, 0);
   else
-    e = exact ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_collide<double, true, 1024, false>, 64, 0)
-              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_collide<float, false, 1024, false>, 64, 0);
+    e = exact ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_collide<double, true, 512, false, false>, 64, 0)
+              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_collide<float, false, 512, false, true>, 64, 0);
   if (e != hipSuccess || nb < 1) nb = 8;
   const char* v = getenv("KG_WAVES_PER_CU");
   if (v && atoi(v) > 0) nb = atoi(v);
