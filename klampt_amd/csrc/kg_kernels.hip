@@ -639,21 +639,24 @@ __device__ inline bool sat_axis(V3T<Real> a, Real aerr, const V3T<Real>* P, cons
 }
 
 // Closed triangle/triangle test by 17 separating axes (2 normals, 9 edge
-// crosses, 6 in-plane edge normals), in coordinates shifted to q[0].
-// EXACT (FP64): TT_SEP / TT_HIT.  Otherwise TT_UNSURE when the decision is
-// within the error bound: tau bounds the vertex position error (x2, both
-// triangles) plus projection rounding; aerr bounds each axis' own rounding.
+// crosses, 6 in-plane edge normals), in coordinates shifted to q[0], spread
+// over the 9 lanes sub = 0..8 of a lane group: every lane tests the edge
+// cross eP(sub / 3) x eQ(sub % 3), then lanes 0..5 one in-plane edge normal
+// each (nP x eP(sub), nQ x eQ(sub - 3)) and lanes 6, 7 the normals nP, nQ.
+// Returns bit 0 when one of the lane's axes certainly separates and bit 1
+// when one of them does not certainly overlap; the pair is separated if any
+// lane of the group sets bit 0, else undecided (FP32) if any sets bit 1,
+// else intersecting.  tau bounds the vertex position error (x2, both
+// triangles) plus projection rounding; each axis' own rounding is bounded
+// separately.  EXACT (FP64) sets bit 0 only.
 template <typename Real, bool EXACT>
-__device__ int tri_tri_sat(const V3T<Real>* p, const V3T<Real>* q, Real tau) {
+__device__ int tri_tri_split(const V3T<Real>* p, const V3T<Real>* q, Real tau, int sub) {
   V3T<Real> P[3], Q[3];
 #pragma unroll
   for (int i = 0; i < 3; i++) {
     P[i] = vsub(p[i], q[0]);
     Q[i] = vsub(q[i], q[0]);
   }
-  // edges are re-formed from the vertices where needed (fewer live registers)
-  auto eP = [&](int i) { return vsub(P[i == 2 ? 0 : i + 1], P[i]); };
-  auto eQ = [&](int i) { return vsub(Q[i == 2 ? 0 : i + 1], Q[i]); };
   const Real u = EXACT ? (Real)0 : (Real)5.9604645e-08;
   Real RD = 0;
   if (!EXACT) {
@@ -662,30 +665,36 @@ __device__ int tri_tri_sat(const V3T<Real>* p, const V3T<Real>* q, Real tau) {
     RD *= (Real)2;
     tau += (Real)8 * u * RD;
   }
+  const V3T<Real> eP0 = vsub(P[1], P[0]), eP1 = vsub(P[2], P[1]), eP2 = vsub(P[0], P[2]);
+  const V3T<Real> eQ0 = vsub(Q[1], Q[0]), eQ1 = vsub(Q[2], Q[1]), eQ2 = vsub(Q[0], Q[2]);
+  const Real lP0 = EXACT ? 0 : vl1(eP0), lP1 = EXACT ? 0 : vl1(eP1), lP2 = EXACT ? 0 : vl1(eP2);
+  const Real lQ0 = EXACT ? 0 : vl1(eQ0), lQ1 = EXACT ? 0 : vl1(eQ1), lQ2 = EXACT ? 0 : vl1(eQ2);
+  auto pick = [](int k, V3T<Real> a0, V3T<Real> a1, V3T<Real> a2) { return k == 0 ? a0 : (k == 1 ? a1 : a2); };
+  auto pickr = [](int k, Real a0, Real a1, Real a2) { return k == 0 ? a0 : (k == 1 ? a1 : a2); };
   bool sure = true;
-  const V3T<Real> nP = vcross(eP(0), eP(1));
-  const V3T<Real> nQ = vcross(eQ(0), eQ(1));
-  const Real lP0 = EXACT ? 0 : vl1(eP(0)), lP1 = EXACT ? 0 : vl1(eP(1)), lP2 = EXACT ? 0 : vl1(eP(2));
-  const Real lQ0 = EXACT ? 0 : vl1(eQ(0)), lQ1 = EXACT ? 0 : vl1(eQ(1)), lQ2 = EXACT ? 0 : vl1(eQ(2));
-  auto lP = [&](int i) { return i == 0 ? lP0 : (i == 1 ? lP1 : lP2); };
-  auto lQ = [&](int i) { return i == 0 ? lQ0 : (i == 1 ? lQ1 : lQ2); };
-  const Real enP = (Real)8 * u * lP0 * lP1, enQ = (Real)8 * u * lQ0 * lQ1;
-  if (sat_axis<Real, EXACT>(nP, enP, P, Q, tau, RD, sure)) return TT_SEP;
-  if (sat_axis<Real, EXACT>(nQ, enQ, P, Q, tau, RD, sure)) return TT_SEP;
-#pragma unroll
-  for (int i = 0; i < 3; i++)
-#pragma unroll
-    for (int j = 0; j < 3; j++)
-      if (sat_axis<Real, EXACT>(vcross(eP(i), eQ(j)), (Real)8 * u * lP(i) * lQ(j), P, Q, tau, RD, sure))
-        return TT_SEP;
-#pragma unroll
-  for (int i = 0; i < 3; i++) {
-    if (sat_axis<Real, EXACT>(vcross(nP, eP(i)), (Real)16 * u * lP0 * lP1 * lP(i), P, Q, tau, RD, sure))
-      return TT_SEP;
-    if (sat_axis<Real, EXACT>(vcross(nQ, eQ(i)), (Real)16 * u * lQ0 * lQ1 * lQ(i), P, Q, tau, RD, sure))
-      return TT_SEP;
+  int out = 0;
+  {
+    const int i = sub / 3, j = sub - 3 * (sub / 3);
+    const V3T<Real> a = vcross(pick(i, eP0, eP1, eP2), pick(j, eQ0, eQ1, eQ2));
+    if (sat_axis<Real, EXACT>(a, (Real)8 * u * pickr(i, lP0, lP1, lP2) * pickr(j, lQ0, lQ1, lQ2), P, Q, tau, RD, sure))
+      out = 1;
   }
-  return (EXACT || sure) ? TT_HIT : TT_UNSURE;
+  if (sub < 8) {
+    const V3T<Real> nP = vcross(eP0, eP1), nQ = vcross(eQ0, eQ1);
+    const bool onP = sub < 3 || sub == 6;
+    const int k = sub < 3 ? sub : sub - 3;
+    V3T<Real> a;
+    Real err;
+    if (sub < 6) {
+      a = onP ? vcross(nP, pick(k, eP0, eP1, eP2)) : vcross(nQ, pick(k, eQ0, eQ1, eQ2));
+      err = onP ? (Real)16 * u * lP0 * lP1 * pickr(k, lP0, lP1, lP2) : (Real)16 * u * lQ0 * lQ1 * pickr(k, lQ0, lQ1, lQ2);
+    } else {
+      a = onP ? nP : nQ;
+      err = onP ? (Real)8 * u * lP0 * lP1 : (Real)8 * u * lQ0 * lQ1;
+    }
+    if (sat_axis<Real, EXACT>(a, err, P, Q, tau, RD, sure)) out = 1;
+  }
+  return out | (sure ? 0 : 2);
 }
 
 template <typename Real>
@@ -1089,34 +1098,53 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SCAP == 256 
     __syncthreads();
   };
 
+  // triangle tests of queued leaf pairs: lane groups of 9 (7 pairs per pass,
+  // at most 3 passes per call) split each pair's axes between them
   auto run_leaves = [&]() {
-    const int m = min(nl, 64);
-    if (lane < m) {
-      const uint64_t e = lq[nl - 1 - lane];
-      const int slot = (int)(e >> 58);
-      const int ta = (int)((e >> 29) & 0x1fffffff), tb = (int)(e & 0x1fffffff);
-      const int rk = s_rank[slot];
-      const int g = (rk >> KG_RANK_GROUP_BIT) & 1, r = rk & ((1 << KG_RANK_GROUP_BIT) - 1);
-      const int ck = s_cfgk[slot];
-      if (!item_dead(mode, s_best[ck], g, r)) {
-        n_tri++;
-        Real rel[12];
+    const int m = min(nl, 21);
+    const int li = lane / 9, sub = lane - 9 * li;
+    for (int b = 0; b < m; b += 7) {
+      const int idx = b + li;
+      const bool act = li < 7 && idx < m;
+      int slot = 0, g = 0, r = 0, ck = 0, bits = 0;
+      bool live = false;
+      if (act) {
+        const uint64_t e = lq[nl - 1 - idx];
+        slot = (int)(e >> 58);
+        const int ta = (int)((e >> 29) & 0x1fffffff), tb = (int)(e & 0x1fffffff);
+        const int rk = s_rank[slot];
+        g = (rk >> KG_RANK_GROUP_BIT) & 1;
+        r = rk & ((1 << KG_RANK_GROUP_BIT) - 1);
+        ck = s_cfgk[slot];
+        live = !item_dead(mode, s_best[ck], g, r);
+        if (live) {
+          Real rel[12];
 #pragma unroll
-        for (int k = 0; k < 12; k++) rel[k] = s_rel[k][slot];
-        V3T<Real> tp[3], tq[3];
-        load_tri<Real>(a.pool, ta, tp);
-        load_tri<Real>(a.pool, tb, tq);
+          for (int k = 0; k < 12; k++) rel[k] = s_rel[k][slot];
+          V3T<Real> tp[3], tq[3];
+          load_tri<Real>(a.pool, ta, tp);
+          load_tri<Real>(a.pool, tb, tq);
 #pragma unroll
-        for (int v = 0; v < 3; v++) tp[v] = xf_apply<Real>(rel, tp[v]);
-        const int res = tri_tri_sat<Real, EXACT>(tp, tq, (Real)s_tau[slot]);
-        if (res == TT_HIT) {
-          const unsigned long long key = ((unsigned long long)(uint32_t)r << 32) | (uint32_t)s_item[slot];
-          atomicMin(&s_best[ck][g], key);
-        } else if (res == TT_UNSURE) {
-          s_unc[slot] = 1;
+          for (int v = 0; v < 3; v++) tp[v] = xf_apply<Real>(rel, tp[v]);
+          bits = tri_tri_split<Real, EXACT>(tp, tq, (Real)s_tau[slot], sub);
         }
       }
-      settle(slot, -1);
+      const uint64_t sepm = ballot64((bits & 1) != 0), unsm = ballot64((bits & 2) != 0);
+      if (act && sub == 0) {
+        if (live) {
+          n_tri++;
+          const uint64_t gm = 0x1ffull << (9 * li);
+          if (!(sepm & gm)) {
+            if (unsm & gm) {
+              s_unc[slot] = 1;
+            } else {
+              const unsigned long long key = ((unsigned long long)(uint32_t)r << 32) | (uint32_t)s_item[slot];
+              atomicMin(&s_best[ck][g], key);
+            }
+          }
+        }
+        settle(slot, -1);
+      }
     }
     nl -= m;
     __syncthreads();
