@@ -4,7 +4,7 @@
 // Collision traversal (k_collide) runs one wavefront per workgroup over
 // narrow-phase pair items (a link mesh against an environment mesh or another
 // link), walking the two OBB trees: each step takes a node pair whose boxes
-// overlap, tests both children of the larger node against the other (15-axis
+// overlap, tests both children of the larger-volume node against the other (15-axis
 // separating-axis test), continues depth first with one overlapping child
 // pair, pools the other, and queues leaf pairs for the 17-axis triangle test.
 // A certain hit kills the remaining work of its configuration (any-hit), of
@@ -1364,7 +1364,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SCAP == 256 
             nlf++;
             continue;
           }
-          const bool splitS = lO || (!lS && (S.e[0] + S.e[1] + S.e[2]) >= (O.e[0] + O.e[1] + O.e[2]));
+          // descend the box of larger volume (fewer box tests than larger
+          // extents on thin and elongated boxes)
+          const bool splitS = lO || (!lS && S.e[0] * S.e[1] * S.e[2] >= O.e[0] * O.e[1] * O.e[2]);
           // side 0: children of an A-node against a B-node; side 1: the reverse
           const int nside = (splitS == aIsS) ? 0 : 1;
           const uint64_t w = splitS ? mk_task(slot, nside, 0, S.ch, y) : mk_task(slot, nside, 0, O.ch, nodeS);
