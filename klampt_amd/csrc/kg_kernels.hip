@@ -771,8 +771,11 @@ __device__ inline void task_frame(const NodeF& O, const Real* rel, TaskFrame<Rea
 
 // 15-axis separating-axis test of child box S against O in the task frame;
 // every right-hand side padded by pad_abs + pad_rel * (sum of extents).
+// fmargin: the smallest slack over the six face axes (how deeply the boxes
+// overlap), used to order the descent.
 template <typename Real>
-__device__ inline bool frame_overlap(const TaskFrame<Real>& F, const NodeF& S, Real pad_abs, Real pad_rel) {
+__device__ inline bool frame_overlap(const TaskFrame<Real>& F, const NodeF& S, Real pad_abs, Real pad_rel,
+                                     Real& fmargin) {
   Real C[3][3], Qa[3][3], T[3], eA[3];
 #pragma unroll
   for (int j = 0; j < 3; j++) eA[j] = (Real)S.e[j];
@@ -788,13 +791,21 @@ __device__ inline bool frame_overlap(const TaskFrame<Real>& F, const NodeF& S, R
   const Real* eB = F.eO;
   const Real pad = pad_abs + pad_rel * (eA[0] + eA[1] + eA[2] + F.sumO);
   bool sep = false;
+  Real fm = (Real)3.0e38;
 #pragma unroll
-  for (int i = 0; i < 3; i++) sep |= fabs(T[i]) > eB[i] + eA[0] * Qa[i][0] + eA[1] * Qa[i][1] + eA[2] * Qa[i][2] + pad;
+  for (int i = 0; i < 3; i++) {
+    const Real rhs = eB[i] + eA[0] * Qa[i][0] + eA[1] * Qa[i][1] + eA[2] * Qa[i][2] + pad;
+    sep |= fabs(T[i]) > rhs;
+    fm = fmin(fm, rhs - fabs(T[i]));
+  }
 #pragma unroll
   for (int j = 0; j < 3; j++) {
     const Real t = T[0] * C[0][j] + T[1] * C[1][j] + T[2] * C[2][j];
-    sep |= fabs(t) > eB[0] * Qa[0][j] + eB[1] * Qa[1][j] + eB[2] * Qa[2][j] + eA[j] + pad;
+    const Real rhs = eB[0] * Qa[0][j] + eB[1] * Qa[1][j] + eB[2] * Qa[2][j] + eA[j] + pad;
+    sep |= fabs(t) > rhs;
+    fm = fmin(fm, rhs - fabs(t));
   }
+  fmargin = fm;
 #pragma unroll
   for (int i = 0; i < 3; i++) {
     const int i1 = (i + 1) % 3, i2 = (i + 2) % 3;
@@ -1350,13 +1361,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SCAP == 256 
         task_frame<Real>(O, rel, F);
         const Real pad = (Real)s_pad[slot];
         const bool lO = O.ch < 0;
+        Real m0 = 0;
 #pragma unroll
         for (int k = 0; k < 2; k++) {
           if (k == 1 && single) break;
           const NodeF& S = k == 0 ? S0 : S1;  // k is a constant after unrolling
           const int nodeS = x + k;
           n_box++;
-          if (!frame_overlap<Real>(F, S, pad, pad_rel)) continue;
+          Real fm;
+          if (!frame_overlap<Real>(F, S, pad, pad_rel, fm)) continue;
           const bool lS = S.ch < 0;
           if (lS && lO) {
             const uint64_t w = aIsS ? mk_leaf(slot, ~S.ch, ~O.ch) : mk_leaf(slot, ~O.ch, ~S.ch);
@@ -1370,7 +1383,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SCAP == 256 
           // side 0: children of an A-node against a B-node; side 1: the reverse
           const int nside = (splitS == aIsS) ? 0 : 1;
           const uint64_t w = splitS ? mk_task(slot, nside, 0, S.ch, y) : mk_task(slot, nside, 0, O.ch, nodeS);
-          if (nout == 0) out0 = w; else out1 = w;
+          // the more deeply overlapping pair is continued (an intersection,
+          // which ends the item, tends to be found sooner), the other pooled
+          if (nout == 0) {
+            out0 = w;
+            m0 = fm;
+          } else if (fm > m0) {
+            out1 = out0;
+            out0 = w;
+          } else {
+            out1 = w;
+          }
           nout++;
         }
       }
