@@ -125,7 +125,7 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                 "traffic_source": traffic_src,
-                "kernel": "k_collide<float, false, 512>", "kernel_ms": round(kernel_s * 1e3, 4),
+                "kernel": "k_collide<float, false, 256, false, true>", "kernel_ms": round(kernel_s * 1e3, 4),
                 "kernel_launches_timed": kcnt, "algorithmic_bytes": int(alg_bytes),
                 "box_tests": int(st["box_tests"]), "tri_tests": int(st["tri_tests"]),
                 "pair_items": int(st["pair_items"]), "fp64_rechecks": int(st["fp64_rechecks"])}
