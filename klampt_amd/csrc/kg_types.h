@@ -64,8 +64,9 @@ struct PairItem {
 };
 
 // What the FP32 traversal needs to start a pair item, computed once per item
-// by k_items: the item transform (mesh A frame -> mesh B frame, FP64
-// product rounded to FP32), its error bounds and the two BVH roots.
+// by the per-link broad phase: the item transform (mesh A frame -> mesh B
+// frame, FP64 product rounded to FP32), its error bounds and the two BVH
+// roots.
 struct __attribute__((aligned(16))) ItemRec {
   float rel[12];  // column-major R, then t
   float tau, pad;
