@@ -561,6 +561,8 @@ void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* 
   ta.status = status + ST_SPILL;
   ta.cpw = std::max(1, std::min(64, env_int("KG_CPW", c->tune_cpw ? c->tune_cpw : 16)));
   ta.leaf_min = std::max(1, std::min(64, env_int("KG_LEAF_MIN", c->tune_leaf ? c->tune_leaf : 1)));
+  ta.tail_lanes = std::max(0, std::min(64, env_int("KG_TAIL_LANES", 16)));
+  ta.tail_reps = std::max(1, std::min(8, env_int("KG_TAIL_REPS", 4)));
   ta.grid = grid32;
   ta.counters = (unsigned long long*)(status + ST_CNT);
   ta.plans = L.plans;

@@ -168,6 +168,8 @@ struct TravArgs {
   int* status;                // [0]: spill overflow flag
   int cpw;                    // configs per chunk
   int leaf_min;               // lanes holding leaf pairs that trigger a triangle-test step
+  int tail_lanes;             // fewer working lanes than this: tail cadence
+  int tail_reps;              // iterations per item-start step in the tail
   int stack_cap;              // per-lane traversal stack entries (LDS)
   int grid;                   // number of waves launched
   unsigned long long* counters;  // [0] box tests, [1] triangle tests, [2] items (nullable)

@@ -1161,12 +1161,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SCAP == 256 
     __syncthreads();
   };
 
+  // In a wave's tail (few lanes with work) the item-start and publishing
+  // steps run only every tail_reps-th iteration: the iterations between
+  // them pop the pool and test boxes, so the deep items that remain descend
+  // several levels per scheduling step.
   bool quit = false;
+  int rep = 0, reps = 1;
   for (;;) {
-    // publish finished configurations and free their entries (the only
-    // place results are written)
-    finalize_done();
-    if (quit) break;
+    const bool full = rep == 0;
+    if (full) {
+      // publish finished configurations and free their entries (the only
+      // place results are written)
+      finalize_done();
+      if (quit) break;
+    }
     // ---- lanes without a continuation pop the pool (bring spilled tasks
     // back first when it runs low)
     const bool hungry = cont == KG_NOTASK;
@@ -1185,6 +1193,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SCAP == 256 
     const int npop = min(nh, sp);
     if (hungry && hp < npop) cont = stk[sp - 1 - hp];
     sp -= npop;
+    if (full) {
     // ---- lanes still without work start new items on free slots
     const bool want = cont == KG_NOTASK;
     const uint64_t wm = ballot64(want);
@@ -1310,10 +1319,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SCAP == 256 
     if (lane == 0)
       for (int e = rel_from; e < fe; e++) settle_entry(e % NE, -1);
     __syncthreads();
+    }
 
     // ---- termination: no path, nothing pooled, no leaves, nothing to start
     const bool working = cont != KG_NOTASK;
-    if (!ballot64(working)) {
+    const uint64_t wk = ballot64(working);
+    if (full) reps = __popcll(wk) < a.tail_lanes ? a.tail_reps : 1;
+    rep = rep + 1 < reps ? rep + 1 : 0;
+    if (!wk) {
+      if (!full) {
+        rep = 0;
+        continue;
+      }
       if (sp == 0 && gsp == 0) {
         if (nl > 0) {
           run_leaves();
