@@ -226,14 +226,21 @@ int kg_ctx_set_tuning(kg_ctx ctx, int cpw, int leaf_min, int scap, int waves_per
 /* Counters of the last batched call on this world: [0] configurations
  * re-checked in FP64, [1] narrow-phase pair items, [2] spill events,
  * [3] broad-phase time (ns), [4] FP32 traversal kernel time (ns), [5] FP64
- * re-check kernel time (ns) -- HIP events on the launch stream -- [6] box
- * tests, [7] triangle tests of the FP32 traversal. */
+ * re-check kernel time (ns) -- HIP events on the launch stream; [3] and [5]
+ * need timing level 2, [4] level 1 or 2 (kg_ctx_set_timing), else 0 -- [6]
+ * box tests, [7] triangle tests of the FP32 traversal. */
 int kg_world_last_stats(kg_world w, int64_t* stats, int nstats);
 /* Sum of the FP32 traversal kernel durations (HIP events on the launch
  * stream) of the feasibility batches issued on this context since the last
  * reset, at most the 64 most recent; count receives how many were summed.
  * Waits for those batches.  No Klampt counterpart (measurement). */
 int kg_ctx_kernel_times(kg_ctx ctx, int reset, int64_t* total_ns, int32_t* count);
+/* Which HIP events the batched calls of this context record on their stream
+ * (each costs a few microseconds of stream time): 0 none, 1 (default) the
+ * FP32 traversal kernel's begin/end (kg_ctx_kernel_times, stats [4]), 2 also
+ * the broad phase and the FP64 pass (stats [3], [5]).  No Klampt
+ * counterpart (measurement). */
+int kg_ctx_set_timing(kg_ctx ctx, int level);
 
 #ifdef __cplusplus
 }

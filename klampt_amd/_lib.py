@@ -90,6 +90,7 @@ SIGNATURES = {
     "kg_autotune": [_c_int, _pvp, _pi, _pi, _vp, _pi, _pf],
     "kg_ctx_set_tuning": [_vp, _c_int, _c_int, _c_int, _c_int],
     "kg_ctx_kernel_times": [_vp, _c_int, _pi64, _pi],
+    "kg_ctx_set_timing": [_vp, _c_int],
 }
 _RESTYPE = {"kg_last_error": ctypes.c_char_p}
 

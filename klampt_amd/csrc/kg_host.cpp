@@ -490,7 +490,12 @@ void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* 
   pa.toff = L.toff;
   pa.ibase = L.ibase;
   const bool fixed = L.ncap * sizeof(PairItem) <= (2048ull << 20);
-  hip_check(hipEventRecord(c->ev[0], s), "hipEventRecord");
+  // timing events cost a few microseconds each on the stream: per-phase ones
+  // only at timing level 2, the traversal kernel's own at level 1 and up
+  const bool phase_ev = c->timing >= 2, kern_ev = c->timing >= 1;
+  c->last_phase_ev = phase_ev;
+  c->last_tr_slot = -1;
+  if (phase_ev) hip_check(hipEventRecord(c->ev[0], s), "hipEventRecord");
   if (fixed || L.multi) {
     need(fixed, "heterogeneous batch exceeds the item capacity limit (2 GiB of pair items)");
     hip_check(c->wsItems.ensure(std::max<size_t>(1, L.ncap) * sizeof(PairItem)), "hipMalloc(items)");
@@ -523,7 +528,7 @@ void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* 
     pa.items = c->wsItems.as<PairItem>();
     hip_check(kgl_prep_fill(pa, s), "prep_fill");
   }
-  hip_check(hipEventRecord(c->ev[1], s), "hipEventRecord");
+  if (phase_ev) hip_check(hipEventRecord(c->ev[1], s), "hipEventRecord");
 
   // defaults measured on 65,536 configurations of the 7-DOF arm against the
   // 200k-triangle clutter: the 256-entry pool leaves LDS for 4 waves/SIMD
@@ -590,12 +595,13 @@ void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* 
   fa.redo_count = status + ST_REDO;
   fa.final_pass = 0;
   const int slot = (int)(c->tr_count % kg_ctx_s::TRING);
-  hip_check(hipEventRecord(c->ev[2], s), "hipEventRecord");
-  hip_check(hipEventRecord(c->tr_beg[slot], s), "hipEventRecord");
+  if (kern_ev) hip_check(hipEventRecord(c->tr_beg[slot], s), "hipEventRecord");
   hip_check(kgl_collide_f32(ta, s), "collide_f32");
-  hip_check(hipEventRecord(c->tr_end[slot], s), "hipEventRecord");
-  hip_check(hipEventRecord(c->ev[3], s), "hipEventRecord");
-  c->tr_count++;
+  if (kern_ev) {
+    hip_check(hipEventRecord(c->tr_end[slot], s), "hipEventRecord");
+    c->last_tr_slot = slot;
+    c->tr_count++;
+  }
 
   // FP64 re-check of the configurations the FP32 pass could not certify;
   // the count stays on the device (waves exit at once when it is zero)
@@ -610,9 +616,9 @@ void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* 
   tb.static_pct = 0;
   tb.nsub = 1;
   tb.fin.final_pass = 1;
-  hip_check(hipEventRecord(c->ev[4], s), "hipEventRecord");
+  if (phase_ev) hip_check(hipEventRecord(c->ev[4], s), "hipEventRecord");
   hip_check(kgl_collide_f64(tb, s), "collide_f64");
-  hip_check(hipEventRecord(c->ev[5], s), "hipEventRecord");
+  if (phase_ev) hip_check(hipEventRecord(c->ev[5], s), "hipEventRecord");
   // the status words stay on the device until someone asks (fetch_status)
   c->stat_src = status;
   hip_check(hipEventRecord(c->evStat, s), "hipEventRecord");
@@ -670,9 +676,26 @@ void read_stats(kg_world_s* w, BatchStats& st) {
   st.tri_tests = (int64_t)cnt[1];
   st.items = (int64_t)cnt[2];
   float ms = 0;
-  if (hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) st.t_prep_ns = (int64_t)(ms * 1e6);
-  if (hipEventElapsedTime(&ms, c->ev[2], c->ev[3]) == hipSuccess) st.t_trav32_ns = (int64_t)(ms * 1e6);
-  if (hipEventElapsedTime(&ms, c->ev[4], c->ev[5]) == hipSuccess) st.t_trav64_ns = (int64_t)(ms * 1e6);
+  if (c->last_tr_slot >= 0) {
+    if (hipEventElapsedTime(&ms, c->tr_beg[c->last_tr_slot], c->tr_end[c->last_tr_slot]) == hipSuccess)
+      st.t_trav32_ns = (int64_t)(ms * 1e6);
+    else
+      (void)hipGetLastError();
+  }
+  if (!c->last_phase_ev) return;
+  // feasibility batches record ev[0,1,4,5]; distance batches ev[2,3]
+  auto span = [&](hipEvent_t a, hipEvent_t b, int64_t& out) {
+    if (hipEventElapsedTime(&ms, a, b) == hipSuccess)
+      out = (int64_t)(ms * 1e6);
+    else
+      (void)hipGetLastError();  // leave no error for a later launch check
+  };
+  if (c->last_tr_slot >= 0) {
+    span(c->ev[0], c->ev[1], st.t_prep_ns);
+    span(c->ev[4], c->ev[5], st.t_trav64_ns);
+  } else {
+    span(c->ev[2], c->ev[3], st.t_trav32_ns);
+  }
 }
 
 // Synchronous wrapper for internal single queries: errors of the FP64 pass
@@ -1655,9 +1678,12 @@ void distance_dev(kg_world_s* w, int ri, const float* Qf, const double* Qd, int 
   da.grid = grid;
   da.ipw = std::max(1, env_int("KG_DIST_IPW", 64));
   da.counters = (unsigned long long*)(status + ST_CNT);
-  hip_check(hipEventRecord(c->ev[2], s), "hipEventRecord");
+  const bool phase_ev = c->timing >= 2;
+  c->last_phase_ev = phase_ev;
+  c->last_tr_slot = -1;
+  if (phase_ev) hip_check(hipEventRecord(c->ev[2], s), "hipEventRecord");
   hip_check(kgl_distance(da, s), "distance");
-  hip_check(hipEventRecord(c->ev[3], s), "hipEventRecord");
+  if (phase_ev) hip_check(hipEventRecord(c->ev[3], s), "hipEventRecord");
   c->last_N = N;
   c->has_batch = true;
   c->stat_src = status;
@@ -2066,6 +2092,13 @@ extern "C" int kg_autotune(int nb, const kg_world* worlds, const int32_t* ris, c
     Layout L = multi_layout(c, nb, worlds, ris, counts, s, &N);
     if (N == 0) return;
     hip_check(c->wsOut8b.ensure(N), "hipMalloc");
+    // the timing below reads the per-phase events (timing level 2)
+    struct Level {
+      kg_ctx_s* c;
+      int saved;
+      ~Level() { c->timing = saved; }
+    } level{c, c->timing};
+    c->timing = 2;
     const int cpws[] = {4, 8, 16, 32};
     const int leafs[] = {1, 8};
     const int scaps[] = {256, 512};
@@ -2103,6 +2136,14 @@ extern "C" int kg_autotune(int nb, const kg_world* worlds, const int32_t* ris, c
     if (chosen)
       for (int i = 0; i < 4; i++) chosen[i] = bc[i];
     if (best_ms) *best_ms = best / 2;
+  });
+}
+
+extern "C" int kg_ctx_set_timing(kg_ctx c, int level) {
+  return guard([&] {
+    need(c != nullptr, "NULL context");
+    need(level >= 0 && level <= 2, "timing level must be 0, 1 or 2");
+    c->timing = level;
   });
 }
 

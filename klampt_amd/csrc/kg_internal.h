@@ -327,6 +327,9 @@ struct kg_ctx_s {
   DevBuf wsQ, wsT64, wsLB, wsJL, wsCounts, wsOffsets, wsItems, wsScanTmp, wsBest, wsUnc, wsChunk, wsSpill,
       wsStatus, wsRedo, wsOut8a, wsOut8b, wsOut8c, wsOut32, wsF, wsG, wsSdfScratch, wsPoff, wsPts, wsLoss, wsDQ,
       wsDl, wsEA, wsEB, wsESeg, wsESoff, wsENs, wsEOoff, wsEOrder, wsEQ, wsEFeas, wsEU, wsEDone, wsRec;
+  int timing = 1;           // 0 none, 1 traversal kernel events, 2 also per-phase events
+  bool last_phase_ev = false;
+  int last_tr_slot = -1;    // kernel-event slot of the last feasibility batch (-1: none)
   int num_cus = 256;
   // traversal settings (autotuner; 0 = default)
   int tune_cpw = 0, tune_leaf = 0, tune_wpc = 0, tune_scap = 0;

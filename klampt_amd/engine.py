@@ -63,6 +63,12 @@ class Context:
         call("kg_ctx_kernel_times", self.h, 1 if reset else 0, ctypes.byref(ns), ctypes.byref(cnt))
         return ns.value * 1e-9, cnt.value
 
+    def set_timing(self, level):
+        """HIP events recorded per batch: 0 none, 1 (default) the traversal
+        kernel (kernel_times), 2 also the broad phase and the FP64 pass
+        (last_stats times)."""
+        call("kg_ctx_set_timing", self.h, int(level))
+
     def close(self):
         if self.h:
             lib().kg_ctx_destroy(self.h)

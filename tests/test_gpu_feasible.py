@@ -208,3 +208,26 @@ def test_set_self_pairs_replaces_defaults(ctx):
     assert np.array_equal(w.pair_list(0), ow.pair_list(orb))
     n = _check_exact(ow, orb, Q, np.c_[feas, env, slf, fp], np.c_[f64, e64, s64, p64], "custom self pairs")
     assert n <= 2
+
+
+def test_timing_levels(ctx, c1):
+    """Timing level 0 records no events, 1 the traversal kernel's, 2 also the
+    broad phase and FP64 pass; results do not depend on it."""
+    sc, w, r, ow, orb = c1
+    Q = S.uniform_configs(sc.robot, 512, seed=41)
+    want = w.feasible_batch(Q)[0]
+    try:
+        for level in (0, 1, 2):
+            ctx.set_timing(level)
+            ctx.kernel_times(reset=True)
+            got = w.feasible_batch(Q)[0]
+            st = w.last_stats()
+            _, count = ctx.kernel_times(reset=True)
+            assert np.array_equal(got, want)
+            assert count == (0 if level == 0 else 1)
+            assert (st["t_trav32_ns"] > 0) == (level >= 1)
+            assert (st["t_prep_ns"] > 0) == (level == 2)
+    finally:
+        ctx.set_timing(1)
+    with pytest.raises(RuntimeError):
+        ctx.set_timing(3)
