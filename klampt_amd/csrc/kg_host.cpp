@@ -512,6 +512,9 @@ void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* 
     if (pa.group) {
       hip_check(c->wsRec.ensure(std::max<size_t>(1, L.ncap) * sizeof(ItemRec)), "hipMalloc(records)");
       pa.recs = c->wsRec.as<ItemRec>();
+      // with start records nothing reads the frames (the FP64 pass forms
+      // its own from q)
+      pa.T64 = nullptr;
     }
     hip_check(kgl_prep_single(pa, s), "prep");
   } else {
@@ -577,6 +580,9 @@ void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* 
   ta.scap = scap;
   ta.static_pct = std::max(0, std::min(100, env_int("KG_STATIC_PCT", 0)));
   ta.recs = pa.recs;
+  ta.Qf = Qf;
+  ta.Qd = Qd;
+  ta.qoff = L.qoff;
   // results are assembled inside the traversal as each configuration's last
   // item finishes; uncertain ones go to the re-check list
   FinArgs& fa = ta.fin;
@@ -594,6 +600,7 @@ void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* 
   fa.redo_list = redo_list;
   fa.redo_count = status + ST_REDO;
   fa.final_pass = 0;
+  fa.force_recheck = env_int("KG_FORCE_RECHECK", 0) != 0;
   const int slot = (int)(c->tr_count % kg_ctx_s::TRING);
   if (kern_ev) hip_check(hipEventRecord(c->tr_beg[slot], s), "hipEventRecord");
   hip_check(kgl_collide_f32(ta, s), "collide_f32");

@@ -106,7 +106,7 @@ struct PrepArgs {
   int mode;
   PlanDev plan;
   PoolDev pool;
-  double* T64;         // N x nlinks x 12
+  double* T64;         // N x nlinks x 12 (nullable for the per-link variant)
   double* LB;          // N x nlinks x 6 (world AABBs)
   uint8_t* jl;         // N: 1 if within joint limits
   int* counts;         // N+1
@@ -145,6 +145,7 @@ struct FinArgs {
   int* redo_list;   // configs needing the FP64 pass
   int* redo_count;
   int final_pass;   // 1: uncertainty cannot remain
+  int force_recheck;  // test hook: every configuration goes to the FP64 pass
   const PlanDev* plans;  // heterogeneous batches (nullable)
   const int* cfg_plan;
 };
@@ -184,6 +185,9 @@ struct TravArgs {
   FinArgs fin;                // result assembly when a configuration's last item finishes
   size_t t_rows;              // frame rows of T64 and the batch size (t_row layout)
   int t_n;
+  const float* Qf;            // joint values (FP64 pass forms its frames from them)
+  const double* Qd;
+  const int* qoff;            // heterogeneous batches: Q offset per configuration
 };
 hipError_t kgl_collide_f32(const TravArgs& a, hipStream_t s);
 

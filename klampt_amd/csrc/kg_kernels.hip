@@ -103,6 +103,28 @@ __device__ inline void joint64(int type, const double* w, double q, X64& J) {
   J.t[0] = J.t[1] = J.t[2] = 0.0;
 }
 
+// FP64 world frame of link l from the joint values q[qb + k]: the joint
+// frames T0_k J(q_k) composed from l up to its root (the broad phase composes
+// the same product in the other association; the two differ by FP64
+// rounding only).
+__device__ inline void fk_link64(const PlanDev& PL, const float* Qf, const double* Qd, size_t qb, int l, X64& X) {
+  auto local = [&](int k, X64& Lx) {
+    const LinkDesc& L = PL.links[k];
+    const double q = Qf ? (double)Qf[qb + k] : Qd[qb + k];
+    X64 J, T0;
+    joint64(L.jtype, L.axis, q, J);
+    x64_load(L.T0, T0);
+    x64_mul(T0, J, Lx);
+  };
+  local(l, X);
+  for (int p = PL.links[l].parent; p >= 0; p = PL.links[p].parent) {
+    X64 Lp, Y;
+    local(p, Lp);
+    x64_mul(Lp, X, Y);
+    X = Y;
+  }
+}
+
 // ------------------------------------------------------------ FK (FP32)
 
 // One lane per configuration; the link table sits in LDS.
@@ -489,7 +511,7 @@ __global__ __launch_bounds__(256) void k_prep_group(PrepArgs a) {
     }
   }
   if (mine) {
-    t_store(a.T64, a.t_rows, MULTI ? (size_t)a.toff[c] + l : (size_t)l * a.N + c, X);
+    if (a.T64) t_store(a.T64, a.t_rows, MULTI ? (size_t)a.toff[c] + l : (size_t)l * a.N + c, X);
     const LinkDesc& L = P.links[l];
     if (L.geom >= 0) {
       const GeomDesc& gd = a.pool.geoms[L.geom];
@@ -908,7 +930,7 @@ __device__ inline void finalize_cfg(const FinArgs& a, int c, unsigned long long 
     const int r1 = h1 ? (int)(b1 >> 32) : KG_RANK_NONE;
     sure = (u0 > r0 || u0 == KG_RANK_NONE) && (u1 > r1 || u1 == KG_RANK_NONE);
   }
-  if (!sure && !a.final_pass) {
+  if ((!sure || a.force_recheck) && !a.final_pass) {
     const int slot = atomicAdd(a.redo_count, 1);
     a.redo_list[slot] = c;
     return;
@@ -939,8 +961,14 @@ __device__ inline void item_frame(const TravArgs& a, const PairItem& P, Real* re
   const int g = (P.rank >> KG_RANK_GROUP_BIT) & 1;
   // both frames are copied into registers: a pointer that may select a
   // private array would place that array in scratch memory
+  // the exact pass forms the frames from q (the broad phase may not have
+  // stored them), the FP32 pass reads them from T64
   X64 FA, FB;
-  t_load(a.T64, a.t_rows, t_row(a.toff, a.t_n, P.cfg, P.a), FA);
+  const size_t qb = a.qoff ? (size_t)a.qoff[P.cfg] : (size_t)P.cfg * PL.nlinks;
+  if (EXACT)
+    fk_link64(PL, a.Qf, a.Qd, qb, P.a, FA);
+  else
+    t_load(a.T64, a.t_rows, t_row(a.toff, a.t_n, P.cfg, P.a), FA);
   const double* XA = FA.R;
   const double* XB = FB.R;
   int geomB;
@@ -950,7 +978,10 @@ __device__ inline void item_frame(const TravArgs& a, const PairItem& P, Real* re
     for (int k = 0; k < 12; k++) FB.R[k] = XE[k];
     geomB = PL.env[P.b].geom;
   } else {
-    t_load(a.T64, a.t_rows, t_row(a.toff, a.t_n, P.cfg, P.b), FB);
+    if (EXACT)
+      fk_link64(PL, a.Qf, a.Qd, qb, P.b, FB);
+    else
+      t_load(a.T64, a.t_rows, t_row(a.toff, a.t_n, P.cfg, P.b), FB);
     geomB = PL.links[P.b].geom;
   }
   const double d0 = XA[9] - XB[9], d1 = XA[10] - XB[10], d2 = XA[11] - XB[11];

@@ -231,3 +231,19 @@ def test_timing_levels(ctx, c1):
         ctx.set_timing(1)
     with pytest.raises(RuntimeError):
         ctx.set_timing(3)
+
+
+def test_fp64_pass_alone_matches_oracle(c1, c2, monkeypatch):
+    """Every configuration sent through the exact FP64 pass (which forms its
+    link frames from q): the oracle's answers in all three result modes."""
+    monkeypatch.setenv("KG_FORCE_RECHECK", "1")
+    for name, (sc, w, r, ow, orb), seed in (("C1", c1, 51), ("C2", c2, 52)):
+        Q = S.uniform_configs(sc.robot, 1024, seed=seed)
+        f_any = w.feasible_batch(Q)[0]
+        assert w.last_stats()["fp64_rechecks"] == 1024
+        f_grp, e_grp, s_grp, _ = w.feasible_batch(Q, flags=True)
+        feas, env, slf, fp = w.feasible_batch(Q, flags=True, first_pair=True)
+        f2, e2, s2, fp2 = ow.feasible_batch(orb, Q)
+        _check_exact(ow, orb, Q, f_any, f2, name + " any")
+        _check_exact(ow, orb, Q, np.c_[f_grp, e_grp, s_grp], np.c_[f2, e2, s2], name + " groups")
+        _check_exact(ow, orb, Q, np.c_[feas, env, slf, fp], np.c_[f2, e2, s2, fp2], name + " ordered")

@@ -48,3 +48,14 @@ def test_autotune_keeps_results(hetero):
     after = engine.feasible_multi_dev(worlds, [0] * len(worlds), counts, dQ).cpu().numpy()
     assert np.array_equal(before, after)
     engine.set_tuning(worlds[0].ctx)  # back to defaults
+
+
+def test_multi_fp64_pass_matches_oracle(hetero, monkeypatch):
+    """Heterogeneous batch with every configuration re-checked in FP64 (frames
+    formed from q through the per-configuration Q offsets)."""
+    monkeypatch.setenv("KG_FORCE_RECHECK", "1")
+    soup, robots, counts, worlds, Qs = hetero
+    multi = engine.feasible_multi(worlds, [0] * len(worlds), counts, Qs)
+    for b, (spec, Q) in enumerate(zip(robots, Qs)):
+        ow, orb = oracle.world_from_scene(S.Scene(spec, terrains=[soup]))
+        assert np.array_equal(multi[b], ow.feasible_batch(orb, Q)[0]), f"robot {b} ({spec.n} DOF)"
