@@ -755,7 +755,7 @@ int kg_ctx_destroy(kg_ctx c) {
                       &c->wsQ, &c->wsT64, &c->wsLB, &c->wsJL, &c->wsCounts, &c->wsOffsets, &c->wsItems,
                       &c->wsScanTmp, &c->wsBest, &c->wsUnc, &c->wsChunk, &c->wsSpill, &c->wsStatus, &c->wsRedo,
                       &c->wsOut8a, &c->wsOut8b, &c->wsOut8c, &c->wsOut32, &c->wsF, &c->wsG,
-                      &c->wsSdfScratch, &c->wsPoff, &c->wsPts, &c->wsLoss, &c->wsDQ, &c->wsDl, &c->wsEA, &c->wsEB,
+                      &c->wsPoff, &c->wsPts, &c->wsLoss, &c->wsDQ, &c->wsDl, &c->wsEA, &c->wsEB,
                       &c->wsESeg, &c->wsESoff, &c->wsENs, &c->wsEOoff, &c->wsEOrder, &c->wsEQ, &c->wsEFeas,
                       &c->wsEU, &c->wsEDone, &c->wsRec, &c->mPlans, &c->mCfgPlan, &c->mQoff, &c->mToff, &c->mIbase};
     for (auto* b : bufs) b->release();
@@ -1520,10 +1520,8 @@ void sdf_penalty_dev(kg_robot_s* r, kg_geom_s* sdf, const float* dQ, int N, cons
   robot_links_dev(r);
   hip_check(c->wsPoff.ensure((n + 1) * sizeof(int)), "hipMalloc");
   hip_check(hipMemcpyAsync(c->wsPoff.p, off.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice, s), "copy offsets");
-  hip_check(c->wsSdfScratch.ensure((size_t)N * 2 * n * 12 * sizeof(float)), "hipMalloc(scratch)");
   SdfDev g = sdf_dev(sdf);
-  hip_check(kgl_sdf_penalty(r->dlinks.as<LinkDesc>(), n, g, dQ, N, dpts, c->wsPoff.as<int>(), dsafe, dloss, ddl, ddQ,
-                            c->wsSdfScratch.as<float>(), bwd, s),
+  hip_check(kgl_sdf_penalty(r->dlinks.as<LinkDesc>(), n, g, dQ, N, dpts, c->wsPoff.as<int>(), dsafe, dloss, ddl, ddQ, bwd, s),
             "sdf_penalty");
 }
 

@@ -229,8 +229,8 @@ hipError_t kgl_fill_int(int* p, int n, int v, hipStream_t s);
 int kgl_collide_grid(int exact, int num_cus, int scap);
 hipError_t kgl_sdf_eval(const SdfDev& g, const float* pts, int n, float* d, float* grad, hipStream_t s);
 hipError_t kgl_sdf_penalty(const LinkDesc* links, int n, const SdfDev& g, const float* Q, int N, const float* pts,
-                           const int* poff, float dsafe, float* loss, const float* dloss, float* dQ, float* scratch,
-                           bool bwd, hipStream_t s);
+                           const int* poff, float dsafe, float* loss, const float* dloss, float* dQ, bool bwd,
+                           hipStream_t s);
 
 // ---- C-ABI object layouts
 
@@ -329,7 +329,7 @@ struct kg_ctx_s {
   hipStream_t done_stream = nullptr;
   bool done_valid = false;
   DevBuf wsQ, wsT64, wsLB, wsJL, wsCounts, wsOffsets, wsItems, wsScanTmp, wsBest, wsUnc, wsChunk, wsSpill,
-      wsStatus, wsRedo, wsOut8a, wsOut8b, wsOut8c, wsOut32, wsF, wsG, wsSdfScratch, wsPoff, wsPts, wsLoss, wsDQ,
+      wsStatus, wsRedo, wsOut8a, wsOut8b, wsOut8c, wsOut32, wsF, wsG, wsPoff, wsPts, wsLoss, wsDQ,
       wsDl, wsEA, wsEB, wsESeg, wsESoff, wsENs, wsEOoff, wsEOrder, wsEQ, wsEFeas, wsEU, wsEDone, wsRec;
   int timing = 1;           // 0 none, 1 traversal kernel events, 2 also per-phase events
   bool last_phase_ev = false;

@@ -1669,46 +1669,43 @@ __global__ __launch_bounds__(256) void k_sdf_eval(SdfDev g, const float* __restr
   }
 }
 
-// SDF hinge penalty, one lane per configuration.
-//   L(q) = sum_k sum_{p in link k} max(0, dsafe - sdf(T_k(q) p))^2
-// BWD: dQ[c][j] = dloss[c] * dL/dq_j by the FK-Jacobian adjoint: with
-// g_p = dL/dx_p = -2 max(0, dsafe - d) grad_d(x_p), each link k accumulates
-// f_k = sum g_p and m_k = sum x_p x g_p; for joint j with world axis w_j and
-// origin o_j, dL/dq_j = w_j . (M_j - o_j x F_j) (revolute) or w_j . F_j
-// (prismatic), F_j, M_j summed over the subtree of j.  FK is recomputed here
-// (nothing is kept from the forward pass).
-//
-// Scratch (structure of arrays over configurations, [row][comp][c]):
-//   rows [0,n)   full link transform, written only for links that are the
-//                parent of a non-consecutive child (branching trees);
-//   rows [n,2n)  w(3) o(3) f(3) m(3) of every link (BWD only).
-template <bool BWD>
-__global__ __launch_bounds__(256) void k_sdf_penalty(const LinkDesc* __restrict__ links, int n, SdfDev g,
-                                                      const float* __restrict__ Q, int N, const float* __restrict__ pts,
-                                                      const int* __restrict__ poff, float dsafe,
-                                                      float* __restrict__ loss_out, const float* __restrict__ dloss,
-                                                      float* __restrict__ dQ, float* __restrict__ scratch) {
+// SDF hinge penalty with a group of GL lanes per configuration (waypoint):
+// the joint frames are formed by lanes 0..n-1 and composed into world frames
+// in LDS by the group's first lane; the points are spread over the group
+// (lane l takes points l, l + GL, ...), each lane accumulating per-link
+// force and moment sums that are added into LDS; the loss is reduced over
+// the group, and the group's first lane runs the FK-Jacobian adjoint
+// (subtree sums added into the parent entries, children after parents):
+//   dL/dq_j = w_j . (M_j - o_j x F_j) (revolute), w_j . F_j (prismatic).
+// Dynamic LDS: gpb groups x n links x (12 frame + 6 sum) floats.
+template <bool BWD, int GL>
+__global__ __launch_bounds__(256) void k_sdf_pen_group(const LinkDesc* __restrict__ links, int n, SdfDev g,
+                                                       const float* __restrict__ Q, int N,
+                                                       const float* __restrict__ pts, const int* __restrict__ poff,
+                                                       float dsafe, float* __restrict__ loss_out,
+                                                       const float* __restrict__ dloss, float* __restrict__ dQ,
+                                                       int gpb) {
+  extern __shared__ float dyn[];
   __shared__ float sT0[KG_MAX_LINKS][12];
   __shared__ float sAx[KG_MAX_LINKS][3];
-  __shared__ int sPar[KG_MAX_LINKS], sTy[KG_MAX_LINKS], sOff[KG_MAX_LINKS + 1], sKeep[KG_MAX_LINKS];
+  __shared__ int sPar[KG_MAX_LINKS], sTy[KG_MAX_LINKS], sOff[KG_MAX_LINKS + 1];
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
     for (int k = 0; k < 12; k++) sT0[i][k] = (float)links[i].T0[k];
     for (int k = 0; k < 3; k++) sAx[i][k] = (float)links[i].axis[k];
     sPar[i] = links[i].parent;
     sTy[i] = links[i].jtype;
-    int keep = 0;
-    for (int j = i + 2; j < n; j++)
-      if (links[j].parent == i) keep = 1;
-    sKeep[i] = keep;
   }
   for (int i = threadIdx.x; i <= n; i += blockDim.x) sOff[i] = poff[i];
+  const int grp = threadIdx.x / GL, l = threadIdx.x % GL;
+  const int c = blockIdx.x * gpb + grp;
+  const bool valid = c < N;
+  float* sT = dyn + (size_t)grp * n * 12;
+  float* sFM = dyn + (size_t)gpb * n * 12 + (size_t)grp * n * 6;
   __syncthreads();
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= N) return;
-  auto S = [&](int row, int comp) -> float& { return scratch[((size_t)row * 12 + comp) * N + c]; };
-  float T[12];
-  float loss = 0.f;
-  for (int k = 0; k < n; k++) {
+  // joint frames T0_k J(q_k)
+  for (int k = l; k < n; k += GL) {
+    for (int i = 0; i < 6; i++) sFM[k * 6 + i] = 0.f;
+    if (!valid) continue;
     const float q = Q[(size_t)c * n + k];
     const float* w = sAx[k];
     float J[12];
@@ -1717,55 +1714,70 @@ __global__ __launch_bounds__(256) void k_sdf_penalty(const LinkDesc* __restrict_
       for (int i = 0; i < 9; i++) J[i] = (i % 4 == 0) ? 1.f : 0.f;
       J[9] = w[0] * q; J[10] = w[1] * q; J[11] = w[2] * q;
     } else {
-      float s, co;
-      sincosf(q, &s, &co);
+      float sn, co;
+      sincosf(q, &sn, &co);
       const float v = 1.f - co;
 #pragma unroll
       for (int col = 0; col < 3; col++)
 #pragma unroll
         for (int row = 0; row < 3; row++) J[col * 3 + row] = w[row] * w[col] * v;
       J[0] += co; J[4] += co; J[8] += co;
-      J[1] += s * w[2]; J[2] -= s * w[1];
-      J[3] -= s * w[2]; J[5] += s * w[0];
-      J[6] += s * w[1]; J[7] -= s * w[0];
+      J[1] += sn * w[2]; J[2] -= sn * w[1];
+      J[3] -= sn * w[2]; J[5] += sn * w[0];
+      J[6] += sn * w[1]; J[7] -= sn * w[0];
       J[9] = J[10] = J[11] = 0.f;
     }
-    float L[12];
     const float* A = sT0[k];
+    float* L = sT + k * 12;
 #pragma unroll
     for (int col = 0; col < 3; col++)
 #pragma unroll
       for (int r = 0; r < 3; r++) L[col * 3 + r] = A[r] * J[col * 3] + A[3 + r] * J[col * 3 + 1] + A[6 + r] * J[col * 3 + 2];
 #pragma unroll
     for (int r = 0; r < 3; r++) L[9 + r] = A[r] * J[9] + A[3 + r] * J[10] + A[6 + r] * J[11] + A[9 + r];
-    const int p = sPar[k];
-    if (p < 0) {
+  }
+  __syncthreads();
+  // world frames (parents precede their children)
+  if (valid && l == 0)
+    for (int k = 0; k < n; k++) {
+      const int p = sPar[k];
+      if (p < 0) continue;
+      const float* P = sT + p * 12;
+      float* T = sT + k * 12;
+      float L[12];
 #pragma unroll
-      for (int i = 0; i < 12; i++) T[i] = L[i];
-    } else {
-      float Pv[12];
-      if (p == k - 1) {
-#pragma unroll
-        for (int i = 0; i < 12; i++) Pv[i] = T[i];
-      } else {
-#pragma unroll
-        for (int i = 0; i < 12; i++) Pv[i] = S(p, i);
-      }
+      for (int i = 0; i < 12; i++) L[i] = T[i];
 #pragma unroll
       for (int col = 0; col < 3; col++)
 #pragma unroll
-        for (int r = 0; r < 3; r++)
-          T[col * 3 + r] = Pv[r] * L[col * 3] + Pv[3 + r] * L[col * 3 + 1] + Pv[6 + r] * L[col * 3 + 2];
+        for (int r = 0; r < 3; r++) T[col * 3 + r] = P[r] * L[col * 3] + P[3 + r] * L[col * 3 + 1] + P[6 + r] * L[col * 3 + 2];
 #pragma unroll
-      for (int r = 0; r < 3; r++) T[9 + r] = Pv[r] * L[9] + Pv[3 + r] * L[10] + Pv[6 + r] * L[11] + Pv[9 + r];
+      for (int r = 0; r < 3; r++) T[9 + r] = P[r] * L[9] + P[3 + r] * L[10] + P[6 + r] * L[11] + P[9 + r];
     }
-    if (sKeep[k]) {
-#pragma unroll
-      for (int i = 0; i < 12; i++) S(k, i) = T[i];
-    }
+  __syncthreads();
+  float loss = 0.f;
+  if (valid) {
+    const int npts = sOff[n];
+    int k = 0, cur = -1;
+    float T[12];
     float f0 = 0.f, f1 = 0.f, f2 = 0.f, m0 = 0.f, m1 = 0.f, m2 = 0.f;
-    for (int pi = sOff[k]; pi < sOff[k + 1]; pi++) {
-      const float px = pts[3 * pi], py = pts[3 * pi + 1], pz = pts[3 * pi + 2];
+    auto flush = [&]() {
+      if (BWD && cur >= 0) {
+        float* F = sFM + cur * 6;
+        atomicAdd(F + 0, f0); atomicAdd(F + 1, f1); atomicAdd(F + 2, f2);
+        atomicAdd(F + 3, m0); atomicAdd(F + 4, m1); atomicAdd(F + 5, m2);
+      }
+      f0 = f1 = f2 = m0 = m1 = m2 = 0.f;
+    };
+    for (int i = l; i < npts; i += GL) {
+      while (i >= sOff[k + 1]) k++;
+      if (k != cur) {
+        flush();
+        cur = k;
+#pragma unroll
+        for (int j = 0; j < 12; j++) T[j] = sT[k * 12 + j];
+      }
+      const float px = pts[3 * i], py = pts[3 * i + 1], pz = pts[3 * i + 2];
       const float x = T[0] * px + T[3] * py + T[6] * pz + T[9];
       const float y = T[1] * px + T[4] * py + T[7] * pz + T[10];
       const float z = T[2] * px + T[5] * py + T[8] * pz + T[11];
@@ -1784,45 +1796,37 @@ __global__ __launch_bounds__(256) void k_sdf_penalty(const LinkDesc* __restrict_
         }
       }
     }
-    if (BWD) {
-      S(n + k, 0) = T[0] * w[0] + T[3] * w[1] + T[6] * w[2];
-      S(n + k, 1) = T[1] * w[0] + T[4] * w[1] + T[7] * w[2];
-      S(n + k, 2) = T[2] * w[0] + T[5] * w[1] + T[8] * w[2];
-      S(n + k, 3) = T[9]; S(n + k, 4) = T[10]; S(n + k, 5) = T[11];
-      S(n + k, 6) = f0; S(n + k, 7) = f1; S(n + k, 8) = f2;
-      S(n + k, 9) = m0; S(n + k, 10) = m1; S(n + k, 11) = m2;
-    }
+    flush();
   }
+#pragma unroll
+  for (int off = GL / 2; off > 0; off >>= 1) loss += __shfl_xor(loss, off, GL);
   if (!BWD) {
-    loss_out[c] = loss;
+    if (valid && l == 0) loss_out[c] = loss;
     return;
   }
+  __syncthreads();
+  if (!valid || l != 0) return;
   const float dl = dloss ? dloss[c] : 1.f;
-  // reverse sweep: subtree sums carried in registers along chains, added to
-  // the parent's scratch row for branching children
-  float cF0 = 0.f, cF1 = 0.f, cF2 = 0.f, cM0 = 0.f, cM1 = 0.f, cM2 = 0.f;
   for (int j = n - 1; j >= 0; j--) {
-    float F0 = S(n + j, 6), F1 = S(n + j, 7), F2 = S(n + j, 8);
-    float M0 = S(n + j, 9), M1 = S(n + j, 10), M2 = S(n + j, 11);
-    if (j + 1 < n && sPar[j + 1] == j) {
-      F0 += cF0; F1 += cF1; F2 += cF2;
-      M0 += cM0; M1 += cM1; M2 += cM2;
-    }
-    const float w0 = S(n + j, 0), w1 = S(n + j, 1), w2 = S(n + j, 2);
-    const float o0 = S(n + j, 3), o1 = S(n + j, 4), o2 = S(n + j, 5);
+    const float* T = sT + j * 12;
+    float* FM = sFM + j * 6;
+    const float* w = sAx[j];
+    const float w0 = T[0] * w[0] + T[3] * w[1] + T[6] * w[2];
+    const float w1 = T[1] * w[0] + T[4] * w[1] + T[7] * w[2];
+    const float w2 = T[2] * w[0] + T[5] * w[1] + T[8] * w[2];
+    const float o0 = T[9], o1 = T[10], o2 = T[11];
+    const float F0 = FM[0], F1 = FM[1], F2 = FM[2], M0 = FM[3], M1 = FM[4], M2 = FM[5];
     float gq;
-    if (sTy[j] == 1) {
+    if (sTy[j] == 1)
       gq = w0 * F0 + w1 * F1 + w2 * F2;
-    } else {
+    else
       gq = w0 * (M0 - (o1 * F2 - o2 * F1)) + w1 * (M1 - (o2 * F0 - o0 * F2)) + w2 * (M2 - (o0 * F1 - o1 * F0));
-    }
     dQ[(size_t)c * n + j] = dl * gq;
     const int p = sPar[j];
-    if (p >= 0 && p != j - 1) {
-      S(n + p, 6) += F0; S(n + p, 7) += F1; S(n + p, 8) += F2;
-      S(n + p, 9) += M0; S(n + p, 10) += M1; S(n + p, 11) += M2;
+    if (p >= 0) {
+      float* PF = sFM + p * 6;
+      for (int i = 0; i < 6; i++) PF[i] += FM[i];
     }
-    cF0 = F0; cF1 = F1; cF2 = F2; cM0 = M0; cM1 = M1; cM2 = M2;
   }
   if (loss_out) loss_out[c] = loss;
 }
@@ -1836,14 +1840,20 @@ hipError_t kgl_sdf_eval(const SdfDev& g, const float* pts, int n, float* d, floa
 }
 
 hipError_t kgl_sdf_penalty(const LinkDesc* links, int n, const SdfDev& g, const float* Q, int N, const float* pts,
-                           const int* poff, float dsafe, float* loss, const float* dloss, float* dQ, float* scratch,
-                           bool bwd, hipStream_t s) {
+                           const int* poff, float dsafe, float* loss, const float* dloss, float* dQ, bool bwd,
+                           hipStream_t s) {
   if (N <= 0) return hipSuccess;
+  // 16 lanes per configuration; groups per block limited by 48 KB of LDS
+  constexpr int GL = 16;
+  const size_t per_group = (size_t)n * 18 * sizeof(float);
+  int gpb = 256 / GL;
+  while (gpb > 1 && gpb * per_group > 48 * 1024) gpb /= 2;
+  const size_t lds = gpb * per_group;
+  const int blocks = (N + gpb - 1) / gpb;
   if (bwd)
-    k_sdf_penalty<true><<<(N + 255) / 256, 256, 0, s>>>(links, n, g, Q, N, pts, poff, dsafe, loss, dloss, dQ, scratch);
+    k_sdf_pen_group<true, GL><<<blocks, gpb * GL, lds, s>>>(links, n, g, Q, N, pts, poff, dsafe, loss, dloss, dQ, gpb);
   else
-    k_sdf_penalty<false><<<(N + 255) / 256, 256, 0, s>>>(links, n, g, Q, N, pts, poff, dsafe, loss, dloss, dQ,
-                                                          scratch);
+    k_sdf_pen_group<false, GL><<<blocks, gpb * GL, lds, s>>>(links, n, g, Q, N, pts, poff, dsafe, loss, dloss, dQ, gpb);
   return hipGetLastError();
 }
 
