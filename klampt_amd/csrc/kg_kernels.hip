@@ -435,9 +435,12 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
 // come from a counter in LDS; their order does not matter, since the ranks
 // carry Klampt's loop order.  MULTI: per-configuration plans (the single
 // plan is then read from the kernel arguments with scalar loads).
+// one wavefront per block: a group never spans waves, so the block barriers
+// only order the wave's own LDS traffic and no wave waits for another
+#define KG_PREP_TB 64
 template <int G, bool MULTI>
-__global__ __launch_bounds__(256) void k_prep_group(PrepArgs a) {
-  constexpr int NG = 256 / G;  // configurations per block
+__global__ __launch_bounds__(KG_PREP_TB) void k_prep_group(PrepArgs a) {
+  constexpr int NG = KG_PREP_TB / G;  // configurations per block
   __shared__ double sF[NG][G][12];
   __shared__ double sB[NG][G][6];
   __shared__ int sAnc[NG][G];
@@ -1522,15 +1525,16 @@ hipError_t kgl_prep_single(const PrepArgs& a, hipStream_t s) {
   const int n = a.N > a.nreset ? a.N : a.nreset;
   if (n <= 0) return hipSuccess;
   if (a.group > 0) {
-    const int ng = 256 / a.group;
-    const int blocks = max((a.N + ng - 1) / ng, (a.nreset + 255) / 256);
+    constexpr int TB = KG_PREP_TB;
+    const int ng = TB / a.group;
+    const int blocks = max((a.N + ng - 1) / ng, (a.nreset + TB - 1) / TB);
     const bool m = a.cfg_plan != nullptr;
     if (a.group == 8)
-      m ? k_prep_group<8, true><<<blocks, 256, 0, s>>>(a) : k_prep_group<8, false><<<blocks, 256, 0, s>>>(a);
+      m ? k_prep_group<8, true><<<blocks, TB, 0, s>>>(a) : k_prep_group<8, false><<<blocks, TB, 0, s>>>(a);
     else if (a.group == 16)
-      m ? k_prep_group<16, true><<<blocks, 256, 0, s>>>(a) : k_prep_group<16, false><<<blocks, 256, 0, s>>>(a);
+      m ? k_prep_group<16, true><<<blocks, TB, 0, s>>>(a) : k_prep_group<16, false><<<blocks, TB, 0, s>>>(a);
     else
-      m ? k_prep_group<32, true><<<blocks, 256, 0, s>>>(a) : k_prep_group<32, false><<<blocks, 256, 0, s>>>(a);
+      m ? k_prep_group<32, true><<<blocks, TB, 0, s>>>(a) : k_prep_group<32, false><<<blocks, TB, 0, s>>>(a);
   } else if (a.lds_links)
     k_prep<2, KG_PREP_LDS_LINKS><<<(n + KG_PREP_LDS_BLOCK - 1) / KG_PREP_LDS_BLOCK, KG_PREP_LDS_BLOCK, 0, s>>>(a);
   else
