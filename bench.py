@@ -206,7 +206,7 @@ def _time_dev(fn, reps):
 
 def run_secondary(args, ctx, sc, world, robot):
     """Distances (C2), edge checks (C4), SDF penalty gradients (C5),
-    heterogeneous batches (C3)."""
+    heterogeneous batches (C3), planners over the batched queries."""
     import numpy as np
     import torch
     from klampt_amd import engine, synthetic as S
@@ -325,6 +325,56 @@ def run_secondary(args, ctx, sc, world, robot):
                         "feasible_fraction": round(float(fh.float().mean().item()), 4)}
     engine.set_tuning(ctx)
     print(f"[bench] C3 hetero: {out['c3_hetero']}", file=sys.stderr, flush=True)
+
+    # planners on the batched queries: PRM roadmap over the 16-box scene
+    # (every candidate edge validated in batched calls) and a lazy PRM query
+    from klampt_amd import plan, robotsim
+    bs = S.boxes_scene(0)
+    W = robotsim.WorldModel()
+    for mesh, xf in bs.objects:
+        W.addRigidObject(robotsim.Geometry3D(robotsim.TriangleMesh(mesh.verts, mesh.tris)), xf[:9], xf[9:])
+    R = robotsim.RobotModel.from_spec(bs.robot)
+    W.addRobot(R)
+    space = plan.RobotCSpace(R, W, eps=0.02)
+    space.rng = np.random.default_rng(5)
+    Qs = S.uniform_configs(bs.robot, 4096, seed=63)
+    Qf = Qs[space.feasible_batch(Qs)]
+    dq = np.linalg.norm(Qf[:256, None] - Qf[None, :256], axis=-1)
+    cand = np.argwhere((dq > 1.5) & (dq < 3.0))
+    cand = cand[cand[:, 0] < cand[:, 1]]
+    blocked = ~space.visible_batch(Qf[cand[:, 0]], Qf[cand[:, 1]])
+    qa, qb = Qf[cand[int(np.argmax(blocked))]].astype(float)
+    mp = plan.MotionPlan(space, type="prm", knn=10)
+    mp.setEndpoints(qa, qb)
+    tp0 = time.perf_counter()
+    for _ in range(4):
+        mp.planMore(8192)
+    tprm = time.perf_counter() - tp0
+    sprm = mp.getStats()
+    lz = plan.MotionPlan(space, type="lazyprm", knn=10)
+    lz.setEndpoints(qa, qb)
+    tl0 = time.perf_counter()
+    while lz.getPath() is None and lz.getStats()["samples"] < 65536:
+        lz.planMore(4096)
+    tlazy = time.perf_counter() - tl0
+    slz = lz.getStats()
+    out["planner"] = {"scene": "7-DOF arm among 16 boxes, edge resolution 0.02 rad (L2)",
+                      "prm_samples": sprm["samples"], "prm_milestones": sprm["milestones"],
+                      "prm_edges_validated": sprm["edges_checked"], "prm_ms": round(tprm * 1e3, 1),
+                      "prm_edges_per_s": round(sprm["edges_checked"] / tprm, 1),
+                      "prm_path_found": mp.getPath() is not None,
+                      "lazyprm_ms_to_path": round(tlazy * 1e3, 1), "lazyprm_edges_checked": slz["edges_checked"],
+                      "lazyprm_samples": slz["samples"], "lazyprm_path_found": lz.getPath() is not None}
+    if not args.no_cpu:
+        V, E = mp.getRoadmap()
+        V, E = np.asarray(V), np.asarray([e for e in mp.edges][:512])
+        obw, obr = oracle.world_from_scene(bs)
+        t1 = time.perf_counter()
+        obw.edge_batch(obr, V[E[:, 0]], V[E[:, 1]], eps=0.02, norm_inf=False)
+        tc = time.perf_counter() - t1
+        out["planner"]["cpu_edges_per_s"] = round(len(E) / tc, 1)
+        out["planner"]["cpu_sample"] = f"first {len(E)} roadmap edges, FP64 oracle, {oracle.nthreads_default()} threads"
+    print(f"[bench] planner: {out['planner']}", file=sys.stderr, flush=True)
     return out
 
 

@@ -1,0 +1,378 @@
+"""Planner integration: Klampt's CSpace, RobotCSpace and MotionPlan shapes on
+the batched feasibility and edge-check engine.
+
+Mirrors
+* ``klampt.plan.cspace.CSpace`` (``Python/klampt/plan/cspace.py:12-215``):
+  bounds, samplers, named feasibility tests, ``inBounds``, ``feasible``,
+  ``isFeasible``, ``isVisible`` (subdivision at ``eps``), ``distance``,
+  ``interpolate``, ``getStats``;
+* ``klampt.plan.robotcspace.RobotCSpace``
+  (``Python/klampt/plan/robotcspace.py:11-130``): joint limits, self and
+  environment collision (``SingleRobotCSpace::IsFeasible``,
+  ``Cpp/Planning/RobotCSpace.cpp:786-823``), ``addConstraint``;
+* ``klampt.plan.cspace.MotionPlan`` (``cspace.py:216-440``): ``setEndpoints``,
+  ``planMore``, ``getPath``, ``getRoadmap``, ``getStats``, ``pathCost``.
+
+What is new here is the batched form of the two queries planners make,
+``feasible_batch(Q)`` and ``visible_batch(A, B)``, and planners that use them:
+``'prm'`` validates every candidate edge of a batch of milestones in one call;
+``'lazyprm'`` (Bohlin and Kavraki's lazy PRM) searches the roadmap of
+unvalidated edges and validates only the edges of the best path, all of them
+in one call per round, until a path survives.
+"""
+import math
+import time
+
+import numpy as np
+
+from . import robotsim
+
+
+class CSpace:
+    """A configuration space (``Python/klampt/plan/cspace.py:12-215``).
+    ``feasible`` runs the added tests in order (bounds only when there are
+    none); ``visible`` checks the straight segment at resolution ``eps`` in
+    Klampt's bisection order (``Python/klampt/plan/robotcspace.py:216-228``
+    discretisation, ``n = ceil(d / eps)`` samples at ``u = i / n``)."""
+
+    def __init__(self):
+        self.eps = 1e-3
+        self.bound = [(0.0, 1.0)]
+        self.properties = {}
+        self.feasibilityTests = None
+        self.feasibilityTestNames = None
+        self.feasibilityTestDependencies = None
+        self.rng = np.random.default_rng()
+        self._stats = {"feasible_calls": 0, "feasible_configs": 0, "visible_calls": 0, "visible_edges": 0}
+
+    def setBounds(self, bound):
+        self.bound = [(float(a), float(b)) for a, b in bound]
+        self.properties["minimum"] = [b[0] for b in self.bound]
+        self.properties["maximum"] = [b[1] for b in self.bound]
+        volume = 1.0
+        for a, b in self.bound:
+            if a != b:
+                volume *= b - a
+        self.properties["volume"] = volume
+
+    def close(self):
+        pass
+
+    def sample(self):
+        lo = np.array([b[0] for b in self.bound])
+        hi = np.array([b[1] for b in self.bound])
+        return list(self.rng.uniform(lo, hi))
+
+    def sampleneighborhood(self, c, r):
+        return [float(self.rng.uniform(max(b[0], ci - r), min(b[1], ci + r))) for ci, b in zip(c, self.bound)]
+
+    def sample_batch(self, n):
+        """n samples of ``sample()`` (uniform in the bounds unless overridden)."""
+        lo = np.array([b[0] for b in self.bound])
+        hi = np.array([b[1] for b in self.bound])
+        return self.rng.uniform(lo, hi, size=(n, len(self.bound)))
+
+    def addFeasibilityTest(self, func, name=None, dependencies=None):
+        if self.feasibilityTests is None:
+            self.feasibilityTests, self.feasibilityTestNames, self.feasibilityTestDependencies = [], [], []
+        if name is not None and not isinstance(name, str):
+            raise TypeError("feasibility test name must be a string")
+        if not callable(func):
+            raise TypeError("feasibility test must be callable")
+        self.feasibilityTests.append(func)
+        self.feasibilityTestNames.append(name if name is not None else f"test_{len(self.feasibilityTests) - 1}")
+        if dependencies is not None:
+            deps = dependencies if isinstance(dependencies, (list, tuple)) else [dependencies]
+            for d in deps:
+                self.feasibilityTestDependencies.append((self.feasibilityTestNames[-1], d))
+
+    def inBounds(self, x):
+        return all(a <= xi <= b for xi, (a, b) in zip(x, self.bound))
+
+    def feasible(self, x):
+        if self.feasibilityTests is None:
+            return self.inBounds(x)
+        return all(test(x) for test in self.feasibilityTests)
+
+    def feasible_batch(self, Q):
+        """Feasibility of each row of Q (bool array)."""
+        return np.array([bool(self.feasible(list(q))) for q in np.asarray(Q, float)], bool)
+
+    def isFeasible(self, x):
+        self._stats["feasible_calls"] += 1
+        self._stats["feasible_configs"] += 1
+        return bool(self.feasible(x))
+
+    def distance(self, a, b):
+        return float(np.linalg.norm(np.asarray(b, float) - np.asarray(a, float)))
+
+    def interpolate(self, a, b, u):
+        a = np.asarray(a, float)
+        return list(a + u * (np.asarray(b, float) - a))
+
+    def visible(self, a, b):
+        return bool(self.visible_batch(np.asarray([a], float), np.asarray([b], float))[0])
+
+    def visible_batch(self, A, B):
+        """Whether each segment A[i] -> B[i] is feasible at its interior
+        samples u = j / n, j = 1..n-1, n = ceil(distance / eps), tested in
+        Klampt's bisection order with an early exit (the endpoints are
+        milestones and not re-tested)."""
+        out = np.ones(len(A), bool)
+        for s, (a, b) in enumerate(zip(np.asarray(A, float), np.asarray(B, float))):
+            n = int(math.ceil(self.distance(a, b) / self.eps))
+            for j in _bisection_order(n):
+                if not self.feasible(self.interpolate(a, b, j / n)):
+                    out[s] = False
+                    break
+        return out
+
+    def isVisible(self, a, b):
+        self._stats["visible_calls"] += 1
+        self._stats["visible_edges"] += 1
+        return self.visible(a, b)
+
+    def getStats(self):
+        return dict(self._stats)
+
+
+def _bisection_order(n):
+    """Interior sample indices 1..n-1 in level-order bisection (midpoints of
+    ever finer halvings), the order the batched edge checker uses."""
+    if n < 2:
+        return []
+    out, seen = [], set()
+    level = [(0, n)]
+    while level:
+        nxt = []
+        for lo, hi in level:
+            if hi - lo < 2:
+                continue
+            mid = (lo + hi) // 2
+            if mid not in seen:
+                seen.add(mid)
+                out.append(mid)
+            nxt.append((lo, mid))
+            nxt.append((mid, hi))
+        level = nxt
+    return out
+
+
+class RobotCSpace(CSpace):
+    """Collision-free motion of one robot of a world
+    (``Python/klampt/plan/robotcspace.py:11-130``): joint limits (closed
+    intervals), self collision and collision with the world's terrains,
+    rigid objects and other robots, per Klampt's default collision mask.
+    Both queries run batched on the GPU; constraints added with
+    ``addConstraint`` are applied on top, on the host."""
+
+    def __init__(self, robot: robotsim.RobotModel, world: robotsim.WorldModel, eps: float = 1e-3):
+        super().__init__()
+        self.robot, self.world = robot, world
+        self._c = robotsim.RobotCSpace(robot, world, eps=eps, norm_inf=False)
+        self.eps = eps
+        qmin, qmax = robot.getJointLimits()
+        self.setBounds(list(zip(qmin, qmax)))
+        self.properties["geodesic"] = 1
+        self._extra = []
+
+    def addConstraint(self, checker, name=None):
+        """An extra feasibility test f(q) -> bool, applied after the
+        collision checks (``robotcspace.py:73-74``)."""
+        if not callable(checker):
+            raise TypeError("constraint must be callable")
+        self._extra.append((checker, name or f"constraint_{len(self._extra)}"))
+
+    def inJointLimits(self, x):
+        return self.inBounds(x)
+
+    def feasible(self, x):
+        return bool(self.feasible_batch(np.asarray([x], float))[0])
+
+    def feasible_batch(self, Q):
+        Q = np.asarray(Q, np.float32).reshape(-1, len(self.bound))
+        self._stats["feasible_configs"] += len(Q)
+        ok = self._c.feasible_batch(Q)[0].astype(bool)
+        for f, _ in self._extra:
+            for i in np.nonzero(ok)[0]:
+                ok[i] = bool(f(list(map(float, Q[i]))))
+        return ok
+
+    def selfCollision(self, x=None):
+        x = self.robot.getConfig() if x is None else x
+        _, env, slf, _ = self._c.feasible_batch(np.asarray([x], np.float32), flags=True)
+        return bool(slf[0])
+
+    def envCollision(self, x=None):
+        x = self.robot.getConfig() if x is None else x
+        _, env, slf, _ = self._c.feasible_batch(np.asarray([x], np.float32), flags=True)
+        return bool(env[0])
+
+    def visible_batch(self, A, B):
+        A = np.asarray(A, np.float32).reshape(-1, len(self.bound))
+        B = np.asarray(B, np.float32).reshape(-1, len(self.bound))
+        self._stats["visible_edges"] += len(A)
+        if len(A) == 0:
+            return np.zeros(0, bool)
+        ok = self._c.edge_batch(A, B) < 0
+        if self._extra:
+            ok &= CSpace.visible_batch(self, A, B)
+        return ok
+
+
+class MotionPlan:
+    """A roadmap planner over a CSpace (``Python/klampt/plan/cspace.py:216-440``).
+
+    type 'prm': each ``planMore(n)`` draws n configurations in one batch,
+    keeps the feasible ones, links each new milestone to its ``knn`` nearest
+    milestones (within ``connectionThreshold``) and validates all those edges
+    in one batch.  type 'lazyprm': the same links are kept unvalidated; after
+    each batch the shortest start-goal path over the links not known to be
+    blocked is found and its unvalidated edges are checked together, until
+    a path survives or none remains.
+    """
+
+    def __init__(self, space: CSpace, type: str = None, **options):
+        self.space = space
+        self.type = type or "lazyprm"
+        if self.type not in ("prm", "lazyprm"):
+            raise ValueError(f"unsupported planner type {self.type!r} (prm, lazyprm)")
+        self.knn = int(options.pop("knn", 10))
+        self.connectionThreshold = float(options.pop("connectionThreshold", math.inf))
+        if options:
+            raise ValueError(f"unknown planner options {sorted(options)}")
+        self.V = np.zeros((0, len(space.bound)))
+        self.edges = {}          # (i, j), i < j -> True (valid), False (blocked), None (not checked)
+        self.start = self.goal = None
+        self.path_idx = None
+        self._stats = {"milestones": 0, "samples": 0, "edges_checked": 0, "check_calls": 0, "time_s": 0.0}
+
+    def close(self):
+        pass
+
+    def setEndpoints(self, start, goal):
+        if not callable(goal) and not isinstance(goal, tuple):
+            ok = self.space.feasible_batch(np.asarray([start, goal], float))
+            if not ok[0]:
+                raise RuntimeError("Start configuration is infeasible")
+            if not ok[1]:
+                raise RuntimeError("Goal configuration is infeasible")
+        else:
+            raise ValueError("goal sets are not supported: pass a configuration")
+        self.V = np.asarray([start, goal], float)
+        self.start, self.goal = 0, 1
+        self.edges = {}
+        self.path_idx = None
+        self._connect(np.array([0, 1]))
+
+    def addMilestone(self, x):
+        self.V = np.vstack([self.V, np.asarray(x, float)[None]])
+        k = len(self.V) - 1
+        self._connect(np.array([k]))
+        return k
+
+    def _connect(self, new):
+        """Candidate links of the milestones in `new` to their nearest
+        milestones (all milestones, Euclidean)."""
+        if len(self.V) < 2:
+            return
+        from scipy.spatial import cKDTree
+        tree = cKDTree(self.V)
+        k = min(self.knn + 1, len(self.V))
+        d, nb = tree.query(self.V[new], k=k)
+        d, nb = np.atleast_2d(d), np.atleast_2d(nb)
+        cand = []
+        for row, i in enumerate(new):
+            for dist, j in zip(d[row], nb[row]):
+                if j == i or dist > self.connectionThreshold:
+                    continue
+                e = (min(i, j), max(i, j))
+                if e not in self.edges:
+                    self.edges[e] = None
+                    cand.append(e)
+        if self.type == "prm":
+            self._validate(cand)
+
+    def _validate(self, edges):
+        edges = [e for e in edges if self.edges[e] is None]
+        if not edges:
+            return
+        E = np.asarray(edges)
+        ok = self.space.visible_batch(self.V[E[:, 0]], self.V[E[:, 1]])
+        for e, v in zip(edges, ok):
+            self.edges[e] = bool(v)
+        self._stats["edges_checked"] += len(edges)
+        self._stats["check_calls"] += 1
+
+    def planMore(self, iterations):
+        """Draws `iterations` configurations in one batch and extends the
+        roadmap; then looks for a path (validating lazily for 'lazyprm')."""
+        if self.start is None:
+            raise RuntimeError("setEndpoints must be called first")
+        t = time.perf_counter()
+        Q = self.space.sample_batch(int(iterations))
+        ok = self.space.feasible_batch(Q)
+        self._stats["samples"] += len(Q)
+        Q = Q[ok]
+        if len(Q):
+            first = len(self.V)
+            self.V = np.vstack([self.V, Q])
+            self._connect(np.arange(first, len(self.V)))
+        self._search()
+        self._stats["milestones"] = len(self.V)
+        self._stats["time_s"] += time.perf_counter() - t
+
+    def _shortest(self, usable):
+        from scipy.sparse import csr_matrix
+        from scipy.sparse.csgraph import dijkstra
+        es = [e for e, v in self.edges.items() if usable(v)]
+        if not es:
+            return None
+        E = np.asarray(es)
+        w = np.linalg.norm(self.V[E[:, 0]] - self.V[E[:, 1]], axis=1) + 1e-12
+        n = len(self.V)
+        G = csr_matrix((np.r_[w, w], (np.r_[E[:, 0], E[:, 1]], np.r_[E[:, 1], E[:, 0]])), shape=(n, n))
+        dist, pred = dijkstra(G, indices=self.start, return_predecessors=True)
+        if not np.isfinite(dist[self.goal]):
+            return None
+        path = [self.goal]
+        while path[-1] != self.start:
+            path.append(int(pred[path[-1]]))
+        return path[::-1]
+
+    def _search(self):
+        if self.type == "prm":
+            self.path_idx = self._shortest(lambda v: v is True)
+            return
+        while True:
+            p = self._shortest(lambda v: v is not False)
+            if p is None:
+                self.path_idx = None
+                return
+            pe = [(min(a, b), max(a, b)) for a, b in zip(p[:-1], p[1:])]
+            unknown = [e for e in pe if self.edges[e] is None]
+            if not unknown:
+                self.path_idx = p
+                return
+            self._validate(unknown)
+
+    def getPath(self, milestone1=None, milestone2=None):
+        if milestone1 is not None or milestone2 is not None:
+            raise ValueError("paths between arbitrary milestones are not supported")
+        if self.path_idx is None:
+            return None
+        return [list(map(float, self.V[i])) for i in self.path_idx]
+
+    def getRoadmap(self):
+        V = [list(map(float, v)) for v in self.V]
+        E = [e for e, v in self.edges.items() if v is True]
+        return V, E
+
+    def getStats(self):
+        s = dict(self._stats)
+        s["edges"] = len(self.edges)
+        s["edges_valid"] = sum(1 for v in self.edges.values() if v is True)
+        return s
+
+    def pathCost(self, path):
+        return sum(self.space.distance(a, b) for a, b in zip(path[:-1], path[1:]))

@@ -1,0 +1,94 @@
+"""Planner layer on a CPU-only C-space (a disk obstacle in the unit square):
+the C-space semantics of ``Python/klampt/plan/cspace.py`` (named feasibility
+tests, bisection visibility at eps) and the batched PRM / lazy PRM
+planners.  The robot C-space on the GPU is covered by tests/test_gpu_plan.py."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from klampt_amd import plan
+
+
+class Disk(plan.CSpace):
+    def __init__(self, seed=0):
+        super().__init__()
+        self.setBounds([(0.0, 1.0), (0.0, 1.0)])
+        self.eps = 1e-2
+        self.rng = np.random.default_rng(seed)
+        self.addFeasibilityTest(lambda x: (x[0] - 0.5) ** 2 + (x[1] - 0.5) ** 2 > 0.3 ** 2, "disk")
+
+
+def test_bisection_order_matches_the_edge_checker():
+    for n in range(0, 70):
+        got = plan._bisection_order(n)
+        assert sorted(got) == list(range(1, n)), n
+        assert list(got) == list(oracle.bisection_order(n)), n
+
+
+def test_visible_equals_a_scan_of_all_samples():
+    sp = Disk(1)
+    rng = np.random.default_rng(2)
+    A, B = rng.uniform(0, 1, (200, 2)), rng.uniform(0, 1, (200, 2))
+    got = sp.visible_batch(A, B)
+    for s in range(200):
+        n = int(math.ceil(np.linalg.norm(B[s] - A[s]) / sp.eps))
+        want = all(sp.feasible(list(A[s] + (j / n) * (B[s] - A[s]))) for j in range(1, n))
+        assert got[s] == want, s
+    assert got.any() and not got.all()
+
+
+def test_feasibility_tests_and_bounds():
+    sp = Disk()
+    assert sp.feasibilityTestNames == ["disk"]
+    assert not sp.feasible([0.5, 0.5]) and sp.feasible([0.05, 0.05])
+    assert sp.inBounds([0.0, 1.0]) and not sp.inBounds([1.0001, 0.5])
+    assert sp.properties["volume"] == 1.0 and sp.properties["minimum"] == [0.0, 0.0]
+    with pytest.raises(TypeError):
+        sp.addFeasibilityTest(3)
+
+
+@pytest.mark.parametrize("ptype", ["prm", "lazyprm"])
+def test_planner_path_is_feasible(ptype):
+    sp = Disk(3)
+    start, goal = [0.1, 0.5], [0.9, 0.5]
+    assert not sp.visible(start, goal)
+    mp = plan.MotionPlan(sp, type=ptype, knn=8)
+    mp.setEndpoints(start, goal)
+    for _ in range(10):
+        mp.planMore(300)
+        if mp.getPath():
+            break
+    path = mp.getPath()
+    assert path is not None and path[0] == start and path[-1] == goal
+    assert all(sp.feasible(q) for q in path)
+    assert sp.visible_batch(np.asarray(path[:-1]), np.asarray(path[1:])).all()
+    # around the disk: longer than the straight line, shorter than the square's perimeter
+    assert 0.8 < mp.pathCost(path) < 2.0
+    V, E = mp.getRoadmap()
+    assert len(V) == mp.getStats()["milestones"] and all(0 <= i < len(V) and 0 <= j < len(V) for i, j in E)
+
+
+def test_lazy_checks_fewer_edges():
+    stats = {}
+    for ptype in ("prm", "lazyprm"):
+        sp = Disk(4)
+        mp = plan.MotionPlan(sp, type=ptype, knn=8)
+        mp.setEndpoints([0.1, 0.5], [0.9, 0.5])
+        mp.planMore(600)
+        assert mp.getPath() is not None
+        stats[ptype] = mp.getStats()
+    assert stats["lazyprm"]["edges_checked"] < stats["prm"]["edges_checked"] / 3
+
+
+def test_errors():
+    sp = Disk()
+    with pytest.raises(RuntimeError):
+        plan.MotionPlan(sp).setEndpoints([0.5, 0.5], [0.9, 0.5])
+    with pytest.raises(ValueError):
+        plan.MotionPlan(sp, type="rrt*")
+    with pytest.raises(ValueError):
+        plan.MotionPlan(sp, bogus=1)
+    with pytest.raises(RuntimeError):
+        plan.MotionPlan(sp).planMore(10)
