@@ -76,6 +76,14 @@ int kg_geom_create_trimesh(kg_ctx ctx, const double* verts, int nverts, const in
 int kg_geom_create_sdf(kg_ctx ctx, const double bmin[3], const double bmax[3], const int32_t dims[3],
                        const float* values, kg_geom* out);
 int kg_geom_destroy(kg_geom g);
+/* A second handle on the same geometry data, as ManagedGeometry::Load hands a
+ * copy of an already loaded file's geometry to every later loader of that
+ * file (Cpp/Modeling/ManagedGeometry.cpp:111-164): a mesh shares its BVH and
+ * device triangles with src (nothing is rebuilt or re-uploaded); an SDF gets
+ * its own device copy of the grid.  The new handle starts with src's current
+ * transform and collision margin and changes them independently; destroying
+ * either handle leaves the other valid. */
+int kg_geom_clone(kg_geom src, kg_geom* out);
 /* Geometry3D.setCurrentTransform / getCurrentTransform
  * (Python/klampt/src/robotsim.cpp:1465-1473) */
 int kg_geom_set_transform(kg_geom g, const double R[9], const double t[3]);

@@ -887,6 +887,37 @@ int kg_geom_destroy(kg_geom g) {
   });
 }
 
+int kg_geom_clone(kg_geom src, kg_geom* out) {
+  return guard([&] {
+    need(src && out, "NULL argument");
+    kg_ctx_s* c = src->ctx;
+    kg_geom_s* g = new kg_geom_s();
+    g->ctx = c;
+    g->kind = src->kind;
+    g->index = src->index;  // meshes: the same pool descriptor, BVH and triangles
+    g->X = src->X;
+    g->margin = src->margin;
+    g->V = src->V;
+    g->T = src->T;
+    if (src->kind == 1) {
+      for (int k = 0; k < 3; k++) {
+        g->dims[k] = src->dims[k];
+        g->bmin[k] = src->bmin[k];
+        g->bmax[k] = src->bmax[k];
+      }
+      size_t bytes = (size_t)src->dims[0] * src->dims[1] * src->dims[2] * sizeof(float);
+      hipError_t e = hipSetDevice(c->device);
+      if (e == hipSuccess) e = g->dvals.ensure(bytes);
+      if (e == hipSuccess) e = hipMemcpy(g->dvals.p, src->dvals.p, bytes, hipMemcpyDeviceToDevice);
+      if (e != hipSuccess) {
+        delete g;
+        hip_check(e, "kg_geom_clone(sdf)");
+      }
+    }
+    *out = g;
+  });
+}
+
 int kg_geom_set_transform(kg_geom g, const double R[9], const double t[3]) {
   return guard([&] {
     need(g && R && t, "NULL argument");

@@ -99,6 +99,16 @@ class Geom:
         g.dims = tuple(int(x) for x in vals.shape)
         return g
 
+    def clone(self):
+        """Another handle on the same data (kg_geom_clone): meshes share the
+        BVH; the transform and margin start as this handle's."""
+        h = ctypes.c_void_p()
+        call("kg_geom_clone", self.h, ctypes.byref(h))
+        g = Geom(self.ctx, h, self.kind)
+        if hasattr(self, "dims"):
+            g.dims = self.dims
+        return g
+
     def set_transform(self, R, t):
         call("kg_geom_set_transform", self.h, _p(_d(R), _pd), _p(_d(t), _pd))
 
@@ -298,6 +308,10 @@ class World:
              ctypes.byref(out))
         self._keep.append(geom)
         return out.value
+
+    def set_rigid_object_transform(self, index, xf12):
+        x = _d(xf12)
+        call("kg_world_set_rigid_object_transform", self.h, index, _p(x[:9].copy(), _pd), _p(x[9:].copy(), _pd))
 
     def add_robot(self, robot):
         out = ctypes.c_int32()

@@ -20,9 +20,33 @@ from typing import List, Optional, Sequence
 import numpy as np
 
 from . import engine
+from . import io as _io
 from ._lib import KlamptGPUError
 
 _default_ctx = None
+_geom_cache = {}
+
+
+def _device_geom(ctx, mesh):
+    """Device geometry for a host mesh.  Meshes loaded from files carry a key
+    (file identity plus any baked transform); the first load of a key builds
+    the BVH and later loads get a handle sharing it (``kg_geom_clone``), as
+    ``ManagedGeometry::Load`` hands out copies of a cached file
+    (``Cpp/Modeling/ManagedGeometry.cpp:111-164``)."""
+    if getattr(mesh, "key", None) is None:
+        return engine.Geom.trimesh(ctx, mesh.verts, mesh.tris)
+    k = (id(ctx), mesh.key)
+    proto = _geom_cache.get(k)
+    if proto is None or proto.ctx is not ctx:
+        proto = engine.Geom.trimesh(ctx, mesh.verts, mesh.tris)
+        _geom_cache[k] = proto
+    return proto.clone()
+
+
+def clear_geometry_cache():
+    """Forget the file-geometry prototypes (handles already given out stay
+    valid)."""
+    _geom_cache.clear()
 
 
 def default_context():
@@ -90,6 +114,10 @@ class Geometry3D:
             self._g = engine.Geom.trimesh(self._ctx, data.vertices, data.indices)
             self._type = "TriangleMesh"
             self._mesh = data
+        elif hasattr(data, "verts") and hasattr(data, "tris"):
+            self._g = _device_geom(self._ctx, data)
+            self._type = "TriangleMesh"
+            self._mesh = TriangleMesh(data.verts, data.tris)
         elif isinstance(data, ImplicitSurface):
             self._g = engine.Geom.sdf(self._ctx, data.bmin, data.bmax, data.values)
             self._type = "ImplicitSurface"
@@ -98,6 +126,25 @@ class Geometry3D:
 
     def type(self) -> str:
         return self._type
+
+    def loadFile(self, fn: str) -> bool:
+        """Load a triangle mesh (.off, .obj, .stl); False when the file cannot
+        be read (``Geometry3D::loadFile``, robotsim.cpp:1433-1460)."""
+        try:
+            mesh = _io.load_mesh(fn)
+        except (_io.LoadError, OSError):
+            return False
+        self.set(mesh)
+        return True
+
+    def saveFile(self, fn: str) -> bool:
+        if self._type != "TriangleMesh":
+            return False
+        try:
+            _io.save_mesh(fn, _io.S.Mesh(self._mesh.vertices, self._mesh.indices))
+        except (_io.LoadError, OSError):
+            return False
+        return True
 
     def empty(self) -> bool:
         return self._g is None or (self._type == "TriangleMesh" and self._mesh.indices.shape[0] == 0)
@@ -181,6 +228,12 @@ class RobotModelLink:
         R, t = self._robot._r.link_transform(self.index)
         return list(R), list(t)
 
+    def getName(self) -> str:
+        return self._robot._names[self.index]
+
+    def setName(self, name: str):
+        self._robot._names[self.index] = name
+
 
 class RobotModel:
     """Kinematic tree + link geometries (``RobotModel``, robotmodel.h:537)."""
@@ -191,19 +244,51 @@ class RobotModel:
         self._r = engine.Robot(self._ctx, parents, T0, axes, jtype, qmin, qmax)
         self._qmin, self._qmax = list(map(float, qmin)), list(map(float, qmax))
         self._geoms: List[Optional[Geometry3D]] = [Geometry3D(ctx=self._ctx) for _ in range(len(parents))]
+        self._names = ["Link_%d" % i for i in range(len(parents))]
+        self._name = "Robot"
 
     @classmethod
     def from_spec(cls, spec, ctx=None):
         rob = cls(spec.parents, spec.T0, spec.axes, spec.jtype, spec.qmin, spec.qmax, ctx)
+        rob._name = getattr(spec, "name", "Robot") or "Robot"
         for l, m in enumerate(spec.link_meshes):
             if m is not None:
-                rob.link(l).setGeometry(Geometry3D(TriangleMesh(m.verts, m.tris), ctx=rob._ctx))
+                rob.link(l).setGeometry(Geometry3D(m, ctx=rob._ctx))
+        if spec.self_pairs is not None:
+            rob._r.set_self_pairs(np.asarray(spec.self_pairs, np.int32).reshape(-1, 2))
         return rob
+
+    @classmethod
+    def from_file(cls, rf, ctx=None):
+        """A robot from ``klampt_amd.io.RobotFile`` (link names, margins and
+        the file's configuration applied)."""
+        rob = cls.from_spec(rf.spec, ctx)
+        rob._name = rf.name
+        if rf.link_names:
+            rob._names = list(rf.link_names)
+        if rf.margins is not None:
+            for l, m in enumerate(rf.margins):
+                if m > 0 and not rob._geoms[l].empty():
+                    rob._geoms[l].setCollisionMargin(float(m))
+        if rf.q is not None:
+            rob.setConfig(list(rf.q))
+        return rob
+
+    def getName(self) -> str:
+        return self._name
+
+    def setName(self, name: str):
+        self._name = name
 
     def numLinks(self) -> int:
         return self._r.n
 
-    def link(self, i: int) -> RobotModelLink:
+    def link(self, i) -> RobotModelLink:
+        """Link by index or by name (``RobotModel.link``)."""
+        if isinstance(i, str):
+            if i not in self._names:
+                raise IndexError("invalid link name %s" % i)
+            i = self._names.index(i)
         if not 0 <= i < self._r.n:
             raise IndexError("invalid link index")
         return RobotModelLink(self, i)
@@ -229,6 +314,45 @@ class RobotModel:
         return self._r.self_collision_enabled(link1, link2)
 
 
+class TerrainModel:
+    """A world's terrain: name, geometry and world ID."""
+
+    def __init__(self, world: "WorldModel", index: int, geom: Geometry3D, name: str):
+        self.world, self.index, self._geom, self._name = world, index, geom, name
+
+    def getName(self) -> str:
+        return self._name
+
+    def getID(self) -> int:
+        return self.index
+
+    def geometry(self) -> Geometry3D:
+        return self._geom
+
+
+class RigidObjectModel:
+    """A world's rigid object: name, geometry, transform and world ID."""
+
+    def __init__(self, world: "WorldModel", index: int, geom: Geometry3D, name: str):
+        self.world, self.index, self._geom, self._name = world, index, geom, name
+
+    def getName(self) -> str:
+        return self._name
+
+    def getID(self) -> int:
+        return len(self.world._terrains) + self.index
+
+    def geometry(self) -> Geometry3D:
+        return self._geom
+
+    def getTransform(self):
+        return list(self._xf[:9]), list(self._xf[9:])
+
+    def setTransform(self, R, t):
+        self._xf = np.r_[np.asarray(R, float), np.asarray(t, float)]
+        self.world._w.set_rigid_object_transform(self.index, self._xf)
+
+
 class WorldModel:
     """Terrains, rigid objects and robots with Klampt's ID layout
     (``Cpp/Modeling/World.cpp:47-53,156-180``)."""
@@ -237,16 +361,130 @@ class WorldModel:
         self._ctx = ctx or default_context()
         self._w = engine.World(self._ctx)
         self._terrains, self._objects, self._robots = [], [], []
+        self._terrain_models, self._object_models = [], []
 
-    def addTerrain(self, geom: Geometry3D) -> int:
+    def addTerrain(self, geom: Geometry3D, name: str = "Terrain") -> int:
         self._terrains.append(geom)
+        self._terrain_models.append(TerrainModel(self, len(self._terrains) - 1, geom, name))
         return self._w.add_terrain(geom._g)
 
-    def addRigidObject(self, geom: Geometry3D, R=None, t=None) -> int:
+    def addRigidObject(self, geom: Geometry3D, R=None, t=None, name: str = "Object") -> int:
         R = [1, 0, 0, 0, 1, 0, 0, 0, 1] if R is None else R
         t = [0, 0, 0] if t is None else t
         self._objects.append(geom)
-        return self._w.add_rigid_object(geom._g, np.r_[np.asarray(R, float), np.asarray(t, float)])
+        m = RigidObjectModel(self, len(self._objects) - 1, geom, name)
+        m._xf = np.r_[np.asarray(R, float), np.asarray(t, float)]
+        self._object_models.append(m)
+        return self._w.add_rigid_object(geom._g, m._xf)
+
+    def numTerrains(self) -> int:
+        return len(self._terrains)
+
+    def numRigidObjects(self) -> int:
+        return len(self._objects)
+
+    def terrain(self, i) -> TerrainModel:
+        return self._by_name(self._terrain_models, i)
+
+    def rigidObject(self, i) -> RigidObjectModel:
+        return self._by_name(self._object_models, i)
+
+    @staticmethod
+    def _by_name(items, i):
+        if isinstance(i, str):
+            for m in items:
+                if m.getName() == i:
+                    return m
+            raise IndexError("no element named %s" % i)
+        return items[i]
+
+    # ---- files (``WorldModel::readFile`` / ``loadRobot`` / ``loadRigidObject``
+    # / ``loadTerrain`` / ``loadElement``, robotsim.cpp:4093-4175,4363-4398;
+    # ``WorldModel::LoadElement``, Cpp/Modeling/World.cpp:751-787)
+
+    def _add_robot_file(self, rf) -> "RobotModel":
+        rob = RobotModel.from_file(rf, self._ctx)
+        rob._world_id = self.addRobot(rob)
+        return rob
+
+    def _add_object_file(self, of) -> RigidObjectModel:
+        g = Geometry3D(of.mesh, ctx=self._ctx)
+        if of.margin > 0:
+            g.setCollisionMargin(of.margin)
+        R, t = _io.xf_mat(of.xf)
+        self.addRigidObject(g, list(R.T.reshape(9)), list(t), name=of.name)
+        return self._object_models[-1]
+
+    def _add_terrain_file(self, tf) -> TerrainModel:
+        g = Geometry3D(tf.mesh, ctx=self._ctx)
+        if tf.margin > 0:
+            g.setCollisionMargin(tf.margin)
+        self.addTerrain(g, name=tf.name)
+        return self._terrain_models[-1]
+
+    def loadRobot(self, fn: str) -> Optional["RobotModel"]:
+        """Robot from .rob / .urdf; None when it cannot be loaded (the
+        reference returns an unbound model)."""
+        try:
+            return self._add_robot_file(_io.load_robot(fn))
+        except (_io.LoadError, OSError):
+            return None
+
+    def loadRigidObject(self, fn: str) -> Optional[RigidObjectModel]:
+        try:
+            return self._add_object_file(_io.load_rigid_object(fn))
+        except (_io.LoadError, OSError):
+            return None
+
+    def loadTerrain(self, fn: str) -> Optional[TerrainModel]:
+        try:
+            return self._add_terrain_file(_io.load_terrain(fn))
+        except (_io.LoadError, OSError):
+            return None
+
+    def loadElement(self, fn: str) -> int:
+        """Robot, rigid object or terrain by extension; its world ID, or -1.
+        A .obj that is not a Klampt object file is tried as a terrain mesh."""
+        ext = _io._ext(fn)
+        if ext in ("rob", "urdf"):
+            r = self.loadRobot(fn)
+            return -1 if r is None else r._world_id
+        if ext == "obj":
+            o = self.loadRigidObject(fn)
+            if o is not None:
+                return o.getID()
+        if ext in ("env", "off", "obj", "stl"):
+            t = self.loadTerrain(fn)
+            return -1 if t is None else t.getID()
+        return -1
+
+    def readFile(self, fn: str) -> bool:
+        """A world XML, robot, rigid object or terrain file into this world;
+        False (and nothing added) when it cannot be loaded."""
+        ext = _io._ext(fn)
+        try:
+            if ext in ("rob", "urdf"):
+                self._add_robot_file(_io.load_robot(fn))
+            elif ext in ("env", "tri", "pcd"):
+                self._add_terrain_file(_io.load_terrain(fn))
+            elif ext == "obj":
+                self._add_object_file(_io.load_rigid_object(fn))
+            elif ext == "xml":
+                wf = _io.load_world(fn)
+                for tf in wf.terrains:
+                    self._add_terrain_file(tf)
+                for of in wf.objects:
+                    self._add_object_file(of)
+                for rf in wf.robots:
+                    self._add_robot_file(rf)
+            else:
+                return False
+        except (_io.LoadError, OSError):
+            return False
+        return True
+
+    def loadFile(self, fn: str) -> bool:
+        return self.readFile(fn)
 
     def addRobot(self, robot: RobotModel) -> int:
         self._robots.append(robot)

@@ -67,6 +67,9 @@ def random_rotation(rng):
 class Mesh:
     verts: np.ndarray   # (V,3) float64, float32-representable
     tris: np.ndarray    # (T,3) int32
+    # identity of the file data this mesh came from (None: built in memory);
+    # meshes with equal keys may share one device copy
+    key: Optional[tuple] = field(default=None, compare=False, repr=False)
 
     @property
     def ntris(self):

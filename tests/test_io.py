@@ -1,0 +1,340 @@
+"""On-disk formats (CPU): the reference's own mesh and rigid-object fixtures
+(tests/objects/cube.off, tests/objects/block.obj, copied as data), known
+answers for the transform conventions (row-major TParent, URDF rpy,
+rotateMoment), the .rob line syntax and size rules, URDF tree order, world
+XML, and write/read round trips."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from klampt_amd import io, synthetic as S
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "objects")
+
+RZ90 = np.array([[0, -1, 0], [1, 0, 0], [0, 0, 1]], float)
+
+
+def _volume(m):
+    P = m.verts[m.tris]
+    return float(np.einsum("ij,ij->i", P[:, 0], np.cross(P[:, 1], P[:, 2])).sum() / 6)
+
+
+def _closed(m):
+    E = np.sort(np.concatenate([m.tris[:, [0, 1]], m.tris[:, [1, 2]], m.tris[:, [2, 0]]]), axis=1)
+    _, c = np.unique(E, axis=0, return_counts=True)
+    return bool((c == 2).all())
+
+
+def _R(x):
+    return io.xf_mat(x)[0]
+
+
+# ---------------------------------------------------------------- meshes
+
+def test_reference_cube_off():
+    m = io.load_mesh(os.path.join(GOLD, "cube.off"))
+    assert m.verts.shape == (8, 3) and m.tris.shape == (12, 3)
+    assert np.array_equal(m.verts.min(0), [0, 0, 0]) and np.array_equal(m.verts.max(0), [1, 1, 1])
+    assert _closed(m)
+    assert abs(abs(_volume(m)) - 1.0) < 1e-12
+
+
+def test_reference_block_object_file():
+    """block.obj: mesh "cube.off", geomscale 0.4, geomtranslate -0.2, T with
+    translation (1, 0, 0.3) (RigidObject.cpp:58-120)."""
+    o = io.load_rigid_object(os.path.join(GOLD, "block.obj"))
+    assert np.allclose(o.mesh.verts.min(0), [-0.2] * 3, atol=1e-15)
+    assert np.allclose(o.mesh.verts.max(0), [0.2] * 3, atol=1e-15)
+    assert abs(abs(_volume(o.mesh)) - 0.4 ** 3) < 1e-12
+    R, t = io.xf_mat(o.xf)
+    assert np.array_equal(R, np.eye(3)) and np.array_equal(t, [1.0, 0.0, 0.3])
+    assert o.name == "block"
+
+
+@pytest.mark.parametrize("ext", ["off", "obj", "stl"])
+def test_mesh_round_trip(tmp_path, ext):
+    m = S.ellipsoid_mesh([0.1, 0.2, 0.15], edge=0.05)
+    p = str(tmp_path / ("m." + ext))
+    io.save_mesh(p, m)
+    r = io.load_mesh(p)
+    assert r.tris.shape == m.tris.shape
+    # same triangles (STL re-indexes vertices; coordinates are float32 there
+    # and the ellipsoid's are float32-representable)
+    a = np.sort(m.verts[m.tris].reshape(-1, 9), axis=0)
+    b = np.sort(r.verts[r.tris].reshape(-1, 9), axis=0)
+    assert np.array_equal(a, b)
+    if ext != "stl":
+        assert np.array_equal(r.verts, m.verts) and np.array_equal(r.tris, m.tris)
+
+
+def test_wavefront_polygons_and_index_forms(tmp_path):
+    p = tmp_path / "q.obj"
+    p.write_text("# quad and a triangle\nv 0 0 0\nv 1 0 0\nv 1 1 0\nv 0 1 0\nvt 0 0\nvn 0 0 1\n"
+                 "f 1/1/1 2/1/1 3//1 4\nv 0 0 1\nf -1 1 2\n")
+    m = io.load_mesh(str(p))
+    assert m.tris.tolist() == [[0, 1, 2], [0, 2, 3], [4, 0, 1]]
+    assert abs(sum(0.5 * np.linalg.norm(np.cross(*(m.verts[t[1:]] - m.verts[t[0]]))) for t in m.tris) - 1.5) < 1e-15
+
+
+def test_ascii_stl_and_off_variants(tmp_path):
+    p = tmp_path / "t.stl"
+    p.write_text("solid t\nfacet normal 0 0 1\n outer loop\n  vertex 0 0 0\n  vertex 1 0 0\n  vertex 0 1 0\n"
+                 " endloop\nendfacet\nfacet normal 0 0 1\n outer loop\n  vertex 1 0 0\n  vertex 1 1 0\n"
+                 "  vertex 0 1 0\n endloop\nendfacet\nendsolid t\n")
+    m = io.load_mesh(str(p))
+    assert m.verts.shape == (4, 3) and m.tris.shape == (2, 3)   # shared corners welded
+    p = tmp_path / "c.off"
+    p.write_text("COFF 4 1 0\n0 0 0 1 0 0 1\n1 0 0 1 0 0 1\n1 1 0 1 0 0 1\n0 1 0 1 0 0 1\n4 0 1 2 3 255 0 0\n")
+    m = io.load_mesh(str(p))
+    assert m.tris.tolist() == [[0, 1, 2], [0, 2, 3]]
+    p = tmp_path / "bad.off"
+    p.write_text("OFF\n3 1 0\n0 0 0\n1 0 0\n0 1 0\n3 0 1 5\n")
+    with pytest.raises(io.LoadError, match="out of range"):
+        io.load_mesh(str(p))
+
+
+def test_mesh_cache_key_tracks_the_file(tmp_path):
+    p = str(tmp_path / "a.off")
+    io.save_mesh(p, S.box_mesh([0.1, 0.1, 0.1]))
+    a, b = io.load_mesh(p), io.load_mesh(p)
+    assert a.key == b.key and a.key is not None
+    assert io.keyed_transform(a, np.diag([2, 2, 2, 1.0])).key == io.keyed_transform(b, np.diag([2, 2, 2, 1.0])).key
+    assert io.keyed_transform(a, np.diag([2, 2, 2, 1.0])).key != io.keyed_transform(a, np.diag([3, 3, 3, 1.0])).key
+    os.utime(p, ns=(1, 1))
+    assert io.load_mesh(p).key != a.key
+
+
+# ---------------------------------------------------------------- rotations
+
+def test_rotation_conventions_known_answers():
+    # rpy (pi/2, 0, pi/2) = Rz(90) Rx(90): x -> y, y -> z, z -> x
+    want = np.array([[0, 0, 1], [1, 0, 0], [0, 1, 0]], float)
+    assert np.allclose(io.rot_rpy(math.pi / 2, 0, math.pi / 2), want, atol=1e-15)
+    assert np.allclose(io.rot_moment([0, 0, math.pi / 2]), RZ90, atol=1e-15)
+    w = np.array([0.3, -0.5, 0.7])
+    R = io.rot_moment(w)
+    assert np.allclose(R @ w, w) and np.allclose(R @ R.T, np.eye(3))
+    assert abs(math.acos((np.trace(R) - 1) / 2) - np.linalg.norm(w)) < 1e-12
+
+
+# ---------------------------------------------------------------- .rob
+
+ROB = r'''# three links, the middle one prismatic
+links "base link" mid tip   # quoted name with a space
+parents -1 0 1
+jointType r p r
+TParent 0 -1 0  1 0 0  0 0 1   0 0 0.1 \
+        1 0 0  0 1 0  0 0 1   0 0 0.3 \
+        1 0 0  0 1 0  0 0 1   0.2 0 0
+axis 0 0 1  1 0 0  0 1 0
+qminDeg -90 -0.5 -45
+qmaxdeg 90 0.5 45
+qdeg 10 0.1 0
+translation 1 2 3
+rotation 0 -1 0 1 0 0 0 0 1
+geometry "cube.off" "" "cube.off"
+geomscale 0.5
+geomtransform 2  1 0 0 0.25  0 1 0 0  0 0 1 0  0 0 0 1
+noselfcollision "base link" tip
+joint normal 0
+joint normal 1
+joint weld 2
+'''
+
+
+def _write(tmp_path, name, text):
+    p = tmp_path / name
+    p.write_text(text)
+    return str(p)
+
+
+def _cube(tmp_path):
+    io.save_mesh(str(tmp_path / "cube.off"), io.load_mesh(os.path.join(GOLD, "cube.off")))
+
+
+def test_rob_file(tmp_path):
+    _cube(tmp_path)
+    rf = io.load_rob(_write(tmp_path, "r.rob", ROB))
+    sp = rf.spec
+    assert rf.link_names == ["base link", "mid", "tip"]
+    assert sp.parents.tolist() == [-1, 0, 1] and sp.jtype.tolist() == [0, 1, 0]
+    # degrees converted except on the prismatic joint
+    assert np.allclose(sp.qmin, [-math.pi / 2, -0.5, -math.pi / 4]) and np.allclose(sp.qmax, [math.pi / 2, 0.5, math.pi / 4])
+    assert np.allclose(rf.q, [math.radians(10), math.radians(0.1), 0])   # qdeg converts every entry
+    # row-major TParent, then the base rotation / translation on the root only
+    R0, t0 = io.xf_mat(sp.T0[0])
+    assert np.allclose(R0, RZ90 @ RZ90) and np.allclose(t0, RZ90 @ [0, 0, 0.1] + [1, 2, 3])
+    assert np.allclose(io.xf_mat(sp.T0[2])[1], [0.2, 0, 0])
+    assert np.allclose(sp.axes, [[0, 0, 1], [1, 0, 0], [0, 1, 0]])
+    # geomscale 0.5 on both meshes, geomtransform shifts link 2 by 0.25 in x
+    assert sp.link_meshes[1] is None and rf.geom_files[1] == ""
+    assert np.allclose(sp.link_meshes[0].verts.max(0), [0.5, 0.5, 0.5])
+    assert np.allclose(sp.link_meshes[2].verts.min(0), [0.25, 0, 0]) and np.allclose(sp.link_meshes[2].verts.max(0), [0.75, 0.5, 0.5])
+    # default pairs minus link-parent (none left but 0-2), then noselfcollision removes it
+    assert sp.self_pairs.shape == (0, 2)
+    assert rf.joints == [("normal", 0), ("normal", 1), ("weld", 2)]
+
+
+def test_rob_defaults_and_selfcollision(tmp_path):
+    rf = io.load_rob(_write(tmp_path, "d.rob", "TParent " + " ".join(["1 0 0 0 1 0 0 0 1 0 0 0.1"] * 4)
+                            + "\nselfcollision 0 3 3 1\n"))
+    sp = rf.spec
+    assert sp.parents.tolist() == [-1, 0, 1, 2] and rf.link_names == ["Link_%d" % i for i in range(4)]
+    assert np.all(np.isinf(sp.qmin)) and np.all(sp.qmin < 0) and np.all(np.isinf(sp.qmax))
+    assert np.allclose(sp.axes, [[0, 0, 1]] * 4) and np.array_equal(rf.q, np.zeros(4))
+    assert sp.self_pairs.tolist() == [[0, 3], [1, 3]]
+
+
+def test_rob_line_syntax():
+    lines = io._rob_lines("a 1 \\\n 2 # c \\\nb \\x # \\\n\\#c d\r\n")
+    assert lines == ["a 1 \n 2 ", "b \\x ", "\\#c d\r"]
+
+
+@pytest.mark.parametrize("text,msg", [
+    ("parents -1 0\nTParent 1 0 0 0 1 0 0 0 1 0 0 0\naxis 0 0 1\njointtype r\nqmin 0\n", "wrong number"),
+    ("alpha 0 0\na 0 0\nd 0 0\ntheta 0 0\n", "D-H"),
+    ("parents -1\n", "no D-H parameters or link transforms"),
+    ("TParent 1 0 0 0 1 0 0 0 1 0 0 0\nmount 0 \"x.rob\"\n", "mount"),
+    ("TParent 1 0 0 0 1 0 0 0 1 0 0 0\nfrobnicate 1\n", "invalid robot property"),
+    ("TParent 1 0 0 0 1 0 0 0 1 0 0 0\njointtype q\n", "invalid joint type"),
+    ("parents 1 -1\nTParent " + " ".join(["1 0 0 0 1 0 0 0 1 0 0 0"] * 2) + "\n", "parents must precede"),
+    ("TParent " + " ".join(["1 0 0 0 1 0 0 0 1 0 0 0"] * 2) + "\nselfcollision 0 5\n", "invalid self-collision"),
+])
+def test_rob_errors(tmp_path, text, msg):
+    with pytest.raises(io.LoadError, match=msg):
+        io.load_rob(_write(tmp_path, "e.rob", text))
+
+
+def test_rob_round_trip(tmp_path):
+    robots = S.hetero_robots(seed=3, nrobots=2, dof_range=(8, 14), tri_range=(200, 400))[0]
+    for k, sp in enumerate(robots):
+        files = []
+        for l, m in enumerate(sp.link_meshes):
+            if m is None:
+                files.append("")
+                continue
+            fn = str(tmp_path / ("r%d_l%d.off" % (k, l)))
+            io.save_mesh(fn, m)
+            files.append(fn)
+        sp.self_pairs = sp.default_self_pairs()[::2]
+        rf = io.RobotFile(sp, link_names=["L%d" % i for i in range(sp.n)], q=np.linspace(-0.1, 0.1, sp.n),
+                          geom_files=files)
+        p = str(tmp_path / ("r%d.rob" % k))
+        io.save_rob(p, rf)
+        back = io.load_rob(p)
+        for f in ("parents", "T0", "axes", "jtype", "qmin", "qmax", "self_pairs"):
+            assert np.array_equal(getattr(back.spec, f), np.asarray(getattr(sp, f))), f
+        assert back.link_names == rf.link_names and np.array_equal(back.q, rf.q)
+        for a, b in zip(back.spec.link_meshes, sp.link_meshes):
+            assert (a is None) == (b is None)
+            if a is not None:
+                assert np.array_equal(a.verts, b.verts) and np.array_equal(a.tris, b.tris)
+
+
+# ---------------------------------------------------------------- URDF
+
+URDF = '''<?xml version="1.0"?>
+<robot name="toy">
+  <link name="world"/>
+  <link name="base"><collision><origin xyz="0 0 0.05"/><geometry><box size="0.2 0.2 0.1"/></geometry></collision></link>
+  <link name="arm_b"><collision><origin xyz="0.1 0 0" rpy="0 0 1.5707963267948966"/>
+      <geometry><mesh filename="cube.off" scale="0.1 0.2 0.3"/></geometry></collision></link>
+  <link name="arm_a"/>
+  <link name="slider"/>
+  <link name="tool"/>
+  <joint name="j0" type="fixed"><parent link="world"/><child link="base"/><origin xyz="1 0 0"/></joint>
+  <joint name="z_second" type="revolute"><parent link="base"/><child link="arm_b"/>
+      <origin xyz="0 0 0.1" rpy="1.5707963267948966 0 1.5707963267948966"/><axis xyz="0 1 0"/>
+      <limit lower="-1" upper="2" effort="1" velocity="1"/></joint>
+  <joint name="a_first" type="continuous"><parent link="base"/><child link="arm_a"/><origin xyz="0 0.1 0"/></joint>
+  <joint name="p" type="prismatic"><parent link="arm_b"/><child link="slider"/><axis xyz="0 0 1"/>
+      <limit lower="0" upper="0.5" effort="1" velocity="1"/></joint>
+  <joint name="f" type="fixed"><parent link="arm_a"/><child link="tool"/></joint>
+  <klampt><noselfcollision pairs="base slider"/></klampt>
+</robot>
+'''
+
+
+def test_urdf_fixed_base(tmp_path):
+    _cube(tmp_path)
+    rf = io.load_urdf(_write(tmp_path, "toy.urdf", URDF))
+    sp = rf.spec
+    # depth first from world; siblings in joint-name order (a_first < z_second)
+    assert rf.link_names == ["base", "arm_a", "tool", "arm_b", "slider"]
+    assert sp.parents.tolist() == [-1, 0, 1, 0, 3]
+    assert sp.jtype.tolist() == [0, 0, 0, 0, 1]
+    R3, t3 = io.xf_mat(sp.T0[3])
+    assert np.allclose(R3, [[0, 0, 1], [1, 0, 0], [0, 1, 0]], atol=1e-15) and np.allclose(t3, [0, 0, 0.1])
+    assert np.allclose(io.xf_mat(sp.T0[0])[1], [1, 0, 0])
+    # axes: given, default (1,0,0) for a revolute/continuous joint without <axis>, z for fixed
+    assert np.allclose(sp.axes, [[0, 0, 1], [1, 0, 0], [0, 0, 1], [0, 1, 0], [0, 0, 1]])
+    assert np.array_equal(sp.qmin, [0, -np.inf, 0, -1, 0]) and np.array_equal(sp.qmax, [0, np.inf, 0, 2, 0.5])
+    assert rf.joints[0] == ("weld", 0) and ("spin", 1) in rf.joints and ("weld", 2) in rf.joints
+    # box primitive at its origin; mesh scaled, then rotated and shifted by the collision origin
+    assert np.allclose(sp.link_meshes[0].verts.min(0), [-0.1, -0.1, 0]) and np.allclose(sp.link_meshes[0].verts.max(0), [0.1, 0.1, 0.1])
+    m = sp.link_meshes[3]
+    assert np.allclose(m.verts.min(0), [-0.1, 0, 0]) and np.allclose(m.verts.max(0), [0.1, 0.1, 0.3])
+    assert sp.link_meshes[1] is None and sp.link_meshes[4] is None
+    pairs = set(map(tuple, sp.self_pairs.tolist()))
+    assert (0, 4) not in pairs and (0, 1) not in pairs and (1, 3) in pairs and (2, 4) in pairs
+
+
+def test_urdf_floating_base(tmp_path):
+    text = URDF.replace('<link name="world"/>', "").replace(
+        '<joint name="j0" type="fixed"><parent link="world"/><child link="base"/><origin xyz="1 0 0"/></joint>', "")
+    _cube(tmp_path)
+    rf = io.load_urdf(_write(tmp_path, "f.urdf", text))
+    assert rf.link_names[:6] == ["base0", "base1", "base2", "base3", "base4", "base"]
+    assert rf.spec.jtype[:6].tolist() == [1, 1, 1, 0, 0, 0]
+    assert np.allclose(rf.spec.axes[:6], [[1, 0, 0], [0, 1, 0], [0, 0, 1], [0, 0, 1], [0, 1, 0], [1, 0, 0]])
+    assert rf.spec.parents.tolist() == [-1, 0, 1, 2, 3, 4, 5, 6, 5, 8]
+    assert rf.joints[0] == ("floating", 5, -1)
+
+
+def test_urdf_errors(tmp_path):
+    with pytest.raises(io.LoadError, match="does not specify limits"):
+        io.load_urdf(_write(tmp_path, "e1.urdf", URDF.replace('<limit lower="-1" upper="2" effort="1" velocity="1"/>', "")))
+    with pytest.raises(io.LoadError, match="two root links"):
+        io.load_urdf(_write(tmp_path, "e2.urdf", URDF.replace('<link name="tool"/>', '<link name="tool"/><link name="x"/>')))
+
+
+# ---------------------------------------------------------------- world XML
+
+def test_world_xml(tmp_path):
+    _cube(tmp_path)
+    _write(tmp_path, "r.rob", ROB)
+    (tmp_path / "block.obj").write_text(open(os.path.join(GOLD, "block.obj")).read())
+    wf = io.load_world(_write(tmp_path, "w.xml", '''<world>
+  <robot name="arm" file="r.rob" config="3  0.1 0.2 0.3" rotateZ="1.5707963267948966" position="0 0 1"/>
+  <rigidObject name="blk" file="block.obj"/>
+  <rigidObject name="c2" position="0 1 0" rotateMoment="0 0 1.5707963267948966">
+    <geometry mesh="cube.off" scale="0.5" translation="-0.25 -0.25 -0.25" margin="0.01"/>
+  </rigidObject>
+  <terrain file="cube.off" scale="4 4 0.1" position="-2 -2 -0.1"/>
+</world>'''))
+    assert len(wf.robots) == 1 and len(wf.objects) == 2 and len(wf.terrains) == 1
+    rf = wf.robots[0]
+    assert rf.name == "arm" and np.array_equal(rf.q, [0.1, 0.2, 0.3])
+    R0, t0 = io.xf_mat(rf.spec.T0[0])
+    assert np.allclose(R0, RZ90 @ RZ90 @ RZ90, atol=1e-15) and np.allclose(t0, RZ90 @ (RZ90 @ [0, 0, 0.1] + [1, 2, 3]) + [0, 0, 1])
+    blk, c2 = wf.objects
+    assert np.allclose(io.xf_mat(blk.xf)[1], [1, 0, 0.3]) and np.allclose(blk.mesh.verts.max(0), [0.2] * 3)
+    R, t = io.xf_mat(c2.xf)
+    assert np.allclose(R, RZ90, atol=1e-15) and np.array_equal(t, [0, 1, 0]) and c2.margin == 0.01
+    assert np.allclose(c2.mesh.verts.min(0), [-0.25] * 3) and np.allclose(c2.mesh.verts.max(0), [0.25] * 3)
+    ter = wf.terrains[0].mesh
+    assert np.allclose(ter.verts.min(0), [-2, -2, -0.1]) and np.allclose(ter.verts.max(0), [2, 2, 0])
+    sc = wf.scene()
+    assert sc.robot is rf.spec and len(sc.objects) == 2 and len(sc.terrains) == 1
+
+
+def test_world_xml_errors(tmp_path):
+    _write(tmp_path, "r.rob", ROB.replace('geometry "cube.off" "" "cube.off"\n', ""))
+    with pytest.raises(io.LoadError, match="configuration size"):
+        io.load_world(_write(tmp_path, "w1.xml", '<world><robot file="r.rob" config="2 0 0"/></world>'))
+    with pytest.raises(io.LoadError, match="no geometry"):
+        io.load_world(_write(tmp_path, "w2.xml", '<world><rigidObject name="x"/></world>'))
+    with pytest.raises(io.LoadError, match="not <world>"):
+        io.load_world(_write(tmp_path, "w3.xml", "<robot/>"))
