@@ -375,6 +375,62 @@ def run_secondary(args, ctx, sc, world, robot):
         out["planner"]["cpu_edges_per_s"] = round(len(E) / tc, 1)
         out["planner"]["cpu_sample"] = f"first {len(E)} roadmap edges, FP64 oracle, {oracle.nthreads_default()} threads"
     print(f"[bench] planner: {out['planner']}", file=sys.stderr, flush=True)
+
+    # on-disk formats: the headline world as files (arm .rob + link .off,
+    # 200k-triangle .off terrain, world XML) loaded to a first answer, and 64
+    # rigid objects from one mesh file (one BVH build, 63 shared handles)
+    # against 64 separately built meshes
+    import shutil
+    import tempfile
+    from klampt_amd import io as kio
+    tmp = tempfile.mkdtemp(prefix="kg_bench_io_")
+    try:
+        files = []
+        for l, m in enumerate(sc.robot.link_meshes):
+            files.append(os.path.join(tmp, "link%d.off" % l))
+            kio.save_mesh(files[-1], m)
+        kio.save_rob(os.path.join(tmp, "arm.rob"), kio.RobotFile(sc.robot, geom_files=files))
+        kio.save_mesh(os.path.join(tmp, "soup.off"), sc.terrains[0])
+        with open(os.path.join(tmp, "world.xml"), "w") as f:
+            f.write('<world>\n  <robot name="arm" file="arm.rob"/>\n  <terrain file="soup.off"/>\n</world>\n')
+        robotsim.clear_geometry_cache()
+        kio._MESH_CACHE.clear()
+        t0 = time.perf_counter()
+        wf = kio.load_world(os.path.join(tmp, "world.xml"))
+        tparse = time.perf_counter() - t0
+        kio._MESH_CACHE.clear()
+        t0 = time.perf_counter()
+        Wl = robotsim.WorldModel()
+        assert Wl.loadFile(os.path.join(tmp, "world.xml"))
+        tload = time.perf_counter() - t0
+        q1 = S.uniform_configs(sc.robot, 1, seed=9)
+        robotsim.RobotCSpace(Wl.robot(0), Wl).feasible_batch(q1)
+        torch.cuda.synchronize()
+        tfirst = time.perf_counter() - t0
+        box = S.ellipsoid_mesh([0.1, 0.15, 0.2], edge=0.01)
+        kio.save_mesh(os.path.join(tmp, "part.off"), box)
+        with open(os.path.join(tmp, "part.obj"), "w") as f:
+            f.write('mesh "part.off"\n')
+        robotsim.clear_geometry_cache()
+        t0 = time.perf_counter()
+        Ws = robotsim.WorldModel()
+        for _ in range(64):
+            Ws.loadRigidObject(os.path.join(tmp, "part.obj"))
+        tshared = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        Wsep = robotsim.WorldModel()
+        for _ in range(64):
+            Wsep.addRigidObject(robotsim.Geometry3D(robotsim.TriangleMesh(box.verts, box.tris)))
+        tsep = time.perf_counter() - t0
+        out["io"] = {"world": "arm .rob + 7 link .off + %d-triangle .off terrain + world XML"
+                              % wf.terrains[0].mesh.tris.shape[0],
+                     "parse_ms": round(tparse * 1e3, 1), "load_ms": round(tload * 1e3, 1),
+                     "load_to_first_answer_ms": round(tfirst * 1e3, 1),
+                     "objects_from_one_file": 64, "triangles_per_object": int(box.tris.shape[0]),
+                     "shared_bvh_load_ms": round(tshared * 1e3, 1), "separate_build_ms": round(tsep * 1e3, 1)}
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    print(f"[bench] io: {out['io']}", file=sys.stderr, flush=True)
     return out
 
 

@@ -165,13 +165,26 @@ def _read_off(path):
     try:
         nv, nf = int(hdr[0]), int(hdr[1])
         V = np.array([[float(x) for x in next(it).split()[:3]] for _ in range(nv)], np.float64).reshape(-1, 3)
-        tris = []
-        for fi in range(nf):
-            tok = next(it).split()
-            k = int(tok[0])
-            tris += _fan([int(x) for x in tok[1:1 + k]], nv, "%s face %d" % (path, fi))
+        flines = [next(it) for _ in range(nf)]
     except (StopIteration, ValueError, IndexError) as e:
         raise LoadError("%s: malformed OFF file (%s)" % (path, e or "truncated")) from None
+    # all-triangle face blocks (the common case) parse in one pass
+    tok = " ".join(flines).split()
+    if len(tok) == 4 * nf:
+        F = np.array(tok, np.int64).reshape(-1, 4) if nf else np.zeros((0, 4), np.int64)
+        if (F[:, 0] == 3).all():
+            T = F[:, 1:]
+            if T.size and (T.min() < 0 or T.max() >= nv):
+                raise LoadError("%s: vertex index out of range (%d vertices)" % (path, nv))
+            return S.Mesh(V, T.astype(np.int32))
+    tris = []
+    try:
+        for fi, ln in enumerate(flines):
+            tok = ln.split()
+            k = int(tok[0])
+            tris += _fan([int(x) for x in tok[1:1 + k]], nv, "%s face %d" % (path, fi))
+    except (ValueError, IndexError) as e:
+        raise LoadError("%s: malformed OFF face (%s)" % (path, e)) from None
     return S.Mesh(V, np.array(tris, np.int32).reshape(-1, 3))
 
 
