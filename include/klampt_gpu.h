@@ -76,6 +76,21 @@ int kg_geom_create_trimesh(kg_ctx ctx, const double* verts, int nverts, const in
 int kg_geom_create_sdf(kg_ctx ctx, const double bmin[3], const double bmax[3], const int32_t dims[3],
                        const float* values, kg_geom* out);
 int kg_geom_destroy(kg_geom g);
+/* TriangleMesh -> ImplicitSurface: Geometry3D.convert("ImplicitSurface",
+ * resolution) (Python/klampt/src/geometry.h:944-1005, robotsim.cpp:1602-1646).
+ * Cell-centred grid of spacing `resolution` (0: the average longest triangle
+ * edge) covering the mesh's vertex bounds plus `pad` cells on every side:
+ * dims[k] = ceil((hi[k]-lo[k]) / h) + 2 pad (at least 1), bmin = lo - pad h.
+ * Value at a cell centre: the Euclidean distance to the closest triangle,
+ * negated where the mesh's winding number around the centre (counted along
+ * +z) is nonzero.  Klampt computes this grid by fast marching from the
+ * surface; here every cell holds the distance that method approximates.  The
+ * new geometry starts with the mesh's current transform. */
+int kg_geom_mesh_to_sdf(kg_geom mesh, double resolution, int pad, kg_geom* out);
+/* ImplicitSurface grid: dims, bounds, and (when values != NULL) the
+ * dims[0]*dims[1]*dims[2] values, C order, copied to host memory
+ * (Geometry3D.getImplicitSurface, geometry.h:322-391). */
+int kg_geom_get_sdf(kg_geom sdf, int32_t dims[3], double bmin[3], double bmax[3], float* values);
 /* A second handle on the same geometry data, as ManagedGeometry::Load hands a
  * copy of an already loaded file's geometry to every later loader of that
  * file (Cpp/Modeling/ManagedGeometry.cpp:111-164): a mesh shares its BVH and

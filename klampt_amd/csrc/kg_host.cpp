@@ -887,6 +887,111 @@ int kg_geom_destroy(kg_geom g) {
   });
 }
 
+int kg_geom_mesh_to_sdf(kg_geom g, double resolution, int pad, kg_geom* out) {
+  return guard([&] {
+    need(g && out, "NULL argument");
+    need(g->kind == 0, "mesh_to_sdf converts a TriangleMesh geometry");
+    need(std::isfinite(resolution) && resolution >= 0, "resolution must be finite and >= 0");
+    need(pad >= 0 && pad <= 1024, "pad must be 0..1024 cells");
+    kg_ctx_s* c = g->ctx;
+    const GeomDesc& gd = c->geoms[g->index];
+    need(gd.ntris > 0, "mesh is empty");
+    need(gd.depth + 2 <= KG_MSDF_STK, "mesh tree is deeper than the conversion's traversal stack");
+    const int nv = (int)(g->V.size() / 3), nt = (int)(g->T.size() / 3);
+    double h = resolution;
+    if (h == 0) {
+      // default: the average triangle diameter (longest edge)
+      double sum = 0;
+      for (int t = 0; t < nt; t++) {
+        double m = 0;
+        for (int e = 0; e < 3; e++) {
+          const double* p = &g->V[3 * g->T[3 * t + e]];
+          const double* q = &g->V[3 * g->T[3 * t + (e + 1) % 3]];
+          const double dd[3] = {p[0] - q[0], p[1] - q[1], p[2] - q[2]};
+          m = std::max(m, norm3(dd));
+        }
+        sum += m;
+      }
+      h = sum / nt;
+    }
+    need(h > 0, "resolution is 0 and the mesh's triangles have no extent");
+    double lo[3], hi[3];
+    for (int k = 0; k < 3; k++) lo[k] = hi[k] = g->V[k];
+    for (int i = 0; i < nv; i++)
+      for (int k = 0; k < 3; k++) {
+        lo[k] = std::min(lo[k], g->V[3 * i + k]);
+        hi[k] = std::max(hi[k], g->V[3 * i + k]);
+      }
+    int dims[3];
+    double bmin[3], bmax[3];
+    size_t total = 1;
+    for (int k = 0; k < 3; k++) {
+      const double cells = std::ceil((hi[k] - lo[k]) / h);
+      need(cells < 1e6, "grid too large for this resolution");
+      dims[k] = std::max(1, (int)cells + 2 * pad);
+      bmin[k] = lo[k] - pad * h;
+      bmax[k] = bmin[k] + dims[k] * h;
+      total *= (size_t)dims[k];
+    }
+    need(total < (1ull << 31), "grid has more than 2^31 cells");
+    hip_check(hipSetDevice(c->device), "hipSetDevice");
+    sync_pool(c);
+    kg_geom_s* s = new kg_geom_s();
+    s->ctx = c;
+    s->kind = 1;
+    s->index = -1;
+    s->X = g->X;
+    for (int k = 0; k < 3; k++) {
+      s->dims[k] = dims[k];
+      s->bmin[k] = bmin[k];
+      s->bmax[k] = bmax[k];
+    }
+    hipError_t e = s->dvals.ensure(total * sizeof(float));
+    if (e == hipSuccess) e = c->wsMsdf.ensure(total * sizeof(int));
+    if (e != hipSuccess) {
+      delete s;
+      hip_check(e, "hipMalloc(mesh_to_sdf)");
+    }
+    MeshSdfArgs a{};
+    a.pool = c->pool;
+    a.root = gd.root;
+    for (int k = 0; k < 3; k++) {
+      a.dims[k] = dims[k];
+      a.bmin[k] = bmin[k];
+    }
+    a.h = h;
+    a.out = s->dvals.as<float>();
+    a.diff = c->wsMsdf.as<int>();
+    StreamScope scope(c, c->stream);
+    e = kgl_fill_int(a.diff, (int)total, 0, c->stream);
+    if (e == hipSuccess) e = kgl_msdf_dist(a, c->stream);
+    if (e == hipSuccess) e = kgl_msdf_sign(a, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+      delete s;
+      hip_check(e, "mesh_to_sdf kernels");
+    }
+    *out = s;
+  });
+}
+
+int kg_geom_get_sdf(kg_geom g, int32_t dims[3], double bmin[3], double bmax[3], float* values) {
+  return guard([&] {
+    need(g && dims && bmin && bmax, "NULL argument");
+    need(g->kind == 1, "geometry is not an ImplicitSurface");
+    for (int k = 0; k < 3; k++) {
+      dims[k] = g->dims[k];
+      bmin[k] = g->bmin[k];
+      bmax[k] = g->bmax[k];
+    }
+    if (values) {
+      const size_t n = (size_t)g->dims[0] * g->dims[1] * g->dims[2];
+      hip_check(hipSetDevice(g->ctx->device), "hipSetDevice");
+      hip_check(hipMemcpy(values, g->dvals.p, n * sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy(sdf)");
+    }
+  });
+}
+
 int kg_geom_clone(kg_geom src, kg_geom* out) {
   return guard([&] {
     need(src && out, "NULL argument");

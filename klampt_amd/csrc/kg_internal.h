@@ -226,6 +226,20 @@ hipError_t kgl_edge_reduce(const int* seg, const int* soff, int nlive, int total
                            const int* ooff, const int* order, int pos0, float* first_u, uint8_t* done, hipStream_t s);
 
 hipError_t kgl_fill_int(int* p, int n, int v, hipStream_t s);
+
+// TriangleMesh -> ImplicitSurface: the grid of cell-centred signed distances.
+#define KG_MSDF_STK 64  // per-lane traversal stack (tree depth + 2 at most)
+struct MeshSdfArgs {
+  PoolDev pool;
+  int root;          // pool index of the mesh's root node
+  int dims[3];
+  double bmin[3];    // grid corner, mesh frame
+  double h;          // cell size
+  float* out;        // dims[0] x dims[1] x dims[2], C order (k fastest)
+  int* diff;         // same size: winding increments (zeroed before k_msdf_sign)
+};
+hipError_t kgl_msdf_dist(const MeshSdfArgs& a, hipStream_t s);
+hipError_t kgl_msdf_sign(const MeshSdfArgs& a, hipStream_t s);
 int kgl_collide_grid(int exact, int num_cus, int scap);
 hipError_t kgl_sdf_eval(const SdfDev& g, const float* pts, int n, float* d, float* grad, hipStream_t s);
 hipError_t kgl_sdf_penalty(const LinkDesc* links, int n, const SdfDev& g, const float* Q, int N, const float* pts,
@@ -330,7 +344,7 @@ struct kg_ctx_s {
   bool done_valid = false;
   DevBuf wsQ, wsT64, wsLB, wsJL, wsCounts, wsOffsets, wsItems, wsScanTmp, wsBest, wsUnc, wsChunk, wsSpill,
       wsStatus, wsRedo, wsOut8a, wsOut8b, wsOut8c, wsOut32, wsF, wsG, wsPoff, wsPts, wsLoss, wsDQ,
-      wsDl, wsEA, wsEB, wsESeg, wsESoff, wsENs, wsEOoff, wsEOrder, wsEQ, wsEFeas, wsEU, wsEDone, wsRec;
+      wsDl, wsEA, wsEB, wsESeg, wsESoff, wsENs, wsEOoff, wsEOrder, wsEQ, wsEFeas, wsEU, wsEDone, wsRec, wsMsdf;
   int timing = 1;           // 0 none, 1 traversal kernel events, 2 also per-phase events
   bool last_phase_ev = false;
   int last_tr_slot = -1;    // kernel-event slot of the last feasibility batch (-1: none)

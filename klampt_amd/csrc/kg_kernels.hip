@@ -2307,3 +2307,228 @@ hipError_t kgl_edge_reduce(const int* seg, const int* soff, int nlive, int total
                                                      done);
   return hipGetLastError();
 }
+
+// ---------------------------------------------------------------- mesh -> SDF
+//
+// TriangleMesh -> ImplicitSurface on a cell-centred grid (cell (i,j,k) centred
+// at bmin + ((i,j,k) + 0.5) h):
+//
+// k_msdf_dist: one lane per cell, one wave per 4x4x4 brick of cells, so the
+// lanes of a wave walk nearly the same paths of the mesh's OBB tree.  Each
+// lane runs a depth-first nearest-triangle search: a node is expanded while
+// the distance from the cell centre to its box is below the best triangle
+// distance so far; the child with the nearer centre is visited first.  FP32;
+// the unsigned distance is written to out.
+//
+// k_msdf_sign: one lane per row (i, j) of cells along z.  The +z line through
+// the row's centres is tested against the tree's boxes (FP32, padded: a
+// conservative cull) and against the triangles in FP64 with the crossing
+// rule below; each crossing adds its winding increment at the first cell
+// above it, and the lane negates the cells of its row whose running winding
+// number is nonzero.
+//
+// Crossing rule (the same expressions as the oracle, no contraction, so the
+// winding numbers are identical): a triangle whose xy projection has signed
+// area s != 0 is ordered counter-clockwise; it is crossed when every edge
+// function w(u,v,q) = (u.x-q.x)(v.y-q.y) - (u.y-q.y)(v.x-q.x) is > 0, or = 0
+// on an edge u->v with u before v in the order (y descending, x ascending);
+// the increment is -1 for s > 0 (normal towards +z), +1 for s < 0.
+
+namespace {
+
+__device__ inline void node_load(const ObbNode* nodes, int n, float c[3], float e[3], float R[9], int& child) {
+  const float4* q = reinterpret_cast<const float4*>(nodes + n);
+  const float4 a = q[0], b = q[1], d = q[2], f = q[3];
+  c[0] = a.x; c[1] = a.y; c[2] = a.z; e[0] = a.w;
+  e[1] = b.x; e[2] = b.y; R[0] = b.z; R[1] = b.w;
+  R[2] = d.x; R[3] = d.y; R[4] = d.z; R[5] = d.w;
+  R[6] = f.x; R[7] = f.y; R[8] = f.z;
+  child = __float_as_int(f.w);
+}
+
+__global__ __launch_bounds__(64) void k_msdf_dist(MeshSdfArgs a) {
+  __shared__ int stk[KG_MSDF_STK][64];
+  const int lane = threadIdx.x;
+  const int nx = a.dims[0], ny = a.dims[1], nz = a.dims[2];
+  const long bx = (nx + 3) >> 2, by = (ny + 3) >> 2, bz = (nz + 3) >> 2;
+  const long nb = bx * by * bz;
+  for (long b = blockIdx.x; b < nb; b += gridDim.x) {
+    const int bk = (int)(b % bz);
+    const long r = b / bz;
+    const int bj = (int)(r % by), bi = (int)(r / by);
+    const int i = bi * 4 + (lane >> 4), j = bj * 4 + ((lane >> 2) & 3), k = bk * 4 + (lane & 3);
+    if (i >= nx || j >= ny || k >= nz) continue;
+    const kgm::V<float> p = kgm::mk<float>((float)(a.bmin[0] + ((double)i + 0.5) * a.h),
+                                           (float)(a.bmin[1] + ((double)j + 0.5) * a.h),
+                                           (float)(a.bmin[2] + ((double)k + 0.5) * a.h));
+    float best = 3.0e38f;
+    int sp = 0;
+    stk[sp++][lane] = a.root;
+    while (sp > 0) {
+      const int n = stk[--sp][lane];
+      float c[3], e[3], R[9];
+      int child;
+      node_load(a.pool.nodes, n, c, e, R, child);
+      const float dx = p.x - c[0], dy = p.y - c[1], dz = p.z - c[2];
+      float lb = 0.f;
+#pragma unroll
+      for (int q = 0; q < 3; q++) {
+        const float s = R[3 * q] * dx + R[3 * q + 1] * dy + R[3 * q + 2] * dz;
+        const float ex = fmaxf(fabsf(s) - e[q], 0.f);
+        lb += ex * ex;
+      }
+      // the box bound carries FP32 rounding: prune only when clearly farther
+      if (lb > best * 1.00001f) continue;
+      if (child < 0) {
+        const int t = ~child;
+        const float4 A = a.pool.tv0[t], B = a.pool.tv1[t], C = a.pool.tv2[t];
+        const kgm::V<float> cp = kgm::closest_on_tri(p, kgm::mk(A.x, A.y, A.z), kgm::mk(B.x, B.y, B.z),
+                                                     kgm::mk(C.x, C.y, C.z));
+        const kgm::V<float> d = kgm::sub(p, cp);
+        best = fminf(best, kgm::dot(d, d));
+        continue;
+      }
+      const float4 c0 = reinterpret_cast<const float4*>(a.pool.nodes + child)[0];
+      const float4 c1 = reinterpret_cast<const float4*>(a.pool.nodes + child + 1)[0];
+      const float d0 = (p.x - c0.x) * (p.x - c0.x) + (p.y - c0.y) * (p.y - c0.y) + (p.z - c0.z) * (p.z - c0.z);
+      const float d1 = (p.x - c1.x) * (p.x - c1.x) + (p.y - c1.y) * (p.y - c1.y) + (p.z - c1.z) * (p.z - c1.z);
+      const bool first0 = d0 <= d1;
+      stk[sp++][lane] = first0 ? child + 1 : child;
+      stk[sp++][lane] = first0 ? child : child + 1;
+    }
+    a.out[((size_t)i * ny + j) * nz + k] = sqrtf(best);
+  }
+}
+
+__device__ inline double msdf_edge(double ux, double uy, double vx, double vy, double qx, double qy) {
+#pragma clang fp contract(off)
+  return (ux - qx) * (vy - qy) - (uy - qy) * (vx - qx);
+}
+
+__device__ inline bool msdf_inc(double w, double ux, double uy, double vx, double vy) {
+  return w > 0 || (w == 0 && (uy > vy || (uy == vy && ux < vx)));
+}
+
+// Winding increment of the +z line through (qx, qy) at triangle t (0: not
+// crossed) and the crossing height.
+__device__ inline int msdf_crossing(const double* t, double qx, double qy, double& zt) {
+#pragma clang fp contract(off)
+  double ax = t[0], ay = t[1], az = t[2];
+  double bx = t[3], by = t[4], bz = t[5];
+  double cx = t[6], cy = t[7], cz = t[8];
+  const double s = msdf_edge(ax, ay, bx, by, cx, cy);
+  if (s == 0) return 0;
+  const int w = s > 0 ? -1 : 1;
+  if (s < 0) {
+    double tx = bx, ty = by, tz = bz;
+    bx = cx; by = cy; bz = cz;
+    cx = tx; cy = ty; cz = tz;
+  }
+  const double wab = msdf_edge(ax, ay, bx, by, qx, qy);
+  const double wbc = msdf_edge(bx, by, cx, cy, qx, qy);
+  const double wca = msdf_edge(cx, cy, ax, ay, qx, qy);
+  if (!msdf_inc(wab, ax, ay, bx, by) || !msdf_inc(wbc, bx, by, cx, cy) || !msdf_inc(wca, cx, cy, ax, ay)) return 0;
+  const double den = (wab + wbc) + wca;
+  if (den == 0) return 0;
+  zt = ((wbc * az + wca * bz) + wab * cz) / den;
+  return w;
+}
+
+__device__ inline double msdf_centre(double b, int k, double h) {
+#pragma clang fp contract(off)
+  return b + ((double)k + 0.5) * h;
+}
+
+// First k in [0, nz] with centre z_k > zt.
+__device__ inline int msdf_first_above(double zt, double bz, double h, int nz) {
+#pragma clang fp contract(off)
+  double f = (zt - bz) / h - 0.5;
+  if (!(f > -2.0)) f = -2.0;
+  if (f > nz + 1.0) f = nz + 1.0;
+  int k = (int)floor(f) + 1;
+  k = k < 0 ? 0 : (k > nz ? nz : k);
+  while (k > 0 && msdf_centre(bz, k - 1, h) > zt) k--;
+  while (k < nz && !(msdf_centre(bz, k, h) > zt)) k++;
+  return k;
+}
+
+__global__ __launch_bounds__(64) void k_msdf_sign(MeshSdfArgs a) {
+  __shared__ int stk[KG_MSDF_STK][64];
+  const int lane = threadIdx.x;
+  const int ny = a.dims[1], nz = a.dims[2];
+  const long rows = (long)a.dims[0] * ny;
+  for (long row = (long)blockIdx.x * 64 + lane; row < rows; row += (long)gridDim.x * 64) {
+    const int i = (int)(row / ny), j = (int)(row % ny);
+    const double qx = msdf_centre(a.bmin[0], i, a.h), qy = msdf_centre(a.bmin[1], j, a.h);
+    const float qxf = (float)qx, qyf = (float)qy;
+    int* diff = a.diff + (size_t)row * nz;
+    int sp = 0;
+    stk[sp++][lane] = a.root;
+    while (sp > 0) {
+      const int n = stk[--sp][lane];
+      float c[3], e[3], R[9];
+      int child;
+      node_load(a.pool.nodes, n, c, e, R, child);
+      // does the line (qx, qy, t) meet the padded box?
+      const float dx = qxf - c[0], dy = qyf - c[1], dz = -c[2];
+      const float pad = 1e-5f * (1.f + fabsf(qxf) + fabsf(qyf) + fabsf(c[0]) + fabsf(c[1]) + fabsf(c[2]));
+      float tlo = -3.0e38f, thi = 3.0e38f;
+      bool hit = true;
+#pragma unroll
+      for (int q = 0; q < 3; q++) {
+        const float s0 = R[3 * q] * dx + R[3 * q + 1] * dy + R[3 * q + 2] * dz;
+        const float r = R[3 * q + 2], ek = e[q] + pad;
+        if (fabsf(r) < 1e-20f) {
+          hit = hit && fabsf(s0) <= ek;
+        } else {
+          float t1 = (-ek - s0) / r, t2 = (ek - s0) / r;
+          if (t1 > t2) {
+            const float tt = t1;
+            t1 = t2;
+            t2 = tt;
+          }
+          tlo = fmaxf(tlo, t1);
+          thi = fminf(thi, t2);
+        }
+      }
+      if (!hit || tlo > thi) continue;
+      if (child < 0) {
+        const int t = ~child;
+        double zt = 0;
+        const int w = msdf_crossing(a.pool.tv64 + 9 * (size_t)t, qx, qy, zt);
+        if (w) {
+          const int k0 = msdf_first_above(zt, a.bmin[2], a.h, nz);
+          if (k0 < nz) diff[k0] += w;
+        }
+        continue;
+      }
+      stk[sp++][lane] = child;
+      stk[sp++][lane] = child + 1;
+    }
+    float* out = a.out + (size_t)row * nz;
+    int wind = 0;
+    for (int k = 0; k < nz; k++) {
+      wind += diff[k];
+      if (wind != 0) out[k] = -out[k];
+    }
+  }
+}
+
+}  // namespace
+
+hipError_t kgl_msdf_dist(const MeshSdfArgs& a, hipStream_t s) {
+  const long nb = (long)((a.dims[0] + 3) >> 2) * ((a.dims[1] + 3) >> 2) * ((a.dims[2] + 3) >> 2);
+  if (nb <= 0) return hipSuccess;
+  const int blocks = (int)(nb < (1l << 20) ? nb : (1l << 20));
+  k_msdf_dist<<<blocks, 64, 0, s>>>(a);
+  return hipGetLastError();
+}
+
+hipError_t kgl_msdf_sign(const MeshSdfArgs& a, hipStream_t s) {
+  const long rows = (long)a.dims[0] * a.dims[1];
+  if (rows <= 0) return hipSuccess;
+  const long nb = (rows + 63) / 64;
+  const int blocks = (int)(nb < (1l << 20) ? nb : (1l << 20));
+  k_msdf_sign<<<blocks, 64, 0, s>>>(a);
+  return hipGetLastError();
+}

@@ -99,6 +99,26 @@ class Geom:
         g.dims = tuple(int(x) for x in vals.shape)
         return g
 
+    def to_sdf(self, resolution=0.0, pad=2):
+        """TriangleMesh -> ImplicitSurface (kg_geom_mesh_to_sdf)."""
+        h = ctypes.c_void_p()
+        call("kg_geom_mesh_to_sdf", self.h, float(resolution), int(pad), ctypes.byref(h))
+        g = Geom(self.ctx, h, "ImplicitSurface")
+        g.dims = g.sdf_grid()[0]
+        return g
+
+    def sdf_grid(self, values=False):
+        """(dims, bmin, bmax[, values]) of an ImplicitSurface."""
+        dims, lo, hi = np.zeros(3, np.int32), np.zeros(3), np.zeros(3)
+        call("kg_geom_get_sdf", self.h, _p(dims, _pi), _p(lo, _pd), _p(hi, _pd), None)
+        dims = tuple(int(x) for x in dims)
+        if not values:
+            return dims, lo, hi
+        v = np.zeros(dims, np.float32)
+        call("kg_geom_get_sdf", self.h, _p(np.zeros(3, np.int32), _pi), _p(np.zeros(3), _pd), _p(np.zeros(3), _pd),
+             _p(v, _pf))
+        return dims, lo, hi, v
+
     def clone(self):
         """Another handle on the same data (kg_geom_clone): meshes share the
         BVH; the transform and margin start as this handle's."""

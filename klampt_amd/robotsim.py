@@ -146,6 +146,36 @@ class Geometry3D:
             return False
         return True
 
+    def convert(self, type: str, param: float = 0.0) -> "Geometry3D":
+        """``Geometry3D.convert`` (robotsim.cpp:1602-1646) for TriangleMesh ->
+        ImplicitSurface / VolumeGrid: param is the grid resolution (0: the
+        average triangle diameter); the grid has 2 cells of padding around
+        the mesh.  Other conversions raise as the reference does for an
+        unavailable one."""
+        if self._g is None:
+            raise RuntimeError("Geometry3D is empty, cannot convert")
+        if type not in ("TriangleMesh", "PointCloud", "VolumeGrid", "ImplicitSurface", "OccupancyGrid",
+                        "GeometricPrimitive", "ConvexHull", "Heightmap"):
+            raise RuntimeError("Invalid desired type specified, must be ConvexHull, TriangleMesh, PointCloud, "
+                               "VolumeGrid, ImplicitSurface, OccupancyGrid, or Heightmap")
+        if param < 0:
+            raise RuntimeError("Invalid conversion parameter, must be nonnegative")
+        if self._type == "TriangleMesh" and type in ("ImplicitSurface", "VolumeGrid"):
+            try:
+                g = self._g.to_sdf(param, 2)
+            except KlamptGPUError as e:
+                raise RuntimeError(str(e)) from None
+            out = Geometry3D(ctx=self._ctx)
+            out._g, out._type = g, "ImplicitSurface"
+            return out
+        raise RuntimeError("Cannot perform the geometry conversion %s -> %s" % (self._type, type))
+
+    def getImplicitSurface(self) -> ImplicitSurface:
+        if self._type != "ImplicitSurface":
+            raise RuntimeError("Geometry3D is not an ImplicitSurface")
+        dims, lo, hi, v = self._g.sdf_grid(values=True)
+        return ImplicitSurface(lo, hi, v)
+
     def empty(self) -> bool:
         return self._g is None or (self._type == "TriangleMesh" and self._mesh.indices.shape[0] == 0)
 

@@ -64,6 +64,8 @@ def lib():
             "or_tritri_depth": (ctypes.c_double, [_d, _d]),
             "or_world_aabb": (None, [_vp, ctypes.c_int, _d, _d, _d]),
             "or_sdf_eval": (None, [_d, _i, _d, _d, _d, ctypes.c_int, _d, _d]),
+            "or_mesh_to_sdf": (None, [_d, ctypes.c_int, _i, ctypes.c_int, _d, ctypes.c_double, _i, _d, _i,
+                                      ctypes.c_int, ctypes.c_int]),
             "or_sdf_penalty": (None, [_vp, ctypes.c_int, _d, _i, _d, _d, _d, ctypes.c_int, _d, _i,
                                       ctypes.c_double, _d, _d, ctypes.c_double, ctypes.c_int]),
         }
@@ -262,6 +264,21 @@ def sdf_eval(grid, pts):
     lib().or_sdf_eval(_p(vals, _d), _p(dims, _i), _p(bmin, _d), _p(bmax, _d), _p(P, _d), P.shape[0],
                       _p(d, _d), _p(g, _d))
     return d, g
+
+
+def mesh_to_sdf(mesh, bmin, h, dims, brute=False, with_winding=False, nthreads=None):
+    """Signed distance grid of a triangle mesh on the given cell-centred grid
+    (exact FP64 distances, sign from the winding number along +z; the
+    definition is at or_mesh_to_sdf in kg_oracle.cpp)."""
+    V = _dd(mesh.verts)
+    T = _ii(mesh.tris)
+    dims = _ii(dims)
+    out = np.zeros(tuple(int(x) for x in dims))
+    wind = np.zeros(out.shape, np.int32) if with_winding else None
+    lib().or_mesh_to_sdf(_p(V, _d), V.shape[0], _p(T, _i), T.shape[0], _p(_dd(bmin), _d), float(h), _p(dims, _i),
+                         _p(out, _d), _p(wind, _i) if wind is not None else None, 1 if brute else 0,
+                         nthreads or nthreads_default())
+    return (out, wind) if with_winding else out
 
 
 def world_from_scene(scene):

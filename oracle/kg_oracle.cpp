@@ -941,6 +941,123 @@ static double sdf_eval(const Grid& g, V3 p, V3* grad) {
   return val;
 }
 
+// ------------------------------------------------------------------ mesh -> SDF
+//
+// TriangleMesh -> ImplicitSurface (Geometry3D.convert("ImplicitSurface", h),
+// Python/klampt/src/geometry.h:962-965).  KrisLibrary's fast-marching
+// conversion is absent from the reference tree; this library defines the grid
+// that method approximates, exactly:
+//   value(i,j,k) = +-(Euclidean distance from the cell centre
+//                  c = bmin + ((i,j,k) + 0.5) h to the closest triangle),
+//   negative where the winding number of the mesh around c is nonzero.
+// The winding number is counted along the +z line through c: a triangle whose
+// xy projection contains (c.x, c.y) and which meets the line below c.z adds
+// -1 when its normal points to +z and +1 when it points to -z.  Points on a
+// projected edge u->v of the counter-clockwise-ordered triangle count when u
+// precedes v in the order (y descending, then x ascending), which evaluates
+// every line at the same infinitesimally shifted position, so an edge or a
+// vertex shared by several triangles is counted once.  The edge functions are
+// exact negations of each other for the two triangles of a shared edge.
+
+static inline double edge_fn(double ux, double uy, double vx, double vy, double qx, double qy) {
+  return (ux - qx) * (vy - qy) - (uy - qy) * (vx - qx);
+}
+
+static inline bool edge_inc(double w, double ux, double uy, double vx, double vy) {
+  return w > 0 || (w == 0 && (uy > vy || (uy == vy && ux < vx)));
+}
+
+// Crossing of the +z line through (qx, qy) with triangle t: returns the
+// winding increment (0: no crossing) and the crossing height in *zt.
+static int z_crossing(const V3* t, double qx, double qy, double* zt) {
+  V3 a = t[0], b = t[1], c = t[2];
+  const double s = edge_fn(a.x, a.y, b.x, b.y, c.x, c.y);
+  if (s == 0) return 0;
+  const int w = s > 0 ? -1 : 1;
+  if (s < 0) std::swap(b, c);
+  const double wab = edge_fn(a.x, a.y, b.x, b.y, qx, qy);
+  const double wbc = edge_fn(b.x, b.y, c.x, c.y, qx, qy);
+  const double wca = edge_fn(c.x, c.y, a.x, a.y, qx, qy);
+  if (!edge_inc(wab, a.x, a.y, b.x, b.y) || !edge_inc(wbc, b.x, b.y, c.x, c.y) ||
+      !edge_inc(wca, c.x, c.y, a.x, a.y))
+    return 0;
+  const double den = (wab + wbc) + wca;
+  if (den == 0) return 0;
+  *zt = ((wbc * a.z + wca * b.z) + wab * c.z) / den;
+  return w;
+}
+
+// First k with z_k = bmin_z + (k + 0.5) h > zt, in [0, nz].
+static int first_above(double zt, double bz, double h, int nz) {
+  double f = (zt - bz) / h - 0.5;
+  if (!(f > -2.0)) f = -2.0;
+  if (f > nz + 1.0) f = nz + 1.0;
+  int k = (int)std::floor(f) + 1;
+  if (k < 0) k = 0;
+  if (k > nz) k = nz;
+  while (k > 0 && bz + ((double)(k - 1) + 0.5) * h > zt) k--;
+  while (k < nz && !(bz + ((double)k + 0.5) * h > zt)) k++;
+  return k;
+}
+
+static double aabb_dist2(const Node& n, V3 p) {
+  double d2 = 0;
+  for (int k = 0; k < 3; k++) {
+    double v = comp(p, k), e = 0;
+    if (v < n.lo[k]) e = n.lo[k] - v;
+    else if (v > n.hi[k]) e = v - n.hi[k];
+    d2 += e * e;
+  }
+  return d2;
+}
+
+static double point_mesh_dist2(const Mesh& m, V3 p, bool brute) {
+  double best = kInf;
+  auto tri = [&](int t) {
+    V3 q = closest_pt_tri(p, m.V[m.T[t][0]], m.V[m.T[t][1]], m.V[m.T[t][2]]);
+    V3 d = p - q;
+    best = std::min(best, dot(d, d));
+  };
+  if (brute || m.nodes.empty()) {
+    for (size_t t = 0; t < m.T.size(); t++) tri((int)t);
+    return best;
+  }
+  int stk[128], sp = 0;
+  stk[sp++] = 0;
+  while (sp) {
+    const Node& n = m.nodes[stk[--sp]];
+    if (aabb_dist2(n, p) >= best) continue;
+    if (n.tri >= 0) { tri(n.tri); continue; }
+    const double dl = aabb_dist2(m.nodes[n.left], p), dr = aabb_dist2(m.nodes[n.right], p);
+    if (dl < dr) { stk[sp++] = n.right; stk[sp++] = n.left; }
+    else { stk[sp++] = n.left; stk[sp++] = n.right; }
+  }
+  return best;
+}
+
+static void line_crossings(const Mesh& m, double qx, double qy, bool brute, std::vector<std::pair<double, int>>& out) {
+  out.clear();
+  auto tri = [&](int t) {
+    V3 tv[3] = {m.V[m.T[t][0]], m.V[m.T[t][1]], m.V[m.T[t][2]]};
+    double zt;
+    int w = z_crossing(tv, qx, qy, &zt);
+    if (w) out.push_back({zt, w});
+  };
+  if (brute || m.nodes.empty()) {
+    for (size_t t = 0; t < m.T.size(); t++) tri((int)t);
+    return;
+  }
+  int stk[128], sp = 0;
+  stk[sp++] = 0;
+  while (sp) {
+    const Node& n = m.nodes[stk[--sp]];
+    if (qx < n.lo[0] || qx > n.hi[0] || qy < n.lo[1] || qy > n.hi[1]) continue;
+    if (n.tri >= 0) { tri(n.tri); continue; }
+    stk[sp++] = n.left;
+    stk[sp++] = n.right;
+  }
+}
+
 }  // namespace orc
 
 using namespace orc;
@@ -1145,6 +1262,37 @@ void or_edge_batch(void* wp, int r, const double* A, const double* B, int nseg, 
     first_u[s] = fu;
     if (nsamples) nsamples[s] = ns;
     if (nchecked) nchecked[s] = checked;
+  });
+}
+
+// Signed distance grid of a triangle mesh (definition above the helpers
+// edge_fn .. line_crossings).  out: dims[0] x dims[1] x dims[2], C order;
+// winding (nullable) receives the winding numbers.
+void or_mesh_to_sdf(const double* V, int nv, const int* T, int nt, const double* bmin, double h, const int* dims,
+                    double* out, int32_t* winding, int brute, int nthreads) {
+  Mesh m;
+  m.V.resize(nv);
+  for (int i = 0; i < nv; i++) m.V[i] = mk(V[3 * i], V[3 * i + 1], V[3 * i + 2]);
+  m.T.resize(nt);
+  for (int i = 0; i < nt; i++) m.T[i] = {T[3 * i], T[3 * i + 1], T[3 * i + 2]};
+  build(m);
+  const int nx = dims[0], ny = dims[1], nz = dims[2];
+  parallel_for(nx * ny, nthreads, [&](int row) {
+    const int i = row / ny, j = row % ny;
+    const double x = bmin[0] + ((double)i + 0.5) * h, y = bmin[1] + ((double)j + 0.5) * h;
+    std::vector<std::pair<double, int>> cr;
+    line_crossings(m, x, y, brute != 0, cr);
+    std::vector<int> diff(nz + 1, 0);
+    for (auto& c : cr) diff[first_above(c.first, bmin[2], h, nz)] += c.second;
+    int wind = 0;
+    for (int k = 0; k < nz; k++) {
+      wind += diff[k];
+      const double z = bmin[2] + ((double)k + 0.5) * h;
+      const double d = std::sqrt(point_mesh_dist2(m, mk(x, y, z), brute != 0));
+      const size_t at = ((size_t)i * ny + j) * nz + k;
+      out[at] = wind != 0 ? -d : d;
+      if (winding) winding[at] = wind;
+    }
   });
 }
 
