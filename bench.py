@@ -376,6 +376,49 @@ def run_secondary(args, ctx, sc, world, robot):
         out["planner"]["cpu_sample"] = f"first {len(E)} roadmap edges, FP64 oracle, {oracle.nthreads_default()} threads"
     print(f"[bench] planner: {out['planner']}", file=sys.stderr, flush=True)
 
+    # TriangleMesh -> ImplicitSurface of the 200k-triangle soup terrain at
+    # 1 cm cells (Geometry3D.convert; the reference's fast-marching
+    # conversion of its complex mesh takes 14.06 s, Manual-Geometry.md:134)
+    soup = sc.terrains[0]
+    gm = engine.Geom.trimesh(ctx, soup.verts, soup.tris)
+    hs = 0.01
+    gm.to_sdf(hs).close()
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        gs = gm.to_sdf(hs)
+        if _ < reps - 1:
+            gs.close()
+    tconv = (time.perf_counter() - t0) / reps
+    dims, lo, hi, vals = gs.sdf_grid(values=True)
+    gs.close()
+    ncell = int(np.prod(dims))
+    out["c6_mesh_to_sdf"] = {"mesh_tris": int(soup.tris.shape[0]), "resolution": hs, "dims": list(dims),
+                             "cells": ncell, "ms": round(tconv * 1e3, 2),
+                             "cells_per_s": round(ncell / tconv, 1)}
+    gd = engine.Geom.trimesh(ctx, soup.verts, soup.tris)
+    t0 = time.perf_counter()
+    gdd = gd.to_sdf(0.0)
+    tdef = time.perf_counter() - t0
+    out["c6_mesh_to_sdf"]["default_resolution"] = {"dims": list(gdd.sdf_grid()[0]), "ms": round(tdef * 1e3, 2)}
+    gdd.close()
+    if not args.no_cpu:
+        # the oracle on 8 x-slabs through the middle of the same grid
+        i0 = dims[0] // 2 - 4
+        sub = (8, dims[1], dims[2])
+        blo = lo + np.array([i0 * hs, 0.0, 0.0])
+        t1 = time.perf_counter()
+        ov, ow = oracle.mesh_to_sdf(soup, blo, hs, sub, with_winding=True)
+        tc = time.perf_counter() - t1
+        gv = vals[i0:i0 + 8]
+        out["c6_mesh_to_sdf"].update({
+            "cpu_cells_per_s": round(ov.size / tc, 1),
+            "cpu_sample": f"8 x-slabs ({ov.size} cells) through the grid's middle, FP64 oracle, "
+                          f"{oracle.nthreads_default()} threads",
+            "max_abs_err_vs_oracle": float(np.abs(gv.astype(np.float64) - ov).max()),
+            "classification_equal": bool(np.array_equal(np.signbit(gv), ow != 0))})
+    print(f"[bench] C6 mesh->SDF: {out['c6_mesh_to_sdf']}", file=sys.stderr, flush=True)
+
     # on-disk formats: the headline world as files (arm .rob + link .off,
     # 200k-triangle .off terrain, world XML) loaded to a first answer, and 64
     # rigid objects from one mesh file (one BVH build, 63 shared handles)

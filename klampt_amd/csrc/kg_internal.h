@@ -228,7 +228,7 @@ hipError_t kgl_edge_reduce(const int* seg, const int* soff, int nlive, int total
 hipError_t kgl_fill_int(int* p, int n, int v, hipStream_t s);
 
 // TriangleMesh -> ImplicitSurface: the grid of cell-centred signed distances.
-#define KG_MSDF_STK 64  // per-lane traversal stack (tree depth + 2 at most)
+#define KG_MSDF_STK 64  // traversal stack entries, one per lane (tree depth + 2 at most)
 struct MeshSdfArgs {
   PoolDev pool;
   int root;          // pool index of the mesh's root node

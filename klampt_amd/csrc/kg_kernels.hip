@@ -2313,19 +2313,21 @@ hipError_t kgl_edge_reduce(const int* seg, const int* soff, int nlive, int total
 // TriangleMesh -> ImplicitSurface on a cell-centred grid (cell (i,j,k) centred
 // at bmin + ((i,j,k) + 0.5) h):
 //
-// k_msdf_dist: one lane per cell, one wave per 4x4x4 brick of cells, so the
-// lanes of a wave walk nearly the same paths of the mesh's OBB tree.  Each
-// lane runs a depth-first nearest-triangle search: a node is expanded while
-// the distance from the cell centre to its box is below the best triangle
-// distance so far; the child with the nearer centre is visited first.  FP32;
-// the unsigned distance is written to out.
+// k_msdf_dist: one lane per cell, one wave per 4x4x4 brick of cells.  The
+// wave runs one depth-first nearest-triangle search for its 64 cells: a node
+// is expanded while, for some lane, the distance from that lane's cell centre
+// to the node's box is below the lane's best triangle distance so far (so
+// every lane still sees every node its own search needs); the child whose
+// centre is nearer the brick's centre is visited first.  The traversal stack
+// holds one entry per lane (64 deep).  FP32; the unsigned distance is
+// written to out.
 //
-// k_msdf_sign: one lane per row (i, j) of cells along z.  The +z line through
-// the row's centres is tested against the tree's boxes (FP32, padded: a
-// conservative cull) and against the triangles in FP64 with the crossing
-// rule below; each crossing adds its winding increment at the first cell
-// above it, and the lane negates the cells of its row whose running winding
-// number is nonzero.
+// k_msdf_sign: one lane per row (i, j) of cells along z, one wave per 8 x 8
+// block of rows walking the tree together.  The +z line through each row's
+// centres is tested against the tree's boxes (FP32, padded: a conservative
+// cull) and against the triangles in FP64 with the crossing rule below; each
+// crossing adds its winding increment at the first cell above it, and the
+// lane negates the cells of its row whose running winding number is nonzero.
 //
 // Crossing rule (the same expressions as the oracle, no contraction, so the
 // winding numbers are identical): a triangle whose xy projection has signed
@@ -2347,7 +2349,6 @@ __device__ inline void node_load(const ObbNode* nodes, int n, float c[3], float 
 }
 
 __global__ __launch_bounds__(64) void k_msdf_dist(MeshSdfArgs a) {
-  __shared__ int stk[KG_MSDF_STK][64];
   const int lane = threadIdx.x;
   const int nx = a.dims[0], ny = a.dims[1], nz = a.dims[2];
   const long bx = (nx + 3) >> 2, by = (ny + 3) >> 2, bz = (nz + 3) >> 2;
@@ -2357,15 +2358,19 @@ __global__ __launch_bounds__(64) void k_msdf_dist(MeshSdfArgs a) {
     const long r = b / bz;
     const int bj = (int)(r % by), bi = (int)(r / by);
     const int i = bi * 4 + (lane >> 4), j = bj * 4 + ((lane >> 2) & 3), k = bk * 4 + (lane & 3);
-    if (i >= nx || j >= ny || k >= nz) continue;
-    const kgm::V<float> p = kgm::mk<float>((float)(a.bmin[0] + ((double)i + 0.5) * a.h),
-                                           (float)(a.bmin[1] + ((double)j + 0.5) * a.h),
-                                           (float)(a.bmin[2] + ((double)k + 0.5) * a.h));
+    const bool act = i < nx && j < ny && k < nz;
+    const kgm::V<float> p = kgm::mk<float>((float)(a.bmin[0] + ((double)min(i, nx - 1) + 0.5) * a.h),
+                                           (float)(a.bmin[1] + ((double)min(j, ny - 1) + 0.5) * a.h),
+                                           (float)(a.bmin[2] + ((double)min(k, nz - 1) + 0.5) * a.h));
+    const float qx = (float)(a.bmin[0] + (bi * 4 + 2.0) * a.h), qy = (float)(a.bmin[1] + (bj * 4 + 2.0) * a.h),
+                qz = (float)(a.bmin[2] + (bk * 4 + 2.0) * a.h);
     float best = 3.0e38f;
-    int sp = 0;
-    stk[sp++][lane] = a.root;
-    while (sp > 0) {
-      const int n = stk[--sp][lane];
+    // the wave walks the tree together: a node is expanded when any lane's
+    // box bound is below that lane's best; the stack lives one entry per
+    // lane (entry s in lane s), its depth is wave-uniform
+    int stk = 0, sp = 0, n = a.root;
+    for (;;) {
+      n = __builtin_amdgcn_readfirstlane(n);
       float c[3], e[3], R[9];
       int child;
       node_load(a.pool.nodes, n, c, e, R, child);
@@ -2378,25 +2383,32 @@ __global__ __launch_bounds__(64) void k_msdf_dist(MeshSdfArgs a) {
         lb += ex * ex;
       }
       // the box bound carries FP32 rounding: prune only when clearly farther
-      if (lb > best * 1.00001f) continue;
-      if (child < 0) {
-        const int t = ~child;
-        const float4 A = a.pool.tv0[t], B = a.pool.tv1[t], C = a.pool.tv2[t];
-        const kgm::V<float> cp = kgm::closest_on_tri(p, kgm::mk(A.x, A.y, A.z), kgm::mk(B.x, B.y, B.z),
-                                                     kgm::mk(C.x, C.y, C.z));
-        const kgm::V<float> d = kgm::sub(p, cp);
-        best = fminf(best, kgm::dot(d, d));
-        continue;
+      const bool need = act && lb <= best * 1.00001f;
+      if (__ballot(need)) {
+        if (child < 0) {
+          const int t = ~child;
+          const float4 A = a.pool.tv0[t], B = a.pool.tv1[t], C = a.pool.tv2[t];
+          const kgm::V<float> cp = kgm::closest_on_tri(p, kgm::mk(A.x, A.y, A.z), kgm::mk(B.x, B.y, B.z),
+                                                       kgm::mk(C.x, C.y, C.z));
+          const kgm::V<float> d = kgm::sub(p, cp);
+          best = fminf(best, kgm::dot(d, d));
+        } else {
+          const float4 c0 = reinterpret_cast<const float4*>(a.pool.nodes + child)[0];
+          const float4 c1 = reinterpret_cast<const float4*>(a.pool.nodes + child + 1)[0];
+          const float d0 = (qx - c0.x) * (qx - c0.x) + (qy - c0.y) * (qy - c0.y) + (qz - c0.z) * (qz - c0.z);
+          const float d1 = (qx - c1.x) * (qx - c1.x) + (qy - c1.y) * (qy - c1.y) + (qz - c1.z) * (qz - c1.z);
+          const bool first0 = d0 <= d1;
+          if (lane == sp) stk = first0 ? child + 1 : child;
+          sp++;
+          n = first0 ? child : child + 1;
+          continue;
+        }
       }
-      const float4 c0 = reinterpret_cast<const float4*>(a.pool.nodes + child)[0];
-      const float4 c1 = reinterpret_cast<const float4*>(a.pool.nodes + child + 1)[0];
-      const float d0 = (p.x - c0.x) * (p.x - c0.x) + (p.y - c0.y) * (p.y - c0.y) + (p.z - c0.z) * (p.z - c0.z);
-      const float d1 = (p.x - c1.x) * (p.x - c1.x) + (p.y - c1.y) * (p.y - c1.y) + (p.z - c1.z) * (p.z - c1.z);
-      const bool first0 = d0 <= d1;
-      stk[sp++][lane] = first0 ? child + 1 : child;
-      stk[sp++][lane] = first0 ? child : child + 1;
+      if (sp == 0) break;
+      sp = __builtin_amdgcn_readfirstlane(sp - 1);
+      n = __builtin_amdgcn_readlane(stk, sp);
     }
-    a.out[((size_t)i * ny + j) * nz + k] = sqrtf(best);
+    if (act) a.out[((size_t)i * ny + j) * nz + k] = sqrtf(best);
   }
 }
 
@@ -2453,19 +2465,19 @@ __device__ inline int msdf_first_above(double zt, double bz, double h, int nz) {
 }
 
 __global__ __launch_bounds__(64) void k_msdf_sign(MeshSdfArgs a) {
-  __shared__ int stk[KG_MSDF_STK][64];
   const int lane = threadIdx.x;
-  const int ny = a.dims[1], nz = a.dims[2];
-  const long rows = (long)a.dims[0] * ny;
-  for (long row = (long)blockIdx.x * 64 + lane; row < rows; row += (long)gridDim.x * 64) {
-    const int i = (int)(row / ny), j = (int)(row % ny);
-    const double qx = msdf_centre(a.bmin[0], i, a.h), qy = msdf_centre(a.bmin[1], j, a.h);
+  const int nx = a.dims[0], ny = a.dims[1], nz = a.dims[2];
+  const long tx = (nx + 7) >> 3, ty = (ny + 7) >> 3;
+  for (long tile = blockIdx.x; tile < tx * ty; tile += gridDim.x) {
+    // one wave per 8 x 8 block of rows
+    const int i = (int)(tile / ty) * 8 + (lane >> 3), j = (int)(tile % ty) * 8 + (lane & 7);
+    const bool act = i < nx && j < ny;
+    const double qx = msdf_centre(a.bmin[0], min(i, nx - 1), a.h), qy = msdf_centre(a.bmin[1], min(j, ny - 1), a.h);
     const float qxf = (float)qx, qyf = (float)qy;
-    int* diff = a.diff + (size_t)row * nz;
-    int sp = 0;
-    stk[sp++][lane] = a.root;
-    while (sp > 0) {
-      const int n = stk[--sp][lane];
+    int* diff = a.diff + ((size_t)min(i, nx - 1) * ny + min(j, ny - 1)) * nz;
+    int stk = 0, sp = 0, n = a.root;
+    for (;;) {
+      n = __builtin_amdgcn_readfirstlane(n);
       float c[3], e[3], R[9];
       int child;
       node_load(a.pool.nodes, n, c, e, R, child);
@@ -2473,7 +2485,7 @@ __global__ __launch_bounds__(64) void k_msdf_sign(MeshSdfArgs a) {
       const float dx = qxf - c[0], dy = qyf - c[1], dz = -c[2];
       const float pad = 1e-5f * (1.f + fabsf(qxf) + fabsf(qyf) + fabsf(c[0]) + fabsf(c[1]) + fabsf(c[2]));
       float tlo = -3.0e38f, thi = 3.0e38f;
-      bool hit = true;
+      bool hit = act;
 #pragma unroll
       for (int q = 0; q < 3; q++) {
         const float s0 = R[3 * q] * dx + R[3 * q + 1] * dy + R[3 * q + 2] * dz;
@@ -2491,25 +2503,35 @@ __global__ __launch_bounds__(64) void k_msdf_sign(MeshSdfArgs a) {
           thi = fminf(thi, t2);
         }
       }
-      if (!hit || tlo > thi) continue;
-      if (child < 0) {
-        const int t = ~child;
-        double zt = 0;
-        const int w = msdf_crossing(a.pool.tv64 + 9 * (size_t)t, qx, qy, zt);
-        if (w) {
-          const int k0 = msdf_first_above(zt, a.bmin[2], a.h, nz);
-          if (k0 < nz) diff[k0] += w;
+      hit = hit && tlo <= thi;
+      if (__ballot(hit)) {
+        if (child < 0) {
+          if (hit) {
+            double zt = 0;
+            const int w = msdf_crossing(a.pool.tv64 + 9 * (size_t)(~child), qx, qy, zt);
+            if (w) {
+              const int k0 = msdf_first_above(zt, a.bmin[2], a.h, nz);
+              if (k0 < nz) diff[k0] += w;
+            }
+          }
+        } else {
+          if (lane == sp) stk = child + 1;
+          sp++;
+          n = child;
+          continue;
         }
-        continue;
       }
-      stk[sp++][lane] = child;
-      stk[sp++][lane] = child + 1;
+      if (sp == 0) break;
+      sp = __builtin_amdgcn_readfirstlane(sp - 1);
+      n = __builtin_amdgcn_readlane(stk, sp);
     }
-    float* out = a.out + (size_t)row * nz;
-    int wind = 0;
-    for (int k = 0; k < nz; k++) {
-      wind += diff[k];
-      if (wind != 0) out[k] = -out[k];
+    if (act) {
+      float* out = a.out + ((size_t)i * ny + j) * nz;
+      int wind = 0;
+      for (int k = 0; k < nz; k++) {
+        wind += diff[k];
+        if (wind != 0) out[k] = -out[k];
+      }
     }
   }
 }
@@ -2525,9 +2547,8 @@ hipError_t kgl_msdf_dist(const MeshSdfArgs& a, hipStream_t s) {
 }
 
 hipError_t kgl_msdf_sign(const MeshSdfArgs& a, hipStream_t s) {
-  const long rows = (long)a.dims[0] * a.dims[1];
-  if (rows <= 0) return hipSuccess;
-  const long nb = (rows + 63) / 64;
+  const long nb = (long)((a.dims[0] + 7) >> 3) * ((a.dims[1] + 7) >> 3);
+  if (nb <= 0) return hipSuccess;
   const int blocks = (int)(nb < (1l << 20) ? nb : (1l << 20));
   k_msdf_sign<<<blocks, 64, 0, s>>>(a);
   return hipGetLastError();
