@@ -304,6 +304,31 @@ def run_secondary(args, ctx, sc, world, robot):
         del g64
     print(f"[bench] C5 SDF: {out['c5_sdf_grad']}", file=sys.stderr, flush=True)
 
+    # PointCloud vs ImplicitSurface distance: the 1M-point cloud, moved by a
+    # transform, against the same 256^3 grid (Geometry3D.distance)
+    pcg = engine.Geom.pointcloud(ctx, P)
+    Rz = np.array([[0.8, -0.6, 0.0], [0.6, 0.8, 0.0], [0.0, 0.0, 1.0]])
+    pcg.set_transform(Rz.T.reshape(9), np.array([0.01, -0.02, 0.03]))
+    pcg.distance(sdf)
+    t0 = time.perf_counter()
+    for _ in range(10):
+        rpc = pcg.distance(sdf)
+    tpc = (time.perf_counter() - t0) / 10
+    out["c7_pointcloud_sdf"] = {"points": int(P.shape[0]), "grid": list(grid.values.shape), "ms": round(tpc * 1e3, 3),
+                                "points_per_s": round(P.shape[0] / tpc, 1), "min_value": round(float(rpc.d), 6),
+                                "note": "host call: one kernel, a 64-bit result copy and a gradient evaluation"}
+    if not args.no_cpu:
+        g64 = S.SDFGrid(grid.bmin, grid.bmax, grid.values.astype(np.float64))
+        sub = P[:200_000] @ Rz.T + np.array([0.01, -0.02, 0.03])
+        t1 = time.perf_counter()
+        dref, _ = oracle.sdf_eval(g64, sub)
+        tc = time.perf_counter() - t1
+        out["c7_pointcloud_sdf"]["cpu_points_per_s"] = round(len(sub) / tc, 1)
+        out["c7_pointcloud_sdf"]["cpu_sample"] = f"first {len(sub)} points, FP64 oracle trilinear values, 1 thread"
+        del g64
+    pcg.close()
+    print(f"[bench] C7 pointcloud-SDF: {out['c7_pointcloud_sdf']}", file=sys.stderr, flush=True)
+
     # C3 heterogeneous robots in one launch
     soup_geom = world._keep[0]
     robots, counts = S.hetero_robots(seed=0, nrobots=64)

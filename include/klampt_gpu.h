@@ -76,6 +76,16 @@ int kg_geom_create_trimesh(kg_ctx ctx, const double* verts, int nverts, const in
 int kg_geom_create_sdf(kg_ctx ctx, const double bmin[3], const double bmax[3], const int32_t dims[3],
                        const float* values, kg_geom* out);
 int kg_geom_destroy(kg_geom g);
+/* PointCloud (Python/klampt/src/geometry.h PointCloud; the points only): pts
+ * npts x 3 doubles in the geometry's local frame.  Point clouds take part in
+ * the single-pair queries against ImplicitSurface geometries
+ * (Cpp/docs/Manual-Geometry.md:134,293-294,341-342): collide = some point's
+ * surface value <= the summed margins, withinDistance = <= tol + margins,
+ * distance = the smallest value minus the margins, with that point, the
+ * surface point along the gradient, and the gradient.  A TriangleMesh against
+ * an ImplicitSurface collides (withinDistance) by its vertices the same way;
+ * its distance is not available (as in the reference's table). */
+int kg_geom_create_pointcloud(kg_ctx ctx, const double* pts, int npts, kg_geom* out);
 /* TriangleMesh -> ImplicitSurface: Geometry3D.convert("ImplicitSurface",
  * resolution) (Python/klampt/src/geometry.h:944-1005, robotsim.cpp:1602-1646).
  * Cell-centred grid of spacing `resolution` (0: the average longest triangle

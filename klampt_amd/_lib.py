@@ -42,6 +42,7 @@ SIGNATURES = {
     "kg_geom_create_sdf": [_vp, _pd, _pd, _pi, _pf, _pvp],
     "kg_geom_destroy": [_vp],
     "kg_geom_clone": [_vp, _pvp],
+    "kg_geom_create_pointcloud": [_vp, _pd, _c_int, _pvp],
     "kg_geom_mesh_to_sdf": [_vp, _c_double, _c_int, _pvp],
     "kg_geom_get_sdf": [_vp, _pi, _pd, _pd, _pf],
     "kg_geom_set_transform": [_vp, _pd, _pd],

@@ -887,6 +887,22 @@ int kg_geom_destroy(kg_geom g) {
   });
 }
 
+int kg_geom_create_pointcloud(kg_ctx c, const double* P, int n, kg_geom* out) {
+  return guard([&] {
+    need(c && out, "NULL argument");
+    need(n >= 0, "negative size");
+    need(n == 0 || P, "NULL point array");
+    for (int i = 0; i < 3 * n; i++) need(std::isfinite(P[i]), "non-finite point coordinate");
+    kg_geom_s* g = new kg_geom_s();
+    g->ctx = c;
+    g->kind = 2;
+    g->index = -1;
+    g->X = xf_identity();
+    g->V.assign(P, P + 3 * (size_t)n);
+    *out = g;
+  });
+}
+
 int kg_geom_mesh_to_sdf(kg_geom g, double resolution, int pad, kg_geom* out) {
   return guard([&] {
     need(g && out, "NULL argument");
@@ -1064,6 +1080,20 @@ int kg_geom_get_bb(kg_geom g, double bmin[3], double bmax[3]) {
         d.le[k] = 0.5 * (g->bmax[k] - g->bmin[k]);
       }
       world_aabb(d, g->X, bmin, bmax);
+      return;
+    }
+    if (g->kind == 2) {
+      // transformed points
+      for (int k = 0; k < 3; k++) {
+        bmin[k] = INFINITY;
+        bmax[k] = -INFINITY;
+      }
+      for (size_t i = 0; i + 2 < g->V.size(); i += 3)
+        for (int k = 0; k < 3; k++) {
+          const double w = g->X.R[k] * g->V[i] + g->X.R[3 + k] * g->V[i + 1] + g->X.R[6 + k] * g->V[i + 2] + g->X.t[k];
+          bmin[k] = std::min(bmin[k], w);
+          bmax[k] = std::max(bmax[k], w);
+        }
       return;
     }
     const GeomDesc& d = g->ctx->geoms[g->index];
@@ -1562,6 +1592,62 @@ PairWorld& pair_world(kg_ctx_s* c, kg_geom_s* a, kg_geom_s* b) {
 
 namespace {
 void pair_distance(kg_geom_s* a, kg_geom_s* b, float stop_below, double upper, kg_dist_result* out, float* dgpu);
+SdfDev sdf_dev(kg_geom_s* g);
+
+// Point sets against implicit surfaces: a PointCloud's points or a
+// TriangleMesh's vertices, each moved by its geometry's transform and
+// evaluated in the surface's grid.  A pair qualifies when exactly one side is
+// an ImplicitSurface; `pts` is then the other side.
+bool point_set_pair(kg_geom_s* a, kg_geom_s* b, kg_geom_s*& pts, kg_geom_s*& sdf) {
+  if (a->kind == 1 && b->kind != 1) {
+    sdf = a;
+    pts = b;
+    return true;
+  }
+  if (b->kind == 1 && a->kind != 1) {
+    sdf = b;
+    pts = a;
+    return true;
+  }
+  return false;
+}
+
+bool point_set_empty(const kg_geom_s* g) { return g->kind == 0 ? geom_empty(g) : g->V.empty(); }
+
+const float4* point_set(kg_geom_s* g, int& n) {
+  n = (int)(g->V.size() / 3);
+  if (!g->dpts.p) {
+    std::vector<float4> P(n);
+    for (int i = 0; i < n; i++) P[i] = make_float4((float)g->V[3 * i], (float)g->V[3 * i + 1], (float)g->V[3 * i + 2], 0.f);
+    hip_check(g->dpts.ensure((size_t)n * sizeof(float4)), "hipMalloc");
+    hip_check(hipMemcpy(g->dpts.p, P.data(), (size_t)n * sizeof(float4), hipMemcpyHostToDevice), "copy points");
+  }
+  return g->dpts.as<float4>();
+}
+
+// Smallest implicit-surface value over the point set; *index = its point.
+double pts_sdf_min(kg_geom_s* pts, kg_geom_s* sdf, int* index) {
+  kg_ctx_s* c = pts->ctx;
+  hip_check(hipSetDevice(c->device), "hipSetDevice");
+  int n = 0;
+  const float4* P = point_set(pts, n);
+  StreamScope scope(c, c->stream);
+  float X[12];
+  for (int i = 0; i < 9; i++) X[i] = (float)pts->X.R[i];
+  for (int i = 0; i < 3; i++) X[9 + i] = (float)pts->X.t[i];
+  hip_check(c->wsG.ensure(64), "hipMalloc");
+  unsigned long long key = ~0ull;
+  hip_check(hipMemcpyAsync(c->wsG.p, &key, sizeof(key), hipMemcpyHostToDevice, c->stream), "copy");
+  hip_check(kgl_pts_sdf_min(P, n, X, sdf_dev(sdf), c->wsG.as<unsigned long long>(), c->stream), "pts_sdf_min");
+  hip_check(hipMemcpyAsync(&key, c->wsG.p, sizeof(key), hipMemcpyDeviceToHost, c->stream), "copy");
+  hip_check(hipStreamSynchronize(c->stream), "sync");
+  uint32_t u = (uint32_t)(key >> 32);
+  u = (u & 0x80000000u) ? (u & 0x7fffffffu) : ~u;
+  float v;
+  std::memcpy(&v, &u, sizeof(v));
+  *index = (int)(key & 0xffffffffu);
+  return v;
+}
 }  // namespace
 
 extern "C" int kg_geom_collide(kg_geom a, kg_geom b, int* out) {
@@ -1569,7 +1655,16 @@ extern "C" int kg_geom_collide(kg_geom a, kg_geom b, int* out) {
     need(a && b && out, "NULL argument");
     need(a->ctx == b->ctx, "geometries belong to different contexts");
     *out = 0;
-    if (a->kind != 0 || b->kind != 0) fail(KG_E_UNSUPPORTED, "collide is implemented for TriangleMesh pairs");
+    if (a->kind != 0 || b->kind != 0) {
+      kg_geom_s *p = nullptr, *f = nullptr;
+      if (!point_set_pair(a, b, p, f))
+        fail(KG_E_UNSUPPORTED,
+             "collide is implemented for TriangleMesh pairs and PointCloud / TriangleMesh against ImplicitSurface");
+      if (point_set_empty(p)) return;
+      int idx = 0;
+      *out = pts_sdf_min(p, f, &idx) <= a->margin + b->margin ? 1 : 0;
+      return;
+    }
     if (geom_empty(a) || geom_empty(b)) return;  // Geometry3D.collides: empty -> False
     const double m = a->margin + b->margin;
     if (m > 0) {
@@ -1944,7 +2039,38 @@ extern "C" int kg_geom_distance(kg_geom a, kg_geom b, const kg_dist_settings* se
   return guard([&] {
     need(a && b && out, "NULL argument");
     need(a->ctx == b->ctx, "geometries belong to different contexts");
-    if (a->kind != 0 || b->kind != 0) fail(KG_E_UNSUPPORTED, "distance is implemented for TriangleMesh pairs");
+    if (a->kind != 0 || b->kind != 0) {
+      kg_geom_s *p = nullptr, *f = nullptr;
+      if (!point_set_pair(a, b, p, f) || p->kind != 2)
+        fail(KG_E_UNSUPPORTED, "distance is implemented for TriangleMesh pairs and PointCloud / ImplicitSurface");
+      if (point_set_empty(p))
+        fail(KG_E_EMPTY, p == a ? "Geometry3D.distance: Geometry is empty"
+                                : "Geometry3D.distance: Other geometry is empty");
+      int idx = 0;
+      const double v = pts_sdf_min(p, f, &idx);
+      // the minimising point in world coordinates, and the surface gradient there
+      const double* q = &p->V[3 * (size_t)idx];
+      double w[3];
+      for (int k = 0; k < 3; k++) w[k] = p->X.R[k] * q[0] + p->X.R[3 + k] * q[1] + p->X.R[6 + k] * q[2] + p->X.t[k];
+      float wf[3] = {(float)w[0], (float)w[1], (float)w[2]}, dv = 0, gr[3] = {0, 0, 0};
+      const int rc = kg_sdf_eval(f, wf, 1, &dv, gr);
+      if (rc != KG_OK) fail(rc, "%s", g_err.c_str());
+      std::memset(out, 0, sizeof(*out));
+      out->d = v - (a->margin + b->margin);
+      out->hasClosestPoints = 1;
+      out->hasGradients = 1;
+      const bool pa = p == a;
+      for (int k = 0; k < 3; k++) {
+        const double surf = w[k] - v * gr[k];
+        out->cp1[k] = pa ? w[k] : surf;
+        out->cp2[k] = pa ? surf : w[k];
+        out->grad1[k] = pa ? gr[k] : -gr[k];
+        out->grad2[k] = pa ? -gr[k] : gr[k];
+      }
+      out->elem1 = pa ? idx : -1;
+      out->elem2 = pa ? -1 : idx;
+      return;
+    }
     if (geom_empty(a)) fail(KG_E_EMPTY, "Geometry3D.distance: Geometry is empty");
     if (geom_empty(b)) fail(KG_E_EMPTY, "Geometry3D.distance: Other geometry is empty");
     const double m = a->margin + b->margin;
@@ -1960,7 +2086,17 @@ extern "C" int kg_geom_within_distance(kg_geom a, kg_geom b, double tol, int* ou
     need(a && b && out, "NULL argument");
     need(tol >= 0, "tolerance must be >= 0");
     *out = 0;
-    if (a->kind != 0 || b->kind != 0) fail(KG_E_UNSUPPORTED, "withinDistance is implemented for TriangleMesh pairs");
+    if (a->kind != 0 || b->kind != 0) {
+      kg_geom_s *p = nullptr, *f = nullptr;
+      if (!point_set_pair(a, b, p, f))
+        fail(KG_E_UNSUPPORTED,
+             "withinDistance is implemented for TriangleMesh pairs and PointCloud / TriangleMesh against "
+             "ImplicitSurface");
+      if (point_set_empty(p)) return;
+      int idx = 0;
+      *out = pts_sdf_min(p, f, &idx) <= tol + a->margin + b->margin ? 1 : 0;
+      return;
+    }
     if (geom_empty(a) || geom_empty(b)) return;
     const double thr = tol + a->margin + b->margin;
     kg_dist_result r;

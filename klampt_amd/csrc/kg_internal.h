@@ -228,7 +228,7 @@ hipError_t kgl_edge_reduce(const int* seg, const int* soff, int nlive, int total
 hipError_t kgl_fill_int(int* p, int n, int v, hipStream_t s);
 
 // TriangleMesh -> ImplicitSurface: the grid of cell-centred signed distances.
-#define KG_MSDF_STK 64  // traversal stack entries, one per lane (tree depth + 2 at most)
+#define KG_MSDF_STK 64  // traversal stack entries (tree depth + 2 at most)
 struct MeshSdfArgs {
   PoolDev pool;
   int root;          // pool index of the mesh's root node
@@ -239,6 +239,11 @@ struct MeshSdfArgs {
   int* diff;         // same size: winding increments (zeroed before k_msdf_sign)
 };
 hipError_t kgl_msdf_dist(const MeshSdfArgs& a, hipStream_t s);
+// Minimum implicit-surface value over a point set: out = min over i of
+// key(value(X * P[i]), i), key = order-preserving float bits << 32 | i
+// (*out must hold ~0 before the call).
+hipError_t kgl_pts_sdf_min(const float4* P, int n, const float X[12], const SdfDev& g, unsigned long long* out,
+                           hipStream_t s);
 hipError_t kgl_msdf_sign(const MeshSdfArgs& a, hipStream_t s);
 int kgl_collide_grid(int exact, int num_cus, int scap);
 hipError_t kgl_sdf_eval(const SdfDev& g, const float* pts, int n, float* d, float* grad, hipStream_t s);
@@ -260,7 +265,7 @@ struct Xf64 {
 
 struct kg_geom_s {
   kg_ctx_s* ctx;
-  int kind;        // 0 triangle mesh, 1 SDF
+  int kind;        // 0 triangle mesh, 1 SDF, 2 point cloud
   int index;       // GeomDesc index (mesh)
   Xf64 X;          // current transform
   // mesh data (kept for host-side queries)
@@ -270,6 +275,7 @@ struct kg_geom_s {
   int dims[3] = {0, 0, 0};
   double bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
   DevBuf dvals;
+  DevBuf dpts;     // points (point cloud) or mesh vertices as float4, uploaded on first use
   bool alive = true;
   double margin = 0;  // collision margin (Geometry3D.setCollisionMargin)
 };

@@ -2322,12 +2322,14 @@ hipError_t kgl_edge_reduce(const int* seg, const int* soff, int nlive, int total
 // holds one entry per lane (64 deep).  FP32; the unsigned distance is
 // written to out.
 //
-// k_msdf_sign: one lane per row (i, j) of cells along z, one wave per 8 x 8
-// block of rows walking the tree together.  The +z line through each row's
-// centres is tested against the tree's boxes (FP32, padded: a conservative
-// cull) and against the triangles in FP64 with the crossing rule below; each
-// crossing adds its winding increment at the first cell above it, and the
-// lane negates the cells of its row whose running winding number is nonzero.
+// k_msdf_sign: one lane per row (i, j) of cells along z, each lane with its
+// own traversal (a line's boxes are not shared enough between neighbouring
+// rows for a wave-wide walk to pay: that variant took 4x longer).  The +z
+// line through the row's centres is tested against the tree's boxes (FP32,
+// padded: a conservative cull) and against the triangles in FP64 with the
+// crossing rule below; each crossing adds its winding increment at the first
+// cell above it, and the lane negates the cells of its row whose running
+// winding number is nonzero.
 //
 // Crossing rule (the same expressions as the oracle, no contraction, so the
 // winding numbers are identical): a triangle whose xy projection has signed
@@ -2348,7 +2350,9 @@ __device__ inline void node_load(const ObbNode* nodes, int n, float c[3], float 
   child = __float_as_int(f.w);
 }
 
-__global__ __launch_bounds__(64) void k_msdf_dist(MeshSdfArgs a) {
+__global__ __launch_bounds__(64) void k_msdf_dist(MeshSdfArgs a, const ObbNode* __restrict__ nodes,
+                                                  const float4* __restrict__ tv0, const float4* __restrict__ tv1,
+                                                  const float4* __restrict__ tv2, float* __restrict__ out) {
   const int lane = threadIdx.x;
   const int nx = a.dims[0], ny = a.dims[1], nz = a.dims[2];
   const long bx = (nx + 3) >> 2, by = (ny + 3) >> 2, bz = (nz + 3) >> 2;
@@ -2373,7 +2377,7 @@ __global__ __launch_bounds__(64) void k_msdf_dist(MeshSdfArgs a) {
       n = __builtin_amdgcn_readfirstlane(n);
       float c[3], e[3], R[9];
       int child;
-      node_load(a.pool.nodes, n, c, e, R, child);
+      node_load(nodes, n, c, e, R, child);
       const float dx = p.x - c[0], dy = p.y - c[1], dz = p.z - c[2];
       float lb = 0.f;
 #pragma unroll
@@ -2387,14 +2391,14 @@ __global__ __launch_bounds__(64) void k_msdf_dist(MeshSdfArgs a) {
       if (__ballot(need)) {
         if (child < 0) {
           const int t = ~child;
-          const float4 A = a.pool.tv0[t], B = a.pool.tv1[t], C = a.pool.tv2[t];
+          const float4 A = tv0[t], B = tv1[t], C = tv2[t];
           const kgm::V<float> cp = kgm::closest_on_tri(p, kgm::mk(A.x, A.y, A.z), kgm::mk(B.x, B.y, B.z),
                                                        kgm::mk(C.x, C.y, C.z));
           const kgm::V<float> d = kgm::sub(p, cp);
           best = fminf(best, kgm::dot(d, d));
         } else {
-          const float4 c0 = reinterpret_cast<const float4*>(a.pool.nodes + child)[0];
-          const float4 c1 = reinterpret_cast<const float4*>(a.pool.nodes + child + 1)[0];
+          const float4 c0 = reinterpret_cast<const float4*>(nodes + child)[0];
+          const float4 c1 = reinterpret_cast<const float4*>(nodes + child + 1)[0];
           const float d0 = (qx - c0.x) * (qx - c0.x) + (qy - c0.y) * (qy - c0.y) + (qz - c0.z) * (qz - c0.z);
           const float d1 = (qx - c1.x) * (qx - c1.x) + (qy - c1.y) * (qy - c1.y) + (qz - c1.z) * (qz - c1.z);
           const bool first0 = d0 <= d1;
@@ -2408,7 +2412,7 @@ __global__ __launch_bounds__(64) void k_msdf_dist(MeshSdfArgs a) {
       sp = __builtin_amdgcn_readfirstlane(sp - 1);
       n = __builtin_amdgcn_readlane(stk, sp);
     }
-    if (act) a.out[((size_t)i * ny + j) * nz + k] = sqrtf(best);
+    if (act) out[((size_t)i * ny + j) * nz + k] = sqrtf(best);
   }
 }
 
@@ -2465,19 +2469,19 @@ __device__ inline int msdf_first_above(double zt, double bz, double h, int nz) {
 }
 
 __global__ __launch_bounds__(64) void k_msdf_sign(MeshSdfArgs a) {
+  __shared__ int stk[KG_MSDF_STK][64];
   const int lane = threadIdx.x;
-  const int nx = a.dims[0], ny = a.dims[1], nz = a.dims[2];
-  const long tx = (nx + 7) >> 3, ty = (ny + 7) >> 3;
-  for (long tile = blockIdx.x; tile < tx * ty; tile += gridDim.x) {
-    // one wave per 8 x 8 block of rows
-    const int i = (int)(tile / ty) * 8 + (lane >> 3), j = (int)(tile % ty) * 8 + (lane & 7);
-    const bool act = i < nx && j < ny;
-    const double qx = msdf_centre(a.bmin[0], min(i, nx - 1), a.h), qy = msdf_centre(a.bmin[1], min(j, ny - 1), a.h);
+  const int ny = a.dims[1], nz = a.dims[2];
+  const long rows = (long)a.dims[0] * ny;
+  for (long row = (long)blockIdx.x * 64 + lane; row < rows; row += (long)gridDim.x * 64) {
+    const int i = (int)(row / ny), j = (int)(row % ny);
+    const double qx = msdf_centre(a.bmin[0], i, a.h), qy = msdf_centre(a.bmin[1], j, a.h);
     const float qxf = (float)qx, qyf = (float)qy;
-    int* diff = a.diff + ((size_t)min(i, nx - 1) * ny + min(j, ny - 1)) * nz;
-    int stk = 0, sp = 0, n = a.root;
-    for (;;) {
-      n = __builtin_amdgcn_readfirstlane(n);
+    int* diff = a.diff + (size_t)row * nz;
+    int sp = 0;
+    stk[sp++][lane] = a.root;
+    while (sp > 0) {
+      const int n = stk[--sp][lane];
       float c[3], e[3], R[9];
       int child;
       node_load(a.pool.nodes, n, c, e, R, child);
@@ -2485,7 +2489,7 @@ __global__ __launch_bounds__(64) void k_msdf_sign(MeshSdfArgs a) {
       const float dx = qxf - c[0], dy = qyf - c[1], dz = -c[2];
       const float pad = 1e-5f * (1.f + fabsf(qxf) + fabsf(qyf) + fabsf(c[0]) + fabsf(c[1]) + fabsf(c[2]));
       float tlo = -3.0e38f, thi = 3.0e38f;
-      bool hit = act;
+      bool hit = true;
 #pragma unroll
       for (int q = 0; q < 3; q++) {
         const float s0 = R[3 * q] * dx + R[3 * q + 1] * dy + R[3 * q + 2] * dz;
@@ -2503,35 +2507,25 @@ __global__ __launch_bounds__(64) void k_msdf_sign(MeshSdfArgs a) {
           thi = fminf(thi, t2);
         }
       }
-      hit = hit && tlo <= thi;
-      if (__ballot(hit)) {
-        if (child < 0) {
-          if (hit) {
-            double zt = 0;
-            const int w = msdf_crossing(a.pool.tv64 + 9 * (size_t)(~child), qx, qy, zt);
-            if (w) {
-              const int k0 = msdf_first_above(zt, a.bmin[2], a.h, nz);
-              if (k0 < nz) diff[k0] += w;
-            }
-          }
-        } else {
-          if (lane == sp) stk = child + 1;
-          sp++;
-          n = child;
-          continue;
+      if (!hit || tlo > thi) continue;
+      if (child < 0) {
+        const int t = ~child;
+        double zt = 0;
+        const int w = msdf_crossing(a.pool.tv64 + 9 * (size_t)t, qx, qy, zt);
+        if (w) {
+          const int k0 = msdf_first_above(zt, a.bmin[2], a.h, nz);
+          if (k0 < nz) diff[k0] += w;
         }
+        continue;
       }
-      if (sp == 0) break;
-      sp = __builtin_amdgcn_readfirstlane(sp - 1);
-      n = __builtin_amdgcn_readlane(stk, sp);
+      stk[sp++][lane] = child;
+      stk[sp++][lane] = child + 1;
     }
-    if (act) {
-      float* out = a.out + ((size_t)i * ny + j) * nz;
-      int wind = 0;
-      for (int k = 0; k < nz; k++) {
-        wind += diff[k];
-        if (wind != 0) out[k] = -out[k];
-      }
+    float* out = a.out + (size_t)row * nz;
+    int wind = 0;
+    for (int k = 0; k < nz; k++) {
+      wind += diff[k];
+      if (wind != 0) out[k] = -out[k];
     }
   }
 }
@@ -2542,14 +2536,75 @@ hipError_t kgl_msdf_dist(const MeshSdfArgs& a, hipStream_t s) {
   const long nb = (long)((a.dims[0] + 3) >> 2) * ((a.dims[1] + 3) >> 2) * ((a.dims[2] + 3) >> 2);
   if (nb <= 0) return hipSuccess;
   const int blocks = (int)(nb < (1l << 20) ? nb : (1l << 20));
-  k_msdf_dist<<<blocks, 64, 0, s>>>(a);
+  k_msdf_dist<<<blocks, 64, 0, s>>>(a, a.pool.nodes, a.pool.tv0, a.pool.tv1, a.pool.tv2, a.out);
   return hipGetLastError();
 }
 
 hipError_t kgl_msdf_sign(const MeshSdfArgs& a, hipStream_t s) {
-  const long nb = (long)((a.dims[0] + 7) >> 3) * ((a.dims[1] + 7) >> 3);
-  if (nb <= 0) return hipSuccess;
+  const long rows = (long)a.dims[0] * a.dims[1];
+  if (rows <= 0) return hipSuccess;
+  const long nb = (rows + 63) / 64;
   const int blocks = (int)(nb < (1l << 20) ? nb : (1l << 20));
   k_msdf_sign<<<blocks, 64, 0, s>>>(a);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- point sets vs implicit surfaces
+//
+// PointCloud / mesh-vertex sets against an ImplicitSurface: every point is
+// moved to world coordinates by the set's transform and evaluated in the
+// surface's grid (trilinear, as k_sdf_eval); the minimum value and its point
+// index are reduced over the wave, the block and the grid (one 64-bit
+// atomicMin per block on an order-preserving key).
+
+namespace {
+
+struct PtsXf {
+  float v[12];  // column-major R, then t
+};
+
+__device__ inline unsigned long long pts_key(float v, int i) {
+  unsigned u = __float_as_uint(v);
+  u = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+  return ((unsigned long long)u << 32) | (unsigned)i;
+}
+
+__global__ __launch_bounds__(256) void k_pts_sdf_min(const float4* __restrict__ P, int n, PtsXf X, SdfDev g,
+                                                     unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long s_min[4];
+  unsigned long long best = ~0ull;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    const float4 p = P[i];
+    const float* R = X.v;
+    const float wx = R[0] * p.x + R[3] * p.y + R[6] * p.z + R[9];
+    const float wy = R[1] * p.x + R[4] * p.y + R[7] * p.z + R[10];
+    const float wz = R[2] * p.x + R[5] * p.y + R[8] * p.z + R[11];
+    const unsigned long long k = pts_key(sdf_world(g, wx, wy, wz, nullptr), i);
+    best = k < best ? k : best;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long other = __shfl_xor(best, o);
+    best = other < best ? other : best;
+  }
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) s_min[wave] = best;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long m = s_min[0];
+    for (int w = 1; w < 4; w++) m = s_min[w] < m ? s_min[w] : m;
+    if (m != ~0ull) atomicMin(out, m);
+  }
+}
+
+}  // namespace
+
+hipError_t kgl_pts_sdf_min(const float4* P, int n, const float X[12], const SdfDev& g, unsigned long long* out,
+                           hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  PtsXf x;
+  for (int i = 0; i < 12; i++) x.v[i] = X[i];
+  const int blocks = (n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024;
+  k_pts_sdf_min<<<blocks, 256, 0, s>>>(P, n, x, g, out);
   return hipGetLastError();
 }

@@ -89,6 +89,13 @@ class Geom:
         return cls(ctx, h, "TriangleMesh")
 
     @classmethod
+    def pointcloud(cls, ctx, points):
+        P = _d(points).reshape(-1, 3)
+        h = ctypes.c_void_p()
+        call("kg_geom_create_pointcloud", ctx.h, _p(P, _pd), P.shape[0], ctypes.byref(h))
+        return cls(ctx, h, "PointCloud")
+
+    @classmethod
     def sdf(cls, ctx, bmin, bmax, values):
         vals = _f(values)
         dims = _i(vals.shape)

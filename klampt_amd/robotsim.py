@@ -89,6 +89,30 @@ class TriangleMesh:
         self.indices = np.zeros((0, 3), np.int32) if indices is None else np.asarray(indices, np.int32).reshape(-1, 3)
 
 
+class PointCloud:
+    """Points (N,3) (``PointCloud``, geometry.h:178-253; properties are not
+    carried: the collision queries use the points only)."""
+
+    def __init__(self, points=None):
+        self.points = np.zeros((0, 3)) if points is None else np.asarray(points, np.float64).reshape(-1, 3)
+
+    def numPoints(self) -> int:
+        return int(self.points.shape[0])
+
+    def getPoints(self):
+        return self.points
+
+    def setPoints(self, points):
+        self.points = np.asarray(points, np.float64).reshape(-1, 3)
+
+    def addPoint(self, p) -> int:
+        self.points = np.vstack([self.points, np.asarray(p, np.float64).reshape(1, 3)])
+        return self.numPoints() - 1
+
+    def getPoint(self, index):
+        return list(self.points[index])
+
+
 class ImplicitSurface:
     """Cell-centred signed distance grid (geometry.h:322-391): values has shape
     dims, cell (i,j,k) covers [bmin + (i,j,k) h, bmin + (i+1,j+1,k+1) h)."""
@@ -121,8 +145,12 @@ class Geometry3D:
         elif isinstance(data, ImplicitSurface):
             self._g = engine.Geom.sdf(self._ctx, data.bmin, data.bmax, data.values)
             self._type = "ImplicitSurface"
+        elif isinstance(data, PointCloud):
+            self._g = engine.Geom.pointcloud(self._ctx, data.points)
+            self._type = "PointCloud"
+            self._pc = PointCloud(data.points.copy())
         else:
-            raise ValueError("Geometry3D supports TriangleMesh and ImplicitSurface data")
+            raise ValueError("Geometry3D supports TriangleMesh, PointCloud and ImplicitSurface data")
 
     def type(self) -> str:
         return self._type
@@ -176,8 +204,14 @@ class Geometry3D:
         dims, lo, hi, v = self._g.sdf_grid(values=True)
         return ImplicitSurface(lo, hi, v)
 
+    def getPointCloud(self) -> PointCloud:
+        if self._type != "PointCloud":
+            raise RuntimeError("Geometry3D is not a PointCloud")
+        return PointCloud(self._pc.points.copy())
+
     def empty(self) -> bool:
-        return self._g is None or (self._type == "TriangleMesh" and self._mesh.indices.shape[0] == 0)
+        return (self._g is None or (self._type == "TriangleMesh" and self._mesh.indices.shape[0] == 0)
+                or (self._type == "PointCloud" and self._pc.numPoints() == 0))
 
     def setCurrentTransform(self, R: Sequence[float], t: Sequence[float]):
         self._g.set_transform(np.asarray(R, np.float64), np.asarray(t, np.float64))

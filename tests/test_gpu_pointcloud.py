@@ -1,0 +1,117 @@
+"""PointCloud and TriangleMesh (by its vertices) against ImplicitSurface on
+the GPU: the smallest surface value over the transformed points equals the
+oracle's FP64 trilinear value within 1e-5 m; collide / withinDistance /
+distance follow it, with the minimising point, its surface point and the
+gradient; unsupported pairs and empty geometry behave as the reference's
+table and Geometry3D rules say."""
+import numpy as np
+import pytest
+
+import oracle
+from klampt_amd import robotsim, synthetic as S
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+def _sphere_grid(r=0.3, h=0.02, n=40):
+    lo = -h * n / 2
+    c = lo + (np.arange(n) + 0.5) * h
+    X, Y, Z = np.meshgrid(c, c, c, indexing="ij")
+    vals = (np.sqrt(X ** 2 + Y ** 2 + Z ** 2) - r).astype(np.float32)
+    return robotsim.ImplicitSurface([lo] * 3, [lo + n * h] * 3, vals)
+
+
+def _xf(rng, scale=0.2):
+    return S.random_rotation(rng), rng.uniform(-scale, scale, 3)
+
+
+def _oracle_min(grid, pts, Ra, ta, Rb, tb):
+    w = pts @ Ra.T + ta
+    local = (w - tb) @ Rb            # Rb^T (w - tb)
+    d, _ = oracle.sdf_eval(grid, local)
+    return d
+
+
+def _set(g, R, t):
+    g.setCurrentTransform(list(S.rot_colmajor(R)), list(t))
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_pointcloud_vs_implicit_surface_matches_oracle(seed):
+    rng = np.random.default_rng(seed)
+    grid = _sphere_grid()
+    sdf = robotsim.Geometry3D(grid)
+    pts = rng.normal(size=(5000, 3)) * 0.15 + [0.3, 0, 0]
+    pc = robotsim.Geometry3D(robotsim.PointCloud(pts))
+    Ra, ta = _xf(rng)
+    Rb, tb = _xf(rng)
+    _set(pc, Ra, ta)
+    _set(sdf, Rb, tb)
+    want = _oracle_min(grid, pts, Ra, ta, Rb, tb)
+    m = float(want.min())
+    r = pc.distance(sdf)
+    assert abs(r.d - m) <= TOL
+    assert abs(want[r.elem1] - m) <= TOL and r.elem2 == -1
+    wpt = pts[r.elem1] @ Ra.T + ta
+    assert np.allclose(r.cp1, wpt, atol=1e-6)
+    assert np.allclose(np.asarray(r.cp2), wpt - r.d * np.asarray(r.grad1), atol=1e-5)
+    # the gradient is the oracle's trilinear gradient at that point (rotated
+    # to world axes), unless the point sits on a cell-centre plane where the
+    # piecewise-linear gradient jumps
+    local = (wpt - tb) @ Rb
+    u = (local - grid.bmin) / 0.02 - 0.5
+    if np.abs(u - np.round(u)).min() > 1e-3:
+        _, go = oracle.sdf_eval(grid, local[None])
+        assert np.allclose(r.grad1, Rb @ go[0], atol=1e-3)
+    assert np.allclose(r.grad2, -np.asarray(r.grad1))
+    # the reversed pair swaps the roles
+    r2 = sdf.distance(pc)
+    assert abs(r2.d - r.d) <= 1e-7 and r2.elem2 == r.elem1 and np.allclose(r2.cp2, r.cp1)
+    if abs(m) > TOL:
+        assert pc.collides(sdf) == (m <= 0) and sdf.collides(pc) == (m <= 0)
+    assert pc.withinDistance(sdf, max(m, 0) + 0.01)
+    if m > 0.01 + TOL:
+        assert not pc.withinDistance(sdf, 0.0)
+    # margins shift the threshold
+    pc.setCollisionMargin(abs(m) + 0.01)
+    assert pc.collides(sdf)
+    assert abs(pc.distance(sdf).d - (m - abs(m) - 0.01)) <= TOL
+
+
+def test_mesh_vertices_vs_implicit_surface():
+    rng = np.random.default_rng(5)
+    grid = _sphere_grid()
+    sdf = robotsim.Geometry3D(grid)
+    box = S.box_mesh([0.1, 0.1, 0.1], edge=0.05)
+    tm = robotsim.Geometry3D(robotsim.TriangleMesh(box.verts, box.tris))
+    hits = 0
+    for k in range(24):
+        Ra, ta = S.random_rotation(rng), rng.uniform(-0.5, 0.5, 3)
+        _set(tm, Ra, ta)
+        m = float(_oracle_min(grid, box.verts, Ra, ta, np.eye(3), np.zeros(3)).min())
+        if abs(m) > TOL:
+            assert tm.collides(sdf) == (m <= 0), k
+            hits += m <= 0
+        assert tm.withinDistance(sdf, max(m, 0) + 1e-3)
+    assert 0 < hits < 24
+    with pytest.raises(RuntimeError):
+        tm.distance(sdf)     # TriangleMesh / ImplicitSurface distance is not available
+
+
+def test_unsupported_pairs_and_empty_geometry():
+    pc = robotsim.Geometry3D(robotsim.PointCloud(np.zeros((4, 3))))
+    box = S.box_mesh([0.1, 0.1, 0.1])
+    tm = robotsim.Geometry3D(robotsim.TriangleMesh(box.verts, box.tris))
+    sdf = robotsim.Geometry3D(_sphere_grid())
+    with pytest.raises(Exception):
+        pc.collides(tm)
+    with pytest.raises(Exception):
+        pc.collides(robotsim.Geometry3D(robotsim.PointCloud(np.ones((2, 3)))))
+    empty = robotsim.Geometry3D(robotsim.PointCloud(np.zeros((0, 3))))
+    assert not empty.collides(sdf) and not sdf.collides(empty)
+    with pytest.raises(RuntimeError, match="empty"):
+        empty.distance(sdf)
+    assert pc.getPointCloud().numPoints() == 4 and pc.type() == "PointCloud"
+    lo, hi = pc.getBB()
+    assert lo == [0, 0, 0] and hi == [0, 0, 0]
