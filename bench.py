@@ -232,6 +232,19 @@ def run_secondary(args, ctx, sc, world, robot):
         out["c2_distance"]["cpu_sample"] = f"{len(Qc)} configurations, FP64 oracle, {oracle.nthreads_default()} threads"
     print(f"[bench] C2 distances: {out['c2_distance']}", file=sys.stderr, flush=True)
 
+    # C8 distance Jacobians (KineTrajOpt's point-collision step): the same
+    # configurations and pairs, bound dcheck = 0.1 m, closest points and
+    # joint-space gradients of every pair within it
+    dj = (torch.empty((args.dist_configs, npairs), dtype=torch.float32, device=dev),
+          torch.empty((args.dist_configs, npairs, 6), dtype=torch.float32, device=dev),
+          torch.empty((args.dist_configs, npairs, sc.robot.n), dtype=torch.float32, device=dev))
+    t = _time_dev(lambda: world.distance_jacobian_batch_dev(Qd, 0.1, out=dj), 3)
+    within = int(torch.isfinite(dj[0]).sum().item())
+    out["c8_distance_jacobian"] = {"configs": args.dist_configs, "pairs_per_config": npairs, "dcheck": 0.1,
+                                   "pairs_within": within, "ms": round(t * 1e3, 3),
+                                   "pair_queries_per_s": round(args.dist_configs * npairs / t, 1)}
+    print(f"[bench] C8 distance Jacobians: {out['c8_distance_jacobian']}", file=sys.stderr, flush=True)
+
     # C4 edges: 8192 segments, L-infinity step 0.01 rad, 10..500 samples
     A, B = S.edge_segments(sc.robot, args.edges, eps=0.01, smin=10, smax=500, seed=8)
     t0 = time.perf_counter()

@@ -206,6 +206,25 @@ int kg_feasible_batch(kg_world w, int robot_index, const float* Q, int N, uint8_
 int kg_feasible_batch_dev(kg_world w, int robot_index, const float* dQ, int N, uint8_t* d_feasible,
                           uint8_t* d_env_hit, uint8_t* d_self_hit, int32_t* d_first_pair, void* stream);
 
+/* Pair distances with closest points and joint-space gradients: the
+ * point-collision step of KineTrajOpt
+ * (Python/klampt/plan/kinetrajopt/kinetrajopt.py:530-556: distance_ext per
+ * link / obstacle pair with upperBound = dcheck, then the normal and
+ * robot.positionJacobian at the link's closest point), batched over
+ * configurations and the kg_world_pair_list pairs.  Per configuration c and
+ * pair p (row-major):
+ *   d_out[c][p]        distance, +inf when no point pair is within dcheck;
+ *   cp_out[c][p][6]    closest points in world coordinates: on the pair's
+ *                      robot link, then on the other side (0 when d = inf;
+ *                      when d = 0, twice a point common to the two closest
+ *                      triangles);
+ *   grad_out[c][p][n]  dd/dq = nhat . (J_a(cp_a) - J_b(cp_b)) with
+ *                      nhat = (cp_a - cp_b) / |cp_a - cp_b|; J_b = 0 for
+ *                      environment geometry; 0 when d = 0 or d = inf. */
+int kg_distance_jacobian_batch(kg_world w, int robot_index, const float* Q, int N, float dcheck, float* d_out,
+                               float* cp_out, float* grad_out);
+int kg_distance_jacobian_batch_dev(kg_world w, int robot_index, const float* dQ, int N, float dcheck, float* d_out,
+                                   float* cp_out, float* grad_out, void* stream);
 /* Per-pair minimum distance for the kg_world_pair_list pairs (AnyCollisionQuery
  * ::Distance, Cpp/Planning/PlannerSettings.cpp:109-115): out[N][npairs], 0 when
  * touching. */

@@ -42,6 +42,8 @@ SIGNATURES = {
     "kg_geom_create_sdf": [_vp, _pd, _pd, _pi, _pf, _pvp],
     "kg_geom_destroy": [_vp],
     "kg_geom_clone": [_vp, _pvp],
+    "kg_distance_jacobian_batch": [_vp, _c_int, _pf, _c_int, _c_float, _pf, _pf, _pf],
+    "kg_distance_jacobian_batch_dev": [_vp, _c_int, _vp, _c_int, _c_float, _vp, _vp, _vp, _vp],
     "kg_geom_create_pointcloud": [_vp, _pd, _c_int, _pvp],
     "kg_geom_mesh_to_sdf": [_vp, _c_double, _c_int, _pvp],
     "kg_geom_get_sdf": [_vp, _pi, _pd, _pd, _pf],

@@ -2011,6 +2011,83 @@ extern "C" int kg_distance_batch_dev(kg_world w, int ri, const float* dQ, int N,
   });
 }
 
+namespace {
+void distance_jacobian_dev(kg_world_s* w, int ri, const float* dQ, int N, float dcheck, float* d_out, float* cp_out,
+                           float* grad_out, hipStream_t s) {
+  kg_ctx_s* c = w->ctx;
+  need(ri >= 0 && ri < (int)w->robots.size(), "robot index out of range");
+  need(dcheck > 0, "dcheck must be > 0");
+  RobotPlan& P = get_plan(w, ri);
+  const int np = (int)P.pairs.size();
+  if (N <= 0 || np == 0) return;
+  hip_check(c->wsDl.ensure((size_t)N * np * sizeof(float)), "hipMalloc");
+  hip_check(c->wsEA.ensure((size_t)N * np * 2 * sizeof(int)), "hipMalloc");
+  distance_dev(w, ri, dQ, nullptr, N, c->wsDl.as<float>(), c->wsEA.as<int>(), 0.0f, dcheck, s);
+  DistJacArgs a{};
+  a.pool = c->pool;
+  a.plan = P.dev;
+  a.T64 = c->wsT64.as<double>();
+  a.N = N;
+  a.npairs = np;
+  a.n = P.dev.nlinks;
+  a.pair_a = P.dpa.as<int>();
+  a.pair_b = P.dpb.as<int>();
+  a.pair_kind = P.dpk.as<int>();
+  a.dist = c->wsDl.as<float>();
+  a.tri = c->wsEA.as<int>();
+  a.dcheck = dcheck;
+  a.d_out = d_out;
+  a.cp_out = cp_out;
+  a.grad_out = grad_out;
+  hip_check(kgl_dist_jac(a, s), "dist_jac");
+}
+}  // namespace
+
+extern "C" int kg_distance_jacobian_batch_dev(kg_world w, int ri, const float* dQ, int N, float dcheck, float* d_out,
+                                              float* cp_out, float* grad_out, void* stream) {
+  return guard([&] {
+    need(w != nullptr, "world is NULL");
+    need(N >= 0, "negative batch size");
+    need(N == 0 || (dQ && d_out && cp_out && grad_out), "NULL buffer");
+    ensure_status_host(w->ctx);
+    hipStream_t s = (hipStream_t)stream;
+    StreamScope scope(w->ctx, s);
+    distance_jacobian_dev(w, ri, dQ, N, dcheck, d_out, cp_out, grad_out, s);
+  });
+}
+
+extern "C" int kg_distance_jacobian_batch(kg_world w, int ri, const float* Q, int N, float dcheck, float* d_out,
+                                          float* cp_out, float* grad_out) {
+  return guard([&] {
+    need(w != nullptr, "world is NULL");
+    need(N >= 0, "negative batch size");
+    need(ri >= 0 && ri < (int)w->robots.size(), "robot index out of range");
+    if (N == 0) return;
+    need(Q && d_out && cp_out && grad_out, "NULL buffer");
+    kg_ctx_s* c = w->ctx;
+    ensure_status_host(c);
+    hip_check(hipSetDevice(c->device), "hipSetDevice");
+    StreamScope scope(c, c->stream);
+    RobotPlan& P = get_plan(w, ri);
+    const int n = w->robots[ri]->n, np = (int)P.pairs.size();
+    if (np == 0) return;
+    hip_check(c->wsQ.ensure((size_t)N * n * sizeof(float)), "hipMalloc");
+    hip_check(c->wsF.ensure((size_t)N * np * sizeof(float)), "hipMalloc");
+    hip_check(c->wsEB.ensure((size_t)N * np * 6 * sizeof(float)), "hipMalloc");
+    hip_check(c->wsEQ.ensure((size_t)N * np * n * sizeof(float)), "hipMalloc");
+    hip_check(hipMemcpyAsync(c->wsQ.p, Q, (size_t)N * n * sizeof(float), hipMemcpyHostToDevice, c->stream), "copy");
+    distance_jacobian_dev(w, ri, c->wsQ.as<float>(), N, dcheck, c->wsF.as<float>(), c->wsEB.as<float>(),
+                          c->wsEQ.as<float>(), c->stream);
+    hip_check(hipMemcpyAsync(d_out, c->wsF.p, (size_t)N * np * sizeof(float), hipMemcpyDeviceToHost, c->stream), "copy");
+    hip_check(hipMemcpyAsync(cp_out, c->wsEB.p, (size_t)N * np * 6 * sizeof(float), hipMemcpyDeviceToHost, c->stream),
+              "copy");
+    hip_check(hipMemcpyAsync(grad_out, c->wsEQ.p, (size_t)N * np * n * sizeof(float), hipMemcpyDeviceToHost, c->stream),
+              "copy");
+    hip_check(hipStreamSynchronize(c->stream), "sync");
+    if (fetch_status(c)[ST_SPILL]) fail(KG_E_LIMIT, "distance traversal exceeded its stack capacity (KG_SPILL_CAP)");
+  });
+}
+
 extern "C" int kg_distance_batch(kg_world w, int ri, const float* Q, int N, float* out) {
   return guard([&] {
     need(w != nullptr, "world is NULL");

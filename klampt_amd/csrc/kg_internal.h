@@ -239,6 +239,23 @@ struct MeshSdfArgs {
   int* diff;         // same size: winding increments (zeroed before k_msdf_sign)
 };
 hipError_t kgl_msdf_dist(const MeshSdfArgs& a, hipStream_t s);
+// Closest points and joint-space distance gradients of batched pair distances.
+struct DistJacArgs {
+  PoolDev pool;
+  PlanDev plan;
+  const double* T64;      // N x nlinks x 12 world frames (k_fk64 layout)
+  int N, npairs, n;
+  const int* pair_a;
+  const int* pair_b;
+  const int* pair_kind;
+  const float* dist;      // N x npairs (k_distance)
+  const int* tri;         // N x npairs x 2 closest triangle pair, pool numbering (-1: none within the bound)
+  float dcheck;
+  float* d_out;           // N x npairs
+  float* cp_out;          // N x npairs x 6
+  float* grad_out;        // N x npairs x n
+};
+hipError_t kgl_dist_jac(const DistJacArgs& a, hipStream_t s);
 // Minimum implicit-surface value over a point set: out = min over i of
 // key(value(X * P[i]), i), key = order-preserving float bits << 32 | i
 // (*out must hold ~0 before the call).

@@ -2608,3 +2608,112 @@ hipError_t kgl_pts_sdf_min(const float4* P, int n, const float X[12], const SdfD
   k_pts_sdf_min<<<blocks, 256, 0, s>>>(P, n, x, g, out);
   return hipGetLastError();
 }
+
+// ---------------------------------------------------------------- distance Jacobians
+//
+// For each (configuration, pair) of a batched distance call: the closest
+// points of the closest triangle pair (FP64, from the FP64 triangles and
+// frames) and the gradient of the distance in joint space,
+//   dd/dq = nhat . (dp_a/dq - dp_b/dq),  nhat = (p_a - p_b) / |p_a - p_b|,
+// with dp/dq_j = w_j x (p - o_j) for a revolute ancestor j of the point's
+// link (w_j = R_j axis_j, o_j = t_j in world coordinates) and w_j for a
+// prismatic one; environment points do not move.  Pairs with no triangle pair
+// within the bound get d = +inf, zero points and a zero gradient; touching or
+// intersecting pairs (d = 0) a zero gradient.
+
+namespace {
+
+__device__ inline void dj_chain(const LinkDesc* links, const double* T, int l, const double p[3], const double nh[3],
+                                double sgn, float* g) {
+  for (int j = l; j >= 0; j = links[j].parent) {
+    const double* X = T + 12 * j;
+    const double* ax = links[j].axis;
+    double w[3];
+    for (int r = 0; r < 3; r++) w[r] = X[r] * ax[0] + X[3 + r] * ax[1] + X[6 + r] * ax[2];
+    double v;
+    if (links[j].jtype == 1) {
+      v = nh[0] * w[0] + nh[1] * w[1] + nh[2] * w[2];
+    } else {
+      const double d0 = p[0] - X[9], d1 = p[1] - X[10], d2 = p[2] - X[11];
+      const double c0 = w[1] * d2 - w[2] * d1, c1 = w[2] * d0 - w[0] * d2, c2 = w[0] * d1 - w[1] * d0;
+      v = nh[0] * c0 + nh[1] * c1 + nh[2] * c2;
+    }
+    g[j] += (float)(sgn * v);
+  }
+}
+
+// A point where segment p0p1 meets triangle T (FP64), if it does.
+__device__ inline bool dj_seg_tri(const kgm::V<double>& p0, const kgm::V<double>& p1, const kgm::V<double>* T,
+                                  kgm::V<double>& x) {
+  const kgm::V<double> e1 = kgm::sub(T[1], T[0]), e2 = kgm::sub(T[2], T[0]);
+  const kgm::V<double> n = kgm::cross(e1, e2);
+  const double s0 = kgm::dot(n, kgm::sub(p0, T[0])), s1 = kgm::dot(n, kgm::sub(p1, T[0]));
+  if ((s0 > 0 && s1 > 0) || (s0 < 0 && s1 < 0) || s0 == s1) return false;
+  const double t = s0 / (s0 - s1);
+  x = kgm::add(p0, kgm::mul(t, kgm::sub(p1, p0)));
+  const kgm::V<double> r = kgm::sub(x, T[0]);
+  const double d00 = kgm::dot(e1, e1), d01 = kgm::dot(e1, e2), d11 = kgm::dot(e2, e2);
+  const double d20 = kgm::dot(r, e1), d21 = kgm::dot(r, e2);
+  const double den = d00 * d11 - d01 * d01;
+  if (!(den > 0)) return false;
+  const double v = (d11 * d20 - d01 * d21) / den, w = (d00 * d21 - d01 * d20) / den;
+  return v >= 0 && w >= 0 && v + w <= 1;
+}
+
+__global__ __launch_bounds__(256) void k_dist_jac(DistJacArgs a) {
+  const long item = (long)blockIdx.x * 256 + threadIdx.x;
+  if (item >= (long)a.N * a.npairs) return;
+  const int c = (int)(item / a.npairs), p = (int)(item % a.npairs);
+  float* g = a.grad_out + item * a.n;
+  for (int j = 0; j < a.n; j++) g[j] = 0.f;
+  float* cp = a.cp_out + item * 6;
+  const float d = a.dist[item];
+  const int ta = a.tri[2 * item], tb = a.tri[2 * item + 1];
+  if (!(d < a.dcheck) || ta < 0 || tb < 0) {
+    a.d_out[item] = __int_as_float(0x7f800000);
+    for (int k = 0; k < 6; k++) cp[k] = 0.f;
+    return;
+  }
+  const int la = a.pair_a[p], b = a.pair_b[p], kind = a.pair_kind[p];
+  const double* T = a.T64 + (size_t)c * a.n * 12;
+  const double* XA = T + 12 * la;
+  const double* XB = kind == 0 ? a.plan.env[b].X : T + 12 * b;
+  kgm::V<double> P[3], Q[3];
+  for (int v = 0; v < 3; v++) {
+    const double* u = a.pool.tv64 + 9 * (size_t)ta + 3 * v;
+    const double* w = a.pool.tv64 + 9 * (size_t)tb + 3 * v;
+    P[v] = kgm::mk(XA[0] * u[0] + XA[3] * u[1] + XA[6] * u[2] + XA[9], XA[1] * u[0] + XA[4] * u[1] + XA[7] * u[2] + XA[10],
+                   XA[2] * u[0] + XA[5] * u[1] + XA[8] * u[2] + XA[11]);
+    Q[v] = kgm::mk(XB[0] * w[0] + XB[3] * w[1] + XB[6] * w[2] + XB[9], XB[1] * w[0] + XB[4] * w[1] + XB[7] * w[2] + XB[10],
+                   XB[2] * w[0] + XB[5] * w[1] + XB[8] * w[2] + XB[11]);
+  }
+  kgm::V<double> pa, pb;
+  const double dd = kgm::tri_dist<double>(P, Q, pa, pb);
+  if (!(dd > 0)) {
+    // touching or intersecting: report one common point, where an edge of
+    // one triangle meets the other (the closest-point pair is not defined)
+    kgm::V<double> x;
+    bool found = false;
+    for (int e = 0; e < 3 && !found; e++) found = dj_seg_tri(P[e], P[(e + 1) % 3], Q, x);
+    for (int e = 0; e < 3 && !found; e++) found = dj_seg_tri(Q[e], Q[(e + 1) % 3], P, x);
+    if (!found) x = pa;
+    pa = pb = x;
+  }
+  a.d_out[item] = d;
+  cp[0] = (float)pa.x; cp[1] = (float)pa.y; cp[2] = (float)pa.z;
+  cp[3] = (float)pb.x; cp[4] = (float)pb.y; cp[5] = (float)pb.z;
+  if (!(dd > 0)) return;
+  const double nh[3] = {(pa.x - pb.x) / dd, (pa.y - pb.y) / dd, (pa.z - pb.z) / dd};
+  const double qa[3] = {pa.x, pa.y, pa.z}, qb[3] = {pb.x, pb.y, pb.z};
+  dj_chain(a.plan.links, T, la, qa, nh, 1.0, g);
+  if (kind == 1) dj_chain(a.plan.links, T, b, qb, nh, -1.0, g);
+}
+
+}  // namespace
+
+hipError_t kgl_dist_jac(const DistJacArgs& a, hipStream_t s) {
+  const long items = (long)a.N * a.npairs;
+  if (items <= 0) return hipSuccess;
+  k_dist_jac<<<(unsigned)((items + 255) / 256), 256, 0, s>>>(a);
+  return hipGetLastError();
+}

@@ -403,6 +403,34 @@ class World:
         call("kg_distance_batch_dev", self.h, robot, _tp(Q), N, _tp(out), _torch_stream())
         return out
 
+    def distance_jacobian_batch_dev(self, Q, dcheck, robot=0, out=None):
+        """Device-tensor form of distance_jacobian_batch: returns (d, cp, grad)
+        torch tensors on Q's device (or fills out = (d, cp, grad))."""
+        import torch
+        N, n = Q.shape
+        npairs = len(self.pair_list(robot))
+        if out is None:
+            out = (torch.empty((N, npairs), dtype=torch.float32, device=Q.device),
+                   torch.empty((N, npairs, 6), dtype=torch.float32, device=Q.device),
+                   torch.empty((N, npairs, n), dtype=torch.float32, device=Q.device))
+        d, cp, g = out
+        call("kg_distance_jacobian_batch_dev", self.h, robot, _tp(Q), N, float(dcheck), _tp(d), _tp(cp), _tp(g),
+             _torch_stream())
+        return d, cp, g
+
+    def distance_jacobian_batch(self, Q, dcheck, robot=0):
+        """Pair distances within dcheck, closest points (N, npairs, 6) and
+        joint-space gradients (N, npairs, n) (kg_distance_jacobian_batch)."""
+        Qf = _f(Q)
+        N, n = Qf.shape
+        npairs = len(self.pair_list(robot))
+        d = np.zeros((N, npairs), np.float32)
+        cp = np.zeros((N, npairs, 6), np.float32)
+        g = np.zeros((N, npairs, n), np.float32)
+        call("kg_distance_jacobian_batch", self.h, robot, _p(Qf, _pf), N, float(dcheck), _p(d, _pf), _p(cp, _pf),
+             _p(g, _pf))
+        return d, cp, g
+
     def edge_batch(self, A, B, eps=0.01, norm_inf=True, robot=0):
         n = self.robots[robot].n
         A, B = _f(A).reshape(-1, n), _f(B).reshape(-1, n)

@@ -1,0 +1,63 @@
+"""Batched pair distances with closest points and joint-space gradients
+(kg_distance_jacobian_batch, KineTrajOpt's point-collision step) against the
+FP64 oracle: distances within 1e-5 m inside the bound and +inf outside it,
+closest points that realise the distance, and gradients equal to central
+finite differences of the oracle's distance."""
+import numpy as np
+import pytest
+
+import oracle
+from klampt_amd import engine, robotsim, synthetic as S
+
+pytestmark = pytest.mark.gpu
+DCHECK = 0.3
+
+
+@pytest.fixture(scope="module")
+def scene():
+    sc = S.boxes_scene(0)
+    w, r = engine.world_from_scene(robotsim.default_context(), sc)
+    ow, orb = oracle.world_from_scene(sc)
+    return sc, w, ow, orb
+
+
+def test_distances_and_closest_points(scene):
+    sc, w, ow, orb = scene
+    Q = S.uniform_configs(sc.robot, 512, seed=81)
+    d, cp, g = w.distance_jacobian_batch(Q, DCHECK)
+    want = ow.distance_batch(orb, Q)
+    assert d.shape == want.shape
+    inside = want < DCHECK - 1e-5
+    outside = want > DCHECK + 1e-5
+    assert inside.sum() > 100 and outside.sum() > 100
+    assert np.abs(d[inside] - want[inside]).max() <= 1e-5
+    assert np.isinf(d[outside]).all()
+    sep = np.linalg.norm(cp[..., :3] - cp[..., 3:], axis=-1)
+    fin = np.isfinite(d)
+    assert np.abs(sep[fin] - d[fin]).max() <= 1e-5
+    assert (d == 0).sum() > 0 and (sep[d == 0] == 0).all()
+    assert (g[~fin] == 0).all() and (cp[~fin] == 0).all()
+
+
+def test_gradients_match_finite_differences(scene):
+    sc, w, ow, orb = scene
+    Q = S.uniform_configs(sc.robot, 48, seed=82).astype(np.float64)
+    d, cp, g = w.distance_jacobian_batch(Q.astype(np.float32), DCHECK)
+    Q32 = Q.astype(np.float32).astype(np.float64)
+    n = sc.robot.n
+    h = 1e-5
+    errs = []
+    for c in range(len(Q32)):
+        P = np.repeat(Q32[c:c + 1], 2 * n, axis=0)
+        for j in range(n):
+            P[2 * j, j] += h
+            P[2 * j + 1, j] -= h
+        D = ow.distance_batch(orb, P)
+        fd = (D[0::2] - D[1::2]) / (2 * h)          # (n, npairs)
+        for p in range(d.shape[1]):
+            if 0.01 < d[c, p] < DCHECK - 0.01:
+                errs.append(np.abs(g[c, p] - fd[:, p]).max())
+    errs = np.array(errs)
+    assert len(errs) > 50
+    assert np.median(errs) < 1e-4
+    assert np.mean(errs < 2e-3) >= 0.99, np.sort(errs)[-5:]
