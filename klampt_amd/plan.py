@@ -219,6 +219,26 @@ class RobotCSpace(CSpace):
             ok &= CSpace.visible_batch(self, A, B)
         return ok
 
+    def collision_pairs(self, Q, dcheck):
+        """The point-collision terms of a trajectory optimiser over
+        waypoints Q (``KineTrajOpt.find_collision_pair``,
+        ``Python/klampt/plan/kinetrajopt/kinetrajopt.py:530-556``), in one
+        batched call: for every waypoint and every link / obstacle (or link
+        / link) pair within dcheck, the distance, the closest points and the
+        gradient of the distance in joint space (the normal times the
+        position Jacobian).  Returns a list of (waypoint, (id_a, id_b), d,
+        cp_a, cp_b, grad) in waypoint and pair order; world IDs as
+        ``pair_list``."""
+        Q = np.asarray(Q, np.float32).reshape(-1, len(self.bound))
+        w = self.world._w
+        d, cp, g = w.distance_jacobian_batch(Q, dcheck, robot=self._c.index)
+        ids = w.pair_list(self._c.index)
+        out = []
+        for c, p in zip(*np.nonzero(np.isfinite(d))):
+            out.append((int(c), tuple(int(x) for x in ids[p]), float(d[c, p]), cp[c, p, :3].astype(float),
+                        cp[c, p, 3:].astype(float), g[c, p].astype(float)))
+        return out
+
 
 class MotionPlan:
     """A roadmap planner over a CSpace (``Python/klampt/plan/cspace.py:216-440``).

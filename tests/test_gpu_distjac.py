@@ -61,3 +61,22 @@ def test_gradients_match_finite_differences(scene):
     assert len(errs) > 50
     assert np.median(errs) < 1e-4
     assert np.mean(errs < 2e-3) >= 0.99, np.sort(errs)[-5:]
+
+
+def test_cspace_collision_pairs():
+    from klampt_amd import plan
+    sc = S.boxes_scene(0)
+    W = robotsim.WorldModel()
+    for mesh, xf in sc.objects:
+        W.addRigidObject(robotsim.Geometry3D(robotsim.TriangleMesh(mesh.verts, mesh.tris)), xf[:9], xf[9:])
+    R = robotsim.RobotModel.from_spec(sc.robot)
+    W.addRobot(R)
+    space = plan.RobotCSpace(R, W)
+    Q = S.uniform_configs(sc.robot, 32, seed=83)
+    terms = space.collision_pairs(Q, DCHECK)
+    d, cp, g = W._w.distance_jacobian_batch(Q, DCHECK)
+    assert len(terms) == int(np.isfinite(d).sum()) > 0
+    ids = W._w.pair_list(0)
+    for c, pair, dist, pa, pb, grad in terms[:50]:
+        p = [tuple(x) for x in ids.tolist()].index(pair)
+        assert dist == d[c, p] and np.allclose(grad, g[c, p]) and abs(np.linalg.norm(pa - pb) - dist) < 1e-5
