@@ -1042,7 +1042,16 @@ __device__ inline uint64_t mk_leaf(int slot, int ta, int tb) {
 // REC: item start records exist (else each item's frame is formed here).
 template <typename Real, bool EXACT, int SCAP, bool FPAIR, bool REC>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SCAP == 256 && REC && !FPAIR ? 4 : 1))) void k_collide(
-    TravArgs a) {
+    TravArgs a, const ObbNode* __restrict__ nodes, const float4* __restrict__ tv0, const float4* __restrict__ tv1,
+    const float4* __restrict__ tv2, const double* __restrict__ tv64, const PairItem* __restrict__ items,
+    const ItemRec* __restrict__ recs) {
+  // the pool, items and records are read-only here: separate restrict
+  // arguments let the compiler treat their loads as unclobbered
+  PoolDev rpool = a.pool;
+  rpool.tv0 = tv0;
+  rpool.tv1 = tv1;
+  rpool.tv2 = tv2;
+  rpool.tv64 = tv64;
   // the small-pool variant also halves the leaf queue and the ring so that
   // four waves fit per SIMD
   constexpr int LQ = SCAP >= 512 ? KG_LQ : KG_LQ / 2;
@@ -1167,8 +1176,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SCAP == 256 
 #pragma unroll
           for (int k = 0; k < 12; k++) rel[k] = s_rel[k][slot];
           V3T<Real> tp[3], tq[3];
-          load_tri<Real>(a.pool, ta, tp);
-          load_tri<Real>(a.pool, tb, tq);
+          load_tri<Real>(rpool, ta, tp);
+          load_tri<Real>(rpool, tb, tq);
 #pragma unroll
           for (int v = 0; v < 3; v++) tp[v] = xf_apply<Real>(rel, tp[v]);
           bits = tri_tri_split<Real, EXACT>(tp, tq, (Real)s_tau[slot], sub);
@@ -1319,10 +1328,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SCAP == 256 
     if (myitem >= 0) {
       n_items++;
       atomicAdd(&e_open[myent], 1);
-      const PairItem P = a.items[myitem];
+      const PairItem P = items[myitem];
       int rootA, rootB;
       if (!EXACT && REC) {
-        const float4* q = reinterpret_cast<const float4*>(a.recs + myitem);
+        const float4* q = reinterpret_cast<const float4*>(recs + myitem);
         const float4 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
         s_rel[0][myslot] = q0.x; s_rel[1][myslot] = q0.y; s_rel[2][myslot] = q0.z; s_rel[3][myslot] = q0.w;
         s_rel[4][myslot] = q1.x; s_rel[5][myslot] = q1.y; s_rel[6][myslot] = q1.z; s_rel[7][myslot] = q1.w;
@@ -1387,9 +1396,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SCAP == 256 
       const int rk = s_rank[slot];
       const int g = (rk >> KG_RANK_GROUP_BIT) & 1, r = rk & ((1 << KG_RANK_GROUP_BIT) - 1);
       if (!item_dead(mode, s_best[s_cfgk[slot]], g, r)) {
-        const NodeF O = load_nodef(a.pool.nodes, y);
-        const NodeF S0 = load_nodef(a.pool.nodes, x);
-        const NodeF S1 = load_nodef(a.pool.nodes, single ? x : x + 1);
+        const NodeF O = load_nodef(nodes, y);
+        const NodeF S0 = load_nodef(nodes, x);
+        const NodeF S1 = load_nodef(nodes, single ? x : x + 1);
         // S (the split node) is in mesh A when A was split (or for a
         // single pair); otherwise the item transform is inverted
         const bool aIsS = single || side == 0;
@@ -1545,9 +1554,11 @@ hipError_t kgl_prep_single(const PrepArgs& a, hipStream_t s) {
 template <typename Real, bool EXACT, int SCAP, bool FPAIR>
 static void launch_rec(const TravArgs& a, hipStream_t s) {
   if (!EXACT && a.recs)
-    k_collide<Real, EXACT, SCAP, FPAIR, !EXACT><<<a.grid, 64, 0, s>>>(a);
+    k_collide<Real, EXACT, SCAP, FPAIR, !EXACT><<<a.grid, 64, 0, s>>>(a, a.pool.nodes, a.pool.tv0, a.pool.tv1,
+                                                                    a.pool.tv2, a.pool.tv64, a.items, a.recs);
   else
-    k_collide<Real, EXACT, SCAP, FPAIR, false><<<a.grid, 64, 0, s>>>(a);
+    k_collide<Real, EXACT, SCAP, FPAIR, false><<<a.grid, 64, 0, s>>>(a, a.pool.nodes, a.pool.tv0, a.pool.tv1,
+                                                                   a.pool.tv2, a.pool.tv64, a.items, a.recs);
 }
 
 template <typename Real, bool EXACT, int SCAP>
