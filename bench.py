@@ -434,9 +434,9 @@ def run_secondary(args, ctx, sc, world, robot):
     out["c6_mesh_to_sdf"] = {"mesh_tris": int(soup.tris.shape[0]), "resolution": hs, "dims": list(dims),
                              "cells": ncell, "ms": round(tconv * 1e3, 2),
                              "cells_per_s": round(ncell / tconv, 1)}
-    gd = engine.Geom.trimesh(ctx, soup.verts, soup.tris)
+    gm.to_sdf(0.0).close()
     t0 = time.perf_counter()
-    gdd = gd.to_sdf(0.0)
+    gdd = gm.to_sdf(0.0)
     tdef = time.perf_counter() - t0
     out["c6_mesh_to_sdf"]["default_resolution"] = {"dims": list(gdd.sdf_grid()[0]), "ms": round(tdef * 1e3, 2)}
     gdd.close()
