@@ -295,6 +295,35 @@ class RobotModelLink:
     def getName(self) -> str:
         return self._robot._names[self.index]
 
+    def getWorldPosition(self, plocal):
+        """World coordinates of a point given in this link's frame."""
+        R, t = self._robot._r.link_transform(self.index)
+        R = np.asarray(R, np.float64).reshape(3, 3).T
+        return list(R @ np.asarray(plocal, np.float64) + np.asarray(t, np.float64))
+
+    def getLocalPosition(self, pworld):
+        """This link's coordinates of a world point."""
+        R, t = self._robot._r.link_transform(self.index)
+        R = np.asarray(R, np.float64).reshape(3, 3).T
+        return list(R.T @ (np.asarray(pworld, np.float64) - np.asarray(t, np.float64)))
+
+    def getPositionJacobian(self, plocal):
+        """3 x n Jacobian of the world position of a point fixed in this link
+        (``RobotModelLink.getPositionJacobian``, robotsim.cpp:4837-4844): column
+        j is w_j x (p - o_j) for a revolute ancestor j (w_j its world axis,
+        o_j its frame origin), w_j for a prismatic one, 0 otherwise."""
+        rob = self._robot
+        p = np.asarray(self.getWorldPosition(plocal))
+        J = np.zeros((3, rob.numLinks()))
+        j = self.index
+        while j >= 0:
+            R, t = rob._r.link_transform(j)
+            R = np.asarray(R, np.float64).reshape(3, 3).T
+            w = R @ rob._axes[j]
+            J[:, j] = w if rob._jtype[j] == 1 else np.cross(w, p - np.asarray(t, np.float64))
+            j = int(rob._spec_parents[j])
+        return J.tolist()
+
     def setName(self, name: str):
         self._robot._names[self.index] = name
 
@@ -310,6 +339,8 @@ class RobotModel:
         self._geoms: List[Optional[Geometry3D]] = [Geometry3D(ctx=self._ctx) for _ in range(len(parents))]
         self._names = ["Link_%d" % i for i in range(len(parents))]
         self._name = "Robot"
+        self._axes = np.asarray(axes, np.float64).reshape(-1, 3)
+        self._jtype = np.asarray(jtype, np.int32)
 
     @classmethod
     def from_spec(cls, spec, ctx=None):

@@ -80,3 +80,25 @@ def test_cspace_collision_pairs():
     for c, pair, dist, pa, pb, grad in terms[:50]:
         p = [tuple(x) for x in ids.tolist()].index(pair)
         assert dist == d[c, p] and np.allclose(grad, g[c, p]) and abs(np.linalg.norm(pa - pb) - dist) < 1e-5
+
+
+def test_link_position_jacobian_matches_finite_differences():
+    spec = S.hetero_robots(seed=4, nrobots=1, dof_range=(14, 14), tri_range=(100, 200))[0][0]
+    spec.jtype[3] = 1      # one prismatic joint in the chain
+    R = robotsim.RobotModel.from_spec(spec)
+    q = S.uniform_configs(spec, 1, seed=5)[0].astype(np.float64)
+    R.setConfig(list(q))
+    link = R.link(spec.n - 1)
+    pl = [0.05, -0.02, 0.1]
+    J = np.asarray(link.getPositionJacobian(pl))
+    assert np.allclose(link.getLocalPosition(link.getWorldPosition(pl)), pl, atol=1e-12)
+    h = 1e-6
+    for j in range(spec.n):
+        qp, qm = q.copy(), q.copy()
+        qp[j] += h
+        qm[j] -= h
+        R.setConfig(list(qp))
+        a = np.asarray(link.getWorldPosition(pl))
+        R.setConfig(list(qm))
+        b = np.asarray(link.getWorldPosition(pl))
+        assert np.allclose(J[:, j], (a - b) / (2 * h), atol=1e-6), j
