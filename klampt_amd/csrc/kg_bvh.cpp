@@ -16,9 +16,18 @@
 // rounded to FP32 first, then the extents are measured in the dual basis of
 // the rounded axes (so slightly non-orthonormal FP32 axes still bound every
 // vertex) and rounded up.
+//
+// Large meshes are built in parallel: the top levels of the tree are built
+// first and their open subtrees (disjoint ranges of the triangle index
+// array) are built on host threads into separate arrays, then spliced in
+// left-to-right order.  Every node is fitted and split exactly as in the
+// serial build, so the tree and the leaf order are the same; only the node
+// numbering differs (top levels first, then each subtree depth first).
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "kg_types.h"
@@ -62,8 +71,8 @@ void jacobi3(double A[3][3], double V[3][3], double w[3]) {
 struct Builder {
   const double* V;
   const int32_t* T;
-  std::vector<double> cen;  // centroids (3 per triangle)
-  std::vector<int> idx;
+  const double* cen;        // centroids (3 per triangle), shared
+  int* idx;                 // triangle order, shared: a builder reorders only its own ranges
   std::vector<ObbNode> nodes;
   std::vector<int> order;   // leaf order -> original triangle
   int depth = 0;
@@ -163,7 +172,7 @@ struct Builder {
     if (bestk < 0) {
       // every centroid in one bin: median along the first axis
       const int mid = b + n / 2;
-      std::nth_element(idx.begin() + b, idx.begin() + mid, idx.begin() + e, [&](int a, int c) {
+      std::nth_element(idx + b, idx + mid, idx + e, [&](int a, int c) {
         const double pa = proj(0, a), pc = proj(0, c);
         return pa < pc || (pa == pc && a < c);
       });
@@ -177,7 +186,7 @@ struct Builder {
     }
     const double scale = NB / (cmax - cmin);
     auto left = [&](int t) { return std::min(NB - 1, (int)((proj(bestk, t) - cmin) * scale)) < bestb; };
-    return (int)(std::stable_partition(idx.begin() + b, idx.begin() + e, left) - idx.begin());
+    return (int)(std::stable_partition(idx + b, idx + e, left) - idx);
   }
 
   void fit(ObbNode& nd, int b, int e) {
@@ -315,23 +324,85 @@ struct Builder {
   }
 };
 
+// An open subtree of the top levels: triangles [b,e) at depth d, its root
+// going into node slot `at`.
+struct Task {
+  int at, b, e, d;
+};
+
+// Top levels: fit and split down to ranges of at most `cut` triangles, which
+// become tasks (the serial build's recursion, stopped early).
+void build_top(Builder& B, int at, int b, int e, int d, int cut, std::vector<Task>& tasks) {
+  if (e - b <= cut) {
+    tasks.push_back({at, b, e, d});
+    return;
+  }
+  B.depth = std::max(B.depth, d);
+  B.fit(B.nodes[at], b, e);
+  const int mid = B.split(b, e, B.nodes[at].R);
+  const int l = (int)B.nodes.size();
+  B.nodes.push_back(ObbNode{});
+  B.nodes.push_back(ObbNode{});
+  B.nodes[at].child = l;
+  build_top(B, l, b, mid, d + 1, cut, tasks);
+  build_top(B, l + 1, mid, e, d + 1, cut, tasks);
+}
+
 }  // namespace
 
 void kg_build_obb_tree(const double* V, int nv, const int32_t* T, int nt, BuiltTree& out) {
   (void)nv;
+  std::vector<double> cen(3 * (size_t)nt);
+  std::vector<int> idx(nt);
+  for (int i = 0; i < nt; i++) {
+    idx[i] = i;
+    for (int k = 0; k < 3; k++)
+      cen[3 * i + k] = (V[3 * T[3 * i] + k] + V[3 * T[3 * i + 1] + k] + V[3 * T[3 * i + 2] + k]) / 3.0;
+  }
   Builder B;
   B.V = V;
   B.T = T;
-  B.cen.resize(3 * (size_t)nt);
-  B.idx.resize(nt);
-  for (int i = 0; i < nt; i++) {
-    B.idx[i] = i;
-    for (int k = 0; k < 3; k++)
-      B.cen[3 * i + k] = (V[3 * T[3 * i] + k] + V[3 * T[3 * i + 1] + k] + V[3 * T[3 * i + 2] + k]) / 3.0;
-  }
+  B.cen = cen.data();
+  B.idx = idx.data();
   B.nodes.reserve(2 * (size_t)nt);
   B.order.reserve(nt);
-  if (nt > 0) B.build(0, nt, 0);
+  const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  if (nt < 8192 || hw == 1) {
+    if (nt > 0) B.build(0, nt, 0);
+  } else {
+    B.nodes.push_back(ObbNode{});
+    std::vector<Task> tasks;
+    build_top(B, 0, 0, nt, 0, std::max(2048, nt / (int)(4 * hw)), tasks);
+    std::vector<Builder> sub(tasks.size());
+    std::vector<std::thread> th;
+    std::atomic_int next(0);
+    for (unsigned w = 0; w < hw; w++)
+      th.emplace_back([&]() {
+        for (int k = next.fetch_add(1); k < (int)tasks.size(); k = next.fetch_add(1)) {
+          Builder& S = sub[k];
+          S.V = V;
+          S.T = T;
+          S.cen = cen.data();
+          S.idx = idx.data();
+          S.nodes.reserve(2 * (size_t)(tasks[k].e - tasks[k].b));
+          S.build(tasks[k].b, tasks[k].e, tasks[k].d);
+        }
+      });
+    for (auto& t : th) t.join();
+    // splice: subtree node 0 goes to its slot, nodes 1.. are appended
+    for (size_t k = 0; k < tasks.size(); k++) {
+      const Builder& S = sub[k];
+      const int base = (int)B.nodes.size() - 1, obase = (int)B.order.size();
+      auto remap = [&](ObbNode nd) {
+        nd.child = nd.child >= 0 ? nd.child + base : ~(~nd.child + obase);
+        return nd;
+      };
+      B.nodes[tasks[k].at] = remap(S.nodes[0]);
+      for (size_t i = 1; i < S.nodes.size(); i++) B.nodes.push_back(remap(S.nodes[i]));
+      B.order.insert(B.order.end(), S.order.begin(), S.order.end());
+      B.depth = std::max(B.depth, S.depth);
+    }
+  }
   out.nodes.swap(B.nodes);
   out.order.swap(B.order);
   out.depth = B.depth;
