@@ -881,6 +881,7 @@ int kg_geom_destroy(kg_geom g) {
   return guard([&] {
     if (!g) return;
     g->dvals.release();
+    g->dpts.release();
     g->alive = false;
     // mesh data stays in the context pool until the context is destroyed
     delete g;
