@@ -51,6 +51,7 @@ import os
 import re
 import struct
 import xml.etree.ElementTree as ET
+from collections import OrderedDict
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional, Tuple
 
@@ -244,7 +245,8 @@ def _read_stl(path):
     return _weld(np.array(P, np.float64).reshape(-1, 3))
 
 
-_MESH_CACHE: Dict[tuple, S.Mesh] = {}
+_MESH_CACHE: "OrderedDict[tuple, S.Mesh]" = OrderedDict()
+_MESH_CACHE_MAX = 64   # parsed files kept (least recently used dropped)
 
 
 def load_mesh(path) -> S.Mesh:
@@ -255,7 +257,9 @@ def load_mesh(path) -> S.Mesh:
         raise LoadError("%s: no such file" % path)
     key = _file_key(path)
     hit = _MESH_CACHE.get(key)
-    if hit is None:
+    if hit is not None:
+        _MESH_CACHE.move_to_end(key)
+    else:
         ext = _ext(path)
         if ext == "off":
             hit = _read_off(path)
@@ -266,6 +270,8 @@ def load_mesh(path) -> S.Mesh:
         else:
             raise LoadError("%s: unsupported mesh format .%s (supported: .off, .obj, .stl)" % (path, ext))
         _MESH_CACHE[key] = hit
+        while len(_MESH_CACHE) > _MESH_CACHE_MAX:
+            _MESH_CACHE.popitem(last=False)
     return S.Mesh(hit.verts.copy(), hit.tris.copy(), key=key)
 
 

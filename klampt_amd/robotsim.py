@@ -160,7 +160,7 @@ class Geometry3D:
         be read (``Geometry3D::loadFile``, robotsim.cpp:1433-1460)."""
         try:
             mesh = _io.load_mesh(fn)
-        except (_io.LoadError, OSError):
+        except (_io.LoadError, OSError, KlamptGPUError):
             return False
         self.set(mesh)
         return True
@@ -170,7 +170,7 @@ class Geometry3D:
             return False
         try:
             _io.save_mesh(fn, _io.S.Mesh(self._mesh.vertices, self._mesh.indices))
-        except (_io.LoadError, OSError):
+        except (_io.LoadError, OSError, KlamptGPUError):
             return False
         return True
 
@@ -522,19 +522,19 @@ class WorldModel:
         reference returns an unbound model)."""
         try:
             return self._add_robot_file(_io.load_robot(fn))
-        except (_io.LoadError, OSError):
+        except (_io.LoadError, OSError, KlamptGPUError):
             return None
 
     def loadRigidObject(self, fn: str) -> Optional[RigidObjectModel]:
         try:
             return self._add_object_file(_io.load_rigid_object(fn))
-        except (_io.LoadError, OSError):
+        except (_io.LoadError, OSError, KlamptGPUError):
             return None
 
     def loadTerrain(self, fn: str) -> Optional[TerrainModel]:
         try:
             return self._add_terrain_file(_io.load_terrain(fn))
-        except (_io.LoadError, OSError):
+        except (_io.LoadError, OSError, KlamptGPUError):
             return None
 
     def loadElement(self, fn: str) -> int:
@@ -574,7 +574,7 @@ class WorldModel:
                     self._add_robot_file(rf)
             else:
                 return False
-        except (_io.LoadError, OSError):
+        except (_io.LoadError, OSError, KlamptGPUError):
             return False
         return True
 
