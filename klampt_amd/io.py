@@ -171,8 +171,13 @@ def _read_off(path):
         raise LoadError("%s: malformed OFF file (%s)" % (path, e or "truncated")) from None
     # all-triangle face blocks (the common case) parse in one pass
     tok = " ".join(flines).split()
+    F = None
     if len(tok) == 4 * nf:
-        F = np.array(tok, np.int64).reshape(-1, 4) if nf else np.zeros((0, 4), np.int64)
+        try:
+            F = np.array(tok, np.int64).reshape(-1, 4) if nf else np.zeros((0, 4), np.int64)
+        except ValueError:
+            F = None   # not all integers: the per-face parse below reports it
+    if F is not None:
         if (F[:, 0] == 3).all():
             T = F[:, 1:]
             if T.size and (T.min() < 0 or T.max() >= nv):
