@@ -84,3 +84,19 @@ def test_robotsim_convert_and_query():
         s.convert("TriangleMesh")
     with pytest.raises(RuntimeError, match="nonnegative"):
         a.convert("ImplicitSurface", -1)
+
+
+def test_edge_cases_empty_flat_and_open_meshes():
+    from klampt_amd._lib import KlamptGPUError
+    ctx = robotsim.default_context()
+    empty = engine.Geom.trimesh(ctx, np.zeros((0, 3)), np.zeros((0, 3), np.int32))
+    with pytest.raises(KlamptGPUError, match="empty"):
+        empty.to_sdf(0.1)
+    # one triangle in the z = 0 plane: an open, flat mesh (dims along z = 2 pad)
+    tri = S.Mesh(np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float64), np.array([[0, 1, 2]], np.int32))
+    dims, lo, hi, v = _convert(tri, 0.1)
+    assert dims == (14, 14, 4)
+    o, w = oracle.mesh_to_sdf(tri, lo, 0.1, dims, with_winding=True)
+    assert np.abs(v - o).max() <= TOL and np.array_equal(np.signbit(v), w != 0)
+    with pytest.raises(KlamptGPUError):
+        engine.Geom.trimesh(ctx, tri.verts, tri.tris).to_sdf(-1.0)
