@@ -1913,7 +1913,7 @@ void distance_dev(kg_world_s* w, int ri, const float* Qf, const double* Qd, int 
   da.spill_cap = spill_cap;
   da.status = status + ST_SPILL;
   da.grid = grid;
-  da.ipw = std::max(1, env_int("KG_DIST_IPW", 64));
+  da.ipw = std::max(1, env_int("KG_DIST_IPW", 4));
   da.counters = (unsigned long long*)(status + ST_CNT);
   const bool phase_ev = c->timing >= 2;
   c->last_phase_ev = phase_ev;
