@@ -17,6 +17,8 @@
 // FP64 triangles.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
+#include <cstdlib>
+#include <type_traits>
 
 #include "kg_internal.h"
 
@@ -1858,17 +1860,30 @@ hipError_t kgl_sdf_penalty(const LinkDesc* links, int n, const SdfDev& g, const 
                            const int* poff, float dsafe, float* loss, const float* dloss, float* dQ, bool bwd,
                            hipStream_t s) {
   if (N <= 0) return hipSuccess;
-  // 16 lanes per configuration; groups per block limited by 48 KB of LDS
-  constexpr int GL = 16;
+  // GL lanes per configuration (KG_SDF_GL: 8, 16, 32 or 64; default 32,
+  // 5% faster than 16 on the 7-link arm, 448 points per waypoint); groups
+  // per block limited by 48 KB of LDS
+  const char* env = getenv("KG_SDF_GL");
+  const int want = env ? atoi(env) : 32;
+  const int GLr = (want == 8 || want == 16 || want == 64) ? want : 32;
   const size_t per_group = (size_t)n * 18 * sizeof(float);
-  int gpb = 256 / GL;
+  int gpb = 256 / GLr;
   while (gpb > 1 && gpb * per_group > 48 * 1024) gpb /= 2;
   const size_t lds = gpb * per_group;
   const int blocks = (N + gpb - 1) / gpb;
-  if (bwd)
-    k_sdf_pen_group<true, GL><<<blocks, gpb * GL, lds, s>>>(links, n, g, Q, N, pts, poff, dsafe, loss, dloss, dQ, gpb);
-  else
-    k_sdf_pen_group<false, GL><<<blocks, gpb * GL, lds, s>>>(links, n, g, Q, N, pts, poff, dsafe, loss, dloss, dQ, gpb);
+  auto go = [&](auto glc) {
+    constexpr int GL = decltype(glc)::value;
+    if (bwd)
+      k_sdf_pen_group<true, GL><<<blocks, gpb * GL, lds, s>>>(links, n, g, Q, N, pts, poff, dsafe, loss, dloss, dQ, gpb);
+    else
+      k_sdf_pen_group<false, GL><<<blocks, gpb * GL, lds, s>>>(links, n, g, Q, N, pts, poff, dsafe, loss, dloss, dQ, gpb);
+  };
+  switch (GLr) {
+    case 8: go(std::integral_constant<int, 8>{}); break;
+    case 64: go(std::integral_constant<int, 64>{}); break;
+    case 16: go(std::integral_constant<int, 16>{}); break;
+    default: go(std::integral_constant<int, 32>{}); break;
+  }
   return hipGetLastError();
 }
 
