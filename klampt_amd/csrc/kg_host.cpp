@@ -2265,8 +2265,11 @@ extern "C" int kg_edge_batch(kg_world w, int ri, const float* A, const float* B,
     std::vector<uint8_t> done(nseg, 0);
     up(c->wsEU, fu.data(), nseg * sizeof(float));
     up(c->wsEDone, done.data(), nseg);
-    // rounds over bisection positions [pos0, pos1): 1, 1, 2, 4, ... wide
-    int pos0 = 0, width = 1;
+    // rounds over bisection positions [pos0, pos0 + width): w0, w0, 2 w0,
+    // 4 w0, ... wide (KG_EDGE_W0, default 16).  The first infeasible sample in
+    // bisection order is the same for any widths; wider rounds mean fewer
+    // round trips and more samples checked past an early hit.
+    int pos0 = 0, width = std::max(1, env_int("KG_EDGE_W0", 16));
     std::vector<int> live, soff;
     for (;;) {
       live.clear();
