@@ -149,12 +149,27 @@ def _read_off(path):
     """Object File Format: optional ``OFF`` header (``COFF`` / ``NOFF`` /
     ``CNOFF`` too: only the first 3 values of a vertex line are used), counts,
     vertex lines, then faces ``n i0 .. i(n-1)`` triangulated as fans."""
-    lines = []
     with open(path, "r", errors="replace") as f:
-        for ln in f:
-            ln = ln.split("#", 1)[0].strip()
-            if ln:
-                lines.append(ln)
+        text = f.read()
+    if "#" not in text:
+        # plain "OFF" with triangles only (the common case): one token pass
+        m = re.match(r"\s*OFF\s+(\d+)\s+(\d+)\s+(\d+)\s", text)
+        if m:
+            nv, nf = int(m.group(1)), int(m.group(2))
+            body = np.fromstring(text[m.end():], dtype=np.float64, sep=" ")
+            if body.size == 3 * nv + 4 * nf:
+                V = body[:3 * nv].reshape(-1, 3)
+                F = body[3 * nv:].reshape(-1, 4)
+                if (F[:, 0] == 3).all() and np.array_equal(F, np.floor(F)):
+                    T = F[:, 1:].astype(np.int64)
+                    if T.size and (T.min() < 0 or T.max() >= nv):
+                        raise LoadError("%s: vertex index out of range (%d vertices)" % (path, nv))
+                    return S.Mesh(V.copy(), T.astype(np.int32))
+    lines = []
+    for ln in text.splitlines():
+        ln = ln.split("#", 1)[0].strip()
+        if ln:
+            lines.append(ln)
     if not lines:
         raise LoadError("%s: empty OFF file" % path)
     it = iter(lines)
@@ -165,8 +180,13 @@ def _read_off(path):
         hdr = first
     try:
         nv, nf = int(hdr[0]), int(hdr[1])
-        V = np.array([[float(x) for x in next(it).split()[:3]] for _ in range(nv)], np.float64).reshape(-1, 3)
+        vlines = [next(it) for _ in range(nv)]
         flines = [next(it) for _ in range(nf)]
+        vt = np.fromstring(" ".join(vlines), dtype=np.float64, sep=" ") if nv else np.zeros(0)
+        if vt.size == 3 * nv:   # plain "x y z" lines (the common case): one pass
+            V = vt.reshape(-1, 3)
+        else:
+            V = np.array([[float(x) for x in ln.split()[:3]] for ln in vlines], np.float64).reshape(-1, 3)
     except (StopIteration, ValueError, IndexError) as e:
         raise LoadError("%s: malformed OFF file (%s)" % (path, e or "truncated")) from None
     # all-triangle face blocks (the common case) parse in one pass
