@@ -243,6 +243,12 @@ def run_secondary(args, ctx, sc, world, robot):
     out["c8_distance_jacobian"] = {"configs": args.dist_configs, "pairs_per_config": npairs, "dcheck": 0.1,
                                    "pairs_within": within, "ms": round(t * 1e3, 3),
                                    "pair_queries_per_s": round(args.dist_configs * npairs / t, 1)}
+    if not args.no_cpu:
+        # the FP64 oracle's pair distances on the same configurations (no
+        # bound, no closest points or Jacobians): a lower bound on the CPU cost
+        out["c8_distance_jacobian"]["cpu_pair_queries_per_s"] = out["c2_distance"]["cpu_pair_distances_per_s"]
+        out["c8_distance_jacobian"]["cpu_sample"] = ("the C2 line's oracle distances (distances only; a lower bound "
+                                                     "on the CPU cost of this query)")
     print(f"[bench] C8 distance Jacobians: {out['c8_distance_jacobian']}", file=sys.stderr, flush=True)
 
     # C4 edges: 8192 segments, L-infinity step 0.01 rad, 10..500 samples
