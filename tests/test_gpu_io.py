@@ -176,3 +176,25 @@ def test_urdf_robot_feasibility_matches_oracle(tmp_path):
     got = _feas(robotsim.RobotCSpace(rob, W).feasible_batch(Q))
     want = _agree(got, ow, orb, Q)
     assert 0 < want.sum() < len(Q)
+
+
+def test_rigid_object_set_transform_moves_it(tmp_path):
+    box = S.box_mesh([0.2, 0.2, 0.2])
+    io.save_mesh(str(tmp_path / "b.off"), box)
+    W = robotsim.WorldModel()
+    obj = W.loadRigidObject(str(tmp_path / "b.off"))          # a mesh file: identity transform
+    assert obj is not None and obj.getName() == "b" and obj.getID() == 0
+    rob = robotsim.RobotModel.from_spec(S.arm7())
+    W.addRobot(rob)
+    space = robotsim.RobotCSpace(rob, W)
+    Q = S.uniform_configs(S.arm7(), 512, seed=75)
+    ow, orb = oracle.world_from_scene(S.Scene(S.arm7(), [], [(box, io.xf12(np.eye(3), np.zeros(3)))]))
+    feas0 = _agree(_feas(space.feasible_batch(Q)), ow, orb, Q)
+    assert not feas0.all()                                     # the box at the base blocks some configurations
+    obj.setTransform([1, 0, 0, 0, 1, 0, 0, 0, 1], [10.0, 10.0, 10.0])
+    assert obj.getTransform()[1] == [10.0, 10.0, 10.0]
+    W2 = robotsim.WorldModel()
+    rob2 = robotsim.RobotModel.from_spec(S.arm7())
+    W2.addRobot(rob2)
+    free = _feas(robotsim.RobotCSpace(rob2, W2).feasible_batch(Q))
+    assert np.array_equal(_feas(space.feasible_batch(Q)), free) and free.any()
