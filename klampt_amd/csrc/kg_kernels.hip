@@ -1075,6 +1075,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SCAP == 256 
   const Real pad_rel = (Real)a.tol.pad_rel;
   uint64_t* gsp_base = a.spill + (size_t)blockIdx.x * a.spill_cap;
   const int ncfg = a.ncfg_dev ? *a.ncfg_dev : a.ncfg;
+  // nothing to do (typically the FP64 pass when no configuration was left
+  // undecided): leave before the set-up, the whole block at once
+  if (ncfg <= 0) return;
   unsigned n_box = 0, n_tri = 0, n_items = 0;
 
   for (int i = lane; i < NE; i += 64) {
