@@ -339,6 +339,34 @@ def save_mesh(path, mesh: S.Mesh):
         raise LoadError("%s: unsupported mesh format .%s" % (path, ext))
 
 
+def mesh_to_points(mesh: S.Mesh, dispersion: float = 0.0) -> np.ndarray:
+    """TriangleMesh -> PointCloud points (``Geometry3D.convert("PointCloud",
+    dispersion)``, Python/klampt/src/geometry.h:959-961): every vertex, plus,
+    on each triangle whose longest edge exceeds the dispersion, the points
+    a + (i/k)(b-a) + (j/k)(c-a), i + j <= k, with k = ceil(longest edge /
+    dispersion), so every surface point lies within the dispersion of a
+    returned point.  dispersion 0: the average longest triangle edge."""
+    V = np.asarray(mesh.verts, np.float64)
+    T = np.asarray(mesh.tris, np.int64)
+    if T.shape[0] == 0:
+        return V.copy()
+    P = V[T]
+    L = np.max(np.linalg.norm(P - np.roll(P, -1, axis=1), axis=-1), axis=1)
+    d = float(dispersion) if dispersion > 0 else float(L.mean())
+    if not d > 0:
+        return V.copy()
+    out = [V]
+    K = np.ceil(L / d).astype(np.int64)
+    for k in np.unique(K[K > 1]):
+        sel = np.nonzero(K == k)[0]
+        ij = np.array([(i, j) for i in range(k + 1) for j in range(k + 1 - i)
+                       if (i, j) not in ((0, 0), (k, 0), (0, k))], np.float64) / k
+        A, B, C = P[sel, 0], P[sel, 1], P[sel, 2]
+        pts = A[:, None] + ij[None, :, :1] * (B - A)[:, None] + ij[None, :, 1:] * (C - A)[:, None]
+        out.append(pts.reshape(-1, 3))
+    return np.concatenate(out)
+
+
 # ---------------------------------------------------------------- primitives
 
 def box_primitive(size) -> S.Mesh:

@@ -176,10 +176,11 @@ class Geometry3D:
 
     def convert(self, type: str, param: float = 0.0) -> "Geometry3D":
         """``Geometry3D.convert`` (robotsim.cpp:1602-1646) for TriangleMesh ->
-        ImplicitSurface / VolumeGrid: param is the grid resolution (0: the
-        average triangle diameter); the grid has 2 cells of padding around
-        the mesh.  Other conversions raise as the reference does for an
-        unavailable one."""
+        ImplicitSurface / VolumeGrid (GPU; param is the grid resolution, 0:
+        the average triangle diameter; 2 cells of padding around the mesh)
+        and TriangleMesh -> PointCloud (param is the dispersion,
+        ``klampt_amd.io.mesh_to_points``).  Other conversions raise as the
+        reference does for an unavailable one."""
         if self._g is None:
             raise RuntimeError("Geometry3D is empty, cannot convert")
         if type not in ("TriangleMesh", "PointCloud", "VolumeGrid", "ImplicitSurface", "OccupancyGrid",
@@ -188,6 +189,12 @@ class Geometry3D:
                                "VolumeGrid, ImplicitSurface, OccupancyGrid, or Heightmap")
         if param < 0:
             raise RuntimeError("Invalid conversion parameter, must be nonnegative")
+        if self._type == "TriangleMesh" and type == "PointCloud":
+            out = Geometry3D(PointCloud(_io.mesh_to_points(_io.S.Mesh(self._mesh.vertices, self._mesh.indices), param)),
+                             ctx=self._ctx)
+            R, t = self.getCurrentTransform()
+            out.setCurrentTransform(R, t)
+            return out
         if self._type == "TriangleMesh" and type in ("ImplicitSurface", "VolumeGrid"):
             try:
                 g = self._g.to_sdf(param, 2)

@@ -115,3 +115,19 @@ def test_unsupported_pairs_and_empty_geometry():
     assert pc.getPointCloud().numPoints() == 4 and pc.type() == "PointCloud"
     lo, hi = pc.getBB()
     assert lo == [0, 0, 0] and hi == [0, 0, 0]
+
+
+def test_mesh_converted_to_pointcloud_against_sdf():
+    from klampt_amd import io
+    box = S.box_mesh([0.1, 0.1, 0.1])
+    tm = robotsim.Geometry3D(robotsim.TriangleMesh(box.verts, box.tris))
+    R, t = S.random_rotation(np.random.default_rng(9)), np.array([0.25, 0.0, 0.0])
+    _set(tm, R, t)
+    pc = tm.convert("PointCloud", 0.02)
+    pts = pc.getPointCloud().getPoints()
+    assert pc.type() == "PointCloud" and np.array_equal(pts, io.mesh_to_points(box, 0.02))
+    assert pc.getCurrentTransform()[1] == list(t)
+    grid = _sphere_grid()
+    sdf = robotsim.Geometry3D(grid)
+    m = float(_oracle_min(grid, pts, R, t, np.eye(3), np.zeros(3)).min())
+    assert abs(pc.distance(sdf).d - m) <= TOL

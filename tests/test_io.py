@@ -338,3 +338,21 @@ def test_world_xml_errors(tmp_path):
         io.load_world(_write(tmp_path, "w2.xml", '<world><rigidObject name="x"/></world>'))
     with pytest.raises(io.LoadError, match="not <world>"):
         io.load_world(_write(tmp_path, "w3.xml", "<robot/>"))
+
+
+def test_mesh_to_points_dispersion():
+    from scipy.spatial import cKDTree
+    m = S.merge_meshes([S.box_mesh([0.3, 0.1, 0.2]), S.ellipsoid_mesh([0.1, 0.2, 0.1], c=[1, 0, 0], edge=0.05)])
+    for disp in (0.05, 0.02):
+        P = io.mesh_to_points(m, disp)
+        assert np.array_equal(P[:m.verts.shape[0]], m.verts)        # every vertex, first
+        # random surface points are all within the dispersion of a sample
+        rng = np.random.default_rng(1)
+        t = rng.integers(0, m.tris.shape[0], 4000)
+        u = rng.random((4000, 2))
+        u = np.where(u.sum(1, keepdims=True) > 1, 1 - u, u)
+        A, B, C = (m.verts[m.tris[t, k]] for k in range(3))
+        Q = A + u[:, :1] * (B - A) + u[:, 1:] * (C - A)
+        d, _ = cKDTree(P).query(Q)
+        assert d.max() <= disp
+    assert io.mesh_to_points(m).shape[0] > m.verts.shape[0]   # default: the average longest edge
