@@ -578,7 +578,7 @@ void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* 
   ta.toff = L.toff;
   ta.ibase = L.ibase;
   ta.scap = scap;
-  ta.static_pct = std::max(0, std::min(100, env_int("KG_STATIC_PCT", 0)));
+  ta.static_pct = std::max(0, std::min(100, env_int("KG_STATIC_PCT", 25)));
   ta.recs = pa.recs;
   ta.Qf = Qf;
   ta.Qd = Qd;
