@@ -35,6 +35,7 @@ def lib():
         sig = {
             "or_world_new": (_vp, []),
             "or_world_free": (None, [_vp]),
+            "or_set_mutation": (None, [ctypes.c_int]),
             "or_add_mesh": (ctypes.c_int, [_vp, _d, ctypes.c_int, _i, ctypes.c_int]),
             "or_add_terrain": (ctypes.c_int, [_vp, ctypes.c_int]),
             "or_add_object": (ctypes.c_int, [_vp, ctypes.c_int, _d]),
@@ -245,6 +246,26 @@ def tritri_dist(a, b):
 def tritri_depth(a, b):
     a, b = _dd(a), _dd(b)
     return lib().or_tritri_depth(_p(a, _d), _p(b, _d))
+
+
+# Planted policy mistakes (test hook, see or_set_mutation in kg_oracle.cpp).
+MUT_NONE, MUT_STABLE_ERASE, MUT_LINKS_FIRST, MUT_NO_REJECT, MUT_ENV_MAJOR = range(5)
+
+
+class mutation:
+    """Context manager: run the oracle with one planted policy mistake, so a
+    test can show that its known answers detect it.  Single-threaded use."""
+
+    def __init__(self, m):
+        self.m = int(m)
+
+    def __enter__(self):
+        lib().or_set_mutation(self.m)
+        return self
+
+    def __exit__(self, *exc):
+        lib().or_set_mutation(MUT_NONE)
+        return False
 
 
 def bisection_order(n):

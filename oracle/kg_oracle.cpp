@@ -765,6 +765,11 @@ static void env_items(const World& w, int r, std::vector<Placed>& out) {
   }
 }
 
+// Planted policy mistakes, for tests that show the first-pair known-answer
+// cases detect them (0 = Klampt's policy, the default; see or_set_mutation).
+enum { MUT_NONE = 0, MUT_STABLE_ERASE = 1, MUT_LINKS_FIRST = 2, MUT_NO_REJECT = 3, MUT_ENV_MAJOR = 4 };
+static int g_mutation = MUT_NONE;
+
 // Set-level quick reject with swap-erase (PlannerSettings.cpp:290-317).
 static void quick_reject(std::vector<Placed>& keep, const std::vector<Placed>& against) {
   double lo[3] = {kInf, kInf, kInf}, hi[3] = {-kInf, -kInf, -kInf};
@@ -775,11 +780,29 @@ static void quick_reject(std::vector<Placed>& keep, const std::vector<Placed>& a
     for (int k = 0; k < 3; k++)
       if (keep[i].lo[k] > hi[k] || keep[i].hi[k] < lo[k]) hit = false;
     if (!hit) {
-      keep[i] = keep.back();
-      keep.pop_back();
+      if (g_mutation == MUT_STABLE_ERASE) {
+        keep.erase(keep.begin() + (long)i);
+      } else {
+        keep[i] = keep.back();
+        keep.pop_back();
+      }
       i--;
     }
   }
+}
+
+// Both reductions of CheckCollision(world, ids1, ids2): set 2 (the
+// environment) against the union of set 1 (the links) first, then set 1
+// against the union of what is left of set 2 (PlannerSettings.cpp:288-317).
+static void reduce_sets(std::vector<Placed>& L1, std::vector<Placed>& L2) {
+  if (g_mutation == MUT_NO_REJECT) return;
+  if (g_mutation == MUT_LINKS_FIRST) {
+    quick_reject(L1, L2);
+    quick_reject(L2, L1);
+    return;
+  }
+  quick_reject(L2, L1);
+  quick_reject(L1, L2);
 }
 
 struct CheckResult {
@@ -801,10 +824,12 @@ static CheckResult check_config(const World& w, int r, const double* q, bool bru
   env_items(w, r, L2);
   std::vector<Placed> S = L1;  // self check uses the unreduced link list
   // env: CheckCollision(world, ids1, ids2, tol=0)
-  quick_reject(L2, L1);
-  quick_reject(L1, L2);
-  for (size_t i = 0; i < L1.size() && res.env1 < 0; i++)
-    for (size_t j = 0; j < L2.size(); j++) {
+  reduce_sets(L1, L2);
+  const bool env_major = g_mutation == MUT_ENV_MAJOR;
+  const size_t no = env_major ? L2.size() : L1.size(), ni = env_major ? L1.size() : L2.size();
+  for (size_t o = 0; o < no && res.env1 < 0; o++)
+    for (size_t k = 0; k < ni; k++) {
+      const size_t i = env_major ? k : o, j = env_major ? o : k;
       if (!(w.en(L1[i].id, L2[j].id) || w.en(L2[j].id, L1[i].id))) continue;
       if (!aabb_hit(L1[i], L2[j])) continue;
       if (mesh_collide(*L1[i].m, L1[i].X, *L2[j].m, L2[j].X, brute)) {
@@ -1067,6 +1092,12 @@ using namespace orc;
 extern "C" {
 
 void* or_world_new() { return new World(); }
+
+// Test hook: plant one of the policy mistakes above in every later call
+// (1 stable erase, 2 links reduced first, 3 no quick reject, 4 environment
+// loop outermost); 0 restores Klampt's policy.  Not thread-safe with calls
+// in flight.
+void or_set_mutation(int m) { g_mutation = m; }
 void or_world_free(void* w) { delete (World*)w; }
 
 int or_add_mesh(void* wp, const double* V, int nv, const int* T, int nt) {
@@ -1178,8 +1209,7 @@ static bool is_feasible(const World& w, int r, const double* q) {
   robot_links(w, r, T, L1);
   env_items(w, r, L2);
   std::vector<Placed> S = L1;
-  quick_reject(L2, L1);
-  quick_reject(L1, L2);
+  reduce_sets(L1, L2);
   for (size_t i = 0; i < L1.size(); i++)
     for (size_t j = 0; j < L2.size(); j++) {
       if (!(w.en(L1[i].id, L2[j].id) || w.en(L2[j].id, L1[i].id))) continue;

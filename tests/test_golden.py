@@ -31,6 +31,77 @@ def _limits_ok(sc, Q):
     return np.all((q >= sc.robot.qmin) & (q <= sc.robot.qmax), axis=1)
 
 
+def _scipy_frames(spec, q):
+    """Link world frames (R, t) by scipy rotations, T_i = T_p T0_i Rot(w_i q_i)."""
+    out = []
+    for i in range(spec.n):
+        R0, t0 = spec.T0[i][:9].reshape(3, 3).T, spec.T0[i][9:]
+        J = Rotation.from_rotvec(spec.axes[i] * float(q[i])).as_matrix()
+        if spec.parents[i] < 0:
+            out.append((R0 @ J, t0.copy()))
+        else:
+            Rp, tp = out[spec.parents[i]]
+            out.append((Rp @ R0 @ J, Rp @ t0 + tp))
+    return out
+
+
+def _aabb(mesh, R, t):
+    lo, hi = mesh.verts.min(0), mesh.verts.max(0)
+    c, e = (lo + hi) / 2, (hi - lo) / 2
+    wc, we = R @ c + t, np.abs(R) @ e
+    return wc - we, wc + we
+
+
+def _klampt_first_pair(sc, q, collides):
+    """First colliding pair of SingleRobotCSpace::CheckCollisionFree for one
+    configuration, restated here in Python from the reference
+    (Cpp/Planning/PlannerSettings.cpp:241-331, RobotCSpace.cpp:794-823):
+    environment set reduced against the links' AABB union, then the links
+    against the environment's, each with swap-erase; links outermost; then
+    the self pairs i < j over the unreduced links.  `collides` maps a world
+    ID pair to its narrow-phase answer.  Rigid objects only (every link /
+    object pair enabled); self pairs are the robot's default set."""
+    nobj = len(sc.objects)
+    rid = nobj
+    links = []
+    for i, (R, t) in enumerate(_scipy_frames(sc.robot, q)):
+        if sc.robot.link_meshes[i] is not None:
+            links.append((rid + 1 + i,) + _aabb(sc.robot.link_meshes[i], R, t))
+    env = []
+    for o, (m, x) in enumerate(sc.objects):
+        env.append((o,) + _aabb(m, x[:9].reshape(3, 3).T, x[9:]))
+
+    def meets(a, b):
+        return bool(np.all(a[1] <= b[2]) and np.all(b[1] <= a[2]))
+
+    def reduce(keep, against):
+        u = (None, np.min([a[1] for a in against], 0), np.max([a[2] for a in against], 0)) if against else None
+        i = 0
+        while i < len(keep):
+            if u is None or not meets(keep[i], u):
+                keep[i] = keep[-1]
+                keep.pop()
+            else:
+                i += 1
+
+    L1, L2 = list(links), list(env)
+    reduce(L2, L1)
+    reduce(L1, L2)
+    for a in L1:
+        for b in L2:
+            if meets(a, b) and collides.get((a[0], b[0]), False):
+                return (a[0], b[0])
+    pairs = {tuple(map(int, p)) for p in sc.robot.default_self_pairs()}
+    for ia in range(len(links)):
+        for ib in range(ia + 1, len(links)):
+            a, b = links[ia], links[ib]
+            if (a[0] - rid - 1, b[0] - rid - 1) not in pairs:
+                continue
+            if meets(a, b) and collides.get((a[0], b[0]), False):
+                return (a[0], b[0])
+    return (-1, -1)
+
+
 def test_c1_oracle_reproduces_fixture(c1):
     z, sc, ow, orb = c1
     Q = z["Q"]
@@ -77,13 +148,14 @@ def test_c1_fixture_is_consistent(c1):
         assert z["self"][c] == int(hit[is_self].any())
         if hit.any():
             # the reported pair intersects, and the environment is checked
-            # before self collision (the order inside each loop depends on
-            # the broad phase's swap-erase pruning)
+            # before self collision
             k = [i for i in range(len(pl)) if tuple(pl[i]) == tuple(z["first_pair"][c])]
             assert len(k) == 1 and hit[k[0]]
             assert is_self[k[0]] == (not hit[~is_self].any())
         else:
             assert tuple(z["first_pair"][c]) == (-1, -1)
+        # and it is exactly the pair Klampt's loops reach first
+        assert _klampt_first_pair(sc, Q[c], dict(zip(map(tuple, pl.tolist()), hit))) == tuple(z["first_pair"][c])
     assert np.all((dist[nb] == 0) == (sep[nb] <= 0))
     assert np.all(dist >= 0)
 
