@@ -64,6 +64,47 @@ int kg_ctx_create(int device, kg_ctx* out);
 int kg_ctx_destroy(kg_ctx ctx);
 int kg_ctx_synchronize(kg_ctx ctx);
 
+/* Engine parameters of a context: the traversal schedule, its capacities and
+ * a test hook, with their defaults in parentheses.  kg_ctx_get_params reads
+ * the current values (the defaults after kg_ctx_create), kg_ctx_set_params
+ * checks and replaces all of them (KG_E_INVALID if one is out of range, and
+ * then nothing changes), kg_autotune writes the fields it tunes.  No Klampt
+ * counterpart: Klampt's collision checks have no schedule to set. */
+typedef struct {
+  int32_t cpw;           /* configurations per chunk a traversal wave takes, 1..64 (16) */
+  int32_t leaf_min;      /* queued leaf pairs that start a triangle-test step, 1..64 (1) */
+  int32_t scap;          /* LDS task-pool entries per traversal wave: 256 or 512 (256) */
+  int32_t waves_per_cu;  /* traversal waves per CU, 0 = the occupancy limit (0) */
+  int32_t static_pct;    /* percent of a batch dealt to the waves up front, 0..100 (25) */
+  int32_t nsub;          /* ranges of the rest of the batch, one counter each, 1..32 (8) */
+  int32_t tail_lanes;    /* a wave with fewer working lanes runs the tail cadence, 0..64 (16) */
+  int32_t tail_reps;     /* box-test iterations per scheduling step in the tail, 1..8 (4) */
+  int32_t spill_cap;     /* HBM spill entries per traversal wave, 0..2^24 (16384); when a
+                            wave's spill region is full its oldest pooled tasks are dropped
+                            and what depends on them is reported as uncertain (below) */
+  int32_t dist_ipw;      /* distance items per chunk, 1..64 (4) */
+  int32_t edge_w0;       /* bisection positions in the first edge-check round, 1..2^20 (16) */
+  int32_t sdf_lanes;     /* lanes per waypoint of the SDF penalty kernels: 8, 16, 32, 64 (32) */
+  int32_t prep_group;    /* 1: lane-per-link broad phase where it applies, 0: one lane per
+                            configuration (1) */
+  int32_t force_recheck; /* test hook: every configuration goes through the FP64 pass (0) */
+} kg_params;
+int kg_ctx_get_params(kg_ctx ctx, kg_params* out);
+int kg_ctx_set_params(kg_ctx ctx, const kg_params* params);
+
+/* Status of the last batched call issued on this context (feasibility,
+ * distance, distance Jacobian, edge round), copied device-to-device into
+ * d_out[0..3] on `stream` in stream order, with no host synchronization (so a
+ * _dev caller or a captured graph can test it on the device):
+ *   [0] 1 if a traversal wave's spill region filled (tasks were dropped);
+ *   [1] results left uncertain by dropped tasks: configurations reported
+ *       infeasible (their open flags set, first pair (-2,-2)) without proof,
+ *       or pair distances reported as a certified lower bound;
+ *   [2] configurations re-checked in FP64;  [3] 0.
+ * When [1] is 0 every result of the call is exact.  Host-pointer calls fail
+ * with KG_E_LIMIT instead when [1] is not 0.  No Klampt counterpart. */
+int kg_ctx_status_dev(kg_ctx ctx, int32_t* d_out, void* stream);
+
 /* ---- geometry ----------------------------------------------------------
  * Triangle mesh: Geometry3D(TriangleMesh) with its collision data
  * (Python/klampt/src/robotsim.cpp:2886-2949 vertices/indices; the BVH built
@@ -75,6 +116,11 @@ int kg_geom_create_trimesh(kg_ctx ctx, const double* verts, int nverts, const in
  * C-order (i,j,k), dims[0] x dims[1] x dims[2], > 0 outside. */
 int kg_geom_create_sdf(kg_ctx ctx, const double bmin[3], const double bmax[3], const int32_t dims[3],
                        const float* values, kg_geom* out);
+/* Releases the handle.  Like Klampt's shared_ptr geometries, a geometry still
+ * held by a robot link, terrain or rigid object stays alive until the last of
+ * them lets it go (kg_robot_set_link_geom with another geometry,
+ * kg_world_set_*_geom, kg_robot_destroy, kg_world_destroy); the handle itself
+ * must not be used again. */
 int kg_geom_destroy(kg_geom g);
 /* PointCloud (Python/klampt/src/geometry.h PointCloud; the points only): pts
  * npts x 3 doubles in the geometry's local frame.  Point clouds take part in
@@ -184,6 +230,11 @@ int kg_world_destroy(kg_world w);
 int kg_world_add_terrain(kg_world w, kg_geom g, int* id_out);
 int kg_world_add_rigid_object(kg_world w, kg_geom g, const double R[9], const double t[3], int* id_out);
 int kg_world_set_rigid_object_transform(kg_world w, int object_index, const double R[9], const double t[3]);
+/* Replace a terrain's or rigid object's geometry in place (world ID and
+ * placement kept), as loading a file into the element's Geometry3D replaces
+ * the world's ManagedGeometry (Python/klampt/src/robotsim.cpp:1443-1453). */
+int kg_world_set_terrain_geom(kg_world w, int terrain_index, kg_geom g);
+int kg_world_set_rigid_object_geom(kg_world w, int object_index, kg_geom g);
 int kg_world_add_robot(kg_world w, kg_robot r, int* id_out);
 int kg_world_num_ids(kg_world w);
 /* WorldPlannerSettings::collisionEnabled(id1,id2) */
@@ -205,6 +256,14 @@ int kg_feasible_batch(kg_world w, int robot_index, const float* Q, int N, uint8_
                       uint8_t* self_hit, int32_t* first_pair);
 int kg_feasible_batch_dev(kg_world w, int robot_index, const float* dQ, int N, uint8_t* d_feasible,
                           uint8_t* d_env_hit, uint8_t* d_self_hit, int32_t* d_first_pair, void* stream);
+/* The same with double-precision configurations, as Klampt's Config is
+ * (SingleRobotCSpace::IsFeasible(const Config&), RobotCSpace.cpp:786): joint
+ * limits are tested on the doubles as given (a value on a limit is inside),
+ * and the link frames are formed from them in FP64. */
+int kg_feasible_batch_f64(kg_world w, int robot_index, const double* Q, int N, uint8_t* feasible, uint8_t* env_hit,
+                          uint8_t* self_hit, int32_t* first_pair);
+int kg_feasible_batch_f64_dev(kg_world w, int robot_index, const double* dQ, int N, uint8_t* d_feasible,
+                              uint8_t* d_env_hit, uint8_t* d_self_hit, int32_t* d_first_pair, void* stream);
 
 /* Pair distances with closest points and joint-space gradients: the
  * point-collision step of KineTrajOpt
@@ -273,14 +332,17 @@ int kg_feasible_multi_dev(int nbatch, const kg_world* worlds, const int32_t* rob
  * receives {cpw, leaf_min, scap, waves_per_cu (0 = occupancy limit)}. */
 int kg_autotune(int nbatch, const kg_world* worlds, const int32_t* robot_indices, const int32_t* counts,
                 const float* dQ, int32_t* chosen, float* best_ms);
+/* Sets the four tuned fields of the context's kg_params; 0 restores a field's
+ * default (scap: 0, 256 or 512). */
 int kg_ctx_set_tuning(kg_ctx ctx, int cpw, int leaf_min, int scap, int waves_per_cu);
 
-/* Counters of the last batched call on this world: [0] configurations
- * re-checked in FP64, [1] narrow-phase pair items, [2] spill events,
- * [3] broad-phase time (ns), [4] FP32 traversal kernel time (ns), [5] FP64
- * re-check kernel time (ns) -- HIP events on the launch stream; [3] and [5]
- * need timing level 2, [4] level 1 or 2 (kg_ctx_set_timing), else 0 -- [6]
- * box tests, [7] triangle tests of the FP32 traversal. */
+/* Counters of the last batched call on this world's context (waits for it):
+ * [0] configurations re-checked in FP64, [1] narrow-phase pair items, [2] 1
+ * if a spill region filled, [3] broad-phase time (ns), [4] FP32 traversal
+ * kernel time (ns), [5] FP64 re-check kernel time (ns) -- HIP events on the
+ * launch stream; [3] and [5] need timing level 2, [4] level 1 or 2
+ * (kg_ctx_set_timing), else 0 -- [6] box tests, [7] triangle tests of the
+ * FP32 traversal, [8] uncertain results (kg_ctx_status_dev [1]). */
 int kg_world_last_stats(kg_world w, int64_t* stats, int nstats);
 /* Sum of the FP32 traversal kernel durations (HIP events on the launch
  * stream) of the feasibility batches issued on this context since the last

@@ -31,6 +31,13 @@ class DistResult(ctypes.Structure):
                 ("grad2", _c_double * 3), ("elem1", _c_int), ("elem2", _c_int)]
 
 
+class Params(ctypes.Structure):
+    """kg_params: engine parameters of a context (include/klampt_gpu.h)."""
+    _fields_ = [(n, ctypes.c_int32) for n in (
+        "cpw", "leaf_min", "scap", "waves_per_cu", "static_pct", "nsub", "tail_lanes", "tail_reps", "spill_cap",
+        "dist_ipw", "edge_w0", "sdf_lanes", "prep_group", "force_recheck")]
+
+
 # name -> argtypes (all return int unless listed in _RESTYPE)
 SIGNATURES = {
     "kg_last_error": [],
@@ -38,6 +45,9 @@ SIGNATURES = {
     "kg_ctx_create": [_c_int, _pvp],
     "kg_ctx_destroy": [_vp],
     "kg_ctx_synchronize": [_vp],
+    "kg_ctx_get_params": [_vp, ctypes.POINTER(Params)],
+    "kg_ctx_set_params": [_vp, ctypes.POINTER(Params)],
+    "kg_ctx_status_dev": [_vp, _vp, _vp],
     "kg_geom_create_trimesh": [_vp, _pd, _c_int, _pi, _c_int, _pvp],
     "kg_geom_create_sdf": [_vp, _pd, _pd, _pi, _pf, _pvp],
     "kg_geom_destroy": [_vp],
@@ -76,6 +86,8 @@ SIGNATURES = {
     "kg_world_add_terrain": [_vp, _vp, _pi],
     "kg_world_add_rigid_object": [_vp, _vp, _pd, _pd, _pi],
     "kg_world_set_rigid_object_transform": [_vp, _c_int, _pd, _pd],
+    "kg_world_set_terrain_geom": [_vp, _c_int, _vp],
+    "kg_world_set_rigid_object_geom": [_vp, _c_int, _vp],
     "kg_world_add_robot": [_vp, _vp, _pi],
     "kg_world_num_ids": [_vp],
     "kg_world_set_collision_enabled": [_vp, _c_int, _c_int, _c_int],
@@ -83,6 +95,8 @@ SIGNATURES = {
     "kg_world_pair_list": [_vp, _c_int, _pi, _c_int, _pi],
     "kg_feasible_batch": [_vp, _c_int, _pf, _c_int, _pu8, _pu8, _pu8, _pi],
     "kg_feasible_batch_dev": [_vp, _c_int, _vp, _c_int, _vp, _vp, _vp, _vp, _vp],
+    "kg_feasible_batch_f64": [_vp, _c_int, _pd, _c_int, _pu8, _pu8, _pu8, _pi],
+    "kg_feasible_batch_f64_dev": [_vp, _c_int, _vp, _c_int, _vp, _vp, _vp, _vp, _vp],
     "kg_distance_batch": [_vp, _c_int, _pf, _c_int, _pf],
     "kg_distance_batch_dev": [_vp, _c_int, _vp, _c_int, _vp, _vp],
     "kg_edge_batch": [_vp, _c_int, _pf, _pf, _c_int, _c_double, _c_int, _pf],

@@ -186,6 +186,20 @@ void sync_pool(kg_ctx_s* c) {
   c->pool_version++;
 }
 
+// Geometries are shared like Klampt's shared_ptr<AnyCollisionGeometry3D>:
+// the handle and every slot holding the geometry (robot link, terrain, rigid
+// object) own a reference; the last release frees it.
+void geom_retain(kg_geom_s* g) {
+  if (g) g->refs++;
+}
+void geom_release(kg_geom_s* g) {
+  if (!g || --g->refs > 0) return;
+  g->dvals.release();
+  g->dpts.release();
+  // mesh data stays in the context pool until the context is destroyed
+  delete g;
+}
+
 bool geom_empty(const kg_geom_s* g) {
   return !g || g->kind != 0 || g->ctx->geoms[g->index].ntris == 0;
 }
@@ -387,25 +401,27 @@ RobotPlan& get_plan(kg_world_s* w, int ri) {
   return P;
 }
 
-int env_int(const char* name, int dflt) {
-  const char* v = std::getenv(name);
-  return v ? std::atoi(v) : dflt;
+// Device tables of a world's plans.
+void release_plans(kg_world_s* w) {
+  for (auto& p : w->plans) {
+    DevBuf* bufs[] = {&p.dlinks, &p.dqmin, &p.dqmax, &p.denv, &p.dlemask, &p.dself, &p.dpa, &p.dpb, &p.dpk};
+    for (auto* b : bufs) b->release();
+  }
 }
 
 struct BatchStats {
   int64_t redo = 0, items = 0, spills = 0;
-  int64_t t_prep_ns = 0, t_trav32_ns = 0, t_trav64_ns = 0, box_tests = 0, tri_tests = 0;
+  int64_t t_prep_ns = 0, t_trav32_ns = 0, t_trav64_ns = 0, box_tests = 0, tri_tests = 0, uncertain = 0;
 };
 
 // Status words of one batch (device, zeroed by the prep kernel):
 //   [0] spill overflow  [1] FP64 re-check count  [2] FP32 chunk counter
 //   [3] FP64 chunk counter  [4..9] u64 counters: box tests, triangle tests,
-//   pair items.
-// status words: flags and counters, then the traversal's chunk counters, one
-// per 128-byte line so that waves fetching from different ranges do not
-// queue on one address
-enum { ST_SPILL = 0, ST_REDO = 1, ST_CHUNK32 = 2, ST_CHUNK64 = 3, ST_CNT = 4, ST_DYN = 32, ST_NSUB = 32,
-       ST_WORDS = ST_DYN + 32 * ST_NSUB };
+//   pair items  [10] uncertain results (work dropped on overflow);
+// then the traversal's chunk counters, one per 128-byte line so that waves
+// fetching from different ranges do not queue on one address
+enum { ST_SPILL = 0, ST_REDO = 1, ST_CHUNK32 = 2, ST_CHUNK64 = 3, ST_CNT = 4, ST_UNC = 10, ST_DYN = 32,
+       ST_NSUB = 32, ST_WORDS = ST_DYN + 32 * ST_NSUB };
 void ensure_status_host(kg_ctx_s* c) {
   if (!c->hstatus) {
     hip_check(hipHostMalloc((void**)&c->hstatus, ST_WORDS * sizeof(int), hipHostMallocDefault), "hipHostMalloc");
@@ -507,7 +523,7 @@ void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* 
     // item's frame when it starts it
     const int ml = L.multi ? L.maxlinks : L.P->dev.nlinks;
     pa.group = ml <= 8 ? 8 : (ml <= 16 ? 16 : (ml <= 32 ? 32 : 0));
-    if (env_int("KG_PREP_GROUP", -1) == 0) pa.group = 0;
+    if (!c->prm.prep_group) pa.group = 0;
     pa.lds_links = !pa.group && !L.multi && ml <= 8;
     if (pa.group) {
       hip_check(c->wsRec.ensure(std::max<size_t>(1, L.ncap) * sizeof(ItemRec)), "hipMalloc(records)");
@@ -534,14 +550,15 @@ void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* 
   if (phase_ev) hip_check(hipEventRecord(c->ev[1], s), "hipEventRecord");
 
   // defaults measured on 65,536 configurations of the 7-DOF arm against the
-  // 200k-triangle clutter: the 256-entry pool leaves LDS for 4 waves/SIMD
-  const int scap_req = env_int("KG_SCAP", c->tune_scap ? c->tune_scap : 256);
-  const int scap = scap_req == 256 ? 256 : 512;
-  const int wpc = env_int("KG_WAVES_PER_CU", c->tune_wpc);
+  // 200k-triangle clutter (kg_default_params)
+  const kg_params& prm = c->prm;
+  const int scap = prm.scap == 256 ? 256 : 512;
+  const int wpc = prm.waves_per_cu;
   const int grid32 = wpc > 0 ? wpc * c->num_cus : kgl_collide_grid(0, c->num_cus, scap);
   const int grid64 = std::min(kgl_collide_grid(1, c->num_cus, scap), std::max(1, c->num_cus));
-  const int spill_cap = env_int("KG_SPILL_CAP", 16384);
-  hip_check(c->wsSpill.ensure((size_t)std::max(grid32, grid64) * spill_cap * sizeof(uint64_t)), "hipMalloc(spill)");
+  const int spill_cap = prm.spill_cap;
+  hip_check(c->wsSpill.ensure(std::max<size_t>(1, (size_t)std::max(grid32, grid64) * spill_cap) * sizeof(uint64_t)),
+            "hipMalloc(spill)");
 
   TravArgs ta{};
   ta.pool = c->pool;
@@ -560,17 +577,17 @@ void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* 
   ta.tol = L.tol32;
   ta.chunk_counter = status + ST_DYN;
   {
-    const int nsub = std::max(1, std::min((int)ST_NSUB, env_int("KG_NSUB", 8)));
+    const int nsub = std::max(1, std::min((int)ST_NSUB, prm.nsub));
     ta.nsub = N >= 64 * nsub ? nsub : 1;
   }
   ta.spill = c->wsSpill.as<uint64_t>();
   ta.spill_cap = spill_cap;
   ta.stack_cap = 0;
   ta.status = status + ST_SPILL;
-  ta.cpw = std::max(1, std::min(64, env_int("KG_CPW", c->tune_cpw ? c->tune_cpw : 16)));
-  ta.leaf_min = std::max(1, std::min(64, env_int("KG_LEAF_MIN", c->tune_leaf ? c->tune_leaf : 1)));
-  ta.tail_lanes = std::max(0, std::min(64, env_int("KG_TAIL_LANES", 16)));
-  ta.tail_reps = std::max(1, std::min(8, env_int("KG_TAIL_REPS", 4)));
+  ta.cpw = prm.cpw;
+  ta.leaf_min = prm.leaf_min;
+  ta.tail_lanes = prm.tail_lanes;
+  ta.tail_reps = prm.tail_reps;
   ta.grid = grid32;
   ta.counters = (unsigned long long*)(status + ST_CNT);
   ta.plans = L.plans;
@@ -578,7 +595,7 @@ void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* 
   ta.toff = L.toff;
   ta.ibase = L.ibase;
   ta.scap = scap;
-  ta.static_pct = std::max(0, std::min(100, env_int("KG_STATIC_PCT", 25)));
+  ta.static_pct = prm.static_pct;
   ta.recs = pa.recs;
   ta.Qf = Qf;
   ta.Qd = Qd;
@@ -600,7 +617,8 @@ void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* 
   fa.redo_list = redo_list;
   fa.redo_count = status + ST_REDO;
   fa.final_pass = 0;
-  fa.force_recheck = env_int("KG_FORCE_RECHECK", 0) != 0;
+  fa.force_recheck = prm.force_recheck;
+  fa.uncertain = status + ST_UNC;
   const int slot = (int)(c->tr_count % kg_ctx_s::TRING);
   if (kern_ev) hip_check(hipEventRecord(c->tr_beg[slot], s), "hipEventRecord");
   hip_check(kgl_collide_f32(ta, s), "collide_f32");
@@ -677,6 +695,7 @@ void read_stats(kg_world_s* w, BatchStats& st) {
   const int* h = fetch_status(c);
   st.spills = h[ST_SPILL];
   st.redo = h[ST_REDO];
+  st.uncertain = h[ST_UNC];
   unsigned long long cnt[3];
   std::memcpy(cnt, h + ST_CNT, sizeof(cnt));
   st.box_tests = (int64_t)cnt[0];
@@ -705,16 +724,21 @@ void read_stats(kg_world_s* w, BatchStats& st) {
   }
 }
 
-// Synchronous wrapper for internal single queries: errors of the FP64 pass
-// (spill overflow) surface here.
-void feasible_sync_check(kg_world_s* w) {
-  BatchStats st;
-  read_stats(w, st);
-  if (st.spills && st.redo > 0)
-    fail(KG_E_LIMIT, "BVH traversal stack exceeded its HBM spill capacity (KG_SPILL_CAP)");
+const char* kOverflowMsg =
+    "BVH traversal dropped work when its HBM spill region filled (kg_params.spill_cap); %d result(s) uncertain";
+
+// Host-pointer calls: results that dropped work left uncertain are an error
+// (the _dev calls report them through kg_ctx_status_dev instead).
+void check_uncertain(kg_ctx_s* c) {
+  const int unc = fetch_status(c)[ST_UNC];
+  if (unc > 0) fail(KG_E_LIMIT, kOverflowMsg, unc);
 }
 
+void feasible_sync_check(kg_world_s* w) { check_uncertain(w->ctx); }
+
 kg_world_s* solo_world(kg_robot_s* r);
+void forget_solo_world(const kg_robot_s* r);
+void forget_pair_world(const kg_ctx_s* c);
 
 }  // namespace
 
@@ -751,6 +775,7 @@ int kg_ctx_destroy(kg_ctx c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     (void)hipDeviceSynchronize();
+    forget_pair_world(c);
     DevBuf* bufs[] = {&c->dnodes, &c->dtv0, &c->dtv1, &c->dtv2, &c->dtv64, &c->dgeoms, &c->dtrisrc,
                       &c->wsQ, &c->wsT64, &c->wsLB, &c->wsJL, &c->wsCounts, &c->wsOffsets, &c->wsItems,
                       &c->wsScanTmp, &c->wsBest, &c->wsUnc, &c->wsChunk, &c->wsSpill, &c->wsStatus, &c->wsRedo,
@@ -880,11 +905,9 @@ int kg_geom_create_sdf(kg_ctx c, const double bmin[3], const double bmax[3], con
 int kg_geom_destroy(kg_geom g) {
   return guard([&] {
     if (!g) return;
-    g->dvals.release();
-    g->dpts.release();
+    need(g->alive, "geometry handle already destroyed");
     g->alive = false;
-    // mesh data stays in the context pool until the context is destroyed
-    delete g;
+    geom_release(g);  // freed now, or when the last robot link / world slot lets go
   });
 }
 
@@ -1152,6 +1175,8 @@ int kg_robot_create(kg_ctx c, int n, const int32_t* parents, const double* T0, c
 int kg_robot_destroy(kg_robot r) {
   return guard([&] {
     if (!r) return;
+    forget_solo_world(r);
+    for (auto* g : r->link_geoms) geom_release(g);
     r->dlinks.release();
     delete r;
   });
@@ -1163,6 +1188,9 @@ int kg_robot_set_link_geom(kg_robot r, int link, kg_geom g) {
     need(link >= 0 && link < r->n, "link index out of range");
     need(!g || g->kind == 0, "link geometry must be a triangle mesh");
     need(!g || g->ctx == r->ctx, "geometry belongs to another context");
+    need(!g || g->alive, "geometry handle was destroyed");
+    geom_retain(g);
+    geom_release(r->link_geoms[link]);
     r->link_geoms[link] = g;
     if (g) g->X = r->Tw[link];
     r->version++;
@@ -1318,15 +1346,14 @@ int kg_world_create(kg_ctx c, kg_world* out) {
 int kg_world_destroy(kg_world w) {
   return guard([&] {
     if (!w) return;
-    for (auto& p : w->plans) {
-      DevBuf* bufs[] = {&p.dlinks, &p.dqmin, &p.dqmax, &p.denv, &p.dlemask, &p.dself, &p.dpa, &p.dpb, &p.dpk};
-      for (auto* b : bufs) b->release();
-    }
+    release_plans(w);
     for (auto it = g_overrides.begin(); it != g_overrides.end(); ++it)
       if (it->first == w) {
         g_overrides.erase(it);
         break;
       }
+    for (auto* g : w->terrains) geom_release(g);
+    for (auto& o : w->objects) geom_release(o.first);
     delete w;
   });
 }
@@ -1342,6 +1369,8 @@ int kg_world_add_terrain(kg_world w, kg_geom g, int* id_out) {
     need(w && g, "NULL argument");
     need(g->kind == 0, "only triangle-mesh terrains are supported");
     need(g->ctx == w->ctx, "geometry belongs to another context");
+    need(g->alive, "geometry handle was destroyed");
+    geom_retain(g);
     w->terrains.push_back(g);
     world_changed(w);
     if (id_out) *id_out = (int)w->terrains.size() - 1;
@@ -1353,9 +1382,43 @@ int kg_world_add_rigid_object(kg_world w, kg_geom g, const double R[9], const do
     need(w && g && R && t, "NULL argument");
     need(g->kind == 0, "only triangle-mesh rigid objects are supported");
     need(g->ctx == w->ctx, "geometry belongs to another context");
+    need(g->alive, "geometry handle was destroyed");
+    geom_retain(g);
     w->objects.push_back({g, xf_from(R, t)});
     world_changed(w);
     if (id_out) *id_out = (int)(w->terrains.size() + w->objects.size()) - 1;
+  });
+}
+
+// A terrain's or rigid object's geometry replaced in place, as loading a file
+// into an element's Geometry3D replaces the world's ManagedGeometry
+// (Python/klampt/src/robotsim.cpp:1443-1453); the world ID and (for a rigid
+// object) the placement stay.
+int kg_world_set_terrain_geom(kg_world w, int ti, kg_geom g) {
+  return guard([&] {
+    need(w && g, "NULL argument");
+    need(ti >= 0 && ti < (int)w->terrains.size(), "terrain index out of range");
+    need(g->kind == 0, "only triangle-mesh terrains are supported");
+    need(g->ctx == w->ctx, "geometry belongs to another context");
+    need(g->alive, "geometry handle was destroyed");
+    geom_retain(g);
+    geom_release(w->terrains[ti]);
+    w->terrains[ti] = g;
+    w->version++;
+  });
+}
+
+int kg_world_set_rigid_object_geom(kg_world w, int oi, kg_geom g) {
+  return guard([&] {
+    need(w && g, "NULL argument");
+    need(oi >= 0 && oi < (int)w->objects.size(), "object index out of range");
+    need(g->kind == 0, "only triangle-mesh rigid objects are supported");
+    need(g->ctx == w->ctx, "geometry belongs to another context");
+    need(g->alive, "geometry handle was destroyed");
+    geom_retain(g);
+    geom_release(w->objects[oi].first);
+    w->objects[oi].first = g;
+    w->version++;
   });
 }
 
@@ -1425,36 +1488,67 @@ int kg_feasible_batch_dev(kg_world w, int ri, const float* dQ, int N, uint8_t* f
   });
 }
 
-int kg_feasible_batch(kg_world w, int ri, const float* Q, int N, uint8_t* feas, uint8_t* env, uint8_t* self,
-                      int32_t* fp) {
+int kg_feasible_batch_f64_dev(kg_world w, int ri, const double* dQ, int N, uint8_t* feas, uint8_t* env,
+                              uint8_t* self, int32_t* fp, void* stream) {
   return guard([&] {
+    need(w != nullptr, "world is NULL");
+    need(N >= 0, "negative batch size");
+    need(N == 0 || (dQ && feas), "NULL buffer");
+    hipStream_t s = (hipStream_t)stream;
+    StreamScope scope(w->ctx, s);
+    feasible_dev(w, ri, nullptr, dQ, N, feas, env, self, fp, s);
+  });
+}
+}  // extern "C"
+
+namespace {
+// Host-pointer feasibility with float (Qf) or double (Qd) configurations.
+void feasible_host(kg_world_s* w, int ri, const float* Qf, const double* Qd, int N, uint8_t* feas, uint8_t* env,
+                   uint8_t* self, int32_t* fp) {
     need(w != nullptr, "world is NULL");
     need(N >= 0, "negative batch size");
     need(ri >= 0 && ri < (int)w->robots.size(), "robot index out of range");
     if (N == 0) return;
-    need(Q && feas, "NULL buffer");
+    need((Qf || Qd) && feas, "NULL buffer");
     kg_ctx_s* c = w->ctx;
     hip_check(hipSetDevice(c->device), "hipSetDevice");
     StreamScope scope(c, c->stream);
     int n = w->robots[ri]->n;
-    size_t qb = (size_t)N * n * sizeof(float);
+    const size_t qb = (size_t)N * n * (Qd ? sizeof(double) : sizeof(float));
     hip_check(c->wsQ.ensure(qb), "hipMalloc");
     hip_check(c->wsOut8a.ensure(N), "hipMalloc");
     hip_check(c->wsOut8b.ensure(N), "hipMalloc");
     hip_check(c->wsOut8c.ensure(N), "hipMalloc");
     hip_check(c->wsOut32.ensure((size_t)N * 2 * sizeof(int)), "hipMalloc");
-    hip_check(hipMemcpyAsync(c->wsQ.p, Q, qb, hipMemcpyHostToDevice, c->stream), "copy Q");
+    hip_check(hipMemcpyAsync(c->wsQ.p, Qd ? (const void*)Qd : (const void*)Qf, qb, hipMemcpyHostToDevice, c->stream),
+              "copy Q");
     uint8_t* dfe = c->wsOut8a.as<uint8_t>();
     uint8_t* den = env ? c->wsOut8b.as<uint8_t>() : nullptr;
     uint8_t* dse = self ? c->wsOut8c.as<uint8_t>() : nullptr;
     int32_t* dfp = fp ? c->wsOut32.as<int32_t>() : nullptr;
-    feasible_dev(w, ri, c->wsQ.as<float>(), nullptr, N, dfe, den, dse, dfp, c->stream);
+    feasible_dev(w, ri, Qd ? nullptr : c->wsQ.as<float>(), Qd ? c->wsQ.as<double>() : nullptr, N, dfe, den, dse, dfp,
+                 c->stream);
     hip_check(hipMemcpyAsync(feas, dfe, N, hipMemcpyDeviceToHost, c->stream), "copy out");
     if (env) hip_check(hipMemcpyAsync(env, den, N, hipMemcpyDeviceToHost, c->stream), "copy out");
     if (self) hip_check(hipMemcpyAsync(self, dse, N, hipMemcpyDeviceToHost, c->stream), "copy out");
     if (fp) hip_check(hipMemcpyAsync(fp, dfp, (size_t)N * 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream), "copy");
     hip_check(hipStreamSynchronize(c->stream), "sync");
     feasible_sync_check(w);
+}
+}  // namespace
+
+extern "C" {
+
+int kg_feasible_batch(kg_world w, int ri, const float* Q, int N, uint8_t* feas, uint8_t* env, uint8_t* self,
+                      int32_t* fp) {
+  return guard([&] { feasible_host(w, ri, Q, nullptr, N, feas, env, self, fp); });
+}
+
+int kg_feasible_batch_f64(kg_world w, int ri, const double* Q, int N, uint8_t* feas, uint8_t* env, uint8_t* self,
+                          int32_t* fp) {
+  return guard([&] {
+    need(Q != nullptr || N == 0, "NULL buffer");
+    feasible_host(w, ri, nullptr, Q, N, feas, env, self, fp);
   });
 }
 
@@ -1463,9 +1557,9 @@ int kg_world_last_stats(kg_world w, int64_t* out, int nstats) {
     need(w && out, "NULL argument");
     BatchStats st;
     read_stats(w, st);
-    int64_t v[8] = {st.redo, st.items, st.spills, st.t_prep_ns, st.t_trav32_ns, st.t_trav64_ns, st.box_tests,
-                    st.tri_tests};
-    for (int i = 0; i < nstats && i < 8; i++) out[i] = v[i];
+    int64_t v[9] = {st.redo, st.items, st.spills, st.t_prep_ns, st.t_trav32_ns, st.t_trav64_ns, st.box_tests,
+                    st.tri_tests, st.uncertain};
+    for (int i = 0; i < nstats && i < 9; i++) out[i] = v[i];
   });
 }
 
@@ -1507,6 +1601,18 @@ kg_world_s* solo_world(kg_robot_s* r) {
   rebuild_mask(w);
   g_solo.push_back({r, w});
   return w;
+}
+
+// The internal one-robot world of a robot (kg_robot_self_collides) goes
+// with the robot, so a later robot at the same address gets a fresh one.
+void forget_solo_world(const kg_robot_s* r) {
+  for (auto it = g_solo.begin(); it != g_solo.end(); ++it)
+    if (it->first == r) {
+      release_plans(it->second);
+      delete it->second;
+      g_solo.erase(it);
+      return;
+    }
 }
 }  // namespace
 
@@ -1588,6 +1694,20 @@ PairWorld& pair_world(kg_ctx_s* c, kg_geom_s* a, kg_geom_s* b) {
   pw->w->version++;
   rebuild_mask(pw->w);
   return *pw;
+}
+
+// The context's internal pair world goes with the context (its geometry
+// slots hold no references: they are set anew before every query).
+void forget_pair_world(const kg_ctx_s* c) {
+  for (auto it = g_pairworlds.begin(); it != g_pairworlds.end(); ++it)
+    if (it->first == c) {
+      release_plans(it->second.w);
+      delete it->second.w;
+      it->second.r->dlinks.release();
+      delete it->second.r;
+      g_pairworlds.erase(it);
+      return;
+    }
 }
 }  // namespace
 
@@ -1753,7 +1873,8 @@ void sdf_penalty_dev(kg_robot_s* r, kg_geom_s* sdf, const float* dQ, int N, cons
   hip_check(c->wsPoff.ensure((n + 1) * sizeof(int)), "hipMalloc");
   hip_check(hipMemcpyAsync(c->wsPoff.p, off.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice, s), "copy offsets");
   SdfDev g = sdf_dev(sdf);
-  hip_check(kgl_sdf_penalty(r->dlinks.as<LinkDesc>(), n, g, dQ, N, dpts, c->wsPoff.as<int>(), dsafe, dloss, ddl, ddQ, bwd, s),
+  hip_check(kgl_sdf_penalty(r->dlinks.as<LinkDesc>(), n, g, dQ, N, dpts, c->wsPoff.as<int>(), dsafe, dloss, ddl, ddQ, bwd,
+                            c->prm.sdf_lanes, s),
             "sdf_penalty");
 }
 
@@ -1893,8 +2014,8 @@ void distance_dev(kg_world_s* w, int ri, const float* Qf, const double* Qd, int 
   int* status = c->wsStatus.as<int>();
   hip_check(kgl_fk64(P.dev.links, n, Qf, Qd, N, c->wsT64.as<double>(), status, ST_WORDS, s), "fk64");
   const int grid = kgl_distance_grid(c->num_cus);
-  const int spill_cap = env_int("KG_SPILL_CAP", 16384);
-  hip_check(c->wsSpill.ensure((size_t)grid * spill_cap * sizeof(uint64_t)), "hipMalloc(spill)");
+  const int spill_cap = c->prm.spill_cap;
+  hip_check(c->wsSpill.ensure(std::max<size_t>(1, (size_t)grid * spill_cap) * sizeof(uint64_t)), "hipMalloc(spill)");
   DistArgs da{};
   da.pool = c->pool;
   da.plan = P.dev;
@@ -1913,8 +2034,9 @@ void distance_dev(kg_world_s* w, int ri, const float* Qf, const double* Qd, int 
   da.spill_cap = spill_cap;
   da.status = status + ST_SPILL;
   da.grid = grid;
-  da.ipw = std::max(1, env_int("KG_DIST_IPW", 4));
+  da.ipw = c->prm.dist_ipw;
   da.counters = (unsigned long long*)(status + ST_CNT);
+  da.uncertain = status + ST_UNC;
   const bool phase_ev = c->timing >= 2;
   c->last_phase_ev = phase_ev;
   c->last_tr_slot = -1;
@@ -1969,7 +2091,7 @@ void pair_distance(kg_geom_s* a, kg_geom_s* b, float stop_below, double upper, k
   hip_check(hipMemcpyAsync(&d, dout, sizeof(float), hipMemcpyDeviceToHost, c->stream), "copy");
   hip_check(hipMemcpyAsync(tri, dtri, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream), "copy");
   hip_check(hipStreamSynchronize(c->stream), "sync");
-  if (fetch_status(c)[ST_SPILL]) fail(KG_E_LIMIT, "distance traversal exceeded its stack capacity (KG_SPILL_CAP)");
+  check_uncertain(c);
   if (dgpu) *dgpu = d;
   std::memset(out, 0, sizeof(*out));
   out->d = d;
@@ -2085,7 +2207,7 @@ extern "C" int kg_distance_jacobian_batch(kg_world w, int ri, const float* Q, in
     hip_check(hipMemcpyAsync(grad_out, c->wsEQ.p, (size_t)N * np * n * sizeof(float), hipMemcpyDeviceToHost, c->stream),
               "copy");
     hip_check(hipStreamSynchronize(c->stream), "sync");
-    if (fetch_status(c)[ST_SPILL]) fail(KG_E_LIMIT, "distance traversal exceeded its stack capacity (KG_SPILL_CAP)");
+    check_uncertain(c);
   });
 }
 
@@ -2109,7 +2231,7 @@ extern "C" int kg_distance_batch(kg_world w, int ri, const float* Q, int N, floa
     distance_dev(w, ri, c->wsQ.as<float>(), nullptr, N, c->wsF.as<float>(), nullptr, 0.0f, 3.0e38f, c->stream);
     hip_check(hipMemcpyAsync(out, c->wsF.p, (size_t)N * np * sizeof(float), hipMemcpyDeviceToHost, c->stream), "copy");
     hip_check(hipStreamSynchronize(c->stream), "sync");
-    if (fetch_status(c)[ST_SPILL]) fail(KG_E_LIMIT, "distance traversal exceeded its stack capacity (KG_SPILL_CAP)");
+    check_uncertain(c);
   });
 }
 
@@ -2269,7 +2391,7 @@ extern "C" int kg_edge_batch(kg_world w, int ri, const float* A, const float* B,
     // 4 w0, ... wide (KG_EDGE_W0, default 16).  The first infeasible sample in
     // bisection order is the same for any widths; wider rounds mean fewer
     // round trips and more samples checked past an early hit.
-    int pos0 = 0, width = std::max(1, env_int("KG_EDGE_W0", 16));
+    int pos0 = 0, width = c->prm.edge_w0;
     std::vector<int> live, soff;
     for (;;) {
       live.clear();
@@ -2428,9 +2550,7 @@ extern "C" int kg_feasible_multi(int nb, const kg_world* worlds, const int32_t* 
                   c->stream);
     hip_check(hipMemcpyAsync(feas, c->wsOut8a.p, N, hipMemcpyDeviceToHost, c->stream), "copy out");
     hip_check(hipStreamSynchronize(c->stream), "sync");
-    const int* st = fetch_status(c);
-    if (st[ST_SPILL] && st[ST_REDO] > 0)
-      fail(KG_E_LIMIT, "BVH traversal stack exceeded its HBM spill capacity (KG_SPILL_CAP)");
+    check_uncertain(c);
   });
 }
 
@@ -2454,8 +2574,13 @@ extern "C" int kg_autotune(int nb, const kg_world* worlds, const int32_t* ris, c
     struct Level {
       kg_ctx_s* c;
       int saved;
-      ~Level() { c->timing = saved; }
-    } level{c, c->timing};
+      kg_params prm;
+      bool done = false;
+      ~Level() {
+        c->timing = saved;
+        if (!done) c->prm = prm;  // an error leaves the parameters as they were
+      }
+    } level{c, c->timing, c->prm};
     c->timing = 2;
     const int cpws[] = {4, 8, 16, 32};
     const int leafs[] = {1, 8};
@@ -2467,10 +2592,10 @@ extern "C" int kg_autotune(int nb, const kg_world* worlds, const int32_t* ris, c
       for (int wpc : wpcs)
         for (int cpw : cpws)
           for (int leaf : leafs) {
-            c->tune_cpw = cpw;
-            c->tune_leaf = leaf;
-            c->tune_scap = scap;
-            c->tune_wpc = wpc;
+            c->prm.cpw = cpw;
+            c->prm.leaf_min = leaf;
+            c->prm.scap = scap;
+            c->prm.waves_per_cu = wpc;
             float tot = 0;
             for (int rep = 0; rep < 3; rep++) {
               feasible_core(c, L, dQ, nullptr, (int)N, c->wsOut8b.as<uint8_t>(), nullptr, nullptr, nullptr, s);
@@ -2487,10 +2612,11 @@ extern "C" int kg_autotune(int nb, const kg_world* worlds, const int32_t* ris, c
               bc[3] = wpc;
             }
           }
-    c->tune_cpw = bc[0];
-    c->tune_leaf = bc[1];
-    c->tune_scap = bc[2];
-    c->tune_wpc = bc[3];
+    c->prm.cpw = bc[0];
+    c->prm.leaf_min = bc[1];
+    c->prm.scap = bc[2];
+    c->prm.waves_per_cu = bc[3];
+    level.done = true;
     if (chosen)
       for (int i = 0; i < 4; i++) chosen[i] = bc[i];
     if (best_ms) *best_ms = best / 2;
@@ -2509,10 +2635,57 @@ extern "C" int kg_ctx_set_tuning(kg_ctx c, int cpw, int leaf_min, int scap, int 
   return guard([&] {
     need(c != nullptr, "ctx is NULL");
     need(cpw >= 0 && cpw <= 64 && leaf_min >= 0 && leaf_min <= 64, "tuning value out of range");
-    need(scap == 0 || scap == 512 || scap == 1024, "scap must be 512 or 1024");
-    c->tune_cpw = cpw;
-    c->tune_leaf = leaf_min;
-    c->tune_scap = scap;
-    c->tune_wpc = waves_per_cu;
+    need(scap == 0 || scap == 256 || scap == 512, "scap must be 0, 256 or 512");
+    need(waves_per_cu >= 0 && waves_per_cu <= 64, "waves_per_cu must be 0..64");
+    const kg_params d = kg_default_params();
+    c->prm.cpw = cpw ? cpw : d.cpw;
+    c->prm.leaf_min = leaf_min ? leaf_min : d.leaf_min;
+    c->prm.scap = scap ? scap : d.scap;
+    c->prm.waves_per_cu = waves_per_cu;
+  });
+}
+
+extern "C" int kg_ctx_get_params(kg_ctx c, kg_params* out) {
+  return guard([&] {
+    need(c && out, "NULL argument");
+    *out = c->prm;
+  });
+}
+
+extern "C" int kg_ctx_set_params(kg_ctx c, const kg_params* p) {
+  return guard([&] {
+    need(c && p, "NULL argument");
+    auto in = [](int v, int lo, int hi) { return v >= lo && v <= hi; };
+    need(in(p->cpw, 1, 64), "cpw must be 1..64");
+    need(in(p->leaf_min, 1, 64), "leaf_min must be 1..64");
+    need(p->scap == 256 || p->scap == 512, "scap must be 256 or 512");
+    need(in(p->waves_per_cu, 0, 64), "waves_per_cu must be 0..64");
+    need(in(p->static_pct, 0, 100), "static_pct must be 0..100");
+    need(in(p->nsub, 1, ST_NSUB), "nsub must be 1..32");
+    need(in(p->tail_lanes, 0, 64), "tail_lanes must be 0..64");
+    need(in(p->tail_reps, 1, 8), "tail_reps must be 1..8");
+    need(in(p->spill_cap, 0, 1 << 24), "spill_cap must be 0..2^24");
+    need(in(p->dist_ipw, 1, 64), "dist_ipw must be 1..64");
+    need(in(p->edge_w0, 1, 1 << 20), "edge_w0 must be 1..2^20");
+    need(p->sdf_lanes == 8 || p->sdf_lanes == 16 || p->sdf_lanes == 32 || p->sdf_lanes == 64,
+         "sdf_lanes must be 8, 16, 32 or 64");
+    need(in(p->prep_group, 0, 1) && in(p->force_recheck, 0, 1), "prep_group and force_recheck must be 0 or 1");
+    c->prm = *p;
+  });
+}
+
+extern "C" int kg_ctx_status_dev(kg_ctx c, int32_t* d_out, void* stream) {
+  return guard([&] {
+    need(c && d_out, "NULL argument");
+    hip_check(hipSetDevice(c->device), "hipSetDevice");
+    hipStream_t s = (hipStream_t)stream;
+    StreamScope scope(c, s);
+    if (!c->stat_src) {
+      // nothing issued yet: all zero
+      hip_check(c->wsStatus.ensure(ST_WORDS * sizeof(int)), "hipMalloc(status)");
+      hip_check(kgl_fill_int(c->wsStatus.as<int>(), ST_WORDS, 0, s), "clear status");
+      c->stat_src = c->wsStatus.as<int>();
+    }
+    hip_check(kgl_status_copy(c->stat_src, ST_SPILL, ST_UNC, ST_REDO, d_out, s), "status copy");
   });
 }

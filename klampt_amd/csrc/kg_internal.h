@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "kg_types.h"
+#include "../../include/klampt_gpu.h"
 
 struct BuiltTree {
   std::vector<ObbNode> nodes;  // local indices (child relative to node 0)
@@ -146,6 +147,7 @@ struct FinArgs {
   int* redo_count;
   int final_pass;   // 1: uncertainty cannot remain
   int force_recheck;  // test hook: every configuration goes to the FP64 pass
+  int* uncertain;     // count of configurations left open by dropped work (exact pass)
   const PlanDev* plans;  // heterogeneous batches (nullable)
   const int* cfg_plan;
 };
@@ -213,6 +215,7 @@ struct DistArgs {
   int grid;
   int ipw;               // items per chunk
   unsigned long long* counters;
+  int* uncertain;        // count of items reported as a lower bound (dropped work)
 };
 hipError_t kgl_distance(const DistArgs& a, hipStream_t s);
 int kgl_distance_grid(int num_cus);
@@ -266,9 +269,31 @@ int kgl_collide_grid(int exact, int num_cus, int scap);
 hipError_t kgl_sdf_eval(const SdfDev& g, const float* pts, int n, float* d, float* grad, hipStream_t s);
 hipError_t kgl_sdf_penalty(const LinkDesc* links, int n, const SdfDev& g, const float* Q, int N, const float* pts,
                            const int* poff, float dsafe, float* loss, const float* dloss, float* dQ, bool bwd,
-                           hipStream_t s);
+                           int lanes, hipStream_t s);
+// Copies the status words [0] spill flag, [1] uncertain count, [2] FP64
+// re-checks of a batch into out[0..3] (out[3] = 0).
+hipError_t kgl_status_copy(const int* status, int w_spill, int w_unc, int w_redo, int32_t* out, hipStream_t s);
 
 // ---- C-ABI object layouts
+
+inline kg_params kg_default_params() {
+  kg_params p;
+  p.cpw = 16;
+  p.leaf_min = 1;
+  p.scap = 256;  // leaves LDS for 4 traversal waves per SIMD
+  p.waves_per_cu = 0;
+  p.static_pct = 25;
+  p.nsub = 8;
+  p.tail_lanes = 16;
+  p.tail_reps = 4;
+  p.spill_cap = 16384;
+  p.dist_ipw = 4;
+  p.edge_w0 = 16;
+  p.sdf_lanes = 32;
+  p.prep_group = 1;
+  p.force_recheck = 0;
+  return p;
+}
 
 struct kg_ctx_s;
 struct kg_geom_s;
@@ -293,7 +318,8 @@ struct kg_geom_s {
   double bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
   DevBuf dvals;
   DevBuf dpts;     // points (point cloud) or mesh vertices as float4, uploaded on first use
-  bool alive = true;
+  bool alive = true;  // the handle has not been destroyed
+  int refs = 1;       // the handle plus every robot link, terrain and rigid object slot holding it
   double margin = 0;  // collision margin (Geometry3D.setCollisionMargin)
 };
 
@@ -372,8 +398,7 @@ struct kg_ctx_s {
   bool last_phase_ev = false;
   int last_tr_slot = -1;    // kernel-event slot of the last feasibility batch (-1: none)
   int num_cus = 256;
-  // traversal settings (autotuner; 0 = default)
-  int tune_cpw = 0, tune_leaf = 0, tune_wpc = 0, tune_scap = 0;
+  kg_params prm = kg_default_params();  // engine parameters (kg_ctx_set_params, kg_autotune)
   // heterogeneous-batch layout cache
   DevBuf mPlans, mCfgPlan, mQoff, mToff, mIbase;
   std::vector<uint64_t> mSig;

@@ -940,12 +940,28 @@ __device__ inline void finalize_cfg(const FinArgs& a, int c, unsigned long long 
     a.redo_list[slot] = c;
     return;
   }
-  if (a.feasible) a.feasible[c] = (jl && !h0 && !h1) ? 1 : 0;
-  if (a.env_hit) a.env_hit[c] = h0 ? 1 : 0;
-  if (a.self_hit) a.self_hit[c] = h1 ? 1 : 0;
+  // Exact pass: an answer can still be open only when the traversal dropped
+  // work (spill region full).  What depends on it is reported on the
+  // colliding side -- infeasible, the group's flag set, first pair (-2,-2)
+  // -- and the configuration is counted, so no dropped work can turn into a
+  // "feasible".
+  const bool open0 = !h0 && u0 != KG_RANK_NONE, open1 = !h1 && u1 != KG_RANK_NONE;
+  if (!sure) atomicAdd(a.uncertain, 1);
+  if (a.feasible) a.feasible[c] = (jl && !h0 && !h1 && !open0 && !open1) ? 1 : 0;
+  if (a.env_hit) a.env_hit[c] = (h0 || open0) ? 1 : 0;
+  if (a.self_hit) a.self_hit[c] = (h1 || open1) ? 1 : 0;
   if (FPAIR && a.first_pair) {
     int id1 = -1, id2 = -1;
-    if (h0 || h1) {
+    if (!sure) {
+      // the environment pair is known when a hit ranks before every open
+      // environment item; the self pair only when the environment is free
+      const int r0 = h0 ? (int)(b0 >> 32) : KG_RANK_NONE;
+      const int r1 = h1 ? (int)(b1 >> 32) : KG_RANK_NONE;
+      const bool env_known = u0 == KG_RANK_NONE || (h0 && u0 > r0);
+      const bool self_known = u1 == KG_RANK_NONE || (h1 && u1 > r1);
+      if (!(env_known && (h0 || self_known))) id1 = id2 = -2;
+    }
+    if (id1 != -2 && (h0 || h1)) {
       const PairItem it = a.items[(uint32_t)((h0 ? b0 : b1) & 0xffffffffu)];
       const PlanDev& PL = a.cfg_plan ? a.plans[a.cfg_plan[c]] : a.plan;
       id1 = PL.links[it.a].wid;
@@ -1587,6 +1603,20 @@ hipError_t kgl_collide_f32(const TravArgs& a, hipStream_t s) { return launch_col
 hipError_t kgl_collide_f64(const TravArgs& a, hipStream_t s) { return launch_collide<double, true>(a, s); }
 
 
+__global__ void k_status_copy(const int* __restrict__ st, int w_spill, int w_unc, int w_redo, int32_t* out) {
+  if (threadIdx.x == 0) {
+    out[0] = st[w_spill];
+    out[1] = st[w_unc];
+    out[2] = st[w_redo];
+    out[3] = 0;
+  }
+}
+
+hipError_t kgl_status_copy(const int* status, int w_spill, int w_unc, int w_redo, int32_t* out, hipStream_t s) {
+  k_status_copy<<<1, 64, 0, s>>>(status, w_spill, w_unc, w_redo, out);
+  return hipGetLastError();
+}
+
 hipError_t kgl_fill_int(int* p, int n, int v, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   k_fill_int<<<(n + 255) / 256, 256, 0, s>>>(p, n, v);
@@ -1603,8 +1633,6 @@ int kgl_collide_grid(int exact, int num_cus, int scap) {
     e = exact ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_collide<double, true, 512, false, false>, 64, 0)
               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_collide<float, false, 512, false, true>, 64, 0);
   if (e != hipSuccess || nb < 1) nb = 8;
-  const char* v = getenv("KG_WAVES_PER_CU");
-  if (v && atoi(v) > 0) nb = atoi(v);
   return nb * num_cus;
 }
 
@@ -1861,14 +1889,12 @@ hipError_t kgl_sdf_eval(const SdfDev& g, const float* pts, int n, float* d, floa
 
 hipError_t kgl_sdf_penalty(const LinkDesc* links, int n, const SdfDev& g, const float* Q, int N, const float* pts,
                            const int* poff, float dsafe, float* loss, const float* dloss, float* dQ, bool bwd,
-                           hipStream_t s) {
+                           int lanes, hipStream_t s) {
   if (N <= 0) return hipSuccess;
-  // GL lanes per configuration (KG_SDF_GL: 8, 16, 32 or 64; default 32,
-  // 5% faster than 16 on the 7-link arm, 448 points per waypoint); groups
-  // per block limited by 48 KB of LDS
-  const char* env = getenv("KG_SDF_GL");
-  const int want = env ? atoi(env) : 32;
-  const int GLr = (want == 8 || want == 16 || want == 64) ? want : 32;
+  // GL lanes per configuration (8, 16, 32 or 64; 32 is 5% faster than 16 on
+  // the 7-link arm, 448 points per waypoint); groups per block limited by
+  // 48 KB of LDS
+  const int GLr = (lanes == 8 || lanes == 16 || lanes == 64) ? lanes : 32;
   const size_t per_group = (size_t)n * 18 * sizeof(float);
   int gpb = 256 / GLr;
   while (gpb > 1 && gpb * per_group > 48 * 1024) gpb /= 2;
@@ -1958,7 +1984,7 @@ __global__ __launch_bounds__(64) void k_distance(DistArgs a) {
   __shared__ uint64_t lq[KG_LQ];
   __shared__ float s_rel[12][64];
   __shared__ float s_pad[64];
-  __shared__ unsigned s_best[64];
+  __shared__ unsigned s_best[64], s_droplb[64];
   __shared__ int s_item[64], s_pend[64], s_ta[64], s_tb[64];
   __shared__ int s_chunk;
 
@@ -1979,31 +2005,59 @@ __global__ __launch_bounds__(64) void k_distance(DistArgs a) {
     int it = i0, sp = 0, gsp = 0, nl = 0;
     uint64_t cont = KG_NOTASK;
 
+    // An item's result is published when its last task settles.  If the
+    // traversal dropped tasks of the item (spill region full) and one of
+    // them could still hold a closer pair, the result is the certified lower
+    // bound min(best, smallest bound of a dropped task) and the item is
+    // counted as uncertain.
+    auto finish = [&](int slot, int delta) {
+      if (atomicAdd(&s_pend[slot], delta) + delta == 0) {
+        const int item = s_item[slot];
+        const unsigned best = s_best[slot], dlb = s_droplb[slot];
+        a.out[item] = __uint_as_float(min(best, dlb));
+        if (dlb < best && best > stop) atomicAdd(a.uncertain, 1);
+        if (a.out_tri) {
+          a.out_tri[2 * item] = s_ta[slot];
+          a.out_tri[2 * item + 1] = s_tb[slot];
+        }
+      }
+    };
     auto ensure_room = [&](int room) {
       if (sp + room <= KG_SCAP) return;
       const int h = KG_SCAP / 2;
       if (gsp + h <= a.spill_cap) {
         for (int i = lane; i < h; i += 64) gsp_base[gsp + i] = stk[i];
         gsp += h;
+        __syncthreads();
       } else {
-        // overflow: drop the bottom half (the results become upper bounds)
-        for (int i = lane; i < h; i += 64) atomicAdd(&s_pend[(int)(stk[i] >> 58)], -1);
+        // overflow: drop the bottom half, keeping for each item the smallest
+        // certified distance bound of its dropped tasks (box bound minus the
+        // item's FP32 padding, at least 0)
+        for (int i = lane; i < h; i += 64) {
+          const uint64_t t = stk[i];
+          const int sl = (int)(t >> 58);
+          const int side = (int)((t >> 57) & 1), single = (int)((t >> 56) & 1);
+          const int x = (int)((t >> 28) & 0xfffffff), y = (int)(t & 0xfffffff);
+          float rel[12];
+#pragma unroll
+          for (int k = 0; k < 12; k++) rel[k] = s_rel[k][sl];
+          const NodeF O = load_nodef(a.pool.nodes, y);
+          const bool aIsS = single || side == 0;
+          float lbm = 3e38f;
+          for (int k = 0; k < (single ? 1 : 2); k++) {
+            const NodeF S = load_nodef(a.pool.nodes, x + k);
+            lbm = fminf(lbm, obb_lower_bound<float>(sel(aIsS, S, O), sel(aIsS, O, S), rel) - s_pad[sl]);
+          }
+          atomicMin(&s_droplb[sl], __float_as_uint(fmaxf(lbm, 0.f)));
+        }
         if (lane == 0) atomicExch(a.status, 1);
+        __syncthreads();
+        for (int i = lane; i < h; i += 64) finish((int)(stk[i] >> 58), -1);
+        __syncthreads();
       }
-      __syncthreads();
       for (int i = lane; i < sp - h; i += 64) stk[i] = stk[i + h];
       sp -= h;
       __syncthreads();
-    };
-    auto finish = [&](int slot, int delta) {
-      if (atomicAdd(&s_pend[slot], delta) + delta == 0) {
-        const int item = s_item[slot];
-        a.out[item] = __uint_as_float(s_best[slot]);
-        if (a.out_tri) {
-          a.out_tri[2 * item] = s_ta[slot];
-          a.out_tri[2 * item + 1] = s_tb[slot];
-        }
-      }
     };
     auto run_leaves = [&]() {
       const int m = min(nl, 64);
@@ -2111,6 +2165,7 @@ __global__ __launch_bounds__(64) void k_distance(DistArgs a) {
         const double ev = u * (8.0 * GA.rmax + 4.0 * sqrt(tn)) + u * GB.rmax;
         s_pad[myslot] = (float)(2.0 * ev + 16.0 * u * (sqrt(tn) + GA.rmax + GB.rmax));
         s_best[myslot] = __float_as_uint(a.upper);
+        s_droplb[myslot] = 0x7f800000u;  // +inf: nothing dropped
         s_item[myslot] = myitem;
         s_ta[myslot] = -1;
         s_tb[myslot] = -1;

@@ -11,7 +11,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import call, lib, DistSettings, DistResult
+from ._lib import call, lib, DistSettings, DistResult, Params
 
 _pd = ctypes.POINTER(ctypes.c_double)
 _pf = ctypes.POINTER(ctypes.c_float)
@@ -62,6 +62,35 @@ class Context:
         ns, cnt = ctypes.c_int64(), ctypes.c_int32()
         call("kg_ctx_kernel_times", self.h, 1 if reset else 0, ctypes.byref(ns), ctypes.byref(cnt))
         return ns.value * 1e-9, cnt.value
+
+    def params(self):
+        """The context's engine parameters (kg_params) as a dict."""
+        p = Params()
+        call("kg_ctx_get_params", self.h, ctypes.byref(p))
+        return {name: int(getattr(p, name)) for name, _ in Params._fields_}
+
+    def set_params(self, **changes):
+        """Change some engine parameters; returns the previous values (a dict
+        that set_params(**old) restores)."""
+        old = self.params()
+        new = dict(old)
+        for k, v in changes.items():
+            if k not in new:
+                raise KeyError(f"unknown engine parameter {k!r}")
+            new[k] = int(v)
+        p = Params(**new)
+        call("kg_ctx_set_params", self.h, ctypes.byref(p))
+        return old
+
+    def status_dev(self, out=None):
+        """Status of the last batched call as a 4-int device tensor [spill
+        flag, uncertain results, FP64 re-checks, 0], written in stream order
+        on the current torch stream (kg_ctx_status_dev)."""
+        import torch
+        if out is None:
+            out = torch.empty(4, dtype=torch.int32, device=f"cuda:{self.device}")
+        call("kg_ctx_status_dev", self.h, _tp(out), _torch_stream())
+        return out
 
     def set_timing(self, level):
         """HIP events recorded per batch: 0 none, 1 (default) the traversal
@@ -336,6 +365,14 @@ class World:
         self._keep.append(geom)
         return out.value
 
+    def set_terrain_geom(self, index, geom):
+        call("kg_world_set_terrain_geom", self.h, index, geom.h)
+        self._keep.append(geom)
+
+    def set_rigid_object_geom(self, index, geom):
+        call("kg_world_set_rigid_object_geom", self.h, index, geom.h)
+        self._keep.append(geom)
+
     def set_rigid_object_transform(self, index, xf12):
         x = _d(xf12)
         call("kg_world_set_rigid_object_transform", self.h, index, _p(x[:9].copy(), _pd), _p(x[9:].copy(), _pd))
@@ -365,25 +402,36 @@ class World:
         return out[:n.value].copy()
 
     def feasible_batch(self, Q, robot=0, flags=False, first_pair=False):
+        """IsFeasible over a batch.  float64 configurations (Klampt's Config)
+        are checked as given (kg_feasible_batch_f64); anything else as
+        float32 (kg_feasible_batch)."""
         n = self.robots[robot].n
-        Q = _f(Q).reshape(-1, n)
+        dbl = np.asarray(Q).dtype == np.float64
+        Q = (_d(Q) if dbl else _f(Q)).reshape(-1, n)
         N = Q.shape[0]
         feas = np.zeros(N, np.uint8)
         env = np.zeros(N, np.uint8) if flags else None
         slf = np.zeros(N, np.uint8) if flags else None
         fp = np.zeros((N, 2), np.int32) if first_pair else None
-        call("kg_feasible_batch", self.h, robot, _p(Q, _pf), N, _p(feas, _pu8), _p(env, _pu8), _p(slf, _pu8),
-             _p(fp, _pi))
+        if dbl:
+            call("kg_feasible_batch_f64", self.h, robot, _p(Q, _pd), N, _p(feas, _pu8), _p(env, _pu8),
+                 _p(slf, _pu8), _p(fp, _pi))
+        else:
+            call("kg_feasible_batch", self.h, robot, _p(Q, _pf), N, _p(feas, _pu8), _p(env, _pu8), _p(slf, _pu8),
+                 _p(fp, _pi))
         return feas, env, slf, fp
 
     def feasible_batch_dev(self, Q, robot=0, feasible=None, env=None, self_hit=None, first_pair=None):
+        """Device-tensor IsFeasible on the current torch stream; Q float32 or
+        float64 (checked as given)."""
         import torch
-        assert Q.is_cuda and Q.dtype == torch.float32 and Q.is_contiguous()
+        assert Q.is_cuda and Q.dtype in (torch.float32, torch.float64) and Q.is_contiguous()
         N = Q.shape[0]
         if feasible is None:
             feasible = torch.empty(N, dtype=torch.uint8, device=Q.device)
-        call("kg_feasible_batch_dev", self.h, robot, _tp(Q), N, _tp(feasible), _tp(env), _tp(self_hit),
-             _tp(first_pair), _torch_stream())
+        name = "kg_feasible_batch_f64_dev" if Q.dtype == torch.float64 else "kg_feasible_batch_dev"
+        call(name, self.h, robot, _tp(Q), N, _tp(feasible), _tp(env), _tp(self_hit), _tp(first_pair),
+             _torch_stream())
         return feasible
 
     def distance_batch(self, Q, robot=0):
@@ -431,7 +479,9 @@ class World:
              _p(g, _pf))
         return d, cp, g
 
-    def edge_batch(self, A, B, eps=0.01, norm_inf=True, robot=0):
+    def edge_batch(self, A, B, eps=0.01, norm_inf=False, robot=0):
+        """First infeasible interior sample u per segment (-1 if none); the
+        segment length is the L2 norm unless norm_inf (the L-infinity norm)."""
         n = self.robots[robot].n
         A, B = _f(A).reshape(-1, n), _f(B).reshape(-1, n)
         out = np.zeros(A.shape[0], np.float32)
@@ -440,10 +490,10 @@ class World:
         return out
 
     def last_stats(self):
-        s = np.zeros(8, np.int64)
-        call("kg_world_last_stats", self.h, s.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), 8)
+        s = np.zeros(9, np.int64)
+        call("kg_world_last_stats", self.h, s.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), 9)
         keys = ["fp64_rechecks", "pair_items", "spills", "t_prep_ns", "t_trav32_ns", "t_trav64_ns",
-                "box_tests", "tri_tests"]
+                "box_tests", "tri_tests", "uncertain"]
         return {k: int(v) for k, v in zip(keys, s)}
 
 

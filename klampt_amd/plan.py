@@ -190,7 +190,8 @@ class RobotCSpace(CSpace):
         return bool(self.feasible_batch(np.asarray([x], float))[0])
 
     def feasible_batch(self, Q):
-        Q = np.asarray(Q, np.float32).reshape(-1, len(self.bound))
+        # double configurations are checked as given (joint limits included)
+        Q = np.asarray(Q, np.float64).reshape(-1, len(self.bound))
         self._stats["feasible_configs"] += len(Q)
         ok = self._c.feasible_batch(Q)[0].astype(bool)
         for f, _ in self._extra:
@@ -200,17 +201,19 @@ class RobotCSpace(CSpace):
 
     def selfCollision(self, x=None):
         x = self.robot.getConfig() if x is None else x
-        _, env, slf, _ = self._c.feasible_batch(np.asarray([x], np.float32), flags=True)
+        _, env, slf, _ = self._c.feasible_batch(np.asarray([x], np.float64), flags=True)
         return bool(slf[0])
 
     def envCollision(self, x=None):
         x = self.robot.getConfig() if x is None else x
-        _, env, slf, _ = self._c.feasible_batch(np.asarray([x], np.float32), flags=True)
+        _, env, slf, _ = self._c.feasible_batch(np.asarray([x], np.float64), flags=True)
         return bool(env[0])
 
     def visible_batch(self, A, B):
         A = np.asarray(A, np.float32).reshape(-1, len(self.bound))
         B = np.asarray(B, np.float32).reshape(-1, len(self.bound))
+        # endpoints on a joint limit stay inside it after the float32 cast
+        A, B = self._f32_inside(A), self._f32_inside(B)
         self._stats["visible_edges"] += len(A)
         if len(A) == 0:
             return np.zeros(0, bool)
@@ -218,6 +221,16 @@ class RobotCSpace(CSpace):
         if self._extra:
             ok &= CSpace.visible_batch(self, A, B)
         return ok
+
+    def _f32_inside(self, X):
+        """float32 values clamped to the float32 numbers inside the double
+        joint limits (a double on a limit can round outside it)."""
+        lo = np.array([b[0] for b in self.bound], np.float64)
+        hi = np.array([b[1] for b in self.bound], np.float64)
+        lo32, hi32 = lo.astype(np.float32), hi.astype(np.float32)
+        lo32 = np.where(lo32.astype(np.float64) < lo, np.nextafter(lo32, np.float32(np.inf)), lo32)
+        hi32 = np.where(hi32.astype(np.float64) > hi, np.nextafter(hi32, np.float32(-np.inf)), hi32)
+        return np.clip(X, lo32, hi32).astype(np.float32)
 
     def collision_pairs(self, Q, dcheck):
         """The point-collision terms of a trajectory optimiser over

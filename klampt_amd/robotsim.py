@@ -130,10 +130,43 @@ class Geometry3D:
         self._ctx = ctx or default_context()
         self._g = None
         self._type = ""
+        # the robot links / terrains / rigid objects this geometry is attached
+        # to: (kind, model, index); set() and loadFile() update them, as
+        # Klampt's world elements share the geometry they were given
+        self._owners = []
         if data is not None:
             self.set(data)
 
+    def _attach(self, kind, model, index):
+        self._owners.append((kind, model, index))
+
+    def _detach(self, kind, model, index):
+        self._owners = [o for o in self._owners if not (o[0] == kind and o[1] is model and o[2] == index)]
+
+    def _notify_owners(self):
+        for kind, model, index in self._owners:
+            if kind == "link":
+                model._r.set_link_geom(index, None if self.empty() else self._g)
+            elif kind == "terrain":
+                model._w.set_terrain_geom(index, self._g)
+            else:
+                model._w.set_rigid_object_geom(index, self._g)
+
     def set(self, data):
+        """Replace the geometry data (``Geometry3D.set``); an attached robot
+        link, terrain or rigid object uses the new data from now on, and the
+        current transform and collision margin carry over."""
+        old = self._g
+        self._set_data(data)
+        if old is not None:
+            R, t = old.get_transform()
+            self._g.set_transform(R, t)
+            m = old.get_margin()
+            if m > 0:
+                self._g.set_margin(m)
+        self._notify_owners()
+
+    def _set_data(self, data):
         if isinstance(data, TriangleMesh):
             self._g = engine.Geom.trimesh(self._ctx, data.vertices, data.indices)
             self._type = "TriangleMesh"
@@ -292,7 +325,12 @@ class RobotModelLink:
         return self._robot._geoms[self.index]
 
     def setGeometry(self, geom: Optional[Geometry3D]):
+        old = self._robot._geoms[self.index]
+        if old is not None:
+            old._detach("link", self._robot, self.index)
         self._robot._geoms[self.index] = geom
+        if geom is not None:
+            geom._attach("link", self._robot, self.index)
         self._robot._r.set_link_geom(self.index, None if geom is None or geom.empty() else geom._g)
 
     def getTransform(self):
@@ -344,6 +382,8 @@ class RobotModel:
         self._r = engine.Robot(self._ctx, parents, T0, axes, jtype, qmin, qmax)
         self._qmin, self._qmax = list(map(float, qmin)), list(map(float, qmax))
         self._geoms: List[Optional[Geometry3D]] = [Geometry3D(ctx=self._ctx) for _ in range(len(parents))]
+        for i, g in enumerate(self._geoms):
+            g._attach("link", self, i)
         self._names = ["Link_%d" % i for i in range(len(parents))]
         self._name = "Robot"
         self._axes = np.asarray(axes, np.float64).reshape(-1, 3)
@@ -468,7 +508,9 @@ class WorldModel:
     def addTerrain(self, geom: Geometry3D, name: str = "Terrain") -> int:
         self._terrains.append(geom)
         self._terrain_models.append(TerrainModel(self, len(self._terrains) - 1, geom, name))
-        return self._w.add_terrain(geom._g)
+        wid = self._w.add_terrain(geom._g)
+        geom._attach("terrain", self, len(self._terrains) - 1)
+        return wid
 
     def addRigidObject(self, geom: Geometry3D, R=None, t=None, name: str = "Object") -> int:
         R = [1, 0, 0, 0, 1, 0, 0, 0, 1] if R is None else R
@@ -477,7 +519,9 @@ class WorldModel:
         m = RigidObjectModel(self, len(self._objects) - 1, geom, name)
         m._xf = np.r_[np.asarray(R, float), np.asarray(t, float)]
         self._object_models.append(m)
-        return self._w.add_rigid_object(geom._g, m._xf)
+        wid = self._w.add_rigid_object(geom._g, m._xf)
+        geom._attach("object", self, len(self._objects) - 1)
+        return wid
 
     def numTerrains(self) -> int:
         return len(self._terrains)
@@ -623,7 +667,7 @@ class RobotCSpace:
         self.eps, self.norm_inf = eps, norm_inf
 
     def isFeasible(self, q) -> bool:
-        return bool(self.world._w.feasible_batch(np.asarray([q], np.float32), robot=self.index)[0][0])
+        return bool(self.world._w.feasible_batch(np.asarray([q], np.float64), robot=self.index)[0][0])
 
     def isVisible(self, a, b) -> bool:
         u = self.world._w.edge_batch(np.asarray([a], np.float32), np.asarray([b], np.float32), self.eps,

@@ -20,3 +20,19 @@ def ctx():
     from klampt_amd import engine
     c = engine.Context(0)
     yield c
+
+
+class engine_params:
+    """Context manager: set some engine parameters of a context for a block
+    and restore the previous values afterwards."""
+
+    def __init__(self, ctx, **changes):
+        self.ctx, self.changes = ctx, changes
+
+    def __enter__(self):
+        self.old = self.ctx.set_params(**self.changes)
+        return self
+
+    def __exit__(self, *exc):
+        self.ctx.set_params(**self.old)
+        return False

@@ -13,6 +13,7 @@ import pytest
 
 import oracle
 from klampt_amd import engine, synthetic as S
+from tests.conftest import engine_params
 
 pytestmark = pytest.mark.gpu
 
@@ -233,10 +234,14 @@ def test_timing_levels(ctx, c1):
         ctx.set_timing(3)
 
 
-def test_fp64_pass_alone_matches_oracle(c1, c2, monkeypatch):
+def test_fp64_pass_alone_matches_oracle(ctx, c1, c2):
     """Every configuration sent through the exact FP64 pass (which forms its
     link frames from q): the oracle's answers in all three result modes."""
-    monkeypatch.setenv("KG_FORCE_RECHECK", "1")
+    with engine_params(ctx, force_recheck=1):
+        _fp64_pass_alone(c1, c2)
+
+
+def _fp64_pass_alone(c1, c2):
     for name, (sc, w, r, ow, orb), seed in (("C1", c1, 51), ("C2", c2, 52)):
         Q = S.uniform_configs(sc.robot, 1024, seed=seed)
         f_any = w.feasible_batch(Q)[0]
@@ -247,3 +252,34 @@ def test_fp64_pass_alone_matches_oracle(c1, c2, monkeypatch):
         _check_exact(ow, orb, Q, f_any, f2, name + " any")
         _check_exact(ow, orb, Q, np.c_[f_grp, e_grp, s_grp], np.c_[f2, e2, s2], name + " groups")
         _check_exact(ow, orb, Q, np.c_[feas, env, slf, fp], np.c_[f2, e2, s2, fp2], name + " ordered")
+
+
+def test_double_configuration_on_a_double_limit(ctx):
+    """A float64 configuration exactly on a float64 joint limit is inside it
+    (closed interval, RobotCSpace.cpp:610-630): the f64 entry point checks
+    the doubles as given, where a float32 cast would round pi/2 up past the
+    limit.  The planner layer uses it, so a start on a limit is accepted."""
+    from klampt_amd import plan, robotsim
+    spec = S.arm7()
+    spec.qmin = np.full(7, -math.pi / 2)
+    spec.qmax = np.full(7, math.pi / 2)
+    sc = S.Scene(spec)
+    w, r = engine.world_from_scene(ctx, sc)
+    ow, orb = oracle.world_from_scene(sc)
+    q = np.zeros((2, 7))
+    q[0, 3] = math.pi / 2
+    q[1, 1] = -math.pi / 2
+    assert float(np.float32(math.pi / 2)) > math.pi / 2
+    f64 = w.feasible_batch(q)[0]
+    assert np.array_equal(f64, ow.feasible_batch(orb, q)[0]) and f64.all()
+    assert not w.feasible_batch(q.astype(np.float32))[0][0]     # the float32 cast leaves the limit
+    import torch
+    assert w.feasible_batch_dev(torch.from_numpy(q).cuda()).cpu().numpy().all()
+    W = robotsim.WorldModel()
+    R = robotsim.RobotModel.from_spec(spec)
+    W.addRobot(R)
+    space = plan.RobotCSpace(R, W, eps=0.05)
+    assert space.feasible(list(q[0])) and space.feasible(list(q[1]))
+    mp = plan.MotionPlan(space, type="prm", knn=5)
+    mp.setEndpoints(list(q[0]), list(q[1]))
+    assert space.visible_batch(q[:1], q[1:])[0] in (True, False)

@@ -198,3 +198,34 @@ def test_rigid_object_set_transform_moves_it(tmp_path):
     W2.addRobot(rob2)
     free = _feas(robotsim.RobotCSpace(rob2, W2).feasible_batch(Q))
     assert np.array_equal(_feas(space.feasible_batch(Q)), free) and free.any()
+
+
+def test_loadfile_into_attached_geometry_updates_the_engine(tmp_path):
+    """Loading a file into a link's or rigid object's Geometry3D replaces the
+    geometry the world checks (robotsim.cpp:1443-1453), with the object's
+    placement kept."""
+    from klampt_amd import io as kio, robotsim
+    spec = S.arm7()
+    spec.link_meshes = [None] * 7
+    W = robotsim.WorldModel()
+    R = robotsim.RobotModel.from_spec(spec)
+    W.addRobot(R)
+    q = [0.0] * 7
+    R.setConfig(q)
+    Rl, tl = R.link(2).getTransform()
+    # an object box centred on link 2's frame origin (placed by its transform)
+    big = S.box_mesh([0.08, 0.08, 0.08])
+    W.addRigidObject(robotsim.Geometry3D(robotsim.TriangleMesh(big.verts, big.tris)),
+                     [1, 0, 0, 0, 1, 0, 0, 0, 1], list(np.asarray(tl) + [0.05, 0.0, 0.0]))
+    space = robotsim.RobotCSpace(R, W)
+    assert space.isFeasible(q)                       # the links have no geometry yet
+    link_mesh = S.box_mesh([0.06, 0.06, 0.06])
+    kio.save_mesh(str(tmp_path / "link.off"), link_mesh)
+    assert R.link(2).geometry().loadFile(str(tmp_path / "link.off"))
+    assert not space.isFeasible(q)                   # the loaded link mesh crosses the object
+    # the object's geometry replaced by a small box: placement kept, no contact
+    kio.save_mesh(str(tmp_path / "tiny.off"), S.box_mesh([0.005, 0.005, 0.005], c=[0.3, 0.3, 0.3]))
+    assert W.rigidObject(0).geometry().loadFile(str(tmp_path / "tiny.off"))
+    assert space.isFeasible(q)
+    Ro, to = W.rigidObject(0).getTransform()
+    assert np.allclose(to, np.asarray(tl) + [0.05, 0.0, 0.0])

@@ -6,6 +6,7 @@ import pytest
 
 import oracle
 from klampt_amd import engine, synthetic as S
+from tests.conftest import engine_params
 
 pytestmark = pytest.mark.gpu
 
@@ -50,12 +51,12 @@ def test_autotune_keeps_results(hetero):
     engine.set_tuning(worlds[0].ctx)  # back to defaults
 
 
-def test_multi_fp64_pass_matches_oracle(hetero, monkeypatch):
+def test_multi_fp64_pass_matches_oracle(ctx, hetero):
     """Heterogeneous batch with every configuration re-checked in FP64 (frames
     formed from q through the per-configuration Q offsets)."""
-    monkeypatch.setenv("KG_FORCE_RECHECK", "1")
     soup, robots, counts, worlds, Qs = hetero
-    multi = engine.feasible_multi(worlds, [0] * len(worlds), counts, Qs)
+    with engine_params(ctx, force_recheck=1):
+        multi = engine.feasible_multi(worlds, [0] * len(worlds), counts, Qs)
     for b, (spec, Q) in enumerate(zip(robots, Qs)):
         ow, orb = oracle.world_from_scene(S.Scene(spec, terrains=[soup]))
         assert np.array_equal(multi[b], ow.feasible_batch(orb, Q)[0]), f"robot {b} ({spec.n} DOF)"
