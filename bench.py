@@ -32,6 +32,7 @@ sys.path.insert(0, HERE)
 BYTES_PER_ENV_NODE = 64
 BYTES_PER_ENV_TRI = 36
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
 TRAFFIC_FILE = os.path.join(HERE, "profiles", "k_collide_f32_soup200k_65536.traffic.json")
 
 
@@ -43,6 +44,13 @@ def copy_probe():
         y.copy_(x)
     torch.cuda.synchronize()
     print("copied 256 MiB x5", flush=True)
+
+
+def engine_trees(world, robot):
+    """The engine's OBB trees of a world's meshes, in the order the oracle
+    numbers them (terrains, rigid objects, then the robot's link meshes)."""
+    geoms = list(world._keep) + [g for g in robot.link_geoms if g is not None]
+    return [g.bvh() for g in geoms]
 
 
 def measured_traffic():
@@ -123,17 +131,21 @@ def main():
           file=sys.stderr, flush=True)
 
     nj = sc.robot.n
-    alg_bytes = (args.configs * (4 * nj + 2) + st["box_tests"] * BYTES_PER_ENV_NODE
+    # the kernel's own work (its box / triangle-test counters, every box test
+    # charged as one environment node)
+    gpu_bytes = (args.configs * (4 * nj + 2) + st["box_tests"] * BYTES_PER_ENV_NODE
                  + st["tri_tests"] * BYTES_PER_ENV_TRI)
-    achieved = alg_bytes / kernel_s / 1e9 if kernel_s > 0 else 0.0
     traffic, traffic_src = measured_traffic()
-    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-                "traffic_source": traffic_src,
+    roofline = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
+                "traffic": traffic, "traffic_source": traffic_src,
                 "kernel": "k_collide<float, false, 256, false, true>", "kernel_ms": round(kernel_s * 1e3, 4),
-                "kernel_launches_timed": kcnt, "algorithmic_bytes": int(alg_bytes),
-                "box_tests": int(st["box_tests"]), "tri_tests": int(st["tri_tests"]),
-                "pair_items": int(st["pair_items"]), "fp64_rechecks": int(st["fp64_rechecks"])}
+                "kernel_launches_timed": kcnt,
+                "gpu_counters": {"box_tests": int(st["box_tests"]), "tri_tests": int(st["tri_tests"]),
+                                 "pair_items": int(st["pair_items"]), "fp64_rechecks": int(st["fp64_rechecks"]),
+                                 "bytes": int(gpu_bytes),
+                                 "frac": round(gpu_bytes / kernel_s / 1e9 / HBM_PEAK_GBS, 5) if kernel_s else None}}
+    if traffic and kernel_s:
+        roofline["traffic_frac"] = round(traffic / kernel_s / 1e9 / HBM_PEAK_GBS, 5)
 
     cpu = None
     if not args.no_cpu:
@@ -167,6 +179,26 @@ def main():
                "results_equal": agree}
         print(f"[bench] cpu {cpu_rate:.4g} checks/s on {nthr} threads, {rate1:.4g} on one; equal={agree}",
               file=sys.stderr, flush=True)
+        # algorithmic bytes and flops: the oracle's sequential IsFeasible (early
+        # returns, first hit) on the engine's own OBB trees, counted per check
+        for g, tree in zip(ow.meshes, engine_trees(world, robot)):
+            ow.set_obb_tree(g, tree)
+        f_cnt, cnt = ow.feasible_counted(orb, Q64, nthreads=nthr)
+        tot = cnt.sum(axis=0)
+        alg_bytes = args.configs * (4 * nj + 2) + int(tot[1]) * BYTES_PER_ENV_NODE + int(tot[2]) * BYTES_PER_ENV_TRI
+        checked = int((cnt.sum(axis=1) > 0).sum())
+        flops = 600 * args.configs + 200 * int(tot[0] + tot[3]) + 150 * int(tot[2] + tot[4])
+        achieved = alg_bytes / kernel_s / 1e9 if kernel_s > 0 else 0.0
+        roofline.update({"achieved": round(achieved, 2), "frac": round(achieved / HBM_PEAK_GBS, 5),
+                         "algorithmic_bytes": int(alg_bytes),
+                         "oracle_counters": {"env_box_tests": int(tot[0]), "env_node_visits": int(tot[1]),
+                                             "env_tri_tests": int(tot[2]), "self_box_tests": int(tot[3]),
+                                             "self_tri_tests": int(tot[4]), "checks_traversed": checked,
+                                             "answers_equal": bool(np.array_equal(f_cnt, f_cpu))},
+                         "fp32_flops": int(flops),
+                         "fp32_flop_frac": round(flops / kernel_s / 1e12 / FP32_PEAK_TFLOPS, 5) if kernel_s else None})
+        print(f"[bench] oracle counters {roofline['oracle_counters']}: {alg_bytes / 1e6:.1f} MB algorithmic, "
+              f"{achieved:.1f} GB/s", file=sys.stderr, flush=True)
 
     secondary = None if args.no_secondary else run_secondary(args, ctx, sc, world, robot)
 
@@ -251,22 +283,31 @@ def run_secondary(args, ctx, sc, world, robot):
                                                      "on the CPU cost of this query)")
     print(f"[bench] C8 distance Jacobians: {out['c8_distance_jacobian']}", file=sys.stderr, flush=True)
 
-    # C4 edges: 8192 segments, L-infinity step 0.01 rad, 10..500 samples
+    # C4 edges: 8192 segments, L-infinity step 0.01 rad, 10..500 samples;
+    # device rounds on endpoints resident in HBM (kg_edge_batch_dev), and the
+    # host-pointer call (copies included) beside it
     A, B = S.edge_segments(sc.robot, args.edges, eps=0.01, smin=10, smax=500, seed=8)
-    t0 = time.perf_counter()
-    fu = world.edge_batch(A, B, eps=0.01, norm_inf=True)
+    dA, dB = torch.from_numpy(A).to(dev), torch.from_numpy(B).to(dev)
+    dfu = torch.empty(args.edges, dtype=torch.float32, device=dev)
+    te = _time_dev(lambda: world.edge_batch_dev(dA, dB, eps=0.01, norm_inf=True, out=dfu), 5)
+    est = ctx.status_dev().cpu().numpy()
+    fu = dfu.cpu().numpy()
     t1 = time.perf_counter()
-    fu = world.edge_batch(A, B, eps=0.01, norm_inf=True)
-    te = time.perf_counter() - t1
-    out["c4_edges"] = {"segments": args.edges, "edges_per_s": round(args.edges / te, 1), "ms": round(te * 1e3, 2),
-                       "free_fraction": round(float((fu < 0).mean()), 4),
-                       "note": "host-pointer call including H2D/D2H copies and per-round host sync"}
+    fh = world.edge_batch(A, B, eps=0.01, norm_inf=True)
+    th = time.perf_counter() - t1
+    out["c4_edges"] = {"segments": args.edges, "edges_per_s": round(args.edges / te, 1), "ms": round(te * 1e3, 3),
+                       "free_fraction": round(float((fu < 0).mean()), 4), "undecided": int(est[1]),
+                       "host_call_ms": round(th * 1e3, 3), "host_equal": bool(np.array_equal(fh, fu)),
+                       "note": "kg_edge_batch_dev: device rounds, no host synchronization; host_call_ms is the "
+                               "host-pointer call with its copies"}
     if not args.no_cpu:
         t1 = time.perf_counter()
-        ow.edge_batch(orb, A[:1024], B[:1024], eps=0.01, norm_inf=True)
+        fo, _, nchk = ow.edge_batch(orb, A, B, eps=0.01, norm_inf=True)
         tc = time.perf_counter() - t1
-        out["c4_edges"]["cpu_edges_per_s"] = round(1024 / tc, 1)
-        out["c4_edges"]["cpu_sample"] = f"first 1024 segments, FP64 oracle, {oracle.nthreads_default()} threads"
+        out["c4_edges"]["cpu_edges_per_s"] = round(len(A) / tc, 1)
+        out["c4_edges"]["cpu_samples_checked"] = int(nchk.sum())
+        out["c4_edges"]["cpu_sample"] = f"all {len(A)} segments, FP64 oracle, {oracle.nthreads_default()} threads"
+        out["c4_edges"]["results_equal"] = bool(np.array_equal(fu, fo.astype(np.float32)))
     print(f"[bench] C4 edges: {out['c4_edges']}", file=sys.stderr, flush=True)
 
     # C5 SDF hinge penalty, forward + backward

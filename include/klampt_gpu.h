@@ -88,6 +88,8 @@ typedef struct {
   int32_t prep_group;    /* 1: lane-per-link broad phase where it applies, 0: one lane per
                             configuration (1) */
   int32_t force_recheck; /* test hook: every configuration goes through the FP64 pass (0) */
+  int32_t edge_budget_mb; /* device workspace of one edge-check round, MiB, 16..2^20 (1024):
+                             it bounds the samples checked per round */
 } kg_params;
 int kg_ctx_get_params(kg_ctx ctx, kg_params* out);
 int kg_ctx_set_params(kg_ctx ctx, const kg_params* params);
@@ -112,6 +114,15 @@ int kg_ctx_status_dev(kg_ctx ctx, int32_t* d_out, void* stream);
  * doubles, tris: nt x 3 int32. */
 int kg_geom_create_trimesh(kg_ctx ctx, const double* verts, int nverts, const int32_t* tris, int ntris,
                            kg_geom* out);
+/* The mesh's collision data: its OBB tree (built once at creation, as
+ * AnyCollisionGeometry3D::InitCollisionData builds PQP's), nnodes x 16 floats
+ * per node: centre (3), half extents (3), axes (9, column-major), then the
+ * child as int32 bits -- >= 0: the first of the two adjacent children (node
+ * index), < 0: a leaf holding triangle ~child (the mesh's own numbering).
+ * Node 0 is the root.  nodes may be NULL to ask for the count only.  Host
+ * memory only (no GPU work).  Klampt does not expose PQP's tree; this is for
+ * measurement and tests. */
+int kg_geom_get_bvh(kg_geom g, float* nodes, int max_nodes, int* nnodes);
 /* ImplicitSurface (Python/klampt/src/geometry.h:322-391): cell-centred values,
  * C-order (i,j,k), dims[0] x dims[1] x dims[2], > 0 outside. */
 int kg_geom_create_sdf(kg_ctx ctx, const double bmin[3], const double bmax[3], const int32_t dims[3],
@@ -296,9 +307,30 @@ int kg_distance_batch_dev(kg_world w, int robot_index, const float* dQ, int N, f
  * interpolation Cpp/Modeling/Interpolate.cpp:13-14; d the L2 norm
  * (norm_inf = 0, Klampt's default) or the L-infinity norm).  Interior samples
  * i = 1..n-1 are checked in level-order bisection; first_u[s] is u of the
- * first infeasible sample in that order, -1 if all are feasible. */
+ * first infeasible sample in that order, -1 if all are feasible.  A[nseg][n]
+ * and B[nseg][n] are the segment endpoints (FP32, or FP64 for the _f64
+ * variants, which interpolate Klampt's double Config exactly).
+ * The checks run as rounds on the device with no host synchronization
+ * between them (kg_params.edge_w0, edge_budget_mb).  Host-pointer calls
+ * cover every segment.  The _dev calls cover segments of at most max_steps
+ * steps n (0: the steps of the joint-range diagonal, which bounds every
+ * segment between configurations within the joint limits; KG_E_INVALID if a
+ * joint range is unbounded); a longer segment's first_u is -2 and it is
+ * counted in kg_ctx_status_dev [1].  d_nsamples (nullable) receives n per
+ * segment.  The _dev calls allocate nothing once a call of the same robot
+ * with as many segments has run, so they can be captured in a HIP graph. */
 int kg_edge_batch(kg_world w, int robot_index, const float* A, const float* B, int nseg, double eps, int norm_inf,
                   float* first_u);
+int kg_edge_batch_f64(kg_world w, int robot_index, const double* A, const double* B, int nseg, double eps,
+                      int norm_inf, float* first_u);
+int kg_edge_batch_dev(kg_world w, int robot_index, const float* dA, const float* dB, int nseg, double eps,
+                      int norm_inf, int max_steps, float* d_first_u, int32_t* d_nsamples, void* stream);
+int kg_edge_batch_f64_dev(kg_world w, int robot_index, const double* dA, const double* dB, int nseg, double eps,
+                          int norm_inf, int max_steps, float* d_first_u, int32_t* d_nsamples, void* stream);
+/* The k-th interior sample index (0 <= k < nsteps-1) in the level-order
+ * bisection order of a segment of nsteps steps; -1 if out of range.  Host
+ * only (no GPU needed). */
+int kg_edge_sample_index(int nsteps, int k);
 
 /* SDF hinge penalty L(q) = sum_{links k} sum_{p in pts_k} max(0, dsafe - sdf(T_k(q) p))^2
  * and its gradient by an FK-Jacobian adjoint (the distance-gradient rule of

@@ -35,7 +35,7 @@ class Params(ctypes.Structure):
     """kg_params: engine parameters of a context (include/klampt_gpu.h)."""
     _fields_ = [(n, ctypes.c_int32) for n in (
         "cpw", "leaf_min", "scap", "waves_per_cu", "static_pct", "nsub", "tail_lanes", "tail_reps", "spill_cap",
-        "dist_ipw", "edge_w0", "sdf_lanes", "prep_group", "force_recheck")]
+        "dist_ipw", "edge_w0", "sdf_lanes", "prep_group", "force_recheck", "edge_budget_mb")]
 
 
 # name -> argtypes (all return int unless listed in _RESTYPE)
@@ -52,6 +52,7 @@ SIGNATURES = {
     "kg_geom_create_sdf": [_vp, _pd, _pd, _pi, _pf, _pvp],
     "kg_geom_destroy": [_vp],
     "kg_geom_clone": [_vp, _pvp],
+    "kg_geom_get_bvh": [_vp, _pf, _c_int, _pi],
     "kg_distance_jacobian_batch": [_vp, _c_int, _pf, _c_int, _c_float, _pf, _pf, _pf],
     "kg_distance_jacobian_batch_dev": [_vp, _c_int, _vp, _c_int, _c_float, _vp, _vp, _vp, _vp],
     "kg_geom_create_pointcloud": [_vp, _pd, _c_int, _pvp],
@@ -100,6 +101,10 @@ SIGNATURES = {
     "kg_distance_batch": [_vp, _c_int, _pf, _c_int, _pf],
     "kg_distance_batch_dev": [_vp, _c_int, _vp, _c_int, _vp, _vp],
     "kg_edge_batch": [_vp, _c_int, _pf, _pf, _c_int, _c_double, _c_int, _pf],
+    "kg_edge_batch_f64": [_vp, _c_int, _pd, _pd, _c_int, _c_double, _c_int, _pf],
+    "kg_edge_batch_dev": [_vp, _c_int, _vp, _vp, _c_int, _c_double, _c_int, _c_int, _vp, _vp, _vp],
+    "kg_edge_batch_f64_dev": [_vp, _c_int, _vp, _vp, _c_int, _c_double, _c_int, _c_int, _vp, _vp, _vp],
+    "kg_edge_sample_index": [_c_int, _c_int],
     "kg_sdf_penalty_fwd": [_vp, _vp, _pf, _c_int, _pf, _pi, _c_float, _pf],
     "kg_sdf_penalty_bwd": [_vp, _vp, _pf, _c_int, _pf, _pi, _c_float, _pf, _pf],
     "kg_sdf_penalty_fwd_dev": [_vp, _vp, _vp, _c_int, _vp, _pi, _c_float, _vp, _vp],

@@ -15,6 +15,7 @@
 
 #include "../../include/klampt_gpu.h"
 #include "kg_internal.h"
+#include "kg_edge.h"
 #include "kg_geom_math.h"
 
 hipError_t kgl_fill_int(int* p, int n, int v, hipStream_t s);
@@ -466,8 +467,11 @@ struct Layout {
   Tol tol32{}, tol64{};
 };
 
+// N_dev (nullable): device-side configuration count, at most N (the capacity
+// the workspaces are sized for); timed = false records no timing events.
 void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* Qd, int N, uint8_t* feas,
-                   uint8_t* env, uint8_t* self, int32_t* fp, hipStream_t s) {
+                   uint8_t* env, uint8_t* self, int32_t* fp, hipStream_t s, const int* N_dev = nullptr,
+                   bool timed = true) {
   hip_check(hipSetDevice(c->device), "hipSetDevice");
   ensure_status_host(c);
   c->last_N = N;
@@ -477,8 +481,20 @@ void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* 
     return;
   }
   const int mode = fp ? KG_MODE_ORDERED : ((env || self) ? KG_MODE_GROUPS : KG_MODE_ANY);
-  hip_check(c->wsT64.ensure(L.nrows * 12 * sizeof(double)), "hipMalloc(T64)");
-  hip_check(c->wsLB.ensure(L.nrows * 6 * sizeof(double)), "hipMalloc(LB)");
+  const bool fixed = L.ncap * sizeof(PairItem) <= (2048ull << 20);
+  need(fixed || !N_dev, "batch with a device-side count exceeds the item capacity limit");
+  // robots of at most 32 links get the lane-per-link broad phase, which
+  // also writes the item start records; otherwise the traversal forms each
+  // item's frame when it starts it
+  const int ml = L.multi ? L.maxlinks : L.P->dev.nlinks;
+  int group = ml <= 8 ? 8 : (ml <= 16 ? 16 : (ml <= 32 ? 32 : 0));
+  if (!c->prm.prep_group || !(fixed || L.multi)) group = 0;
+  // frames are stored only for the serial broad phase (the per-link one
+  // writes start records, and the FP64 pass forms its frames from q); link
+  // boxes only for the count / scan path
+  const bool lds_links = !group && !L.multi && ml <= 8 && (fixed || L.multi);
+  if (!group) hip_check(c->wsT64.ensure(L.nrows * 12 * sizeof(double)), "hipMalloc(T64)");
+  if (!group && !lds_links) hip_check(c->wsLB.ensure(L.nrows * 6 * sizeof(double)), "hipMalloc(LB)");
   hip_check(c->wsJL.ensure((size_t)N), "hipMalloc(jl)");
   hip_check(c->wsCounts.ensure((size_t)(N + 1) * sizeof(int)), "hipMalloc(counts)");
   hip_check(c->wsOffsets.ensure((size_t)(N + 1) * sizeof(int)), "hipMalloc(offsets)");
@@ -505,10 +521,10 @@ void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* 
   pa.qoff = L.qoff;
   pa.toff = L.toff;
   pa.ibase = L.ibase;
-  const bool fixed = L.ncap * sizeof(PairItem) <= (2048ull << 20);
+  pa.N_dev = N_dev;
   // timing events cost a few microseconds each on the stream: per-phase ones
   // only at timing level 2, the traversal kernel's own at level 1 and up
-  const bool phase_ev = c->timing >= 2, kern_ev = c->timing >= 1;
+  const bool phase_ev = timed && c->timing >= 2, kern_ev = timed && c->timing >= 1;
   c->last_phase_ev = phase_ev;
   c->last_tr_slot = -1;
   if (phase_ev) hip_check(hipEventRecord(c->ev[0], s), "hipEventRecord");
@@ -518,13 +534,8 @@ void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* 
     pa.items = c->wsItems.as<PairItem>();
     pa.cap = L.cap;
     pa.offsets = nullptr;
-    // robots of at most 32 links get the lane-per-link broad phase, which
-    // also writes the item start records; otherwise the traversal forms each
-    // item's frame when it starts it
-    const int ml = L.multi ? L.maxlinks : L.P->dev.nlinks;
-    pa.group = ml <= 8 ? 8 : (ml <= 16 ? 16 : (ml <= 32 ? 32 : 0));
-    if (!c->prm.prep_group) pa.group = 0;
-    pa.lds_links = !pa.group && !L.multi && ml <= 8;
+    pa.group = group;
+    pa.lds_links = lds_links;
     if (pa.group) {
       hip_check(c->wsRec.ensure(std::max<size_t>(1, L.ncap) * sizeof(ItemRec)), "hipMalloc(records)");
       pa.recs = c->wsRec.as<ItemRec>();
@@ -572,7 +583,7 @@ void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* 
   ta.cap = pa.cap;
   ta.cfg_list = nullptr;
   ta.ncfg = N;
-  ta.ncfg_dev = nullptr;
+  ta.ncfg_dev = N_dev;
   ta.mode = mode;
   ta.tol = L.tol32;
   ta.chunk_counter = status + ST_DYN;
@@ -781,8 +792,9 @@ int kg_ctx_destroy(kg_ctx c) {
                       &c->wsScanTmp, &c->wsBest, &c->wsUnc, &c->wsChunk, &c->wsSpill, &c->wsStatus, &c->wsRedo,
                       &c->wsOut8a, &c->wsOut8b, &c->wsOut8c, &c->wsOut32, &c->wsF, &c->wsG,
                       &c->wsPoff, &c->wsPts, &c->wsLoss, &c->wsDQ, &c->wsDl, &c->wsEA, &c->wsEB,
-                      &c->wsESeg, &c->wsESoff, &c->wsENs, &c->wsEOoff, &c->wsEOrder, &c->wsEQ, &c->wsEFeas,
-                      &c->wsEU, &c->wsEDone, &c->wsRec, &c->mPlans, &c->mCfgPlan, &c->mQoff, &c->mToff, &c->mIbase};
+                      &c->wsESeg, &c->wsESoff, &c->wsENs, &c->wsECur, &c->wsECtl, &c->wsEStat, &c->wsEQ,
+                      &c->wsEFeas, &c->wsEU, &c->wsEDone, &c->wsRec, &c->wsMsdf, &c->mPlans, &c->mCfgPlan,
+                      &c->mQoff, &c->mToff, &c->mIbase};
     for (auto* b : bufs) b->release();
     if (c->hstatus) (void)hipHostFree(c->hstatus);
     for (auto& e : c->ev)
@@ -869,6 +881,30 @@ int kg_geom_create_trimesh(kg_ctx c, const double* V, int nv, const int32_t* T, 
     c->geoms.push_back(d);
     c->pool_dirty = true;
     *out = g;
+  });
+}
+
+int kg_geom_get_bvh(kg_geom g, float* nodes, int max_nodes, int* nnodes) {
+  return guard([&] {
+    need(g && nnodes, "NULL argument");
+    need(g->kind == 0, "not a triangle mesh");
+    kg_ctx_s* c = g->ctx;
+    const GeomDesc& d = c->geoms[g->index];
+    const int nn = d.root < 0 ? 0 : d.nnodes;
+    *nnodes = nn;
+    if (!nodes) return;
+    need(max_nodes >= nn, "node buffer too small");
+    for (int i = 0; i < nn; i++) {
+      const ObbNode& o = c->nodes[(size_t)d.root + i];
+      float* p = nodes + 16 * (size_t)i;
+      for (int k = 0; k < 3; k++) {
+        p[k] = o.c[k];
+        p[3 + k] = o.e[k];
+      }
+      for (int k = 0; k < 9; k++) p[6 + k] = o.R[k];
+      const int child = o.child >= 0 ? o.child - d.root : ~c->trisrc[~o.child];
+      std::memcpy(p + 15, &child, sizeof(int));
+    }
   });
 }
 
@@ -2309,124 +2345,223 @@ extern "C" int kg_geom_within_distance(kg_geom a, kg_geom b, double tol, int* ou
 // ---------------------------------------------------------------- edge checks
 namespace {
 
-// Level-order bisection over the interior sample indices 1..n-1 of [0,n].
-void bisection_order(int n, std::vector<int>& order) {
-  order.clear();
-  if (n < 2) return;
-  std::vector<std::pair<int, int>> q;
-  q.reserve(n);
-  q.push_back({0, n});
-  for (size_t h = 0; h < q.size(); h++) {
-    const int lo = q[h].first, hi = q[h].second;
-    if (hi - lo < 2) continue;
-    const int mid = lo + (hi - lo) / 2;
-    order.push_back(mid);
-    q.push_back({lo, mid});
-    q.push_back({mid, hi});
+// Straight-line edge checks on device buffers (SingleRobotCSpace::PathChecker
+// -> EpsilonEdgeChecker, Cpp/Planning/RobotCSpace.cpp:835-837).  Rounds of
+// plan / samples / feasibility pipeline / reduce run on the device with no
+// host synchronization; the host only launches a schedule fixed in advance:
+// round r is w0 2^r positions wide (kg_params.edge_w0), at most the sample
+// budget over the live segments, and the schedule has enough rounds to cover
+// max_pos positions of every segment even at the narrowest width the budget
+// allows.  Segments needing more positions than max_pos come out -2
+// (undecided) and are counted in the status words.  A batch of more
+// segments than the budget's samples runs as consecutive chunks.
+void edge_core(kg_world_s* w, int ri, const float* Af, const float* Bf, const double* Ad, const double* Bd, int nseg,
+               double eps, int norm_inf, int max_pos, float* first_u, int32_t* nsamples, hipStream_t s) {
+  kg_ctx_s* c = w->ctx;
+  need(ri >= 0 && ri < (int)w->robots.size(), "robot index out of range");
+  need(nseg >= 0, "negative segment count");
+  need(eps > 0 && std::isfinite(eps), "eps must be positive");  // motionplanning.cpp:727-728
+  need(max_pos >= 0, "negative position bound");
+  if (nseg == 0) return;
+  reject_margins(w);
+  ensure_status_host(c);
+  RobotPlan& P = get_plan(w, ri);
+  const int n = P.dev.nlinks;
+  const kg_params& prm = c->prm;
+  // feasibility workspace bytes per sample (fixed-capacity item layout)
+  const int cap = active_links(w->robots[ri]) * P.nenv + P.nself;
+  const bool group = prm.prep_group && n <= 32;
+  size_t per = (size_t)n * sizeof(double) + 1 + 1 + 3 * sizeof(int) + (size_t)cap * sizeof(PairItem) +
+               (group ? (size_t)cap * sizeof(ItemRec) : (size_t)n * 18 * sizeof(double));
+  const size_t budget_bytes = (size_t)prm.edge_budget_mb << 20;
+  int B = (int)std::max<size_t>(1, std::min<size_t>(budget_bytes / per, (size_t)1 << 24));
+  // the item layout of one round must stay under the traversal's 2 GiB limit
+  B = (int)std::min<size_t>((size_t)B, std::max<size_t>(1, ((size_t)2048 << 20) / std::max<size_t>(1, (size_t)cap * sizeof(PairItem))));
+  const int chunk = std::min(nseg, B);
+
+  hip_check(c->wsENs.ensure((size_t)nseg * sizeof(int)), "hipMalloc");
+  hip_check(c->wsECur.ensure((size_t)nseg * sizeof(int)), "hipMalloc");
+  hip_check(c->wsEDone.ensure((size_t)nseg), "hipMalloc");
+  hip_check(c->wsESeg.ensure((size_t)chunk * sizeof(int)), "hipMalloc");
+  hip_check(c->wsESoff.ensure((size_t)(chunk + 1) * sizeof(int)), "hipMalloc");
+  hip_check(c->wsECtl.ensure(8 * sizeof(int)), "hipMalloc");
+  hip_check(c->wsEQ.ensure((size_t)B * n * sizeof(double)), "hipMalloc");
+  hip_check(c->wsEFeas.ensure((size_t)B), "hipMalloc");
+  hip_check(c->wsEStat.ensure(ST_WORDS * sizeof(int)), "hipMalloc");
+  hip_check(c->wsStatus.ensure(ST_WORDS * sizeof(int)), "hipMalloc(status)");
+
+  Layout L;
+  L.P = &P;
+  L.cap = cap;
+  L.maxcap = cap;
+  L.nrows = (size_t)B * n;
+  L.ncap = (size_t)cap * B;
+  L.tol32 = P.tol32;
+  L.tol64 = P.tol64;
+
+  EdgeArgs ea{};
+  ea.n = n;
+  ea.eps = eps;
+  ea.norm_inf = norm_inf ? 1 : 0;
+  ea.max_pos = max_pos;
+  ea.budget = B;
+  ea.seg = c->wsESeg.as<int>();
+  ea.soff = c->wsESoff.as<int>();
+  ea.ctl = c->wsECtl.as<int>();
+  ea.Qd = c->wsEQ.as<double>();
+  ea.feas = c->wsEFeas.as<uint8_t>();
+  ea.st = c->wsStatus.as<int>();
+  ea.w_spill = ST_SPILL;
+  ea.w_unc = ST_UNC;
+  ea.w_redo = ST_REDO;
+  ea.out_status = c->wsEStat.as<int>();
+  // the control words accumulate over the chunks: cleared once, by the first
+  // chunk's init kernel
+  for (int s0 = 0; s0 < nseg; s0 += chunk) {
+    const int ns = std::min(chunk, nseg - s0);
+    ea.nseg = ns;
+    ea.Af = Af ? Af + (size_t)s0 * n : nullptr;
+    ea.Bf = Bf ? Bf + (size_t)s0 * n : nullptr;
+    ea.Ad = Ad ? Ad + (size_t)s0 * n : nullptr;
+    ea.Bd = Bd ? Bd + (size_t)s0 * n : nullptr;
+    ea.ns = c->wsENs.as<int>() + s0;
+    ea.cur = c->wsECur.as<int>() + s0;
+    ea.done = c->wsEDone.as<uint8_t>() + s0;
+    ea.first_u = first_u + s0;
+    ea.nsamples_out = nsamples ? nsamples + s0 : nullptr;
+    ea.ctl = c->wsECtl.as<int>();
+    hip_check(kgl_edge_init(ea, s0 == 0, s), "edge_init");
+    // schedule: widths w0 2^r; each round gives every live segment at least
+    // min(width, B / ns) positions
+    const int floor_w = std::max(1, B / ns);
+    int64_t covered = 0;
+    for (int r = 0; covered < max_pos; r++) {
+      const int width = (int)std::min<int64_t>((int64_t)prm.edge_w0 << std::min(r, 20), 1 << 20);
+      covered += std::min(width, floor_w);
+      hip_check(kgl_edge_round(ea, width, 0, s), "edge_plan");
+      feasible_core(c, L, nullptr, ea.Qd, B, c->wsEFeas.as<uint8_t>(), nullptr, nullptr, nullptr, s,
+                    ea.ctl + 1, false);
+      hip_check(kgl_edge_round(ea, width, 1, s), "edge_reduce");
+    }
+    hip_check(kgl_edge_finish(ea, s0 + chunk >= nseg, s), "edge_finish");
   }
+  c->last_N = nseg;
+  c->has_batch = true;
+  c->last_tr_slot = -1;
+  c->last_phase_ev = false;
+  c->stat_src = c->wsEStat.as<int>();
+  hip_check(hipEventRecord(c->evStat, s), "hipEventRecord");
+}
+
+// Positions every segment of a batch may need: max_steps - 1, or from the
+// joint ranges when max_steps is 0 (no segment between two configurations
+// within the limits is longer than the range box's diagonal).
+int edge_max_pos(const kg_robot_s* r, double eps, int norm_inf, int max_steps) {
+  if (max_steps > 0) return max_steps - 1;
+  double acc = 0;
+  for (int k = 0; k < r->n; k++) {
+    const double d = r->qmax[k] - r->qmin[k];
+    if (norm_inf) acc = std::max(acc, std::fabs(d));
+    else acc += d * d;
+  }
+  const double D = norm_inf ? acc : std::sqrt(acc);
+  const double m = std::ceil(D / eps) + 1;
+  need(std::isfinite(m) && m < 1e9,
+       "unbounded joint range: give max_steps (the most steps a segment of the batch may need)");
+  return (int)m;
+}
+
+// Host-pointer edge checks: exact steps per segment on the host (the
+// schedule covers the longest), the device rounds, one copy back.
+void edge_host(kg_world_s* w, int ri, const float* Af, const float* Bf, const double* Ad, const double* Bd, int nseg,
+               double eps, int norm_inf, float* first_u) {
+  need(w != nullptr, "world is NULL");
+  need(ri >= 0 && ri < (int)w->robots.size(), "robot index out of range");
+  need(nseg >= 0, "negative segment count");
+  need(eps > 0 && std::isfinite(eps), "eps must be positive");  // motionplanning.cpp:727-728
+  if (nseg == 0) return;
+  need((Af || Ad) && (Bf || Bd) && first_u, "NULL buffer");
+  kg_ctx_s* c = w->ctx;
+  const int n = w->robots[ri]->n;
+  int maxm = 0;
+  for (int s = 0; s < nseg; s++) {
+#pragma clang fp contract(off)
+    double acc = 0;
+    for (int k = 0; k < n; k++) {
+      const size_t i = (size_t)s * n + k;
+      const double d = (Ad ? Ad[i] : (double)Af[i]) - (Bd ? Bd[i] : (double)Bf[i]);
+      if (norm_inf) acc = std::max(acc, std::fabs(d));
+      else acc += d * d;
+    }
+    const double m = std::ceil((norm_inf ? acc : std::sqrt(acc)) / eps);
+    need(m < 1e7, "segment needs more than 1e7 samples");  // also rejects NaN
+    maxm = std::max(maxm, (int)m);
+  }
+  hip_check(hipSetDevice(c->device), "hipSetDevice");
+  StreamScope scope(c, c->stream);
+  const size_t eb = Ad ? sizeof(double) : sizeof(float);
+  hip_check(c->wsEA.ensure((size_t)nseg * n * eb), "hipMalloc");
+  hip_check(c->wsEB.ensure((size_t)nseg * n * eb), "hipMalloc");
+  hip_check(c->wsEU.ensure((size_t)nseg * sizeof(float)), "hipMalloc");
+  hip_check(hipMemcpyAsync(c->wsEA.p, Ad ? (const void*)Ad : (const void*)Af, (size_t)nseg * n * eb,
+                           hipMemcpyHostToDevice, c->stream), "copy");
+  hip_check(hipMemcpyAsync(c->wsEB.p, Bd ? (const void*)Bd : (const void*)Bf, (size_t)nseg * n * eb,
+                           hipMemcpyHostToDevice, c->stream), "copy");
+  // one position of margin over the host's step counts
+  edge_core(w, ri, Ad ? nullptr : c->wsEA.as<float>(), Bd ? nullptr : c->wsEB.as<float>(),
+            Ad ? c->wsEA.as<double>() : nullptr, Ad ? c->wsEB.as<double>() : nullptr, nseg, eps, norm_inf, maxm,
+            c->wsEU.as<float>(), nullptr, c->stream);
+  hip_check(hipMemcpyAsync(first_u, c->wsEU.p, (size_t)nseg * sizeof(float), hipMemcpyDeviceToHost, c->stream),
+            "copy");
+  hip_check(hipStreamSynchronize(c->stream), "sync");
+  check_uncertain(c);
 }
 
 }  // namespace
 
 extern "C" int kg_edge_batch(kg_world w, int ri, const float* A, const float* B, int nseg, double eps, int norm_inf,
                              float* first_u) {
-  return guard([&] {
-    need(w != nullptr, "world is NULL");
-    need(ri >= 0 && ri < (int)w->robots.size(), "robot index out of range");
-    need(nseg >= 0, "negative segment count");
-    need(eps > 0 && std::isfinite(eps), "eps must be positive");  // motionplanning.cpp:727-728
-    if (nseg == 0) return;
-    need(A && B && first_u, "NULL buffer");
-    kg_ctx_s* c = w->ctx;
-    ensure_status_host(c);
-    hip_check(hipSetDevice(c->device), "hipSetDevice");
-    StreamScope scope(c, c->stream);
-    const int n = w->robots[ri]->n;
-    // sample counts n_s = ceil(d/eps) (robotcspace.py:216-228), d in L2 or Linf
-    std::vector<int> ns(nseg), ooff(nseg);
-    std::vector<int> order_all;
-    std::vector<int> tmp;
-    std::vector<std::pair<int, int>> cache;  // n -> offset
-    for (int s = 0; s < nseg; s++) {
-      // no FMA contraction: n_s must equal ceil(d/eps) computed with separate
-      // roundings, or a sample count (and every u) would change
-#pragma clang fp contract(off)
-      double acc = 0;
-      for (int k = 0; k < n; k++) {
-        const double d = (double)A[(size_t)s * n + k] - (double)B[(size_t)s * n + k];
-        if (norm_inf) acc = std::max(acc, std::fabs(d));
-        else acc += d * d;
-      }
-      const double dist = norm_inf ? acc : std::sqrt(acc);
-      const double m = std::ceil(dist / eps);
-      need(m < 1e7, "segment needs more than 1e7 samples");
-      ns[s] = (int)m;
-      int off = -1;
-      for (auto& p : cache)
-        if (p.first == ns[s]) off = p.second;
-      if (off < 0) {
-        bisection_order(ns[s], tmp);
-        off = (int)order_all.size();
-        order_all.insert(order_all.end(), tmp.begin(), tmp.end());
-        cache.push_back({ns[s], off});
-      }
-      ooff[s] = off;
-      first_u[s] = -1.0f;
-    }
-    hipStream_t st = c->stream;
-    auto up = [&](DevBuf& b, const void* src, size_t bytes) {
-      hip_check(b.ensure(std::max<size_t>(bytes, 16)), "hipMalloc");
-      if (bytes) hip_check(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, st), "copy");
-    };
-    up(c->wsEA, A, (size_t)nseg * n * sizeof(float));
-    up(c->wsEB, B, (size_t)nseg * n * sizeof(float));
-    up(c->wsENs, ns.data(), nseg * sizeof(int));
-    up(c->wsEOoff, ooff.data(), nseg * sizeof(int));
-    up(c->wsEOrder, order_all.data(), order_all.size() * sizeof(int));
-    std::vector<float> fu(nseg, -1.0f);
-    std::vector<uint8_t> done(nseg, 0);
-    up(c->wsEU, fu.data(), nseg * sizeof(float));
-    up(c->wsEDone, done.data(), nseg);
-    // rounds over bisection positions [pos0, pos0 + width): w0, w0, 2 w0,
-    // 4 w0, ... wide (KG_EDGE_W0, default 16).  The first infeasible sample in
-    // bisection order is the same for any widths; wider rounds mean fewer
-    // round trips and more samples checked past an early hit.
-    int pos0 = 0, width = c->prm.edge_w0;
-    std::vector<int> live, soff;
-    for (;;) {
-      live.clear();
-      soff.clear();
-      int total = 0;
-      for (int s = 0; s < nseg; s++) {
-        const int m = ns[s] - 1;  // interior samples
-        if (done[s] || pos0 >= m) continue;
-        live.push_back(s);
-        soff.push_back(total);
-        total += std::min(width, m - pos0);
-      }
-      if (live.empty()) break;
-      up(c->wsESeg, live.data(), live.size() * sizeof(int));
-      up(c->wsESoff, soff.data(), soff.size() * sizeof(int));
-      hip_check(c->wsEQ.ensure((size_t)total * n * sizeof(double)), "hipMalloc");
-      hip_check(c->wsEFeas.ensure((size_t)total), "hipMalloc");
-      hip_check(kgl_edge_samples(c->wsEA.as<float>(), c->wsEB.as<float>(), n, c->wsESeg.as<int>(), c->wsESoff.as<int>(),
-                                 (int)live.size(), c->wsENs.as<int>(), c->wsEOoff.as<int>(), c->wsEOrder.as<int>(), pos0,
-                                 total, c->wsEQ.as<double>(), st),
-                "edge_samples");
-      feasible_dev(w, ri, nullptr, c->wsEQ.as<double>(), total, c->wsEFeas.as<uint8_t>(), nullptr, nullptr, nullptr, st);
-      hip_check(kgl_edge_reduce(c->wsESeg.as<int>(), c->wsESoff.as<int>(), (int)live.size(), total,
-                                c->wsEFeas.as<uint8_t>(), c->wsENs.as<int>(), c->wsEOoff.as<int>(), c->wsEOrder.as<int>(),
-                                pos0, c->wsEU.as<float>(), c->wsEDone.as<uint8_t>(), st),
-                "edge_reduce");
-      hip_check(hipMemcpyAsync(done.data(), c->wsEDone.p, nseg, hipMemcpyDeviceToHost, st), "copy");
-      hip_check(hipStreamSynchronize(st), "sync");
-      feasible_sync_check(w);
-      pos0 += width;
-      width = std::min(width * 2, 1 << 20);
-    }
-    hip_check(hipMemcpyAsync(first_u, c->wsEU.p, nseg * sizeof(float), hipMemcpyDeviceToHost, st), "copy");
-    hip_check(hipStreamSynchronize(st), "sync");
-  });
+  return guard([&] { edge_host(w, ri, A, B, nullptr, nullptr, nseg, eps, norm_inf, first_u); });
+}
+
+extern "C" int kg_edge_batch_f64(kg_world w, int ri, const double* A, const double* B, int nseg, double eps,
+                                 int norm_inf, float* first_u) {
+  return guard([&] { edge_host(w, ri, nullptr, nullptr, A, B, nseg, eps, norm_inf, first_u); });
+}
+
+namespace {
+void edge_dev(kg_world w, int ri, const float* Af, const float* Bf, const double* Ad, const double* Bd, int nseg,
+              double eps, int norm_inf, int max_steps, float* first_u, int32_t* nsamples, void* stream) {
+  need(w != nullptr, "world is NULL");
+  need(ri >= 0 && ri < (int)w->robots.size(), "robot index out of range");
+  need(nseg >= 0, "negative segment count");
+  need(max_steps >= 0, "negative max_steps");
+  need(eps > 0 && std::isfinite(eps), "eps must be positive");
+  if (nseg == 0) return;
+  need((Af || Ad) && (Bf || Bd) && first_u, "NULL buffer");
+  hipStream_t s = (hipStream_t)stream;
+  StreamScope scope(w->ctx, s);
+  edge_core(w, ri, Af, Bf, Ad, Bd, nseg, eps, norm_inf, edge_max_pos(w->robots[ri], eps, norm_inf, max_steps),
+            first_u, nsamples, s);
+}
+}  // namespace
+
+extern "C" int kg_edge_batch_dev(kg_world w, int ri, const float* dA, const float* dB, int nseg, double eps,
+                                 int norm_inf, int max_steps, float* d_first_u, int32_t* d_nsamples, void* stream) {
+  return guard([&] { edge_dev(w, ri, dA, dB, nullptr, nullptr, nseg, eps, norm_inf, max_steps, d_first_u, d_nsamples,
+                              stream); });
+}
+
+extern "C" int kg_edge_batch_f64_dev(kg_world w, int ri, const double* dA, const double* dB, int nseg, double eps,
+                                     int norm_inf, int max_steps, float* d_first_u, int32_t* d_nsamples,
+                                     void* stream) {
+  return guard([&] { edge_dev(w, ri, nullptr, nullptr, dA, dB, nseg, eps, norm_inf, max_steps, d_first_u, d_nsamples,
+                              stream); });
+}
+
+extern "C" int kg_edge_sample_index(int nsteps, int k) {
+  if (nsteps < 2 || k < 0 || k >= nsteps - 1) return -1;
+  return kg_bisect_index(nsteps, k);
 }
 
 // ---------------------------------------------------------------- heterogeneous batches
@@ -2670,6 +2805,7 @@ extern "C" int kg_ctx_set_params(kg_ctx c, const kg_params* p) {
     need(p->sdf_lanes == 8 || p->sdf_lanes == 16 || p->sdf_lanes == 32 || p->sdf_lanes == 64,
          "sdf_lanes must be 8, 16, 32 or 64");
     need(in(p->prep_group, 0, 1) && in(p->force_recheck, 0, 1), "prep_group and force_recheck must be 0 or 1");
+    need(in(p->edge_budget_mb, 16, 1 << 20), "edge_budget_mb must be 16..2^20");
     c->prm = *p;
   });
 }

@@ -127,6 +127,8 @@ struct PrepArgs {
   int group;           // > 0: one lane per link in groups of this many lanes (8, 16, 32)
   ItemRec* recs;       // nullable: item start records (written by the per-link variant only)
   size_t t_rows;       // frame rows of T64 (structure of arrays, see t_row)
+  const int* N_dev;    // nullable: device-side configuration count (<= N, the capacity); the
+                       // kernels then loop over the configurations on a bounded grid
 };
 hipError_t kgl_prep_count(const PrepArgs& a, hipStream_t s);
 hipError_t kgl_scan(int* counts, int* offsets, int n, DevBuf& tmp, hipStream_t s);
@@ -222,11 +224,37 @@ int kgl_distance_grid(int num_cus);
 hipError_t kgl_fk64(const LinkDesc* links, int n, const float* Qf, const double* Qd, int N, double* T64, int* reset,
                     int nreset, hipStream_t s);
 
-hipError_t kgl_edge_samples(const float* A, const float* B, int n, const int* seg, const int* soff, int nlive,
-                            const int* nsamp, const int* ooff, const int* order, int pos0, int total, double* Qd,
-                            hipStream_t s);
-hipError_t kgl_edge_reduce(const int* seg, const int* soff, int nlive, int total, const uint8_t* feas, const int* nsamp,
-                           const int* ooff, const int* order, int pos0, float* first_u, uint8_t* done, hipStream_t s);
+// Device-side edge checks (rounds of kgl_edge_round, see kg_kernels.hip).
+struct EdgeArgs {
+  const float* Af;       // nseg x n endpoints, FP32 (or FP64 below)
+  const float* Bf;
+  const double* Ad;
+  const double* Bd;
+  int n, nseg;
+  double eps;
+  int norm_inf;
+  int max_pos;           // bisection positions the round schedule covers per segment
+  int budget;            // samples per round (capacity of the feasibility workspace)
+  int* ns;               // nseg: steps n_s
+  int* cur;              // nseg: positions checked
+  uint8_t* done;         // nseg
+  float* first_u;        // nseg: result
+  int32_t* nsamples_out; // nullable: n_s per segment
+  int* seg;              // nseg: live segments of the round
+  int* soff;             // nseg + 1: their sample offsets
+  int* ctl;              // 8 words: [0] live, [1] samples, [2] spill, [3] uncertain,
+                         // [4] FP64 re-checks (accumulated), [5] undecided segments,
+                         // [6] segments with a NaN coordinate
+  double* Qd;            // budget x n samples
+  const uint8_t* feas;   // budget: feasibility of the round's samples
+  const int* st;         // status words of the feasibility pipeline
+  int w_spill, w_unc, w_redo;
+  int* out_status;       // status words of the call (same layout)
+};
+hipError_t kgl_edge_init(const EdgeArgs& a, bool clear_ctl, hipStream_t s);
+// phase 0: plan + samples; phase 1: reduce
+hipError_t kgl_edge_round(const EdgeArgs& a, int width, int phase, hipStream_t s);
+hipError_t kgl_edge_finish(const EdgeArgs& a, bool write_status, hipStream_t s);
 
 hipError_t kgl_fill_int(int* p, int n, int v, hipStream_t s);
 
@@ -292,6 +320,7 @@ inline kg_params kg_default_params() {
   p.sdf_lanes = 32;
   p.prep_group = 1;
   p.force_recheck = 0;
+  p.edge_budget_mb = 1024;
   return p;
 }
 
@@ -393,7 +422,8 @@ struct kg_ctx_s {
   bool done_valid = false;
   DevBuf wsQ, wsT64, wsLB, wsJL, wsCounts, wsOffsets, wsItems, wsScanTmp, wsBest, wsUnc, wsChunk, wsSpill,
       wsStatus, wsRedo, wsOut8a, wsOut8b, wsOut8c, wsOut32, wsF, wsG, wsPoff, wsPts, wsLoss, wsDQ,
-      wsDl, wsEA, wsEB, wsESeg, wsESoff, wsENs, wsEOoff, wsEOrder, wsEQ, wsEFeas, wsEU, wsEDone, wsRec, wsMsdf;
+      wsDl, wsEA, wsEB, wsESeg, wsESoff, wsENs, wsECur, wsECtl, wsEStat, wsEQ, wsEFeas, wsEU, wsEDone, wsRec,
+      wsMsdf;
   int timing = 1;           // 0 none, 1 traversal kernel events, 2 also per-phase events
   bool last_phase_ev = false;
   int last_tr_slot = -1;    // kernel-event slot of the last feasibility batch (-1: none)

@@ -21,6 +21,7 @@
 #include <type_traits>
 
 #include "kg_internal.h"
+#include "kg_edge.h"
 
 #define KG_CPW_MAX 64
 #define KG_NE 64  // configuration entries per wave (ring)
@@ -260,13 +261,9 @@ __device__ int swap_erase_pos(int j, int E, int m, F rej) {
 // LDSN > 0: single-plan batches of at most LDSN links keep the link boxes in
 // LDS (structure of arrays over the block's lanes) instead of HBM.
 template <int MODE, int LDSN>
-__global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
+__device__ inline void prep_cfg(const PrepArgs& a, int c, double* sLB) {
   constexpr bool FILL = MODE != 0;
   constexpr int BS = LDSN > 0 ? KG_PREP_LDS_BLOCK : 256;
-  __shared__ double sLB[LDSN > 0 ? LDSN * 6 * BS : 1];
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (MODE != 1 && c < a.nreset) a.reset[c] = 0;
-  if (c >= a.N) return;
   const PlanDev& P = a.cfg_plan ? a.plans[a.cfg_plan[c]] : a.plan;
   const int n = P.nlinks;
   const size_t tb = a.toff ? (size_t)a.toff[c] : (size_t)c * n;
@@ -423,6 +420,18 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
   if (MODE != 1) a.counts[c] = cnt;
 }
 
+// One lane per configuration; with a device-side count the lanes loop over
+// the configurations (the grid is then bounded by the launcher).
+template <int MODE, int LDSN>
+__global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
+  constexpr int BS = LDSN > 0 ? KG_PREP_LDS_BLOCK : 256;
+  __shared__ double sLB[LDSN > 0 ? LDSN * 6 * BS : 1];
+  const int c0 = blockIdx.x * blockDim.x + threadIdx.x;
+  if (MODE != 1 && c0 < a.nreset) a.reset[c0] = 0;
+  const int N = a.N_dev ? min(*a.N_dev, a.N) : a.N;
+  for (int c = c0; c < N; c += gridDim.x * blockDim.x) prep_cfg<MODE, LDSN>(a, c, sLB);
+}
+
 // Broad phase with one lane per (configuration, link): groups of G lanes
 // (G = 8, 16 or 32, at least the robot's link count) take one configuration
 // each, of a single plan or of the configuration's own plan in a
@@ -448,12 +457,16 @@ __global__ __launch_bounds__(KG_PREP_TB) void k_prep_group(PrepArgs a) {
   __shared__ int sAnc[NG][G];
   __shared__ int scnt[NG], sjl[NG];
   const int t = threadIdx.x, g = t / G, l = t % G;
-  const int c = blockIdx.x * NG + g;
   {
     const int r = blockIdx.x * blockDim.x + t;
     if (r < a.nreset) a.reset[r] = 0;
   }
-  const bool valid = c < a.N;
+  // with a device-side count the blocks loop over the batch (block-uniform
+  // trip count, so the barriers below are reached by the whole block)
+  const int Nc = a.N_dev ? min(*a.N_dev, a.N) : a.N;
+  for (int cb = blockIdx.x * NG; cb < Nc; cb += gridDim.x * NG) {
+  const int c = cb + g;
+  const bool valid = c < Nc;
   const PlanDev& P = MULTI ? a.plans[valid ? a.cfg_plan[c] : 0] : a.plan;
   const int n = valid ? P.nlinks : 0;
   const bool mine = l < n;
@@ -615,6 +628,7 @@ __global__ __launch_bounds__(KG_PREP_TB) void k_prep_group(PrepArgs a) {
   if (valid && l == 0) {
     a.jl[c] = jl ? 1 : 0;
     a.counts[c] = go ? scnt[g] : 0;
+  }
   }
 }
 
@@ -1554,10 +1568,14 @@ hipError_t kgl_prep_fill(const PrepArgs& a, hipStream_t s) {
 hipError_t kgl_prep_single(const PrepArgs& a, hipStream_t s) {
   const int n = a.N > a.nreset ? a.N : a.nreset;
   if (n <= 0) return hipSuccess;
+  // a device-side count (at most a.N) bounds the grid: the blocks loop
+  constexpr int kMaxBlocksDev = 4096;
   if (a.group > 0) {
     constexpr int TB = KG_PREP_TB;
     const int ng = TB / a.group;
-    const int blocks = max((a.N + ng - 1) / ng, (a.nreset + TB - 1) / TB);
+    int blocks = (a.N + ng - 1) / ng;
+    if (a.N_dev) blocks = min(blocks, kMaxBlocksDev);
+    blocks = max(blocks, (a.nreset + TB - 1) / TB);
     const bool m = a.cfg_plan != nullptr;
     if (a.group == 8)
       m ? k_prep_group<8, true><<<blocks, TB, 0, s>>>(a) : k_prep_group<8, false><<<blocks, TB, 0, s>>>(a);
@@ -1565,10 +1583,15 @@ hipError_t kgl_prep_single(const PrepArgs& a, hipStream_t s) {
       m ? k_prep_group<16, true><<<blocks, TB, 0, s>>>(a) : k_prep_group<16, false><<<blocks, TB, 0, s>>>(a);
     else
       m ? k_prep_group<32, true><<<blocks, TB, 0, s>>>(a) : k_prep_group<32, false><<<blocks, TB, 0, s>>>(a);
-  } else if (a.lds_links)
-    k_prep<2, KG_PREP_LDS_LINKS><<<(n + KG_PREP_LDS_BLOCK - 1) / KG_PREP_LDS_BLOCK, KG_PREP_LDS_BLOCK, 0, s>>>(a);
-  else
-    k_prep<2, 0><<<(n + 255) / 256, 256, 0, s>>>(a);
+  } else if (a.lds_links) {
+    int blocks = (n + KG_PREP_LDS_BLOCK - 1) / KG_PREP_LDS_BLOCK;
+    if (a.N_dev) blocks = max(min(blocks, kMaxBlocksDev), (a.nreset + KG_PREP_LDS_BLOCK - 1) / KG_PREP_LDS_BLOCK);
+    k_prep<2, KG_PREP_LDS_LINKS><<<blocks, KG_PREP_LDS_BLOCK, 0, s>>>(a);
+  } else {
+    int blocks = (n + 255) / 256;
+    if (a.N_dev) blocks = max(min(blocks, kMaxBlocksDev), (a.nreset + 255) / 256);
+    k_prep<2, 0><<<blocks, 256, 0, s>>>(a);
+  }
   return hipGetLastError();
 }
 
@@ -2330,65 +2353,195 @@ namespace {
 // seg[s] (s found by binary search in the per-segment slot offsets) at
 // bisection position pos0 + (j - soff[s]); q = a*(1-u) + b*u with u = i/n
 // (Cpp/Modeling/Interpolate.cpp:13-14), i the sample index at that position.
-__global__ __launch_bounds__(256) void k_edge_samples(const float* __restrict__ A, const float* __restrict__ B, int n,
-                                                     const int* __restrict__ seg, const int* __restrict__ soff,
-                                                     int nlive, const int* __restrict__ nsamp,
-                                                     const int* __restrict__ ooff, const int* __restrict__ order,
-                                                     int pos0, int total, double* __restrict__ Qd) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= total) return;
-  int lo = 0, hi = nlive - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (soff[mid] <= j) lo = mid; else hi = mid - 1;
-  }
-  const int s = seg[lo];
-  const int k = pos0 + (j - soff[lo]);
-  const int ns = nsamp[s];
-  const int i = order[ooff[s] + k];
-  const double u = (double)i / (double)ns;
-  for (int d = 0; d < n; d++) {
+// ---------------------------------------------------------------- edge checks
+//
+// Straight-line edge checks run as rounds on the device, with no host
+// synchronization between them.  Each segment s has n_s steps and a cursor
+// (bisection positions checked so far).  A round gives every live segment
+// its next min(width, positions left) positions (width: the scheduled width,
+// lowered so that the round fits the sample budget), lays the samples out
+// contiguously (segment order), checks them with the feasibility pipeline,
+// and then each segment either records its first infeasible position of the
+// round (the first in bisection order overall, since the earlier rounds held
+// none) or advances its cursor.
+
+// Endpoint coordinate k of segment s (side 0: start a, 1: end b).
+__device__ inline double edge_q(const EdgeArgs& a, int side, int s, int k) {
+  const size_t i = (size_t)s * a.n + k;
+  if (a.Ad) return side ? a.Bd[i] : a.Ad[i];
+  return (double)(side ? a.Bf[i] : a.Af[i]);
+}
+
+// Segment steps n_s = ceil(d / eps) with d the L2 or L-infinity norm of
+// b - a, in FP64 with separate roundings (the same expression as the host
+// and the oracle).
+__global__ __launch_bounds__(256) void k_edge_init(EdgeArgs a, int clear_ctl) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (clear_ctl && s < 8) a.ctl[s] = 0;
+  if (s >= a.nseg) return;
+  double acc = 0;
+  for (int k = 0; k < a.n; k++) {
 #pragma clang fp contract(off)
-    double q = (double)A[(size_t)s * n + d] * (1.0 - u);
-    q += (double)B[(size_t)s * n + d] * u;
-    Qd[(size_t)j * n + d] = q;
+    const double d = edge_q(a, 0, s, k) - edge_q(a, 1, s, k);
+    if (a.norm_inf)
+      acc = fmax(acc, fabs(d));
+    else
+      acc += d * d;
+  }
+  const double dist = a.norm_inf ? acc : sqrt(acc);
+  const double m = ceil(dist / a.eps);
+  int ns;
+  bool bad = false;
+  if (!(m >= 0)) {  // NaN input
+    ns = 0;
+    bad = true;
+  } else {
+    ns = m > (double)(1 << 30) ? (1 << 30) : (int)m;
+  }
+  a.ns[s] = ns;
+  a.cur[s] = 0;
+  a.done[s] = (ns < 2 || bad) ? 1 : 0;
+  a.first_u[s] = bad ? -2.0f : -1.0f;
+  if (bad) atomicAdd(&a.ctl[6], 1);
+  if (a.nsamples_out) a.nsamples_out[s] = ns;
+}
+
+// Round plan (one block): live segments, the round's width, each live
+// segment's sample offset (exclusive scan in segment order).
+#define KG_EDGE_PLAN_TB 1024
+__global__ __launch_bounds__(KG_EDGE_PLAN_TB) void k_edge_plan(EdgeArgs a, int width) {
+  using Scan = hipcub::BlockScan<int, KG_EDGE_PLAN_TB>;
+  __shared__ typename Scan::TempStorage tmp;
+  __shared__ int s_nlive;
+  const int t = threadIdx.x;
+  const int per = (a.nseg + KG_EDGE_PLAN_TB - 1) / KG_EDGE_PLAN_TB;
+  const int s0 = t * per, s1 = min(a.nseg, s0 + per);
+  auto left = [&](int s) {  // positions of s this schedule may still check
+    if (a.done[s]) return 0;
+    const int m = min(a.ns[s] - 1, a.max_pos);
+    return max(0, m - a.cur[s]);
+  };
+  int nl = 0;
+  for (int s = s0; s < s1; s++) nl += left(s) > 0;
+  int nlive_pre, nlive;
+  Scan(tmp).ExclusiveSum(nl, nlive_pre, nlive);
+  if (t == 0) s_nlive = nlive;
+  __syncthreads();
+  nlive = s_nlive;
+  const int w = nlive > 0 ? max(1, min(width, a.budget / nlive)) : 0;
+  int ns_loc = 0;
+  for (int s = s0; s < s1; s++) ns_loc += min(w, left(s));
+  int off, total;
+  __syncthreads();
+  Scan(tmp).ExclusiveSum(ns_loc, off, total);
+  int li = nlive_pre;
+  for (int s = s0; s < s1; s++) {
+    const int k = min(w, left(s));
+    if (k > 0) {
+      a.seg[li] = s;
+      a.soff[li] = off;
+      li++;
+      off += k;
+    }
+  }
+  if (t == 0) {
+    a.soff[nlive] = total;
+    a.ctl[0] = nlive;
+    a.ctl[1] = total;
   }
 }
 
-// First infeasible position of each live segment in this round.
-__global__ __launch_bounds__(256) void k_edge_reduce(const int* __restrict__ seg, const int* __restrict__ soff, int nlive,
-                                                    int total, const uint8_t* __restrict__ feas,
-                                                    const int* __restrict__ nsamp, const int* __restrict__ ooff,
-                                                    const int* __restrict__ order, int pos0, float* __restrict__ first_u,
-                                                    uint8_t* __restrict__ done) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nlive) return;
-  const int b = soff[t], e = t + 1 < nlive ? soff[t + 1] : total;
-  const int s = seg[t];
-  for (int j = b; j < e; j++)
-    if (!feas[j]) {
-      const int i = order[ooff[s] + pos0 + (j - b)];
-      first_u[s] = (float)((double)i / (double)nsamp[s]);
-      done[s] = 1;
-      return;
+// Sample j of the round: segment by binary search over the offsets, its
+// position cursor + (j - offset) in bisection order, q = a (1-u) + b u.
+__global__ __launch_bounds__(256) void k_edge_samples(EdgeArgs a) {
+  const int nlive = a.ctl[0], total = a.ctl[1];
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < total; j += gridDim.x * blockDim.x) {
+    int lo = 0, hi = nlive - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (a.soff[mid] <= j) lo = mid; else hi = mid - 1;
     }
+    const int s = a.seg[lo];
+    const int ns = a.ns[s];
+    const int i = kg_bisect_index(ns, a.cur[s] + (j - a.soff[lo]));
+    const double u = (double)i / (double)ns;
+    for (int d = 0; d < a.n; d++) {
+#pragma clang fp contract(off)
+      double q = edge_q(a, 0, s, d) * (1.0 - u);
+      q += edge_q(a, 1, s, d) * u;
+      a.Qd[(size_t)j * a.n + d] = q;
+    }
+  }
+}
+
+// First infeasible sample of each live segment in the round, else the
+// cursor advances; the round's status words are accumulated.
+__global__ __launch_bounds__(256) void k_edge_reduce(EdgeArgs a) {
+  const int nlive = a.ctl[0];
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g == 0) {
+    a.ctl[2] |= a.st[a.w_spill];
+    a.ctl[3] += a.st[a.w_unc];
+    a.ctl[4] += a.st[a.w_redo];
+  }
+  for (int t = g; t < nlive; t += gridDim.x * blockDim.x) {
+    const int s = a.seg[t];
+    const int b = a.soff[t], e = a.soff[t + 1];
+    int j = b;
+    while (j < e && a.feas[j]) j++;
+    if (j < e) {
+      const int ns = a.ns[s];
+      const int i = kg_bisect_index(ns, a.cur[s] + (j - b));
+      a.first_u[s] = (float)((double)i / (double)ns);
+      a.done[s] = 1;
+    } else {
+      a.cur[s] += e - b;
+    }
+  }
+}
+
+// Segments the schedule did not finish (more positions than it covers) are
+// reported undecided (-2) and counted.
+__global__ __launch_bounds__(256) void k_edge_finish(EdgeArgs a) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.nseg) return;
+  if (!a.done[s] && a.cur[s] < a.ns[s] - 1) {
+    a.first_u[s] = -2.0f;
+    atomicAdd(&a.ctl[5], 1);
+  }
+}
+
+// The call's status words, in the layout of a feasibility batch's.
+__global__ void k_edge_status(EdgeArgs a) {
+  if (threadIdx.x == 0) {
+    a.out_status[a.w_spill] = a.ctl[2];
+    a.out_status[a.w_unc] = a.ctl[3] + a.ctl[5] + a.ctl[6];
+    a.out_status[a.w_redo] = a.ctl[4];
+  }
 }
 
 }  // namespace
 
-hipError_t kgl_edge_samples(const float* A, const float* B, int n, const int* seg, const int* soff, int nlive,
-                            const int* nsamp, const int* ooff, const int* order, int pos0, int total, double* Qd,
-                            hipStream_t s) {
-  if (total <= 0) return hipSuccess;
-  k_edge_samples<<<(total + 255) / 256, 256, 0, s>>>(A, B, n, seg, soff, nlive, nsamp, ooff, order, pos0, total, Qd);
+hipError_t kgl_edge_init(const EdgeArgs& a, bool clear_ctl, hipStream_t s) {
+  k_edge_init<<<(max(a.nseg, 8) + 255) / 256, 256, 0, s>>>(a, clear_ctl ? 1 : 0);
   return hipGetLastError();
 }
 
-hipError_t kgl_edge_reduce(const int* seg, const int* soff, int nlive, int total, const uint8_t* feas, const int* nsamp,
-                           const int* ooff, const int* order, int pos0, float* first_u, uint8_t* done, hipStream_t s) {
-  if (nlive <= 0) return hipSuccess;
-  k_edge_reduce<<<(nlive + 255) / 256, 256, 0, s>>>(seg, soff, nlive, total, feas, nsamp, ooff, order, pos0, first_u,
-                                                     done);
+hipError_t kgl_edge_round(const EdgeArgs& a, int width, int phase, hipStream_t s) {
+  // grid-stride kernels: enough blocks to cover the budget, at most 1024
+  const int blocks = max(1, min(1024, (a.budget + 255) / 256));
+  if (phase == 0) {
+    k_edge_plan<<<1, KG_EDGE_PLAN_TB, 0, s>>>(a, width);
+    k_edge_samples<<<blocks, 256, 0, s>>>(a);
+  } else {
+    k_edge_reduce<<<max(1, min(1024, (a.nseg + 255) / 256)), 256, 0, s>>>(a);
+  }
+  return hipGetLastError();
+}
+
+hipError_t kgl_edge_finish(const EdgeArgs& a, bool write_status, hipStream_t s) {
+  k_edge_finish<<<(max(a.nseg, 1) + 255) / 256, 256, 0, s>>>(a);
+  if (write_status) k_edge_status<<<1, 64, 0, s>>>(a);
   return hipGetLastError();
 }
 

@@ -135,6 +135,16 @@ class Geom:
         g.dims = tuple(int(x) for x in vals.shape)
         return g
 
+    def bvh(self):
+        """The mesh's OBB tree as an (nnodes, 16) float32 array (centre,
+        half extents, column-major axes, child as int32 bits; see
+        kg_geom_get_bvh)."""
+        n = ctypes.c_int32()
+        call("kg_geom_get_bvh", self.h, None, 0, ctypes.byref(n))
+        out = np.zeros((max(1, n.value), 16), np.float32)
+        call("kg_geom_get_bvh", self.h, _p(out, _pf), n.value, ctypes.byref(n))
+        return out[:n.value].copy()
+
     def to_sdf(self, resolution=0.0, pad=2):
         """TriangleMesh -> ImplicitSurface (kg_geom_mesh_to_sdf)."""
         h = ctypes.c_void_p()
@@ -481,12 +491,31 @@ class World:
 
     def edge_batch(self, A, B, eps=0.01, norm_inf=False, robot=0):
         """First infeasible interior sample u per segment (-1 if none); the
-        segment length is the L2 norm unless norm_inf (the L-infinity norm)."""
+        segment length is the L2 norm unless norm_inf (the L-infinity norm).
+        float64 endpoints (Klampt's Config) are interpolated as given
+        (kg_edge_batch_f64); anything else as float32."""
         n = self.robots[robot].n
-        A, B = _f(A).reshape(-1, n), _f(B).reshape(-1, n)
+        dbl = np.asarray(A).dtype == np.float64 and np.asarray(B).dtype == np.float64
+        cv, pt, name = (_d, _pd, "kg_edge_batch_f64") if dbl else (_f, _pf, "kg_edge_batch")
+        A, B = cv(A).reshape(-1, n), cv(B).reshape(-1, n)
+        assert A.shape == B.shape
         out = np.zeros(A.shape[0], np.float32)
-        call("kg_edge_batch", self.h, robot, _p(A, _pf), _p(B, _pf), A.shape[0], float(eps), 1 if norm_inf else 0,
-             _p(out, _pf))
+        call(name, self.h, robot, _p(A, pt), _p(B, pt), A.shape[0], float(eps), 1 if norm_inf else 0, _p(out, _pf))
+        return out
+
+    def edge_batch_dev(self, A, B, eps=0.01, norm_inf=False, robot=0, max_steps=0, out=None, nsamples=None):
+        """Device-tensor edge checks on the current torch stream (float32 or
+        float64 endpoints); returns first_u (-2 for a segment longer than
+        max_steps steps, see kg_edge_batch_dev)."""
+        import torch
+        assert A.is_cuda and B.is_cuda and A.dtype == B.dtype and A.shape == B.shape
+        assert A.is_contiguous() and B.is_contiguous()
+        nseg = A.shape[0]
+        if out is None:
+            out = torch.empty(nseg, dtype=torch.float32, device=A.device)
+        name = "kg_edge_batch_f64_dev" if A.dtype == torch.float64 else "kg_edge_batch_dev"
+        call(name, self.h, robot, _tp(A), _tp(B), nseg, float(eps), 1 if norm_inf else 0, int(max_steps), _tp(out),
+             _tp(nsamples), _torch_stream())
         return out
 
     def last_stats(self):

@@ -69,6 +69,9 @@ def lib():
                                       ctypes.c_int, ctypes.c_int]),
             "or_sdf_penalty": (None, [_vp, ctypes.c_int, _d, _i, _d, _d, _d, ctypes.c_int, _d, _i,
                                       ctypes.c_double, _d, _d, ctypes.c_double, ctypes.c_int]),
+            "or_mesh_set_obb": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float), ctypes.c_int]),
+            "or_feasible_counted": (ctypes.c_int, [_vp, ctypes.c_int, _d, ctypes.c_int, _u8,
+                                                   ctypes.POINTER(ctypes.c_int64), ctypes.c_int]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -101,6 +104,7 @@ class OracleWorld:
         self.L = lib()
         self.h = self.L.or_world_new()
         self._robots = []
+        self.meshes = []  # oracle mesh index per add_mesh call, in order
 
     def __del__(self):
         try:
@@ -111,7 +115,36 @@ class OracleWorld:
     def add_mesh(self, mesh):
         V = _dd(mesh.verts)
         T = _ii(mesh.tris)
-        return self.L.or_add_mesh(self.h, _p(V, _d), V.shape[0], _p(T, _i), T.shape[0])
+        g = self.L.or_add_mesh(self.h, _p(V, _d), V.shape[0], _p(T, _i), T.shape[0])
+        self.meshes.append(g)
+        return g
+
+    def set_obb_tree(self, g, nodes):
+        """Give mesh g an OBB tree: (nnodes, 16) float32 records (centre,
+        half extents, column-major axes, child as int32 bits), e.g. the
+        engine's own tree (engine.Geom.bvh), so that feasible_counted
+        traverses the same tree."""
+        nodes = np.ascontiguousarray(nodes, np.float32)
+        assert nodes.ndim == 2 and nodes.shape[1] == 16
+        rc = self.L.or_mesh_set_obb(self.h, g, nodes.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), nodes.shape[0])
+        if rc != 0:
+            raise ValueError("invalid OBB tree for mesh %d" % g)
+
+    def feasible_counted(self, r, Q, nthreads=None):
+        """IsFeasible with Klampt's early returns on the OBB trees given by
+        set_obb_tree, with the work of each check: (feasible[N], counts[N, 5])
+        with counts = environment box tests, environment-node visits,
+        environment triangle tests, self box tests, self triangle tests."""
+        Q = _dd(Q)
+        N = Q.shape[0]
+        feas = np.zeros(N, np.uint8)
+        cnt = np.zeros((N, 5), np.int64)
+        rc = self.L.or_feasible_counted(self.h, r, _p(Q, _d), N, _p(feas, _u8),
+                                        cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                        nthreads or nthreads_default())
+        if rc != 0:
+            raise ValueError("a mesh of the check has no OBB tree (set_obb_tree)")
+        return feas, cnt
 
     def add_terrain(self, mesh):
         return self.L.or_add_terrain(self.h, self.add_mesh(mesh))

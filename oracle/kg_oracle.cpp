@@ -135,10 +135,20 @@ struct Node {
   int left, right, tri;  // tri >= 0 => leaf
 };
 
+// A node of an oriented-box tree given from outside (the engine's BVH, for
+// counting traversal work on the same tree): points c + R diag(e) s,
+// |s_k| <= 1, R column-major; child >= 0 is the first of two adjacent
+// children, child < 0 a leaf holding triangle ~child.
+struct ObbN {
+  double c[3], e[3], R[9];
+  int child;
+};
+
 struct Mesh {
   std::vector<V3> V;
   std::vector<std::array<int, 3>> T;
   std::vector<Node> nodes;
+  std::vector<ObbN> obb;  // optional (or_mesh_set_obb)
   double lmin[3], lmax[3];
   bool empty() const { return T.empty(); }
 };
@@ -625,6 +635,108 @@ static double mesh_separation(const Mesh& A, const Xf& XA, const Mesh& B, const 
 // World AABB of a mesh placed at X: the transformed local AABB (8 corners).
 // (AnyCollisionGeometry3D::GetAABB lives in KrisLibrary; this definition is
 // the one the GPU library uses too.)
+// Work of one sequential traversal of two OBB trees, to define the
+// algorithmic bytes of the collision path: box tests, environment-node
+// visits (the root, then two nodes each time the environment side is split;
+// a split of the link side re-uses the environment node it holds) and
+// triangle tests.
+struct ObbCount {
+  int64_t box = 0, env_nodes = 0, tri = 0;
+};
+
+// 15-axis overlap test of node a (frame A) and node b (frame B), X: B -> A;
+// slack: the smallest face-axis margin (how deeply they overlap).
+static bool obbn_overlap(const ObbN& a, const ObbN& b, const Xf& X, double& slack) {
+  // b's centre and axes in frame A, then in a's axes
+  V3 cb = apply(X, mk(b.c[0], b.c[1], b.c[2]));
+  V3 d = cb - mk(a.c[0], a.c[1], a.c[2]);
+  double T[3], R[3][3], Q[3][3];
+  for (int i = 0; i < 3; i++) {
+    V3 ai = mk(a.R[3 * i], a.R[3 * i + 1], a.R[3 * i + 2]);
+    T[i] = dot(ai, d);
+    for (int j = 0; j < 3; j++) {
+      V3 bj = rot(X, mk(b.R[3 * j], b.R[3 * j + 1], b.R[3 * j + 2]));
+      R[i][j] = dot(ai, bj);
+      Q[i][j] = std::fabs(R[i][j]) + 1e-12;
+    }
+  }
+  const double* ea = a.e;
+  const double* eb = b.e;
+  slack = kInf;
+  for (int i = 0; i < 3; i++) {
+    const double rhs = ea[i] + eb[0] * Q[i][0] + eb[1] * Q[i][1] + eb[2] * Q[i][2];
+    if (std::fabs(T[i]) > rhs) return false;
+    slack = std::min(slack, rhs - std::fabs(T[i]));
+  }
+  for (int j = 0; j < 3; j++) {
+    const double t = T[0] * R[0][j] + T[1] * R[1][j] + T[2] * R[2][j];
+    const double rhs = ea[0] * Q[0][j] + ea[1] * Q[1][j] + ea[2] * Q[2][j] + eb[j];
+    if (std::fabs(t) > rhs) return false;
+    slack = std::min(slack, rhs - std::fabs(t));
+  }
+  for (int i = 0; i < 3; i++) {
+    const int i1 = (i + 1) % 3, i2 = (i + 2) % 3;
+    for (int j = 0; j < 3; j++) {
+      const int j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+      const double t = T[i2] * R[i1][j] - T[i1] * R[i2][j];
+      const double ra = ea[i1] * Q[i2][j] + ea[i2] * Q[i1][j];
+      const double rb = eb[j1] * Q[i][j2] + eb[j2] * Q[i][j1];
+      if (std::fabs(t) > ra + rb) return false;
+    }
+  }
+  return true;
+}
+
+// First-hit collide of link mesh A against mesh B on their OBB trees,
+// depth first: the node of larger box volume is split (a leaf never), the
+// more deeply overlapping child pair is explored first, the first
+// intersecting triangle pair ends the query.  b_env: B is environment
+// geometry (its node visits are counted).
+static bool obb_collide_count(const Mesh& A, const Xf& XA, const Mesh& B, const Xf& XB, bool b_env, ObbCount& cnt) {
+  if (A.empty() || B.empty()) return false;
+  const Xf X = compose(inverse(XA), XB);  // B local -> A local
+  auto vol = [](const ObbN& n) { return n.e[0] * n.e[1] * n.e[2]; };
+  V3 ta[3], tb[3];
+  std::vector<std::pair<int, int>> st;
+  st.reserve(256);
+  double sl;
+  cnt.box++;
+  if (b_env) cnt.env_nodes++;
+  if (!obbn_overlap(A.obb[0], B.obb[0], X, sl)) return false;
+  st.push_back({0, 0});
+  while (!st.empty()) {
+    const auto [ia, ib] = st.back();
+    st.pop_back();
+    const ObbN& na = A.obb[ia];
+    const ObbN& nb = B.obb[ib];
+    const bool la = na.child < 0, lb = nb.child < 0;
+    if (la && lb) {
+      cnt.tri++;
+      tri_world(A, ~na.child, XA, ta);
+      tri_world(B, ~nb.child, XB, tb);
+      if (tri_tri(ta, tb)) return true;
+      continue;
+    }
+    const bool splitA = lb || (!la && vol(na) >= vol(nb));
+    std::pair<int, int> kid[2];
+    double depth[2];
+    int nk = 0;
+    for (int k = 0; k < 2; k++) {
+      const int ca = splitA ? na.child + k : ia, cb = splitA ? ib : nb.child + k;
+      cnt.box++;
+      if (b_env && !splitA) cnt.env_nodes++;
+      if (obbn_overlap(A.obb[ca], B.obb[cb], X, sl)) {
+        kid[nk] = {ca, cb};
+        depth[nk] = sl;
+        nk++;
+      }
+    }
+    if (nk == 2 && depth[1] > depth[0]) std::swap(kid[0], kid[1]);
+    for (int k = nk - 1; k >= 0; k--) st.push_back(kid[k]);  // deeper one on top
+  }
+  return false;
+}
+
 static void world_aabb(const Mesh& m, const Xf& X, double* lo, double* hi) {
   double c[3], e[3];
   for (int k = 0; k < 3; k++) {
@@ -1229,6 +1341,86 @@ void or_isfeasible_batch(void* wp, int r, const double* Q, int N, uint8_t* feas,
   World& w = *(World*)wp;
   int n = w.robots[r].n;
   parallel_for(N, nthreads, [&](int c) { feas[c] = is_feasible(w, r, Q + (size_t)c * n) ? 1 : 0; });
+}
+
+// SingleRobotCSpace::IsFeasible with its early returns, traversing the OBB
+// trees given by or_mesh_set_obb, with its work counted per configuration:
+// out[c*5 + 0..4] = environment box tests, environment-node visits,
+// environment triangle tests, self box tests, self triangle tests; feas[c]
+// the answer.  Returns -1 (nothing computed) if a mesh of the check lacks
+// an OBB tree.
+static bool is_feasible_counted(const World& w, int r, const double* q, int64_t* out) {
+  const Robot& R = w.robots[r];
+  for (int k = 0; k < 5; k++) out[k] = 0;
+  for (int k = 0; k < R.n; k++)
+    if (q[k] < R.qmin[k] || q[k] > R.qmax[k]) return false;
+  std::vector<Xf> T;
+  fk(R, q, T);
+  std::vector<Placed> L1, L2;
+  robot_links(w, r, T, L1);
+  env_items(w, r, L2);
+  std::vector<Placed> S = L1;
+  reduce_sets(L1, L2);
+  ObbCount ce, cs;
+  bool hit = false;
+  for (size_t i = 0; i < L1.size() && !hit; i++)
+    for (size_t j = 0; j < L2.size(); j++) {
+      if (!(w.en(L1[i].id, L2[j].id) || w.en(L2[j].id, L1[i].id))) continue;
+      if (!aabb_hit(L1[i], L2[j])) continue;
+      if (obb_collide_count(*L1[i].m, L1[i].X, *L2[j].m, L2[j].X, true, ce)) {
+        hit = true;
+        break;
+      }
+    }
+  for (size_t i = 0; i < S.size() && !hit; i++)
+    for (size_t j = i + 1; j < S.size(); j++) {
+      if (!(w.en(S[i].id, S[j].id) || w.en(S[i].id, S[i].id))) continue;
+      if (!aabb_hit(S[i], S[j])) continue;
+      if (obb_collide_count(*S[i].m, S[i].X, *S[j].m, S[j].X, false, cs)) {
+        hit = true;
+        break;
+      }
+    }
+  out[0] = ce.box;
+  out[1] = ce.env_nodes;
+  out[2] = ce.tri;
+  out[3] = cs.box;
+  out[4] = cs.tri;
+  return !hit;
+}
+
+int or_mesh_set_obb(void* wp, int g, const float* nodes, int nn) {
+  World& w = *(World*)wp;
+  Mesh& m = w.geoms[g];
+  m.obb.resize(nn);
+  for (int i = 0; i < nn; i++) {
+    const float* p = nodes + 16 * (size_t)i;
+    ObbN& o = m.obb[i];
+    for (int k = 0; k < 3; k++) {
+      o.c[k] = p[k];
+      o.e[k] = p[3 + k];
+    }
+    for (int k = 0; k < 9; k++) o.R[k] = p[6 + k];
+    std::memcpy(&o.child, p + 15, sizeof(int));
+    if (o.child >= nn || (o.child < 0 && ~o.child >= (int)m.T.size())) return -1;
+  }
+  return 0;
+}
+
+int or_feasible_counted(void* wp, int r, const double* Q, int N, uint8_t* feas, int64_t* out, int nthreads) {
+  World& w = *(World*)wp;
+  const Robot& R = w.robots[r];
+  for (int j = 0; j < R.n; j++)
+    if (R.geom[j] >= 0 && !w.geoms[R.geom[j]].empty() && w.geoms[R.geom[j]].obb.empty()) return -1;
+  for (int t : w.terrains)
+    if (!w.geoms[t].empty() && w.geoms[t].obb.empty()) return -1;
+  for (auto& o : w.objects)
+    if (!w.geoms[o.first].empty() && w.geoms[o.first].obb.empty()) return -1;
+  const int n = R.n;
+  parallel_for(N, nthreads, [&](int c) {
+    feas[c] = is_feasible_counted(w, r, Q + (size_t)c * n, out + 5 * (size_t)c) ? 1 : 0;
+  });
+  return 0;
 }
 
 int or_pair_list(void* wp, int r, int32_t* out, int maxp) {

@@ -33,3 +33,19 @@ def test_errors_without_device_calls():
     assert L.kg_geom_collide(None, None, None) == -1
     assert b"NULL" in L.kg_last_error()
     assert L.kg_version() >= 1
+
+
+def test_edge_sample_index_matches_oracle_bisection_order():
+    """The closed-form k-th bisection sample (kg_edge_sample_index, used by
+    the device edge rounds) equals the oracle's queue-built level order."""
+    import numpy as np
+    import oracle
+    L = _lib.lib()
+    rng = np.random.default_rng(0)
+    ns = list(range(0, 700)) + [1023, 1024, 1025, 4095, 4096, 4097] + list(rng.integers(2, 200000, 12))
+    for n in ns:
+        n = int(n)
+        order = oracle.bisection_order(n)
+        got = np.array([L.kg_edge_sample_index(n, k) for k in range(max(0, n - 1))], np.int32)
+        assert np.array_equal(got, order), n
+        assert L.kg_edge_sample_index(n, -1) == -1 and L.kg_edge_sample_index(n, max(0, n - 1)) == -1
