@@ -152,7 +152,9 @@ struct StreamScope {
   hipStream_t s;
   StreamScope(kg_ctx_s* c_, hipStream_t s_) : c(c_), s(s_) {
     hip_check(hipSetDevice(c->device), "hipSetDevice");
-    if (!c->evDone) hip_check(hipEventCreateWithFlags(&c->evDone, hipEventDisableTiming), "hipEventCreate");
+    if (!c->evDone)
+      hip_check(hipEventCreateWithFlags(&c->evDone, hipEventDisableTiming | hipEventDisableSystemFence),
+                "hipEventCreate");
     if (c->done_valid && c->done_stream != s) hip_check(hipStreamWaitEvent(s, c->evDone, 0), "hipStreamWaitEvent");
   }
   ~StreamScope() {
@@ -426,11 +428,17 @@ enum { ST_SPILL = 0, ST_REDO = 1, ST_CHUNK32 = 2, ST_CHUNK64 = 3, ST_CNT = 4, ST
 void ensure_status_host(kg_ctx_s* c) {
   if (!c->hstatus) {
     hip_check(hipHostMalloc((void**)&c->hstatus, ST_WORDS * sizeof(int), hipHostMallocDefault), "hipHostMalloc");
-    for (auto& e : c->ev) hip_check(hipEventCreate(&e), "hipEventCreate");
-    hip_check(hipEventCreateWithFlags(&c->evStat, hipEventDisableTiming), "hipEventCreate");
+    // The events of a batch only order work on this device (stream waits)
+    // or time it; none needs the system-scope fence a default event does
+    // (an L2 writeback and invalidate on every record).  Host reads of
+    // device results go through copies, which order themselves.
+    constexpr unsigned kTimed = hipEventDisableSystemFence;
+    constexpr unsigned kOrder = hipEventDisableTiming | hipEventDisableSystemFence;
+    for (auto& e : c->ev) hip_check(hipEventCreateWithFlags(&e, kTimed), "hipEventCreate");
+    hip_check(hipEventCreateWithFlags(&c->evStat, kOrder), "hipEventCreate");
     for (int i = 0; i < kg_ctx_s::TRING; i++) {
-      hip_check(hipEventCreate(&c->tr_beg[i]), "hipEventCreate");
-      hip_check(hipEventCreate(&c->tr_end[i]), "hipEventCreate");
+      hip_check(hipEventCreateWithFlags(&c->tr_beg[i], kTimed), "hipEventCreate");
+      hip_check(hipEventCreateWithFlags(&c->tr_end[i], kTimed), "hipEventCreate");
     }
   }
 }
