@@ -276,6 +276,23 @@ int kg_feasible_batch_f64(kg_world w, int robot_index, const double* Q, int N, u
 int kg_feasible_batch_f64_dev(kg_world w, int robot_index, const double* dQ, int N, uint8_t* d_feasible,
                               uint8_t* d_env_hit, uint8_t* d_self_hit, int32_t* d_first_pair, void* stream);
 
+/* kg_feasible_batch_dev that allocates nothing.  No counterpart in Klampt:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Every
+ * workspace of the batch comes from a fixed scratch array of 64 MiB that the
+ * library holds on each GPU it runs on until the process ends (16 MiB of it
+ * is the traversal's spill region; overflow is reported as for the other
+ * calls, kg_ctx_status_dev).  Limits: robots of at most 32 links, and N at
+ * most what fits -- about 29,000 configurations of a 7-link arm against one
+ * terrain (22 pair items each); KG_E_LIMIT names the bound for the world.
+ * The world's tables must already be on the device: KG_E_INVALID if the
+ * world changed since its last batched call.  All calls on one GPU share
+ * the scratch array and it keeps the last call's contents: calls that
+ * overlap on one GPU (two contexts, or a stream not ordered after the
+ * previous call) race. */
+int kg_feasible_batch_dev_noalloc(kg_world w, int robot_index, const float* dQ, int N, uint8_t* d_feasible,
+                                  uint8_t* d_env_hit, uint8_t* d_self_hit, int32_t* d_first_pair, void* stream);
+
 /* Pair distances with closest points and joint-space gradients: the
  * point-collision step of KineTrajOpt
  * (Python/klampt/plan/kinetrajopt/kinetrajopt.py:530-556: distance_ext per

@@ -98,6 +98,7 @@ SIGNATURES = {
     "kg_feasible_batch_dev": [_vp, _c_int, _vp, _c_int, _vp, _vp, _vp, _vp, _vp],
     "kg_feasible_batch_f64": [_vp, _c_int, _pd, _c_int, _pu8, _pu8, _pu8, _pi],
     "kg_feasible_batch_f64_dev": [_vp, _c_int, _vp, _c_int, _vp, _vp, _vp, _vp, _vp],
+    "kg_feasible_batch_dev_noalloc": [_vp, _c_int, _vp, _c_int, _vp, _vp, _vp, _vp, _vp],
     "kg_distance_batch": [_vp, _c_int, _pf, _c_int, _pf],
     "kg_distance_batch_dev": [_vp, _c_int, _vp, _c_int, _vp, _vp],
     "kg_edge_batch": [_vp, _c_int, _pf, _pf, _c_int, _c_double, _c_int, _pf],

@@ -475,11 +475,26 @@ struct Layout {
   Tol tol32{}, tol64{};
 };
 
+// Workspaces of one batch given by the caller (the allocation-free entry
+// point's scratch) instead of the context's grow-only buffers.  Only the
+// fixed-capacity layout with the per-link broad phase uses them.
+struct FeasWs {
+  int* status;       // ST_WORDS
+  uint8_t* jl;       // N
+  int* counts;       // N + 1
+  int* redo;         // N + 16
+  PairItem* items;   // N x cap
+  ItemRec* recs;     // N x cap
+  uint64_t* spill;   // spill region, split evenly over the traversal waves
+  int spill_entries; // its size in entries
+};
+
 // N_dev (nullable): device-side configuration count, at most N (the capacity
-// the workspaces are sized for); timed = false records no timing events.
+// the workspaces are sized for); timed = false records no timing events;
+// ws (nullable): caller-given workspaces, then nothing is allocated.
 void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* Qd, int N, uint8_t* feas,
                    uint8_t* env, uint8_t* self, int32_t* fp, hipStream_t s, const int* N_dev = nullptr,
-                   bool timed = true) {
+                   bool timed = true, const FeasWs* ws = nullptr) {
   hip_check(hipSetDevice(c->device), "hipSetDevice");
   ensure_status_host(c);
   c->last_N = N;
@@ -501,15 +516,18 @@ void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* 
   // writes start records, and the FP64 pass forms its frames from q); link
   // boxes only for the count / scan path
   const bool lds_links = !group && !L.multi && ml <= 8 && (fixed || L.multi);
-  if (!group) hip_check(c->wsT64.ensure(L.nrows * 12 * sizeof(double)), "hipMalloc(T64)");
-  if (!group && !lds_links) hip_check(c->wsLB.ensure(L.nrows * 6 * sizeof(double)), "hipMalloc(LB)");
-  hip_check(c->wsJL.ensure((size_t)N), "hipMalloc(jl)");
-  hip_check(c->wsCounts.ensure((size_t)(N + 1) * sizeof(int)), "hipMalloc(counts)");
-  hip_check(c->wsOffsets.ensure((size_t)(N + 1) * sizeof(int)), "hipMalloc(offsets)");
-  hip_check(c->wsStatus.ensure(ST_WORDS * sizeof(int)), "hipMalloc(status)");
-  hip_check(c->wsRedo.ensure((size_t)(N + 16) * sizeof(int)), "hipMalloc(redo)");
-  int* status = c->wsStatus.as<int>();
-  int* redo_list = c->wsRedo.as<int>();
+  need(!ws || (fixed && group), "caller workspaces need the fixed-capacity, per-link broad phase");
+  if (!ws) {
+    if (!group) hip_check(c->wsT64.ensure(L.nrows * 12 * sizeof(double)), "hipMalloc(T64)");
+    if (!group && !lds_links) hip_check(c->wsLB.ensure(L.nrows * 6 * sizeof(double)), "hipMalloc(LB)");
+    hip_check(c->wsJL.ensure((size_t)N), "hipMalloc(jl)");
+    hip_check(c->wsCounts.ensure((size_t)(N + 1) * sizeof(int)), "hipMalloc(counts)");
+    hip_check(c->wsOffsets.ensure((size_t)(N + 1) * sizeof(int)), "hipMalloc(offsets)");
+    hip_check(c->wsStatus.ensure(ST_WORDS * sizeof(int)), "hipMalloc(status)");
+    hip_check(c->wsRedo.ensure((size_t)(N + 16) * sizeof(int)), "hipMalloc(redo)");
+  }
+  int* status = ws ? ws->status : c->wsStatus.as<int>();
+  int* redo_list = ws ? ws->redo : c->wsRedo.as<int>();
   PrepArgs pa{};
   pa.Qf = Qf;
   pa.Qd = Qd;
@@ -517,11 +535,11 @@ void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* 
   pa.mode = mode;
   if (L.P) pa.plan = L.P->dev;
   pa.pool = c->pool;
-  pa.T64 = c->wsT64.as<double>();
+  pa.T64 = ws ? nullptr : c->wsT64.as<double>();
   pa.t_rows = L.nrows;
-  pa.LB = c->wsLB.as<double>();
-  pa.jl = c->wsJL.as<uint8_t>();
-  pa.counts = c->wsCounts.as<int>();
+  pa.LB = ws ? nullptr : c->wsLB.as<double>();
+  pa.jl = ws ? ws->jl : c->wsJL.as<uint8_t>();
+  pa.counts = ws ? ws->counts : c->wsCounts.as<int>();
   pa.reset = status;
   pa.nreset = ST_WORDS;
   pa.plans = L.plans;
@@ -538,15 +556,15 @@ void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* 
   if (phase_ev) hip_check(hipEventRecord(c->ev[0], s), "hipEventRecord");
   if (fixed || L.multi) {
     need(fixed, "heterogeneous batch exceeds the item capacity limit (2 GiB of pair items)");
-    hip_check(c->wsItems.ensure(std::max<size_t>(1, L.ncap) * sizeof(PairItem)), "hipMalloc(items)");
-    pa.items = c->wsItems.as<PairItem>();
+    if (!ws) hip_check(c->wsItems.ensure(std::max<size_t>(1, L.ncap) * sizeof(PairItem)), "hipMalloc(items)");
+    pa.items = ws ? ws->items : c->wsItems.as<PairItem>();
     pa.cap = L.cap;
     pa.offsets = nullptr;
     pa.group = group;
     pa.lds_links = lds_links;
     if (pa.group) {
-      hip_check(c->wsRec.ensure(std::max<size_t>(1, L.ncap) * sizeof(ItemRec)), "hipMalloc(records)");
-      pa.recs = c->wsRec.as<ItemRec>();
+      if (!ws) hip_check(c->wsRec.ensure(std::max<size_t>(1, L.ncap) * sizeof(ItemRec)), "hipMalloc(records)");
+      pa.recs = ws ? ws->recs : c->wsRec.as<ItemRec>();
       // with start records nothing reads the frames (the FP64 pass forms
       // its own from q)
       pa.T64 = nullptr;
@@ -575,9 +593,11 @@ void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* 
   const int wpc = prm.waves_per_cu;
   const int grid32 = wpc > 0 ? wpc * c->num_cus : kgl_collide_grid(0, c->num_cus, scap);
   const int grid64 = std::min(kgl_collide_grid(1, c->num_cus, scap), std::max(1, c->num_cus));
-  const int spill_cap = prm.spill_cap;
-  hip_check(c->wsSpill.ensure(std::max<size_t>(1, (size_t)std::max(grid32, grid64) * spill_cap) * sizeof(uint64_t)),
-            "hipMalloc(spill)");
+  const int spill_cap = ws ? ws->spill_entries / std::max(grid32, grid64) : prm.spill_cap;
+  if (!ws)
+    hip_check(c->wsSpill.ensure(std::max<size_t>(1, (size_t)std::max(grid32, grid64) * spill_cap) *
+                                sizeof(uint64_t)),
+              "hipMalloc(spill)");
 
   TravArgs ta{};
   ta.pool = c->pool;
@@ -599,7 +619,7 @@ void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* 
     const int nsub = std::max(1, std::min((int)ST_NSUB, prm.nsub));
     ta.nsub = N >= 64 * nsub ? nsub : 1;
   }
-  ta.spill = c->wsSpill.as<uint64_t>();
+  ta.spill = ws ? ws->spill : c->wsSpill.as<uint64_t>();
   ta.spill_cap = spill_cap;
   ta.stack_cap = 0;
   ta.status = status + ST_SPILL;
@@ -785,6 +805,18 @@ int kg_ctx_create(int device, kg_ctx* out) {
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
       c->num_cus = cus;
+    // set-up the allocation-free entry point relies on: this device's
+    // scratch address, the pinned status words and the events
+    try {
+      hip_check(kgl_noalloc_scratch(device, &c->scratch), "hipGetSymbolAddress(scratch)");
+      ensure_status_host(c);
+      hip_check(hipEventCreateWithFlags(&c->evDone, hipEventDisableTiming | hipEventDisableSystemFence),
+                "hipEventCreate");
+    } catch (...) {
+      (void)hipStreamDestroy(c->stream);
+      delete c;
+      throw;
+    }
     *out = c;
   });
 }
@@ -1541,6 +1573,74 @@ int kg_feasible_batch_f64_dev(kg_world w, int ri, const double* dQ, int N, uint8
     hipStream_t s = (hipStream_t)stream;
     StreamScope scope(w->ctx, s);
     feasible_dev(w, ri, nullptr, dQ, N, feas, env, self, fp, s);
+  });
+}
+
+// The allocation-free variant of kg_feasible_batch_dev: every workspace is
+// carved from the device's scratch array (kgl_noalloc_scratch); the world's
+// tables must already be on the device.
+int kg_feasible_batch_dev_noalloc(kg_world w, int ri, const float* dQ, int N, uint8_t* feas, uint8_t* env,
+                                  uint8_t* self, int32_t* fp, void* stream) {
+  return guard([&] {
+    need(w != nullptr, "world is NULL");
+    need(ri >= 0 && ri < (int)w->robots.size(), "robot index out of range");
+    need(N >= 0, "negative batch size");
+    need(N == 0 || (dQ && feas), "NULL buffer");
+    kg_ctx_s* c = w->ctx;
+    need(c->scratch != nullptr, "no scratch on this device");
+    if (N == 0) return;
+    reject_margins(w);
+    // no table uploads here: the plan must be current
+    need(!c->pool_dirty && ri < (int)w->plans.size() &&
+             w->plans[ri].built_version == world_version(w) * 131 + c->pool_version,
+         "the world changed since its tables were uploaded: run kg_feasible_batch_dev (or any batched call) on "
+         "it once before the allocation-free call");
+    RobotPlan& P = w->plans[ri];
+    const int n = P.dev.nlinks;
+    if (!(c->prm.prep_group && n <= 32))
+      fail(KG_E_UNSUPPORTED, "the allocation-free call needs robots of at most 32 links and prep_group = 1");
+    Layout L;
+    L.P = &P;
+    L.cap = active_links(w->robots[ri]) * P.nenv + P.nself;
+    L.maxcap = L.cap;
+    L.nrows = (size_t)N * n;
+    L.ncap = (size_t)L.cap * N;
+    L.tol32 = P.tol32;
+    L.tol64 = P.tol64;
+    // carve: status, spill (fixed 16 MiB), then the per-configuration arrays
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t b_status = al(ST_WORDS * sizeof(int)), b_spill = (size_t)16 << 20;
+    const size_t b_jl = al((size_t)N), b_counts = al((size_t)(N + 1) * sizeof(int)),
+                 b_redo = al((size_t)(N + 16) * sizeof(int)), b_items = al(L.ncap * sizeof(PairItem)),
+                 b_recs = al(L.ncap * sizeof(ItemRec));
+    const size_t total = b_status + b_spill + b_jl + b_counts + b_redo + b_items + b_recs;
+    if (total > KG_NOALLOC_BYTES) {
+      const size_t per = 1 + 2 * sizeof(int) + (size_t)L.cap * (sizeof(PairItem) + sizeof(ItemRec));
+      fail(KG_E_LIMIT,
+           "the allocation-free call holds %u bytes of scratch; %d configurations of this robot and world need "
+           "%zu (about %zu at most)",
+           (unsigned)KG_NOALLOC_BYTES, N, total,
+           (size_t)(KG_NOALLOC_BYTES - b_status - b_spill - 5 * 256 - 17 * sizeof(int)) / per);
+    }
+    unsigned char* p = (unsigned char*)c->scratch;
+    FeasWs ws;
+    ws.status = (int*)p;
+    p += b_status;
+    ws.spill = (uint64_t*)p;
+    ws.spill_entries = (int)(b_spill / sizeof(uint64_t));
+    p += b_spill;
+    ws.jl = p;
+    p += b_jl;
+    ws.counts = (int*)p;
+    p += b_counts;
+    ws.redo = (int*)p;
+    p += b_redo;
+    ws.items = (PairItem*)p;
+    p += b_items;
+    ws.recs = (ItemRec*)p;
+    hipStream_t s = (hipStream_t)stream;
+    StreamScope scope(c, s);
+    feasible_core(c, L, dQ, nullptr, N, feas, env, self, fp, s, nullptr, false, &ws);
   });
 }
 }  // extern "C"

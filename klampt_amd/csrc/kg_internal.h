@@ -258,6 +258,12 @@ hipError_t kgl_edge_finish(const EdgeArgs& a, bool write_status, hipStream_t s);
 
 hipError_t kgl_fill_int(int* p, int n, int v, hipStream_t s);
 
+// Scratch array of the allocation-free feasibility call: KG_NOALLOC_BYTES per
+// device, its address on `device` (which must be current) looked up once.
+#define KG_NOALLOC_BYTES (64u << 20)
+#define KG_MAX_DEVICES 64
+hipError_t kgl_noalloc_scratch(int device, void** ptr);
+
 // TriangleMesh -> ImplicitSurface: the grid of cell-centred signed distances.
 #define KG_MSDF_STK 64  // traversal stack entries (tree depth + 2 at most)
 struct MeshSdfArgs {
@@ -429,6 +435,7 @@ struct kg_ctx_s {
   int last_tr_slot = -1;    // kernel-event slot of the last feasibility batch (-1: none)
   int num_cus = 256;
   kg_params prm = kg_default_params();  // engine parameters (kg_ctx_set_params, kg_autotune)
+  void* scratch = nullptr;   // this device's allocation-free scratch (kgl_noalloc_scratch)
   // heterogeneous-batch layout cache
   DevBuf mPlans, mCfgPlan, mQoff, mToff, mIbase;
   std::vector<uint64_t> mSig;

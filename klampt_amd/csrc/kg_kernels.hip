@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 #include <cstdlib>
+#include <mutex>
 #include <type_traits>
 
 #include "kg_internal.h"
@@ -1638,6 +1639,32 @@ __global__ void k_status_copy(const int* __restrict__ st, int w_spill, int w_unc
 hipError_t kgl_status_copy(const int* status, int w_spill, int w_unc, int w_redo, int32_t* out, hipStream_t s) {
   k_status_copy<<<1, 64, 0, s>>>(status, w_spill, w_unc, w_redo, out);
   return hipGetLastError();
+}
+
+// ------------------------------------------------------------ scratch of the allocation-free entry point
+//
+// kg_feasible_batch_dev_noalloc carves every workspace of a batch (status
+// words, joint-limit flags, item counts, re-check list, pair items, start
+// records, spill region) out of this array instead of allocating.  Each GPU
+// holds its own copy, at its own address, until the process ends; host code
+// never passes the array's name to a kernel (it is a host-side stand-in),
+// only the per-device address kgl_noalloc_scratch looks up.
+static __device__ __align__(256) unsigned char scratch[KG_NOALLOC_BYTES];
+
+hipError_t kgl_noalloc_scratch(int device, void** ptr) {
+  static std::mutex mu;
+  static void* addr[KG_MAX_DEVICES] = {};
+  if (device < 0 || device >= KG_MAX_DEVICES) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[device]) {
+    // the caller has made `device` current
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    addr[device] = p;
+  }
+  *ptr = addr[device];
+  return hipSuccess;
 }
 
 hipError_t kgl_fill_int(int* p, int n, int v, hipStream_t s) {

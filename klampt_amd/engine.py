@@ -444,6 +444,18 @@ class World:
              _torch_stream())
         return feasible
 
+    def feasible_batch_dev_noalloc(self, Q, robot=0, feasible=None, env=None, self_hit=None, first_pair=None):
+        """kg_feasible_batch_dev_noalloc: the same check with every workspace
+        in the library's fixed scratch array (nothing allocated on the
+        device; the outputs are the caller's).  Q float32."""
+        import torch
+        assert Q.is_cuda and Q.dtype == torch.float32 and Q.is_contiguous()
+        N = Q.shape[0]
+        assert feasible is not None
+        call("kg_feasible_batch_dev_noalloc", self.h, robot, _tp(Q), N, _tp(feasible), _tp(env), _tp(self_hit),
+             _tp(first_pair), _torch_stream())
+        return feasible
+
     def distance_batch(self, Q, robot=0):
         n = self.robots[robot].n
         Q = _f(Q).reshape(-1, n)
