@@ -6,11 +6,15 @@ step = one kg_feasible_batch_dev call (IsFeasible semantics: joint limits,
 environment, self collision) over the whole batch with inputs resident in
 HBM.  Prints one JSON line.
 
-Also reported: the roofline of the traversal kernel (algorithmic bytes from
-its own box/triangle-test counters over its average HIP-event duration in
-the timed steps; HBM traffic from the committed rocprofv3 counter summary),
-and the FP64 CPU path (IsFeasible with Klampt's early returns) timed on the
-host cores, all threads and one thread.
+Also reported: the roofline of the traversal kernel -- algorithmic bytes from
+the oracle's sequential IsFeasible (early returns, first hit) counted on the
+engine's own OBB trees over the same configurations (64 B per environment
+node visited, 36 B per environment triangle tested, 4 B per joint value in,
+2 B out) over the kernel's average HIP-event duration in the timed steps;
+beside it the fraction from the kernel's own box/triangle-test counters, the
+HBM traffic fraction (committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary)
+and the FP32 flop fraction -- and the FP64 CPU path (IsFeasible with
+Klampt's early returns) timed on the host cores, all threads and one thread.
 
 ``python bench.py --copy-probe`` only runs a plain 256 MiB device copy five
 times: the calibration workload for the FETCH_SIZE / WRITE_SIZE counters.
@@ -46,6 +50,37 @@ def copy_probe():
     print("copied 256 MiB x5", flush=True)
 
 
+def host_info():
+    """Host cores and CPU model of the machine the CPU baseline ran on."""
+    info = {"nproc": os.cpu_count()}
+    try:
+        info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        pass
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    info["cpu_model"] = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return info
+
+
+def fk_pose_error(robot, ow, orb, Q):
+    """Largest FP32 forward-kinematics error against the FP64 oracle over Q:
+    link origin position (m) and rotation entries (max |R32 - R64|)."""
+    import numpy as np
+    T32 = robot.fk_batch(Q).astype(np.float64)
+    dp = dr = 0.0
+    for c in range(Q.shape[0]):
+        T64 = ow.fk(orb, Q[c].astype(np.float64))
+        dr = max(dr, float(np.abs(T32[c, :, :9] - T64[:, :9]).max()))
+        dp = max(dp, float(np.abs(T32[c, :, 9:] - T64[:, 9:]).max()))
+    return {"configs": int(Q.shape[0]), "max_position_error_m": dp, "max_rotation_entry_error": dr}
+
+
 def engine_trees(world, robot):
     """The engine's OBB trees of a world's meshes, in the order the oracle
     numbers them (terrains, rigid objects, then the robot's link meshes)."""
@@ -73,7 +108,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
-    ap.add_argument("--dist-configs", type=int, default=8192)
+    ap.add_argument("--dist-configs", type=int, default=65536)
     ap.add_argument("--edges", type=int, default=8192)
     ap.add_argument("--traj", type=int, default=16384)
     ap.add_argument("--copy-probe", action="store_true")
@@ -199,6 +234,27 @@ def main():
                          "fp32_flop_frac": round(flops / kernel_s / 1e12 / FP32_PEAK_TFLOPS, 5) if kernel_s else None})
         print(f"[bench] oracle counters {roofline['oracle_counters']}: {alg_bytes / 1e6:.1f} MB algorithmic, "
               f"{achieved:.1f} GB/s", file=sys.stderr, flush=True)
+        # contact band (1e-6 m): booleans that differ from the oracle, those of
+        # them in the band, and the band's population in a sample (FP64 signed
+        # separations of every pair of the first configurations)
+        gpu_f = dfeas.cpu().numpy()
+        diff = np.nonzero(gpu_f != f_cpu)[0]
+        in_band_diff = 0
+        if len(diff):
+            sep = ow.distance_batch(orb, Q64[diff], signed=True)
+            in_band_diff = int((np.abs(sep).min(axis=1) < 1e-6).sum())
+        nb = min(512, len(Q64))
+        t1 = time.perf_counter()
+        sep = ow.distance_batch(orb, Q64[:nb], signed=True)
+        band_sample = int((np.abs(sep).min(axis=1) < 1e-6).sum())
+        cpu["contact_band"] = {"band_m": 1e-6, "mismatches": int(len(diff)), "mismatches_in_band": in_band_diff,
+                               "fp32_undecided_rechecked": int(st["fp64_rechecks"]),
+                               "sample_configs": nb, "sample_in_band": band_sample,
+                               "sample_seconds": round(time.perf_counter() - t1, 2)}
+        cpu["fk_fp32"] = fk_pose_error(robot, ow, orb, Q[:4096])
+        cpu["host"] = host_info()
+        print(f"[bench] contact band {cpu['contact_band']}, FK {cpu['fk_fp32']}, host {cpu['host']}",
+              file=sys.stderr, flush=True)
 
     secondary = None if args.no_secondary else run_secondary(args, ctx, sc, world, robot)
 
@@ -226,14 +282,18 @@ def main():
 
 
 def _time_dev(fn, reps):
+    """Seconds per call of fn (device calls on torch's current stream): HIP
+    events recorded on that stream around reps calls, after one warm-up."""
     import torch
     fn()
     torch.cuda.synchronize()
-    t = time.perf_counter()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
     for _ in range(reps):
         fn()
+    e1.record()
     torch.cuda.synchronize()
-    return (time.perf_counter() - t) / reps
+    return e0.elapsed_time(e1) * 1e-3 / reps
 
 
 def run_secondary(args, ctx, sc, world, robot):

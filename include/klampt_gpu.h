@@ -82,10 +82,12 @@ typedef struct {
   int32_t spill_cap;     /* HBM spill entries per traversal wave, 0..2^24 (16384); when a
                             wave's spill region is full its oldest pooled tasks are dropped
                             and what depends on them is reported as uncertain (below) */
-  int32_t dist_ipw;      /* distance items per chunk, 1..64 (4) */
+  int32_t dist_ipw;      /* distance items a traversal wave takes per fetch, 1..64 (16) */
   int32_t edge_w0;       /* bisection positions in the first edge-check round, 1..2^20 (16) */
   int32_t sdf_lanes;     /* lanes per waypoint of the SDF penalty kernels: 8, 16, 32, 64 (32) */
-  int32_t prep_group;    /* 1: lane-per-link broad phase where it applies, 0: one lane per
+  int32_t prep_group;    /* 1: lane-per-link broad phase where it applies (groups of 8, 16
+                            or 32 lanes, the fewest that hold the robot's links), 8 / 16 / 32:
+                            groups of at least that many lanes, 0: one lane per
                             configuration (1) */
   int32_t force_recheck; /* test hook: every configuration goes through the FP64 pass (0) */
   int32_t edge_budget_mb; /* device workspace of one edge-check round, MiB, 16..2^20 (1024):
@@ -189,7 +191,12 @@ int kg_geom_collide(kg_geom a, kg_geom b, int* out);
 /* Geometry3D.withinDistance (Python/klampt/src/robotsim.cpp:1685-1692) */
 int kg_geom_within_distance(kg_geom a, kg_geom b, double tol, int* out);
 /* Geometry3D.distance_ext (Python/klampt/src/robotsim.cpp:1777-1819):
- * unsigned TriangleMesh/TriangleMesh distance, 0 when touching. */
+ * unsigned TriangleMesh/TriangleMesh distance, 0 when touching.  settings
+ * (nullable) as DistanceQuerySettings: upperBound caps the search (a result
+ * above it is not searched for), relErr / absErr let the search stop early
+ * once a subtree cannot improve the result by more than either tolerance:
+ * the result d then satisfies D <= d <= max(D (1 + relErr), D + absErr) for
+ * the exact distance D (both 0: exact). */
 int kg_geom_distance(kg_geom a, kg_geom b, const kg_dist_settings* settings, kg_dist_result* out);
 /* Geometry3D.distance_point_ext on an ImplicitSurface
  * (Python/klampt/src/robotsim.cpp:1740-1775, geometry.h:1052-1063): signed

@@ -400,6 +400,15 @@ RobotPlan& get_plan(kg_world_s* w, int ri) {
   P.dev.nlinks = R->n;
   P.dev.nenv = P.nenv;
   P.dev.nself = P.nself;
+  {
+    // parents precede their children
+    std::vector<int> dep(R->n);
+    P.dev.depth = 0;
+    for (int l = 0; l < R->n; l++) {
+      dep[l] = R->links[l].parent < 0 ? 1 : dep[R->links[l].parent] + 1;
+      P.dev.depth = std::max(P.dev.depth, dep[l]);
+    }
+  }
   P.built_version = ver;
   return P;
 }
@@ -472,6 +481,7 @@ struct Layout {
   int cap = 0;                      // per-configuration capacity (single)
   int maxcap = 0;                   // largest per-configuration capacity
   int maxlinks = 0;                 // largest link count over the plans
+  int maxdepth = 0;                 // longest root-to-link chain over the plans
   Tol tol32{}, tol64{};
 };
 
@@ -512,6 +522,8 @@ void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* 
   const int ml = L.multi ? L.maxlinks : L.P->dev.nlinks;
   int group = ml <= 8 ? 8 : (ml <= 16 ? 16 : (ml <= 32 ? 32 : 0));
   if (!c->prm.prep_group || !(fixed || L.multi)) group = 0;
+  // prep_group 8 / 16 / 32: at least that many lanes per configuration
+  if (group && c->prm.prep_group > group) group = c->prm.prep_group;
   // frames are stored only for the serial broad phase (the per-link one
   // writes start records, and the FP64 pass forms its frames from q); link
   // boxes only for the count / scan path
@@ -548,6 +560,10 @@ void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* 
   pa.toff = L.toff;
   pa.ibase = L.ibase;
   pa.N_dev = N_dev;
+  {
+    const int depth = L.multi ? L.maxdepth : L.P->dev.depth;
+    while ((1 << pa.jump_rounds) < depth) pa.jump_rounds++;
+  }
   // timing events cost a few microseconds each on the stream: per-phase ones
   // only at timing level 2, the traversal kernel's own at level 1 and up
   const bool phase_ev = timed && c->timing >= 2, kern_ev = timed && c->timing >= 1;
@@ -1856,7 +1872,8 @@ void forget_pair_world(const kg_ctx_s* c) {
 }  // namespace
 
 namespace {
-void pair_distance(kg_geom_s* a, kg_geom_s* b, float stop_below, double upper, kg_dist_result* out, float* dgpu);
+void pair_distance(kg_geom_s* a, kg_geom_s* b, float stop_below, double upper, kg_dist_result* out, float* dgpu,
+                   double rel_err = 0, double abs_err = 0);
 SdfDev sdf_dev(kg_geom_s* g);
 
 // Point sets against implicit surfaces: a PointCloud's points or a
@@ -2146,7 +2163,7 @@ namespace {
 
 // Per-pair minimum distances for robot `ri` on device buffers (async).
 void distance_dev(kg_world_s* w, int ri, const float* Qf, const double* Qd, int N, float* out, int* out_tri,
-                  float stop_below, float upper, hipStream_t s) {
+                  float stop_below, float upper, hipStream_t s, float rel_err = 0.f, float abs_err = 0.f) {
   kg_ctx_s* c = w->ctx;
   hip_check(hipSetDevice(c->device), "hipSetDevice");
   RobotPlan& P = get_plan(w, ri);
@@ -2171,6 +2188,8 @@ void distance_dev(kg_world_s* w, int ri, const float* Qf, const double* Qd, int 
   da.pair_kind = P.dpk.as<int>();
   da.stop_below = stop_below;
   da.upper = upper;
+  da.rel_err = rel_err;
+  da.abs_err = abs_err;
   da.out = out;
   da.out_tri = out_tri;
   da.chunk_counter = status + ST_CHUNK32;
@@ -2216,7 +2235,8 @@ double pair_tri_dist(const kg_geom_s* a, int ta, const kg_geom_s* b, int tb, dou
 }
 
 // Single pair: distance with closest points via the pair world.
-void pair_distance(kg_geom_s* a, kg_geom_s* b, float stop_below, double upper, kg_dist_result* out, float* dgpu) {
+void pair_distance(kg_geom_s* a, kg_geom_s* b, float stop_below, double upper, kg_dist_result* out, float* dgpu,
+                   double rel_err, double abs_err) {
   kg_ctx_s* c = a->ctx;
   ensure_status_host(c);
   PairWorld& pw = pair_world(c, a, b);
@@ -2229,7 +2249,8 @@ void pair_distance(kg_geom_s* a, kg_geom_s* b, float stop_below, double upper, k
   float* dout = (float*)c->wsOut32.p;
   int* dtri = (int*)((char*)c->wsOut32.p + 16);
   const float up = std::isfinite(upper) ? (float)upper : 3.0e38f;
-  distance_dev(pw.w, 0, nullptr, c->wsG.as<double>(), 1, dout, dtri, stop_below, up, c->stream);
+  distance_dev(pw.w, 0, nullptr, c->wsG.as<double>(), 1, dout, dtri, stop_below, up, c->stream, (float)rel_err,
+               (float)abs_err);
   float d = 0;
   int tri[2] = {-1, -1};
   hip_check(hipMemcpyAsync(&d, dout, sizeof(float), hipMemcpyDeviceToHost, c->stream), "copy");
@@ -2419,7 +2440,9 @@ extern "C" int kg_geom_distance(kg_geom a, kg_geom b, const kg_dist_settings* se
     if (geom_empty(b)) fail(KG_E_EMPTY, "Geometry3D.distance: Other geometry is empty");
     const double m = a->margin + b->margin;
     const double upper = settings ? settings->upperBound + m : INFINITY;
-    pair_distance(a, b, 0.0f, upper, out, nullptr);
+    const double rel = settings ? settings->relErr : 0, abse = settings ? settings->absErr : 0;
+    need(rel >= 0 && abse >= 0, "relErr and absErr must be >= 0");
+    pair_distance(a, b, 0.0f, upper, out, nullptr, rel, abse);
     // margins shrink the distance (negative when the fattened geometries overlap)
     out->d -= m;
   });
@@ -2478,7 +2501,7 @@ void edge_core(kg_world_s* w, int ri, const float* Af, const float* Bf, const do
   const kg_params& prm = c->prm;
   // feasibility workspace bytes per sample (fixed-capacity item layout)
   const int cap = active_links(w->robots[ri]) * P.nenv + P.nself;
-  const bool group = prm.prep_group && n <= 32;
+  const bool group = prm.prep_group && n <= 8;  // the broad phase feasible_core picks with a device count
   size_t per = (size_t)n * sizeof(double) + 1 + 1 + 3 * sizeof(int) + (size_t)cap * sizeof(PairItem) +
                (group ? (size_t)cap * sizeof(ItemRec) : (size_t)n * 18 * sizeof(double));
   const size_t budget_bytes = (size_t)prm.edge_budget_mb << 20;
@@ -2751,6 +2774,7 @@ Layout multi_layout(kg_ctx_s* c, int nb, kg_world_s* const* worlds, const int32_
   L.ncap = caps;
   L.maxcap = *std::max_element(caps_b.begin(), caps_b.end());
   L.maxlinks = *std::max_element(n_b.begin(), n_b.end());
+  for (const PlanDev& d : devplans) L.maxdepth = std::max(L.maxdepth, d.depth);
   L.tol32 = t32;
   L.tol64 = t64;
   (void)s;
@@ -2912,7 +2936,10 @@ extern "C" int kg_ctx_set_params(kg_ctx c, const kg_params* p) {
     need(in(p->edge_w0, 1, 1 << 20), "edge_w0 must be 1..2^20");
     need(p->sdf_lanes == 8 || p->sdf_lanes == 16 || p->sdf_lanes == 32 || p->sdf_lanes == 64,
          "sdf_lanes must be 8, 16, 32 or 64");
-    need(in(p->prep_group, 0, 1) && in(p->force_recheck, 0, 1), "prep_group and force_recheck must be 0 or 1");
+    need(p->prep_group == 0 || p->prep_group == 1 || p->prep_group == 8 || p->prep_group == 16 ||
+             p->prep_group == 32,
+         "prep_group must be 0, 1, 8, 16 or 32");
+    need(in(p->force_recheck, 0, 1), "force_recheck must be 0 or 1");
     need(in(p->edge_budget_mb, 16, 1 << 20), "edge_budget_mb must be 16..2^20");
     c->prm = *p;
   });

@@ -84,7 +84,7 @@ struct PlanDev {
   int nlinks;
   int nenv;
   int nself;
-  int pad_;
+  int depth;              // longest root-to-link chain, in links
 };
 
 // Error-bound parameters of the FP32 narrow phase.
@@ -129,6 +129,8 @@ struct PrepArgs {
   size_t t_rows;       // frame rows of T64 (structure of arrays, see t_row)
   const int* N_dev;    // nullable: device-side configuration count (<= N, the capacity); the
                        // kernels then loop over the configurations on a bounded grid
+  int jump_rounds;     // per-link variant, groups of 16 / 32: pointer-jumping rounds,
+                       // ceil(log2(depth)) over the batch's plans
 };
 hipError_t kgl_prep_count(const PrepArgs& a, hipStream_t s);
 hipError_t kgl_scan(int* counts, int* offsets, int n, DevBuf& tmp, hipStream_t s);
@@ -208,6 +210,7 @@ struct DistArgs {
   const int* pair_kind;
   float stop_below;      // an item stops once its distance is <= this
   float upper;           // initial bound (DistanceQuerySettings::upperBound)
+  float rel_err, abs_err;  // DistanceQuerySettings relErr / absErr (0: exact)
   float* out;            // N x npairs
   int* out_tri;          // nullable: N x npairs x 2 closest triangle pair (pool numbering)
   int* chunk_counter;
@@ -321,7 +324,7 @@ inline kg_params kg_default_params() {
   p.tail_lanes = 16;
   p.tail_reps = 4;
   p.spill_cap = 16384;
-  p.dist_ipw = 4;
+  p.dist_ipw = 16;
   p.edge_w0 = 16;
   p.sdf_lanes = 32;
   p.prep_group = 1;

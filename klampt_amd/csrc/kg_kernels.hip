@@ -450,7 +450,7 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
 // one wavefront per block: a group never spans waves, so the block barriers
 // only order the wave's own LDS traffic and no wave waits for another
 #define KG_PREP_TB 64
-template <int G, bool MULTI>
+template <int G, bool MULTI, bool LOOP>
 __global__ __launch_bounds__(KG_PREP_TB) void k_prep_group(PrepArgs a) {
   constexpr int NG = KG_PREP_TB / G;  // configurations per block
   __shared__ double sF[NG][G][12];
@@ -462,10 +462,12 @@ __global__ __launch_bounds__(KG_PREP_TB) void k_prep_group(PrepArgs a) {
     const int r = blockIdx.x * blockDim.x + t;
     if (r < a.nreset) a.reset[r] = 0;
   }
-  // with a device-side count the blocks loop over the batch (block-uniform
-  // trip count, so the barriers below are reached by the whole block)
-  const int Nc = a.N_dev ? min(*a.N_dev, a.N) : a.N;
-  for (int cb = blockIdx.x * NG; cb < Nc; cb += gridDim.x * NG) {
+  // LOOP (a device-side count): the blocks loop over the batch (block-uniform
+  // trip count, so the barriers below are reached by the whole block); else
+  // one group of configurations per block
+  const int Nc = LOOP ? min(*a.N_dev, a.N) : a.N;
+  const int cb0 = blockIdx.x * NG, step = LOOP ? (int)gridDim.x * NG : 1;
+  for (int cb = cb0; LOOP ? cb < Nc : cb == cb0; cb += step) {
   const int c = cb + g;
   const bool valid = c < Nc;
   const PlanDev& P = MULTI ? a.plans[valid ? a.cfg_plan[c] : 0] : a.plan;
@@ -496,7 +498,7 @@ __global__ __launch_bounds__(KG_PREP_TB) void k_prep_group(PrepArgs a) {
     __syncthreads();
     if (valid && l == 0)
       for (int k = 0; k < n; k++) {
-        const int p = P.links[k].parent;
+        const int p = sAnc[g][k];  // the parent, stored above by link k's lane
         if (p >= 0) {
           X64 Pp, Lx, W;
           x64_load(sF[g][p], Pp);
@@ -510,8 +512,13 @@ __global__ __launch_bounds__(KG_PREP_TB) void k_prep_group(PrepArgs a) {
   } else {
     // world frames: after each round X is the product of the joint frames
     // from the link `anc` (exclusive) down to this one, and `anc` is twice as
-    // far up
-    while (__syncthreads_or(anc >= 0)) {
+    // far up.  The round count, ceil(log2(depth)) over the batch's plans, is
+    // a kernel argument, so the loop is uniform by construction.  A
+    // data-dependent exit (a block-wide OR, or a wave vote) let the compiler,
+    // which lowers the block barriers of a one-wave block to wave barriers,
+    // split the loop into nested loops that lanes left at different rounds,
+    // mixing frames of different rounds and voting over part of the wave.
+    for (int round = 0; round < a.jump_rounds; round++) {
       X64 Y;
       int up = -1;
       if (anc >= 0) {
@@ -605,7 +612,8 @@ __global__ __launch_bounds__(KG_PREP_TB) void k_prep_group(PrepArgs a) {
           it.b = (int16_t)e;
           it.rank = ordered ? (p1 * 65536 + p2) : 0;
           a.items[idx] = it;
-          if (a.recs) write_item_rec(a.recs + idx, X.R, E.X, a.pool.geoms[L.geom], a.pool.geoms[E.geom]);
+          // the link frame from LDS (keeps it out of registers across the loop)
+          if (a.recs) write_item_rec(a.recs + idx, sF[g][l], E.X, a.pool.geoms[L.geom], a.pool.geoms[E.geom]);
         }
       }
     }
@@ -1578,12 +1586,26 @@ hipError_t kgl_prep_single(const PrepArgs& a, hipStream_t s) {
     if (a.N_dev) blocks = min(blocks, kMaxBlocksDev);
     blocks = max(blocks, (a.nreset + TB - 1) / TB);
     const bool m = a.cfg_plan != nullptr;
-    if (a.group == 8)
-      m ? k_prep_group<8, true><<<blocks, TB, 0, s>>>(a) : k_prep_group<8, false><<<blocks, TB, 0, s>>>(a);
-    else if (a.group == 16)
-      m ? k_prep_group<16, true><<<blocks, TB, 0, s>>>(a) : k_prep_group<16, false><<<blocks, TB, 0, s>>>(a);
-    else
-      m ? k_prep_group<32, true><<<blocks, TB, 0, s>>>(a) : k_prep_group<32, false><<<blocks, TB, 0, s>>>(a);
+    if (a.N_dev) {
+      if (a.group == 8)
+        m ? k_prep_group<8, true, true><<<blocks, TB, 0, s>>>(a) : k_prep_group<8, false, true><<<blocks, TB, 0, s>>>(a);
+      else if (a.group == 16)
+        m ? k_prep_group<16, true, true><<<blocks, TB, 0, s>>>(a)
+          : k_prep_group<16, false, true><<<blocks, TB, 0, s>>>(a);
+      else
+        m ? k_prep_group<32, true, true><<<blocks, TB, 0, s>>>(a)
+          : k_prep_group<32, false, true><<<blocks, TB, 0, s>>>(a);
+    } else {
+      if (a.group == 8)
+        m ? k_prep_group<8, true, false><<<blocks, TB, 0, s>>>(a)
+          : k_prep_group<8, false, false><<<blocks, TB, 0, s>>>(a);
+      else if (a.group == 16)
+        m ? k_prep_group<16, true, false><<<blocks, TB, 0, s>>>(a)
+          : k_prep_group<16, false, false><<<blocks, TB, 0, s>>>(a);
+      else
+        m ? k_prep_group<32, true, false><<<blocks, TB, 0, s>>>(a)
+          : k_prep_group<32, false, false><<<blocks, TB, 0, s>>>(a);
+    }
   } else if (a.lds_links) {
     int blocks = (n + KG_PREP_LDS_BLOCK - 1) / KG_PREP_LDS_BLOCK;
     if (a.N_dev) blocks = max(min(blocks, kMaxBlocksDev), (a.nreset + KG_PREP_LDS_BLOCK - 1) / KG_PREP_LDS_BLOCK);
@@ -2042,24 +2064,19 @@ __global__ __launch_bounds__(64) void k_distance(DistArgs a) {
   uint64_t* gsp_base = a.spill + (size_t)blockIdx.x * a.spill_cap;
   const int nitems = a.N * a.npairs;
   const unsigned stop = __float_as_uint(a.stop_below);
+  const float rel1 = 1.0f + a.rel_err, abs_err = a.abs_err;
   unsigned n_box = 0, n_tri = 0;
 
-  for (;;) {
-    if (lane == 0) s_chunk = atomicAdd(a.chunk_counter, 1);
-    __syncthreads();
-    const int i0 = s_chunk * a.ipw;
-    if (i0 >= nitems) break;
-    const int i1 = min(i0 + a.ipw, nitems);
-    s_pend[lane] = 0;
-    __syncthreads();
-    int it = i0, sp = 0, gsp = 0, nl = 0;
-    uint64_t cont = KG_NOTASK;
-
-    // An item's result is published when its last task settles.  If the
-    // traversal dropped tasks of the item (spill region full) and one of
-    // them could still hold a closer pair, the result is the certified lower
-    // bound min(best, smallest bound of a dropped task) and the item is
-    // counted as uncertain.
+  // Items stream through the wave: whenever lanes are free and a slot is
+  // open a new item starts, from the current range of ipw items; a range is
+  // fetched from the global counter only when the previous one is spent, so
+  // no lane waits for the slowest item of a range.
+  s_pend[lane] = 0;
+  __syncthreads();
+  int it = 0, i1 = 0, sp = 0, gsp = 0, nl = 0;
+  bool exhausted = false;
+  uint64_t cont = KG_NOTASK;
+  {
     auto finish = [&](int slot, int delta) {
       if (atomicAdd(&s_pend[slot], delta) + delta == 0) {
         const int item = s_item[slot];
@@ -2167,11 +2184,25 @@ __global__ __launch_bounds__(64) void k_distance(DistArgs a) {
       const int npop = min(nh, sp);
       if (hungry && hp < npop) cont = stk[sp - 1 - hp];
       sp -= npop;
-      // new items on free slots
+      // new items on free slots, from the current range (the next range is
+      // fetched once this one is spent)
       const bool want = cont == KG_NOTASK;
       const uint64_t wm = ballot64(want);
       const uint64_t fm = ballot64(s_pend[lane] == 0);
-      const int navail = min(min(__popcll(wm), __popcll(fm)), i1 - it);
+      const int nwant = min(__popcll(wm), __popcll(fm));
+      if (nwant > 0 && it == i1 && !exhausted) {
+        if (lane == 0) s_chunk = atomicAdd(a.chunk_counter, 1);
+        __syncthreads();
+        const int i0 = __builtin_amdgcn_readfirstlane(s_chunk) * a.ipw;
+        __syncthreads();
+        if (i0 >= nitems) {
+          exhausted = true;
+        } else {
+          it = i0;
+          i1 = min(i0 + a.ipw, nitems);
+        }
+      }
+      const int navail = min(nwant, i1 - it);
       int myitem = -1, myslot = -1;
       const int wp = lane_prefix(wm);
       if (want && wp < navail) {
@@ -2227,7 +2258,10 @@ __global__ __launch_bounds__(64) void k_distance(DistArgs a) {
       const bool working = cont != KG_NOTASK;
       if (!ballot64(working)) {
         if (sp == 0 && gsp == 0) {
-          if (nl == 0) break;
+          if (nl == 0) {
+            if (exhausted && it == i1) break;
+            continue;
+          }
           run_leaves();
         }
         continue;
@@ -2260,7 +2294,11 @@ __global__ __launch_bounds__(64) void k_distance(DistArgs a) {
             const NodeF B = sel(aIsS, O, S);
             n_box++;
             const float lb = obb_lower_bound<float>(A, B, rel);
-            if (lb > best + pad) continue;
+            // no closer pair below (certified bound lb - pad), or none closer
+            // by more than the tolerances (DistanceQuerySettings relErr /
+            // absErr: the result is then within them of the exact distance)
+            const float lbc = lb - pad;
+            if (lbc > best || lbc * rel1 > best || lbc + abs_err > best) continue;
             const bool la = A.ch < 0, lbf = B.ch < 0;
             if (la && lbf) {
               const uint64_t w = mk_leaf(slot, ~A.ch, ~B.ch);

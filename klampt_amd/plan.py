@@ -18,7 +18,13 @@ What is new here is the batched form of the two queries planners make,
 ``'prm'`` validates every candidate edge of a batch of milestones in one call;
 ``'lazyprm'`` (Bohlin and Kavraki's lazy PRM) searches the roadmap of
 unvalidated edges and validates only the edges of the best path, all of them
-in one call per round, until a path survives.
+in one call per round, until a path survives; ``'prm*'`` and ``'lazyprm*'``
+connect each milestone to its k(n) = ceil(e (1 + 1/d) ln n) nearest
+neighbours (Karaman and Frazzoli's PRM* rule, d the dimension) as the roadmap
+grows; ``'rrt'`` grows one tree from the start and one from the goal
+(``bidirectional``, Klampt's default) by a batch of extensions per
+``planMore`` call -- every new node's feasibility in one call, every new edge
+in one call, the connection attempts between the trees in one call.
 """
 import math
 import time
@@ -254,7 +260,7 @@ class RobotCSpace(CSpace):
 
 
 class MotionPlan:
-    """A roadmap planner over a CSpace (``Python/klampt/plan/cspace.py:216-440``).
+    """A planner over a CSpace (``Python/klampt/plan/cspace.py:216-440``).
 
     type 'prm': each ``planMore(n)`` draws n configurations in one batch,
     keeps the feasible ones, links each new milestone to its ``knn`` nearest
@@ -262,18 +268,36 @@ class MotionPlan:
     in one batch.  type 'lazyprm': the same links are kept unvalidated; after
     each batch the shortest start-goal path over the links not known to be
     blocked is found and its unvalidated edges are checked together, until
-    a path survives or none remains.
+    a path survives or none remains.  'prm*' / 'lazyprm*': the same with
+    k(n) = ceil(e (1 + 1/d) ln n) neighbours.  'rrt': each ``planMore(n)``
+    draws n targets, extends the nearest tree node towards each by at most
+    ``perturbationRadius`` (the start and goal trees alternate when
+    ``bidirectional``), checks the new nodes and then their edges in one call
+    each, and tries to link every new node to the other tree's nearest node
+    within ``perturbationRadius`` in one more call; the first link found
+    gives the path.
     """
+
+    TYPES = ("prm", "lazyprm", "prm*", "lazyprm*", "rrt")
 
     def __init__(self, space: CSpace, type: str = None, **options):
         self.space = space
         self.type = type or "lazyprm"
-        if self.type not in ("prm", "lazyprm"):
-            raise ValueError(f"unsupported planner type {self.type!r} (prm, lazyprm)")
+        if self.type not in self.TYPES:
+            raise ValueError(f"unsupported planner type {self.type!r} ({', '.join(self.TYPES)})")
         self.knn = int(options.pop("knn", 10))
         self.connectionThreshold = float(options.pop("connectionThreshold", math.inf))
+        # RRT: the longest extension (Klampt's perturbationRadius) and whether
+        # a second tree grows from the goal
+        self.perturbationRadius = float(options.pop("perturbationRadius", 0.1))
+        self.bidirectional = bool(int(options.pop("bidirectional", 1)))
         if options:
             raise ValueError(f"unknown planner options {sorted(options)}")
+        self.lazy = self.type in ("lazyprm", "lazyprm*")
+        self.star = self.type.endswith("*")
+        self.rrt = self.type == "rrt"
+        self.parent = None       # RRT: parent milestone index (-1 at a root)
+        self.tree = None         # RRT: 0 start tree, 1 goal tree
         self.V = np.zeros((0, len(space.bound)))
         self.edges = {}          # (i, j), i < j -> True (valid), False (blocked), None (not checked)
         self.start = self.goal = None
@@ -296,13 +320,27 @@ class MotionPlan:
         self.start, self.goal = 0, 1
         self.edges = {}
         self.path_idx = None
+        if self.rrt:
+            self.parent = np.array([-1, -1])
+            self.tree = np.array([0, 1])
+            self._rrt_try_connect(np.array([0]))
+            return
         self._connect(np.array([0, 1]))
 
     def addMilestone(self, x):
+        if self.rrt:
+            raise ValueError("the 'rrt' planner does not take milestones")
         self.V = np.vstack([self.V, np.asarray(x, float)[None]])
         k = len(self.V) - 1
         self._connect(np.array([k]))
         return k
+
+    def _k(self):
+        """Neighbours per milestone: knn, or the PRM* rule for the '*' types."""
+        if not self.star:
+            return self.knn
+        d = self.V.shape[1]
+        return max(1, int(math.ceil(math.e * (1.0 + 1.0 / d) * math.log(max(2, len(self.V))))))
 
     def _connect(self, new):
         """Candidate links of the milestones in `new` to their nearest
@@ -311,7 +349,7 @@ class MotionPlan:
             return
         from scipy.spatial import cKDTree
         tree = cKDTree(self.V)
-        k = min(self.knn + 1, len(self.V))
+        k = min(self._k() + 1, len(self.V))
         d, nb = tree.query(self.V[new], k=k)
         d, nb = np.atleast_2d(d), np.atleast_2d(nb)
         cand = []
@@ -323,7 +361,7 @@ class MotionPlan:
                 if e not in self.edges:
                     self.edges[e] = None
                     cand.append(e)
-        if self.type == "prm":
+        if not self.lazy:
             self._validate(cand)
 
     def _validate(self, edges):
@@ -337,12 +375,105 @@ class MotionPlan:
         self._stats["edges_checked"] += len(edges)
         self._stats["check_calls"] += 1
 
+    def _steer(self, near, target):
+        """Configurations from `near` towards `target`, at most
+        perturbationRadius away (row-wise)."""
+        d = target - near
+        L = np.linalg.norm(d, axis=1)
+        scale = np.minimum(1.0, self.perturbationRadius / np.maximum(L, 1e-300))
+        return near + d * scale[:, None]
+
+    def _rrt_try_connect(self, new):
+        """Links each node in `new` to the nearest node of the other tree
+        when that one is within perturbationRadius (goal node for a
+        unidirectional plan); all candidate links checked in one call."""
+        if self.path_idx is not None or len(new) == 0:
+            return
+        from scipy.spatial import cKDTree
+        pairs = []
+        for t in (0, 1):
+            mine = new[self.tree[new] == t]
+            other = np.nonzero(self.tree == 1 - t)[0]
+            if len(mine) == 0 or len(other) == 0:
+                continue
+            d, j = cKDTree(self.V[other]).query(self.V[mine], k=1)
+            for a, dist, jj in zip(mine, np.atleast_1d(d), np.atleast_1d(j)):
+                if dist <= self.perturbationRadius:
+                    pairs.append((int(a), int(other[jj])))
+        if not pairs:
+            return
+        P = np.asarray(pairs)
+        ok = self.space.visible_batch(self.V[P[:, 0]], self.V[P[:, 1]])
+        self._stats["edges_checked"] += len(P)
+        self._stats["check_calls"] += 1
+        for (a, b), v in zip(pairs, ok):
+            self.edges[(min(a, b), max(a, b))] = bool(v)
+        hits = [p for p, v in zip(pairs, ok) if v]
+        if hits:
+            def root_path(i):
+                out = [i]
+                while self.parent[out[-1]] >= 0:
+                    out.append(int(self.parent[out[-1]]))
+                return out
+            best = None
+            for a, b in hits:
+                pa, pb = root_path(a), root_path(b)
+                ps, pg = (pa, pb) if self.tree[a] == 0 else (pb, pa)
+                cand = ps[::-1] + pg
+                cost = sum(np.linalg.norm(self.V[x] - self.V[y]) for x, y in zip(cand[:-1], cand[1:]))
+                if best is None or cost < best[0]:
+                    best = (cost, cand)
+            self.path_idx = best[1]
+
+    def _rrt_more(self, iterations):
+        """One batch of RRT extensions: `iterations` targets drawn at once,
+        each extends the nearest node of the tree it is given to (the trees
+        take turns when bidirectional), feasibility and edge checks batched."""
+        from scipy.spatial import cKDTree
+        n = int(iterations)
+        Xt = self.space.sample_batch(n)
+        trees = np.arange(n) % 2 if self.bidirectional else np.zeros(n, int)
+        near = np.zeros(n, int)
+        for t in (0, 1):
+            sel = np.nonzero(trees == t)[0]
+            nodes = np.nonzero(self.tree == t)[0]
+            if len(sel) == 0:
+                continue
+            _, j = cKDTree(self.V[nodes]).query(Xt[sel], k=1)
+            near[sel] = nodes[np.atleast_1d(j)]
+        Xn = self._steer(self.V[near], Xt)
+        ok = self.space.feasible_batch(Xn)
+        self._stats["samples"] += n
+        idx = np.nonzero(ok)[0]
+        if len(idx) == 0:
+            return
+        vis = self.space.visible_batch(self.V[near[idx]], Xn[idx])
+        self._stats["edges_checked"] += len(idx)
+        self._stats["check_calls"] += 1
+        keep = idx[vis]
+        if len(keep) == 0:
+            return
+        first = len(self.V)
+        self.V = np.vstack([self.V, Xn[keep]])
+        self.parent = np.r_[self.parent, near[keep]]
+        self.tree = np.r_[self.tree, trees[keep] if self.bidirectional else np.zeros(len(keep), int)]
+        for k, p in zip(range(first, len(self.V)), near[keep]):
+            self.edges[(int(min(k, p)), int(max(k, p)))] = True
+        self._rrt_try_connect(np.arange(first, len(self.V)))
+
     def planMore(self, iterations):
         """Draws `iterations` configurations in one batch and extends the
-        roadmap; then looks for a path (validating lazily for 'lazyprm')."""
+        roadmap (or the trees); then looks for a path (validating lazily for
+        the lazy types)."""
         if self.start is None:
             raise RuntimeError("setEndpoints must be called first")
         t = time.perf_counter()
+        if self.rrt:
+            if self.path_idx is None:
+                self._rrt_more(iterations)
+            self._stats["milestones"] = len(self.V)
+            self._stats["time_s"] += time.perf_counter() - t
+            return
         Q = self.space.sample_batch(int(iterations))
         ok = self.space.feasible_batch(Q)
         self._stats["samples"] += len(Q)
@@ -374,7 +505,7 @@ class MotionPlan:
         return path[::-1]
 
     def _search(self):
-        if self.type == "prm":
+        if not self.lazy:
             self.path_idx = self._shortest(lambda v: v is True)
             return
         while True:

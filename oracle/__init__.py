@@ -70,6 +70,9 @@ def lib():
             "or_sdf_penalty": (None, [_vp, ctypes.c_int, _d, _i, _d, _d, _d, ctypes.c_int, _d, _i,
                                       ctypes.c_double, _d, _d, ctypes.c_double, ctypes.c_int]),
             "or_mesh_set_obb": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float), ctypes.c_int]),
+            "or_hull_distance": (ctypes.c_double, [_d, ctypes.c_int, _d, ctypes.c_int, _d, ctypes.c_int, _d, _d, _d,
+                                                   _i]),
+            "or_hull_support": (None, [_d, ctypes.c_int, _d, _d, _d]),
             "or_feasible_counted": (ctypes.c_int, [_vp, ctypes.c_int, _d, ctypes.c_int, _u8,
                                                    ctypes.POINTER(ctypes.c_int64), ctypes.c_int]),
         }
@@ -346,3 +349,25 @@ def world_from_scene(scene):
     r = w.add_robot(scene.robot)
     w.init_default()
     return w, r
+
+
+def hull_distance(A, xa, B, xb, xa2=None):
+    """GJK distance between the convex hull of points A placed by xa (and,
+    when xa2 is given, also by xa2: the hull of both copies) and the hull of
+    points B placed by xb.  Returns (d, cp_a, cp_b, iterations)."""
+    A, B = _dd(A).reshape(-1, 3), _dd(B).reshape(-1, 3)
+    X = _dd(np.r_[xa, xa2] if xa2 is not None else xa)
+    xb = _dd(xb)
+    ca, cb = np.zeros(3), np.zeros(3)
+    it = np.zeros(1, np.int32)
+    d = lib().or_hull_distance(_p(A, _d), A.shape[0], _p(X, _d), 2 if xa2 is not None else 1, _p(B, _d),
+                               B.shape[0], _p(xb, _d), _p(ca, _d), _p(cb, _d), _p(it, _i))
+    return d, ca, cb, int(it[0])
+
+
+def hull_support(A, x, d):
+    """Support point of the hull of points A placed by x in direction d."""
+    A = _dd(A).reshape(-1, 3)
+    out = np.zeros(3)
+    lib().or_hull_support(_p(A, _d), A.shape[0], _p(_dd(x), _d), _p(_dd(d), _d), _p(out, _d))
+    return out

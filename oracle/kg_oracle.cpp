@@ -46,6 +46,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <initializer_list>
 #include <limits>
 #include <thread>
 #include <vector>
@@ -737,6 +738,188 @@ static bool obb_collide_count(const Mesh& A, const Xf& XA, const Mesh& B, const 
   return false;
 }
 
+// ------------------------------------------------------------------ convex hulls
+//
+// Distance between convex hulls of point sets by GJK (Gilbert, Johnson and
+// Keerthi 1988) with the closest point of each simplex to the origin found
+// by Voronoi regions (Ericson, Real-Time Collision Detection 5.1.5-5.1.6),
+// carrying barycentric weights so the closest points on both hulls follow.
+// A hull is given by points and one or two rigid placements (two: the hull
+// of both placed copies, the swept volume KineTrajOpt checks,
+// kinetrajopt.py:603-611 setConvexHullGroup).
+
+struct HullSet {
+  const double* P;   // n x 3 local points
+  int n;
+  Xf X[2];
+  int nx;            // placements (1 or 2)
+};
+
+static V3 hull_support(const HullSet& h, V3 d, int* which) {
+  double best = -kInf;
+  V3 out = mk(0, 0, 0);
+  for (int k = 0; k < h.nx; k++) {
+    const V3 dl = mk(dot(mk(h.X[k].R[0], h.X[k].R[1], h.X[k].R[2]), d),
+                     dot(mk(h.X[k].R[3], h.X[k].R[4], h.X[k].R[5]), d),
+                     dot(mk(h.X[k].R[6], h.X[k].R[7], h.X[k].R[8]), d));
+    const double off = dot(mk(h.X[k].t[0], h.X[k].t[1], h.X[k].t[2]), d);  // the placement's translation
+    for (int i = 0; i < h.n; i++) {
+      const V3 p = mk(h.P[3 * i], h.P[3 * i + 1], h.P[3 * i + 2]);
+      const double v = dot(p, dl) + off;
+      if (v > best) {
+        best = v;
+        out = apply(h.X[k], p);
+        if (which) *which = k;
+      }
+    }
+  }
+  return out;
+}
+
+// Closest point to the origin of the simplex w[0..n), as weights lam over
+// its vertices; the simplex is reduced to the vertices with nonzero weight.
+static V3 simplex_closest(V3* w, V3* a, V3* b, int& n, double* lam) {
+  auto keep = [&](std::initializer_list<int> idx) {
+    V3 w2[4], a2[4], b2[4];
+    double l2[4];
+    int m = 0;
+    for (int i : idx) {
+      w2[m] = w[i]; a2[m] = a[i]; b2[m] = b[i]; l2[m] = lam[i]; m++;
+    }
+    for (int i = 0; i < m; i++) { w[i] = w2[i]; a[i] = a2[i]; b[i] = b2[i]; lam[i] = l2[i]; }
+    n = m;
+  };
+  if (n == 1) {
+    lam[0] = 1;
+    return w[0];
+  }
+  if (n == 2) {
+    const V3 ab = w[1] - w[0];
+    const double t = -dot(w[0], ab), den = dot(ab, ab);
+    if (t <= 0) { lam[0] = 1; keep({0}); return w[0]; }
+    if (t >= den) { lam[1] = 1; keep({1}); return w[0]; }
+    lam[1] = t / den; lam[0] = 1 - lam[1];
+    return lam[0] * w[0] + lam[1] * w[1];
+  }
+  if (n == 3) {
+    // closest point on triangle (w0, w1, w2) to the origin, regions as in
+    // closest_pt_tri with p = 0
+    const V3 A = w[0], B = w[1], C = w[2];
+    const V3 ab = B - A, ac = C - A, ap = mk(0, 0, 0) - A;
+    const double d1 = dot(ab, ap), d2 = dot(ac, ap);
+    if (d1 <= 0 && d2 <= 0) { lam[0] = 1; keep({0}); return A; }
+    const V3 bp = mk(0, 0, 0) - B;
+    const double d3 = dot(ab, bp), d4 = dot(ac, bp);
+    if (d3 >= 0 && d4 <= d3) { lam[1] = 1; keep({1}); return B; }
+    const double vc = d1 * d4 - d3 * d2;
+    if (vc <= 0 && d1 >= 0 && d3 <= 0) {
+      const double v = d1 / (d1 - d3);
+      lam[0] = 1 - v; lam[1] = v; keep({0, 1});
+      return A + v * ab;
+    }
+    const V3 cp = mk(0, 0, 0) - C;
+    const double d5 = dot(ab, cp), d6 = dot(ac, cp);
+    if (d6 >= 0 && d5 <= d6) { lam[2] = 1; keep({2}); return C; }
+    const double vb = d5 * d2 - d1 * d6;
+    if (vb <= 0 && d2 >= 0 && d6 <= 0) {
+      const double v = d2 / (d2 - d6);
+      lam[0] = 1 - v; lam[2] = v; keep({0, 2});
+      return A + v * ac;
+    }
+    const double va = d3 * d6 - d5 * d4;
+    if (va <= 0 && (d4 - d3) >= 0 && (d5 - d6) >= 0) {
+      const double v = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+      lam[1] = 1 - v; lam[2] = v; keep({1, 2});
+      return B + v * (C - B);
+    }
+    const double den = 1.0 / (va + vb + vc);
+    const double v = vb * den, ww = vc * den;
+    lam[0] = 1 - v - ww; lam[1] = v; lam[2] = ww;
+    return A + v * ab + ww * ac;
+  }
+  // tetrahedron: inside -> the origin; else the closest of the faces the
+  // origin lies outside of
+  const int F[4][3] = {{0, 1, 2}, {0, 3, 1}, {0, 2, 3}, {1, 3, 2}};
+  const int opp[4] = {3, 2, 1, 0};
+  bool inside = true;
+  double bestd = kInf;
+  V3 best = mk(0, 0, 0);
+  V3 bw[4], ba[4], bb[4];
+  double bl[4];
+  int bn = 0;
+  for (int f = 0; f < 4; f++) {
+    const V3 p0 = w[F[f][0]], p1 = w[F[f][1]], p2 = w[F[f][2]], q = w[opp[f]];
+    const V3 nrm = cross(p1 - p0, p2 - p0);
+    const double so = dot(mk(0, 0, 0) - p0, nrm), sq = dot(q - p0, nrm);
+    if (so * sq >= 0) continue;  // origin on the same side as the fourth vertex
+    inside = false;
+    V3 tw[4] = {p0, p1, p2}, ta[4] = {a[F[f][0]], a[F[f][1]], a[F[f][2]]}, tb[4] = {b[F[f][0]], b[F[f][1]], b[F[f][2]]};
+    double tl[4] = {0, 0, 0, 0};
+    int tn = 3;
+    const V3 c = simplex_closest(tw, ta, tb, tn, tl);
+    const double dd = dot(c, c);
+    if (dd < bestd) {
+      bestd = dd;
+      best = c;
+      bn = tn;
+      for (int i = 0; i < tn; i++) { bw[i] = tw[i]; ba[i] = ta[i]; bb[i] = tb[i]; bl[i] = tl[i]; }
+    }
+  }
+  if (inside) {
+    // barycentric weights of the origin
+    const double vol = dot(w[1] - w[0], cross(w[2] - w[0], w[3] - w[0]));
+    const V3 o = mk(0, 0, 0);
+    lam[1] = dot(o - w[0], cross(w[2] - w[0], w[3] - w[0])) / vol;
+    lam[2] = dot(w[1] - w[0], cross(o - w[0], w[3] - w[0])) / vol;
+    lam[3] = dot(w[1] - w[0], cross(w[2] - w[0], o - w[0])) / vol;
+    lam[0] = 1 - lam[1] - lam[2] - lam[3];
+    return o;
+  }
+  n = bn;
+  for (int i = 0; i < bn; i++) { w[i] = bw[i]; a[i] = ba[i]; b[i] = bb[i]; lam[i] = bl[i]; }
+  return best;
+}
+
+// GJK distance between hulls A and B (0 when they overlap), with closest
+// points (when separated).
+static double hull_distance(const HullSet& A, const HullSet& B, V3* cpa, V3* cpb, int* iters) {
+  V3 w[4], a[4], b[4];
+  double lam[4] = {1, 0, 0, 0};
+  int n = 1;
+  a[0] = hull_support(A, mk(1, 0, 0), nullptr);
+  b[0] = hull_support(B, mk(-1, 0, 0), nullptr);
+  w[0] = a[0] - b[0];
+  V3 v = w[0];
+  double dist = std::sqrt(dot(v, v));
+  int it = 0;
+  for (; it < 200; it++) {
+    const double vv = dot(v, v);
+    if (vv <= 1e-30) { dist = 0; break; }
+    const V3 na = hull_support(A, mk(-v.x, -v.y, -v.z), nullptr);
+    const V3 nb = hull_support(B, v, nullptr);
+    const V3 nw = na - nb;
+    // converged: the new support point does not get closer than v by more
+    // than the relative tolerance
+    if (vv - dot(v, nw) <= 1e-14 * vv) break;
+    bool dup = false;
+    for (int i = 0; i < n; i++)
+      if (w[i].x == nw.x && w[i].y == nw.y && w[i].z == nw.z) dup = true;
+    if (dup) break;
+    w[n] = nw; a[n] = na; b[n] = nb; lam[n] = 0; n++;
+    v = simplex_closest(w, a, b, n, lam);
+    if (n == 4) { dist = 0; break; }  // origin inside the tetrahedron
+    const double nd = std::sqrt(dot(v, v));
+    if (nd >= dist - 1e-15 * dist && it > 0) { dist = nd; break; }
+    dist = nd;
+  }
+  if (iters) *iters = it;
+  V3 pa = mk(0, 0, 0), pb = mk(0, 0, 0);
+  for (int i = 0; i < n; i++) { pa = pa + lam[i] * a[i]; pb = pb + lam[i] * b[i]; }
+  if (cpa) *cpa = pa;
+  if (cpb) *cpb = pb;
+  return dist;
+}
+
 static void world_aabb(const Mesh& m, const Xf& X, double* lo, double* hi) {
   double c[3], e[3];
   for (int k = 0; k < 3; k++) {
@@ -1387,6 +1570,27 @@ static bool is_feasible_counted(const World& w, int r, const double* q, int64_t*
   out[3] = cs.box;
   out[4] = cs.tri;
   return !hit;
+}
+
+// Distance between the hull of points A (na x 3, placed by XA[0] and, when
+// nxa == 2, also by XA[1]: the hull of both copies) and the hull of points B
+// (placed by XB); closest points cpa, cpb (world).  Returns the distance.
+double or_hull_distance(const double* A, int na, const double* XA, int nxa, const double* B, int nb,
+                        const double* XB, double* cpa, double* cpb, int* iters) {
+  HullSet ha{A, na, {from12(XA), from12(nxa == 2 ? XA + 12 : XA)}, nxa == 2 ? 2 : 1};
+  HullSet hb{B, nb, {from12(XB), from12(XB)}, 1};
+  V3 pa, pb;
+  const double d = hull_distance(ha, hb, &pa, &pb, iters);
+  if (cpa) { cpa[0] = pa.x; cpa[1] = pa.y; cpa[2] = pa.z; }
+  if (cpb) { cpb[0] = pb.x; cpb[1] = pb.y; cpb[2] = pb.z; }
+  return d;
+}
+
+// Support point of the hull of points A placed by X (12) in direction d.
+void or_hull_support(const double* A, int na, const double* X, const double* d, double* out) {
+  HullSet h{A, na, {from12(X), from12(X)}, 1};
+  const V3 p = hull_support(h, mk(d[0], d[1], d[2]), nullptr);
+  out[0] = p.x; out[1] = p.y; out[2] = p.z;
 }
 
 int or_mesh_set_obb(void* wp, int g, const float* nodes, int nn) {

@@ -110,3 +110,28 @@ def test_collision_margins_single_pair(ctx):
     w2.add_robot(engine.Robot.from_spec(ctx, sc.robot))
     with pytest.raises(KlamptGPUError):
         w2.feasible_batch(S.uniform_configs(sc.robot, 4, seed=1))
+
+
+def test_distance_relerr_abserr_bounds(ctx):
+    """DistanceQuerySettings relErr / absErr: the search may stop early, and
+    the result d stays within the larger tolerance of the exact distance D:
+    D <= d <= max(D (1 + relErr), D + absErr) (FP32 slack 1e-5 m), with
+    closest points that realise d."""
+    rng = np.random.default_rng(21)
+    a = S.ellipsoid_mesh([0.12, 0.08, 0.1], edge=0.01)
+    b = S.ellipsoid_mesh([0.2, 0.1, 0.15], edge=0.01)
+    ga, gb = engine.Geom.trimesh(ctx, a.verts, a.tris), engine.Geom.trimesh(ctx, b.verts, b.tris)
+    for k in range(24):
+        xa = np.r_[S.random_rotation(rng).T.ravel(), rng.uniform(-0.4, 0.4, 3)]
+        xb = np.r_[S.random_rotation(rng).T.ravel(), rng.uniform(-0.4, 0.4, 3)]
+        ga.set_transform(xa[:9], xa[9:])
+        gb.set_transform(xb[:9], xb[9:])
+        D = ga.distance(gb).d
+        for rel, ab in ((0.2, 0.0), (0.0, 0.02), (0.5, 0.05)):
+            r = ga.distance(gb, relErr=rel, absErr=ab)
+            # the search stops once either tolerance is met
+            assert D - TOL <= r.d <= max(D * (1 + rel), D + ab) + TOL, k
+            if r.d > 1e-6:
+                assert abs(np.linalg.norm(np.array(r.cp1) - np.array(r.cp2)) - r.d) < 1e-5, k
+    with pytest.raises(Exception):
+        ga.distance(gb, relErr=-1.0)

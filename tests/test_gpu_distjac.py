@@ -102,3 +102,46 @@ def test_link_position_jacobian_matches_finite_differences():
         R.setConfig(list(qm))
         b = np.asarray(link.getWorldPosition(pl))
         assert np.allclose(J[:, j], (a - b) / (2 * h), atol=1e-6), j
+
+
+def test_min_distance_autograd_value_and_gradient(scene):
+    """autodiff.MinDistance: per configuration the smallest pair distance
+    (the oracle's, within 1e-5 m, or the bound when every pair is beyond it)
+    and dL/dq equal to FP64 central differences of the oracle's minimum
+    where the closest pair is unique and away from contact."""
+    import torch
+    from klampt_amd import autodiff
+    sc, w, ow, orb = scene
+    Q = S.uniform_configs(sc.robot, 96, seed=84)
+    D = ow.distance_batch(orb, Q.astype(np.float64))
+    # a bound at the median of the positive minima: about half of the
+    # separated configurations lie beyond it
+    dmin = D.min(axis=1)
+    bound = float(np.median(dmin[dmin > 0]))
+    md = autodiff.MinDistance(w, 0, upperBound=bound)
+    Qt = torch.from_numpy(Q).cuda().requires_grad_(True)
+    val = md(Qt)
+    val.sum().backward()
+    v = val.detach().cpu().numpy()
+    G = Qt.grad.cpu().numpy()
+    want = np.minimum(dmin, bound)
+    assert np.abs(v - want).max() <= 1e-5
+    beyond = dmin > bound + 1e-5
+    print(f"bound {bound:.4f} m: {int(beyond.sum())} of {len(Q)} configurations beyond it")
+    assert beyond.any() and (G[beyond] == 0).all()
+    n, h = sc.robot.n, 1e-5
+    Q32 = Q.astype(np.float64)
+    checked = 0
+    for c in range(len(Q)):
+        srt = np.sort(D[c])
+        if not (0.01 < srt[0] < bound - 0.01 and srt[1] - srt[0] > 1e-3):
+            continue
+        P = np.repeat(Q32[c:c + 1], 2 * n, axis=0)
+        for j in range(n):
+            P[2 * j, j] += h
+            P[2 * j + 1, j] -= h
+        M = ow.distance_batch(orb, P).min(axis=1)
+        fd = (M[0::2] - M[1::2]) / (2 * h)
+        assert np.abs(G[c] - fd).max() < 2e-3, c
+        checked += 1
+    assert checked >= 10

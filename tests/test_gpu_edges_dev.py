@@ -165,3 +165,27 @@ def test_edge_dev_graph_replay(ctx, soup):
             got = out.cpu().numpy()
             assert np.array_equal(got, eager[k]), f"replay {rep}, set {k}"
     print("graph replays equal the eager results")
+
+
+def test_long_robot_edges_vs_oracle(ctx, soup):
+    """Edge rounds for robots of more than 8 links (a device-count broad
+    phase in groups of 16 and 32 lanes, world frames by pointer jumping): an
+    11-link chain and a 20-link branched robot in the soup, and the same
+    segments with the one-lane-per-configuration broad phase."""
+    sc, _, _, _ = soup
+    robots, _ = S.hetero_robots(seed=0, nrobots=4)
+    for spec in (robots[3], robots[2]):
+        assert spec.n > 8
+        scene = S.Scene(spec, terrains=sc.terrains)
+        w, r = engine.world_from_scene(ctx, scene)
+        ow, orb = oracle.world_from_scene(scene)
+        A, B = S.edge_segments(spec, 256, eps=0.02, smin=5, smax=200, seed=17)
+        print(f"{spec.n}-link robot, 256 segments", flush=True)
+        fu = w.edge_batch(A, B, eps=0.02, norm_inf=True)
+        fo, ns, _ = ow.edge_batch(orb, A, B, eps=0.02, norm_inf=True)
+        bad = np.nonzero(fu != fo.astype(np.float32))[0]
+        print(f"  free {(fo < 0).mean():.2f}, mismatches {len(bad)}")
+        _check_band(ow, orb, A, B, 0.02, ns, bad)
+        with engine_params(ctx, prep_group=0):
+            fs = w.edge_batch(A, B, eps=0.02, norm_inf=True)
+        assert np.array_equal(fs, fu)

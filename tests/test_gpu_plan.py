@@ -62,7 +62,7 @@ def test_added_constraint_is_applied(boxes):
     assert np.array_equal(space.feasible_batch(Q), base & (Q[:, 0] > 0))
 
 
-@pytest.mark.parametrize("ptype", ["prm", "lazyprm"])
+@pytest.mark.parametrize("ptype", ["prm", "lazyprm", "prm*", "lazyprm*", "rrt"])
 def test_planned_path_passes_the_oracle(boxes, ptype):
     sc, W, R, ow, orb = boxes
     space = plan.RobotCSpace(R, W, eps=0.02)
@@ -79,10 +79,11 @@ def test_planned_path_passes_the_oracle(boxes, ptype):
     i, j = cand[int(np.argmax(blocked))]
     assert blocked.any()
     start, goal = Qf[i].astype(float), Qf[j].astype(float)
-    mp = plan.MotionPlan(space, type=ptype, knn=15)
+    opts = {"perturbationRadius": 0.5} if ptype == "rrt" else ({} if ptype.endswith("*") else {"knn": 15})
+    mp = plan.MotionPlan(space, type=ptype, **opts)
     mp.setEndpoints(start, goal)
-    for _ in range(8):
-        mp.planMore(4096)
+    for _ in range(8 if ptype != "rrt" else 64):
+        mp.planMore(4096 if ptype != "rrt" else 512)
         if mp.getPath():
             break
     path = mp.getPath()

@@ -49,12 +49,13 @@ def test_feasibility_tests_and_bounds():
         sp.addFeasibilityTest(3)
 
 
-@pytest.mark.parametrize("ptype", ["prm", "lazyprm"])
+@pytest.mark.parametrize("ptype", ["prm", "lazyprm", "prm*", "lazyprm*", "rrt"])
 def test_planner_path_is_feasible(ptype):
     sp = Disk(3)
     start, goal = [0.1, 0.5], [0.9, 0.5]
     assert not sp.visible(start, goal)
-    mp = plan.MotionPlan(sp, type=ptype, knn=8)
+    opts = {"perturbationRadius": 0.15} if ptype == "rrt" else {"knn": 8}
+    mp = plan.MotionPlan(sp, type=ptype, **opts)
     mp.setEndpoints(start, goal)
     for _ in range(10):
         mp.planMore(300)
@@ -92,3 +93,38 @@ def test_errors():
         plan.MotionPlan(sp, bogus=1)
     with pytest.raises(RuntimeError):
         plan.MotionPlan(sp).planMore(10)
+
+
+def test_prm_star_neighbour_count_grows_with_the_roadmap():
+    sp = Disk(5)
+    mp = plan.MotionPlan(sp, type="prm*")
+    mp.setEndpoints([0.1, 0.5], [0.9, 0.5])
+    k0 = mp._k()
+    mp.planMore(400)
+    k1 = mp._k()
+    n, d = len(mp.V), 2
+    assert k1 == math.ceil(math.e * 1.5 * math.log(n)) and k1 > k0
+
+
+@pytest.mark.parametrize("bidirectional", [1, 0])
+def test_rrt_tree_edges_are_short_and_valid(bidirectional):
+    sp = Disk(6)
+    mp = plan.MotionPlan(sp, type="rrt", perturbationRadius=0.1, bidirectional=bidirectional)
+    mp.setEndpoints([0.1, 0.5], [0.9, 0.5])
+    for _ in range(40):
+        mp.planMore(64)
+        if mp.getPath():
+            break
+    path = mp.getPath()
+    assert path is not None
+    P = np.asarray(path)
+    assert np.all(np.linalg.norm(P[1:] - P[:-1], axis=1) <= 0.1 + 1e-9)
+    assert sp.visible_batch(P[:-1], P[1:]).all() and all(sp.feasible(q) for q in path)
+    # every tree node's parent is within the radius and in the same tree
+    par = mp.parent
+    for i in range(2, len(mp.V)):
+        assert mp.tree[i] == mp.tree[par[i]] and np.linalg.norm(mp.V[i] - mp.V[par[i]]) <= 0.1 + 1e-9
+    if not bidirectional:
+        assert np.all(mp.tree[2:] == 0)
+    with pytest.raises(ValueError):
+        mp.addMilestone([0.2, 0.2])
