@@ -437,10 +437,9 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
 // (G = 8, 16 or 32, at least the robot's link count) take one configuration
 // each, of a single plan or of the configuration's own plan in a
 // heterogeneous batch (fixed-capacity item layout).  Each lane computes its
-// joint frame T0 J(q) in FP64; the group composes the world frames in LDS,
-// for G = 8 by its first lane down the link list (parents precede their
-// children), for longer robots by pointer jumping over the parent links
-// (ceil(log2 depth) rounds of all lanes);
+// joint frame T0 J(q) in FP64; the group composes the world frames in LDS by
+// pointer jumping over the parent links (ceil(log2 depth) rounds of all
+// lanes: 3 for a 7-link chain, against 6 dependent products down the list);
 // each lane then forms its link box, and the group emits the items: a link's
 // environment pairs by its own lane, the self pairs round-robin over the
 // group, each with its start record.  Item positions inside a configuration
@@ -494,46 +493,30 @@ __global__ __launch_bounds__(KG_PREP_TB) void k_prep_group(PrepArgs a) {
     x64_store(sF[g][l], X);
     sAnc[g][l] = anc;
   }
-  if constexpr (G == 8) {
+  // world frames by pointer jumping: after each round X is the product of the
+  // joint frames from the link `anc` (exclusive) down to this one, and `anc`
+  // is twice as far up.  The round count, ceil(log2(depth)) over the batch's
+  // plans, is a kernel argument, so the loop is uniform by construction.  A
+  // data-dependent exit (a block-wide OR, or a wave vote) let the compiler,
+  // which lowers the block barriers of a one-wave block to wave barriers,
+  // split the loop into nested loops that lanes left at different rounds,
+  // mixing frames of different rounds and voting over part of the wave.
+  for (int round = 0; round < a.jump_rounds; round++) {
+    X64 Y;
+    int up = -1;
     __syncthreads();
-    if (valid && l == 0)
-      for (int k = 0; k < n; k++) {
-        const int p = sAnc[g][k];  // the parent, stored above by link k's lane
-        if (p >= 0) {
-          X64 Pp, Lx, W;
-          x64_load(sF[g][p], Pp);
-          x64_load(sF[g][k], Lx);
-          x64_mul(Pp, Lx, W);
-          x64_store(sF[g][k], W);
-        }
-      }
+    if (anc >= 0) {
+      x64_load(sF[g][anc], Y);
+      up = sAnc[g][anc];
+    }
     __syncthreads();
-    if (mine) x64_load(sF[g][l], X);
-  } else {
-    // world frames: after each round X is the product of the joint frames
-    // from the link `anc` (exclusive) down to this one, and `anc` is twice as
-    // far up.  The round count, ceil(log2(depth)) over the batch's plans, is
-    // a kernel argument, so the loop is uniform by construction.  A
-    // data-dependent exit (a block-wide OR, or a wave vote) let the compiler,
-    // which lowers the block barriers of a one-wave block to wave barriers,
-    // split the loop into nested loops that lanes left at different rounds,
-    // mixing frames of different rounds and voting over part of the wave.
-    for (int round = 0; round < a.jump_rounds; round++) {
-      X64 Y;
-      int up = -1;
-      if (anc >= 0) {
-        x64_load(sF[g][anc], Y);
-        up = sAnc[g][anc];
-      }
-      __syncthreads();
-      if (anc >= 0) {
-        X64 Z;
-        x64_mul(Y, X, Z);
-        X = Z;
-        anc = up;
-        x64_store(sF[g][l], X);
-        sAnc[g][l] = anc;
-      }
+    if (anc >= 0) {
+      X64 Z;
+      x64_mul(Y, X, Z);
+      X = Z;
+      anc = up;
+      x64_store(sF[g][l], X);
+      sAnc[g][l] = anc;
     }
   }
   if (mine) {
