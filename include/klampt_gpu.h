@@ -39,6 +39,7 @@ typedef struct kg_ctx_s* kg_ctx;
 typedef struct kg_geom_s* kg_geom;
 typedef struct kg_robot_s* kg_robot;
 typedef struct kg_world_s* kg_world;
+typedef struct kg_serve_s* kg_serve;
 
 /* DistanceQuerySettings (Python/klampt/src/geometry.h:631-638) */
 typedef struct {
@@ -381,6 +382,34 @@ int kg_feasible_multi(int nbatch, const kg_world* worlds, const int32_t* robot_i
                       const float* Q, uint8_t* feasible);
 int kg_feasible_multi_dev(int nbatch, const kg_world* worlds, const int32_t* robot_indices, const int32_t* counts,
                           const float* dQ, uint8_t* d_feasible, void* stream);
+/* Serving pipeline: a heterogeneous feasibility batch (as kg_feasible_multi_dev:
+ * robot robot_indices[b] of worlds[b] at counts[b] configurations, dQ /
+ * d_feasible the concatenations) and nedge edge batches (as kg_edge_batch_dev:
+ * edge_counts[g] segments of robot edge_robots[g] of edge_worlds[g], dA / dB /
+ * d_first_u the concatenations, one eps, norm and max_steps for all), captured
+ * by the library into one HIP graph.  No Klampt counterpart: it batches the
+ * SingleRobotCSpace::IsFeasible and PathChecker(a,b)->IsVisible() calls of a
+ * planner iteration (Cpp/Planning/RobotCSpace.cpp:786,835-837) into one graph
+ * launch.  The buffers are the caller's and stay fixed: write new inputs into
+ * them, then kg_serve_run replays the graph on `stream` (ordered after the
+ * context's earlier calls, as any _dev call).  The pipeline is issued once at
+ * creation to size the context's workspaces, then captured.  kg_serve_run
+ * recaptures first when the graph is stale: a world or geometry of the
+ * pipeline changed, kg_params changed (kg_autotune, kg_ctx_set_params), or a
+ * context workspace was reallocated by a larger call of another shape.
+ * kg_ctx_status_dev after a run reports the whole pipeline (spill flags ORed,
+ * uncertain / undecided and re-check counts summed).  A feasibility batch
+ * whose pair items exceed the fixed-capacity layout (2 GiB) is rejected
+ * (KG_E_INVALID); the pipelines and calls of one context share its
+ * workspaces, so they run one at a time in stream order.  kg_serve_captures
+ * reports how many captures the pipeline has taken. */
+int kg_serve_create(int nbatch, const kg_world* worlds, const int32_t* robot_indices, const int32_t* counts,
+                    const float* dQ, uint8_t* d_feasible, int nedge, const kg_world* edge_worlds,
+                    const int32_t* edge_robots, const int32_t* edge_counts, const float* dA, const float* dB,
+                    double eps, int norm_inf, int max_steps, float* d_first_u, kg_serve* out);
+int kg_serve_run(kg_serve serve, void* stream);
+int kg_serve_captures(kg_serve serve, int32_t* out);
+int kg_serve_destroy(kg_serve serve);
 /* Autotuner: times the traversal settings (configurations per wave chunk,
  * leaf batch threshold, LDS pool capacity, waves per CU) on the given batch
  * (device Q; the device is synchronized first, so dQ may come from any

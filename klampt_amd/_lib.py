@@ -114,6 +114,11 @@ SIGNATURES = {
     "kg_feasible_multi": [_c_int, _pvp, _pi, _pi, _pf, _pu8],
     "kg_feasible_multi_dev": [_c_int, _pvp, _pi, _pi, _vp, _vp, _vp],
     "kg_autotune": [_c_int, _pvp, _pi, _pi, _vp, _pi, _pf],
+    "kg_serve_create": [_c_int, _pvp, _pi, _pi, _vp, _vp, _c_int, _pvp, _pi, _pi, _vp, _vp, _c_double, _c_int,
+                        _c_int, _vp, _pvp],
+    "kg_serve_run": [_vp, _vp],
+    "kg_serve_captures": [_vp, _pi],
+    "kg_serve_destroy": [_vp],
     "kg_ctx_set_tuning": [_vp, _c_int, _c_int, _c_int, _c_int],
     "kg_ctx_kernel_times": [_vp, _c_int, _pi64, _pi],
     "kg_ctx_set_timing": [_vp, _c_int],

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -2501,7 +2502,7 @@ void edge_core(kg_world_s* w, int ri, const float* Af, const float* Bf, const do
   const kg_params& prm = c->prm;
   // feasibility workspace bytes per sample (fixed-capacity item layout)
   const int cap = active_links(w->robots[ri]) * P.nenv + P.nself;
-  const bool group = prm.prep_group && n <= 8;  // the broad phase feasible_core picks with a device count
+  const bool group = prm.prep_group && n <= 32;  // the broad phase feasible_core picks
   size_t per = (size_t)n * sizeof(double) + 1 + 1 + 3 * sizeof(int) + (size_t)cap * sizeof(PairItem) +
                (group ? (size_t)cap * sizeof(ItemRec) : (size_t)n * 18 * sizeof(double));
   const size_t budget_bytes = (size_t)prm.edge_budget_mb << 20;
@@ -2942,6 +2943,239 @@ extern "C" int kg_ctx_set_params(kg_ctx c, const kg_params* p) {
     need(in(p->force_recheck, 0, 1), "force_recheck must be 0 or 1");
     need(in(p->edge_budget_mb, 16, 1 << 20), "edge_budget_mb must be 16..2^20");
     c->prm = *p;
+  });
+}
+
+// ---------------------------------------------------------------- serving graph
+//
+// The batching form of the engine: a heterogeneous feasibility batch (as
+// kg_feasible_multi_dev) and the edge batches of several robots (as
+// kg_edge_batch_dev each) on fixed device buffers, issued once to size every
+// workspace and then captured into one HIP graph that kg_serve_run replays.
+// Many SingleRobotCSpace::IsFeasible and PathChecker(a,b)->IsVisible() calls
+// of a planner's iteration (Cpp/Planning/RobotCSpace.cpp:786,835-837) become
+// one graph launch; the edge rounds compact the live segments on the device.
+struct kg_serve_s {
+  kg_ctx_s* c = nullptr;
+  std::vector<kg_world_s*> fw, ew;
+  std::vector<int32_t> fri, fcnt, eri, ecnt;
+  const float* dQ = nullptr;
+  uint8_t* dfeas = nullptr;
+  const float* dA = nullptr;
+  const float* dB = nullptr;
+  float* dU = nullptr;
+  double eps = 0.01;
+  int norm_inf = 0;
+  int max_steps = 0;
+  DevBuf stat;  // ST_WORDS ints: the status words of the whole pipeline
+  hipStream_t cs = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  // what the captured graph baked in
+  uint64_t gen = 0;
+  kg_params prm{};
+  std::vector<int> versions;
+  int captures = 0;
+};
+
+namespace {
+
+void serve_drop_graph(kg_serve_s* sv) {
+  if (sv->exec) (void)hipGraphExecDestroy(sv->exec);
+  if (sv->graph) (void)hipGraphDestroy(sv->graph);
+  sv->exec = nullptr;
+  sv->graph = nullptr;
+}
+
+// The pipeline on stream s: feasibility batch, then each edge group, each
+// part's status words folded into sv->stat.
+void serve_issue(kg_serve_s* sv, hipStream_t s) {
+  kg_ctx_s* c = sv->c;
+  bool first = true;
+  if (!sv->fw.empty()) {
+    size_t N = 0;
+    Layout L = multi_layout(c, (int)sv->fw.size(), sv->fw.data(), sv->fri.data(), sv->fcnt.data(), s, &N);
+    need(L.ncap * sizeof(PairItem) <= (2048ull << 20), "serving feasibility batch exceeds the item capacity limit");
+    if (N > 0) {
+      feasible_core(c, L, sv->dQ, nullptr, (int)N, sv->dfeas, nullptr, nullptr, nullptr, s, nullptr, false);
+      hip_check(kgl_status_accum(c->wsStatus.as<int>(), sv->stat.as<int>(), ST_SPILL, ST_UNC, ST_REDO, 1, s),
+                "status");
+      first = false;
+    }
+  }
+  size_t qo = 0, so = 0;
+  for (size_t g = 0; g < sv->ew.size(); g++) {
+    kg_world_s* w = sv->ew[g];
+    const int n = w->robots[sv->eri[g]]->n;
+    const int cnt = sv->ecnt[g];
+    if (cnt > 0) {
+      edge_core(w, sv->eri[g], sv->dA + qo, sv->dB + qo, nullptr, nullptr, cnt, sv->eps, sv->norm_inf,
+                edge_max_pos(w->robots[sv->eri[g]], sv->eps, sv->norm_inf, sv->max_steps), sv->dU + so, nullptr, s);
+      hip_check(kgl_status_accum(c->wsEStat.as<int>(), sv->stat.as<int>(), ST_SPILL, ST_UNC, ST_REDO,
+                                 first ? 1 : 0, s),
+                "status");
+      first = false;
+    }
+    qo += (size_t)cnt * n;
+    so += (size_t)cnt;
+  }
+  if (first) hip_check(kgl_fill_int(sv->stat.as<int>(), ST_WORDS, 0, s), "clear status");
+}
+
+// Plan versions of every (world, robot) of the pipeline; rebuilding a plan
+// here (the world changed) moves g_devbuf_gen when its tables grow.
+std::vector<int> serve_versions(kg_serve_s* sv) {
+  std::vector<int> v;
+  for (size_t b = 0; b < sv->fw.size(); b++) v.push_back(get_plan(sv->fw[b], sv->fri[b]).built_version);
+  for (size_t g = 0; g < sv->ew.size(); g++) v.push_back(get_plan(sv->ew[g], sv->eri[g]).built_version);
+  return v;
+}
+
+// Warm-up issue (sizes every workspace), then the capture.  A capture during
+// which a workspace grew (the warm-up and the capture disagree) is dropped
+// and taken again.
+void serve_capture(kg_serve_s* sv) {
+  kg_ctx_s* c = sv->c;
+  hip_check(hipSetDevice(c->device), "hipSetDevice");
+  serve_drop_graph(sv);
+  for (int attempt = 0;; attempt++) {
+    need(attempt < 3, "serving pipeline: workspaces kept growing during capture");
+    {
+      StreamScope scope(c, sv->cs);
+      serve_issue(sv, sv->cs);
+    }
+    hip_check(hipStreamSynchronize(sv->cs), "sync");
+    const std::vector<int> ver = serve_versions(sv);
+    const uint64_t g0 = g_devbuf_gen.load();
+    hip_check(hipStreamBeginCapture(sv->cs, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
+    try {
+      serve_issue(sv, sv->cs);
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(sv->cs, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    hip_check(hipStreamEndCapture(sv->cs, &sv->graph), "hipStreamEndCapture");
+    if (g_devbuf_gen.load() != g0) {
+      serve_drop_graph(sv);
+      continue;
+    }
+    hip_check(hipGraphInstantiate(&sv->exec, sv->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    sv->gen = g0;
+    sv->prm = c->prm;
+    sv->versions = ver;
+    sv->captures++;
+    // the capture stream's last work (the warm-up) is done; the graph's
+    // launches are ordered by kg_serve_run's stream scope
+    return;
+  }
+}
+
+}  // namespace
+
+extern "C" int kg_serve_create(int nbatch, const kg_world* worlds, const int32_t* robot_indices,
+                               const int32_t* counts, const float* dQ, uint8_t* d_feasible, int nedge,
+                               const kg_world* edge_worlds, const int32_t* edge_robots, const int32_t* edge_counts,
+                               const float* dA, const float* dB, double eps, int norm_inf, int max_steps,
+                               float* d_first_u, kg_serve* out) {
+  return guard([&] {
+    need(out != nullptr, "NULL argument");
+    *out = nullptr;
+    need(nbatch >= 0 && nedge >= 0 && nbatch + nedge >= 1, "empty pipeline");
+    need(nbatch == 0 || (worlds && robot_indices && counts), "NULL feasibility batch arrays");
+    need(nedge == 0 || (edge_worlds && edge_robots && edge_counts), "NULL edge batch arrays");
+    need(eps > 0 && std::isfinite(eps), "eps must be positive");  // motionplanning.cpp:727-728
+    need(max_steps >= 0, "negative max_steps");
+    kg_ctx_s* c = nbatch ? worlds[0]->ctx : edge_worlds[0]->ctx;
+    std::unique_ptr<kg_serve_s> sv(new kg_serve_s);
+    sv->c = c;
+    size_t N = 0;
+    for (int b = 0; b < nbatch; b++) {
+      need(worlds[b] && worlds[b]->ctx == c, "worlds belong to different contexts");
+      need(robot_indices[b] >= 0 && robot_indices[b] < (int)worlds[b]->robots.size(), "robot index out of range");
+      need(counts[b] >= 0, "negative configuration count");
+      sv->fw.push_back(worlds[b]);
+      sv->fri.push_back(robot_indices[b]);
+      sv->fcnt.push_back(counts[b]);
+      N += counts[b];
+    }
+    need(N == 0 || (dQ && d_feasible), "NULL feasibility buffer");
+    size_t nseg = 0;
+    for (int g = 0; g < nedge; g++) {
+      need(edge_worlds[g] && edge_worlds[g]->ctx == c, "worlds belong to different contexts");
+      need(edge_robots[g] >= 0 && edge_robots[g] < (int)edge_worlds[g]->robots.size(), "robot index out of range");
+      need(edge_counts[g] >= 0, "negative segment count");
+      sv->ew.push_back(edge_worlds[g]);
+      sv->eri.push_back(edge_robots[g]);
+      sv->ecnt.push_back(edge_counts[g]);
+      nseg += edge_counts[g];
+      // rejects an unbounded joint range when max_steps is 0
+      (void)edge_max_pos(edge_worlds[g]->robots[edge_robots[g]], eps, norm_inf, max_steps);
+    }
+    need(nseg == 0 || (dA && dB && d_first_u), "NULL edge buffer");
+    sv->dQ = dQ;
+    sv->dfeas = d_feasible;
+    sv->dA = dA;
+    sv->dB = dB;
+    sv->dU = d_first_u;
+    sv->eps = eps;
+    sv->norm_inf = norm_inf ? 1 : 0;
+    sv->max_steps = max_steps;
+    hip_check(hipSetDevice(c->device), "hipSetDevice");
+    hip_check(sv->stat.ensure(ST_WORDS * sizeof(int)), "hipMalloc(status)");
+    hip_check(hipStreamCreateWithFlags(&sv->cs, hipStreamNonBlocking), "hipStreamCreate");
+    try {
+      serve_capture(sv.get());
+    } catch (...) {
+      serve_drop_graph(sv.get());
+      (void)hipStreamDestroy(sv->cs);
+      sv->stat.release();
+      throw;
+    }
+    *out = sv.release();
+  });
+}
+
+extern "C" int kg_serve_run(kg_serve sv, void* stream) {
+  return guard([&] {
+    need(sv != nullptr, "NULL argument");
+    kg_ctx_s* c = sv->c;
+    hip_check(hipSetDevice(c->device), "hipSetDevice");
+    // recapture when what the graph baked in has changed: a plan (the world
+    // or its geometry), the engine parameters, or a workspace address
+    const std::vector<int> ver = serve_versions(sv);
+    if (ver != sv->versions || std::memcmp(&sv->prm, &c->prm, sizeof(kg_params)) != 0 ||
+        g_devbuf_gen.load() != sv->gen)
+      serve_capture(sv);
+    hipStream_t s = (hipStream_t)stream;
+    StreamScope scope(c, s);
+    hip_check(hipGraphLaunch(sv->exec, s), "hipGraphLaunch");
+    c->stat_src = sv->stat.as<int>();
+    c->has_batch = true;
+    c->last_tr_slot = -1;
+    c->last_phase_ev = false;
+    hip_check(hipEventRecord(c->evStat, s), "hipEventRecord");
+  });
+}
+
+extern "C" int kg_serve_captures(kg_serve sv, int32_t* out) {
+  return guard([&] {
+    need(sv && out, "NULL argument");
+    *out = sv->captures;
+  });
+}
+
+extern "C" int kg_serve_destroy(kg_serve sv) {
+  return guard([&] {
+    if (!sv) return;
+    (void)hipSetDevice(sv->c->device);
+    (void)hipStreamSynchronize(sv->cs);
+    if (sv->c->done_valid) (void)hipEventSynchronize(sv->c->evDone);
+    serve_drop_graph(sv);
+    (void)hipStreamDestroy(sv->cs);
+    sv->stat.release();
+    delete sv;
   });
 }
 

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
 #include <string>
 #include <vector>
 
@@ -16,6 +17,12 @@ struct BuiltTree {
 };
 void kg_build_obb_tree(const double* V, int nv, const int32_t* T, int nt, BuiltTree& out);
 
+// Count of device-buffer frees in the process (a buffer growing frees its
+// old block): a captured HIP graph holds workspace addresses, so its owner
+// recaptures when the count has moved since the capture.  A first allocation
+// invalidates no address and is not counted.
+inline std::atomic<uint64_t> g_devbuf_gen{0};
+
 // Grow-only device buffer.
 struct DevBuf {
   void* p = nullptr;
@@ -23,6 +30,7 @@ struct DevBuf {
   hipError_t ensure(size_t n) {
     if (n <= bytes) return hipSuccess;
     if (p) {
+      g_devbuf_gen.fetch_add(1);
       // kernels still queued on any stream may read the old block
       hipError_t e = hipDeviceSynchronize();
       if (e != hipSuccess) return e;
@@ -39,6 +47,7 @@ struct DevBuf {
   }
   void release() {
     if (p) {
+      g_devbuf_gen.fetch_add(1);
       (void)hipDeviceSynchronize();
       (void)hipFree(p);
     }
@@ -129,8 +138,8 @@ struct PrepArgs {
   size_t t_rows;       // frame rows of T64 (structure of arrays, see t_row)
   const int* N_dev;    // nullable: device-side configuration count (<= N, the capacity); the
                        // kernels then loop over the configurations on a bounded grid
-  int jump_rounds;     // per-link variant, groups of 16 / 32: pointer-jumping rounds,
-                       // ceil(log2(depth)) over the batch's plans
+  int jump_rounds;     // per-link variant: pointer-jumping rounds, ceil(log2(depth)) over
+                       // the batch's plans
 };
 hipError_t kgl_prep_count(const PrepArgs& a, hipStream_t s);
 hipError_t kgl_scan(int* counts, int* offsets, int n, DevBuf& tmp, hipStream_t s);
@@ -310,6 +319,9 @@ hipError_t kgl_sdf_penalty(const LinkDesc* links, int n, const SdfDev& g, const 
 // Copies the status words [0] spill flag, [1] uncertain count, [2] FP64
 // re-checks of a batch into out[0..3] (out[3] = 0).
 hipError_t kgl_status_copy(const int* status, int w_spill, int w_unc, int w_redo, int32_t* out, hipStream_t s);
+// acc[w] = st[w] (first) or acc[w] OR= / += st[w] (spill flag / counts), for
+// the status words w_spill, w_unc, w_redo of one part of a pipeline
+hipError_t kgl_status_accum(const int* st, int* acc, int w_spill, int w_unc, int w_redo, int first, hipStream_t s);
 
 // ---- C-ABI object layouts
 

@@ -1646,6 +1646,21 @@ hipError_t kgl_status_copy(const int* status, int w_spill, int w_unc, int w_redo
   return hipGetLastError();
 }
 
+__global__ void k_status_accum(const int* __restrict__ st, int* __restrict__ acc, int w_spill, int w_unc, int w_redo,
+                               int first) {
+  if (threadIdx.x == 0) {
+    const int sp = st[w_spill], un = st[w_unc], rd = st[w_redo];
+    acc[w_spill] = first ? sp : (acc[w_spill] | sp);
+    acc[w_unc] = first ? un : acc[w_unc] + un;
+    acc[w_redo] = first ? rd : acc[w_redo] + rd;
+  }
+}
+
+hipError_t kgl_status_accum(const int* st, int* acc, int w_spill, int w_unc, int w_redo, int first, hipStream_t s) {
+  k_status_accum<<<1, 64, 0, s>>>(st, acc, w_spill, w_unc, w_redo, first);
+  return hipGetLastError();
+}
+
 // ------------------------------------------------------------ scratch of the allocation-free entry point
 //
 // kg_feasible_batch_dev_noalloc carves every workspace of a batch (status

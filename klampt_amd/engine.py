@@ -588,5 +588,57 @@ def autotune(worlds, robots, counts, dQ):
     return tuple(int(x) for x in ch) + (ms.value,)
 
 
+class Serve:
+    """kg_serve_*: a feasibility batch over several robots and the edge
+    batches of several robots, captured by the library into one HIP graph and
+    replayed by run().  The device buffers are fixed at creation: write new
+    configurations / segment endpoints into ``Q``, ``A``, ``B`` (float32 CUDA
+    tensors, concatenated per batch / group) and call run(); results land in
+    ``feasible`` (uint8) and ``first_u`` (float32)."""
+
+    def __init__(self, ctx, feas=(), edges=(), eps=0.01, norm_inf=False, max_steps=0, device="cuda"):
+        import torch
+        self.ctx = ctx
+        fw = [w for w, _, _ in feas]
+        fr, fc = [int(r) for _, r, _ in feas], [int(c) for _, _, c in feas]
+        ew = [w for w, _, _ in edges]
+        er, ec = [int(r) for _, r, _ in edges], [int(c) for _, _, c in edges]
+        nq = sum(c * w.robots[r].n for w, r, c in feas)
+        ne = sum(c * w.robots[r].n for w, r, c in edges)
+        self.Q = torch.zeros(max(1, nq), dtype=torch.float32, device=device)
+        self.feasible = torch.zeros(max(1, sum(fc)), dtype=torch.uint8, device=device)
+        self.A = torch.zeros(max(1, ne), dtype=torch.float32, device=device)
+        self.B = torch.zeros(max(1, ne), dtype=torch.float32, device=device)
+        self.first_u = torch.zeros(max(1, sum(ec)), dtype=torch.float32, device=device)
+        self.feas_counts, self.edge_counts = fc, ec
+        nb, W, R, C = _multi_args(fw, fr, fc)
+        ne_, EW, ER, EC = _multi_args(ew, er, ec)
+        torch.cuda.synchronize()
+        h = ctypes.c_void_p()
+        call("kg_serve_create", nb, W, _p(R, _pi), _p(C, _pi), _tp(self.Q), _tp(self.feasible), ne_, EW,
+             _p(ER, _pi), _p(EC, _pi), _tp(self.A), _tp(self.B), float(eps), 1 if norm_inf else 0, int(max_steps),
+             _tp(self.first_u), ctypes.byref(h))
+        self.h = h
+
+    def run(self):
+        call("kg_serve_run", self.h, _torch_stream())
+
+    def captures(self):
+        out = ctypes.c_int32()
+        call("kg_serve_captures", self.h, ctypes.byref(out))
+        return out.value
+
+    def close(self):
+        if self.h:
+            call("kg_serve_destroy", self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def set_tuning(ctx, cpw=0, leaf_min=0, scap=0, waves_per_cu=0):
     call("kg_ctx_set_tuning", ctx.h, cpw, leaf_min, scap, waves_per_cu)
