@@ -320,6 +320,35 @@ int kg_distance_jacobian_batch(kg_world w, int robot_index, const float* Q, int 
                                float* cp_out, float* grad_out);
 int kg_distance_jacobian_batch_dev(kg_world w, int robot_index, const float* dQ, int N, float dcheck, float* d_out,
                                    float* cp_out, float* grad_out, void* stream);
+/* KineTrajOpt sweep terms (Python/klampt/plan/kinetrajopt/kinetrajopt.py:
+ * 530-634, find_collision_pair with compute_sweep_collision): for each segment
+ * k (configurations Q0[k] -> Q1[k], FP64 as Klampt's Config), link l and
+ * obstacle o, the distance between the convex hull of link l's geometry
+ * placed at both configurations (setConvexHullGroup of the two copies of
+ * convert('ConvexHull'), :603-611) and the convex hull of the obstacle, by GJK
+ * in FP64, and the add_sweep terms (:556-583).  Obstacles are the world's
+ * terrains then its rigid objects, in world ID order, as convex hulls of their
+ * points at their placements (kinetrajopt.py:495-501 takes the terrains; the
+ * rigid objects follow them here); kg_world_num_obstacles counts them.  Per
+ * query q = (k * nlinks + l) * nobs + o:
+ *   d_out[q]          distance; +inf when not below dcheck (DistanceQuerySettings
+ *                     upperBound = dcheck, kept only if d < dcheck) or when
+ *                     either side has no geometry; 0 when the hulls overlap
+ *                     (then no terms: Klampt's penetration depth is not formed);
+ *   info_out[q][16]   alpha, normal (from the obstacle to the swept hull),
+ *                     pswept and pobs (the closest points), p0 and p1 (the
+ *                     support points of the hull at Q0[k] and at Q1[k] in
+ *                     -normal); zeros without terms;
+ *   coef_out[q][2][n] normal . J_l(Q0[k], p0) and normal . J_l(Q1[k], p1) (the
+ *                     lin_coef1 / lin_coef2 of the sweep constraint, :454-461,
+ *                     whose weights are alpha and 1 - alpha).
+ * With Q0[k] == Q1[k] the swept hull is the link's hull at that
+ * configuration: the point terms of the hull geometries. */
+int kg_world_num_obstacles(kg_world w, int* out);
+int kg_sweep_batch(kg_world w, int robot_index, const double* Q0, const double* Q1, int N, double dcheck,
+                   double* d_out, double* info_out, double* coef_out);
+int kg_sweep_batch_dev(kg_world w, int robot_index, const double* dQ0, const double* dQ1, int N, double dcheck,
+                       double* d_out, double* info_out, double* coef_out, void* stream);
 /* Per-pair minimum distance for the kg_world_pair_list pairs (AnyCollisionQuery
  * ::Distance, Cpp/Planning/PlannerSettings.cpp:109-115): out[N][npairs], 0 when
  * touching. */

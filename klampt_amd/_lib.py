@@ -117,6 +117,9 @@ SIGNATURES = {
     "kg_serve_create": [_c_int, _pvp, _pi, _pi, _vp, _vp, _c_int, _pvp, _pi, _pi, _vp, _vp, _c_double, _c_int,
                         _c_int, _vp, _pvp],
     "kg_serve_run": [_vp, _vp],
+    "kg_world_num_obstacles": [_vp, _pi],
+    "kg_sweep_batch": [_vp, _c_int, _pd, _pd, _c_int, _c_double, _pd, _pd, _pd],
+    "kg_sweep_batch_dev": [_vp, _c_int, _vp, _vp, _c_int, _c_double, _vp, _vp, _vp, _vp],
     "kg_serve_captures": [_vp, _pi],
     "kg_serve_destroy": [_vp],
     "kg_ctx_set_tuning": [_vp, _c_int, _c_int, _c_int, _c_int],

@@ -200,6 +200,7 @@ void geom_release(kg_geom_s* g) {
   if (!g || --g->refs > 0) return;
   g->dvals.release();
   g->dpts.release();
+  g->dv64.release();
   // mesh data stays in the context pool until the context is destroyed
   delete g;
 }
@@ -2331,6 +2332,132 @@ void distance_jacobian_dev(kg_world_s* w, int ri, const float* dQ, int N, float 
   hip_check(kgl_dist_jac(a, s), "dist_jac");
 }
 }  // namespace
+
+// ---------------------------------------------------------------- KineTrajOpt sweep terms
+namespace {
+
+// FP64 points of a mesh or point cloud on the device (its convex hull's
+// support map reads them); nullptr for an SDF or empty geometry.
+const double* hull_points(kg_geom_s* g, int& n) {
+  n = 0;
+  if (!g || g->kind == 1 || g->V.empty() || (g->kind == 0 && geom_empty(g))) return nullptr;
+  n = (int)(g->V.size() / 3);
+  if (!g->dv64.p) {
+    hip_check(g->dv64.ensure(g->V.size() * sizeof(double)), "hipMalloc");
+    hip_check(hipMemcpy(g->dv64.p, g->V.data(), g->V.size() * sizeof(double), hipMemcpyHostToDevice), "copy points");
+  }
+  return g->dv64.as<double>();
+}
+
+// Hull table of a robot's plan: its links' hulls (link frames come from FK),
+// then the obstacles in world ID order, terrains then rigid objects, each at
+// its placement.  Rebuilt with the plan.
+void ensure_hulls(kg_world_s* w, int ri, RobotPlan& P) {
+  if (P.hull_version == P.built_version) return;
+  kg_robot_s* R = w->robots[ri];
+  std::vector<HullRef> T;
+  for (int l = 0; l < R->n; l++) {
+    HullRef h{};
+    h.V = hull_points(R->link_geoms[l], h.nv);
+    T.push_back(h);
+  }
+  auto add_obs = [&](kg_geom_s* g, const Xf64& X) {
+    HullRef h{};
+    h.V = hull_points(g, h.nv);
+    for (int i = 0; i < 9; i++) h.X[i] = X.R[i];
+    for (int i = 0; i < 3; i++) h.X[9 + i] = X.t[i];
+    T.push_back(h);
+  };
+  for (kg_geom_s* g : w->terrains) add_obs(g, g->X);
+  for (auto& o : w->objects) add_obs(o.first, o.second);
+  P.nobs = (int)T.size() - R->n;
+  upload(P.dhull, T);
+  P.hull_version = P.built_version;
+}
+
+void sweep_dev(kg_world_s* w, int ri, const double* dQ0, const double* dQ1, int N, double dcheck, double* d_out,
+               double* info_out, double* coef_out, hipStream_t s) {
+  need(w != nullptr, "world is NULL");
+  need(ri >= 0 && ri < (int)w->robots.size(), "robot index out of range");
+  need(N >= 0, "negative batch size");
+  need(dcheck > 0, "dcheck must be positive");
+  kg_ctx_s* c = w->ctx;
+  RobotPlan& P = get_plan(w, ri);
+  ensure_hulls(w, ri, P);
+  const int n = w->robots[ri]->n;
+  if (N == 0 || P.nobs == 0) return;
+  need(dQ0 && dQ1 && d_out && info_out && coef_out, "NULL buffer");
+  hip_check(c->wsT64.ensure((size_t)2 * N * n * 12 * sizeof(double)), "hipMalloc(T64)");
+  double* T0 = c->wsT64.as<double>();
+  double* T1 = T0 + (size_t)N * n * 12;
+  hip_check(kgl_fk64(P.dev.links, n, nullptr, dQ0, N, T0, nullptr, 0, s), "fk64");
+  hip_check(kgl_fk64(P.dev.links, n, nullptr, dQ1, N, T1, nullptr, 0, s), "fk64");
+  SweepArgs a{};
+  a.links = P.dev.links;
+  a.n = n;
+  a.N = N;
+  a.O = P.nobs;
+  a.T0 = T0;
+  a.T1 = T1;
+  a.lhull = P.dhull.as<HullRef>();
+  a.ohull = P.dhull.as<HullRef>() + n;
+  a.dcheck = dcheck;
+  a.d_out = d_out;
+  a.info_out = info_out;
+  a.coef_out = coef_out;
+  hip_check(kgl_sweep(a, s), "sweep");
+}
+
+}  // namespace
+
+extern "C" int kg_world_num_obstacles(kg_world w, int* out) {
+  return guard([&] {
+    need(w && out, "NULL argument");
+    *out = (int)(w->terrains.size() + w->objects.size());
+  });
+}
+
+extern "C" int kg_sweep_batch_dev(kg_world w, int ri, const double* dQ0, const double* dQ1, int N, double dcheck,
+                                  double* d_out, double* info_out, double* coef_out, void* stream) {
+  return guard([&] {
+    need(w != nullptr, "world is NULL");
+    hip_check(hipSetDevice(w->ctx->device), "hipSetDevice");
+    hipStream_t s = (hipStream_t)stream;
+    StreamScope scope(w->ctx, s);
+    sweep_dev(w, ri, dQ0, dQ1, N, dcheck, d_out, info_out, coef_out, s);
+  });
+}
+
+extern "C" int kg_sweep_batch(kg_world w, int ri, const double* Q0, const double* Q1, int N, double dcheck,
+                              double* d_out, double* info_out, double* coef_out) {
+  return guard([&] {
+    need(w != nullptr, "world is NULL");
+    need(ri >= 0 && ri < (int)w->robots.size(), "robot index out of range");
+    need(N >= 0, "negative batch size");
+    if (N == 0) return;
+    need(Q0 && Q1 && d_out && info_out && coef_out, "NULL buffer");
+    kg_ctx_s* c = w->ctx;
+    hip_check(hipSetDevice(c->device), "hipSetDevice");
+    StreamScope scope(c, c->stream);
+    const int n = w->robots[ri]->n;
+    const size_t O = w->terrains.size() + w->objects.size();
+    const size_t nq = (size_t)N * n, nr = (size_t)N * n * O;
+    if (nr == 0) return;
+    hip_check(c->wsEA.ensure(2 * nq * sizeof(double)), "hipMalloc");
+    hip_check(c->wsEB.ensure(nr * (1 + 16 + 2 * (size_t)n) * sizeof(double)), "hipMalloc");
+    double* dQ = c->wsEA.as<double>();
+    double* dd = c->wsEB.as<double>();
+    double* di = dd + nr;
+    double* dc = di + nr * 16;
+    hip_check(hipMemcpyAsync(dQ, Q0, nq * sizeof(double), hipMemcpyHostToDevice, c->stream), "copy");
+    hip_check(hipMemcpyAsync(dQ + nq, Q1, nq * sizeof(double), hipMemcpyHostToDevice, c->stream), "copy");
+    sweep_dev(w, ri, dQ, dQ + nq, N, dcheck, dd, di, dc, c->stream);
+    hip_check(hipMemcpyAsync(d_out, dd, nr * sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy");
+    hip_check(hipMemcpyAsync(info_out, di, nr * 16 * sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy");
+    hip_check(hipMemcpyAsync(coef_out, dc, nr * 2 * n * sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy");
+    hip_check(hipStreamSynchronize(c->stream), "sync");
+  });
+}
 
 extern "C" int kg_distance_jacobian_batch_dev(kg_world w, int ri, const float* dQ, int N, float dcheck, float* d_out,
                                               float* cp_out, float* grad_out, void* stream) {

@@ -305,6 +305,29 @@ struct DistJacArgs {
   float* grad_out;        // N x npairs x n
 };
 hipError_t kgl_dist_jac(const DistJacArgs& a, hipStream_t s);
+// A convex hull as the local points it spans, with a placement (obstacles;
+// links take theirs from the frames).
+struct HullRef {
+  const double* V;  // nv x 3 (FP64)
+  int nv;           // 0: no geometry
+  int pad_;
+  double X[12];     // column-major R, then t
+};
+struct SweepArgs {
+  const LinkDesc* links;
+  int n;                  // links
+  int N;                  // segments
+  int O;                  // obstacles
+  const double* T0;       // N x n x 12 frames at each segment's first configuration
+  const double* T1;       // ... and at its second
+  const HullRef* lhull;   // n
+  const HullRef* ohull;   // O
+  double dcheck;
+  double* d_out;          // N x n x O
+  double* info_out;       // N x n x O x 16: alpha, normal, pswept, pobs, p0, p1
+  double* coef_out;       // N x n x O x 2n: normal . J(q0, p0), normal . J(q1, p1)
+};
+hipError_t kgl_sweep(const SweepArgs& a, hipStream_t s);
 // Minimum implicit-surface value over a point set: out = min over i of
 // key(value(X * P[i]), i), key = order-preserving float bits << 32 | i
 // (*out must hold ~0 before the call).
@@ -368,6 +391,7 @@ struct kg_geom_s {
   double bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
   DevBuf dvals;
   DevBuf dpts;     // points (point cloud) or mesh vertices as float4, uploaded on first use
+  DevBuf dv64;     // the same points in FP64 (convex-hull support maps), uploaded on first use
   bool alive = true;  // the handle has not been destroyed
   int refs = 1;       // the handle plus every robot link, terrain and rigid object slot holding it
   double margin = 0;  // collision margin (Geometry3D.setCollisionMargin)
@@ -398,6 +422,10 @@ struct RobotPlan {
   PlanDev dev{};
   Tol tol32{}, tol64{};
   int maxdepth_a = 0, maxdepth_b = 0;
+  // convex hulls of the links and obstacles (KineTrajOpt sweep terms)
+  DevBuf dhull;
+  int hull_version = -1;
+  int nobs = 0;
 };
 
 struct kg_world_s {

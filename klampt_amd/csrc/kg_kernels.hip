@@ -2923,8 +2923,9 @@ hipError_t kgl_pts_sdf_min(const float4* P, int n, const float X[12], const SdfD
 
 namespace {
 
+template <typename G>
 __device__ inline void dj_chain(const LinkDesc* links, const double* T, int l, const double p[3], const double nh[3],
-                                double sgn, float* g) {
+                                double sgn, G* g) {
   for (int j = l; j >= 0; j = links[j].parent) {
     const double* X = T + 12 * j;
     const double* ax = links[j].axis;
@@ -2938,7 +2939,7 @@ __device__ inline void dj_chain(const LinkDesc* links, const double* T, int l, c
       const double c0 = w[1] * d2 - w[2] * d1, c1 = w[2] * d0 - w[0] * d2, c2 = w[0] * d1 - w[1] * d0;
       v = nh[0] * c0 + nh[1] * c1 + nh[2] * c2;
     }
-    g[j] += (float)(sgn * v);
+    g[j] += (G)(sgn * v);
   }
 }
 
@@ -3015,5 +3016,267 @@ hipError_t kgl_dist_jac(const DistJacArgs& a, hipStream_t s) {
   const long items = (long)a.N * a.npairs;
   if (items <= 0) return hipSuccess;
   k_dist_jac<<<(unsigned)((items + 255) / 256), 256, 0, s>>>(a);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------ KineTrajOpt sweep terms
+//
+// Per (segment k, link l, obstacle o): GJK distance (FP64) between the convex
+// hull of link l's vertices placed at both ends of the segment (the swept
+// volume of KineTrajOpt's setConvexHullGroup, kinetrajopt.py:603-611) and the
+// obstacle's hull, then the add_sweep terms (kinetrajopt.py:556-583): the
+// normal, the two placements' support points p0, p1 in -normal, alpha =
+// |p1 - pswept| / (|p0 - pswept| + |p1 - pswept|) and normal . J at p0 (the
+// segment's first configuration) and at p1 (its second).  One wave per
+// query: the support maps are maxima over the vertices, taken by all lanes
+// and reduced across the wave; the simplex logic runs uniformly on every
+// lane (identical data), and lane 0 writes.
+namespace {
+
+struct SwV {
+  double x, y, z;
+};
+__device__ inline SwV sw(double x, double y, double z) { return {x, y, z}; }
+__device__ inline SwV operator-(SwV a, SwV b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ inline SwV operator+(SwV a, SwV b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ inline SwV operator*(double s, SwV a) { return {s * a.x, s * a.y, s * a.z}; }
+__device__ inline double swdot(SwV a, SwV b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+__device__ inline SwV sw_apply(const double* X, const double* v) {
+  return sw(X[0] * v[0] + X[3] * v[1] + X[6] * v[2] + X[9], X[1] * v[0] + X[4] * v[1] + X[7] * v[2] + X[10],
+            X[2] * v[0] + X[5] * v[1] + X[8] * v[2] + X[11]);
+}
+
+// Support point of the copies of local points V[nv] placed by X0 (and X1 if
+// given) in direction d: the (copy, vertex) with the largest d . (R v + t),
+// ties to the lowest copy, then vertex.  Every lane of the wave calls it with
+// the same arguments and receives the same point.
+__device__ SwV sw_support(const double* __restrict__ V, int nv, const double* X0, const double* X1, SwV d) {
+  const int lane = threadIdx.x & 63;
+  double best = -1.0 / 0.0;
+  int bi = 0x7fffffff;
+  for (int k = 0; k < (X1 ? 2 : 1); k++) {
+    const double* X = k ? X1 : X0;
+    // d in the copy's frame, and the translation's part
+    const double l0 = X[0] * d.x + X[1] * d.y + X[2] * d.z;
+    const double l1 = X[3] * d.x + X[4] * d.y + X[5] * d.z;
+    const double l2 = X[6] * d.x + X[7] * d.y + X[8] * d.z;
+    const double off = X[9] * d.x + X[10] * d.y + X[11] * d.z;
+    for (int i = lane; i < nv; i += 64) {
+      const double val = l0 * V[3 * i] + l1 * V[3 * i + 1] + l2 * V[3 * i + 2] + off;
+      if (val > best) {
+        best = val;
+        bi = k * nv + i;
+      }
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const double ob = __shfl_xor(best, o);
+    const int oi = __shfl_xor(bi, o);
+    if (ob > best || (ob == best && oi < bi)) {
+      best = ob;
+      bi = oi;
+    }
+  }
+  // no vertex beat -inf (a NaN direction): the first vertex of the first copy
+  if (bi == 0x7fffffff) bi = 0;
+  const int k = X1 ? bi / nv : 0, i = bi % nv;
+  return sw_apply(k ? X1 : X0, V + 3 * i);
+}
+
+// Closest point to the origin of the simplex w[0..m) (m <= 4): over the
+// vertex subsets, the closest point of each subset's affine hull (normal
+// equations of its edge vectors) is a point of the simplex when all its
+// barycentric weights are positive; the answer is the nearest such point.
+// lam receives the weights over w (0 outside the chosen subset).
+__device__ SwV sw_simplex_min(const SwV* w, int m, double* lam, int& chosen) {
+  double bestd = 1.0 / 0.0;
+  SwV bestp = w[0];
+  chosen = 1;
+  for (int s = 1; s < (1 << m); s++) {
+    int idx[4], k = 0;
+    for (int i = 0; i < m; i++)
+      if (s >> i & 1) idx[k++] = i;
+    double mu[4] = {1, 0, 0, 0};
+    if (k > 1) {
+      const SwV p0 = w[idx[0]];
+      SwV e[3];
+      for (int j = 1; j < k; j++) e[j - 1] = w[idx[j]] - p0;
+      const int q = k - 1;
+      double G[3][3], r[3];
+      for (int a = 0; a < q; a++) {
+        r[a] = -swdot(e[a], p0);
+        for (int b = 0; b < q; b++) G[a][b] = swdot(e[a], e[b]);
+      }
+      double x[3];
+      double scale = 0;
+      for (int a = 0; a < q; a++) scale = fmax(scale, G[a][a]);
+      if (q == 1) {
+        if (!(G[0][0] > 0)) continue;
+        x[0] = r[0] / G[0][0];
+      } else if (q == 2) {
+        const double det = G[0][0] * G[1][1] - G[0][1] * G[1][0];
+        if (!(det > 1e-24 * scale * scale)) continue;
+        x[0] = (r[0] * G[1][1] - G[0][1] * r[1]) / det;
+        x[1] = (G[0][0] * r[1] - r[0] * G[1][0]) / det;
+      } else {
+        const double c00 = G[1][1] * G[2][2] - G[1][2] * G[2][1];
+        const double c01 = G[1][2] * G[2][0] - G[1][0] * G[2][2];
+        const double c02 = G[1][0] * G[2][1] - G[1][1] * G[2][0];
+        const double det = G[0][0] * c00 + G[0][1] * c01 + G[0][2] * c02;
+        if (!(det > 1e-36 * scale * scale * scale)) continue;
+        x[0] = (r[0] * c00 + G[0][1] * (r[2] * G[1][2] - r[1] * G[2][2]) + G[0][2] * (r[1] * G[2][1] - r[2] * G[1][1])) / det;
+        x[1] = (G[0][0] * (r[1] * G[2][2] - r[2] * G[1][2]) + r[0] * c01 + G[0][2] * (r[2] * G[1][0] - r[1] * G[2][0])) / det;
+        x[2] = (G[0][0] * (r[2] * G[1][1] - r[1] * G[2][1]) + G[0][1] * (r[1] * G[2][0] - r[2] * G[1][0]) + r[0] * c02) / det;
+      }
+      double sum = 0;
+      bool ok = true;
+      for (int a = 0; a < q; a++) {
+        mu[a + 1] = x[a];
+        sum += x[a];
+        ok = ok && x[a] > 0;
+      }
+      mu[0] = 1 - sum;
+      if (!(ok && mu[0] > 0)) continue;
+    }
+    SwV p = sw(0, 0, 0);
+    for (int j = 0; j < k; j++) p = p + mu[j] * w[idx[j]];
+    const double dd = swdot(p, p);
+    if (dd < bestd) {
+      bestd = dd;
+      bestp = p;
+      chosen = s;
+      for (int i = 0; i < 4; i++) lam[i] = 0;
+      for (int j = 0; j < k; j++) lam[idx[j]] = mu[j];
+    }
+  }
+  return bestp;
+}
+
+// GJK distance between the hull of A's copies (XA0, XA1 nullable) and the
+// hull of B (XB); 0 when they overlap.  ca, cb: the closest points.
+__device__ double sw_gjk(const double* VA, int na, const double* XA0, const double* XA1, const double* VB, int nb,
+                         const double* XB, SwV& ca, SwV& cb) {
+  SwV w[4], pa[4], pb[4];
+  double lam[4] = {1, 0, 0, 0};
+  int m = 1;
+  pa[0] = sw_apply(XA0, VA);
+  pb[0] = sw_apply(XB, VB);
+  w[0] = pa[0] - pb[0];
+  SwV v = w[0];
+  double dist = sqrt(swdot(v, v));
+  for (int it = 0; it < 64; it++) {
+    const double vv = swdot(v, v);
+    if (vv <= 1e-30) {
+      dist = 0;
+      break;
+    }
+    const SwV a = sw_support(VA, na, XA0, XA1, -1.0 * v);
+    const SwV b = sw_support(VB, nb, XB, nullptr, v);
+    const SwV nw = a - b;
+    // the new support point cannot bring v closer by more than the tolerance
+    if (vv - swdot(v, nw) <= 1e-13 * vv) break;
+    bool dup = false;
+    for (int i = 0; i < m; i++) dup = dup || (w[i].x == nw.x && w[i].y == nw.y && w[i].z == nw.z);
+    if (dup) break;
+    w[m] = nw;
+    pa[m] = a;
+    pb[m] = b;
+    m++;
+    int chosen;
+    double l2[4];
+    const SwV nv = sw_simplex_min(w, m, l2, chosen);
+    // keep the chosen vertices
+    int k = 0;
+    for (int i = 0; i < m; i++)
+      if (chosen >> i & 1) {
+        w[k] = w[i];
+        pa[k] = pa[i];
+        pb[k] = pb[i];
+        lam[k] = l2[i];
+        k++;
+      }
+    m = k;
+    if (m == 4) {
+      dist = 0;
+      break;
+    }
+    const double nd = sqrt(swdot(nv, nv));
+    v = nv;
+    if (nd >= dist) {
+      dist = nd;
+      break;
+    }
+    dist = nd;
+  }
+  ca = sw(0, 0, 0);
+  cb = sw(0, 0, 0);
+  for (int i = 0; i < m; i++) {
+    ca = ca + lam[i] * pa[i];
+    cb = cb + lam[i] * pb[i];
+  }
+  return dist;
+}
+
+__global__ __launch_bounds__(64) void k_sweep(SweepArgs a) {
+  const long total = (long)a.N * a.n * a.O;
+  const int lane = threadIdx.x;
+  for (long q = blockIdx.x; q < total; q += gridDim.x) {
+    const int o = (int)(q % a.O);
+    const int l = (int)((q / a.O) % a.n);
+    const int c = (int)(q / ((long)a.O * a.n));
+    double* info = a.info_out + q * 16;
+    double* coef = a.coef_out + q * 2 * a.n;
+    const HullRef& H = a.lhull[l];
+    const HullRef& B = a.ohull[o];
+    const double* X0 = a.T0 + ((size_t)c * a.n + l) * 12;
+    const double* X1 = a.T1 + ((size_t)c * a.n + l) * 12;
+    SwV ps, po;
+    double d = 1.0 / 0.0;
+    if (H.nv > 0 && B.nv > 0) d = sw_gjk(H.V, H.nv, X0, X1, B.V, B.nv, B.X, ps, po);
+    if (!(d < a.dcheck) || !(d > 0)) {
+      // beyond the bound (not a constraint), or overlapping hulls (no
+      // separating normal): the distance, and no terms
+      if (lane == 0) {
+        a.d_out[q] = d < a.dcheck ? d : 1.0 / 0.0;
+        for (int k = 0; k < 16; k++) info[k] = 0;
+        for (int j = 0; j < 2 * a.n; j++) coef[j] = 0;
+        if (d < a.dcheck) {
+          info[4] = ps.x; info[5] = ps.y; info[6] = ps.z;
+          info[7] = po.x; info[8] = po.y; info[9] = po.z;
+        }
+      }
+      continue;
+    }
+    const SwV nrm = (1.0 / d) * (ps - po);
+    const SwV p0 = sw_support(H.V, H.nv, X0, nullptr, -1.0 * nrm);
+    const SwV p1 = sw_support(H.V, H.nv, X1, nullptr, -1.0 * nrm);
+    const SwV e0 = p0 - ps, e1 = p1 - ps;
+    const double d0 = sqrt(swdot(e0, e0)), d1 = sqrt(swdot(e1, e1));
+    const double alpha = d0 + d1 > 0 ? d1 / (d0 + d1) : 0.5;
+    if (lane == 0) {
+      a.d_out[q] = d;
+      info[0] = alpha;
+      info[1] = nrm.x; info[2] = nrm.y; info[3] = nrm.z;
+      info[4] = ps.x; info[5] = ps.y; info[6] = ps.z;
+      info[7] = po.x; info[8] = po.y; info[9] = po.z;
+      info[10] = p0.x; info[11] = p0.y; info[12] = p0.z;
+      info[13] = p1.x; info[14] = p1.y; info[15] = p1.z;
+      for (int j = 0; j < 2 * a.n; j++) coef[j] = 0;
+      const double nh[3] = {nrm.x, nrm.y, nrm.z};
+      const double q0[3] = {p0.x, p0.y, p0.z}, q1[3] = {p1.x, p1.y, p1.z};
+      dj_chain(a.links, a.T0 + (size_t)c * a.n * 12, l, q0, nh, 1.0, coef);
+      dj_chain(a.links, a.T1 + (size_t)c * a.n * 12, l, q1, nh, 1.0, coef + a.n);
+    }
+  }
+}
+
+}  // namespace
+
+hipError_t kgl_sweep(const SweepArgs& a, hipStream_t s) {
+  const long total = (long)a.N * a.n * a.O;
+  if (total <= 0) return hipSuccess;
+  const unsigned grid = (unsigned)(total < 16384 ? total : 16384);
+  k_sweep<<<grid, 64, 0, s>>>(a);
   return hipGetLastError();
 }

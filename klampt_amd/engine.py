@@ -456,6 +456,37 @@ class World:
              _tp(first_pair), _torch_stream())
         return feasible
 
+    def num_obstacles(self):
+        out = ctypes.c_int32()
+        call("kg_world_num_obstacles", self.h, ctypes.byref(out))
+        return out.value
+
+    def sweep_batch(self, Q0, Q1, dcheck, robot=0):
+        """KineTrajOpt sweep terms (kg_sweep_batch) for segments Q0[k] -> Q1[k]
+        (float64): d (N, n, O), info (N, n, O, 16: alpha, normal, pswept,
+        pobs, p0, p1) and coef (N, n, O, 2, n: normal . J at p0 / at p1)."""
+        n = self.robots[robot].n
+        Q0, Q1 = _d(Q0).reshape(-1, n), _d(Q1).reshape(-1, n)
+        N, O = Q0.shape[0], self.num_obstacles()
+        d = np.full((N, n, O), np.inf)
+        info = np.zeros((N, n, O, 16))
+        coef = np.zeros((N, n, O, 2, n))
+        call("kg_sweep_batch", self.h, robot, _p(Q0, _pd), _p(Q1, _pd), N, float(dcheck), _p(d, _pd), _p(info, _pd),
+             _p(coef, _pd))
+        return d, info, coef
+
+    def sweep_batch_dev(self, Q0, Q1, dcheck, robot=0):
+        """kg_sweep_batch_dev on float64 CUDA tensors; returns device tensors."""
+        import torch
+        n = self.robots[robot].n
+        N, O = Q0.shape[0], self.num_obstacles()
+        d = torch.empty((N, n, O), dtype=torch.float64, device=Q0.device)
+        info = torch.empty((N, n, O, 16), dtype=torch.float64, device=Q0.device)
+        coef = torch.empty((N, n, O, 2, n), dtype=torch.float64, device=Q0.device)
+        call("kg_sweep_batch_dev", self.h, robot, _tp(Q0.contiguous()), _tp(Q1.contiguous()), N, float(dcheck),
+             _tp(d), _tp(info), _tp(coef), _torch_stream())
+        return d, info, coef
+
     def distance_batch(self, Q, robot=0):
         n = self.robots[robot].n
         Q = _f(Q).reshape(-1, n)

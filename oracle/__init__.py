@@ -371,3 +371,52 @@ def hull_support(A, x, d):
     out = np.zeros(3)
     lib().or_hull_support(_p(A, _d), A.shape[0], _p(_dd(x), _d), _p(_dd(d), _d), _p(out, _d))
     return out
+
+
+def sweep_terms(ow, r, link_points, l, q0, q1, B, xb, dcheck, fd_h=1e-5):
+    """KineTrajOpt's sweep terms for one (segment, link, obstacle), restating
+    find_collision_pair / add_sweep (Python/klampt/plan/kinetrajopt/
+    kinetrajopt.py:556-583,603-611) in FP64: the GJK distance between the hull
+    of the link's points at both configurations and the obstacle's hull B
+    (placed by xb), kept below dcheck; the normal from the obstacle to the
+    swept hull; the support points p0, p1 of the link's hull at q0 and at q1
+    in -normal; alpha = |p1 - pswept| / (|p0 - pswept| + |p1 - pswept|); and
+    normal . J at p0 (q0) and at p1 (q1), with J the position Jacobian of the
+    link point by central differences of the oracle's FK (step fd_h).
+    Returns None when d >= dcheck, (d,) when the hulls overlap, else
+    (d, alpha, normal, pswept, pobs, p0, p1, coef0, coef1)."""
+    q0, q1 = _dd(q0), _dd(q1)
+    X0, X1 = ow.fk(r, q0)[l], ow.fk(r, q1)[l]
+    d, ps, po, _ = hull_distance(link_points, X0, B, xb, xa2=X1)
+    if not d < dcheck:
+        return None
+    if not d > 0:
+        return (d,)
+    nrm = (ps - po) / d
+    p0 = hull_support(link_points, X0, -nrm)
+    p1 = hull_support(link_points, X1, -nrm)
+    d0, d1 = np.linalg.norm(p0 - ps), np.linalg.norm(p1 - ps)
+    alpha = d1 / (d0 + d1) if d0 + d1 > 0 else 0.5
+
+    return (d, alpha, nrm, ps, po, p0, p1, normal_row(ow, r, l, q0, p0, nrm, fd_h),
+            normal_row(ow, r, l, q1, p1, nrm, fd_h))
+
+
+def normal_row(ow, r, l, q, p, nrm, fd_h=1e-5):
+    """nrm . J, J the position Jacobian at configuration q of the point of
+    link l that is at world position p (robotsim positionJacobian), by
+    central differences of the oracle's FK with step fd_h."""
+    q = _dd(q)
+    X = ow.fk(r, q)[l]
+    R, t = X[:9].reshape(3, 3).T, X[9:]
+    pl = R.T @ (np.asarray(p, np.float64) - t)
+    out = np.zeros(len(q))
+    for j in range(len(q)):
+        qp, qm = q.copy(), q.copy()
+        qp[j] += fd_h
+        qm[j] -= fd_h
+        Xp, Xm = ow.fk(r, qp)[l], ow.fk(r, qm)[l]
+        a = Xp[:9].reshape(3, 3).T @ pl + Xp[9:]
+        b = Xm[:9].reshape(3, 3).T @ pl + Xm[9:]
+        out[j] = np.asarray(nrm) @ ((a - b) / (2 * fd_h))
+    return out

@@ -851,7 +851,10 @@ static V3 simplex_closest(V3* w, V3* a, V3* b, int& n, double* lam) {
     const V3 p0 = w[F[f][0]], p1 = w[F[f][1]], p2 = w[F[f][2]], q = w[opp[f]];
     const V3 nrm = cross(p1 - p0, p2 - p0);
     const double so = dot(mk(0, 0, 0) - p0, nrm), sq = dot(q - p0, nrm);
-    if (so * sq >= 0) continue;  // origin on the same side as the fourth vertex
+    // origin strictly on the fourth vertex's side of this face: not the
+    // closest feature.  A flat tetrahedron (sq = 0) contains no interior
+    // point, so every face is then examined.
+    if (so * sq > 0) continue;
     inside = false;
     V3 tw[4] = {p0, p1, p2}, ta[4] = {a[F[f][0]], a[F[f][1]], a[F[f][2]]}, tb[4] = {b[F[f][0]], b[F[f][1]], b[F[f][2]]};
     double tl[4] = {0, 0, 0, 0};
