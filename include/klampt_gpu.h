@@ -84,15 +84,17 @@ typedef struct {
                             wave's spill region is full its oldest pooled tasks are dropped
                             and what depends on them is reported as uncertain (below) */
   int32_t dist_ipw;      /* distance items a traversal wave takes per fetch, 1..64 (16) */
-  int32_t edge_w0;       /* bisection positions in the first edge-check round, 1..2^20 (16) */
+  int32_t edge_w0;       /* bisection positions in the first edge-check round, 1..2^20 (2);
+                            round r is at most w0 edge_growth^r wide */
   int32_t sdf_lanes;     /* lanes per waypoint of the SDF penalty kernels: 8, 16, 32, 64 (32) */
   int32_t prep_group;    /* 1: lane-per-link broad phase where it applies (groups of 8, 16
                             or 32 lanes, the fewest that hold the robot's links), 8 / 16 / 32:
                             groups of at least that many lanes, 0: one lane per
                             configuration (1) */
   int32_t force_recheck; /* test hook: every configuration goes through the FP64 pass (0) */
-  int32_t edge_budget_mb; /* device workspace of one edge-check round, MiB, 16..2^20 (1024):
+  int32_t edge_budget_mb; /* device workspace of one edge-check round, MiB, 16..2^20 (4096):
                              it bounds the samples checked per round */
+  int32_t edge_growth;    /* width factor between edge-check rounds, 2..1024 (16) */
 } kg_params;
 int kg_ctx_get_params(kg_ctx ctx, kg_params* out);
 int kg_ctx_set_params(kg_ctx ctx, const kg_params* params);

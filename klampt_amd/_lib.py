@@ -35,7 +35,8 @@ class Params(ctypes.Structure):
     """kg_params: engine parameters of a context (include/klampt_gpu.h)."""
     _fields_ = [(n, ctypes.c_int32) for n in (
         "cpw", "leaf_min", "scap", "waves_per_cu", "static_pct", "nsub", "tail_lanes", "tail_reps", "spill_cap",
-        "dist_ipw", "edge_w0", "sdf_lanes", "prep_group", "force_recheck", "edge_budget_mb")]
+        "dist_ipw", "edge_w0", "sdf_lanes", "prep_group", "force_recheck", "edge_budget_mb",
+        "edge_growth")]
 
 
 # name -> argtypes (all return int unless listed in _RESTYPE)

@@ -2608,10 +2608,16 @@ namespace {
 // -> EpsilonEdgeChecker, Cpp/Planning/RobotCSpace.cpp:835-837).  Rounds of
 // plan / samples / feasibility pipeline / reduce run on the device with no
 // host synchronization; the host only launches a schedule fixed in advance:
-// round r is w0 2^r positions wide (kg_params.edge_w0), at most the sample
+// round r is w0 g^r positions wide (kg_params.edge_w0, edge_growth), at most the sample
 // budget over the live segments, and the schedule has enough rounds to cover
 // max_pos positions of every segment even at the narrowest width the budget
-// allows.  Segments needing more positions than max_pos come out -2
+// allows.  Most colliding segments fail at their first bisection samples
+// (on the 8192 arm segments in the 200k soup, 3 of 4 at the midpoint), while
+// free segments must check every sample: a narrow first round wastes little
+// on the former, and fast-growing ones cover the latter in few rounds.  A
+// round costs at least the traversal's critical path (about 0.2 ms there,
+// against 2-4 ns per sample), so few wide rounds beat many narrow ones:
+// widths 2, 32, 512 take 3 rounds to the 6 of 1, 4, 16, ...  Segments needing more positions than max_pos come out -2
 // (undecided) and are counted in the status words.  A batch of more
 // segments than the budget's samples runs as consecutive chunks.
 void edge_core(kg_world_s* w, int ri, const float* Af, const float* Bf, const double* Ad, const double* Bd, int nseg,
@@ -2690,12 +2696,13 @@ void edge_core(kg_world_s* w, int ri, const float* Af, const float* Bf, const do
     ea.nsamples_out = nsamples ? nsamples + s0 : nullptr;
     ea.ctl = c->wsECtl.as<int>();
     hip_check(kgl_edge_init(ea, s0 == 0, s), "edge_init");
-    // schedule: widths w0 2^r; each round gives every live segment at least
+    // schedule: widths w0 g^r; each round gives every live segment at least
     // min(width, B / ns) positions
     const int floor_w = std::max(1, B / ns);
     int64_t covered = 0;
-    for (int r = 0; covered < max_pos; r++) {
-      const int width = (int)std::min<int64_t>((int64_t)prm.edge_w0 << std::min(r, 20), 1 << 20);
+    int64_t wr = prm.edge_w0;
+    for (int r = 0; covered < max_pos; r++, wr = std::min<int64_t>(wr * prm.edge_growth, 1 << 20)) {
+      const int width = (int)wr;
       covered += std::min(width, floor_w);
       hip_check(kgl_edge_round(ea, width, 0, s), "edge_plan");
       feasible_core(c, L, nullptr, ea.Qd, B, c->wsEFeas.as<uint8_t>(), nullptr, nullptr, nullptr, s,
@@ -3069,6 +3076,7 @@ extern "C" int kg_ctx_set_params(kg_ctx c, const kg_params* p) {
          "prep_group must be 0, 1, 8, 16 or 32");
     need(in(p->force_recheck, 0, 1), "force_recheck must be 0 or 1");
     need(in(p->edge_budget_mb, 16, 1 << 20), "edge_budget_mb must be 16..2^20");
+    need(in(p->edge_growth, 2, 1024), "edge_growth must be 2..1024");
     c->prm = *p;
   });
 }

@@ -360,11 +360,12 @@ inline kg_params kg_default_params() {
   p.tail_reps = 4;
   p.spill_cap = 16384;
   p.dist_ipw = 16;
-  p.edge_w0 = 16;
+  p.edge_w0 = 2;
   p.sdf_lanes = 32;
   p.prep_group = 1;
   p.force_recheck = 0;
-  p.edge_budget_mb = 1024;
+  p.edge_budget_mb = 4096;
+  p.edge_growth = 16;
   return p;
 }
 

@@ -2470,22 +2470,23 @@ __global__ __launch_bounds__(256) void k_edge_init(EdgeArgs a, int clear_ctl) {
 }
 
 // Round plan (one block): live segments, the round's width, each live
-// segment's sample offset (exclusive scan in segment order).
+// segment's sample offset.  Thread t takes segments t, t + TB, ... (coalesced
+// loads); the samples are laid out thread by thread (an exclusive scan over
+// the threads' counts), which is all the later kernels need: the records are
+// in increasing offset order.
 #define KG_EDGE_PLAN_TB 1024
 __global__ __launch_bounds__(KG_EDGE_PLAN_TB) void k_edge_plan(EdgeArgs a, int width) {
   using Scan = hipcub::BlockScan<int, KG_EDGE_PLAN_TB>;
   __shared__ typename Scan::TempStorage tmp;
   __shared__ int s_nlive;
   const int t = threadIdx.x;
-  const int per = (a.nseg + KG_EDGE_PLAN_TB - 1) / KG_EDGE_PLAN_TB;
-  const int s0 = t * per, s1 = min(a.nseg, s0 + per);
   auto left = [&](int s) {  // positions of s this schedule may still check
     if (a.done[s]) return 0;
     const int m = min(a.ns[s] - 1, a.max_pos);
     return max(0, m - a.cur[s]);
   };
   int nl = 0;
-  for (int s = s0; s < s1; s++) nl += left(s) > 0;
+  for (int s = t; s < a.nseg; s += KG_EDGE_PLAN_TB) nl += left(s) > 0;
   int nlive_pre, nlive;
   Scan(tmp).ExclusiveSum(nl, nlive_pre, nlive);
   if (t == 0) s_nlive = nlive;
@@ -2493,12 +2494,12 @@ __global__ __launch_bounds__(KG_EDGE_PLAN_TB) void k_edge_plan(EdgeArgs a, int w
   nlive = s_nlive;
   const int w = nlive > 0 ? max(1, min(width, a.budget / nlive)) : 0;
   int ns_loc = 0;
-  for (int s = s0; s < s1; s++) ns_loc += min(w, left(s));
+  for (int s = t; s < a.nseg; s += KG_EDGE_PLAN_TB) ns_loc += min(w, left(s));
   int off, total;
   __syncthreads();
   Scan(tmp).ExclusiveSum(ns_loc, off, total);
   int li = nlive_pre;
-  for (int s = s0; s < s1; s++) {
+  for (int s = t; s < a.nseg; s += KG_EDGE_PLAN_TB) {
     const int k = min(w, left(s));
     if (k > 0) {
       a.seg[li] = s;
@@ -2538,27 +2539,40 @@ __global__ __launch_bounds__(256) void k_edge_samples(EdgeArgs a) {
 }
 
 // First infeasible sample of each live segment in the round, else the
-// cursor advances; the round's status words are accumulated.
+// cursor advances; the round's status words are accumulated.  One wave per
+// live segment: its slots 64 at a time, the first infeasible by ballot (the
+// loop's bounds and exit are wave-uniform).
 __global__ __launch_bounds__(256) void k_edge_reduce(EdgeArgs a) {
   const int nlive = a.ctl[0];
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g == 0) {
+  const int lane = threadIdx.x & 63;
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int nwaves = (gridDim.x * blockDim.x) >> 6;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
     a.ctl[2] |= a.st[a.w_spill];
     a.ctl[3] += a.st[a.w_unc];
     a.ctl[4] += a.st[a.w_redo];
   }
-  for (int t = g; t < nlive; t += gridDim.x * blockDim.x) {
+  for (int t = wave; t < nlive; t += nwaves) {
     const int s = a.seg[t];
     const int b = a.soff[t], e = a.soff[t + 1];
-    int j = b;
-    while (j < e && a.feas[j]) j++;
-    if (j < e) {
-      const int ns = a.ns[s];
-      const int i = kg_bisect_index(ns, a.cur[s] + (j - b));
-      a.first_u[s] = (float)((double)i / (double)ns);
-      a.done[s] = 1;
-    } else {
-      a.cur[s] += e - b;
+    int first = -1;
+    for (int j0 = b; j0 < e; j0 += 64) {
+      const int j = j0 + lane;
+      const uint64_t m = __ballot(j < e && !a.feas[j]);
+      if (m) {
+        first = j0 + __ffsll((unsigned long long)m) - 1;
+        break;
+      }
+    }
+    if (lane == 0) {
+      if (first >= 0) {
+        const int ns = a.ns[s];
+        const int i = kg_bisect_index(ns, a.cur[s] + (first - b));
+        a.first_u[s] = (float)((double)i / (double)ns);
+        a.done[s] = 1;
+      } else {
+        a.cur[s] += e - b;
+      }
     }
   }
 }
@@ -2597,7 +2611,8 @@ hipError_t kgl_edge_round(const EdgeArgs& a, int width, int phase, hipStream_t s
     k_edge_plan<<<1, KG_EDGE_PLAN_TB, 0, s>>>(a, width);
     k_edge_samples<<<blocks, 256, 0, s>>>(a);
   } else {
-    k_edge_reduce<<<max(1, min(1024, (a.nseg + 255) / 256)), 256, 0, s>>>(a);
+    // one wave per live segment (4 per block), grid-stride
+    k_edge_reduce<<<max(1, min(2048, (a.nseg + 3) / 4)), 256, 0, s>>>(a);
   }
   return hipGetLastError();
 }
