@@ -157,11 +157,17 @@ struct StreamScope {
       hip_check(hipEventCreateWithFlags(&c->evDone, hipEventDisableTiming | hipEventDisableSystemFence),
                 "hipEventCreate");
     if (c->done_valid && c->done_stream != s) hip_check(hipStreamWaitEvent(s, c->evDone, 0), "hipStreamWaitEvent");
+    // feasibility batches still running on other streams' slots (their
+    // buffers are their own, but a plain call may read their status words or
+    // replace tables they read)
+    for (auto& f : c->fslot)
+      if (f.valid) hip_check(hipStreamWaitEvent(s, f.done, 0), "hipStreamWaitEvent");
   }
   ~StreamScope() {
     if (hipEventRecord(c->evDone, s) == hipSuccess) {
       c->done_stream = s;
       c->done_valid = true;
+      c->last_slotted = false;
     }
   }
   StreamScope(const StreamScope&) = delete;
@@ -172,6 +178,8 @@ struct StreamScope {
 void sync_pool(kg_ctx_s* c) {
   if (!c->pool_dirty) return;
   hip_check(hipSetDevice(c->device), "hipSetDevice");
+  // batches on other streams may still read the pool being replaced
+  hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
   upload(c->dnodes, c->nodes);
   upload(c->dtv0, c->tv0);
   upload(c->dtv1, c->tv1);
@@ -280,6 +288,8 @@ RobotPlan& get_plan(kg_world_s* w, int ri) {
   RobotPlan& P = w->plans[ri];
   int ver = world_version(w) * 131 + c->pool_version;
   if (P.built_version == ver) return P;
+  // batches on other streams may still read the tables being replaced
+  hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
   rebuild_mask(w);
   kg_robot_s* R = w->robots[ri];
   // links
@@ -863,6 +873,11 @@ int kg_ctx_destroy(kg_ctx c) {
     }
     if (c->evStat) (void)hipEventDestroy(c->evStat);
     if (c->evDone) (void)hipEventDestroy(c->evDone);
+    for (auto& f : c->fslot) {
+      DevBuf* fb[] = {&f.status, &f.jl, &f.counts, &f.redo, &f.items, &f.recs, &f.spill};
+      for (auto* b : fb) b->release();
+      if (f.done) (void)hipEventDestroy(f.done);
+    }
     (void)hipStreamDestroy(c->stream);
     delete c;
   });
@@ -873,6 +888,8 @@ int kg_ctx_synchronize(kg_ctx c) {
     need(c != nullptr, "ctx is NULL");
     hip_check(hipSetDevice(c->device), "hipSetDevice");
     if (c->done_valid) hip_check(hipEventSynchronize(c->evDone), "hipEventSynchronize");
+    for (auto& f : c->fslot)
+      if (f.valid) hip_check(hipEventSynchronize(f.done), "hipEventSynchronize");
     hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
   });
 }
@@ -1570,15 +1587,90 @@ int kg_world_pair_list(kg_world w, int ri, int32_t* pairs, int maxp, int* np) {
   });
 }
 
+}  // extern "C"
+
+namespace {
+// A device-pointer feasibility batch on the context's next workspace slot
+// (the fixed-capacity, per-link broad-phase path; anything else runs as a
+// plain call in context order).  The batch waits for the previous batch on
+// its slot and for the last plain call, not for batches on the other slots.
+void feasible_dev_slot(kg_world_s* w, int ri, const float* Qf, const double* Qd, int N, uint8_t* feas, uint8_t* env,
+                       uint8_t* self, int32_t* fp, hipStream_t s) {
+  kg_ctx_s* c = w->ctx;
+  need(ri >= 0 && ri < (int)w->robots.size(), "robot index out of range");
+  hip_check(hipSetDevice(c->device), "hipSetDevice");
+  reject_margins(w);
+  const int n = w->robots[ri]->n;
+  // get_plan synchronizes the device before it replaces tables that batches
+  // on other streams may read
+  RobotPlan& P = get_plan(w, ri);
+  Layout L;
+  L.P = &P;
+  L.cap = active_links(w->robots[ri]) * P.nenv + P.nself;
+  L.maxcap = L.cap;
+  L.nrows = (size_t)N * n;
+  L.ncap = (size_t)L.cap * N;
+  L.tol32 = P.tol32;
+  L.tol64 = P.tol64;
+  const bool slotted = N > 0 && c->prm.prep_group && n <= 32 && L.ncap * sizeof(PairItem) <= (2048ull << 20);
+  if (!slotted) {
+    StreamScope scope(c, s);
+    feasible_core(c, L, Qf, Qd, N, feas, env, self, fp, s);
+    return;
+  }
+  if (!c->evDone)
+    hip_check(hipEventCreateWithFlags(&c->evDone, hipEventDisableTiming | hipEventDisableSystemFence),
+              "hipEventCreate");
+  kg_ctx_s::FeasSlot& f = c->fslot[c->fslot_next];
+  c->fslot_next = (c->fslot_next + 1) % kg_ctx_s::FEAS_SLOTS;
+  if (!f.done)
+    hip_check(hipEventCreateWithFlags(&f.done, hipEventDisableTiming | hipEventDisableSystemFence),
+              "hipEventCreate");
+  // the slot's previous batch, and the last plain call (shared tables)
+  if (f.valid) hip_check(hipStreamWaitEvent(s, f.done, 0), "hipStreamWaitEvent");
+  if (c->done_valid && !c->last_slotted && c->done_stream != s)
+    hip_check(hipStreamWaitEvent(s, c->evDone, 0), "hipStreamWaitEvent");
+  const kg_params& prm = c->prm;
+  const int scap = prm.scap == 256 ? 256 : 512;
+  const int grid32 = prm.waves_per_cu > 0 ? prm.waves_per_cu * c->num_cus : kgl_collide_grid(0, c->num_cus, scap);
+  const int grid64 = std::min(kgl_collide_grid(1, c->num_cus, scap), std::max(1, c->num_cus));
+  const size_t spill_entries = (size_t)std::max(grid32, grid64) * prm.spill_cap;
+  hip_check(f.status.ensure(ST_WORDS * sizeof(int)), "hipMalloc(status)");
+  hip_check(f.jl.ensure((size_t)N), "hipMalloc(jl)");
+  hip_check(f.counts.ensure((size_t)(N + 1) * sizeof(int)), "hipMalloc(counts)");
+  hip_check(f.redo.ensure((size_t)(N + 16) * sizeof(int)), "hipMalloc(redo)");
+  hip_check(f.items.ensure(std::max<size_t>(1, L.ncap) * sizeof(PairItem)), "hipMalloc(items)");
+  hip_check(f.recs.ensure(std::max<size_t>(1, L.ncap) * sizeof(ItemRec)), "hipMalloc(records)");
+  hip_check(f.spill.ensure(std::max<size_t>(1, spill_entries) * sizeof(uint64_t)), "hipMalloc(spill)");
+  FeasWs ws;
+  ws.status = f.status.as<int>();
+  ws.jl = f.jl.as<uint8_t>();
+  ws.counts = f.counts.as<int>();
+  ws.redo = f.redo.as<int>();
+  ws.items = f.items.as<PairItem>();
+  ws.recs = f.recs.as<ItemRec>();
+  ws.spill = f.spill.as<uint64_t>();
+  ws.spill_entries = (int)std::min<size_t>(spill_entries, (size_t)INT32_MAX);
+  feasible_core(c, L, Qf, Qd, N, feas, env, self, fp, s, nullptr, true, &ws);
+  hip_check(hipEventRecord(f.done, s), "hipEventRecord");
+  f.valid = true;
+  if (hipEventRecord(c->evDone, s) == hipSuccess) {
+    c->done_stream = s;
+    c->done_valid = true;
+    c->last_slotted = true;
+  }
+}
+}  // namespace
+
+extern "C" {
+
 int kg_feasible_batch_dev(kg_world w, int ri, const float* dQ, int N, uint8_t* feas, uint8_t* env, uint8_t* self,
                           int32_t* fp, void* stream) {
   return guard([&] {
     need(w != nullptr, "world is NULL");
     need(N >= 0, "negative batch size");
     need(N == 0 || (dQ && feas), "NULL buffer");
-    hipStream_t s = (hipStream_t)stream;
-    StreamScope scope(w->ctx, s);
-    feasible_dev(w, ri, dQ, nullptr, N, feas, env, self, fp, s);
+    feasible_dev_slot(w, ri, dQ, nullptr, N, feas, env, self, fp, (hipStream_t)stream);
   });
 }
 
@@ -1588,9 +1680,7 @@ int kg_feasible_batch_f64_dev(kg_world w, int ri, const double* dQ, int N, uint8
     need(w != nullptr, "world is NULL");
     need(N >= 0, "negative batch size");
     need(N == 0 || (dQ && feas), "NULL buffer");
-    hipStream_t s = (hipStream_t)stream;
-    StreamScope scope(w->ctx, s);
-    feasible_dev(w, ri, nullptr, dQ, N, feas, env, self, fp, s);
+    feasible_dev_slot(w, ri, nullptr, dQ, N, feas, env, self, fp, (hipStream_t)stream);
   });
 }
 

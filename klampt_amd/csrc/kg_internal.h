@@ -470,6 +470,19 @@ struct kg_ctx_s {
   int64_t tr_count = 0;
   hipStream_t done_stream = nullptr;
   bool done_valid = false;
+  bool last_slotted = false;  // the last call was a feasibility batch on a workspace slot
+  // Workspace slots of device-pointer feasibility batches: a batch takes the
+  // next slot and waits only for that slot's previous batch, so batches
+  // issued on different streams run concurrently (each fills the idle tail
+  // of the traversal before it).
+  struct FeasSlot {
+    DevBuf status, jl, counts, redo, items, recs, spill;
+    hipEvent_t done = nullptr;
+    bool valid = false;
+  };
+  static constexpr int FEAS_SLOTS = 3;
+  FeasSlot fslot[FEAS_SLOTS];
+  int fslot_next = 0;
   DevBuf wsQ, wsT64, wsLB, wsJL, wsCounts, wsOffsets, wsItems, wsScanTmp, wsBest, wsUnc, wsChunk, wsSpill,
       wsStatus, wsRedo, wsOut8a, wsOut8b, wsOut8c, wsOut32, wsF, wsG, wsPoff, wsPts, wsLoss, wsDQ,
       wsDl, wsEA, wsEB, wsESeg, wsESoff, wsENs, wsECur, wsECtl, wsEStat, wsEQ, wsEFeas, wsEU, wsEDone, wsRec,
