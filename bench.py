@@ -105,6 +105,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--configs", type=int, default=65536)
+    ap.add_argument("--streams", type=int, default=2,
+                    help="streams the timed batches alternate over (1: one batch at a time)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
@@ -140,18 +142,38 @@ def main():
         world.feasible_batch_dev(dQ, feasible=dfeas)
     torch.cuda.synchronize()
 
+    # (1) one stream: batch latency, and the traversal kernel's duration per
+    # launch (HIP events on the launch stream, launches not overlapping)
     ctx.kernel_times(reset=True)
     torch.cuda.synchronize()
     t_start = time.perf_counter()
     for _ in range(args.steps):
         world.feasible_batch_dev(dQ, feasible=dfeas)
     torch.cuda.synchronize()
+    elapsed1 = time.perf_counter() - t_start
+    ksum, kcnt = ctx.kernel_times(reset=True)
+    kernel_s = ksum / max(1, kcnt)
+    single = {"checks_per_s": round(args.configs * args.steps / elapsed1, 1),
+              "ms_per_batch": round(elapsed1 / args.steps * 1e3, 4)}
+    # (2) the timed region: the same batches issued alternately on S streams
+    # (the context runs batches of different streams concurrently, each on
+    # its own workspace slot): whole-job throughput
+    streams = [torch.cuda.Stream() for _ in range(args.streams)]
+    outs = [dfeas] + [torch.empty_like(dfeas) for _ in range(args.streams - 1)]
+    for i in range(2 * args.streams):
+        with torch.cuda.stream(streams[i % args.streams]):
+            world.feasible_batch_dev(dQ, feasible=outs[i % args.streams])
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        with torch.cuda.stream(streams[i % args.streams]):
+            world.feasible_batch_dev(dQ, feasible=outs[i % args.streams])
+    torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
     ms_per_step = elapsed / args.steps * 1e3
     value = args.configs * args.steps / elapsed
-    # traversal kernel duration: HIP events on the launch stream, every step
-    ksum, kcnt = ctx.kernel_times(reset=True)
-    kernel_s = ksum / max(1, kcnt)
+    for o in outs[1:]:
+        assert torch.equal(o, dfeas), "batches on different streams disagree"
     # counters are deterministic per batch: read those of the last step
     st = world.last_stats()
     # broad-phase duration from one more (untimed) batch with per-phase events
@@ -160,7 +182,9 @@ def main():
     t_prep = world.last_stats()["t_prep_ns"]
     ctx.set_timing(1)
     feas_frac = float(dfeas.float().mean().item())
-    print(f"[bench] {value:.4g} checks/s, {ms_per_step:.3f} ms/step, traversal kernel {kernel_s * 1e3:.4f} ms "
+    print(f"[bench] {value:.4g} checks/s on {args.streams} streams ({ms_per_step:.3f} ms/step); one stream "
+          f"{single['checks_per_s']:.4g} checks/s ({single['ms_per_batch']:.3f} ms/batch), traversal kernel "
+          f"{kernel_s * 1e3:.4f} ms "
           f"(avg of {kcnt}), broad phase {t_prep / 1e6:.4f} ms, feasible {feas_frac:.3f}, "
           f"fp64 rechecks/step {st['fp64_rechecks']}",
           file=sys.stderr, flush=True)
@@ -211,6 +235,7 @@ def main():
                          f"over the first {len(Q1)}, {t1s:.1f}s",
                "single_thread_value": round(rate1, 1),
                "gpu_over_cpu": round(value / cpu_rate, 2), "gpu_over_one_thread": round(value / rate1, 1),
+               "gpu_over_cpu_one_stream": round(single["checks_per_s"] / cpu_rate, 2),
                "results_equal": agree}
         print(f"[bench] cpu {cpu_rate:.4g} checks/s on {nthr} threads, {rate1:.4g} on one; equal={agree}",
               file=sys.stderr, flush=True)
@@ -273,7 +298,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "7-DOF arm vs 200k-triangle soup terrain, feasibility (IsFeasible)",
                    "configs": args.configs, "scene_tris": int(sc.terrains[0].ntris), "link_tris": 512,
-                   "feasible_fraction": round(feas_frac, 4)},
+                   "feasible_fraction": round(feas_frac, 4), "streams": args.streams,
+                   "one_stream": single},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "secondary": secondary,
@@ -470,6 +496,56 @@ def run_secondary(args, ctx, sc, world, robot):
                         "feasible_fraction": round(float(fh.float().mean().item()), 4)}
     engine.set_tuning(ctx)
     print(f"[bench] C3 hetero: {out['c3_hetero']}", file=sys.stderr, flush=True)
+
+    # serving form: the C3 batch and the C4 edges as one library-captured HIP
+    # graph (kg_serve_*), replayed on resident buffers; equal to the eager calls
+    sv = engine.Serve(ctx, feas=[(w, 0, c) for w, c in zip(worlds, counts)], edges=[(world, 0, args.edges)],
+                      eps=0.01, norm_inf=True)
+    sv.Q.copy_(Qh)
+    sv.A.copy_(dA.reshape(-1))
+    sv.B.copy_(dB.reshape(-1))
+    ts = _time_dev(sv.run, 5)
+    out["serve"] = {"feasibility_configs": int(sum(counts)), "robots": 64, "edge_segments": args.edges,
+                    "ms": round(ts * 1e3, 3), "graph_captures": sv.captures(),
+                    "eager_ms": round((th + te) * 1e3, 3),
+                    "results_equal": bool(torch.equal(sv.feasible, fh) and torch.equal(sv.first_u, dfu)),
+                    "note": "one kg_serve_run (graph launch) per planner iteration; eager_ms: the C3 and C4 "
+                            "calls issued one after the other"}
+    sv.close()
+    print(f"[bench] serve: {out['serve']}", file=sys.stderr, flush=True)
+
+    # KineTrajOpt sweep terms: trajectory steps of the arm against the 16 boxes
+    scb = S.boxes_scene(0)
+    wb, rb = engine.world_from_scene(ctx, scb)
+    nseg_sw = 4096
+    rng = np.random.default_rng(9)
+    Q0 = S.uniform_configs(scb.robot, nseg_sw, seed=9).astype(np.float64)
+    stp = rng.normal(size=Q0.shape)
+    stp *= rng.uniform(0.02, 0.3, (nseg_sw, 1)) / np.abs(stp).max(axis=1, keepdims=True)
+    Q1 = Q0 + stp
+    d0, d1 = torch.from_numpy(Q0).to(dev), torch.from_numpy(Q1).to(dev)
+    tsw = _time_dev(lambda: wb.sweep_batch_dev(d0, d1, 0.25), 3)
+    dsw, _, _ = wb.sweep_batch_dev(d0, d1, 0.25)
+    nq = nseg_sw * scb.robot.n * len(scb.objects)
+    out["c9_sweep"] = {"segments": nseg_sw, "queries": nq, "ms": round(tsw * 1e3, 3),
+                       "queries_per_s": round(nq / tsw, 1),
+                       "below_dcheck": int(torch.isfinite(dsw).sum().item()), "dcheck": 0.25}
+    if not args.no_cpu:
+        import oracle
+        ow9, or9 = oracle.world_from_scene(scb)
+        t1 = time.perf_counter()
+        nsample = 0
+        for k in range(64):
+            X0, X1 = ow9.fk(or9, Q0[k]), ow9.fk(or9, Q1[k])
+            for l in range(scb.robot.n):
+                for mesh, xf in scb.objects:
+                    oracle.hull_distance(scb.robot.link_meshes[l].verts, X0[l], mesh.verts, xf, xa2=X1[l])
+                    nsample += 1
+        tc = time.perf_counter() - t1
+        out["c9_sweep"]["cpu_queries_per_s"] = round(nsample / tc, 1)
+        out["c9_sweep"]["cpu_sample"] = (f"first 64 segments ({nsample} queries), FP64 oracle GJK distance only, "
+                                         "one thread, called per query from Python")
+    print(f"[bench] C9 sweep terms: {out['c9_sweep']}", file=sys.stderr, flush=True)
 
     # planners on the batched queries: PRM roadmap over the 16-box scene
     # (every candidate edge validated in batched calls) and a lazy PRM query
