@@ -2018,14 +2018,52 @@ __device__ inline Real obb_lower_bound(const NodeF& A, const NodeF& B, const Rea
   }
   const Real eA0 = A.e[0], eA1 = A.e[1], eA2 = A.e[2], eB0 = B.e[0], eB1 = B.e[1], eB2 = B.e[2];
   const Real eA[3] = {eA0, eA1, eA2}, eB[3] = {eB0, eB1, eB2};
-  Real lb = sqrt(T[0] * T[0] + T[1] * T[1] + T[2] * T[2]) - sqrt(eA0 * eA0 + eA1 * eA1 + eA2 * eA2) -
-            sqrt(eB0 * eB0 + eB1 * eB1 + eB2 * eB2);
+  // the axis through the two centres: |T| - (sum_i eB_i |T_i| + sum_j eA_j
+  // |T . a_j|) / |T|, at least the bounding-sphere bound |T| - |eA| - |eB|
+  // (Cauchy-Schwarz), and the best cheap axis for diagonally offset boxes
+  const Real tt = T[0] * T[0] + T[1] * T[1] + T[2] * T[2];
+  Real rc = eB0 * fabs(T[0]) + eB1 * fabs(T[1]) + eB2 * fabs(T[2]);
+  Real lb = (Real)0;
 #pragma unroll
   for (int i = 0; i < 3; i++) lb = fmax(lb, fabs(T[i]) - (eB[i] + eA[0] * Qa[i][0] + eA[1] * Qa[i][1] + eA[2] * Qa[i][2]));
 #pragma unroll
   for (int j = 0; j < 3; j++) {
     const Real t = T[0] * C[0][j] + T[1] * C[1][j] + T[2] * C[2][j];
+    rc += eA[j] * fabs(t);
     lb = fmax(lb, fabs(t) - (eB[0] * Qa[0][j] + eB[1] * Qa[1][j] + eB[2] * Qa[2][j] + eA[j]));
+  }
+  if (tt > (Real)0) lb = fmax(lb, (tt - rc) * rsqrt(tt));
+  // the axis from B's point nearest A's centre to that centre: the gap along
+  // it is |d| minus A's half-width on it (B's support there is its nearest
+  // point); and the same from A's side (B's centre in A's frame is -C^T T)
+  {
+    Real d[3], dd = 0;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      d[i] = T[i] - fmin(fmax(T[i], -eB[i]), eB[i]);
+      dd += d[i] * d[i];
+    }
+    if (dd > (Real)0) {
+      Real ra = 0;
+#pragma unroll
+      for (int j = 0; j < 3; j++) ra += eA[j] * fabs(d[0] * C[0][j] + d[1] * C[1][j] + d[2] * C[2][j]);
+      lb = fmax(lb, (dd - ra) * rsqrt(dd));
+    }
+  }
+  {
+    Real u[3], e[3], ee = 0;
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      u[j] = -(T[0] * C[0][j] + T[1] * C[1][j] + T[2] * C[2][j]);
+      e[j] = u[j] - fmin(fmax(u[j], -eA[j]), eA[j]);
+      ee += e[j] * e[j];
+    }
+    if (ee > (Real)0) {
+      Real rb = 0;
+#pragma unroll
+      for (int i = 0; i < 3; i++) rb += eB[i] * fabs(e[0] * C[i][0] + e[1] * C[i][1] + e[2] * C[i][2]);
+      lb = fmax(lb, (ee - rb) * rsqrt(ee));
+    }
   }
 #pragma unroll
   for (int i = 0; i < 3; i++) {
