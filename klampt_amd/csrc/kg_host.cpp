@@ -209,6 +209,7 @@ void geom_release(kg_geom_s* g) {
   g->dvals.release();
   g->dpts.release();
   g->dv64.release();
+  g->dv8.release();
   // mesh data stays in the context pool until the context is destroyed
   delete g;
 }
@@ -2075,7 +2076,9 @@ SdfDev sdf_dev(kg_geom_s* g) {
   need(g && g->kind == 1, "geometry is not an ImplicitSurface");
   SdfDev d{};
   d.v = g->dvals.as<float>();
+  d.v8 = g->dv8.as<float4>();  // present once a penalty call has built it
   for (int k = 0; k < 3; k++) {
+    d.cdims[k] = std::max(g->dims[k] - 1, 1);
     d.dims[k] = g->dims[k];
     const double h = (g->bmax[k] - g->bmin[k]) / g->dims[k];
     d.bmin[k] = (float)g->bmin[k];
@@ -2125,6 +2128,16 @@ void sdf_penalty_dev(kg_robot_s* r, kg_geom_s* sdf, const float* dQ, int N, cons
   robot_links_dev(r);
   hip_check(c->wsPoff.ensure((n + 1) * sizeof(int)), "hipMalloc");
   hip_check(hipMemcpyAsync(c->wsPoff.p, off.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice, s), "copy offsets");
+  if (!sdf->dv8.p) {
+    // the corner-per-cell copy the penalty kernels read (8x the grid: 0.5 GB
+    // for 256^3 cells; grids whose copy would pass 4 GiB keep the gathers)
+    const size_t ncell =
+        (size_t)std::max(sdf->dims[0] - 1, 1) * std::max(sdf->dims[1] - 1, 1) * std::max(sdf->dims[2] - 1, 1);
+    if (ncell * 2 * sizeof(float4) <= (4ull << 30)) {
+      hip_check(sdf->dv8.ensure(ncell * 2 * sizeof(float4)), "hipMalloc(sdf corners)");
+      hip_check(kgl_sdf_build8(sdf->dvals.as<float>(), sdf->dims, sdf->dv8.as<float4>(), s), "sdf corners");
+    }
+  }
   SdfDev g = sdf_dev(sdf);
   hip_check(kgl_sdf_penalty(r->dlinks.as<LinkDesc>(), n, g, dQ, N, dpts, c->wsPoff.as<int>(), dsafe, dloss, ddl, ddQ, bwd,
                             c->prm.sdf_lanes, s),

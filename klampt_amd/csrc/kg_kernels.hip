@@ -1736,12 +1736,21 @@ __device__ inline float sdf_trilinear(const SdfDev& g, float x, float y, float z
     f[a] = g.dims[a] > 1 ? (float)(v - (double)i) : 0.f;
     if (g.dims[a] == 1) gs[a] = 0.f;
   }
-  const int s1 = g.dims[2], s0 = g.dims[1] * g.dims[2];
-  const float* v = g.v;
-  const float c000 = v[i0[0] * s0 + i0[1] * s1 + i0[2]], c001 = v[i0[0] * s0 + i0[1] * s1 + i1[2]];
-  const float c010 = v[i0[0] * s0 + i1[1] * s1 + i0[2]], c011 = v[i0[0] * s0 + i1[1] * s1 + i1[2]];
-  const float c100 = v[i1[0] * s0 + i0[1] * s1 + i0[2]], c101 = v[i1[0] * s0 + i0[1] * s1 + i1[2]];
-  const float c110 = v[i1[0] * s0 + i1[1] * s1 + i0[2]], c111 = v[i1[0] * s0 + i1[1] * s1 + i1[2]];
+  float c000, c001, c010, c011, c100, c101, c110, c111;
+  if (g.v8) {
+    // the cell's 8 corners, stored together
+    const size_t cell = ((size_t)i0[0] * g.cdims[1] + i0[1]) * g.cdims[2] + i0[2];
+    const float4 a = g.v8[2 * cell], b = g.v8[2 * cell + 1];
+    c000 = a.x; c001 = a.y; c010 = a.z; c011 = a.w;
+    c100 = b.x; c101 = b.y; c110 = b.z; c111 = b.w;
+  } else {
+    const int s1 = g.dims[2], s0 = g.dims[1] * g.dims[2];
+    const float* v = g.v;
+    c000 = v[i0[0] * s0 + i0[1] * s1 + i0[2]]; c001 = v[i0[0] * s0 + i0[1] * s1 + i1[2]];
+    c010 = v[i0[0] * s0 + i1[1] * s1 + i0[2]]; c011 = v[i0[0] * s0 + i1[1] * s1 + i1[2]];
+    c100 = v[i1[0] * s0 + i0[1] * s1 + i0[2]]; c101 = v[i1[0] * s0 + i0[1] * s1 + i1[2]];
+    c110 = v[i1[0] * s0 + i1[1] * s1 + i0[2]]; c111 = v[i1[0] * s0 + i1[1] * s1 + i1[2]];
+  }
   const float fx = f[0], fy = f[1], fz = f[2];
   const float c00 = c000 * (1.f - fz) + c001 * fz, c01 = c010 * (1.f - fz) + c011 * fz;
   const float c10 = c100 * (1.f - fz) + c101 * fz, c11 = c110 * (1.f - fz) + c111 * fz;
@@ -1802,7 +1811,7 @@ __global__ __launch_bounds__(256) void k_sdf_pen_group(const LinkDesc* __restric
                                                        const float* __restrict__ pts, const int* __restrict__ poff,
                                                        float dsafe, float* __restrict__ loss_out,
                                                        const float* __restrict__ dloss, float* __restrict__ dQ,
-                                                       int gpb) {
+                                                       int gpb, int nb8) {
   extern __shared__ float dyn[];
   __shared__ float sT0[KG_MAX_LINKS][12];
   __shared__ float sAx[KG_MAX_LINKS][3];
@@ -1815,7 +1824,12 @@ __global__ __launch_bounds__(256) void k_sdf_pen_group(const LinkDesc* __restric
   }
   for (int i = threadIdx.x; i <= n; i += blockDim.x) sOff[i] = poff[i];
   const int grp = threadIdx.x / GL, l = threadIdx.x % GL;
-  const int c = blockIdx.x * gpb + grp;
+  // XCD-aware order: the hardware deals blocks to the 8 XCDs round-robin, so
+  // block b runs on XCD b % 8; with nb8 > 0 the blocks of one XCD take a
+  // contiguous run of waypoints (neighbouring waypoints of a trajectory gather
+  // the same grid lines, which then stay in that XCD's L2)
+  const int lb = nb8 ? (int)(blockIdx.x & 7) * nb8 + (int)(blockIdx.x >> 3) : (int)blockIdx.x;
+  const int c = lb * gpb + grp;
   const bool valid = c < N;
   float* sT = dyn + (size_t)grp * n * 12;
   float* sFM = dyn + (size_t)gpb * n * 12 + (size_t)grp * n * 6;
@@ -1957,6 +1971,32 @@ hipError_t kgl_sdf_eval(const SdfDev& g, const float* pts, int n, float* d, floa
   return hipGetLastError();
 }
 
+namespace {
+__global__ __launch_bounds__(256) void k_sdf_build8(const float* __restrict__ v, int dx, int dy, int dz,
+                                                   float4* __restrict__ out) {
+  const int cx = max(dx - 1, 1), cy = max(dy - 1, 1), cz = max(dz - 1, 1);
+  const size_t ncell = (size_t)cx * cy * cz;
+  for (size_t cell = (size_t)blockIdx.x * blockDim.x + threadIdx.x; cell < ncell;
+       cell += (size_t)gridDim.x * blockDim.x) {
+    const int k = (int)(cell % cz), j = (int)((cell / cz) % cy), i = (int)(cell / ((size_t)cz * cy));
+    // the corner rule of sdf_trilinear: the upper index is clamped to the grid
+    const int i1 = min(i + 1, dx - 1), j1 = min(j + 1, dy - 1), k1 = min(k + 1, dz - 1);
+    const size_t s1 = dz, s0 = (size_t)dy * dz;
+    out[2 * cell] = make_float4(v[i * s0 + j * s1 + k], v[i * s0 + j * s1 + k1], v[i * s0 + j1 * s1 + k],
+                                v[i * s0 + j1 * s1 + k1]);
+    out[2 * cell + 1] = make_float4(v[i1 * s0 + j * s1 + k], v[i1 * s0 + j * s1 + k1], v[i1 * s0 + j1 * s1 + k],
+                                    v[i1 * s0 + j1 * s1 + k1]);
+  }
+}
+}  // namespace
+
+hipError_t kgl_sdf_build8(const float* v, const int dims[3], float4* out, hipStream_t s) {
+  const size_t ncell = (size_t)std::max(dims[0] - 1, 1) * std::max(dims[1] - 1, 1) * std::max(dims[2] - 1, 1);
+  const unsigned blocks = (unsigned)std::min<size_t>((ncell + 255) / 256, 65536);
+  k_sdf_build8<<<blocks, 256, 0, s>>>(v, dims[0], dims[1], dims[2], out);
+  return hipGetLastError();
+}
+
 hipError_t kgl_sdf_penalty(const LinkDesc* links, int n, const SdfDev& g, const float* Q, int N, const float* pts,
                            const int* poff, float dsafe, float* loss, const float* dloss, float* dQ, bool bwd,
                            int lanes, hipStream_t s) {
@@ -1969,13 +2009,22 @@ hipError_t kgl_sdf_penalty(const LinkDesc* links, int n, const SdfDev& g, const 
   int gpb = 256 / GLr;
   while (gpb > 1 && gpb * per_group > 48 * 1024) gpb /= 2;
   const size_t lds = gpb * per_group;
-  const int blocks = (N + gpb - 1) / gpb;
+  int blocks = (N + gpb - 1) / gpb;
+  // XCD-aware block order for large batches: the grid is padded to a
+  // multiple of 8 (padding blocks find no waypoint)
+  int nb8 = 0;
+  if (blocks >= 64) {
+    nb8 = (blocks + 7) / 8;
+    blocks = 8 * nb8;
+  }
   auto go = [&](auto glc) {
     constexpr int GL = decltype(glc)::value;
     if (bwd)
-      k_sdf_pen_group<true, GL><<<blocks, gpb * GL, lds, s>>>(links, n, g, Q, N, pts, poff, dsafe, loss, dloss, dQ, gpb);
+      k_sdf_pen_group<true, GL><<<blocks, gpb * GL, lds, s>>>(links, n, g, Q, N, pts, poff, dsafe, loss, dloss, dQ, gpb,
+                                                             nb8);
     else
-      k_sdf_pen_group<false, GL><<<blocks, gpb * GL, lds, s>>>(links, n, g, Q, N, pts, poff, dsafe, loss, dloss, dQ, gpb);
+      k_sdf_pen_group<false, GL><<<blocks, gpb * GL, lds, s>>>(links, n, g, Q, N, pts, poff, dsafe, loss, dloss, dQ,
+                                                              gpb, nb8);
   };
   switch (GLr) {
     case 8: go(std::integral_constant<int, 8>{}); break;
