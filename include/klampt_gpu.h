@@ -432,8 +432,10 @@ int kg_feasible_multi_dev(int nbatch, const kg_world* worlds, const int32_t* rob
  * uncertain / undecided and re-check counts summed).  A feasibility batch
  * whose pair items exceed the fixed-capacity layout (2 GiB) is rejected
  * (KG_E_INVALID); the pipelines and calls of one context share its
- * workspaces, so they run one at a time in stream order.  kg_serve_captures
- * reports how many captures the pipeline has taken. */
+ * workspaces, so they run one at a time in stream order.  The worlds, their
+ * robots and geometries and the buffers must outlive the pipeline
+ * (kg_serve_destroy first).  kg_serve_captures reports how many captures the
+ * pipeline has taken. */
 int kg_serve_create(int nbatch, const kg_world* worlds, const int32_t* robot_indices, const int32_t* counts,
                     const float* dQ, uint8_t* d_feasible, int nedge, const kg_world* edge_worlds,
                     const int32_t* edge_robots, const int32_t* edge_counts, const float* dA, const float* dB,
