@@ -509,8 +509,9 @@ def run_secondary(args, ctx, sc, world, robot):
                     "ms": round(ts * 1e3, 3), "graph_captures": sv.captures(),
                     "eager_ms": round((th + te) * 1e3, 3),
                     "results_equal": bool(torch.equal(sv.feasible, fh) and torch.equal(sv.first_u, dfu)),
-                    "note": "one kg_serve_run (graph launch) per planner iteration; eager_ms: the C3 and C4 "
-                            "calls issued one after the other"}
+                    "note": "one kg_serve_run (graph launch) per planner iteration, the feasibility batch and the "
+                            "edge rounds as two branches of the graph; eager_ms: the C3 and C4 calls issued one "
+                            "after the other"}
     sv.close()
     print(f"[bench] serve: {out['serve']}", file=sys.stderr, flush=True)
 

@@ -3206,7 +3206,12 @@ struct kg_serve_s {
   int norm_inf = 0;
   int max_steps = 0;
   DevBuf stat;  // ST_WORDS ints: the status words of the whole pipeline
-  hipStream_t cs = nullptr;
+  DevBuf estat;  // ST_WORDS ints: the edge parts' status (their own branch)
+  // the feasibility batch's workspaces when it runs beside the edges
+  kg_ctx_s::FeasSlot fws;
+  hipStream_t cs = nullptr, cs2 = nullptr;
+  hipEvent_t eFork = nullptr, eJoin = nullptr;
+  bool fork = false;  // two branches: the feasibility batch beside the edge batches
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   // what the captured graph baked in
@@ -3225,22 +3230,63 @@ void serve_drop_graph(kg_serve_s* sv) {
   sv->graph = nullptr;
 }
 
-// The pipeline on stream s: feasibility batch, then each edge group, each
-// part's status words folded into sv->stat.
+// Everything the pipeline owns (after its streams have drained).
+void serve_release(kg_serve_s* sv) {
+  if (sv->cs) (void)hipStreamSynchronize(sv->cs);
+  if (sv->cs2) (void)hipStreamSynchronize(sv->cs2);
+  serve_drop_graph(sv);
+  if (sv->cs) (void)hipStreamDestroy(sv->cs);
+  if (sv->cs2) (void)hipStreamDestroy(sv->cs2);
+  if (sv->eFork) (void)hipEventDestroy(sv->eFork);
+  if (sv->eJoin) (void)hipEventDestroy(sv->eJoin);
+  sv->cs = sv->cs2 = nullptr;
+  sv->eFork = sv->eJoin = nullptr;
+  DevBuf* bufs[] = {&sv->stat, &sv->estat, &sv->fws.status, &sv->fws.jl, &sv->fws.counts, &sv->fws.redo,
+                    &sv->fws.items, &sv->fws.recs, &sv->fws.spill};
+  for (auto* b : bufs) b->release();
+}
+
+// The pipeline on stream s.  With sv->fork, two branches: the feasibility
+// batch on s (on the pipeline's own workspaces) and the edge batches on cs2
+// (the context's), forked and joined with events, so that the edge rounds,
+// bound by the traversal's critical path, overlap the batch; otherwise one
+// after the other.  Each branch folds its status words into its own
+// accumulator; the edge branch's is added after the join.
 void serve_issue(kg_serve_s* sv, hipStream_t s) {
   kg_ctx_s* c = sv->c;
-  bool first = true;
+  const bool fork = sv->fork;
+  hipStream_t se = fork ? sv->cs2 : s;
+  if (fork) {
+    hip_check(hipEventRecord(sv->eFork, s), "hipEventRecord");
+    hip_check(hipStreamWaitEvent(se, sv->eFork, 0), "hipStreamWaitEvent");
+  }
+  bool have_feas = false;
   if (!sv->fw.empty()) {
     size_t N = 0;
     Layout L = multi_layout(c, (int)sv->fw.size(), sv->fw.data(), sv->fri.data(), sv->fcnt.data(), s, &N);
     need(L.ncap * sizeof(PairItem) <= (2048ull << 20), "serving feasibility batch exceeds the item capacity limit");
     if (N > 0) {
-      feasible_core(c, L, sv->dQ, nullptr, (int)N, sv->dfeas, nullptr, nullptr, nullptr, s, nullptr, false);
-      hip_check(kgl_status_accum(c->wsStatus.as<int>(), sv->stat.as<int>(), ST_SPILL, ST_UNC, ST_REDO, 1, s),
+      FeasWs ws;
+      if (fork) {
+        kg_ctx_s::FeasSlot& f = sv->fws;
+        ws.status = f.status.as<int>();
+        ws.jl = f.jl.as<uint8_t>();
+        ws.counts = f.counts.as<int>();
+        ws.redo = f.redo.as<int>();
+        ws.items = f.items.as<PairItem>();
+        ws.recs = f.recs.as<ItemRec>();
+        ws.spill = f.spill.as<uint64_t>();
+        ws.spill_entries = (int)std::min<size_t>(f.spill.bytes / sizeof(uint64_t), (size_t)INT32_MAX);
+      }
+      feasible_core(c, L, sv->dQ, nullptr, (int)N, sv->dfeas, nullptr, nullptr, nullptr, s, nullptr, false,
+                    fork ? &ws : nullptr);
+      hip_check(kgl_status_accum(fork ? ws.status : c->wsStatus.as<int>(), sv->stat.as<int>(), ST_SPILL, ST_UNC,
+                                 ST_REDO, 1, s),
                 "status");
-      first = false;
+      have_feas = true;
     }
   }
+  bool have_edges = false;
   size_t qo = 0, so = 0;
   for (size_t g = 0; g < sv->ew.size(); g++) {
     kg_world_s* w = sv->ew[g];
@@ -3248,16 +3294,52 @@ void serve_issue(kg_serve_s* sv, hipStream_t s) {
     const int cnt = sv->ecnt[g];
     if (cnt > 0) {
       edge_core(w, sv->eri[g], sv->dA + qo, sv->dB + qo, nullptr, nullptr, cnt, sv->eps, sv->norm_inf,
-                edge_max_pos(w->robots[sv->eri[g]], sv->eps, sv->norm_inf, sv->max_steps), sv->dU + so, nullptr, s);
-      hip_check(kgl_status_accum(c->wsEStat.as<int>(), sv->stat.as<int>(), ST_SPILL, ST_UNC, ST_REDO,
-                                 first ? 1 : 0, s),
+                edge_max_pos(w->robots[sv->eri[g]], sv->eps, sv->norm_inf, sv->max_steps), sv->dU + so, nullptr,
+                se);
+      hip_check(kgl_status_accum(c->wsEStat.as<int>(), sv->estat.as<int>(), ST_SPILL, ST_UNC, ST_REDO,
+                                 have_edges ? 0 : 1, se),
                 "status");
-      first = false;
+      have_edges = true;
     }
     qo += (size_t)cnt * n;
     so += (size_t)cnt;
   }
-  if (first) hip_check(kgl_fill_int(sv->stat.as<int>(), ST_WORDS, 0, s), "clear status");
+  if (fork) {
+    hip_check(hipEventRecord(sv->eJoin, se), "hipEventRecord");
+    hip_check(hipStreamWaitEvent(s, sv->eJoin, 0), "hipStreamWaitEvent");
+  }
+  if (have_edges)
+    hip_check(kgl_status_accum(sv->estat.as<int>(), sv->stat.as<int>(), ST_SPILL, ST_UNC, ST_REDO, have_feas ? 0 : 1,
+                               s),
+              "status");
+  if (!have_feas && !have_edges) hip_check(kgl_fill_int(sv->stat.as<int>(), ST_WORDS, 0, s), "clear status");
+}
+
+// Whether the feasibility batch can run beside the edges (its own workspaces:
+// the fixed-capacity, per-link broad-phase path), and those workspaces sized.
+void serve_size_branches(kg_serve_s* sv) {
+  kg_ctx_s* c = sv->c;
+  sv->fork = false;
+  size_t N = 0, nseg = 0;
+  for (int x : sv->fcnt) N += x;
+  for (int x : sv->ecnt) nseg += x;
+  if (N == 0 || nseg == 0 || !c->prm.prep_group) return;
+  Layout L = multi_layout(c, (int)sv->fw.size(), sv->fw.data(), sv->fri.data(), sv->fcnt.data(), c->stream, &N);
+  if (L.maxlinks > 32 || L.ncap * sizeof(PairItem) > (2048ull << 20)) return;
+  const kg_params& prm = c->prm;
+  const int scap = prm.scap == 256 ? 256 : 512;
+  const int grid32 = prm.waves_per_cu > 0 ? prm.waves_per_cu * c->num_cus : kgl_collide_grid(0, c->num_cus, scap);
+  const int grid64 = std::min(kgl_collide_grid(1, c->num_cus, scap), std::max(1, c->num_cus));
+  const size_t spill_entries = (size_t)std::max(grid32, grid64) * prm.spill_cap;
+  kg_ctx_s::FeasSlot& f = sv->fws;
+  hip_check(f.status.ensure(ST_WORDS * sizeof(int)), "hipMalloc(status)");
+  hip_check(f.jl.ensure(N), "hipMalloc(jl)");
+  hip_check(f.counts.ensure((N + 1) * sizeof(int)), "hipMalloc(counts)");
+  hip_check(f.redo.ensure((N + 16) * sizeof(int)), "hipMalloc(redo)");
+  hip_check(f.items.ensure(std::max<size_t>(1, L.ncap) * sizeof(PairItem)), "hipMalloc(items)");
+  hip_check(f.recs.ensure(std::max<size_t>(1, L.ncap) * sizeof(ItemRec)), "hipMalloc(records)");
+  hip_check(f.spill.ensure(std::max<size_t>(1, spill_entries) * sizeof(uint64_t)), "hipMalloc(spill)");
+  sv->fork = true;
 }
 
 // Plan versions of every (world, robot) of the pipeline; rebuilding a plan
@@ -3278,10 +3360,12 @@ void serve_capture(kg_serve_s* sv) {
   serve_drop_graph(sv);
   for (int attempt = 0;; attempt++) {
     need(attempt < 3, "serving pipeline: workspaces kept growing during capture");
+    serve_size_branches(sv);
     {
       StreamScope scope(c, sv->cs);
       serve_issue(sv, sv->cs);
     }
+    hip_check(hipStreamSynchronize(sv->cs2), "sync");
     hip_check(hipStreamSynchronize(sv->cs), "sync");
     const std::vector<int> ver = serve_versions(sv);
     const uint64_t g0 = g_devbuf_gen.load();
@@ -3362,13 +3446,15 @@ extern "C" int kg_serve_create(int nbatch, const kg_world* worlds, const int32_t
     sv->max_steps = max_steps;
     hip_check(hipSetDevice(c->device), "hipSetDevice");
     hip_check(sv->stat.ensure(ST_WORDS * sizeof(int)), "hipMalloc(status)");
+    hip_check(sv->estat.ensure(ST_WORDS * sizeof(int)), "hipMalloc(status)");
     hip_check(hipStreamCreateWithFlags(&sv->cs, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipStreamCreateWithFlags(&sv->cs2, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipEventCreateWithFlags(&sv->eFork, hipEventDisableTiming | hipEventDisableSystemFence), "hipEventCreate");
+    hip_check(hipEventCreateWithFlags(&sv->eJoin, hipEventDisableTiming | hipEventDisableSystemFence), "hipEventCreate");
     try {
       serve_capture(sv.get());
     } catch (...) {
-      serve_drop_graph(sv.get());
-      (void)hipStreamDestroy(sv->cs);
-      sv->stat.release();
+      serve_release(sv.get());
       throw;
     }
     *out = sv.release();
@@ -3408,11 +3494,8 @@ extern "C" int kg_serve_destroy(kg_serve sv) {
   return guard([&] {
     if (!sv) return;
     (void)hipSetDevice(sv->c->device);
-    (void)hipStreamSynchronize(sv->cs);
     if (sv->c->done_valid) (void)hipEventSynchronize(sv->c->evDone);
-    serve_drop_graph(sv);
-    (void)hipStreamDestroy(sv->cs);
-    sv->stat.release();
+    serve_release(sv);
     delete sv;
   });
 }
