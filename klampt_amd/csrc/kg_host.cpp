@@ -9,7 +9,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <list>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -1905,10 +1907,15 @@ struct PairWorld {
   kg_robot_s* r = nullptr;
   kg_world_s* w = nullptr;
 };
-std::vector<std::pair<const kg_ctx_s*, PairWorld>> g_pairworlds;
+// One entry per context, shared by every context of the process: a list
+// (entries stay in place as others come and go) under a mutex; each entry is
+// used only by its own context's calls.
+std::list<std::pair<const kg_ctx_s*, PairWorld>> g_pairworlds;
+std::mutex g_pairworlds_mu;
 
 PairWorld& pair_world(kg_ctx_s* c, kg_geom_s* a, kg_geom_s* b) {
   PairWorld* pw = nullptr;
+  std::lock_guard<std::mutex> lock(g_pairworlds_mu);
   for (auto& p : g_pairworlds)
     if (p.first == c) pw = &p.second;
   if (!pw) {
@@ -1952,6 +1959,7 @@ PairWorld& pair_world(kg_ctx_s* c, kg_geom_s* a, kg_geom_s* b) {
 // The context's internal pair world goes with the context (its geometry
 // slots hold no references: they are set anew before every query).
 void forget_pair_world(const kg_ctx_s* c) {
+  std::lock_guard<std::mutex> lock(g_pairworlds_mu);
   for (auto it = g_pairworlds.begin(); it != g_pairworlds.end(); ++it)
     if (it->first == c) {
       release_plans(it->second.w);
