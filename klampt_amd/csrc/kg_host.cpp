@@ -443,12 +443,16 @@ struct BatchStats {
 
 // Status words of one batch (device, zeroed by the prep kernel):
 //   [0] spill overflow  [1] FP64 re-check count  [2] FP32 chunk counter
-//   [3] FP64 chunk counter  [4..9] u64 counters: box tests, triangle tests,
-//   pair items  [10] uncertain results (work dropped on overflow);
+//   [3] FP64 chunk counter  [10] uncertain results (work dropped on
+//   overflow);
 // then the traversal's chunk counters, one per 128-byte line so that waves
-// fetching from different ranges do not queue on one address
-enum { ST_SPILL = 0, ST_REDO = 1, ST_CHUNK32 = 2, ST_CHUNK64 = 3, ST_CNT = 4, ST_UNC = 10, ST_DYN = 32,
-       ST_NSUB = 32, ST_WORDS = ST_DYN + 32 * ST_NSUB };
+// fetching from different ranges do not queue on one address, a "drained"
+// flag per range (and one for the FP64 pass), again one per line, and
+// KG_CNT_COPIES lines of u64 statistics (box tests, triangle tests, pair
+// items), each wave adding to the copy blockIdx % KG_CNT_COPIES
+enum { ST_SPILL = 0, ST_REDO = 1, ST_CHUNK32 = 2, ST_CHUNK64 = 3, ST_UNC = 10, ST_DYN = 32, ST_NSUB = 32,
+       ST_DRAIN = ST_DYN + 32 * ST_NSUB, ST_DRAIN64 = ST_DRAIN + 32 * ST_NSUB, ST_CNT = ST_DRAIN64 + 32,
+       ST_WORDS = ST_CNT + 32 * KG_CNT_COPIES };
 void ensure_status_host(kg_ctx_s* c) {
   if (!c->hstatus) {
     hip_check(hipHostMalloc((void**)&c->hstatus, ST_WORDS * sizeof(int), hipHostMallocDefault), "hipHostMalloc");
@@ -646,6 +650,7 @@ void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* 
   ta.mode = mode;
   ta.tol = L.tol32;
   ta.chunk_counter = status + ST_DYN;
+  ta.chunk_drained = status + ST_DRAIN;
   {
     const int nsub = std::max(1, std::min((int)ST_NSUB, prm.nsub));
     ta.nsub = N >= 64 * nsub ? nsub : 1;
@@ -708,6 +713,7 @@ void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* 
   tb.grid = grid64;
   tb.counters = nullptr;
   tb.chunk_counter = status + ST_CHUNK64;
+  tb.chunk_drained = status + ST_DRAIN64;
   tb.static_pct = 0;
   tb.nsub = 1;
   tb.fin.final_pass = 1;
@@ -766,11 +772,13 @@ void read_stats(kg_world_s* w, BatchStats& st) {
   st.spills = h[ST_SPILL];
   st.redo = h[ST_REDO];
   st.uncertain = h[ST_UNC];
-  unsigned long long cnt[3];
-  std::memcpy(cnt, h + ST_CNT, sizeof(cnt));
-  st.box_tests = (int64_t)cnt[0];
-  st.tri_tests = (int64_t)cnt[1];
-  st.items = (int64_t)cnt[2];
+  for (int k = 0; k < KG_CNT_COPIES; k++) {
+    unsigned long long cnt[3];
+    std::memcpy(cnt, h + ST_CNT + 32 * k, sizeof(cnt));
+    st.box_tests += (int64_t)cnt[0];
+    st.tri_tests += (int64_t)cnt[1];
+    st.items += (int64_t)cnt[2];
+  }
   float ms = 0;
   if (c->last_tr_slot >= 0) {
     if (hipEventElapsedTime(&ms, c->tr_beg[c->last_tr_slot], c->tr_end[c->last_tr_slot]) == hipSuccess)

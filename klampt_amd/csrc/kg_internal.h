@@ -17,6 +17,10 @@ struct BuiltTree {
 };
 void kg_build_obb_tree(const double* V, int nv, const int32_t* T, int nt, BuiltTree& out);
 
+// Copies of the traversal statistics (128-byte lines of the status words):
+// a wave adds its counts to copy blockIdx % KG_CNT_COPIES.
+#define KG_CNT_COPIES 16
+
 // Count of device-buffer frees in the process (a buffer growing frees its
 // old block): a captured HIP graph holds workspace addresses, so its owner
 // recaptures when the count has moved since the capture.  A first allocation
@@ -184,6 +188,7 @@ struct TravArgs {
   int mode;
   Tol tol;
   int* chunk_counter;
+  int* chunk_drained;         // per range (same stride): set once a wave found it drained
   uint64_t* spill;            // per-wave global overflow stacks
   int spill_cap;              // entries per wave
   int* status;                // [0]: spill overflow flag

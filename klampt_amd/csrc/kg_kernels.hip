@@ -27,6 +27,7 @@
 #define KG_CPW_MAX 64
 #define KG_NE 64  // configuration entries per wave (ring)
 #define KG_COUNTER_STRIDE 32   // ints between the traversal's chunk counters (128 B)
+#define KG_MIN_WAVES 64        // traversal waves kept for the smallest batches
 #define KG_PREP_LDS_LINKS 8    // link boxes in LDS for robots of at most this many links
 #define KG_PREP_LDS_BLOCK 128  // block size of that variant (48 KB of LDS)
 #define KG_NONE64 0xffffffffffffffffull
@@ -1100,6 +1101,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SCAP == 256 
   // nothing to do (typically the FP64 pass when no configuration was left
   // undecided): leave before the set-up, the whole block at once
   if (ncfg <= 0) return;
+  // a small batch runs on about one wave per configuration (at least
+  // KG_MIN_WAVES): every further wave would only probe the counters
+  const int nw = min((int)gridDim.x, max(KG_MIN_WAVES, ncfg));
+  if ((int)blockIdx.x >= nw) return;
   unsigned n_box = 0, n_tri = 0, n_items = 0;
 
   for (int i = lane; i < NE; i += 64) {
@@ -1123,11 +1128,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SCAP == 256 
   // ring positions (monotonic, wave-uniform): [head, tail) fetched entries,
   // fe = entry whose items are being started, [it, itend) its next items
   // static share: S0 configurations per wave, the rest dynamic from dyn0
-  const int S0 = (int)(((long long)ncfg * a.static_pct / 100) / (int)gridDim.x);
-  const int dyn0 = S0 * (int)gridDim.x;
+  const int S0 = (int)(((long long)ncfg * a.static_pct / 100) / nw);
+  const int dyn0 = S0 * nw;
   const int nsub = max(1, a.nsub);
   const int rlen = (ncfg - dyn0 + nsub - 1) / nsub;  // configurations per dynamic range
-  const int wpr = max(1, (int)gridDim.x / nsub);     // waves per range
+  const int wpr = max(1, nw / nsub);                 // waves per range
   int sub = (int)blockIdx.x % nsub, tried = 0;
   int head = 0, tail = 0, fe = 0, it = 0, itend = 0, last_k0 = dyn0 + sub * rlen, soff = 0;
   bool fe_open = false, exhausted = false;
@@ -1290,15 +1295,29 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SCAP == 256 
               break;
             }
             const int r0 = dyn0 + sub * rlen, r1 = min(ncfg, r0 + rlen);
-            const int ch = min(room, max(1, min(a.cpw, (r1 - last_k0) / wpr)));
-            if (lane == 0) s_chunk = r0 + atomicAdd(a.chunk_counter + sub * KG_COUNTER_STRIDE, ch);
-            __syncthreads();
-            k0 = __builtin_amdgcn_readfirstlane(s_chunk);
-            __syncthreads();
-            if (k0 < r1) {
-              last_k0 = k0;
-              nk = min(ch, r1 - k0);
-              break;
+            // A range that is empty, or that a wave already found drained,
+            // is passed without an atomic: atomics on one address from all
+            // waves queue behind each other (tens of ns each), so thousands
+            // of waves each probing every drained counter would cost the
+            // launch about 0.2 ms however little work it holds.  The flag is
+            // a hint read through the caches (a stale 0 costs one probe).
+            int* drained = a.chunk_drained + sub * KG_COUNTER_STRIDE;
+            bool dry = r0 >= r1;
+            if (!dry)
+              dry = __builtin_amdgcn_readfirstlane(
+                        __hip_atomic_load(drained, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) != 0;
+            if (!dry) {
+              const int ch = min(room, max(1, min(a.cpw, (r1 - last_k0) / wpr)));
+              if (lane == 0) s_chunk = r0 + atomicAdd(a.chunk_counter + sub * KG_COUNTER_STRIDE, ch);
+              __syncthreads();
+              k0 = __builtin_amdgcn_readfirstlane(s_chunk);
+              __syncthreads();
+              if (k0 < r1) {
+                last_k0 = k0;
+                nk = min(ch, r1 - k0);
+                break;
+              }
+              if (lane == 0) __hip_atomic_store(drained, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
             sub = sub + 1 == nsub ? 0 : sub + 1;
             tried++;
@@ -1512,10 +1531,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SCAP == 256 
     __syncthreads();
     if (nl >= a.leaf_min) run_leaves();
   }
-  if (a.counters) {
-    atomicAdd(&a.counters[0], (unsigned long long)n_box);
-    atomicAdd(&a.counters[1], (unsigned long long)n_tri);
-    atomicAdd(&a.counters[2], (unsigned long long)n_items);
+  if (a.counters && (n_box | n_tri | n_items)) {
+    // statistics: waves add to one of KG_CNT_COPIES lines (atomics on one
+    // line from every wave would queue for tens of microseconds)
+    unsigned long long* cnt = a.counters + (blockIdx.x % KG_CNT_COPIES) * (KG_COUNTER_STRIDE / 2);
+    atomicAdd(&cnt[0], (unsigned long long)n_box);
+    atomicAdd(&cnt[1], (unsigned long long)n_tri);
+    atomicAdd(&cnt[2], (unsigned long long)n_items);
   }
 }
 
@@ -2437,9 +2459,10 @@ __global__ __launch_bounds__(64) void k_distance(DistArgs a) {
     }
     __syncthreads();
   }
-  if (a.counters) {
-    atomicAdd(&a.counters[0], (unsigned long long)n_box);
-    atomicAdd(&a.counters[1], (unsigned long long)n_tri);
+  if (a.counters && (n_box | n_tri)) {
+    unsigned long long* cnt = a.counters + (blockIdx.x % KG_CNT_COPIES) * (KG_COUNTER_STRIDE / 2);
+    atomicAdd(&cnt[0], (unsigned long long)n_box);
+    atomicAdd(&cnt[1], (unsigned long long)n_tri);
   }
 }
 
