@@ -76,7 +76,7 @@ typedef struct {
   int32_t leaf_min;      /* queued leaf pairs that start a triangle-test step, 1..64 (1) */
   int32_t scap;          /* LDS task-pool entries per traversal wave: 256 or 512 (256) */
   int32_t waves_per_cu;  /* traversal waves per CU, 0 = the occupancy limit (0) */
-  int32_t static_pct;    /* percent of a batch dealt to the waves up front, 0..100 (25) */
+  int32_t static_pct;    /* percent of a batch dealt to the waves up front, 0..100 (60) */
   int32_t nsub;          /* ranges of the rest of the batch, one counter each, 1..32 (8) */
   int32_t tail_lanes;    /* a wave with fewer working lanes runs the tail cadence, 0..64 (16) */
   int32_t tail_reps;     /* box-test iterations per scheduling step in the tail, 1..8 (4) */
@@ -107,7 +107,7 @@ int kg_ctx_set_params(kg_ctx ctx, const kg_params* params);
  *   [1] results left uncertain by dropped tasks: configurations reported
  *       infeasible (their open flags set, first pair (-2,-2)) without proof,
  *       or pair distances reported as a certified lower bound;
- *   [2] configurations re-checked in FP64;  [3] 0.
+ *   [2] configurations traversed again in FP64;  [3] 0.
  * When [1] is 0 every result of the call is exact.  Host-pointer calls fail
  * with KG_E_LIMIT instead when [1] is not 0.  No Klampt counterpart. */
 int kg_ctx_status_dev(kg_ctx ctx, int32_t* d_out, void* stream);
@@ -455,12 +455,15 @@ int kg_autotune(int nbatch, const kg_world* worlds, const int32_t* robot_indices
 int kg_ctx_set_tuning(kg_ctx ctx, int cpw, int leaf_min, int scap, int waves_per_cu);
 
 /* Counters of the last batched call on this world's context (waits for it):
- * [0] configurations re-checked in FP64, [1] narrow-phase pair items, [2] 1
+ * [0] configurations traversed again in FP64, [1] narrow-phase pair items, [2] 1
  * if a spill region filled, [3] broad-phase time (ns), [4] FP32 traversal
  * kernel time (ns), [5] FP64 re-check kernel time (ns) -- HIP events on the
  * launch stream; [3] and [5] need timing level 2, [4] level 1 or 2
  * (kg_ctx_set_timing), else 0 -- [6] box tests, [7] triangle tests of the
- * FP32 traversal, [8] uncertain results (kg_ctx_status_dev [1]). */
+ * FP32 traversal, [8] uncertain results (kg_ctx_status_dev [1]), [9]
+ * triangle pairs the FP32 test left undecided, each decided by the FP64
+ * pass's exact test alone (any-hit and group modes; in the ordered
+ * first-pair mode such a configuration is traversed again, [0]). */
 int kg_world_last_stats(kg_world w, int64_t* stats, int nstats);
 /* Sum of the FP32 traversal kernel durations (HIP events on the launch
  * stream) of the feasibility batches issued on this context since the last

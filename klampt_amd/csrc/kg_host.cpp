@@ -438,21 +438,27 @@ void release_plans(kg_world_s* w) {
 
 struct BatchStats {
   int64_t redo = 0, items = 0, spills = 0;
-  int64_t t_prep_ns = 0, t_trav32_ns = 0, t_trav64_ns = 0, box_tests = 0, tri_tests = 0, uncertain = 0;
+  int64_t t_prep_ns = 0, t_trav32_ns = 0, t_trav64_ns = 0, box_tests = 0, tri_tests = 0, uncertain = 0, xpairs = 0;
 };
 
 // Status words of one batch (device, zeroed by the prep kernel):
 //   [0] spill overflow  [1] FP64 re-check count  [2] FP32 chunk counter
-//   [3] FP64 chunk counter  [10] uncertain results (work dropped on
-//   overflow);
+//   [3] FP64 chunk counter  [4] undecided triangle pairs recorded for the
+//   exact test  [10] uncertain results (work dropped on overflow);
 // then the traversal's chunk counters, one per 128-byte line so that waves
 // fetching from different ranges do not queue on one address, a "drained"
 // flag per range (and one for the FP64 pass), again one per line, and
 // KG_CNT_COPIES lines of u64 statistics (box tests, triangle tests, pair
 // items), each wave adding to the copy blockIdx % KG_CNT_COPIES
-enum { ST_SPILL = 0, ST_REDO = 1, ST_CHUNK32 = 2, ST_CHUNK64 = 3, ST_UNC = 10, ST_DYN = 32, ST_NSUB = 32,
+enum { ST_SPILL = 0, ST_REDO = 1, ST_CHUNK32 = 2, ST_CHUNK64 = 3, ST_XPAIR = 4, ST_UNC = 10, ST_DYN = 32, ST_NSUB = 32,
        ST_DRAIN = ST_DYN + 32 * ST_NSUB, ST_DRAIN64 = ST_DRAIN + 32 * ST_NSUB, ST_CNT = ST_DRAIN64 + 32,
        ST_WORDS = ST_CNT + 32 * KG_CNT_COPIES };
+
+// The re-check list of a batch of N configurations (N + 16 ints), then, at a
+// 16-byte boundary, KG_XPAIR_CAP records of undecided triangle pairs for the
+// FP64 pass's exact test.
+inline size_t redo_head(size_t N) { return (N + 16 + 3) & ~(size_t)3; }
+inline size_t redo_ints(size_t N) { return redo_head(N) + 4 * (size_t)KG_XPAIR_CAP; }
 void ensure_status_host(kg_ctx_s* c) {
   if (!c->hstatus) {
     hip_check(hipHostMalloc((void**)&c->hstatus, ST_WORDS * sizeof(int), hipHostMallocDefault), "hipHostMalloc");
@@ -511,7 +517,7 @@ struct FeasWs {
   int* status;       // ST_WORDS
   uint8_t* jl;       // N
   int* counts;       // N + 1
-  int* redo;         // N + 16
+  int* redo;         // redo_ints(N): re-check list, then the undecided pairs
   PairItem* items;   // N x cap
   ItemRec* recs;     // N x cap
   uint64_t* spill;   // spill region, split evenly over the traversal waves
@@ -555,7 +561,7 @@ void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* 
     hip_check(c->wsCounts.ensure((size_t)(N + 1) * sizeof(int)), "hipMalloc(counts)");
     hip_check(c->wsOffsets.ensure((size_t)(N + 1) * sizeof(int)), "hipMalloc(offsets)");
     hip_check(c->wsStatus.ensure(ST_WORDS * sizeof(int)), "hipMalloc(status)");
-    hip_check(c->wsRedo.ensure((size_t)(N + 16) * sizeof(int)), "hipMalloc(redo)");
+    hip_check(c->wsRedo.ensure(redo_ints(N) * sizeof(int)), "hipMalloc(redo)");
   }
   int* status = ws ? ws->status : c->wsStatus.as<int>();
   int* redo_list = ws ? ws->redo : c->wsRedo.as<int>();
@@ -694,6 +700,14 @@ void feasible_core(kg_ctx_s* c, const Layout& L, const float* Qf, const double* 
   fa.final_pass = 0;
   fa.force_recheck = prm.force_recheck;
   fa.uncertain = status + ST_UNC;
+  // triangle pairs the FP32 test leaves undecided are decided one by one by
+  // the FP64 pass's exact test, which can only add hits (any-hit and group
+  // modes); the ordered first-pair mode traverses such a configuration again
+  if (mode != KG_MODE_ORDERED) {
+    fa.xpairs = reinterpret_cast<int4*>(redo_list + redo_head(N));
+    fa.xpair_count = status + ST_XPAIR;
+    fa.xpair_cap = KG_XPAIR_CAP;
+  }
   const int slot = (int)(c->tr_count % kg_ctx_s::TRING);
   if (kern_ev) hip_check(hipEventRecord(c->tr_beg[slot], s), "hipEventRecord");
   hip_check(kgl_collide_f32(ta, s), "collide_f32");
@@ -772,6 +786,7 @@ void read_stats(kg_world_s* w, BatchStats& st) {
   st.spills = h[ST_SPILL];
   st.redo = h[ST_REDO];
   st.uncertain = h[ST_UNC];
+  st.xpairs = h[ST_XPAIR];
   for (int k = 0; k < KG_CNT_COPIES; k++) {
     unsigned long long cnt[3];
     std::memcpy(cnt, h + ST_CNT + 32 * k, sizeof(cnt));
@@ -1649,7 +1664,7 @@ void feasible_dev_slot(kg_world_s* w, int ri, const float* Qf, const double* Qd,
   hip_check(f.status.ensure(ST_WORDS * sizeof(int)), "hipMalloc(status)");
   hip_check(f.jl.ensure((size_t)N), "hipMalloc(jl)");
   hip_check(f.counts.ensure((size_t)(N + 1) * sizeof(int)), "hipMalloc(counts)");
-  hip_check(f.redo.ensure((size_t)(N + 16) * sizeof(int)), "hipMalloc(redo)");
+  hip_check(f.redo.ensure(redo_ints(N) * sizeof(int)), "hipMalloc(redo)");
   hip_check(f.items.ensure(std::max<size_t>(1, L.ncap) * sizeof(PairItem)), "hipMalloc(items)");
   hip_check(f.recs.ensure(std::max<size_t>(1, L.ncap) * sizeof(ItemRec)), "hipMalloc(records)");
   hip_check(f.spill.ensure(std::max<size_t>(1, spill_entries) * sizeof(uint64_t)), "hipMalloc(spill)");
@@ -1730,7 +1745,7 @@ int kg_feasible_batch_dev_noalloc(kg_world w, int ri, const float* dQ, int N, ui
     auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
     const size_t b_status = al(ST_WORDS * sizeof(int)), b_spill = (size_t)16 << 20;
     const size_t b_jl = al((size_t)N), b_counts = al((size_t)(N + 1) * sizeof(int)),
-                 b_redo = al((size_t)(N + 16) * sizeof(int)), b_items = al(L.ncap * sizeof(PairItem)),
+                 b_redo = al(redo_ints(N) * sizeof(int)), b_items = al(L.ncap * sizeof(PairItem)),
                  b_recs = al(L.ncap * sizeof(ItemRec));
     const size_t total = b_status + b_spill + b_jl + b_counts + b_redo + b_items + b_recs;
     if (total > KG_NOALLOC_BYTES) {
@@ -1739,7 +1754,7 @@ int kg_feasible_batch_dev_noalloc(kg_world w, int ri, const float* dQ, int N, ui
            "the allocation-free call holds %u bytes of scratch; %d configurations of this robot and world need "
            "%zu (about %zu at most)",
            (unsigned)KG_NOALLOC_BYTES, N, total,
-           (size_t)(KG_NOALLOC_BYTES - b_status - b_spill - 5 * 256 - 17 * sizeof(int)) / per);
+           (size_t)(KG_NOALLOC_BYTES - b_status - b_spill - 5 * 256 - (redo_ints(0) + 4) * sizeof(int)) / per);
     }
     unsigned char* p = (unsigned char*)c->scratch;
     FeasWs ws;
@@ -1820,9 +1835,9 @@ int kg_world_last_stats(kg_world w, int64_t* out, int nstats) {
     need(w && out, "NULL argument");
     BatchStats st;
     read_stats(w, st);
-    int64_t v[9] = {st.redo, st.items, st.spills, st.t_prep_ns, st.t_trav32_ns, st.t_trav64_ns, st.box_tests,
-                    st.tri_tests, st.uncertain};
-    for (int i = 0; i < nstats && i < 9; i++) out[i] = v[i];
+    int64_t v[10] = {st.redo, st.items, st.spills, st.t_prep_ns, st.t_trav32_ns, st.t_trav64_ns, st.box_tests,
+                     st.tri_tests, st.uncertain, st.xpairs};
+    for (int i = 0; i < nstats && i < 10; i++) out[i] = v[i];
   });
 }
 
@@ -2734,11 +2749,15 @@ namespace {
 // (on the 8192 arm segments in the 200k soup, 3 of 4 at the midpoint), while
 // free segments must check every sample: a narrow first round wastes little
 // on the former, and fast-growing ones cover the latter in few rounds.  A
-// round costs at least the traversal's critical path (about 0.2 ms there,
+// round costs at least the traversal's critical path (0.1-0.2 ms there,
 // against 2-4 ns per sample), so few wide rounds beat many narrow ones:
-// widths 2, 32, 512 take 3 rounds to the 6 of 1, 4, 16, ...  Segments needing more positions than max_pos come out -2
-// (undecided) and are counted in the status words.  A batch of more
-// segments than the budget's samples runs as consecutive chunks.
+// widths 2, 32, 512 take 3 rounds to the 6 of 1, 4, 16, ...  A last round
+// that would be no wider than the one before it is folded into that one
+// (2, 32, 596 for a joint-range diagonal of 630 positions: the 4th round
+// would only cover 84 positions, and every round costs its launches).
+// Segments needing more positions than max_pos come out -2 (undecided) and
+// are counted in the status words.  A batch of more segments than the
+// budget's samples runs as consecutive chunks.
 void edge_core(kg_world_s* w, int ri, const float* Af, const float* Bf, const double* Ad, const double* Bd, int nseg,
                double eps, int norm_inf, int max_pos, float* first_u, int32_t* nsamples, hipStream_t s) {
   kg_ctx_s* c = w->ctx;
@@ -2821,7 +2840,9 @@ void edge_core(kg_world_s* w, int ri, const float* Af, const float* Bf, const do
     int64_t covered = 0;
     int64_t wr = prm.edge_w0;
     for (int r = 0; covered < max_pos; r++, wr = std::min<int64_t>(wr * prm.edge_growth, 1 << 20)) {
-      const int width = (int)wr;
+      int64_t wf = wr;
+      if (covered + wf < max_pos && max_pos - covered - wf <= wf) wf = max_pos - covered;
+      const int width = (int)std::min<int64_t>(wf, 1 << 20);
       covered += std::min(width, floor_w);
       hip_check(kgl_edge_round(ea, width, 0, s), "edge_plan");
       feasible_core(c, L, nullptr, ea.Qd, B, c->wsEFeas.as<uint8_t>(), nullptr, nullptr, nullptr, s,
@@ -3351,7 +3372,7 @@ void serve_size_branches(kg_serve_s* sv) {
   hip_check(f.status.ensure(ST_WORDS * sizeof(int)), "hipMalloc(status)");
   hip_check(f.jl.ensure(N), "hipMalloc(jl)");
   hip_check(f.counts.ensure((N + 1) * sizeof(int)), "hipMalloc(counts)");
-  hip_check(f.redo.ensure((N + 16) * sizeof(int)), "hipMalloc(redo)");
+  hip_check(f.redo.ensure(redo_ints(N) * sizeof(int)), "hipMalloc(redo)");
   hip_check(f.items.ensure(std::max<size_t>(1, L.ncap) * sizeof(PairItem)), "hipMalloc(items)");
   hip_check(f.recs.ensure(std::max<size_t>(1, L.ncap) * sizeof(ItemRec)), "hipMalloc(records)");
   hip_check(f.spill.ensure(std::max<size_t>(1, spill_entries) * sizeof(uint64_t)), "hipMalloc(spill)");

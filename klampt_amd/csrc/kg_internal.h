@@ -20,6 +20,10 @@ void kg_build_obb_tree(const double* V, int nv, const int32_t* T, int nt, BuiltT
 // Copies of the traversal statistics (128-byte lines of the status words):
 // a wave adds its counts to copy blockIdx % KG_CNT_COPIES.
 #define KG_CNT_COPIES 16
+// Triangle pairs the FP32 test left undecided, per batch, that the FP64
+// pass decides one by one (beyond this count a configuration is traversed
+// again in FP64 instead).
+#define KG_XPAIR_CAP 1024
 
 // Count of device-buffer frees in the process (a buffer growing frees its
 // old block): a captured HIP graph holds workspace addresses, so its owner
@@ -172,6 +176,12 @@ struct FinArgs {
   int* uncertain;     // count of configurations left open by dropped work (exact pass)
   const PlanDev* plans;  // heterogeneous batches (nullable)
   const int* cfg_plan;
+  // nullable: undecided triangle pairs {configuration, item, triangle of A,
+  // triangle of B} recorded by the FP32 pass for the exact test of the FP64
+  // pass (any-hit and group modes only: a pair can then only add a hit)
+  int4* xpairs;
+  int* xpair_count;   // may exceed xpair_cap: the pairs beyond it were not recorded
+  int xpair_cap;
 };
 
 struct TravArgs {
@@ -367,7 +377,7 @@ inline kg_params kg_default_params() {
   p.leaf_min = 1;
   p.scap = 256;  // leaves LDS for 4 traversal waves per SIMD
   p.waves_per_cu = 0;
-  p.static_pct = 25;
+  p.static_pct = 60;
   p.nsub = 8;
   p.tail_lanes = 16;
   p.tail_reps = 4;

@@ -1032,6 +1032,23 @@ __device__ inline void item_frame(const TravArgs& a, const PairItem& P, Real* re
   slot_bounds<Real, EXACT>(a, sqrt(tn), GA.rmax, GB.rmax, tau, pad);
 }
 
+// The exact pass's verdict on one leaf pair (triangle ta of mesh A, tb of
+// mesh B) of item P, for lane `sub` of its 9-lane group: the item frame
+// formed in FP64 from q and the FP64 triangle test, as k_collide<double,
+// true> computes them for the pair.  Bit 0: separated.
+__device__ inline int exact_leaf(const TravArgs& a, const PairItem& P, const PoolDev& pool, int ta, int tb, int sub) {
+  double rel[12];
+  float tau, pad;
+  int rootA, rootB;
+  item_frame<double, true>(a, P, rel, tau, pad, rootA, rootB);
+  V3T<double> tp[3], tq[3];
+  load_tri<double>(pool, ta, tp);
+  load_tri<double>(pool, tb, tq);
+#pragma unroll
+  for (int v = 0; v < 3; v++) tp[v] = xf_apply<double>(rel, tp[v]);
+  return tri_tri_split<double, true>(tp, tq, 0.0, sub);
+}
+
 // Task word: slot(6) | side(1) | single(1) | X(28) | Y(28).
 //   single = 1: test the pair (X as A-node, Y as B-node).
 //   single = 0: X is the first of two sibling children; test (X+k, Y) when
@@ -1098,6 +1115,31 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SCAP == 256 
   const Real pad_rel = (Real)a.tol.pad_rel;
   uint64_t* gsp_base = a.spill + (size_t)blockIdx.x * a.spill_cap;
   const int ncfg = a.ncfg_dev ? *a.ncfg_dev : a.ncfg;
+  if (EXACT && a.fin.xpairs) {
+    // Undecided triangle pairs recorded by the FP32 pass, 7 per wave pass in
+    // 9-lane groups: an intersecting one makes its configuration collide
+    // (the FP32 pass finished it as if the pair were separated)
+    const int np = min(*a.fin.xpair_count, a.fin.xpair_cap);
+    const int li = lane / 9, sub = lane - 9 * li;
+    for (int b = (int)blockIdx.x * 7; b < np; b += (int)gridDim.x * 7) {
+      const int k = b + li;
+      const bool act = li < 7 && k < np;
+      int bits = 1, c = 0, g = 0;
+      if (act) {
+        const int4 x = a.fin.xpairs[k];
+        const PairItem P = items[x.y];
+        c = x.x;
+        g = (P.rank >> KG_RANK_GROUP_BIT) & 1;
+        bits = exact_leaf(a, P, rpool, x.z, x.w, sub);
+      }
+      const uint64_t sepm = ballot64((bits & 1) != 0);
+      if (act && sub == 0 && !(sepm & (0x1ffull << (9 * li)))) {
+        if (a.fin.feasible) a.fin.feasible[c] = 0;
+        if (g == 0 && a.fin.env_hit) a.fin.env_hit[c] = 1;
+        if (g == 1 && a.fin.self_hit) a.fin.self_hit[c] = 1;
+      }
+    }
+  }
   // nothing to do (typically the FP64 pass when no configuration was left
   // undecided): leave before the set-up, the whole block at once
   if (ncfg <= 0) return;
@@ -1192,12 +1234,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SCAP == 256 
     for (int b = 0; b < m; b += 7) {
       const int idx = b + li;
       const bool act = li < 7 && idx < m;
-      int slot = 0, g = 0, r = 0, ck = 0, bits = 0;
+      int slot = 0, g = 0, r = 0, ck = 0, bits = 0, ta = 0, tb = 0;
       bool live = false;
       if (act) {
         const uint64_t e = lq[nl - 1 - idx];
         slot = (int)(e >> 58);
-        const int ta = (int)((e >> 29) & 0x1fffffff), tb = (int)(e & 0x1fffffff);
+        ta = (int)((e >> 29) & 0x1fffffff);
+        tb = (int)(e & 0x1fffffff);
         const int rk = s_rank[slot];
         g = (rk >> KG_RANK_GROUP_BIT) & 1;
         r = rk & ((1 << KG_RANK_GROUP_BIT) - 1);
@@ -1222,7 +1265,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SCAP == 256 
           const uint64_t gm = 0x1ffull << (9 * li);
           if (!(sepm & gm)) {
             if (unsm & gm) {
-              s_unc[slot] = 1;
+              // undecided: recorded for the FP64 pass's exact test of this
+              // pair (the item goes on as if it were separated), or, when
+              // there is no room, the configuration is traversed again
+              int k = a.fin.xpair_cap;
+              if (!EXACT && a.fin.xpairs) k = atomicAdd(a.fin.xpair_count, 1);
+              if (k < a.fin.xpair_cap)
+                a.fin.xpairs[k] = make_int4(e_cfg[ck], s_item[slot], ta, tb);
+              else
+                s_unc[slot] = 1;
             } else {
               const unsigned long long key = ((unsigned long long)(uint32_t)r << 32) | (uint32_t)s_item[slot];
               atomicMin(&s_best[ck][g], key);

@@ -562,10 +562,10 @@ class World:
         return out
 
     def last_stats(self):
-        s = np.zeros(9, np.int64)
-        call("kg_world_last_stats", self.h, s.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), 9)
+        s = np.zeros(10, np.int64)
+        call("kg_world_last_stats", self.h, s.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), 10)
         keys = ["fp64_rechecks", "pair_items", "spills", "t_prep_ns", "t_trav32_ns", "t_trav64_ns",
-                "box_tests", "tri_tests", "uncertain"]
+                "box_tests", "tri_tests", "uncertain", "exact_pairs"]
         return {k: int(v) for k, v in zip(keys, s)}
 
 
