@@ -2634,20 +2634,31 @@ __global__ __launch_bounds__(256) void k_edge_init(EdgeArgs a, int clear_ctl) {
 // segment's sample offset.  Thread t takes segments t, t + TB, ... (coalesced
 // loads); the samples are laid out thread by thread (an exclusive scan over
 // the threads' counts), which is all the later kernels need: the records are
-// in increasing offset order.
+// in increasing offset order.  The positions each segment has left are read
+// from HBM once (loads of several segments in flight per thread) and kept in
+// LDS for the first KG_EDGE_PLAN_LDS segments: the three passes over the
+// segments otherwise wait on a chain of dependent loads per thread.
 #define KG_EDGE_PLAN_TB 1024
+#define KG_EDGE_PLAN_LDS 8192
 __global__ __launch_bounds__(KG_EDGE_PLAN_TB) void k_edge_plan(EdgeArgs a, int width) {
   using Scan = hipcub::BlockScan<int, KG_EDGE_PLAN_TB>;
   __shared__ typename Scan::TempStorage tmp;
   __shared__ int s_nlive;
+  __shared__ int s_left[KG_EDGE_PLAN_LDS];
   const int t = threadIdx.x;
-  auto left = [&](int s) {  // positions of s this schedule may still check
+  auto left_g = [&](int s) {  // positions of s this schedule may still check
     if (a.done[s]) return 0;
     const int m = min(a.ns[s] - 1, a.max_pos);
     return max(0, m - a.cur[s]);
   };
+  auto left = [&](int s) { return s < KG_EDGE_PLAN_LDS ? s_left[s] : left_g(s); };
   int nl = 0;
-  for (int s = t; s < a.nseg; s += KG_EDGE_PLAN_TB) nl += left(s) > 0;
+#pragma unroll 8
+  for (int s = t; s < a.nseg; s += KG_EDGE_PLAN_TB) {
+    const int L = left_g(s);
+    if (s < KG_EDGE_PLAN_LDS) s_left[s] = L;
+    nl += L > 0;
+  }
   int nlive_pre, nlive;
   Scan(tmp).ExclusiveSum(nl, nlive_pre, nlive);
   if (t == 0) s_nlive = nlive;
@@ -2676,9 +2687,9 @@ __global__ __launch_bounds__(KG_EDGE_PLAN_TB) void k_edge_plan(EdgeArgs a, int w
   }
 }
 
-// Sample j of the round: segment by binary search over the offsets, its
+// Sample j of a wide round: segment by binary search over the offsets, its
 // position cursor + (j - offset) in bisection order, q = a (1-u) + b u.
-__global__ __launch_bounds__(256) void k_edge_samples(EdgeArgs a) {
+__global__ __launch_bounds__(256) void k_edge_samples_flat(EdgeArgs a) {
   const int nlive = a.ctl[0], total = a.ctl[1];
   for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < total; j += gridDim.x * blockDim.x) {
     int lo = 0, hi = nlive - 1;
@@ -2695,6 +2706,30 @@ __global__ __launch_bounds__(256) void k_edge_samples(EdgeArgs a) {
       double q = edge_q(a, 0, s, d) * (1.0 - u);
       q += edge_q(a, 1, s, d) * u;
       a.Qd[(size_t)j * a.n + d] = q;
+    }
+  }
+}
+
+// The same samples for a narrow round (at most 64 positions per segment),
+// one wave per live segment: no search, the segment's offset read directly.
+__global__ __launch_bounds__(256) void k_edge_samples(EdgeArgs a) {
+  const int nlive = a.ctl[0];
+  const int lane = threadIdx.x & 63;
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int nwaves = (gridDim.x * blockDim.x) >> 6;
+  for (int li = wave; li < nlive; li += nwaves) {
+    const int s = a.seg[li];
+    const int o = a.soff[li], k = a.soff[li + 1] - o;
+    const int ns = a.ns[s], c0 = a.cur[s];
+    for (int j = lane; j < k; j += 64) {
+      const int i = kg_bisect_index(ns, c0 + j);
+      const double u = (double)i / (double)ns;
+      for (int d = 0; d < a.n; d++) {
+#pragma clang fp contract(off)
+        double q = edge_q(a, 0, s, d) * (1.0 - u);
+        q += edge_q(a, 1, s, d) * u;
+        a.Qd[(size_t)(o + j) * a.n + d] = q;
+      }
     }
   }
 }
@@ -2766,14 +2801,17 @@ hipError_t kgl_edge_init(const EdgeArgs& a, bool clear_ctl, hipStream_t s) {
 }
 
 hipError_t kgl_edge_round(const EdgeArgs& a, int width, int phase, hipStream_t s) {
-  // grid-stride kernels: enough blocks to cover the budget, at most 1024
-  const int blocks = max(1, min(1024, (a.budget + 255) / 256));
+  // the reduction and a narrow round's samples: one wave per live segment
+  // (4 per block); a wide round's samples: one thread each; grid-stride
+  const int blocks = max(1, min(2048, (a.nseg + 3) / 4));
   if (phase == 0) {
     k_edge_plan<<<1, KG_EDGE_PLAN_TB, 0, s>>>(a, width);
-    k_edge_samples<<<blocks, 256, 0, s>>>(a);
+    if (width <= 64)
+      k_edge_samples<<<blocks, 256, 0, s>>>(a);
+    else
+      k_edge_samples_flat<<<max(1, min(1024, (a.budget + 255) / 256)), 256, 0, s>>>(a);
   } else {
-    // one wave per live segment (4 per block), grid-stride
-    k_edge_reduce<<<max(1, min(2048, (a.nseg + 3) / 4)), 256, 0, s>>>(a);
+    k_edge_reduce<<<blocks, 256, 0, s>>>(a);
   }
   return hipGetLastError();
 }
