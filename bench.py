@@ -274,6 +274,7 @@ def main():
         band_sample = int((np.abs(sep).min(axis=1) < 1e-6).sum())
         cpu["contact_band"] = {"band_m": 1e-6, "mismatches": int(len(diff)), "mismatches_in_band": in_band_diff,
                                "fp32_undecided_rechecked": int(st["fp64_rechecks"]),
+                               "fp32_undecided_pairs": int(st["exact_pairs"]),
                                "sample_configs": nb, "sample_in_band": band_sample,
                                "sample_seconds": round(time.perf_counter() - t1, 2)}
         cpu["fk_fp32"] = fk_pose_error(robot, ow, orb, Q[:4096])

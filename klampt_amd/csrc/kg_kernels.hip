@@ -1130,7 +1130,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SCAP == 256 
         const PairItem P = items[x.y];
         c = x.x;
         g = (P.rank >> KG_RANK_GROUP_BIT) & 1;
-        bits = exact_leaf(a, P, rpool, x.z, x.w, sub);
+        // a configuration already decided by a certain hit (recorded before
+        // the hit was found) cannot change: no test
+        const uint8_t* flag = a.mode == KG_MODE_ANY ? a.fin.feasible : (g == 0 ? a.fin.env_hit : a.fin.self_hit);
+        const bool moot = flag && flag[c] == (a.mode == KG_MODE_ANY ? 0 : 1);
+        if (!moot) bits = exact_leaf(a, P, rpool, x.z, x.w, sub);
       }
       const uint64_t sepm = ballot64((bits & 1) != 0);
       if (act && sub == 0 && !(sepm & (0x1ffull << (9 * li)))) {
