@@ -308,11 +308,13 @@ def main():
     print(json.dumps(line), flush=True)
 
 
-def _time_dev(fn, reps):
+def _time_dev(fn, reps, warm=1):
     """Seconds per call of fn (device calls on torch's current stream): HIP
-    events recorded on that stream around reps calls, after one warm-up."""
+    events recorded on that stream around reps calls, after warm warm-up
+    calls (enough to touch every workspace slot the calls rotate over)."""
     import torch
-    fn()
+    for _ in range(warm):
+        fn()
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()
@@ -376,9 +378,10 @@ def run_secondary(args, ctx, sc, world, robot):
     A, B = S.edge_segments(sc.robot, args.edges, eps=0.01, smin=10, smax=500, seed=8)
     dA, dB = torch.from_numpy(A).to(dev), torch.from_numpy(B).to(dev)
     dfu = torch.empty(args.edges, dtype=torch.float32, device=dev)
-    te = _time_dev(lambda: world.edge_batch_dev(dA, dB, eps=0.01, norm_inf=True, out=dfu), 5)
+    te = _time_dev(lambda: world.edge_batch_dev(dA, dB, eps=0.01, norm_inf=True, out=dfu), 5, warm=2)
     est = ctx.status_dev().cpu().numpy()
     fu = dfu.cpu().numpy()
+    world.edge_batch(A, B, eps=0.01, norm_inf=True)   # sizes the context's own workspaces
     t1 = time.perf_counter()
     fh = world.edge_batch(A, B, eps=0.01, norm_inf=True)
     th = time.perf_counter() - t1
@@ -387,6 +390,27 @@ def run_secondary(args, ctx, sc, world, robot):
                        "host_call_ms": round(th * 1e3, 3), "host_equal": bool(np.array_equal(fh, fu)),
                        "note": "kg_edge_batch_dev: device rounds, no host synchronization; host_call_ms is the "
                                "host-pointer call with its copies"}
+    # whole-job rate with two batches in flight: batches alternating over two
+    # streams of the same context run on its two edge workspace slots; the
+    # rounds of one batch leave most of the GPU idle between them
+    e_streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    e_outs = [torch.empty_like(dfu), torch.empty_like(dfu)]
+
+    def two_in_flight(reps):
+        for k in range(reps):
+            with torch.cuda.stream(e_streams[k % 2]):
+                world.edge_batch_dev(dA, dB, eps=0.01, norm_inf=True, out=e_outs[k % 2])
+    two_in_flight(4)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    two_in_flight(20)
+    torch.cuda.synchronize()
+    t2 = (time.perf_counter() - t1) / 20
+    out["c4_edges"]["two_in_flight"] = {
+        "ms_per_batch": round(t2 * 1e3, 3), "edges_per_s": round(args.edges / t2, 1),
+        "results_equal": bool(all(np.array_equal(o.cpu().numpy(), fu) for o in e_outs)),
+        "note": "one context, batches alternating over two streams on its edge workspace slots "
+                "(whole-job rate, host clock)"}
     if not args.no_cpu:
         t1 = time.perf_counter()
         fo, _, nchk = ow.edge_batch(orb, A, B, eps=0.01, norm_inf=True)

@@ -373,8 +373,15 @@ int kg_distance_batch_dev(kg_world w, int robot_index, const float* dQ, int N, f
  * segment between configurations within the joint limits; KG_E_INVALID if a
  * joint range is unbounded); a longer segment's first_u is -2 and it is
  * counted in kg_ctx_status_dev [1].  d_nsamples (nullable) receives n per
- * segment.  The _dev calls allocate nothing once a call of the same robot
- * with as many segments has run, so they can be captured in a HIP graph. */
+ * segment.  The _dev calls of a robot of at most 32 links run on a ring of
+ * two workspace slots per context, as kg_feasible_batch_dev does on its
+ * three: a call waits for the previous call on its slot and for the last
+ * plain call, so calls issued on different streams run concurrently (two
+ * batches in flight fill the GPU the rounds of one leave idle).  They
+ * allocate nothing once two calls of the same robot with as many segments
+ * have run (one per slot), so they can be captured in a HIP graph; a
+ * captured call holds one slot, and its replays race with _dev edge calls
+ * issued on other streams that the caller does not order after them. */
 int kg_edge_batch(kg_world w, int robot_index, const float* A, const float* B, int nseg, double eps, int norm_inf,
                   float* first_u);
 int kg_edge_batch_f64(kg_world w, int robot_index, const double* A, const double* B, int nseg, double eps,

@@ -164,6 +164,8 @@ struct StreamScope {
     // replace tables they read)
     for (auto& f : c->fslot)
       if (f.valid) hip_check(hipStreamWaitEvent(s, f.done, 0), "hipStreamWaitEvent");
+    for (auto& e : c->eslot)
+      if (e.f.valid) hip_check(hipStreamWaitEvent(s, e.f.done, 0), "hipStreamWaitEvent");
   }
   ~StreamScope() {
     if (hipEventRecord(c->evDone, s) == hipSuccess) {
@@ -904,6 +906,12 @@ int kg_ctx_destroy(kg_ctx c) {
       for (auto* b : fb) b->release();
       if (f.done) (void)hipEventDestroy(f.done);
     }
+    for (auto& e : c->eslot) {
+      DevBuf* eb[] = {&e.f.status, &e.f.jl, &e.f.counts, &e.f.redo, &e.f.items, &e.f.recs, &e.f.spill,
+                      &e.ns, &e.cur, &e.done, &e.seg, &e.soff, &e.ctl, &e.q, &e.feas, &e.stat};
+      for (auto* b : eb) b->release();
+      if (e.f.done) (void)hipEventDestroy(e.f.done);
+    }
     (void)hipStreamDestroy(c->stream);
     delete c;
   });
@@ -916,6 +924,8 @@ int kg_ctx_synchronize(kg_ctx c) {
     if (c->done_valid) hip_check(hipEventSynchronize(c->evDone), "hipEventSynchronize");
     for (auto& f : c->fslot)
       if (f.valid) hip_check(hipEventSynchronize(f.done), "hipEventSynchronize");
+    for (auto& e : c->eslot)
+      if (e.f.valid) hip_check(hipEventSynchronize(e.f.done), "hipEventSynchronize");
     hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
   });
 }
@@ -1616,6 +1626,41 @@ int kg_world_pair_list(kg_world w, int ri, int32_t* pairs, int maxp, int* np) {
 }  // extern "C"
 
 namespace {
+// True while s captures a HIP graph: a wait on work recorded before the
+// capture cannot be part of the graph, so slotted calls then skip their slot
+// waits (the caller orders the graph's replays against other work).
+bool capturing(hipStream_t s) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  return hipStreamIsCapturing(s, &st) == hipSuccess && st == hipStreamCaptureStatusActive;
+}
+
+// A slot's feasibility workspaces for a batch of N configurations and ncap
+// pair items (grown as needed).
+FeasWs slot_ws(kg_ctx_s* c, kg_ctx_s::FeasSlot& f, int N, size_t ncap) {
+  const kg_params& prm = c->prm;
+  const int scap = prm.scap == 256 ? 256 : 512;
+  const int grid32 = prm.waves_per_cu > 0 ? prm.waves_per_cu * c->num_cus : kgl_collide_grid(0, c->num_cus, scap);
+  const int grid64 = std::min(kgl_collide_grid(1, c->num_cus, scap), std::max(1, c->num_cus));
+  const size_t spill_entries = (size_t)std::max(grid32, grid64) * prm.spill_cap;
+  hip_check(f.status.ensure(ST_WORDS * sizeof(int)), "hipMalloc(status)");
+  hip_check(f.jl.ensure((size_t)N), "hipMalloc(jl)");
+  hip_check(f.counts.ensure((size_t)(N + 1) * sizeof(int)), "hipMalloc(counts)");
+  hip_check(f.redo.ensure(redo_ints(N) * sizeof(int)), "hipMalloc(redo)");
+  hip_check(f.items.ensure(std::max<size_t>(1, ncap) * sizeof(PairItem)), "hipMalloc(items)");
+  hip_check(f.recs.ensure(std::max<size_t>(1, ncap) * sizeof(ItemRec)), "hipMalloc(records)");
+  hip_check(f.spill.ensure(std::max<size_t>(1, spill_entries) * sizeof(uint64_t)), "hipMalloc(spill)");
+  FeasWs ws;
+  ws.status = f.status.as<int>();
+  ws.jl = f.jl.as<uint8_t>();
+  ws.counts = f.counts.as<int>();
+  ws.redo = f.redo.as<int>();
+  ws.items = f.items.as<PairItem>();
+  ws.recs = f.recs.as<ItemRec>();
+  ws.spill = f.spill.as<uint64_t>();
+  ws.spill_entries = (int)std::min<size_t>(spill_entries, (size_t)INT32_MAX);
+  return ws;
+}
+
 // A device-pointer feasibility batch on the context's next workspace slot
 // (the fixed-capacity, per-link broad-phase path; anything else runs as a
 // plain call in context order).  The batch waits for the previous batch on
@@ -1653,30 +1698,11 @@ void feasible_dev_slot(kg_world_s* w, int ri, const float* Qf, const double* Qd,
     hip_check(hipEventCreateWithFlags(&f.done, hipEventDisableTiming | hipEventDisableSystemFence),
               "hipEventCreate");
   // the slot's previous batch, and the last plain call (shared tables)
-  if (f.valid) hip_check(hipStreamWaitEvent(s, f.done, 0), "hipStreamWaitEvent");
-  if (c->done_valid && !c->last_slotted && c->done_stream != s)
+  const bool cap = capturing(s);
+  if (f.valid && !cap) hip_check(hipStreamWaitEvent(s, f.done, 0), "hipStreamWaitEvent");
+  if (c->done_valid && !c->last_slotted && c->done_stream != s && !cap)
     hip_check(hipStreamWaitEvent(s, c->evDone, 0), "hipStreamWaitEvent");
-  const kg_params& prm = c->prm;
-  const int scap = prm.scap == 256 ? 256 : 512;
-  const int grid32 = prm.waves_per_cu > 0 ? prm.waves_per_cu * c->num_cus : kgl_collide_grid(0, c->num_cus, scap);
-  const int grid64 = std::min(kgl_collide_grid(1, c->num_cus, scap), std::max(1, c->num_cus));
-  const size_t spill_entries = (size_t)std::max(grid32, grid64) * prm.spill_cap;
-  hip_check(f.status.ensure(ST_WORDS * sizeof(int)), "hipMalloc(status)");
-  hip_check(f.jl.ensure((size_t)N), "hipMalloc(jl)");
-  hip_check(f.counts.ensure((size_t)(N + 1) * sizeof(int)), "hipMalloc(counts)");
-  hip_check(f.redo.ensure(redo_ints(N) * sizeof(int)), "hipMalloc(redo)");
-  hip_check(f.items.ensure(std::max<size_t>(1, L.ncap) * sizeof(PairItem)), "hipMalloc(items)");
-  hip_check(f.recs.ensure(std::max<size_t>(1, L.ncap) * sizeof(ItemRec)), "hipMalloc(records)");
-  hip_check(f.spill.ensure(std::max<size_t>(1, spill_entries) * sizeof(uint64_t)), "hipMalloc(spill)");
-  FeasWs ws;
-  ws.status = f.status.as<int>();
-  ws.jl = f.jl.as<uint8_t>();
-  ws.counts = f.counts.as<int>();
-  ws.redo = f.redo.as<int>();
-  ws.items = f.items.as<PairItem>();
-  ws.recs = f.recs.as<ItemRec>();
-  ws.spill = f.spill.as<uint64_t>();
-  ws.spill_entries = (int)std::min<size_t>(spill_entries, (size_t)INT32_MAX);
+  FeasWs ws = slot_ws(c, f, N, L.ncap);
   feasible_core(c, L, Qf, Qd, N, feas, env, self, fp, s, nullptr, true, &ws);
   hip_check(hipEventRecord(f.done, s), "hipEventRecord");
   f.valid = true;
@@ -2759,7 +2785,8 @@ namespace {
 // are counted in the status words.  A batch of more segments than the
 // budget's samples runs as consecutive chunks.
 void edge_core(kg_world_s* w, int ri, const float* Af, const float* Bf, const double* Ad, const double* Bd, int nseg,
-               double eps, int norm_inf, int max_pos, float* first_u, int32_t* nsamples, hipStream_t s) {
+               double eps, int norm_inf, int max_pos, float* first_u, int32_t* nsamples, hipStream_t s,
+               kg_ctx_s::EdgeSlot* es = nullptr) {
   kg_ctx_s* c = w->ctx;
   need(ri >= 0 && ri < (int)w->robots.size(), "robot index out of range");
   need(nseg >= 0, "negative segment count");
@@ -2782,16 +2809,32 @@ void edge_core(kg_world_s* w, int ri, const float* Af, const float* Bf, const do
   B = (int)std::min<size_t>((size_t)B, std::max<size_t>(1, ((size_t)2048 << 20) / std::max<size_t>(1, (size_t)cap * sizeof(PairItem))));
   const int chunk = std::min(nseg, B);
 
-  hip_check(c->wsENs.ensure((size_t)nseg * sizeof(int)), "hipMalloc");
-  hip_check(c->wsECur.ensure((size_t)nseg * sizeof(int)), "hipMalloc");
-  hip_check(c->wsEDone.ensure((size_t)nseg), "hipMalloc");
-  hip_check(c->wsESeg.ensure((size_t)chunk * sizeof(int)), "hipMalloc");
-  hip_check(c->wsESoff.ensure((size_t)(chunk + 1) * sizeof(int)), "hipMalloc");
-  hip_check(c->wsECtl.ensure(8 * sizeof(int)), "hipMalloc");
-  hip_check(c->wsEQ.ensure((size_t)B * n * sizeof(double)), "hipMalloc");
-  hip_check(c->wsEFeas.ensure((size_t)B), "hipMalloc");
-  hip_check(c->wsEStat.ensure(ST_WORDS * sizeof(int)), "hipMalloc");
-  hip_check(c->wsStatus.ensure(ST_WORDS * sizeof(int)), "hipMalloc(status)");
+  // the edge buffers: the slot's, or the context's for a plain call
+  DevBuf& bNs = es ? es->ns : c->wsENs;
+  DevBuf& bCur = es ? es->cur : c->wsECur;
+  DevBuf& bDone = es ? es->done : c->wsEDone;
+  DevBuf& bSeg = es ? es->seg : c->wsESeg;
+  DevBuf& bSoff = es ? es->soff : c->wsESoff;
+  DevBuf& bCtl = es ? es->ctl : c->wsECtl;
+  DevBuf& bQ = es ? es->q : c->wsEQ;
+  DevBuf& bFeas = es ? es->feas : c->wsEFeas;
+  DevBuf& bStat = es ? es->stat : c->wsEStat;
+  hip_check(bNs.ensure((size_t)nseg * sizeof(int)), "hipMalloc");
+  hip_check(bCur.ensure((size_t)nseg * sizeof(int)), "hipMalloc");
+  hip_check(bDone.ensure((size_t)nseg), "hipMalloc");
+  hip_check(bSeg.ensure((size_t)chunk * sizeof(int)), "hipMalloc");
+  hip_check(bSoff.ensure((size_t)(chunk + 1) * sizeof(int)), "hipMalloc");
+  hip_check(bCtl.ensure(8 * sizeof(int)), "hipMalloc");
+  hip_check(bQ.ensure((size_t)B * n * sizeof(double)), "hipMalloc");
+  hip_check(bFeas.ensure((size_t)B), "hipMalloc");
+  hip_check(bStat.ensure(ST_WORDS * sizeof(int)), "hipMalloc");
+  FeasWs ws;
+  if (es) {
+    need(group, "edge workspace slots need the per-link broad phase");
+    ws = slot_ws(c, es->f, B, (size_t)cap * B);
+  } else {
+    hip_check(c->wsStatus.ensure(ST_WORDS * sizeof(int)), "hipMalloc(status)");
+  }
 
   Layout L;
   L.P = &P;
@@ -2808,16 +2851,16 @@ void edge_core(kg_world_s* w, int ri, const float* Af, const float* Bf, const do
   ea.norm_inf = norm_inf ? 1 : 0;
   ea.max_pos = max_pos;
   ea.budget = B;
-  ea.seg = c->wsESeg.as<int>();
-  ea.soff = c->wsESoff.as<int>();
-  ea.ctl = c->wsECtl.as<int>();
-  ea.Qd = c->wsEQ.as<double>();
-  ea.feas = c->wsEFeas.as<uint8_t>();
-  ea.st = c->wsStatus.as<int>();
+  ea.seg = bSeg.as<int>();
+  ea.soff = bSoff.as<int>();
+  ea.ctl = bCtl.as<int>();
+  ea.Qd = bQ.as<double>();
+  ea.feas = bFeas.as<uint8_t>();
+  ea.st = es ? ws.status : c->wsStatus.as<int>();
   ea.w_spill = ST_SPILL;
   ea.w_unc = ST_UNC;
   ea.w_redo = ST_REDO;
-  ea.out_status = c->wsEStat.as<int>();
+  ea.out_status = bStat.as<int>();
   // the control words accumulate over the chunks: cleared once, by the first
   // chunk's init kernel
   for (int s0 = 0; s0 < nseg; s0 += chunk) {
@@ -2827,12 +2870,12 @@ void edge_core(kg_world_s* w, int ri, const float* Af, const float* Bf, const do
     ea.Bf = Bf ? Bf + (size_t)s0 * n : nullptr;
     ea.Ad = Ad ? Ad + (size_t)s0 * n : nullptr;
     ea.Bd = Bd ? Bd + (size_t)s0 * n : nullptr;
-    ea.ns = c->wsENs.as<int>() + s0;
-    ea.cur = c->wsECur.as<int>() + s0;
-    ea.done = c->wsEDone.as<uint8_t>() + s0;
+    ea.ns = bNs.as<int>() + s0;
+    ea.cur = bCur.as<int>() + s0;
+    ea.done = bDone.as<uint8_t>() + s0;
     ea.first_u = first_u + s0;
     ea.nsamples_out = nsamples ? nsamples + s0 : nullptr;
-    ea.ctl = c->wsECtl.as<int>();
+    ea.ctl = bCtl.as<int>();
     hip_check(kgl_edge_init(ea, s0 == 0, s), "edge_init");
     // schedule: widths w0 g^r; each round gives every live segment at least
     // min(width, B / ns) positions
@@ -2845,8 +2888,8 @@ void edge_core(kg_world_s* w, int ri, const float* Af, const float* Bf, const do
       const int width = (int)std::min<int64_t>(wf, 1 << 20);
       covered += std::min(width, floor_w);
       hip_check(kgl_edge_round(ea, width, 0, s), "edge_plan");
-      feasible_core(c, L, nullptr, ea.Qd, B, c->wsEFeas.as<uint8_t>(), nullptr, nullptr, nullptr, s,
-                    ea.ctl + 1, false);
+      feasible_core(c, L, nullptr, ea.Qd, B, bFeas.as<uint8_t>(), nullptr, nullptr, nullptr, s, ea.ctl + 1, false,
+                    es ? &ws : nullptr);
       hip_check(kgl_edge_round(ea, width, 1, s), "edge_reduce");
     }
     hip_check(kgl_edge_finish(ea, s0 + chunk >= nseg, s), "edge_finish");
@@ -2855,7 +2898,7 @@ void edge_core(kg_world_s* w, int ri, const float* Af, const float* Bf, const do
   c->has_batch = true;
   c->last_tr_slot = -1;
   c->last_phase_ev = false;
-  c->stat_src = c->wsEStat.as<int>();
+  c->stat_src = bStat.as<int>();
   hip_check(hipEventRecord(c->evStat, s), "hipEventRecord");
 }
 
@@ -2946,9 +2989,37 @@ void edge_dev(kg_world w, int ri, const float* Af, const float* Bf, const double
   if (nseg == 0) return;
   need((Af || Ad) && (Bf || Bd) && first_u, "NULL buffer");
   hipStream_t s = (hipStream_t)stream;
-  StreamScope scope(w->ctx, s);
-  edge_core(w, ri, Af, Bf, Ad, Bd, nseg, eps, norm_inf, edge_max_pos(w->robots[ri], eps, norm_inf, max_steps),
-            first_u, nsamples, s);
+  kg_ctx_s* c = w->ctx;
+  const int mp = edge_max_pos(w->robots[ri], eps, norm_inf, max_steps);
+  // on the context's next edge slot (per-link broad phase): the batch waits
+  // for the previous batch on its slot and for the last plain call, not for
+  // batches on other slots, so batches issued on different streams overlap
+  if (!(c->prm.prep_group && w->robots[ri]->n <= 32)) {
+    StreamScope scope(c, s);
+    edge_core(w, ri, Af, Bf, Ad, Bd, nseg, eps, norm_inf, mp, first_u, nsamples, s);
+    return;
+  }
+  hip_check(hipSetDevice(c->device), "hipSetDevice");
+  if (!c->evDone)
+    hip_check(hipEventCreateWithFlags(&c->evDone, hipEventDisableTiming | hipEventDisableSystemFence),
+              "hipEventCreate");
+  kg_ctx_s::EdgeSlot& es = c->eslot[c->eslot_next];
+  c->eslot_next = (c->eslot_next + 1) % kg_ctx_s::EDGE_SLOTS;
+  if (!es.f.done)
+    hip_check(hipEventCreateWithFlags(&es.f.done, hipEventDisableTiming | hipEventDisableSystemFence),
+              "hipEventCreate");
+  const bool cap = capturing(s);
+  if (es.f.valid && !cap) hip_check(hipStreamWaitEvent(s, es.f.done, 0), "hipStreamWaitEvent");
+  if (c->done_valid && !c->last_slotted && c->done_stream != s && !cap)
+    hip_check(hipStreamWaitEvent(s, c->evDone, 0), "hipStreamWaitEvent");
+  edge_core(w, ri, Af, Bf, Ad, Bd, nseg, eps, norm_inf, mp, first_u, nsamples, s, &es);
+  hip_check(hipEventRecord(es.f.done, s), "hipEventRecord");
+  es.f.valid = true;
+  if (hipEventRecord(c->evDone, s) == hipSuccess) {
+    c->done_stream = s;
+    c->done_valid = true;
+    c->last_slotted = true;
+  }
 }
 }  // namespace
 

@@ -507,6 +507,16 @@ struct kg_ctx_s {
   static constexpr int FEAS_SLOTS = 3;
   FeasSlot fslot[FEAS_SLOTS];
   int fslot_next = 0;
+  // Workspace slots of device-pointer edge batches, on the same terms: the
+  // feasibility workspaces of the batch's rounds and its edge buffers
+  // (segment state, round plan, samples, status).
+  struct EdgeSlot {
+    FeasSlot f;
+    DevBuf ns, cur, done, seg, soff, ctl, q, feas, stat;
+  };
+  static constexpr int EDGE_SLOTS = 2;
+  EdgeSlot eslot[EDGE_SLOTS];
+  int eslot_next = 0;
   DevBuf wsQ, wsT64, wsLB, wsJL, wsCounts, wsOffsets, wsItems, wsScanTmp, wsBest, wsUnc, wsChunk, wsSpill,
       wsStatus, wsRedo, wsOut8a, wsOut8b, wsOut8c, wsOut32, wsF, wsG, wsPoff, wsPts, wsLoss, wsDQ,
       wsDl, wsEA, wsEB, wsESeg, wsESoff, wsENs, wsECur, wsECtl, wsEStat, wsEQ, wsEFeas, wsEU, wsEDone, wsRec,

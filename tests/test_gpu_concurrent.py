@@ -74,3 +74,73 @@ def test_world_change_between_concurrent_batches(ctx):
     after = ow2.feasible_batch(orb2, Qa)[0]
     assert not np.array_equal(after, before[0])
     assert np.array_equal(oa2.cpu().numpy(), after)
+
+
+def test_edge_batches_on_two_streams_equal_serial(ctx):
+    """Device-pointer edge batches take the context's edge workspace slots:
+    batches issued alternately on two streams run concurrently and each
+    equals the same batch run alone (and the oracle); a feasibility batch
+    and a host-pointer edge call issued between them keep their order."""
+    import torch
+    sc = S.soup_scene(0, target_tris=60_000)
+    w, r = engine.world_from_scene(ctx, sc)
+    sets = [S.edge_segments(sc.robot, 1024 + 256 * k, eps=0.02, smin=5, smax=150, seed=80 + k) for k in range(4)]
+    dev = [(torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda()) for A, B in sets]
+    serial = [w.edge_batch_dev(dA, dB, eps=0.02).cpu().numpy() for dA, dB in dev]
+    outs = [torch.full((len(A),), 7.0, dtype=torch.float32, device="cuda") for A, _ in sets]
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    Qf = S.uniform_configs(sc.robot, 4096, seed=85)
+    dQ = torch.from_numpy(Qf).cuda()
+    fq = torch.empty(4096, dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+    print("4 edge batches on 2 streams, a feasibility batch and a host edge call between them", flush=True)
+    for k in range(4):
+        with torch.cuda.stream(streams[k % 2]):
+            w.edge_batch_dev(dev[k][0], dev[k][1], eps=0.02, out=outs[k])
+            if k == 1:
+                w.feasible_batch_dev(dQ, feasible=fq)
+        if k == 2:
+            host = w.edge_batch(sets[0][0], sets[0][1], eps=0.02)
+    torch.cuda.synchronize()
+    for k in range(4):
+        assert np.array_equal(outs[k].cpu().numpy(), serial[k]), f"edge batch {k}"
+    assert np.array_equal(host, serial[0])
+    ow, orb = oracle.world_from_scene(sc)
+    assert np.array_equal(fq.cpu().numpy(), ow.feasible_batch(orb, Qf)[0])
+    fo, _, _ = ow.edge_batch(orb, sets[1][0], sets[1][1], eps=0.02, norm_inf=False)
+    assert np.array_equal(serial[1], fo.astype(np.float32))
+    st = ctx.status_dev().cpu().numpy()
+    assert st[0] == 0 and st[1] == 0
+
+
+def test_feasibility_batch_captured_in_a_graph(ctx):
+    """kg_feasible_batch_dev captured in a HIP graph after one warm-up call
+    per workspace slot: replays with new inputs copied into the captured
+    buffer, and eager work between them, equal the eager results."""
+    import torch
+    sc = S.boxes_scene(3)
+    w, r = engine.world_from_scene(ctx, sc)
+    sets = [S.uniform_configs(sc.robot, 4096, seed=90 + k) for k in range(3)]
+    eager = [w.feasible_batch(Q)[0] for Q in sets]
+    dQ = torch.from_numpy(sets[0]).cuda()
+    out = torch.empty(4096, dtype=torch.uint8, device="cuda")
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        for _ in range(3):
+            w.feasible_batch_dev(dQ, feasible=out)
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    print("capturing a feasibility batch", flush=True)
+    with torch.cuda.graph(g, stream=side):
+        w.feasible_batch_dev(dQ, feasible=out)
+    torch.cuda.synchronize()
+    for rep in range(2):
+        for k, Q in enumerate(sets):
+            dQ.copy_(torch.from_numpy(Q))
+            out.fill_(7)
+            w.distance_batch(sets[0][:16])
+            g.replay()
+            torch.cuda.synchronize()
+            assert np.array_equal(out.cpu().numpy(), eager[k]), f"replay {rep}, set {k}"
