@@ -534,22 +534,29 @@ __global__ __launch_bounds__(KG_PREP_TB) void k_prep_group(PrepArgs a) {
     }
   }
   // links with geometry, as a mask over the group's lanes
-  const uint64_t bal = __ballot(mine && P.links[l].geom >= 0);
-  const uint32_t active = (uint32_t)(bal >> ((t & 63) & ~(G - 1))) & (G == 32 ? 0xffffffffu : ((1u << G) - 1));
+  const bool act_l = mine && P.links[l].geom >= 0;
+  const uint64_t bal = __ballot(act_l);
+  const int gshift = (t & 63) & ~(G - 1);
+  const uint32_t gmask = G == 32 ? 0xffffffffu : ((1u << G) - 1);
+  const uint32_t active = (uint32_t)(bal >> gshift) & gmask;
+  // union of the active links' boxes: a min / max reduction over the group's
+  // lanes (exact, as the per-link loop it replaces)
+  double ulo[3], uhi[3];
+  for (int r = 0; r < 3; r++) {
+    ulo[r] = act_l ? sB[g][l][r] : 1e300;
+    uhi[r] = act_l ? sB[g][l][3 + r] : -1e300;
+  }
+#pragma unroll
+  for (int off = G / 2; off >= 1; off >>= 1)
+    for (int r = 0; r < 3; r++) {
+      ulo[r] = fmin(ulo[r], __shfl_xor(ulo[r], off, G));
+      uhi[r] = fmax(uhi[r], __shfl_xor(uhi[r], off, G));
+    }
   __syncthreads();
   const bool jl = sjl[g] != 0;
   const bool go = valid && (a.mode != KG_MODE_ANY || jl);
   const bool ordered = a.mode == KG_MODE_ORDERED;
   if (go) {
-    // union of the active links' boxes
-    double ulo[3] = {1e300, 1e300, 1e300}, uhi[3] = {-1e300, -1e300, -1e300};
-    for (int k = 0; k < n; k++)
-      if (active >> k & 1) {
-        for (int r = 0; r < 3; r++) {
-          ulo[r] = fmin(ulo[r], sB[g][k][r]);
-          uhi[r] = fmax(uhi[r], sB[g][k][3 + r]);
-        }
-      }
     // environment entries surviving against the link union, their union
     double elo[3] = {1e300, 1e300, 1e300}, ehi[3] = {-1e300, -1e300, -1e300};
     int m2 = 0;
@@ -563,15 +570,16 @@ __global__ __launch_bounds__(KG_PREP_TB) void k_prep_group(PrepArgs a) {
         }
       }
     }
-    // links surviving against the environment union (active-list positions)
+    // links surviving against the environment union (active-list positions):
+    // each lane tests its own link, the group's verdicts are gathered by
+    // ballot (every lane of the group is in this branch) and compacted
+    const uint64_t rb = __ballot(act_l && !box_hit(sB[g][l], sB[g][l] + 3, elo, ehi));
+    const uint32_t rej = (uint32_t)(rb >> gshift) & gmask;
     uint32_t lrej = 0;
     {
       int pos = 0;
-      for (int k = 0; k < n; k++)
-        if (active >> k & 1) {
-          if (!box_hit(sB[g][k], sB[g][k] + 3, elo, ehi)) lrej |= 1u << pos;
-          pos++;
-        }
+      for (uint32_t m = active; m; m &= m - 1, pos++)
+        if (rej >> (__ffs(m) - 1) & 1) lrej |= 1u << pos;
     }
     const int nact = __popc(active), m1 = nact - __popc(lrej);
     const int base = MULTI ? a.ibase[c] : c * a.cap;
