@@ -102,8 +102,8 @@ def measured_traffic():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--configs", type=int, default=65536)
     ap.add_argument("--streams", type=int, default=2,
                     help="streams the timed batches alternate over (1: one batch at a time)")
