@@ -259,6 +259,90 @@ class RobotCSpace(CSpace):
         return out
 
 
+def _knn(Q, V, k):
+    """Distances and indices of the k nearest rows of V (Euclidean) for each
+    row of Q, nearest first: exact FP64 distances on the GPU through torch
+    when one is present (in row blocks of bounded size), a k-d tree on the
+    host otherwise."""
+    try:
+        import torch
+        gpu = torch.cuda.is_available()
+    except ImportError:
+        gpu = False
+    if gpu and len(V) * len(Q) > 1 << 16:
+        # squared differences summed per coordinate, as the k-d tree does
+        # (torch.cdist's direct mode gave rows of other query points on
+        # 3,000 x 6,393 FP64 inputs here)
+        dV = torch.from_numpy(np.ascontiguousarray(V, np.float64)).cuda()
+        rows = max(1, (1 << 24) // max(1, len(V) * V.shape[1]))
+        ds, js = [], []
+        for r0 in range(0, len(Q), rows):
+            dQ = torch.from_numpy(np.ascontiguousarray(Q[r0:r0 + rows], np.float64)).cuda()
+            D2 = ((dQ[:, None, :] - dV[None, :, :]) ** 2).sum(dim=2)
+            d2, j = torch.topk(D2, k, dim=1, largest=False, sorted=True)
+            ds.append(torch.sqrt(d2).cpu().numpy())
+            js.append(j.cpu().numpy())
+        return np.concatenate(ds), np.concatenate(js)
+    from scipy.spatial import cKDTree
+    d, j = cKDTree(V).query(Q, k=k)
+    return np.asarray(d).reshape(len(Q), -1), np.asarray(j).reshape(len(Q), -1)
+
+
+class _EdgeStore:
+    """Roadmap edges (i < j) as arrays: the endpoints and a state per edge
+    (-1 not checked, 0 blocked, 1 valid), with their keys i * 2^32 + j kept
+    sorted for membership tests, so that a batch of candidate links is
+    added, deduplicated and validated without a Python loop per edge."""
+
+    def __init__(self):
+        self.a = np.zeros(0, np.int64)
+        self.b = np.zeros(0, np.int64)
+        self.state = np.zeros(0, np.int8)
+        self._keys = np.zeros(0, np.int64)   # sorted
+        self._pos = np.zeros(0, np.int64)    # edge index of each sorted key
+
+    def __len__(self):
+        return len(self.a)
+
+    def find(self, a, b):
+        """Edge index of each (a, b), a < b, or -1."""
+        k = (np.asarray(a, np.int64) << 32) | np.asarray(b, np.int64)
+        i = np.searchsorted(self._keys, k)
+        i = np.minimum(i, max(0, len(self._keys) - 1))
+        hit = (len(self._keys) > 0) & (self._keys[i] == k) if len(self._keys) else np.zeros(len(k), bool)
+        return np.where(hit, self._pos[i] if len(self._pos) else -1, -1)
+
+    def put(self, a, b, state):
+        """Adds the edges (a, b), a < b, not present yet, with `state`
+        (edges already present keep theirs); returns the indices of the
+        edges added."""
+        a = np.asarray(a, np.int64)
+        b = np.asarray(b, np.int64)
+        k = np.unique((a << 32) | b)
+        new = k[self.find(k >> 32, k & 0xffffffff) < 0] if len(self._keys) else k
+        n0 = len(self.a)
+        idx = np.arange(n0, n0 + len(new))
+        self.a = np.r_[self.a, new >> 32]
+        self.b = np.r_[self.b, new & 0xffffffff]
+        self.state = np.r_[self.state, np.full(len(new), state, np.int8)]
+        keys = np.r_[self._keys, new]
+        pos = np.r_[self._pos, idx]
+        order = np.argsort(keys, kind="stable")
+        self._keys, self._pos = keys[order], pos[order]
+        return idx
+
+    def set(self, a, b, state):
+        """Adds or overwrites the edges (a, b), a < b, with `state` (one
+        state per edge)."""
+        state = np.broadcast_to(np.asarray(state, np.int8), np.shape(a))
+        self.put(a, b, -1)
+        self.state[self.find(a, b)] = state
+
+    def as_dict(self):
+        val = {-1: None, 0: False, 1: True}
+        return {(int(i), int(j)): val[int(v)] for i, j, v in zip(self.a, self.b, self.state)}
+
+
 class MotionPlan:
     """A planner over a CSpace (``Python/klampt/plan/cspace.py:216-440``).
 
@@ -299,13 +383,19 @@ class MotionPlan:
         self.parent = None       # RRT: parent milestone index (-1 at a root)
         self.tree = None         # RRT: 0 start tree, 1 goal tree
         self.V = np.zeros((0, len(space.bound)))
-        self.edges = {}          # (i, j), i < j -> True (valid), False (blocked), None (not checked)
+        self._E = _EdgeStore()
         self.start = self.goal = None
         self.path_idx = None
         self._stats = {"milestones": 0, "samples": 0, "edges_checked": 0, "check_calls": 0, "time_s": 0.0}
 
     def close(self):
         pass
+
+    @property
+    def edges(self):
+        """The roadmap's edges: (i, j), i < j -> True (valid), False
+        (blocked), None (not checked)."""
+        return self._E.as_dict()
 
     def setEndpoints(self, start, goal):
         if not callable(goal) and not isinstance(goal, tuple):
@@ -318,7 +408,7 @@ class MotionPlan:
             raise ValueError("goal sets are not supported: pass a configuration")
         self.V = np.asarray([start, goal], float)
         self.start, self.goal = 0, 1
-        self.edges = {}
+        self._E = _EdgeStore()
         self.path_idx = None
         if self.rrt:
             self.parent = np.array([-1, -1])
@@ -347,32 +437,26 @@ class MotionPlan:
         milestones (all milestones, Euclidean)."""
         if len(self.V) < 2:
             return
-        from scipy.spatial import cKDTree
-        tree = cKDTree(self.V)
         k = min(self._k() + 1, len(self.V))
-        d, nb = tree.query(self.V[new], k=k)
-        d, nb = np.atleast_2d(d), np.atleast_2d(nb)
-        cand = []
-        for row, i in enumerate(new):
-            for dist, j in zip(d[row], nb[row]):
-                if j == i or dist > self.connectionThreshold:
-                    continue
-                e = (min(i, j), max(i, j))
-                if e not in self.edges:
-                    self.edges[e] = None
-                    cand.append(e)
+        d, nb = _knn(self.V[new], self.V, k)
+        i = np.repeat(np.asarray(new, np.int64), nb.shape[1])
+        j = nb.ravel().astype(np.int64)
+        keep = (j != i) & (d.ravel() <= self.connectionThreshold) & (j < len(self.V))
+        i, j = i[keep], j[keep]
+        cand = self._E.put(np.minimum(i, j), np.maximum(i, j), -1)
         if not self.lazy:
             self._validate(cand)
 
-    def _validate(self, edges):
-        edges = [e for e in edges if self.edges[e] is None]
-        if not edges:
+    def _validate(self, idx):
+        """Checks the not yet checked edges among the edge indices `idx`
+        in one batched call."""
+        idx = np.asarray(idx, np.int64)
+        idx = idx[self._E.state[idx] < 0]
+        if len(idx) == 0:
             return
-        E = np.asarray(edges)
-        ok = self.space.visible_batch(self.V[E[:, 0]], self.V[E[:, 1]])
-        for e, v in zip(edges, ok):
-            self.edges[e] = bool(v)
-        self._stats["edges_checked"] += len(edges)
+        ok = self.space.visible_batch(self.V[self._E.a[idx]], self.V[self._E.b[idx]])
+        self._E.state[idx] = np.asarray(ok, bool).astype(np.int8)
+        self._stats["edges_checked"] += len(idx)
         self._stats["check_calls"] += 1
 
     def _steer(self, near, target):
@@ -406,8 +490,7 @@ class MotionPlan:
         ok = self.space.visible_batch(self.V[P[:, 0]], self.V[P[:, 1]])
         self._stats["edges_checked"] += len(P)
         self._stats["check_calls"] += 1
-        for (a, b), v in zip(pairs, ok):
-            self.edges[(min(a, b), max(a, b))] = bool(v)
+        self._E.set(np.minimum(P[:, 0], P[:, 1]), np.maximum(P[:, 0], P[:, 1]), np.asarray(ok, bool))
         hits = [p for p, v in zip(pairs, ok) if v]
         if hits:
             def root_path(i):
@@ -457,8 +540,8 @@ class MotionPlan:
         self.V = np.vstack([self.V, Xn[keep]])
         self.parent = np.r_[self.parent, near[keep]]
         self.tree = np.r_[self.tree, trees[keep] if self.bidirectional else np.zeros(len(keep), int)]
-        for k, p in zip(range(first, len(self.V)), near[keep]):
-            self.edges[(int(min(k, p)), int(max(k, p)))] = True
+        kk = np.arange(first, len(self.V))
+        self._E.set(np.minimum(kk, near[keep]), np.maximum(kk, near[keep]), True)
         self._rrt_try_connect(np.arange(first, len(self.V)))
 
     def planMore(self, iterations):
@@ -486,13 +569,15 @@ class MotionPlan:
         self._stats["milestones"] = len(self.V)
         self._stats["time_s"] += time.perf_counter() - t
 
-    def _shortest(self, usable):
+    def _shortest(self, unchecked_ok):
+        """Shortest start-goal path over the valid edges (and the unchecked
+        ones when unchecked_ok), as milestone indices, or None."""
         from scipy.sparse import csr_matrix
         from scipy.sparse.csgraph import dijkstra
-        es = [e for e, v in self.edges.items() if usable(v)]
-        if not es:
+        sel = self._E.state != 0 if unchecked_ok else self._E.state == 1
+        if not sel.any():
             return None
-        E = np.asarray(es)
+        E = np.stack([self._E.a[sel], self._E.b[sel]], axis=1)
         w = np.linalg.norm(self.V[E[:, 0]] - self.V[E[:, 1]], axis=1) + 1e-12
         n = len(self.V)
         G = csr_matrix((np.r_[w, w], (np.r_[E[:, 0], E[:, 1]], np.r_[E[:, 1], E[:, 0]])), shape=(n, n))
@@ -506,16 +591,17 @@ class MotionPlan:
 
     def _search(self):
         if not self.lazy:
-            self.path_idx = self._shortest(lambda v: v is True)
+            self.path_idx = self._shortest(False)
             return
         while True:
-            p = self._shortest(lambda v: v is not False)
+            p = self._shortest(True)
             if p is None:
                 self.path_idx = None
                 return
-            pe = [(min(a, b), max(a, b)) for a, b in zip(p[:-1], p[1:])]
-            unknown = [e for e in pe if self.edges[e] is None]
-            if not unknown:
+            pa, pb = np.asarray(p[:-1]), np.asarray(p[1:])
+            idx = self._E.find(np.minimum(pa, pb), np.maximum(pa, pb))
+            unknown = idx[self._E.state[idx] < 0]
+            if len(unknown) == 0:
                 self.path_idx = p
                 return
             self._validate(unknown)
@@ -529,13 +615,14 @@ class MotionPlan:
 
     def getRoadmap(self):
         V = [list(map(float, v)) for v in self.V]
-        E = [e for e, v in self.edges.items() if v is True]
+        ok = self._E.state == 1
+        E = [(int(i), int(j)) for i, j in zip(self._E.a[ok], self._E.b[ok])]
         return V, E
 
     def getStats(self):
         s = dict(self._stats)
-        s["edges"] = len(self.edges)
-        s["edges_valid"] = sum(1 for v in self.edges.values() if v is True)
+        s["edges"] = len(self._E)
+        s["edges_valid"] = int((self._E.state == 1).sum())
         return s
 
     def pathCost(self, path):

@@ -96,3 +96,16 @@ def test_planned_path_passes_the_oracle(boxes, ptype):
     assert ow.feasible_batch(orb, P)[0].all()
     fu, _, _ = ow.edge_batch(orb, P[:-1], P[1:], eps=0.02, norm_inf=False)
     assert (fu < 0).all()
+
+
+def test_gpu_nearest_neighbours_equal_kd_tree():
+    """The roadmap's k-nearest-neighbour query on the GPU (exact FP64
+    distances) returns the k-d tree's neighbours and distances."""
+    from scipy.spatial import cKDTree
+    rng = np.random.default_rng(3)
+    V = rng.uniform(-3, 3, (6400, 7))
+    Q = V[rng.choice(len(V), 3000, replace=False)]
+    d, j = plan._knn(Q, V, 11)
+    d2, j2 = cKDTree(V).query(Q, k=11)
+    assert np.allclose(d, d2, rtol=0, atol=1e-12)
+    assert np.array_equal(j, j2)

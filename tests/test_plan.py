@@ -128,3 +128,23 @@ def test_rrt_tree_edges_are_short_and_valid(bidirectional):
         assert np.all(mp.tree[2:] == 0)
     with pytest.raises(ValueError):
         mp.addMilestone([0.2, 0.2])
+
+
+def test_edge_store_dedup_find_and_states():
+    """The roadmap's edge arrays: a batch with repeats and edges already
+    present adds each new edge once, unchecked; find maps pairs to indices
+    (-1 when absent); set adds or overwrites states; the dict view follows."""
+    E = plan._EdgeStore()
+    added = E.put(np.array([0, 1, 0, 2]), np.array([5, 3, 5, 4]), -1)
+    assert len(E) == 3 and len(added) == 3
+    again = E.put(np.array([1, 6]), np.array([3, 7]), -1)
+    assert len(again) == 1 and len(E) == 4
+    assert E.find(np.array([0, 2, 1, 3]), np.array([5, 4, 3, 9])).tolist()[3] == -1
+    idx = E.find(np.array([0, 2]), np.array([5, 4]))
+    assert (idx >= 0).all() and E.a[idx].tolist() == [0, 2] and E.b[idx].tolist() == [5, 4]
+    E.set(np.array([0, 8]), np.array([5, 9]), np.array([True, False]))
+    d = E.as_dict()
+    assert d[(0, 5)] is True and d[(8, 9)] is False and d[(1, 3)] is None and len(d) == 5
+    big = 1 << 20
+    E.put(np.array([big]), np.array([big + 1]), 1)
+    assert E.find(np.array([big]), np.array([big + 1]))[0] == 5
