@@ -473,15 +473,14 @@ class MotionPlan:
         unidirectional plan); all candidate links checked in one call."""
         if self.path_idx is not None or len(new) == 0:
             return
-        from scipy.spatial import cKDTree
         pairs = []
         for t in (0, 1):
             mine = new[self.tree[new] == t]
             other = np.nonzero(self.tree == 1 - t)[0]
             if len(mine) == 0 or len(other) == 0:
                 continue
-            d, j = cKDTree(self.V[other]).query(self.V[mine], k=1)
-            for a, dist, jj in zip(mine, np.atleast_1d(d), np.atleast_1d(j)):
+            d, j = _knn(self.V[mine], self.V[other], 1)
+            for a, dist, jj in zip(mine, d.ravel(), j.ravel()):
                 if dist <= self.perturbationRadius:
                     pairs.append((int(a), int(other[jj])))
         if not pairs:
@@ -512,7 +511,6 @@ class MotionPlan:
         """One batch of RRT extensions: `iterations` targets drawn at once,
         each extends the nearest node of the tree it is given to (the trees
         take turns when bidirectional), feasibility and edge checks batched."""
-        from scipy.spatial import cKDTree
         n = int(iterations)
         Xt = self.space.sample_batch(n)
         trees = np.arange(n) % 2 if self.bidirectional else np.zeros(n, int)
@@ -522,8 +520,8 @@ class MotionPlan:
             nodes = np.nonzero(self.tree == t)[0]
             if len(sel) == 0:
                 continue
-            _, j = cKDTree(self.V[nodes]).query(Xt[sel], k=1)
-            near[sel] = nodes[np.atleast_1d(j)]
+            _, j = _knn(Xt[sel], self.V[nodes], 1)
+            near[sel] = nodes[j.ravel()]
         Xn = self._steer(self.V[near], Xt)
         ok = self.space.feasible_batch(Xn)
         self._stats["samples"] += n
