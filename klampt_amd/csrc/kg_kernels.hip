@@ -3315,13 +3315,19 @@ __device__ SwV sw_support(const double* __restrict__ V, int nv, const double* X0
 // equations of its edge vectors) is a point of the simplex when all its
 // barycentric weights are positive; the answer is the nearest such point.
 // lam receives the weights over w (0 outside the chosen subset).
-__device__ SwV sw_simplex_min(const SwV* w, int m, double* lam, int& chosen) {
+// The subsets are visited in order, unrolled over the 15 a 4-vertex simplex
+// has, so that every small array below is indexed by constants and kept in
+// registers (with runtime bounds they went to scratch memory).
+__device__ __forceinline__ SwV sw_simplex_min(const SwV* w, int m, double* lam, int& chosen) {
   double bestd = 1.0 / 0.0;
   SwV bestp = w[0];
   chosen = 1;
-  for (int s = 1; s < (1 << m); s++) {
+#pragma unroll
+  for (int s = 1; s < 16; s++) {
+    if (s >= (1 << m)) break;
     int idx[4], k = 0;
-    for (int i = 0; i < m; i++)
+#pragma unroll
+    for (int i = 0; i < 4; i++)
       if (s >> i & 1) idx[k++] = i;
     double mu[4] = {1, 0, 0, 0};
     if (k > 1) {
@@ -3381,10 +3387,15 @@ __device__ SwV sw_simplex_min(const SwV* w, int m, double* lam, int& chosen) {
 
 // GJK distance between the hull of A's copies (XA0, XA1 nullable) and the
 // hull of B (XB); 0 when they overlap.  ca, cb: the closest points.
+// Every lane runs the same iterations on the same values (the support maps
+// are wave reductions), so the simplex lives once per wave in LDS, where its
+// runtime-indexed updates cost an LDS access instead of scratch memory.
 __device__ double sw_gjk(const double* VA, int na, const double* XA0, const double* XA1, const double* VB, int nb,
                          const double* XB, SwV& ca, SwV& cb) {
-  SwV w[4], pa[4], pb[4];
-  double lam[4] = {1, 0, 0, 0};
+  __shared__ SwV w[4], pa[4], pb[4];
+  __shared__ double lam[4];
+  lam[0] = 1;
+  lam[1] = lam[2] = lam[3] = 0;
   int m = 1;
   pa[0] = sw_apply(XA0, VA);
   pb[0] = sw_apply(XB, VB);
