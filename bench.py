@@ -591,6 +591,11 @@ def run_secondary(args, ctx, sc, world, robot):
     cand = cand[cand[:, 0] < cand[:, 1]]
     blocked = ~space.visible_batch(Qf[cand[:, 0]], Qf[cand[:, 1]])
     qa, qb = Qf[cand[int(np.argmax(blocked))]].astype(float)
+    # warm-up (first use of the nearest-neighbour and edge-check kernels)
+    warm = plan.MotionPlan(space, type="prm", knn=10)
+    warm.setEndpoints(qa, qb)
+    warm.planMore(2048)
+    space.rng = np.random.default_rng(5)   # the timed roadmap draws the same samples as without it
     mp = plan.MotionPlan(space, type="prm", knn=10)
     mp.setEndpoints(qa, qb)
     tp0 = time.perf_counter()
