@@ -1343,9 +1343,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SCAP == 256 
       while (head < tail && __builtin_amdgcn_readfirstlane(e_cfg[head % NE]) < 0) head++;
       const int room = NE - (tail - head);
       if (room > 0) {
-        int k0 = 0, nk = 0;
+        int k0 = 0, nk = 0, kst = 1;
         if (soff < S0) {
-          k0 = (int)blockIdx.x * S0 + soff;
+          // a heterogeneous batch's static share is interleaved over the
+          // waves (this wave's configurations blockIdx, blockIdx + nw, ...):
+          // its configurations come robot by robot, and contiguous slices
+          // would give some waves only the heaviest robot's (3% on the 64-robot
+          // batch); a single robot's batch keeps contiguous slices, whose
+          // items sit together in memory (interleaved: 3% slower)
+          if (a.cfg_plan) {
+            k0 = soff * nw + (int)blockIdx.x;
+            kst = nw;
+          } else {
+            k0 = (int)blockIdx.x * S0 + soff;
+          }
           nk = min(room, S0 - soff);
           soff += nk;
         } else {
@@ -1390,7 +1401,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SCAP == 256 
         if (nk > 0) {
           if (lane < nk) {
             const int e = (tail + lane) % NE;
-            const int cc = cfg_of(k0 + lane);
+            const int cc = cfg_of(k0 + lane * kst);
             e_cfg[e] = cc;
             range(cc, e_beg[e], e_end[e]);
             e_open[e] = 1;
