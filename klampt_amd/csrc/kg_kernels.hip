@@ -3289,11 +3289,10 @@ __device__ inline SwV sw_apply(const double* X, const double* v) {
 }
 
 // Support point of the copies of local points V[nv] placed by X0 (and X1 if
-// given) in direction d: the (copy, vertex) with the largest d . (R v + t),
-// ties to the lowest copy, then vertex.  Every lane of the wave calls it with
-// the same arguments and receives the same point.
-__device__ SwV sw_support(const double* __restrict__ V, int nv, const double* X0, const double* X1, SwV d) {
-  const int lane = threadIdx.x & 63;
+// given) in direction d: the index k * nv + i of the (copy k, vertex i) with
+// the largest d . (R v + t), ties to the lowest copy, then vertex (the first
+// maximum of the scan).  A NaN direction gives index 0.
+__device__ inline int sw_support(const double* __restrict__ V, int nv, const double* X0, const double* X1, SwV d) {
   double best = -1.0 / 0.0;
   int bi = 0x7fffffff;
   for (int k = 0; k < (X1 ? 2 : 1); k++) {
@@ -3303,7 +3302,8 @@ __device__ SwV sw_support(const double* __restrict__ V, int nv, const double* X0
     const double l1 = X[3] * d.x + X[4] * d.y + X[5] * d.z;
     const double l2 = X[6] * d.x + X[7] * d.y + X[8] * d.z;
     const double off = X[9] * d.x + X[10] * d.y + X[11] * d.z;
-    for (int i = lane; i < nv; i += 64) {
+#pragma unroll 4
+    for (int i = 0; i < nv; i++) {
       const double val = l0 * V[3 * i] + l1 * V[3 * i + 1] + l2 * V[3 * i + 2] + off;
       if (val > best) {
         best = val;
@@ -3311,110 +3311,135 @@ __device__ SwV sw_support(const double* __restrict__ V, int nv, const double* X0
       }
     }
   }
-  for (int o = 32; o > 0; o >>= 1) {
-    const double ob = __shfl_xor(best, o);
-    const int oi = __shfl_xor(bi, o);
-    if (ob > best || (ob == best && oi < bi)) {
-      best = ob;
-      bi = oi;
-    }
-  }
-  // no vertex beat -inf (a NaN direction): the first vertex of the first copy
-  if (bi == 0x7fffffff) bi = 0;
-  const int k = X1 ? bi / nv : 0, i = bi % nv;
+  return bi == 0x7fffffff ? 0 : bi;
+}
+
+// The point of a support index.
+__device__ inline SwV sw_point(const double* V, int nv, const double* X0, const double* X1, int bi) {
+  const int k = X1 ? bi / nv : 0, i = bi - k * nv;
   return sw_apply(k ? X1 : X0, V + 3 * i);
 }
 
+// Vertex subsets of a simplex as bit masks: the number of members and the
+// index of the j-th member, evaluated at compile time.
+__host__ __device__ constexpr int sw_nbits(int s) { return (s & 1) + (s >> 1 & 1) + (s >> 2 & 1) + (s >> 3 & 1); }
+__host__ __device__ constexpr int sw_bit(int s, int j) {
+  return j == 0 ? ((s & 1) ? 0 : (s & 2) ? 1 : (s & 4) ? 2 : 3) : sw_bit(s & (s - 1), j - 1);
+}
+
+// One vertex subset S of w: the closest point of its affine hull (normal
+// equations of its edge vectors) when all its barycentric weights are
+// positive; it replaces the best so far when strictly nearer.  Every index
+// is a compile-time constant, so the small arrays stay in registers.
+template <int S>
+__device__ __forceinline__ void sw_subset(const SwV* w, double& bestd, SwV& bestp, int& chosen, double* lam) {
+  constexpr int K = sw_nbits(S), Q = K - 1;
+  double mu[4] = {1, 0, 0, 0};
+  if constexpr (K > 1) {
+    const SwV p0 = w[sw_bit(S, 0)];
+    SwV e[3];
+#pragma unroll
+    for (int j = 1; j < K; j++) e[j - 1] = w[sw_bit(S, j)] - p0;
+    double G[3][3], r[3];
+#pragma unroll
+    for (int i = 0; i < Q; i++) {
+      r[i] = -swdot(e[i], p0);
+#pragma unroll
+      for (int j = 0; j < Q; j++) G[i][j] = swdot(e[i], e[j]);
+    }
+    double x[3];
+    double scale = 0;
+#pragma unroll
+    for (int i = 0; i < Q; i++) scale = fmax(scale, G[i][i]);
+    if constexpr (Q == 1) {
+      if (!(G[0][0] > 0)) return;
+      x[0] = r[0] / G[0][0];
+    } else if constexpr (Q == 2) {
+      const double det = G[0][0] * G[1][1] - G[0][1] * G[1][0];
+      if (!(det > 1e-24 * scale * scale)) return;
+      x[0] = (r[0] * G[1][1] - G[0][1] * r[1]) / det;
+      x[1] = (G[0][0] * r[1] - r[0] * G[1][0]) / det;
+    } else {
+      const double c00 = G[1][1] * G[2][2] - G[1][2] * G[2][1];
+      const double c01 = G[1][2] * G[2][0] - G[1][0] * G[2][2];
+      const double c02 = G[1][0] * G[2][1] - G[1][1] * G[2][0];
+      const double det = G[0][0] * c00 + G[0][1] * c01 + G[0][2] * c02;
+      if (!(det > 1e-36 * scale * scale * scale)) return;
+      x[0] = (r[0] * c00 + G[0][1] * (r[2] * G[1][2] - r[1] * G[2][2]) + G[0][2] * (r[1] * G[2][1] - r[2] * G[1][1])) / det;
+      x[1] = (G[0][0] * (r[1] * G[2][2] - r[2] * G[1][2]) + r[0] * c01 + G[0][2] * (r[2] * G[1][0] - r[1] * G[2][0])) / det;
+      x[2] = (G[0][0] * (r[2] * G[1][1] - r[1] * G[2][1]) + G[0][1] * (r[1] * G[2][0] - r[2] * G[1][0]) + r[0] * c02) / det;
+    }
+    double sum = 0;
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < Q; i++) {
+      mu[i + 1] = x[i];
+      sum += x[i];
+      ok = ok && x[i] > 0;
+    }
+    mu[0] = 1 - sum;
+    if (!(ok && mu[0] > 0)) return;
+  }
+  SwV p = sw(0, 0, 0);
+#pragma unroll
+  for (int j = 0; j < K; j++) p = p + mu[j] * w[sw_bit(S, j)];
+  const double dd = swdot(p, p);
+  if (dd < bestd) {
+    bestd = dd;
+    bestp = p;
+    chosen = S;
+#pragma unroll
+    for (int i = 0; i < 4; i++) lam[i] = 0;
+#pragma unroll
+    for (int j = 0; j < K; j++) lam[sw_bit(S, j)] = mu[j];
+  }
+}
+
 // Closest point to the origin of the simplex w[0..m) (m <= 4): over the
-// vertex subsets, the closest point of each subset's affine hull (normal
-// equations of its edge vectors) is a point of the simplex when all its
-// barycentric weights are positive; the answer is the nearest such point.
-// lam receives the weights over w (0 outside the chosen subset).
-// The subsets are visited in order, unrolled over the 15 a 4-vertex simplex
-// has, so that every small array below is indexed by constants and kept in
-// registers (with runtime bounds they went to scratch memory).
+// vertex subsets, in increasing mask order, the nearest closest point of a
+// subset's affine hull that lies in the simplex.  lam receives the weights
+// over w (0 outside the chosen subset), chosen the subset's mask.
 __device__ __forceinline__ SwV sw_simplex_min(const SwV* w, int m, double* lam, int& chosen) {
   double bestd = 1.0 / 0.0;
   SwV bestp = w[0];
   chosen = 1;
-#pragma unroll
-  for (int s = 1; s < 16; s++) {
-    if (s >= (1 << m)) break;
-    int idx[4], k = 0;
-#pragma unroll
-    for (int i = 0; i < 4; i++)
-      if (s >> i & 1) idx[k++] = i;
-    double mu[4] = {1, 0, 0, 0};
-    if (k > 1) {
-      const SwV p0 = w[idx[0]];
-      SwV e[3];
-      for (int j = 1; j < k; j++) e[j - 1] = w[idx[j]] - p0;
-      const int q = k - 1;
-      double G[3][3], r[3];
-      for (int a = 0; a < q; a++) {
-        r[a] = -swdot(e[a], p0);
-        for (int b = 0; b < q; b++) G[a][b] = swdot(e[a], e[b]);
-      }
-      double x[3];
-      double scale = 0;
-      for (int a = 0; a < q; a++) scale = fmax(scale, G[a][a]);
-      if (q == 1) {
-        if (!(G[0][0] > 0)) continue;
-        x[0] = r[0] / G[0][0];
-      } else if (q == 2) {
-        const double det = G[0][0] * G[1][1] - G[0][1] * G[1][0];
-        if (!(det > 1e-24 * scale * scale)) continue;
-        x[0] = (r[0] * G[1][1] - G[0][1] * r[1]) / det;
-        x[1] = (G[0][0] * r[1] - r[0] * G[1][0]) / det;
-      } else {
-        const double c00 = G[1][1] * G[2][2] - G[1][2] * G[2][1];
-        const double c01 = G[1][2] * G[2][0] - G[1][0] * G[2][2];
-        const double c02 = G[1][0] * G[2][1] - G[1][1] * G[2][0];
-        const double det = G[0][0] * c00 + G[0][1] * c01 + G[0][2] * c02;
-        if (!(det > 1e-36 * scale * scale * scale)) continue;
-        x[0] = (r[0] * c00 + G[0][1] * (r[2] * G[1][2] - r[1] * G[2][2]) + G[0][2] * (r[1] * G[2][1] - r[2] * G[1][1])) / det;
-        x[1] = (G[0][0] * (r[1] * G[2][2] - r[2] * G[1][2]) + r[0] * c01 + G[0][2] * (r[2] * G[1][0] - r[1] * G[2][0])) / det;
-        x[2] = (G[0][0] * (r[2] * G[1][1] - r[1] * G[2][1]) + G[0][1] * (r[1] * G[2][0] - r[2] * G[1][0]) + r[0] * c02) / det;
-      }
-      double sum = 0;
-      bool ok = true;
-      for (int a = 0; a < q; a++) {
-        mu[a + 1] = x[a];
-        sum += x[a];
-        ok = ok && x[a] > 0;
-      }
-      mu[0] = 1 - sum;
-      if (!(ok && mu[0] > 0)) continue;
-    }
-    SwV p = sw(0, 0, 0);
-    for (int j = 0; j < k; j++) p = p + mu[j] * w[idx[j]];
-    const double dd = swdot(p, p);
-    if (dd < bestd) {
-      bestd = dd;
-      bestp = p;
-      chosen = s;
-      for (int i = 0; i < 4; i++) lam[i] = 0;
-      for (int j = 0; j < k; j++) lam[idx[j]] = mu[j];
-    }
+  sw_subset<1>(w, bestd, bestp, chosen, lam);
+  if (m >= 2) {
+    sw_subset<2>(w, bestd, bestp, chosen, lam);
+    sw_subset<3>(w, bestd, bestp, chosen, lam);
+  }
+  if (m >= 3) {
+    sw_subset<4>(w, bestd, bestp, chosen, lam);
+    sw_subset<5>(w, bestd, bestp, chosen, lam);
+    sw_subset<6>(w, bestd, bestp, chosen, lam);
+    sw_subset<7>(w, bestd, bestp, chosen, lam);
+  }
+  if (m >= 4) {
+    sw_subset<8>(w, bestd, bestp, chosen, lam);
+    sw_subset<9>(w, bestd, bestp, chosen, lam);
+    sw_subset<10>(w, bestd, bestp, chosen, lam);
+    sw_subset<11>(w, bestd, bestp, chosen, lam);
+    sw_subset<12>(w, bestd, bestp, chosen, lam);
+    sw_subset<13>(w, bestd, bestp, chosen, lam);
+    sw_subset<14>(w, bestd, bestp, chosen, lam);
+    sw_subset<15>(w, bestd, bestp, chosen, lam);
   }
   return bestp;
 }
 
 // GJK distance between the hull of A's copies (XA0, XA1 nullable) and the
-// hull of B (XB); 0 when they overlap.  ca, cb: the closest points.
-// Every lane runs the same iterations on the same values (the support maps
-// are wave reductions), so the simplex lives once per wave in LDS, where its
-// runtime-indexed updates cost an LDS access instead of scratch memory.
+// hull of B (XB); 0 when they overlap.  ca, cb: the closest points.  One
+// lane runs one query: the simplex and its vertices' support indices live in
+// registers, every runtime-indexed update written as selects over the four
+// slots.
 __device__ double sw_gjk(const double* VA, int na, const double* XA0, const double* XA1, const double* VB, int nb,
                          const double* XB, SwV& ca, SwV& cb) {
-  __shared__ SwV w[4], pa[4], pb[4];
-  __shared__ double lam[4];
-  lam[0] = 1;
-  lam[1] = lam[2] = lam[3] = 0;
+  SwV w[4];
+  int ia[4] = {0, 0, 0, 0}, ib[4] = {0, 0, 0, 0};
+  double lam[4] = {1, 0, 0, 0};
   int m = 1;
-  pa[0] = sw_apply(XA0, VA);
-  pb[0] = sw_apply(XB, VB);
-  w[0] = pa[0] - pb[0];
+  w[0] = sw_apply(XA0, VA) - sw_apply(XB, VB);
+  w[1] = w[2] = w[3] = w[0];
   SwV v = w[0];
   double dist = sqrt(swdot(v, v));
   for (int it = 0; it < 64; it++) {
@@ -3423,31 +3448,51 @@ __device__ double sw_gjk(const double* VA, int na, const double* XA0, const doub
       dist = 0;
       break;
     }
-    const SwV a = sw_support(VA, na, XA0, XA1, -1.0 * v);
-    const SwV b = sw_support(VB, nb, XB, nullptr, v);
-    const SwV nw = a - b;
+    const int ja = sw_support(VA, na, XA0, XA1, -1.0 * v);
+    const int jb = sw_support(VB, nb, XB, nullptr, v);
+    const SwV nw = sw_point(VA, na, XA0, XA1, ja) - sw_point(VB, nb, XB, nullptr, jb);
     // the new support point cannot bring v closer by more than the tolerance
     if (vv - swdot(v, nw) <= 1e-13 * vv) break;
     bool dup = false;
-    for (int i = 0; i < m; i++) dup = dup || (w[i].x == nw.x && w[i].y == nw.y && w[i].z == nw.z);
+#pragma unroll
+    for (int i = 0; i < 4; i++) dup = dup || (i < m && w[i].x == nw.x && w[i].y == nw.y && w[i].z == nw.z);
     if (dup) break;
-    w[m] = nw;
-    pa[m] = a;
-    pb[m] = b;
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+      if (i == m) {
+        w[i] = nw;
+        ia[i] = ja;
+        ib[i] = jb;
+      }
     m++;
     int chosen;
-    double l2[4];
+    double l2[4] = {0, 0, 0, 0};
     const SwV nv = sw_simplex_min(w, m, l2, chosen);
-    // keep the chosen vertices
+    // keep the chosen vertices, in order
+    SwV w2[4] = {w[0], w[0], w[0], w[0]};
+    int ia2[4] = {0, 0, 0, 0}, ib2[4] = {0, 0, 0, 0};
+    double lam2[4] = {lam[0], lam[1], lam[2], lam[3]};
     int k = 0;
-    for (int i = 0; i < m; i++)
-      if (chosen >> i & 1) {
-        w[k] = w[i];
-        pa[k] = pa[i];
-        pb[k] = pb[i];
-        lam[k] = l2[i];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+      if (i < m && (chosen >> i & 1)) {
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+          if (j == k) {
+            w2[j] = w[i];
+            ia2[j] = ia[i];
+            ib2[j] = ib[i];
+            lam2[j] = l2[i];
+          }
         k++;
       }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      w[j] = w2[j];
+      ia[j] = ia2[j];
+      ib[j] = ib2[j];
+      lam[j] = lam2[j];
+    }
     m = k;
     if (m == 4) {
       dist = 0;
@@ -3463,64 +3508,66 @@ __device__ double sw_gjk(const double* VA, int na, const double* XA0, const doub
   }
   ca = sw(0, 0, 0);
   cb = sw(0, 0, 0);
-  for (int i = 0; i < m; i++) {
-    ca = ca + lam[i] * pa[i];
-    cb = cb + lam[i] * pb[i];
-  }
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+    if (i < m) {
+      ca = ca + lam[i] * sw_point(VA, na, XA0, XA1, ia[i]);
+      cb = cb + lam[i] * sw_point(VB, nb, XB, nullptr, ib[i]);
+    }
   return dist;
 }
 
-__global__ __launch_bounds__(64) void k_sweep(SweepArgs a) {
-  const long total = (long)a.N * a.n * a.O;
-  const int lane = threadIdx.x;
-  for (long q = blockIdx.x; q < total; q += gridDim.x) {
-    const int o = (int)(q % a.O);
-    const int l = (int)((q / a.O) % a.n);
-    const int c = (int)(q / ((long)a.O * a.n));
-    double* info = a.info_out + q * 16;
-    double* coef = a.coef_out + q * 2 * a.n;
-    const HullRef& H = a.lhull[l];
-    const HullRef& B = a.ohull[o];
-    const double* X0 = a.T0 + ((size_t)c * a.n + l) * 12;
-    const double* X1 = a.T1 + ((size_t)c * a.n + l) * 12;
-    SwV ps, po;
-    double d = 1.0 / 0.0;
-    if (H.nv > 0 && B.nv > 0) d = sw_gjk(H.V, H.nv, X0, X1, B.V, B.nv, B.X, ps, po);
-    if (!(d < a.dcheck) || !(d > 0)) {
-      // beyond the bound (not a constraint), or overlapping hulls (no
-      // separating normal): the distance, and no terms
-      if (lane == 0) {
-        a.d_out[q] = d < a.dcheck ? d : 1.0 / 0.0;
-        for (int k = 0; k < 16; k++) info[k] = 0;
-        for (int j = 0; j < 2 * a.n; j++) coef[j] = 0;
-        if (d < a.dcheck) {
-          info[4] = ps.x; info[5] = ps.y; info[6] = ps.z;
-          info[7] = po.x; info[8] = po.y; info[9] = po.z;
-        }
-      }
-      continue;
-    }
-    const SwV nrm = (1.0 / d) * (ps - po);
-    const SwV p0 = sw_support(H.V, H.nv, X0, nullptr, -1.0 * nrm);
-    const SwV p1 = sw_support(H.V, H.nv, X1, nullptr, -1.0 * nrm);
-    const SwV e0 = p0 - ps, e1 = p1 - ps;
-    const double d0 = sqrt(swdot(e0, e0)), d1 = sqrt(swdot(e1, e1));
-    const double alpha = d0 + d1 > 0 ? d1 / (d0 + d1) : 0.5;
-    if (lane == 0) {
-      a.d_out[q] = d;
-      info[0] = alpha;
-      info[1] = nrm.x; info[2] = nrm.y; info[3] = nrm.z;
+// One lane per (segment, link, obstacle) query.  The queries of a wave share
+// their link (each link's N x O queries are padded to whole waves), so the
+// link hull's address and its vertex scan are wave-uniform: its vertices come
+// through scalar loads, and every load precedes the kernel's first store.
+// Four waves per SIMD (128 VGPRs, a 112-byte spill) run the 4,096-segment
+// batch 12% faster than two (174 VGPRs, no spill).
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_sweep(SweepArgs a, int wpl) {
+  const int l = (int)blockIdx.x / wpl;
+  const long p = (long)((int)blockIdx.x - l * wpl) * 64 + threadIdx.x;
+  if (p >= (long)a.N * a.O) return;
+  const int c = (int)(p / a.O), o = (int)(p - (long)c * a.O);
+  const long q = ((long)c * a.n + l) * a.O + o;
+  const HullRef& H = a.lhull[l];
+  const HullRef& B = a.ohull[o];
+  const double* X0 = a.T0 + ((size_t)c * a.n + l) * 12;
+  const double* X1 = a.T1 + ((size_t)c * a.n + l) * 12;
+  SwV ps = sw(0, 0, 0), po = sw(0, 0, 0);
+  double d = 1.0 / 0.0;
+  if (H.nv > 0 && B.nv > 0) d = sw_gjk(H.V, H.nv, X0, X1, B.V, B.nv, B.X, ps, po);
+  double* info = a.info_out + q * 16;
+  double* coef = a.coef_out + q * 2 * a.n;
+  if (!(d < a.dcheck) || !(d > 0)) {
+    // beyond the bound (not a constraint), or overlapping hulls (no
+    // separating normal): the distance, and no terms
+    a.d_out[q] = d < a.dcheck ? d : 1.0 / 0.0;
+    for (int k = 0; k < 16; k++) info[k] = 0;
+    for (int j = 0; j < 2 * a.n; j++) coef[j] = 0;
+    if (d < a.dcheck) {
       info[4] = ps.x; info[5] = ps.y; info[6] = ps.z;
       info[7] = po.x; info[8] = po.y; info[9] = po.z;
-      info[10] = p0.x; info[11] = p0.y; info[12] = p0.z;
-      info[13] = p1.x; info[14] = p1.y; info[15] = p1.z;
-      for (int j = 0; j < 2 * a.n; j++) coef[j] = 0;
-      const double nh[3] = {nrm.x, nrm.y, nrm.z};
-      const double q0[3] = {p0.x, p0.y, p0.z}, q1[3] = {p1.x, p1.y, p1.z};
-      dj_chain(a.links, a.T0 + (size_t)c * a.n * 12, l, q0, nh, 1.0, coef);
-      dj_chain(a.links, a.T1 + (size_t)c * a.n * 12, l, q1, nh, 1.0, coef + a.n);
     }
+    return;
   }
+  const SwV nrm = (1.0 / d) * (ps - po);
+  const SwV p0 = sw_point(H.V, H.nv, X0, nullptr, sw_support(H.V, H.nv, X0, nullptr, -1.0 * nrm));
+  const SwV p1 = sw_point(H.V, H.nv, X1, nullptr, sw_support(H.V, H.nv, X1, nullptr, -1.0 * nrm));
+  const SwV e0 = p0 - ps, e1 = p1 - ps;
+  const double d0 = sqrt(swdot(e0, e0)), d1 = sqrt(swdot(e1, e1));
+  const double alpha = d0 + d1 > 0 ? d1 / (d0 + d1) : 0.5;
+  a.d_out[q] = d;
+  info[0] = alpha;
+  info[1] = nrm.x; info[2] = nrm.y; info[3] = nrm.z;
+  info[4] = ps.x; info[5] = ps.y; info[6] = ps.z;
+  info[7] = po.x; info[8] = po.y; info[9] = po.z;
+  info[10] = p0.x; info[11] = p0.y; info[12] = p0.z;
+  info[13] = p1.x; info[14] = p1.y; info[15] = p1.z;
+  for (int j = 0; j < 2 * a.n; j++) coef[j] = 0;
+  const double nh[3] = {nrm.x, nrm.y, nrm.z};
+  const double q0[3] = {p0.x, p0.y, p0.z}, q1[3] = {p1.x, p1.y, p1.z};
+  dj_chain(a.links, a.T0 + (size_t)c * a.n * 12, l, q0, nh, 1.0, coef);
+  dj_chain(a.links, a.T1 + (size_t)c * a.n * 12, l, q1, nh, 1.0, coef + a.n);
 }
 
 }  // namespace
@@ -3528,7 +3575,9 @@ __global__ __launch_bounds__(64) void k_sweep(SweepArgs a) {
 hipError_t kgl_sweep(const SweepArgs& a, hipStream_t s) {
   const long total = (long)a.N * a.n * a.O;
   if (total <= 0) return hipSuccess;
-  const unsigned grid = (unsigned)(total < 16384 ? total : 16384);
-  k_sweep<<<grid, 64, 0, s>>>(a);
+  // waves per link: each link's N x O queries, one per lane
+  const long wpl = ((long)a.N * a.O + 63) / 64;
+  if (wpl * a.n > 0x7fffffffL) return hipErrorInvalidValue;
+  k_sweep<<<(unsigned)(wpl * a.n), 64, 0, s>>>(a, (int)wpl);
   return hipGetLastError();
 }

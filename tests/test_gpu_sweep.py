@@ -51,8 +51,8 @@ def test_sweep_terms_vs_oracle(ctx, scene):
     spec = sc.robot
     n, O = spec.n, len(sc.objects)
     assert w.num_obstacles() == O
-    Q0, Q1 = _segments(spec, 48, seed=3)
-    print("sweep terms: 48 segments x 7 links x 16 boxes", flush=True)
+    Q0, Q1 = _segments(spec, 96, seed=3)
+    print("sweep terms: 96 segments x 7 links x 16 boxes", flush=True)
     d, info, coef = w.sweep_batch(Q0, Q1, DCHECK)
     nterms = ntie = 0
     for k in range(len(Q0)):
@@ -107,7 +107,9 @@ def test_sweep_terms_vs_oracle(ctx, scene):
 
 def test_point_terms_and_dev_path(ctx, scene):
     """Q0 == Q1: the hull of one placement, alpha 1/2 and equal rows; the
-    device-pointer call returns the host call's values."""
+    device-pointer call returns the host call's values; a batch whose
+    queries do not fill whole waves (37 segments x 16 boxes per link) returns
+    the same values as the same segments inside a larger batch."""
     import torch
     sc, w, ow, orb = scene
     Q0, Q1 = _segments(sc.robot, 32, seed=4)
@@ -119,3 +121,6 @@ def test_point_terms_and_dev_path(ctx, scene):
     h = w.sweep_batch(Q0, Q1, DCHECK)
     assert np.array_equal(dd.cpu().numpy(), h[0])
     assert np.array_equal(di.cpu().numpy(), h[1]) and np.array_equal(dc.cpu().numpy(), h[2])
+    r = w.sweep_batch(Q0[:37], Q1[:37], DCHECK)
+    for x, y in zip(r, h):
+        assert np.array_equal(x, y[:37])
