@@ -3518,16 +3518,17 @@ __device__ double sw_gjk(const double* VA, int na, const double* XA0, const doub
 }
 
 // One lane per (segment, link, obstacle) query.  The queries of a wave share
-// their link (each link's N x O queries are padded to whole waves), so the
-// link hull's address and its vertex scan are wave-uniform: its vertices come
-// through scalar loads, and every load precedes the kernel's first store.
+// their link and obstacle (each pair's N segments are padded to whole waves),
+// so both hulls' addresses and their vertex scans are wave-uniform: their
+// vertices come through scalar loads, and every load precedes the kernel's
+// first store.
 // Four waves per SIMD (128 VGPRs, a 112-byte spill) run the 4,096-segment
 // batch 12% faster than two (174 VGPRs, no spill).
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_sweep(SweepArgs a, int wpl) {
-  const int l = (int)blockIdx.x / wpl;
-  const long p = (long)((int)blockIdx.x - l * wpl) * 64 + threadIdx.x;
-  if (p >= (long)a.N * a.O) return;
-  const int c = (int)(p / a.O), o = (int)(p - (long)c * a.O);
+  const int lo = (int)blockIdx.x / wpl;
+  const int l = lo / a.O, o = lo - l * a.O;
+  const int c = ((int)blockIdx.x - lo * wpl) * 64 + (int)threadIdx.x;
+  if (c >= a.N) return;
   const long q = ((long)c * a.n + l) * a.O + o;
   const HullRef& H = a.lhull[l];
   const HullRef& B = a.ohull[o];
@@ -3575,9 +3576,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
 hipError_t kgl_sweep(const SweepArgs& a, hipStream_t s) {
   const long total = (long)a.N * a.n * a.O;
   if (total <= 0) return hipSuccess;
-  // waves per link: each link's N x O queries, one per lane
-  const long wpl = ((long)a.N * a.O + 63) / 64;
-  if (wpl * a.n > 0x7fffffffL) return hipErrorInvalidValue;
-  k_sweep<<<(unsigned)(wpl * a.n), 64, 0, s>>>(a, (int)wpl);
+  // waves per (link, obstacle) pair: its N segments, one per lane
+  const long wpl = ((long)a.N + 63) / 64;
+  if (wpl * a.n * a.O > 0x7fffffffL) return hipErrorInvalidValue;
+  k_sweep<<<(unsigned)(wpl * a.n * a.O), 64, 0, s>>>(a, (int)wpl);
   return hipGetLastError();
 }

@@ -108,7 +108,7 @@ def test_sweep_terms_vs_oracle(ctx, scene):
 def test_point_terms_and_dev_path(ctx, scene):
     """Q0 == Q1: the hull of one placement, alpha 1/2 and equal rows; the
     device-pointer call returns the host call's values; a batch whose
-    queries do not fill whole waves (37 segments x 16 boxes per link) returns
+    queries do not fill whole waves (37 segments per link and box) returns
     the same values as the same segments inside a larger batch."""
     import torch
     sc, w, ow, orb = scene
