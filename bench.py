@@ -601,6 +601,7 @@ def run_secondary(args, ctx, sc, world, robot):
     tp0 = time.perf_counter()
     for _ in range(4):
         mp.planMore(8192)
+    prm_path = mp.getPath()   # the shortest path over the grown roadmap, searched once
     tprm = time.perf_counter() - tp0
     sprm = mp.getStats()
     lz = plan.MotionPlan(space, type="lazyprm", knn=10)
@@ -614,7 +615,7 @@ def run_secondary(args, ctx, sc, world, robot):
                       "prm_samples": sprm["samples"], "prm_milestones": sprm["milestones"],
                       "prm_edges_validated": sprm["edges_checked"], "prm_ms": round(tprm * 1e3, 1),
                       "prm_edges_per_s": round(sprm["edges_checked"] / tprm, 1),
-                      "prm_path_found": mp.getPath() is not None,
+                      "prm_path_found": prm_path is not None,
                       "lazyprm_ms_to_path": round(tlazy * 1e3, 1), "lazyprm_edges_checked": slz["edges_checked"],
                       "lazyprm_samples": slz["samples"], "lazyprm_path_found": lz.getPath() is not None}
     if not args.no_cpu:

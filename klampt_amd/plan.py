@@ -30,6 +30,10 @@ import math
 import time
 
 import numpy as np
+# the roadmap's shortest-path search (loaded with the module, not on the
+# first path query)
+from scipy.sparse import csr_matrix
+from scipy.sparse.csgraph import dijkstra
 
 from . import robotsim
 
@@ -386,6 +390,7 @@ class MotionPlan:
         self._E = _EdgeStore()
         self.start = self.goal = None
         self.path_idx = None
+        self._path_stale = False  # PRM: roadmap grown since the last path search
         self._stats = {"milestones": 0, "samples": 0, "edges_checked": 0, "check_calls": 0, "time_s": 0.0}
 
     def close(self):
@@ -410,6 +415,7 @@ class MotionPlan:
         self.start, self.goal = 0, 1
         self._E = _EdgeStore()
         self.path_idx = None
+        self._path_stale = False
         if self.rrt:
             self.parent = np.array([-1, -1])
             self.tree = np.array([0, 1])
@@ -570,8 +576,6 @@ class MotionPlan:
     def _shortest(self, unchecked_ok):
         """Shortest start-goal path over the valid edges (and the unchecked
         ones when unchecked_ok), as milestone indices, or None."""
-        from scipy.sparse import csr_matrix
-        from scipy.sparse.csgraph import dijkstra
         sel = self._E.state != 0 if unchecked_ok else self._E.state == 1
         if not sel.any():
             return None
@@ -589,7 +593,9 @@ class MotionPlan:
 
     def _search(self):
         if not self.lazy:
-            self.path_idx = self._shortest(False)
+            # the shortest path over the valid links is searched when it is
+            # asked for (getPath), once, not after every batch
+            self._path_stale = True
             return
         while True:
             p = self._shortest(True)
@@ -607,6 +613,9 @@ class MotionPlan:
     def getPath(self, milestone1=None, milestone2=None):
         if milestone1 is not None or milestone2 is not None:
             raise ValueError("paths between arbitrary milestones are not supported")
+        if self._path_stale:
+            self.path_idx = self._shortest(False)
+            self._path_stale = False
         if self.path_idx is None:
             return None
         return [list(map(float, self.V[i])) for i in self.path_idx]
