@@ -3522,8 +3522,8 @@ __device__ double sw_gjk(const double* VA, int na, const double* XA0, const doub
 // so both hulls' addresses and their vertex scans are wave-uniform: their
 // vertices come through scalar loads, and every load precedes the kernel's
 // first store.
-// Four waves per SIMD (128 VGPRs, a 112-byte spill) run the 4,096-segment
-// batch 12% faster than two (174 VGPRs, no spill).
+// Four waves per SIMD (128 VGPRs, a small spill) run the 4,096-segment batch
+// 3-5% faster than two (174 VGPRs, no spill) or three.
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_sweep(SweepArgs a, int wpl) {
   const int lo = (int)blockIdx.x / wpl;
   const int l = lo / a.O, o = lo - l * a.O;
